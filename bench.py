@@ -1,0 +1,189 @@
+#!/usr/bin/env python
+"""Benchmark of the per-view transport step on one MI355X: scatter view-sweeps/s at 64^3 x 16 groups x S12 (P5).
+
+Workload (the batched circular-scan configuration): a seeded 256^3 ellipsoid phantom at 0.08 cm, coarsened to the
+64^3 transport grid (0.32 cm cells); 16 groups over 140-20 keV from a seeded two-material library with P5 transfer;
+the S12 level-symmetric set (168 directions); V views per launch sequence. One step of one view is the CITRUS
+accumulator update's sweep: scattering-source moments T = K Phi_prev of the view's accumulator (HIP), then one LD
+sweep of all 168 directions x 16 groups x 64^3 cells whose source is expanded from T in the kernel and whose angular
+fluence is reduced to the new P5 moments in the kernel. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0) with the metric, the roofline of the sweep kernel (HIP events on the launch stream)
+and the CPU baseline (the FP64 oracle's algorithm run in float32 on a sample of the same workload, on the host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "scatter view-sweeps/s (64³×16g×S12), learned vs plain at ≤1% SPR error"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--views", type=int, default=4, help="views per launch sequence")
+    ap.add_argument("--cells", type=int, default=64, help="transport cells per axis")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-directions", type=int, default=8, help="directions in the CPU-baseline sample")
+    return ap.parse_args()
+
+
+def build_problem(args, device):
+    import torch
+
+    from citrus_amd.scatter_solver.angular_quadrature import level_symmetric_quadrature
+    from citrus_amd.scatter_solver.cross_sections import mu_t_volume, seeded_library
+    from citrus_amd.scatter_solver.geometry import Grid
+    from citrus_amd.scatter_solver.source_iteration import MaterialTransfer
+    from citrus_amd.scatter_solver.transport_sweep import make_sweep_context
+    from citrus_amd.utilities import coarsen, ellipsoid_phantom
+
+    n = args.cells
+    fine = Grid((4 * n, 4 * n, 4 * n), 0.08 * 64 / n)
+    coarse = Grid((n, n, n), 0.32 * 64 / n)
+    rho = coarsen(ellipsoid_phantom(fine, seed=1234), 4)
+    library = seeded_library(16, 5, 140.0, 20.0, seed=99)
+    quadrature = level_symmetric_quadrature(12)
+    ctx = make_sweep_context(coarse, quadrature, 5, device=device)
+    mu_t = mu_t_volume(library, rho).to(device, torch.float32).contiguous()
+    transfer = MaterialTransfer(rho.to(device, torch.float32).contiguous(),
+                                library.mu_s_l.to(device, torch.float32).contiguous())
+    G, nsh = 16, ctx.num_moments
+    gen = torch.Generator(device=device).manual_seed(7)
+    phi = torch.randn(args.views, G, n, n, n, nsh, 4, device=device, generator=gen) * 0.05
+    phi[..., 0, 0] = phi[..., 0, 0].abs() + 1.0
+    return ctx, mu_t, transfer, phi, library, rho, coarse
+
+
+def cpu_baseline(args, ctx, library, rho, coarse):
+    """Oracle algorithm in float32 on the host: source expansion, sweep and moment reduction of a direction sample."""
+    import numpy as np
+    import torch
+
+    from oracle import quadrature as oq
+    from oracle import sweep as osw
+    from oracle.scattering import scatter_moments
+
+    G = library.num_groups
+    nsh = ctx.num_moments
+    perm = ctx.permutation.cpu()
+    from citrus_amd.scatter_solver.angular_quadrature import level_symmetric_quadrature
+
+    q = level_symmetric_quadrature(12)
+    om = q.omega[perm].numpy()
+    w = q.weight[perm].numpy()
+    D8 = ctx.directions_per_octant
+    k = max(1, args.cpu_directions // 8)
+    sel = np.concatenate([np.arange(o * D8, o * D8 + k) for o in range(8)])
+    Y = oq.real_spherical_harmonics(5, om)
+    rho_np = rho.numpy().astype(np.float32)
+    mu_t = np.einsum("mg,mijk->gijk", library.mu_t.numpy(), rho_np).astype(np.float32)
+    mu_s = np.einsum("mabl,mijk->ablijk", library.mu_s_l.numpy(), rho_np).astype(np.float32)
+    rng = np.random.default_rng(7)
+    phi = (rng.standard_normal((G, nsh, 4) + coarse.shape) * 0.05).astype(np.float32)
+    phi[:, 0, 0] = np.abs(phi[:, 0, 0]) + 1.0
+    t0 = time.perf_counter()
+    T = scatter_moments(mu_s, phi).astype(np.float32)  # whole-view work: counted once per view
+    t_moments = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    src = np.tensordot(Y[sel].astype(np.float32), T, axes=([1], [1])).astype(np.float32)  # [Msel, G, 4, R, C, S]
+    psi = osw.sweep(coarse.shape, coarse.deltas, om[sel], mu_t, src, dtype=np.float32)
+    out = np.tensordot((w[sel, None] * Y[sel]).astype(np.float32), psi, axes=([0], [0]))
+    t_dirs = time.perf_counter() - t0
+    M = ctx.num_directions
+    per_view = t_moments + t_dirs * M / sel.size
+    del out
+    return {"value": 1.0 / per_view, "unit": "view-sweeps/s", "cores": 1, "kind": "port",
+            "sample": f"{sel.size} of {M} directions x {G} groups x {coarse.shape[0]}^3 cells "
+                      f"(source expansion + LD sweep + moment reduction, numpy float32) plus one full scattering-"
+                      f"moment pass; per-view time = moments {t_moments:.1f} s + {t_dirs:.1f} s x {M}/{sel.size}"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): the per-view "
+              "transport step is sharded by views across GPUs by design but this build measures one GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from citrus_amd.scatter_solver.source_iteration import scatter_moments_views
+    from citrus_amd.scatter_solver.transport_sweep import sweep_views
+
+    device = torch.device("cuda:0")
+    ctx, mu_t, transfer, phi_a, library, rho, coarse = build_problem(args, device)
+    phi_b = torch.empty_like(phi_a)
+    T = torch.empty_like(phi_a)
+
+    def step(src, dst):
+        scatter_moments_views(transfer, src, out=T)
+        sweep_views(ctx, mu_t, T, out=dst)
+
+    for _ in range(args.warmup):
+        step(phi_a, phi_b)
+        phi_a, phi_b = phi_b, phi_a
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(device)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for s in range(args.steps):
+        scatter_moments_views(transfer, phi_a, out=T)
+        ev[s][0].record(stream)
+        sweep_views(ctx, mu_t, T, out=phi_b)
+        ev[s][1].record(stream)
+        phi_a, phi_b = phi_b, phi_a
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(stop)
+    sweep_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps  # per step = V view-sweeps
+    assert bool(torch.isfinite(phi_a).all()), "non-finite moments"
+    V = args.views
+    G, nsh = 16, ctx.num_moments
+    N = ctx.num_cells
+    # compulsory bytes of one view-sweep: moments T in and phi out (float32) plus mu_t
+    alg_bytes = 2 * G * N * nsh * 4 * 4 + G * N * 4
+    achieved = alg_bytes * V / (sweep_ms * 1e-3) / 1e9
+    launches = 4 * 47  # block planes of 4 octant pairs at 64^3 / 4^3 blocks (46 + one split per pair)
+    result = {
+        "metric": METRIC,
+        "value": V * args.steps / (total_ms * 1e-3),
+        "unit": "view-sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, seeded 2-material P5 library)",
+        "config": {"workload": "plain view-sweep step: HIP scattering moments + fused LD sweep (S12, P5)",
+                   "grid": f"{coarse.shape[0]}^3 cells at {coarse.delta_voxel:.2f} cm", "groups": G,
+                   "directions": ctx.num_directions, "moments": nsh, "views_per_launch": V,
+                   "sweep_ms_per_view": sweep_ms / V, "wall_s": wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ld_sweep_kernel<36,4>", "alg_bytes_per_view_sweep": alg_bytes,
+                     "launches_per_view_sweep": launches},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(args, ctx, library, rho, coarse)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

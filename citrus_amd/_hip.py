@@ -1,0 +1,109 @@
+"""Loader of the HIP library ``libcitrus_hip.so`` (ctypes) and the C-ABI argument structures.
+
+The library is built in-tree by ``citrus_amd.build`` for gfx950. There is no fallback: when the library is missing or
+fails to load, every entry point raises ``RuntimeError`` naming the cause.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
+ABI_VERSION = 1
+DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
+
+_lib = None
+_load_error: str | None = None
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_fp = ctypes.POINTER(ctypes.c_float)
+c_void = ctypes.c_void_p
+
+
+class SweepArgs(ctypes.Structure):
+    """Mirror of ``CitrusSweepArgs`` (include/citrus_hip.h)."""
+
+    _fields_ = [
+        ("G", c_int), ("nsh", c_int), ("D8", c_int), ("d0", c_int), ("dc", c_int), ("V", c_int),
+        ("accumulate", c_int), ("write_psi", c_int),
+        ("mu_t", c_void), ("T", c_void), ("source", c_void), ("phi", c_void),
+        ("Y", c_void), ("Yw", c_void), ("cdir", c_void), ("faces", c_void),
+    ]
+
+
+_SIGNATURES = {
+    "citrus_last_error": (ctypes.c_char_p, []),
+    "citrus_abi_version": (c_int, []),
+    "citrus_sweep_plan_create": (c_int, [c_int, c_int, c_int, c_int, ctypes.POINTER(c_void)]),
+    "citrus_sweep_plan_destroy": (None, [c_void]),
+    "citrus_sweep_face_floats": (c_i64, [c_void, c_int, c_int, c_int]),
+    "citrus_sweep_run": (c_int, [c_void, ctypes.POINTER(SweepArgs), c_void]),
+    "citrus_scatter_moments_dense": (c_int, [c_void, c_int, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),
+    "citrus_scatter_moments_factored": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_int, c_int,
+                                                c_i64, c_int, c_void]),
+}
+
+
+def exported_symbols() -> list[str]:
+    """Names of every C-ABI entry point this wrapper binds."""
+    return list(_SIGNATURES)
+
+
+def library() -> ctypes.CDLL:
+    """The loaded library; raises ``RuntimeError`` if it is missing, disabled or of the wrong ABI version."""
+    global _lib, _load_error
+    if os.environ.get(DISABLE_ENV_VAR, "") not in ("", "0"):
+        raise RuntimeError(f"the HIP library is disabled by {DISABLE_ENV_VAR}")
+    if _lib is not None:
+        return _lib
+    if _load_error is not None:
+        raise RuntimeError(_load_error)
+    if not LIBRARY_PATH.exists():
+        _load_error = f"{LIBRARY_PATH} is missing: build it with `python -m citrus_amd.build`"
+        raise RuntimeError(_load_error)
+    try:
+        lib = ctypes.CDLL(str(LIBRARY_PATH))
+        for name, (restype, argtypes) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+    except (OSError, AttributeError) as e:
+        _load_error = f"cannot load {LIBRARY_PATH}: {e}"
+        raise RuntimeError(_load_error) from e
+    if lib.citrus_abi_version() != ABI_VERSION:
+        _load_error = f"{LIBRARY_PATH} has ABI {lib.citrus_abi_version()}, expected {ABI_VERSION}: rebuild it"
+        raise RuntimeError(_load_error)
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    """Raise ``RuntimeError`` with the library's message when a call returned a nonzero status."""
+    if rc != 0:
+        msg = _lib.citrus_last_error().decode() if _lib is not None else ""
+        raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    """Device address of a tensor (None stays None)."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_of(device: torch.device) -> int:
+    """Raw HIP stream handle of torch's current stream on ``device``."""
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_gpu_tensor(t: torch.Tensor, name: str, dtype=torch.float32) -> None:
+    """Raise ``ValueError`` unless ``t`` is a contiguous tensor of ``dtype`` on a GPU."""
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be on a GPU device, got {t.device}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")

@@ -1,0 +1,466 @@
+// LD discrete-ordinates wavefront sweep for gfx950 (MI355X).
+//
+// Work decomposition. The grid is cut into cubic blocks of B^3 cells. For one octant the blocks form diagonal
+// "block planes" bi' + bj' + bk' = d (primes: coordinates mirrored into the octant's sweep order); all blocks of a
+// block plane are independent. One workgroup sweeps one (view, octant, group, block) task for all dc directions of
+// the chunk: it walks the block's 3B-2 cell planes, keeps the three face-flux arrays of every direction in LDS
+// (updated in place, since the cells of one plane touch disjoint face entries), and exchanges the block's outgoing
+// faces with its downstream neighbours through a global face buffer between launches.
+//
+// Octants o and 7-o mirror every axis, so block b sits on block plane d for o and K-d for 7-o (K = sum(nb_a-1)):
+// one launch runs both octants of such a pair on block plane d without ever touching the same block twice, except
+// on the middle plane when K is even, which is split into two launches. Launch order therefore fixes the order in
+// which the eight octants add into each cell's moments, and the sweep is bitwise reproducible.
+//
+// Per cell plane the workgroup (1) stages the plane's scattering-source moments T [NSH][4] in LDS, (2) solves every
+// (cell, direction) pair: source S_k = sum_n Y[d][n] T[n][k] (+ an explicit source), closed-form LD solve
+// (citrus/scatter_solver/linear_discontinuous_cell.py:59-87, docs/source/method.rst:254-279), outgoing faces back to
+// LDS, and (3) reduces the plane's angular fluence to moments phi[n][k] = sum_d Yw[d][n] psi[d][k] and adds them to
+// HBM. The angular fluence never leaves the chip.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "citrus_hip.h"
+#include "common.h"
+
+namespace citrus {
+namespace {
+
+constexpr int kThreads = 256;
+
+// Cells of a B^3 block grouped by diagonal plane a + b + c = p, in sweep-local coordinates.
+template <int B>
+struct PlaneTable {
+  static constexpr int kPlanes = 3 * B - 2;
+  int offset[3 * B - 1];
+  int cell[B * B * B];  // a | b << 8 | c << 16  (a along x, b along y, c along z)
+  constexpr PlaneTable() : offset(), cell() {
+    int n = 0;
+    for (int p = 0; p < kPlanes; ++p) {
+      offset[p] = n;
+      for (int a = 0; a < B; ++a)
+        for (int b = 0; b < B; ++b) {
+          int c = p - a - b;
+          if (c >= 0 && c < B) cell[n++] = a | (b << 8) | (c << 16);
+        }
+    }
+    offset[kPlanes] = n;
+  }
+  constexpr int max_plane() const {
+    int m = 0;
+    for (int p = 0; p < kPlanes; ++p) m = (offset[p + 1] - offset[p]) > m ? (offset[p + 1] - offset[p]) : m;
+    return m;
+  }
+};
+
+__constant__ PlaneTable<4> kPlanes4 = PlaneTable<4>();
+__constant__ PlaneTable<8> kPlanes8 = PlaneTable<8>();
+
+template <int B>
+__device__ __forceinline__ const PlaneTable<B>& planes();
+template <>
+__device__ __forceinline__ const PlaneTable<4>& planes<4>() { return kPlanes4; }
+template <>
+__device__ __forceinline__ const PlaneTable<8>& planes<8>() { return kPlanes8; }
+
+struct LaunchArgs {
+  int R, C, S, N;
+  int nbr, nbc, nbs, nblocks;  // blocks along rows (y), cols (x), slices (z)
+  int G, D8, d0, dc, V;
+  int first;      // this launch belongs to the first pass and the first chunk: first visits store phi
+  int write_psi;  // psi overwrites the explicit source in place
+  int ntask;
+  const int4* tasks;  // (octant | first_in_pair << 3 | slot << 4, block row, block col, block slice)
+  const float* mu;
+  const float* T;
+  float* src;
+  float* phi;
+  const float* Y;
+  const float* Yw;
+  const float* cdir;
+  float* faces;
+  int64_t tview, sview, fview;
+};
+
+template <int NSH>
+struct ReduceShape {
+  static constexpr int kPer = NSH >= 16 ? 2 : 1;             // moments per thread in the reduction
+  static constexpr int kChunks = (NSH + kPer - 1) / kPer;
+};
+
+template <int NSH, int B>
+__global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
+  constexpr int BB = B * B;
+  constexpr PlaneTable<B> host_table{};
+  constexpr int PMAX = host_table.max_plane();
+  constexpr int KPER = ReduceShape<NSH>::kPer;
+  constexpr int KCH = ReduceShape<NSH>::kChunks;
+  const PlaneTable<B>& tab = planes<B>();
+
+  extern __shared__ float lds[];
+  const int dc = a.dc;
+  float* face = lds;                           // [3][dc][BB][3]
+  float* tpl = face + 9 * dc * BB;             // [PMAX][NSH][4]
+  float* psl = tpl + PMAX * NSH * 4;           // [PMAX][dc][4]
+  float* ywl = psl + PMAX * dc * 4;            // [dc][NSH]
+
+  const int tid = threadIdx.x;
+  const int task = blockIdx.x % a.ntask;
+  const int rest = blockIdx.x / a.ntask;
+  const int g = rest % a.G;
+  const int v = rest / a.G;
+  const int4 e = a.tasks[task];
+  const int o = e.x & 7;
+  const bool first_visit = a.first && ((e.x >> 3) & 1);
+  const int slot = (e.x >> 4) & 1;
+  const int bi = e.y, bj = e.z, bk = e.w;
+  const int sx = (o & 4) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 1) ? -1 : 1;
+  const float fsx = (float)sx, fsy = (float)sy, fsz = (float)sz;
+  const int blk = (bi * a.nbc + bj) * a.nbs + bk;
+  const int x0 = bj * B, y0 = bi * B, z0 = bk * B;
+
+  const int64_t face_stride = (int64_t)dc * BB * 3;  // floats per (block, axis)
+  float* gface_v = a.faces + (int64_t)v * a.fview + ((int64_t)slot * a.G + g) * a.nblocks * 3 * face_stride;
+
+  // ---- direction tables ---------------------------------------------------------------------------------------
+  const int mbase = o * a.D8 + a.d0;
+  for (int i = tid; i < dc * NSH; i += kThreads) ywl[i] = a.Yw[(int64_t)(mbase + i / NSH) * NSH + i % NSH];
+  const int cpr = kThreads / dc;  // cells per round of the solve phase
+  const int my_d = tid % dc;
+  const bool solver = tid < cpr * dc;
+  float y[NSH];
+#pragma unroll
+  for (int n = 0; n < NSH; ++n) y[n] = a.Y[(int64_t)(mbase + my_d) * NSH + n];
+  const float cx = a.cdir[(mbase + my_d) * 3 + 0];
+  const float cy = a.cdir[(mbase + my_d) * 3 + 1];
+  const float cz = a.cdir[(mbase + my_d) * 3 + 2];
+
+  // ---- incoming faces: the upstream block's outgoing faces, or vacuum -----------------------------------------
+  {
+    const int ub[3][3] = {{bi, bj - sx, bk}, {bi - sy, bj, bk}, {bi, bj, bk - sz}};
+    for (int ax = 0; ax < 3; ++ax) {
+      const int ui = ub[ax][0], uj = ub[ax][1], uk = ub[ax][2];
+      const bool inside = ui >= 0 && ui < a.nbr && uj >= 0 && uj < a.nbc && uk >= 0 && uk < a.nbs;
+      float* dst = face + ax * face_stride;
+      if (inside) {
+        const float* srcf = gface_v + ((int64_t)((ui * a.nbc + uj) * a.nbs + uk) * 3 + ax) * face_stride;
+        for (int i = tid; i < face_stride; i += kThreads) dst[i] = srcf[i];
+      } else {
+        for (int i = tid; i < face_stride; i += kThreads) dst[i] = 0.0f;
+      }
+    }
+  }
+
+  const float* mu_g = a.mu + (int64_t)g * a.N;
+  const float* T_g = a.T ? a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4 : nullptr;
+  float* phi_g = a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
+  float* src_v = a.src ? a.src + (int64_t)v * a.sview : nullptr;
+
+  // cell of plane slot: physical flat index or -1 if outside the grid
+  auto cell_of = [&](int packed, int& lx, int& ly, int& lz) -> int {
+    const int pa = packed & 0xff, pb = (packed >> 8) & 0xff, pc = (packed >> 16) & 0xff;
+    lx = sx > 0 ? pa : B - 1 - pa;
+    ly = sy > 0 ? pb : B - 1 - pb;
+    lz = sz > 0 ? pc : B - 1 - pc;
+    const int x = x0 + lx, yy = y0 + ly, z = z0 + lz;
+    if (x >= a.C || yy >= a.R || z >= a.S) return -1;
+    return (yy * a.C + x) * a.S + z;
+  };
+
+  for (int p = 0; p < PlaneTable<B>::kPlanes; ++p) {
+    const int pbeg = tab.offset[p];
+    const int P = tab.offset[p + 1] - pbeg;
+    // (1) stage T of the plane's cells
+    if (T_g) {
+      for (int i = tid; i < P * NSH; i += kThreads) {
+        const int ci = i / NSH, n = i % NSH;
+        int lx, ly, lz;
+        const int cell = cell_of(tab.cell[pbeg + ci], lx, ly, lz);
+        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (cell >= 0) t = reinterpret_cast<const float4*>(T_g + (int64_t)cell * NSH * 4)[n];
+        reinterpret_cast<float4*>(tpl)[ci * NSH + n] = t;
+      }
+    }
+    __syncthreads();
+    // (2) solve every (cell, direction) pair of the plane
+    if (solver) {
+      for (int ci = tid / dc; ci < P; ci += cpr) {
+        int lx, ly, lz;
+        const int cell = cell_of(tab.cell[pbeg + ci], lx, ly, lz);
+        float4 pout = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (cell >= 0) {
+          float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+          if (T_g) {
+            const float4* t4 = reinterpret_cast<const float4*>(tpl) + ci * NSH;
+#pragma unroll
+            for (int n = 0; n < NSH; ++n) {
+              const float4 t = t4[n];
+              s0 = fmaf(y[n], t.x, s0);
+              s1 = fmaf(y[n], t.y, s1);
+              s2 = fmaf(y[n], t.z, s2);
+              s3 = fmaf(y[n], t.w, s3);
+            }
+          }
+          float4* sp = nullptr;
+          if (src_v) {
+            sp = reinterpret_cast<float4*>(src_v + (((int64_t)(o * dc + my_d) * a.N + cell) * a.G + g) * 4);
+            const float4 q = *sp;
+            s0 += q.x; s1 += q.y; s2 += q.z; s3 += q.w;
+          }
+          float* fx = face + 0 * face_stride + (my_d * BB + ly * B + lz) * 3;
+          float* fy = face + 1 * face_stride + (my_d * BB + lx * B + lz) * 3;
+          float* fz = face + 2 * face_stride + (my_d * BB + lx * B + ly) * 3;
+          // upstream faces: [0] face average psi_0 + s_a psi_a, [1], [2] transverse moments (see face writes below)
+          const float pbx = fx[0], xy = fx[1], xz = fx[2];
+          const float pby = fy[0], yx = fy[1], yz = fy[2];
+          const float pbz = fz[0], zx = fz[1], zy = fz[2];
+          const float mu = mu_g[cell];
+          const float r0 = s0 + (cx * pbx + cy * pby + cz * pbz);
+          const float rx = s1 - 3.0f * fsx * cx * pbx + (cy * yx + cz * zx);
+          const float ry = s2 - 3.0f * fsy * cy * pby + (cx * xy + cz * zy);
+          const float rz = s3 - 3.0f * fsz * cz * pbz + (cx * xz + cy * yz);
+          const float csum = cx + cy + cz;
+          const float dx = 3.0f * cx + (csum - cx) + mu;
+          const float dy = 3.0f * cy + (csum - cy) + mu;
+          const float dz = 3.0f * cz + (csum - cz) + mu;
+          const float qx = cx / dx, qy = cy / dy, qz = cz / dz;
+          const float den = csum + mu + 3.0f * (cx * qx + cy * qy + cz * qz);
+          const float p0 = (r0 - (fsx * qx * rx + fsy * qy * ry + fsz * qz * rz)) / den;
+          const float px = (rx + 3.0f * fsx * cx * p0) / dx;
+          const float py = (ry + 3.0f * fsy * cy * p0) / dy;
+          const float pz = (rz + 3.0f * fsz * cz * p0) / dz;
+          fx[0] = p0 + fsx * px; fx[1] = py; fx[2] = pz;
+          fy[0] = p0 + fsy * py; fy[1] = px; fy[2] = pz;
+          fz[0] = p0 + fsz * pz; fz[1] = px; fz[2] = py;
+          pout = make_float4(p0, px, py, pz);
+          if (a.write_psi) *sp = pout;
+        }
+        reinterpret_cast<float4*>(psl)[ci * dc + my_d] = pout;
+      }
+    }
+    __syncthreads();
+    // (3) reduce the plane's angular fluence to moments and add them into phi
+    for (int i = tid; i < P * KCH; i += kThreads) {
+      const int ci = i / KCH, n0 = (i % KCH) * KPER;
+      int lx, ly, lz;
+      const int cell = cell_of(tab.cell[pbeg + ci], lx, ly, lz);
+      if (cell < 0) continue;
+      float4 acc[KPER];
+#pragma unroll
+      for (int j = 0; j < KPER; ++j) acc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4* ps = reinterpret_cast<const float4*>(psl) + ci * dc;
+      for (int d = 0; d < dc; ++d) {
+        const float4 q = ps[d];
+#pragma unroll
+        for (int j = 0; j < KPER; ++j) {
+          if (n0 + j < NSH) {
+            const float w = ywl[d * NSH + n0 + j];
+            acc[j].x = fmaf(w, q.x, acc[j].x);
+            acc[j].y = fmaf(w, q.y, acc[j].y);
+            acc[j].z = fmaf(w, q.z, acc[j].z);
+            acc[j].w = fmaf(w, q.w, acc[j].w);
+          }
+        }
+      }
+      float4* dst = reinterpret_cast<float4*>(phi_g + (int64_t)cell * NSH * 4);
+#pragma unroll
+      for (int j = 0; j < KPER; ++j) {
+        if (n0 + j < NSH) {
+          if (first_visit) {
+            dst[n0 + j] = acc[j];
+          } else {
+            float4 old = dst[n0 + j];
+            old.x += acc[j].x; old.y += acc[j].y; old.z += acc[j].z; old.w += acc[j].w;
+            dst[n0 + j] = old;
+          }
+        }
+      }
+    }
+    // no barrier needed here: the next plane's stage phase writes tpl only, and its solve phase (which overwrites
+    // psl) starts after the next barrier
+  }
+  __syncthreads();
+
+  // ---- outgoing faces for the downstream blocks ---------------------------------------------------------------
+  {
+    const int db[3][3] = {{bi, bj + sx, bk}, {bi + sy, bj, bk}, {bi, bj, bk + sz}};
+    for (int ax = 0; ax < 3; ++ax) {
+      const int di = db[ax][0], dj = db[ax][1], dk = db[ax][2];
+      if (!(di >= 0 && di < a.nbr && dj >= 0 && dj < a.nbc && dk >= 0 && dk < a.nbs)) continue;
+      float* dst = gface_v + ((int64_t)blk * 3 + ax) * face_stride;
+      const float* srcf = face + ax * face_stride;
+      for (int i = tid; i < face_stride; i += kThreads) dst[i] = srcf[i];
+    }
+  }
+}
+
+template <int B>
+constexpr int plane_max() {
+  return PlaneTable<B>().max_plane();
+}
+
+template <int NSH, int B>
+size_t lds_bytes(int dc) {
+  return sizeof(float) * (size_t)(9 * dc * B * B + plane_max<B>() * NSH * 4 + plane_max<B>() * dc * 4 + dc * NSH);
+}
+
+constexpr size_t kMaxLds = 64 * 1024;
+
+template <int NSH, int B>
+hipError_t launch_one(const LaunchArgs& la, int nwg, hipStream_t stream) {
+  const size_t lds = lds_bytes<NSH, B>(la.dc);
+  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((ld_sweep_kernel<NSH, B>), dim3(nwg), dim3(kThreads), lds, stream, la);
+  return hipGetLastError();
+}
+
+template <int B>
+hipError_t launch_nsh(int nsh, const LaunchArgs& la, int nwg, hipStream_t stream) {
+  switch (nsh) {
+    case 1: return launch_one<1, B>(la, nwg, stream);
+    case 4: return launch_one<4, B>(la, nwg, stream);
+    case 9: return launch_one<9, B>(la, nwg, stream);
+    case 16: return launch_one<16, B>(la, nwg, stream);
+    case 25: return launch_one<25, B>(la, nwg, stream);
+    case 36: return launch_one<36, B>(la, nwg, stream);
+    case 49: return launch_one<49, B>(la, nwg, stream);
+    case 64: return launch_one<64, B>(la, nwg, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+}  // namespace citrus
+
+// ======================================================================================================================
+// schedule
+// ======================================================================================================================
+
+struct CitrusSweepPlan {
+  int R, C, S, B;
+  int nbr, nbc, nbs, nblocks;
+  int K;  // sum of (blocks - 1) over the axes: block planes run 0..K
+  // launches in order: (offset into the task table, count)
+  std::vector<std::pair<int, int>> launches;
+  std::vector<int> launch_pass;  // pass (octant pair 0..3) of each launch
+  int4* d_tasks = nullptr;
+  int ntasks = 0;
+};
+
+namespace {
+
+int block_plane(int o, int bi, int bj, int bk, int nbr, int nbc, int nbs) {
+  const int a = (o & 4) ? nbc - 1 - bj : bj;
+  const int b = (o & 2) ? nbr - 1 - bi : bi;
+  const int c = (o & 1) ? nbs - 1 - bk : bk;
+  return a + b + c;
+}
+
+}  // namespace
+
+extern "C" int citrus_sweep_plan_create(int R, int C, int S, int block, CitrusSweepPlan** out) {
+  using namespace citrus;
+  if (!out) return fail(CITRUS_ERR_ARG, "plan pointer is NULL");
+  *out = nullptr;
+  if (R < 1 || C < 1 || S < 1) return fail(CITRUS_ERR_ARG, "grid shape must be positive");
+  if (block != 4) return fail(CITRUS_ERR_UNSUPPORTED, "block edge must be 4");
+  auto* plan = new CitrusSweepPlan();
+  plan->R = R; plan->C = C; plan->S = S; plan->B = block;
+  plan->nbr = (R + block - 1) / block;
+  plan->nbc = (C + block - 1) / block;
+  plan->nbs = (S + block - 1) / block;
+  plan->nblocks = plan->nbr * plan->nbc * plan->nbs;
+  plan->K = (plan->nbr - 1) + (plan->nbc - 1) + (plan->nbs - 1);
+  const int K = plan->K;
+  std::vector<int4> table;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int oa = pass, ob = 7 - pass;
+    for (int d = 0; d <= K; ++d) {
+      // octant oa then octant ob; on the middle plane of an even K each gets its own launch (oa first)
+      const bool split = (2 * d == K);
+      for (int part = 0; part < (split ? 2 : 1); ++part) {
+        const int start = (int)table.size();
+        for (int which = 0; which < 2; ++which) {
+          if (split && which != part) continue;
+          const int o = which == 0 ? oa : ob;
+          for (int bi = 0; bi < plan->nbr; ++bi)
+            for (int bj = 0; bj < plan->nbc; ++bj)
+              for (int bk = 0; bk < plan->nbs; ++bk) {
+                const int dd = block_plane(o, bi, bj, bk, plan->nbr, plan->nbc, plan->nbs);
+                if (dd != d) continue;
+                // the pair's first visit of this block: the octant whose block plane comes first (oa on a tie)
+                const bool first = which == 0 ? (2 * d <= K) : (2 * d < K);
+                table.push_back(make_int4(o | ((first ? 1 : 0) << 3) | (which << 4), bi, bj, bk));
+              }
+        }
+        const int count = (int)table.size() - start;
+        if (count > 0) {
+          plan->launches.emplace_back(start, count);
+          plan->launch_pass.push_back(pass);
+        }
+      }
+    }
+  }
+  plan->ntasks = (int)table.size();
+  hipError_t e = hipMalloc(&plan->d_tasks, sizeof(int4) * table.size());
+  if (e == hipSuccess) e = hipMemcpy(plan->d_tasks, table.data(), sizeof(int4) * table.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (plan->d_tasks) (void)hipFree(plan->d_tasks);
+    delete plan;
+    return fail(CITRUS_ERR_HIP, std::string("sweep plan upload: ") + hipGetErrorString(e));
+  }
+  *out = plan;
+  return CITRUS_OK;
+}
+
+extern "C" void citrus_sweep_plan_destroy(CitrusSweepPlan* plan) {
+  if (!plan) return;
+  if (plan->d_tasks) (void)hipFree(plan->d_tasks);
+  delete plan;
+}
+
+extern "C" int64_t citrus_sweep_face_floats(const CitrusSweepPlan* plan, int G, int dc, int V) {
+  if (!plan || G < 1 || dc < 1 || V < 1) return -1;
+  return (int64_t)V * 2 * G * plan->nblocks * 3 * (int64_t)dc * plan->B * plan->B * 3;
+}
+
+extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepArgs* args, void* stream) {
+  using namespace citrus;
+  if (!plan || !args) return fail(CITRUS_ERR_ARG, "plan or args is NULL");
+  const CitrusSweepArgs& A = *args;
+  if (A.G < 1 || A.V < 1) return fail(CITRUS_ERR_ARG, "G and V must be >= 1");
+  if (A.dc < 1 || A.dc > 32) return fail(CITRUS_ERR_ARG, "dc must be in 1..32");
+  if (A.d0 < 0 || A.d0 + A.dc > A.D8) return fail(CITRUS_ERR_ARG, "chunk [d0, d0+dc) outside the octant");
+  if (!A.mu_t || !A.phi || !A.Y || !A.Yw || !A.cdir || !A.faces) return fail(CITRUS_ERR_ARG, "NULL table or buffer");
+  if (!A.T && !A.source) return fail(CITRUS_ERR_ARG, "need T moments and/or an explicit source");
+  if (A.write_psi && !A.source) return fail(CITRUS_ERR_ARG, "write_psi needs the explicit source buffer");
+  const int N = plan->R * plan->C * plan->S;
+  LaunchArgs la{};
+  la.R = plan->R; la.C = plan->C; la.S = plan->S; la.N = N;
+  la.nbr = plan->nbr; la.nbc = plan->nbc; la.nbs = plan->nbs; la.nblocks = plan->nblocks;
+  la.G = A.G; la.D8 = A.D8; la.d0 = A.d0; la.dc = A.dc; la.V = A.V;
+  la.write_psi = A.write_psi;
+  la.mu = A.mu_t; la.T = A.T; la.src = A.source; la.phi = A.phi;
+  la.Y = A.Y; la.Yw = A.Yw; la.cdir = A.cdir; la.faces = A.faces;
+  la.tview = (int64_t)A.G * N * A.nsh * 4;
+  la.sview = (int64_t)8 * A.dc * N * A.G * 4;
+  la.fview = (int64_t)2 * A.G * plan->nblocks * 3 * (int64_t)A.dc * plan->B * plan->B * 3;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (size_t l = 0; l < plan->launches.size(); ++l) {
+    la.tasks = plan->d_tasks + plan->launches[l].first;
+    la.ntask = plan->launches[l].second;
+    la.first = (!A.accumulate && plan->launch_pass[l] == 0) ? 1 : 0;
+    const int64_t nwg = (int64_t)la.ntask * A.G * A.V;
+    if (nwg > 0x7fffffff) return fail(CITRUS_ERR_ARG, "too many workgroups in one launch");
+    hipError_t e = plan->B == 4 ? launch_nsh<4>(A.nsh, la, (int)nwg, s) : launch_nsh<8>(A.nsh, la, (int)nwg, s);
+    if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
+    if (e == hipErrorInvalidConfiguration) return fail(CITRUS_ERR_UNSUPPORTED, "chunk needs more than 64 KiB of LDS");
+    if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
+  }
+  return CITRUS_OK;
+}

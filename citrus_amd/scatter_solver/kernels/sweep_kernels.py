@@ -1,0 +1,151 @@
+"""Host wrappers of the HIP LD sweep (``citrus_sweep_*`` in include/citrus_hip.h).
+
+Counterpart of ``citrus/scatter_solver/kernels/sweep_kernels.py``: ``kernels_usable`` (:85-108, cached per device,
+never raises), the self-check (:111-143) and the launch wrapper ``sweep_planes`` (:265-290). Here the self-check
+verifies exact particle balance (outflow + absorption = source, a property of the LD equations) of a small random
+problem swept on the device; there is no fallback path, so an unusable kernel makes ``sweep`` raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import weakref
+from typing import Optional
+
+import torch
+from torch import Tensor
+
+from ... import _hip
+
+SELF_CHECK_REL_TOL = 1e-4
+BLOCK_EDGE = 4
+MAX_CHUNK = 32
+_SELF_CHECK: dict[str, tuple[bool, str]] = {}
+
+
+class SweepSchedule:
+    """Owner of a native ``CitrusSweepPlan`` (block-diagonal task tables on the device) and its face scratch."""
+
+    def __init__(self, shape: tuple[int, int, int], device: torch.device, block: int = BLOCK_EDGE):
+        lib = _hip.library()
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            _hip.check(lib.citrus_sweep_plan_create(int(shape[0]), int(shape[1]), int(shape[2]), int(block),
+                                                    ctypes.byref(handle)), "citrus_sweep_plan_create")
+        self.handle = handle
+        self.shape = tuple(int(v) for v in shape)
+        self.device = device
+        self.block = block
+        self._faces: Optional[Tensor] = None
+        self._finalizer = weakref.finalize(self, lib.citrus_sweep_plan_destroy, handle)
+
+    def faces(self, G: int, dc: int, V: int) -> Tensor:
+        """Face scratch for (G, dc, V), grown on demand and reused across sweeps."""
+        n = int(_hip.library().citrus_sweep_face_floats(self.handle, G, dc, V))
+        if n < 0:
+            raise ValueError("invalid face-buffer request")
+        if self._faces is None or self._faces.numel() < n:
+            self._faces = torch.empty(n, device=self.device, dtype=torch.float32)
+        return self._faces
+
+    def run(self, *, mu: Tensor, phi: Tensor, Y: Tensor, Yw: Tensor, cdir: Tensor, G: int, nsh: int, D8: int,
+            d0: int, dc: int, V: int = 1, T: Optional[Tensor] = None, source: Optional[Tensor] = None,
+            accumulate: bool = False, write_psi: bool = False) -> None:
+        """Enqueue one chunk sweep on torch's current stream (no host synchronization)."""
+        N = self.shape[0] * self.shape[1] * self.shape[2]
+        for name, t in (("mu", mu), ("phi", phi), ("Y", Y), ("Yw", Yw), ("cdir", cdir)):
+            _hip.require_gpu_tensor(t, name)
+        if mu.numel() != G * N or phi.numel() != V * G * N * nsh * 4:
+            raise ValueError("mu or phi size does not match the grid")
+        if T is not None:
+            _hip.require_gpu_tensor(T, "T")
+            if T.numel() != V * G * N * nsh * 4:
+                raise ValueError("T size does not match [V, G, N, nsh, 4]")
+        if source is not None:
+            _hip.require_gpu_tensor(source, "source")
+            if source.numel() != V * 8 * dc * N * G * 4:
+                raise ValueError("source size does not match [V, 8 dc, N, G, 4]")
+        if Y.shape != (8 * D8, nsh) or Yw.shape != (8 * D8, nsh) or cdir.shape != (8 * D8, 3):
+            raise ValueError("direction tables do not match the quadrature")
+        faces = self.faces(G, dc, V)
+        args = _hip.SweepArgs(G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V, accumulate=int(accumulate),
+                              write_psi=int(write_psi), mu_t=mu.data_ptr(), T=_hip.ptr(T), source=_hip.ptr(source),
+                              phi=phi.data_ptr(), Y=Y.data_ptr(), Yw=Yw.data_ptr(), cdir=cdir.data_ptr(),
+                              faces=faces.data_ptr())
+        lib = _hip.library()
+        with torch.cuda.device(self.device):
+            _hip.check(lib.citrus_sweep_run(self.handle, ctypes.byref(args), _hip.stream_of(self.device)),
+                       "citrus_sweep_run")
+
+
+def hip_available() -> tuple[bool, str]:
+    """Whether the HIP library loads (honouring the ``CITRUS_DISABLE_HIP`` kill switch)."""
+    if os.environ.get(_hip.DISABLE_ENV_VAR, "") not in ("", "0"):
+        return False, f"disabled by {_hip.DISABLE_ENV_VAR}"
+    try:
+        _hip.library()
+    except RuntimeError as e:
+        return False, str(e)
+    if not torch.cuda.is_available():
+        return False, "no GPU device"
+    return True, "HIP library loaded"
+
+
+def kernels_usable(device) -> tuple[bool, str]:
+    """Whether the HIP sweep is usable on ``device``: library loaded, gfx950 device and the self-check passed.
+
+    The verdict is cached per device; the function never raises.
+    """
+    ok, why = hip_available()
+    if not ok:
+        return ok, why
+    device = torch.device(device)
+    if device.type != "cuda":
+        return False, "not a GPU device"
+    key = str(device)
+    if key not in _SELF_CHECK:
+        try:
+            arch = torch.cuda.get_device_properties(device).gcnArchName
+            if not arch.startswith("gfx950"):
+                _SELF_CHECK[key] = (False, f"device is {arch}, the library is built for gfx950")
+            else:
+                _SELF_CHECK[key] = _self_check(device)
+        except Exception as e:  # noqa: BLE001
+            _SELF_CHECK[key] = (False, f"self-check failed ({type(e).__name__}: {str(e)[:300]})")
+    return _SELF_CHECK[key]
+
+
+def _self_check(device: torch.device) -> tuple[bool, str]:
+    """Particle balance of every direction and group of a small random problem, evaluated in float64."""
+    from ..angular_quadrature import product_quadrature
+    from ..geometry import Grid
+    from ..transport_sweep import _sweep_unchecked, make_sweep_context
+
+    grid = Grid((3, 4, 5), 0.7, 1.3, 0.9)
+    context = make_sweep_context(grid, product_quadrature(4, 8), 1, device=device, dtype=torch.float32)
+    G = 3
+    gen = torch.Generator().manual_seed(11)
+    mu = (torch.rand(G, *grid.shape, generator=gen) * 2.0 + 0.1)
+    src = torch.randn(context.num_directions, G, 4, *grid.shape, generator=gen)
+    src[:, :, 0] = src[:, :, 0].abs() + 0.1
+    _, psi = _sweep_unchecked(context, mu.to(device), lambda idx: src.to(device).index_select(0, idx),
+                              direction_chunk=3, return_psi=True)
+    psi = psi.double().cpu()
+    mu64, src64 = mu.double(), src.double()
+    dx, dy, dz = grid.deltas
+    V = dx * dy * dz
+    omega = context.quadrature.omega.double().cpu()
+    worst = 0.0
+    for m in range(context.num_directions):
+        out = torch.zeros(G, dtype=torch.float64)
+        for a, dim in ((0, 2), (1, 1), (2, 3)):  # axis -> dim of [G, R, C, S]
+            s = 1.0 if omega[m, a] >= 0 else -1.0
+            face = (psi[m, :, 0] + s * psi[m, :, 1 + a]).select(dim, -1 if s > 0 else 0)
+            out += abs(float(omega[m, a])) * (V / grid.deltas[a]) * face.reshape(G, -1).sum(1)
+        lhs = out + V * (mu64 * psi[m, :, 0]).reshape(G, -1).sum(1)
+        rhs = V * src64[m, :, 0].reshape(G, -1).sum(1)
+        worst = max(worst, float(((lhs - rhs).abs() / rhs.abs().clamp_min(1e-30)).max()))
+    if not worst <= SELF_CHECK_REL_TOL:
+        return False, f"self-check: particle balance of the HIP sweep off by {worst:.2e} (rel)"
+    return True, f"HIP sweep passes the particle-balance self-check (max rel dev {worst:.1e})"

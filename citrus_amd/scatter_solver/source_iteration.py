@@ -1,0 +1,220 @@
+"""Source iteration over collision order and the scattering-source moments (HIP).
+
+Keeps the interface of ``citrus/scatter_solver/source_iteration.py``: ``SourceIterationResult`` (:28),
+``scatter_moments`` (:49), ``scatter_source_function`` (:65), ``multiple_collision_source_moments`` (:109),
+``source_iteration`` (:138). Scattering-source moments live in the group-major layout ``[G, R, C, S, (L+1)^2, 4]``
+of the HIP kernels; the source functions built here carry them (``hip_moments``) so the sweep kernel expands
+``sum_n Y_n(Omega) T_n`` itself. The convergence norms are taken in float64 on the device.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import torch
+from torch import Tensor
+
+from .. import _hip
+from .transport_sweep import SourceFunction, SweepContext, sweep
+
+__all__ = ["SourceIterationResult", "MaterialTransfer", "source_iteration", "scatter_moments",
+           "scatter_source_function", "summed_source_function", "multiple_collision_source_moments"]
+
+
+@dataclass
+class SourceIterationResult:
+    """Summed moments, number of sweeps, relative-change history, per-sweep norms, convergence flag (:28-46)."""
+
+    phi_total: Tensor
+    num_iterations: int
+    history: list[float] = field(default_factory=list)
+    phi_norms: list[float] = field(default_factory=list)
+    converged: bool = False
+
+
+@dataclass(frozen=True, eq=False)
+class MaterialTransfer:
+    """Legendre transfer moments in factored form, ``mu_s_l(cell) = sum_m rho[m, cell] sigma[m, gi, go, l]``.
+
+    The dense ``[G_in, G_out, L+1, R, C, S]`` volume of ``transfer_moments_volume`` (cross_sections.py:886) is never
+    formed: the HIP kernel combines the densities with the per-material table on the fly.
+
+    Attributes:
+        rho: [M, R, C, S] float32 densities on the GPU in g/cm^3.
+        sigma: [M, G, G, L+1] float32 mass transfer moments in cm^2/g.
+    """
+
+    rho: Tensor
+    sigma: Tensor
+    downscatter_only: bool = field(init=False, default=False)
+
+    def __post_init__(self):
+        # one look at the small table when the transfer is built, so no later call synchronizes the host
+        s = self.sigma.detach().cpu()
+        lower = torch.tril(torch.ones(s.shape[1], s.shape[2], dtype=torch.bool), diagonal=-1)  # g_in > g_out
+        object.__setattr__(self, "downscatter_only", bool((s[:, lower] == 0).all()))
+
+
+def _group_major(phi: Tensor) -> Tensor:
+    """Contiguous [G, R, C, S, nsh, 4] storage of moments given as [G, nsh, 4, R, C, S] (free for sweep outputs)."""
+    return phi.permute(0, 3, 4, 5, 1, 2).contiguous()
+
+
+def scatter_moments(mu_s_l_volume, phi: Tensor) -> Tensor:
+    """Scattering-source moments ``T[g, n] = sum_g' mu_s_l[g', g, l(n)] phi[g', n]`` (source_iteration.py:49).
+
+    Args:
+        mu_s_l_volume: dense ``[G_in, G_out, L+1, R, C, S]`` float32 transfer volume, or a ``MaterialTransfer``.
+        phi: [G, (L+1)^2, 4, R, C, S] float32 moments on the GPU.
+
+    Returns:
+        T [G, (L+1)^2, 4, R, C, S], a view of group-major storage.
+    """
+    G, nsh = int(phi.shape[0]), int(phi.shape[1])
+    spatial = tuple(phi.shape[3:])
+    N = spatial[0] * spatial[1] * spatial[2]
+    src = _group_major(phi)
+    _hip.require_gpu_tensor(src, "phi")
+    out = torch.empty_like(src)
+    lib = _hip.library()
+    stream = _hip.stream_of(phi.device)
+    with torch.cuda.device(phi.device):
+        if isinstance(mu_s_l_volume, MaterialTransfer):
+            rho = mu_s_l_volume.rho.contiguous()
+            sigma = mu_s_l_volume.sigma.contiguous()
+            _hip.require_gpu_tensor(rho, "rho")
+            _hip.require_gpu_tensor(sigma, "sigma")
+            if tuple(rho.shape[1:]) != spatial or sigma.shape[1] != G or sigma.shape[2] != G:
+                raise ValueError("MaterialTransfer shapes do not match phi")
+            _hip.check(lib.citrus_scatter_moments_factored(
+                rho.data_ptr(), int(rho.shape[0]), sigma.data_ptr(), int(sigma.shape[3]) - 1,
+                int(mu_s_l_volume.downscatter_only), src.data_ptr(), out.data_ptr(), G, nsh, N, 1, stream),
+                "citrus_scatter_moments_factored")
+        else:
+            transfer = mu_s_l_volume.contiguous()
+            _hip.require_gpu_tensor(transfer, "mu_s_l_volume")
+            if transfer.shape[0] != G or transfer.shape[1] != G or tuple(transfer.shape[3:]) != spatial:
+                raise ValueError(f"transfer volume {tuple(transfer.shape)} does not match phi {tuple(phi.shape)}")
+            _hip.check(lib.citrus_scatter_moments_dense(transfer.data_ptr(), int(transfer.shape[2]) - 1,
+                                                        src.data_ptr(), out.data_ptr(), G, nsh, N, 1, stream),
+                       "citrus_scatter_moments_dense")
+    return out.permute(0, 4, 5, 1, 2, 3)
+
+
+class MomentSource:
+    """Source function ``S(Omega_m) = sum_n Y_n(Omega_m) T_n`` (+ an optional extra cell-major source).
+
+    The sweep recognises ``hip_moments`` and expands the moments inside its kernel. Called directly, it follows the
+    cell-major source protocol of ``scatter_source_function`` (source_iteration.py:94-103).
+    """
+
+    cell_major = True
+
+    def __init__(self, context: SweepContext, moments: Tensor, extra: Optional[SourceFunction] = None):
+        self.context = context
+        self.hip_moments = moments  # [G, R, C, S, nsh, 4]
+        self.extra = extra
+
+    def __call__(self, direction_index: Tensor, out: Optional[Tensor] = None, accumulate: bool = False) -> Tensor:
+        G, R, C, S, nsh, _ = self.hip_moments.shape
+        Y = self.context.basis.Y.to(device=self.hip_moments.device, dtype=self.hip_moments.dtype)
+        Y_sel = Y.index_select(0, direction_index.to(Y.device))
+        matrix = self.hip_moments.permute(4, 1, 2, 3, 0, 5).reshape(nsh, -1)  # [nsh, (R, C, S, G, 4)]
+        shape = (Y_sel.shape[0], R, C, S, G, 4)
+        if out is None:
+            result = (Y_sel @ matrix).view(shape)
+            if self.extra is not None:
+                result = result + self.extra(direction_index, out=torch.empty_like(result))
+            return result
+        flat = out.view(Y_sel.shape[0], -1)
+        if self.extra is not None:
+            self.extra(direction_index, out=out)
+            flat.addmm_(Y_sel, matrix)
+        elif accumulate:
+            flat.addmm_(Y_sel, matrix)
+        else:
+            torch.mm(Y_sel, matrix, out=flat)
+        return out
+
+
+def scatter_source_function(sweep_context: SweepContext, T: Tensor) -> MomentSource:
+    """Source function of the scattering-source moments T [G, (L+1)^2, 4, R, C, S] (source_iteration.py:65)."""
+    if T.shape[1] != sweep_context.num_moments:
+        raise ValueError(f"T has {T.shape[1]} spherical-harmonic moments, context has {sweep_context.num_moments}")
+    if T.dim() != 6 or T.shape[2] != 4 or tuple(T.shape[3:]) != tuple(sweep_context.shape):
+        raise ValueError(f"T must be [G, {sweep_context.num_moments}, 4, {sweep_context.shape}], got {tuple(T.shape)}")
+    return MomentSource(sweep_context, _group_major(T))
+
+
+def summed_source_function(moment_source: MomentSource, extra: SourceFunction) -> MomentSource:
+    """``extra + moment_source`` in one sweep: the summed source of the CITRUS step (citrus.py:232-236)."""
+    if not source_is_cell_major_fn(extra):
+        raise ValueError("the extra source must be cell-major")
+    return MomentSource(moment_source.context, moment_source.hip_moments, extra=extra)
+
+
+def source_is_cell_major_fn(fn) -> bool:
+    return bool(getattr(fn, "cell_major", False))
+
+
+def scatter_moments_views(transfer: MaterialTransfer, phi_views: Tensor, out: Optional[Tensor] = None) -> Tensor:
+    """Batched ``scatter_moments`` of V views sharing one factored transfer: [V, G, R, C, S, nsh, 4] in and out."""
+    V, G, R, C, S, nsh, four = (int(v) for v in phi_views.shape)
+    _hip.require_gpu_tensor(phi_views, "phi_views")
+    rho, sigma = transfer.rho.contiguous(), transfer.sigma.contiguous()
+    if out is None:
+        out = torch.empty_like(phi_views)
+    lib = _hip.library()
+    with torch.cuda.device(phi_views.device):
+        _hip.check(lib.citrus_scatter_moments_factored(
+            rho.data_ptr(), int(rho.shape[0]), sigma.data_ptr(), int(sigma.shape[3]) - 1,
+            int(transfer.downscatter_only), phi_views.data_ptr(), out.data_ptr(), G, nsh, R * C * S, V,
+            _hip.stream_of(phi_views.device)), "citrus_scatter_moments_factored")
+    return out
+
+
+def multiple_collision_source_moments(mu_s_l_volume_coarse, phi_total: Tensor) -> Tensor:
+    """Scattering operator applied to the total collided fluence (source_iteration.py:109)."""
+    return scatter_moments(mu_s_l_volume_coarse, phi_total)
+
+
+def _scalar_norms(*phis: Tensor) -> list[float]:
+    """float64 norms of the scalar-flux cell means ``phi[:, 0, 0]``, one host synchronization (:126-135)."""
+    return torch.stack([torch.linalg.vector_norm(p[:, 0, 0].to(torch.float64)) for p in phis]).tolist()
+
+
+def source_iteration(sweep_context: SweepContext, mu_t_coarse: Tensor, mu_s_l_volume_coarse,
+                     first_collision_source_function: SourceFunction, tolerance: float, max_iterations: int,
+                     direction_chunk: Optional[int] = None,
+                     callback: Optional[Callable[[int, Tensor, float], None]] = None) -> SourceIterationResult:
+    """Source iteration over collision order from the first-collision source (source_iteration.py:138-204)."""
+    if max_iterations < 1:
+        raise ValueError("max_iterations must be >= 1")
+    phi_total: Optional[Tensor] = None
+    history: list[float] = []
+    norms: list[float] = []
+    converged = False
+    source = first_collision_source_function
+    n = 0
+    for n in range(1, max_iterations + 1):
+        phi_n = sweep(sweep_context, mu_t_coarse, source, direction_chunk=direction_chunk)
+        source = None
+        phi_total = phi_n.clone() if phi_total is None else phi_total.add_(phi_n)
+        norm_n, norm_total = _scalar_norms(phi_n, phi_total)
+        rel = norm_n / norm_total if norm_total > 0 else 0.0
+        history.append(rel)
+        norms.append(norm_n)
+        if callback is not None:
+            callback(n, phi_n, rel)
+        if rel < tolerance:
+            converged = True
+            break
+        if n == max_iterations:
+            break
+        source = scatter_source_function(sweep_context, scatter_moments(mu_s_l_volume_coarse, phi_n))
+        del phi_n
+    assert phi_total is not None
+    return SourceIterationResult(phi_total=phi_total, num_iterations=n, history=history, phi_norms=norms,
+                                 converged=converged)

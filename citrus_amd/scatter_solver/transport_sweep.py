@@ -1,0 +1,318 @@
+"""Batched linear-discontinuous discrete-ordinates sweep on the transport grid (HIP).
+
+Keeps the interface of ``citrus/scatter_solver/transport_sweep.py``: ``SweepContext`` (:73), ``make_sweep_context``
+(:156), ``sweep`` (:457) with its source-function protocol (:38-43, :379-450): a source function maps [Dsel] reordered
+direction indices to [Dsel, G, 4, R, C, S] moments, may write into an ``out=`` buffer, or (``cell_major = True``) writes
+[Dsel, R, C, S, G, 4] into the sweep's buffer. It is called once per direction chunk with all 8 dc directions,
+octant-major. The result ``phi [G, (L+1)^2, 4, R, C, S]`` is a view of group-major storage
+``[G, R, C, S, (L+1)^2, 4]``, the layout the HIP kernels stream.
+
+Backends: "auto" and "hip". The sweep runs only on the HIP kernels (float32 on a gfx950 device); there is no CPU
+fallback, and an unusable kernel raises ``RuntimeError``. Sources that carry scattering-source moments
+(``scatter_source_function``) are expanded inside the sweep kernel, so neither the per-direction source nor the
+angular fluence is materialised.
+"""
+
+from __future__ import annotations
+
+import inspect
+from dataclasses import dataclass
+from typing import Callable, Literal, Optional, Union
+
+import torch
+from torch import Tensor
+
+from .angular_quadrature import Quadrature, SphericalHarmonicBasis, spherical_harmonic_basis
+from .geometry import Grid
+from .kernels import sweep_kernels
+from .wavefront import OCTANT_SIGNS, SweepPlan, build_sweep_plan
+
+__all__ = ["SourceFunction", "Backend", "SweepContext", "make_sweep_context", "sweep",
+           "source_function_accepts_out", "source_is_cell_major"]
+
+SourceFunction = Callable[[Tensor], Tensor]
+Backend = Literal["auto", "hip"]
+_BACKENDS = ("auto", "hip")
+
+
+@dataclass(frozen=True, eq=False)
+class HipTables:
+    """Device tables of the HIP sweep: schedule, float32 SH tables and direction coefficients."""
+
+    schedule: sweep_kernels.SweepSchedule
+    Y: Tensor      # [M, nsh]
+    Yw: Tensor     # [M, nsh]
+    cdir: Tensor   # [M, 3]
+
+
+@dataclass(frozen=True, eq=False)
+class SweepContext:
+    """Everything the sweep needs that depends only on the grid and the quadrature (transport_sweep.py:73-153).
+
+    The quadrature is reordered by octant: reordered direction ``m = octant * D8 + d``.
+    """
+
+    shape: tuple[int, int, int]
+    deltas: tuple[float, float, float]
+    L: int
+    plan: SweepPlan
+    quadrature: Quadrature
+    permutation: Tensor
+    basis: SphericalHarmonicBasis
+    directions_per_octant: int
+    c: Tensor
+    s: Tensor
+    backend: str = "auto"
+    hip: Optional[HipTables] = None
+
+    @property
+    def num_cells(self) -> int:
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    @property
+    def num_planes(self) -> int:
+        return self.plan.num_planes
+
+    @property
+    def num_directions(self) -> int:
+        return 8 * self.directions_per_octant
+
+    @property
+    def num_moments(self) -> int:
+        return (self.L + 1) ** 2
+
+    @property
+    def device(self) -> torch.device:
+        return self.c.device
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return self.c.dtype
+
+    def direction_indices(self, start: int, stop: int, octants: Optional[Tensor] = None) -> Tensor:
+        """Reordered indices ``octants[i] * D8 + start + j`` of the per-octant chunk [start, stop), octant-major."""
+        d = torch.arange(start, stop, device=self.device, dtype=torch.int64)
+        o = torch.arange(8, device=self.device, dtype=torch.int64) if octants is None else octants.to(self.device)
+        return (o[:, None] * self.directions_per_octant + d[None, :]).reshape(-1)
+
+
+def _check_backend(backend: str) -> None:
+    if backend in ("torch", "triton"):
+        raise ValueError(f"backend {backend!r} does not exist in this implementation: the sweep runs on the HIP "
+                         "kernels only (backends 'auto' and 'hip')")
+    if backend not in _BACKENDS:
+        raise ValueError(f"backend must be one of {_BACKENDS}, got {backend!r}")
+
+
+def make_sweep_context(grid: Grid, quadrature: Quadrature, L: int, device: Union[torch.device, str, None] = None,
+                       dtype: torch.dtype = torch.float32, backend: Backend = "auto") -> SweepContext:
+    """Build the sweep context of a grid and a quadrature, reordering the quadrature by octant.
+
+    The quadrature must hold the same number of directions in every octant. On a GPU device with float32 tables the
+    HIP schedule and device tables are built here (backend "hip" requires them and raises if they cannot be built).
+    """
+    _check_backend(backend)
+    D8 = quadrature.directions_per_octant
+    if D8 is None or D8 < 1:
+        raise ValueError("the quadrature must hold the same (non-zero) number of directions in every octant")
+    if L < 0:
+        raise ValueError("L must be >= 0")
+    if L > 7:
+        raise ValueError("the HIP sweep supports L <= 7 ((L+1)^2 <= 64 moments)")
+    device = torch.device(quadrature.device if device is None else device)
+    if backend == "hip" and device.type != "cuda":
+        raise ValueError("backend='hip' requires a GPU device")
+    permutation, _ = quadrature.by_octant()
+    reordered64 = quadrature.reordered(permutation).to(dtype=torch.float64)
+    expected = torch.arange(8).repeat_interleave(D8)
+    if not torch.equal(reordered64.octant.cpu(), expected):
+        raise RuntimeError("octant reordering failed")
+    basis64 = spherical_harmonic_basis(L, reordered64.to(device="cpu"))
+    deltas = tuple(float(v) for v in grid.deltas)
+    omega = reordered64.omega.cpu().reshape(8, D8, 3)
+    c = omega.abs() / torch.tensor(deltas, dtype=torch.float64)
+    s = torch.where(omega < 0, -torch.ones_like(omega), torch.ones_like(omega))
+    if not torch.equal(s.to(torch.int64), OCTANT_SIGNS[:, None, :].expand(8, D8, 3)):
+        raise RuntimeError("direction signs disagree with the octant signs")
+    hip = None
+    if device.type == "cuda" and dtype == torch.float32:
+        try:
+            schedule = sweep_kernels.SweepSchedule(tuple(grid.shape), device)
+            hip = HipTables(schedule=schedule,
+                            Y=basis64.Y.to(device=device, dtype=torch.float32).contiguous(),
+                            Yw=basis64.Y_weighted.to(device=device, dtype=torch.float32).contiguous(),
+                            cdir=c.reshape(-1, 3).to(device=device, dtype=torch.float32).contiguous())
+        except RuntimeError:
+            if backend == "hip":
+                raise
+    elif backend == "hip":
+        raise ValueError("backend='hip' needs float32 tables on a GPU device")
+    basis = SphericalHarmonicBasis(L=L, Y=basis64.Y.to(device=device, dtype=dtype),
+                                   Y_weighted=basis64.Y_weighted.to(device=device, dtype=dtype),
+                                   degree_of_moment=basis64.degree_of_moment.to(device))
+    return SweepContext(shape=tuple(grid.shape), deltas=deltas, L=L, plan=build_sweep_plan(tuple(grid.shape)),
+                        quadrature=reordered64.to(device=device, dtype=dtype), permutation=permutation.to(device),
+                        basis=basis, directions_per_octant=D8, c=c.to(device=device, dtype=dtype).contiguous(),
+                        s=s.to(device=device, dtype=dtype).contiguous(), backend=backend, hip=hip)
+
+
+def source_function_accepts_out(source_function: SourceFunction) -> bool:
+    """Whether a source function takes an ``out`` keyword (transport_sweep.py:379-393)."""
+    try:
+        params = inspect.signature(source_function).parameters
+    except (TypeError, ValueError):
+        return False
+    p = params.get("out")
+    return p is not None and p.kind in (inspect.Parameter.POSITIONAL_OR_KEYWORD, inspect.Parameter.KEYWORD_ONLY)
+
+
+def source_is_cell_major(source_function: SourceFunction) -> bool:
+    """Whether a source function writes cell-major [Dsel, R, C, S, G, 4] moments (its ``cell_major`` attribute)."""
+    return bool(getattr(source_function, "cell_major", False))
+
+
+def _resolve_chunk(context: SweepContext, direction_chunk: Optional[int]) -> int:
+    D8 = context.directions_per_octant
+    if direction_chunk is None:
+        chunk = D8
+    else:
+        chunk = int(direction_chunk)
+        if chunk < 1:
+            raise ValueError("direction_chunk must be >= 1")
+    return min(chunk, D8, sweep_kernels.MAX_CHUNK)
+
+
+def _fill_source(context: SweepContext, source_function: SourceFunction, d0: int, d1: int, G: int,
+                 out: Tensor) -> None:
+    """Evaluate the source of chunk [d0, d1) of every octant into the cell-major buffer ``out`` [8 dc, N, G, 4]
+    (transport_sweep.py:408-450)."""
+    chunk = d1 - d0
+    N = context.num_cells
+    index = context.direction_indices(d0, d1)
+    if source_is_cell_major(source_function):
+        view = out.view((8 * chunk,) + tuple(context.shape) + (G, 4))
+        got = source_function(index, out=view)
+        if got.data_ptr() != view.data_ptr() or tuple(got.shape) != tuple(view.shape):
+            raise ValueError("a cell-major source function must write its [Dsel, R, C, S, G, 4] moments into `out`")
+        return
+    expected = (8 * chunk, G, 4) + tuple(context.shape)
+    if source_function_accepts_out(source_function):
+        got = source_function(index, out=torch.empty(expected, device=out.device, dtype=out.dtype))
+    else:
+        got = source_function(index)
+    if tuple(got.shape) != expected:
+        raise ValueError(f"source_function returned shape {tuple(got.shape)}, expected {expected}")
+    out.view(8 * chunk, N, 4 * G).copy_(got.reshape(8 * chunk, 4 * G, N).transpose(1, 2))
+
+
+def _resolve_hip(context: SweepContext, backend: Optional[str], mu_t: Tensor) -> None:
+    chosen = context.backend if backend is None else backend
+    _check_backend(chosen)
+    if mu_t.device.type != "cuda" or mu_t.dtype != torch.float32:
+        raise ValueError(f"the HIP sweep needs float32 inputs on a GPU device, got {mu_t.dtype} on {mu_t.device}")
+    if context.hip is None:
+        ok, why = sweep_kernels.hip_available()
+        raise RuntimeError(f"the sweep context has no HIP tables (float32 GPU context needed): {why}")
+    ok, why = sweep_kernels.kernels_usable(mu_t.device)
+    if not ok:
+        raise RuntimeError(f"the HIP sweep kernel is unusable: {why}")
+
+
+def sweep(sweep_context: SweepContext, mu_t: Tensor, source_function: SourceFunction,
+          direction_chunk: Optional[int] = None, return_psi: bool = False, backend: Optional[Backend] = None
+          ) -> Union[Tensor, tuple[Tensor, Tensor]]:
+    """LD discrete-ordinates sweep of all directions and groups with vacuum boundaries (transport_sweep.py:457).
+
+    Args:
+        sweep_context: context of the grid and the quadrature.
+        mu_t: [G, R, C, S] float32 total attenuation in 1/cm on the GPU.
+        source_function: source function (see the module docstring); one carrying ``hip_moments`` (from
+            ``scatter_source_function``) is expanded inside the kernel.
+        direction_chunk: directions per octant swept together. Defaults to all (at most 64).
+        return_psi: also return the angular fluence [M, G, 4, R, C, S] in reordered direction order.
+        backend: "auto" or "hip". Defaults to the context's backend.
+
+    Returns:
+        phi [G, (L+1)^2, 4, R, C, S] (a view of [G, R, C, S, (L+1)^2, 4] storage), and psi with ``return_psi``.
+    """
+    if mu_t.dim() != 4 or tuple(mu_t.shape[1:]) != tuple(sweep_context.shape):
+        raise ValueError(f"mu_t must be [G, {sweep_context.shape}], got {tuple(mu_t.shape)}")
+    _resolve_hip(sweep_context, backend, mu_t)
+    return _sweep_unchecked(sweep_context, mu_t, source_function, direction_chunk, return_psi)
+
+
+def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: SourceFunction,
+                     direction_chunk: Optional[int] = None, return_psi: bool = False):
+    G = int(mu_t.shape[0])
+    R, C, S = context.shape
+    N = context.num_cells
+    D8 = context.directions_per_octant
+    nsh = context.num_moments
+    hip = context.hip
+    device = mu_t.device
+    mu = mu_t.reshape(G, N).contiguous()
+    Dc = _resolve_chunk(context, direction_chunk)
+    T = getattr(source_function, "hip_moments", None)
+    extra = getattr(source_function, "extra", None) if T is not None else None
+    if T is not None:
+        if T.shape != (G, R, C, S, nsh, 4) or not T.is_contiguous() or T.device != device:
+            raise ValueError(f"moment source must carry contiguous [G, R, C, S, {nsh}, 4] moments on {device}")
+    phi_store = torch.empty(G, R, C, S, nsh, 4, device=device, dtype=torch.float32)
+    psi_out = torch.empty(context.num_directions, G, 4, N, device=device, dtype=torch.float32) if return_psi else None
+    for d0 in range(0, D8, Dc):
+        d1 = min(d0 + Dc, D8)
+        dc = d1 - d0
+        buf = None
+        if T is None or extra is not None or return_psi:
+            buf = torch.empty(8 * dc, N, G, 4, device=device, dtype=torch.float32)
+            if T is None:
+                _fill_source(context, source_function, d0, d1, G, buf)
+            elif extra is not None:
+                _fill_source(context, extra, d0, d1, G, buf)
+            else:
+                buf.zero_()
+        hip.schedule.run(mu=mu, phi=phi_store, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0, dc=dc,
+                         T=T, source=buf, accumulate=d0 > 0, write_psi=return_psi)
+        if psi_out is not None:
+            psi_out[context.direction_indices(d0, d1)] = buf.permute(0, 2, 3, 1)
+    phi = phi_store.permute(0, 4, 5, 1, 2, 3)
+    if psi_out is not None:
+        return phi, psi_out.view(context.num_directions, G, 4, R, C, S)
+    return phi
+
+
+def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out: Optional[Tensor] = None,
+                direction_chunk: Optional[int] = None) -> Tensor:
+    """Sweep V views that share the material fields, each from its own scattering-source moments, in one launch
+    sequence: the batched form of one ``sweep`` per view of an outer iteration (citrus.py:1920-1935).
+
+    Args:
+        sweep_context: context of the grid and the quadrature (float32, GPU).
+        mu_t: [G, R, C, S] float32 total attenuation shared by the views.
+        moments: [V, G, R, C, S, nsh, 4] float32 contiguous scattering-source moments (group-major layout).
+        out: optional [V, G, R, C, S, nsh, 4] float32 output buffer.
+        direction_chunk: directions per octant per launch sequence. Defaults to all.
+
+    Returns:
+        [V, G, R, C, S, nsh, 4] fluence moments.
+    """
+    if mu_t.dim() != 4 or tuple(mu_t.shape[1:]) != tuple(sweep_context.shape):
+        raise ValueError(f"mu_t must be [G, {sweep_context.shape}], got {tuple(mu_t.shape)}")
+    _resolve_hip(sweep_context, None, mu_t)
+    G = int(mu_t.shape[0])
+    R, C, S = sweep_context.shape
+    nsh = sweep_context.num_moments
+    if moments.dim() != 7 or tuple(moments.shape[1:]) != (G, R, C, S, nsh, 4) or not moments.is_contiguous():
+        raise ValueError(f"moments must be contiguous [V, {G}, {R}, {C}, {S}, {nsh}, 4]")
+    V = int(moments.shape[0])
+    if out is None:
+        out = torch.empty_like(moments)
+    D8 = sweep_context.directions_per_octant
+    Dc = _resolve_chunk(sweep_context, direction_chunk)
+    hip = sweep_context.hip
+    mu = mu_t.reshape(G, -1).contiguous()
+    for d0 in range(0, D8, Dc):
+        dc = min(d0 + Dc, D8) - d0
+        hip.schedule.run(mu=mu, phi=out, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V,
+                         T=moments, accumulate=d0 > 0)
+    return out

@@ -1,0 +1,95 @@
+"""The C-ABI library loads and exports every entry point declared in include/*.h; host-side API checks (CPU)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+import torch
+
+from citrus_amd import _hip
+from citrus_amd.scatter_solver.angular_quadrature import level_symmetric_quadrature, product_quadrature
+from citrus_amd.scatter_solver.geometry import Grid
+from citrus_amd.scatter_solver.transport_sweep import make_sweep_context, sweep
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions() -> list[str]:
+    names = []
+    for header in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        names += re.findall(r"\b(citrus_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    if not _hip.LIBRARY_PATH.exists():
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(str(_hip.LIBRARY_PATH))
+    declared = declared_functions()
+    assert declared, "no declarations found"
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_hip.exported_symbols()) == set(declared), "the ctypes bindings and the header disagree"
+    assert _hip.library().citrus_abi_version() == _hip.ABI_VERSION
+
+
+def test_host_side_errors_without_device():
+    if not _hip.LIBRARY_PATH.exists():
+        pytest.skip("library not built")
+    lib = _hip.library()
+    plan = ctypes.c_void_p()
+    assert lib.citrus_sweep_plan_create(0, 4, 4, 4, ctypes.byref(plan)) == 1
+    assert b"positive" in lib.citrus_last_error()
+    assert lib.citrus_sweep_plan_create(4, 4, 4, 5, ctypes.byref(plan)) == 3
+    assert lib.citrus_sweep_run(None, None, None) == 1
+
+
+def test_cpu_context_and_sweep_rejects_cpu():
+    grid = Grid((3, 4, 5), 0.5)
+    ctx = make_sweep_context(grid, level_symmetric_quadrature(4), 2, device="cpu")
+    assert ctx.num_directions == 24 and ctx.directions_per_octant == 3 and ctx.hip is None
+    assert ctx.num_planes == 3 + 4 + 5 - 2
+    with pytest.raises(ValueError):
+        sweep(ctx, torch.ones(2, 3, 4, 5), lambda idx: torch.zeros(idx.numel(), 2, 4, 3, 4, 5))
+    with pytest.raises(ValueError):
+        make_sweep_context(grid, product_quadrature(4, 8), 1, device="cpu", backend="hip")
+    with pytest.raises(ValueError):
+        make_sweep_context(grid, product_quadrature(4, 8), 1, device="cpu", backend="triton")
+
+
+def test_wavefront_plan_covers_each_entry_once():
+    from citrus_amd.scatter_solver.wavefront import build_sweep_plan
+
+    plan = build_sweep_plan((3, 4, 2))
+    N = plan.num_cells
+    pairs = set(zip(plan.octant.tolist(), plan.cell.tolist()))
+    assert len(pairs) == 8 * N and plan.num_planes == 7
+    # upstream neighbours lie on the previous plane of the same octant
+    plane_of = {}
+    for p in range(plan.num_planes):
+        sl = plan.plane_entries(p)
+        for o, c in zip(plan.octant[sl].tolist(), plan.cell[sl].tolist()):
+            plane_of[(o, c)] = p
+    for e in range(8 * N):
+        o, c = int(plan.octant[e]), int(plan.cell[e])
+        for u in plan.upstream[e].tolist():
+            if u >= 0:
+                assert plane_of[(o, u)] == plane_of[(o, c)] - 1
+
+
+def test_product_sh_matches_oracle():
+    import numpy as np
+
+    from citrus_amd.scatter_solver.angular_quadrature import real_spherical_harmonics
+    from oracle import quadrature as oq
+
+    for q in (level_symmetric_quadrature(12), product_quadrature(8, 16)):
+        Y = real_spherical_harmonics(7, q.omega).numpy()
+        assert np.abs(Y - oq.real_spherical_harmonics(7, q.omega.numpy())).max() < 1e-12
+    om, w = oq.level_symmetric_quadrature(12)
+    q = level_symmetric_quadrature(12)
+    assert np.abs(q.omega.numpy() - om).max() < 1e-15 and np.abs(q.weight.numpy() - w).max() < 1e-15

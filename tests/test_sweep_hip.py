@@ -1,0 +1,201 @@
+"""Parity of the HIP LD sweep with the FP64 oracle (GPU).
+
+Mirrors the reference's sweep tests (tests/test_transport_sweep.py): 1-D slab (:91), particle balance (:155), dense
+global assembly (:237), on-the-fly moments (:340), chunk invariance (:355), heterogeneous random problem (:400),
+source called once per chunk (:461), cell-major sources (:496); plus the fused moment source and the configs'
+shapes. The float32 bar is ``F32_REL = 1e-5`` of the field maximum."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from citrus_amd.scatter_solver.angular_quadrature import level_symmetric_quadrature, product_quadrature
+from citrus_amd.scatter_solver.geometry import Grid
+from citrus_amd.scatter_solver.source_iteration import scatter_source_function
+from citrus_amd.scatter_solver.transport_sweep import make_sweep_context, source_is_cell_major, sweep
+from oracle import checks, quadrature as oq, sweep as osw
+from tests.conftest import gpu_device
+from tests.helpers import (F32_REL, direction_source_function, octant_set, oracle_sweep, random_problem,
+                           rel_err)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_slab_oracle(axis, sign):
+    dev = gpu_device()
+    rng = np.random.default_rng(10 + axis)
+    K, G = 9, 2
+    deltas = (0.7, 1.3, 0.9)
+    shape = [3, 3, 3]
+    vol_dim = {0: 1, 1: 0, 2: 2}[axis]
+    shape[vol_dim] = K
+    grid = Grid(tuple(shape), deltas[0], deltas[1] / deltas[0], deltas[2] / deltas[0])
+    d = [0.0, 0.0, 0.0]
+    d[axis] = float(sign)
+    o = int(oq.octant_of(np.array(d)))
+    q = octant_set(replace={o: d})
+    ctx = make_sweep_context(grid, q, 1, device=dev)
+    mu1 = rng.random((G, K)) * 1.5 + 0.05
+    S0, S1 = rng.random((G, K)), rng.standard_normal((G, K)) * 0.3
+
+    def along(v):
+        view = [G, 1, 1, 1]
+        view[1 + vol_dim] = K
+        return torch.from_numpy(np.broadcast_to(v.reshape(view), (G, *shape)).copy())
+
+    src = torch.zeros(G, 4, *shape, dtype=torch.float64)
+    src[:, 0] = along(S0)
+    src[:, 1 + axis] = along(S1)
+    src32 = src.to(dev, torch.float32)
+    _, psi = sweep(ctx, along(mu1).to(dev, torch.float32), lambda idx: src32[None].expand(idx.numel(), *src32.shape),
+                   return_psi=True)
+    psi = psi.cpu().double()
+    for g in range(G):
+        p0, p1 = checks.slab_1d(mu1[g], S0[g], S1[g], deltas[axis], float(sign))
+        r0 = along(np.broadcast_to(p0, (G, K)).copy())[g]
+        r1 = along(np.broadcast_to(p1, (G, K)).copy())[g]
+        tol = F32_REL * max(1.0, float(r0.abs().max()))
+        assert float((psi[o, g, 0] - r0).abs().max()) < tol
+        assert float((psi[o, g, 1 + axis] - r1).abs().max()) < tol
+
+
+def test_particle_balance_and_dense_assembly():
+    dev = gpu_device()
+    shape = (3, 4, 3)
+    grid = Grid(shape, 0.9, 1.2 / 0.9, 0.75 / 0.9)
+    q = octant_set(base=(0.55, 0.38, 0.74))
+    ctx = make_sweep_context(grid, q, 1, device=dev)
+    mu, src = random_problem(shape, 2, 8, dev, seed=3)
+    _, psi = sweep(ctx, mu, direction_source_function(src), return_psi=True)
+    psi, mu64, src64 = psi.cpu().double().numpy(), mu.cpu().double().numpy(), src.cpu().double().numpy()
+    om, _ = (q.omega[ctx.permutation.cpu()].numpy(), None)
+    for m in range(8):
+        lhs, rhs = checks.outflow_balance(psi[m], om[m], grid.deltas, mu64, src64[m])
+        assert np.allclose(lhs, rhs, rtol=1e-5, atol=0)
+        for g in range(2):
+            ref = checks.dense_global_solve(om[m], grid.deltas, mu64[g], src64[m, g])
+            assert rel_err(psi[m, g], ref) < F32_REL
+
+
+@pytest.mark.parametrize("chunk", [None, 5])
+def test_random_heterogeneous_matches_oracle(chunk):
+    """12x10x14 (partial blocks), G=6, 128 directions, L=3, thin-to-thick cells, sparse signed sources."""
+    dev = gpu_device()
+    shape, G = (12, 10, 14), 6
+    gen = torch.Generator().manual_seed(21)
+    mu = torch.exp(torch.empty(G, *shape, dtype=torch.float64).uniform_(-3.0, 1.5, generator=gen))
+    grid = Grid(shape, 0.8, 1.15 / 0.8, 0.9 / 0.8)
+    q = product_quadrature(8, 16)
+    ctx = make_sweep_context(grid, q, 3, device=dev)
+    src = torch.randn(ctx.num_directions, G, 4, *shape, generator=gen, dtype=torch.float64)
+    src[:, :, 0] = src[:, :, 0].abs() * (torch.rand(G, *shape, generator=gen, dtype=torch.float64) < 0.3)
+    mu32, src32 = mu.to(dev, torch.float32), src.to(dev, torch.float32)
+    phi, psi = sweep(ctx, mu32, direction_source_function(src32), direction_chunk=chunk, return_psi=True)
+    psi_ref, phi_ref = oracle_sweep(ctx, q, mu32, src32)
+    assert rel_err(psi, psi_ref) < F32_REL and rel_err(phi, phi_ref) < F32_REL
+    assert float(psi[:, :, 0].min()) < 0.0  # negative cell means are exercised
+
+
+def test_moments_match_ordinates_and_chunks():
+    dev = gpu_device()
+    grid = Grid((3, 4, 5), 0.6, 1.1 / 0.6, 0.9 / 0.6)
+    q = product_quadrature(4, 8)
+    ctx = make_sweep_context(grid, q, 3, device=dev)
+    mu, src = random_problem(grid.shape, 2, ctx.num_directions, dev)
+    phi, psi = sweep(ctx, mu, direction_source_function(src), return_psi=True)
+    assert phi.shape == (2, 16, 4, 3, 4, 5)
+    Yw = ctx.basis.Y_weighted.to(dev)
+    ref = torch.einsum("mn,mgk...->gnk...", Yw.double(), psi.double())
+    assert rel_err(phi, ref) < F32_REL
+    for chunk in (1, 3):
+        assert rel_err(sweep(ctx, mu, direction_source_function(src), direction_chunk=chunk), phi) < F32_REL
+
+
+@pytest.mark.parametrize("chunk", [None, 3])
+def test_source_called_once_per_chunk_and_cell_major(chunk):
+    dev = gpu_device()
+    grid = Grid((4, 5, 3), 0.7, 1.1 / 0.7, 0.9 / 0.7)
+    q = product_quadrature(4, 8)
+    ctx = make_sweep_context(grid, q, 2, device=dev)
+    mu, src = random_problem(grid.shape, 3, ctx.num_directions, dev)
+    calls = []
+
+    def plain(index):
+        calls.append(index.clone())
+        return src.index_select(0, index)
+
+    cm_src = src.permute(0, 3, 4, 5, 1, 2).contiguous()
+
+    def cell_major(index, out=None):
+        assert out is not None and out.shape == (index.numel(),) + tuple(cm_src.shape[1:])
+        return out.copy_(cm_src.index_select(0, index))
+
+    cell_major.cell_major = True
+    assert source_is_cell_major(cell_major)
+    ref = sweep(ctx, mu, direction_source_function(src), direction_chunk=chunk)
+    assert torch.equal(sweep(ctx, mu, plain, direction_chunk=chunk), ref)
+    assert torch.equal(sweep(ctx, mu, cell_major, direction_chunk=chunk), ref)
+    dc = 4 if chunk is None else 3
+    starts = list(range(0, 4, dc))
+    assert len(calls) == len(starts)
+    for d0, index in zip(starts, calls):
+        assert torch.equal(index, ctx.direction_indices(d0, min(d0 + dc, 4)))
+
+
+@pytest.mark.parametrize("shape,G,order,L,chunk", [((5, 7, 6), 3, 8, 3, None), ((9, 4, 6), 2, 12, 5, 7)])
+def test_fused_moment_source_matches_oracle(shape, G, order, L, chunk):
+    """Source expanded from SH moments inside the kernel vs the oracle's explicit expansion."""
+    dev = gpu_device()
+    grid = Grid(shape, 0.5)
+    q = level_symmetric_quadrature(order)
+    ctx = make_sweep_context(grid, q, L, device=dev)
+    gen = torch.Generator().manual_seed(7)
+    mu = (torch.rand(G, *shape, generator=gen) + 0.3).to(dev)
+    T = torch.randn(G, ctx.num_moments, 4, *shape, generator=gen).to(dev)
+    phi = sweep(ctx, mu, scatter_source_function(ctx, T), direction_chunk=chunk)
+    om, w = q.omega[ctx.permutation.cpu()].numpy(), q.weight[ctx.permutation.cpu()].numpy()
+    Y = oq.real_spherical_harmonics(L, om)
+    src = osw.sources_from_moments(T.double().cpu().numpy(), Y)
+    psi = osw.sweep(shape, grid.deltas, om, mu.double().cpu().numpy(), src)
+    assert rel_err(phi, osw.moments(psi, w, Y)) < F32_REL
+
+
+def test_config0_shape_fused_sweep():
+    """32^3 cells, 8 groups, S8 (80 directions), P3 (16 moments): one fused sweep vs the oracle."""
+    dev = gpu_device()
+    shape, G, L = (32, 32, 32), 8, 3
+    grid = Grid(shape, 0.4)
+    q = level_symmetric_quadrature(8)
+    ctx = make_sweep_context(grid, q, L, device=dev)
+    gen = torch.Generator().manual_seed(1)
+    mu = (torch.rand(G, *shape, generator=gen) * 0.45 + 0.15).to(dev)
+    T = (torch.randn(G, ctx.num_moments, 4, *shape, generator=gen) * 0.1).to(dev)
+    T[:, 0, 0] = T[:, 0, 0].abs() + 0.05
+    phi = sweep(ctx, mu, scatter_source_function(ctx, T))
+    om, w = q.omega[ctx.permutation.cpu()].numpy(), q.weight[ctx.permutation.cpu()].numpy()
+    Y = oq.real_spherical_harmonics(L, om)
+    psi = osw.sweep(shape, grid.deltas, om, mu.double().cpu().numpy(),
+                    osw.sources_from_moments(T.double().cpu().numpy(), Y))
+    err = rel_err(phi, osw.moments(psi, w, Y))
+    print(f"config0 fused sweep rel err {err:.2e}")
+    assert err < F32_REL
+
+
+def test_backend_errors():
+    dev = gpu_device()
+    grid = Grid((3, 3, 4), 0.5)
+    q = product_quadrature(4, 8)
+    ctx = make_sweep_context(grid, q, 1, device=dev)
+    mu, src = random_problem(grid.shape, 2, 32, dev)
+    with pytest.raises(ValueError):
+        sweep(ctx, mu, direction_source_function(src), backend="torch")
+    with pytest.raises(ValueError):
+        sweep(ctx, mu.double(), direction_source_function(src.double()))
+    with pytest.raises(ValueError):
+        make_sweep_context(grid, q, 1, device=dev, backend="bogus")
