@@ -46,6 +46,22 @@ _SIGNATURES = {
     "citrus_scatter_moments_dense": (c_int, [c_void, c_int, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),
     "citrus_scatter_moments_factored": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_int, c_int,
                                                 c_i64, c_int, c_void]),
+    "citrus_ray_segments": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_int, c_i64, c_void,
+                                    c_void]),
+    "citrus_ray_trace_indices": (c_int, [c_int, c_int, c_int, c_void, c_void, c_void, c_i64, c_int, c_void, c_void,
+                                         c_void, c_void]),
+    "citrus_ray_lattice": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_void, c_void,
+                                   c_void, c_void]),
+    "citrus_ray_primary": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_int, c_void,
+                                   c_void]),
+    "citrus_ray_oblique": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_void, c_int,
+                                   c_void, c_i64, c_i64, c_void]),
+    "citrus_ray_downstream": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_void, c_int,
+                                      c_void, c_void]),
+    "citrus_group_transmission": (c_int, [c_void, c_int, c_i64, c_void, c_void, c_void, c_int, c_int, c_void, c_void,
+                                          c_void, c_void]),
+    "citrus_uncollided_fluence": (c_int, [c_void, c_int, c_void, c_void, c_void, c_void, c_void, c_int, c_int,
+                                          c_void, c_void, c_void, c_void, c_void, c_void, c_void, c_void]),
 }
 
 

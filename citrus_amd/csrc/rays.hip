@@ -1,0 +1,473 @@
+// Exact line integrals of voxel fields (Siddon / Amanatides-Woo traversal) for gfx950.
+//
+// One thread traces one ray. The ray's end points are generated in the kernel (explicit segments, focal spot to
+// lattice points, focal spot to pixel centres, pixel lines along a direction, voxel centres to the panel), so no
+// per-ray geometry ever goes through HBM. The segment clipping, entry voxel and plane parameters are computed and
+// compared in float64 with one IEEE operation per step (contraction is off in this file), which makes the voxel
+// sequence identical to a float64 CPU traversal written with the same operations; the field values are summed in
+// float32 with Kahan compensation, so long rays keep float32 relative accuracy. Fields are channel-last
+// [rows][cols][slices][channels] so the channels of a voxel are one contiguous read.
+//
+// Traversal rules (citrus/scatter_solver/kernels/ray_tracing_kernels.py:178-327): slab clip to [a_lo, a_hi] within
+// [0, 1]; entry index floor (ceil - 1 along a descending axis) clamped to the grid; step to the nearest plane, ties in
+// the order x, y, z; the next plane parameter is recomputed from the voxel index at every step.
+
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "citrus_hip.h"
+#include "common.h"
+
+namespace citrus {
+namespace {
+
+constexpr int kRayThreads = 128;
+constexpr double kBig = 1e30;
+
+struct Box {
+  double lo[3], hi[3], delta[3];
+  int n[3];  // voxels along x (cols), y (rows), z (slices)
+};
+
+enum Mode : int {
+  kSegments = 0,   // p0[i], p1[i] given (p0 may be one shared point)
+  kLattice = 1,    // p0 = point, p1 = centre of lattice point i
+  kPixels = 2,     // p0 = focal spot, p1 = (sub-)pixel centre
+  kOblique = 3,    // line through pixel centre along direction k, clipped to the box
+  kDownstream = 4  // voxel centre to the panel along direction k
+};
+
+struct Panel {
+  double source[3], center[3], uh[3], vh[3], nh[3];
+  double sdd;
+  int curved, rows, cols;
+  double drow, dch, off_row, off_ch;
+};
+
+struct RayJob {
+  Box box;
+  int nch;
+  int64_t nrays;   // rays per direction (pixels, lattice points, voxels or segments)
+  int K;           // directions (1 unless oblique / downstream)
+  const float* vol;
+  int64_t vol_k_stride;  // elements between the fields of consecutive directions (0: shared field)
+  // segments
+  const double* p0;
+  const double* p1;
+  int p0_shared;
+  // lattice: rays from lat_point to origin + (j dx, i dy, k dz), shape (rows, cols, slices)
+  double lat_point[3], lat_origin[3], lat_delta[3];
+  int lat_shape[3];
+  // pixels
+  Panel panel;
+  int samples;  // sub-rays per pixel axis
+  // directions
+  const double* omega;  // [K][3]
+  double half_diag;     // oblique: half length of the segment
+  double box_center[3];
+  double diag;          // downstream: clamp of the panel distance
+  // output: out[k * ok + c * oc + i] (i = ray within direction)
+  float* out;
+  int64_t ok, oc;
+  // optional voxel-sequence capture (segments mode)
+  int32_t* seq_idx;
+  float* seq_len;
+  int32_t* seq_count;
+  int seq_max;
+};
+
+__device__ __forceinline__ double entry_index(double p0, double d, double a, double lo, double delta, int n) {
+  const double f = (p0 + a * d - lo) / delta;
+  double i = d < 0.0 ? ceil(f) - 1.0 : floor(f);
+  i = fmax(i, 0.0);
+  i = fmin(i, (double)(n - 1));
+  return i;
+}
+
+__device__ __forceinline__ double plane_param(double lo, double delta, int idx, bool pos, double p0, double inv,
+                                              bool nz) {
+  const int k = idx + (pos ? 1 : 0);
+  return nz ? (lo + (double)k * delta - p0) * inv : kBig;
+}
+
+// Integrate the channel-last field along p0 -> p1. Returns |p1 - p0| and accumulates chord * value per channel.
+template <int CH, bool CAPTURE>
+__device__ void trace(const RayJob& J, const double* p0, const double* p1, const float* vol, float* sum,
+                      int32_t* sidx, float* slen, int32_t* scount) {
+  const Box& B = J.box;
+  double d[3], inv[3], amin[3], amax[3];
+  bool nz[3], pos[3];
+  for (int a = 0; a < 3; ++a) {
+    d[a] = p1[a] - p0[a];
+    nz[a] = d[a] != 0.0;
+    inv[a] = 1.0 / (nz[a] ? d[a] : 1.0);
+    const double a1 = (B.lo[a] - p0[a]) * inv[a];
+    const double a2 = (B.hi[a] - p0[a]) * inv[a];
+    const bool ins = (p0[a] >= B.lo[a]) && (p0[a] <= B.hi[a]);
+    amin[a] = nz[a] ? fmin(a1, a2) : (ins ? -kBig : kBig);
+    amax[a] = nz[a] ? fmax(a1, a2) : (ins ? kBig : -kBig);
+    pos[a] = d[a] > 0.0;
+  }
+  double a_lo = fmax(fmax(fmax(amin[0], amin[1]), amin[2]), 0.0);
+  double a_hi = fmin(fmin(fmin(amax[0], amax[1]), amax[2]), 1.0);
+  const double length = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  int count = 0;
+  if (a_hi > a_lo) {
+    int idx[3];
+    double an[3];
+    int step[3];
+    for (int a = 0; a < 3; ++a) {
+      idx[a] = (int)entry_index(p0[a], d[a], a_lo, B.lo[a], B.delta[a], B.n[a]);
+      step[a] = pos[a] ? 1 : (d[a] < 0.0 ? -1 : 0);
+      an[a] = plane_param(B.lo[a], B.delta[a], idx[a], pos[a], p0[a], inv[a], nz[a]);
+    }
+    float comp[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) comp[c] = 0.0f;
+    double a = a_lo;
+    const int max_steps = B.n[0] + B.n[1] + B.n[2] + 3;
+    for (int s = 0; s < max_steps; ++s) {
+      const double nxt = fmin(fmin(an[0], an[1]), fmin(an[2], a_hi));
+      const double seg = fmax(nxt - a, 0.0);
+      const bool inb = idx[0] >= 0 && idx[0] < B.n[0] && idx[1] >= 0 && idx[1] < B.n[1] && idx[2] >= 0 &&
+                       idx[2] < B.n[2];
+      if (inb && seg > 0.0) {
+        const int64_t flat = ((int64_t)idx[1] * B.n[0] + idx[0]) * B.n[2] + idx[2];
+        if (CAPTURE) {
+          if (count < J.seq_max) {
+            sidx[count] = (int32_t)flat;
+            slen[count] = (float)seg;
+          }
+        } else {
+          const float chord = (float)(seg * length);
+          const float* v = vol + flat * J.nch;
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            if (c < J.nch) {  // Kahan-compensated sum
+              const float y = v[c] * chord - comp[c];
+              const float t = sum[c] + y;
+              comp[c] = (t - sum[c]) - y;
+              sum[c] = t;
+            }
+          }
+        }
+        ++count;
+      }
+      const bool sx = (an[0] <= an[1]) && (an[0] <= an[2]);
+      const bool sy = (an[1] < an[0]) && (an[1] <= an[2]);
+      const bool sz = (an[2] < an[0]) && (an[2] < an[1]);
+      if (sx) { idx[0] += step[0]; an[0] = plane_param(B.lo[0], B.delta[0], idx[0], pos[0], p0[0], inv[0], nz[0]); }
+      if (sy) { idx[1] += step[1]; an[1] = plane_param(B.lo[1], B.delta[1], idx[1], pos[1], p0[1], inv[1], nz[1]); }
+      if (sz) { idx[2] += step[2]; an[2] = plane_param(B.lo[2], B.delta[2], idx[2], pos[2], p0[2], inv[2], nz[2]); }
+      if (!(nxt < a_hi)) break;
+      a = nxt;
+    }
+  }
+  if (CAPTURE) *scount = count;
+}
+
+__device__ void panel_point(const Panel& P, double u, double v, double* out) {
+  if (!P.curved) {
+    for (int a = 0; a < 3; ++a) out[a] = P.center[a] + u * P.uh[a] + v * P.vh[a];
+    return;
+  }
+  const double g = u / P.sdd;
+  const double su = P.sdd * sin(g);
+  const double sh = sin(0.5 * g);
+  const double sn = 2.0 * P.sdd * sh * sh;
+  const double sv = v * cos(g);
+  for (int a = 0; a < 3; ++a) out[a] = P.center[a] + su * P.uh[a] + sn * P.nh[a] + sv * P.vh[a];
+}
+
+__device__ __forceinline__ double pixel_u(const Panel& P, int col) {
+  return ((double)col - (double)(P.cols - 1) / 2.0) * P.dch - P.off_ch;
+}
+__device__ __forceinline__ double pixel_v(const Panel& P, int row) {
+  return ((double)row - (double)(P.rows - 1) / 2.0) * P.drow - P.off_row;
+}
+
+// panel distance of p along w (flat plane or the larger root on the cylinder), +inf when unreachable
+__device__ double panel_distance(const Panel& P, const double* p, const double* w) {
+  if (!P.curved) {
+    const double den = w[0] * P.nh[0] + w[1] * P.nh[1] + w[2] * P.nh[2];
+    const double num = (P.center[0] - p[0]) * P.nh[0] + (P.center[1] - p[1]) * P.nh[1] +
+                       (P.center[2] - p[2]) * P.nh[2];
+    return num / den;
+  }
+  const double dx = p[0] - P.source[0], dy = p[1] - P.source[1];
+  const double a = w[0] * w[0] + w[1] * w[1];
+  const double b = dx * w[0] + dy * w[1];
+  const double c = dx * dx + dy * dy - P.sdd * P.sdd;
+  const double disc = b * b - a * c;
+  const double root = sqrt(fmax(disc, 0.0));
+  const double den = b >= 0.0 ? b + root : (a > 0.0 ? a : 1.0);
+  const double num = b >= 0.0 ? -c : root - b;
+  if (!(a > 0.0 && disc >= 0.0 && den != 0.0)) return INFINITY;
+  return num / den;
+}
+
+template <int MODE, int CH>
+__global__ __launch_bounds__(kRayThreads) void ray_kernel(RayJob J) {
+  const int64_t gid = (int64_t)blockIdx.x * kRayThreads + threadIdx.x;
+  const int64_t total = J.nrays * J.K;
+  if (gid >= total) return;
+  const int k = (int)(gid / J.nrays);
+  const int64_t i = gid - (int64_t)k * J.nrays;
+  float sum[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) sum[c] = 0.0f;
+  const float* vol = J.vol + (int64_t)k * J.vol_k_stride;
+  double p0[3], p1[3];
+  bool write_inf = false;
+  if (MODE == kSegments) {
+    const double* q0 = J.p0_shared ? J.p0 : J.p0 + 3 * i;
+    for (int a = 0; a < 3; ++a) { p0[a] = q0[a]; p1[a] = J.p1[3 * i + a]; }
+    if (J.seq_idx) {
+      trace<CH, true>(J, p0, p1, vol, sum, J.seq_idx + i * J.seq_max, J.seq_len + i * J.seq_max,
+                      J.seq_count + i);
+      return;
+    }
+    trace<CH, false>(J, p0, p1, vol, sum, nullptr, nullptr, nullptr);
+  } else if (MODE == kLattice) {
+    const int S = J.lat_shape[2], C = J.lat_shape[1];
+    const int kk = (int)(i % S), jj = (int)((i / S) % C), ii = (int)(i / ((int64_t)S * C));
+    for (int a = 0; a < 3; ++a) p0[a] = J.lat_point[a];
+    p1[0] = J.lat_origin[0] + (double)jj * J.lat_delta[0];
+    p1[1] = J.lat_origin[1] + (double)ii * J.lat_delta[1];
+    p1[2] = J.lat_origin[2] + (double)kk * J.lat_delta[2];
+    trace<CH, false>(J, p0, p1, vol, sum, nullptr, nullptr, nullptr);
+  } else if (MODE == kPixels) {
+    const Panel& P = J.panel;
+    const int row = (int)(i / P.cols), col = (int)(i % P.cols);
+    const int n = J.samples;
+    for (int a = 0; a < 3; ++a) p0[a] = P.source[a];
+    if (n == 1) {
+      panel_point(P, pixel_u(P, col), pixel_v(P, row), p1);
+      trace<CH, false>(J, p0, p1, vol, sum, nullptr, nullptr, nullptr);
+    } else {
+      // -log(mean exp(-tau)) over the n x n midpoint lattice of the pixel, by a running log-sum-exp per channel
+      float m[CH], acc[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) { m[c] = INFINITY; acc[c] = 0.0f; }
+      for (int sv = 0; sv < n; ++sv)
+        for (int su = 0; su < n; ++su) {
+          const double ou = ((2.0 * su + 1.0) / (2.0 * n) - 0.5) * P.dch;
+          const double ov = ((2.0 * sv + 1.0) / (2.0 * n) - 0.5) * P.drow;
+          panel_point(P, pixel_u(P, col) + ou, pixel_v(P, row) + ov, p1);
+          float t[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) t[c] = 0.0f;
+          trace<CH, false>(J, p0, p1, vol, t, nullptr, nullptr, nullptr);
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            if (t[c] < m[c]) { acc[c] = acc[c] * expf(t[c] - m[c]) + 1.0f; m[c] = t[c]; }
+            else acc[c] += expf(m[c] - t[c]);
+          }
+        }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) sum[c] = m[c] - logf(acc[c] / (float)(n * n));
+    }
+  } else if (MODE == kOblique) {
+    const Panel& P = J.panel;
+    const int row = (int)(i / P.cols), col = (int)(i % P.cols);
+    double p[3];
+    panel_point(P, pixel_u(P, col), pixel_v(P, row), p);
+    const double* w = J.omega + 3 * k;
+    const double w2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double cp = ((J.box_center[0] - p[0]) * w[0] + (J.box_center[1] - p[1]) * w[1] +
+                       (J.box_center[2] - p[2]) * w[2]) / w2;
+    const double h = J.half_diag / sqrt(w2);
+    for (int a = 0; a < 3; ++a) {
+      p0[a] = p[a] + (cp - h) * w[a];
+      p1[a] = p[a] + (cp + h) * w[a];
+    }
+    trace<CH, false>(J, p0, p1, vol, sum, nullptr, nullptr, nullptr);
+  } else {  // kDownstream: voxel centre i of the traced grid to the panel along omega[k]
+    const Box& B = J.box;
+    const int S = B.n[2], C = B.n[0];
+    const int kk = (int)(i % S), jj = (int)((i / S) % C), ii = (int)(i / ((int64_t)S * C));
+    const double* w = J.omega + 3 * k;
+    const Panel& P = J.panel;
+    const double toward = -(w[0] * P.nh[0] + w[1] * P.nh[1] + w[2] * P.nh[2]);
+    p0[0] = B.lo[0] + ((double)jj + 0.5) * B.delta[0];
+    p0[1] = B.lo[1] + ((double)ii + 0.5) * B.delta[1];
+    p0[2] = B.lo[2] + ((double)kk + 0.5) * B.delta[2];
+    if (!(toward > 0.0)) {
+      write_inf = true;
+    } else {
+      double t = panel_distance(P, p0, w);
+      t = fmin(fmax(t, 0.0), J.diag);
+      for (int a = 0; a < 3; ++a) p1[a] = p0[a] + t * w[a];
+      trace<CH, false>(J, p0, p1, vol, sum, nullptr, nullptr, nullptr);
+    }
+  }
+  float* out = J.out + (int64_t)k * J.ok + i;
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+    if (c < J.nch) out[(int64_t)c * J.oc] = write_inf ? INFINITY : sum[c];
+}
+
+template <int MODE>
+hipError_t launch_mode(const RayJob& J, hipStream_t s) {
+  const int64_t total = J.nrays * J.K;
+  if (total == 0) return hipSuccess;
+  const int64_t blocks = (total + kRayThreads - 1) / kRayThreads;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)blocks);
+  if (J.nch <= 1) hipLaunchKernelGGL((ray_kernel<MODE, 1>), grid, dim3(kRayThreads), 0, s, J);
+  else if (J.nch <= 2) hipLaunchKernelGGL((ray_kernel<MODE, 2>), grid, dim3(kRayThreads), 0, s, J);
+  else if (J.nch <= 4) hipLaunchKernelGGL((ray_kernel<MODE, 4>), grid, dim3(kRayThreads), 0, s, J);
+  else if (J.nch <= 8) hipLaunchKernelGGL((ray_kernel<MODE, 8>), grid, dim3(kRayThreads), 0, s, J);
+  else if (J.nch <= 16) hipLaunchKernelGGL((ray_kernel<MODE, 16>), grid, dim3(kRayThreads), 0, s, J);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+int init_job(RayJob& J, const float* volume, int R, int C, int S, int nch, const double* box6) {
+  if (R < 1 || C < 1 || S < 1) return fail(CITRUS_ERR_ARG, "grid shape must be positive");
+  if (nch < 1 || nch > 16) return fail(CITRUS_ERR_ARG, "1..16 channels supported");
+  if (!box6) return fail(CITRUS_ERR_ARG, "box is NULL");
+  J = RayJob{};
+  for (int a = 0; a < 3; ++a) {
+    J.box.lo[a] = box6[a];
+    J.box.delta[a] = box6[3 + a];
+  }
+  J.box.n[0] = C; J.box.n[1] = R; J.box.n[2] = S;
+  for (int a = 0; a < 3; ++a) J.box.hi[a] = J.box.lo[a] + J.box.delta[a] * (double)J.box.n[a];
+  J.nch = nch;
+  J.vol = volume;
+  J.K = 1;
+  J.samples = 1;
+  return CITRUS_OK;
+}
+
+int set_panel(RayJob& J, const double* panel20, const int* panel_ints) {
+  if (!panel20 || !panel_ints) return fail(CITRUS_ERR_ARG, "panel description is NULL");
+  Panel& P = J.panel;
+  for (int a = 0; a < 3; ++a) {
+    P.source[a] = panel20[a];
+    P.center[a] = panel20[3 + a];
+    P.uh[a] = panel20[6 + a];
+    P.vh[a] = panel20[9 + a];
+    P.nh[a] = panel20[12 + a];
+  }
+  P.sdd = panel20[15];
+  P.drow = panel20[16];
+  P.dch = panel20[17];
+  P.off_row = panel20[18];
+  P.off_ch = panel20[19];
+  P.curved = panel_ints[0];
+  P.rows = panel_ints[1];
+  P.cols = panel_ints[2];
+  if (P.rows < 1 || P.cols < 1) return fail(CITRUS_ERR_ARG, "panel must have rows and columns");
+  return CITRUS_OK;
+}
+
+int finish(hipError_t e, const char* what) {
+  if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_ARG, std::string(what) + ": too many rays or channels");
+  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return CITRUS_OK;
+}
+
+}  // namespace
+}  // namespace citrus
+
+using namespace citrus;
+
+extern "C" int citrus_ray_segments(const float* volume, int R, int C, int S, int nch, const double* box,
+                                   const double* p0, const double* p1, int p0_shared, int64_t n, float* out,
+                                   void* stream) {
+  RayJob J;
+  if (int rc = init_job(J, volume, R, C, S, nch, box)) return rc;
+  if (!volume || !p0 || !p1 || !out) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  J.p0 = p0; J.p1 = p1; J.p0_shared = p0_shared; J.nrays = n;
+  J.out = out; J.ok = 0; J.oc = n;
+  return finish(launch_mode<kSegments>(J, reinterpret_cast<hipStream_t>(stream)), "citrus_ray_segments");
+}
+
+extern "C" int citrus_ray_trace_indices(int R, int C, int S, const double* box, const double* p0, const double* p1,
+                                        int64_t n, int max_len, int32_t* seq_idx, float* seq_len,
+                                        int32_t* seq_count, void* stream) {
+  RayJob J;
+  if (int rc = init_job(J, nullptr, R, C, S, 1, box)) return rc;
+  if (!p0 || !p1 || !seq_idx || !seq_len || !seq_count || max_len < 1) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  J.p0 = p0; J.p1 = p1; J.nrays = n;
+  J.seq_idx = seq_idx; J.seq_len = seq_len; J.seq_count = seq_count; J.seq_max = max_len;
+  return finish(launch_mode<kSegments>(J, reinterpret_cast<hipStream_t>(stream)), "citrus_ray_trace_indices");
+}
+
+extern "C" int citrus_ray_lattice(const float* volume, int R, int C, int S, int nch, const double* box,
+                                  const double* point, const double* lattice_origin, const double* lattice_delta,
+                                  const int* lattice_shape, float* out, void* stream) {
+  RayJob J;
+  if (int rc = init_job(J, volume, R, C, S, nch, box)) return rc;
+  if (!volume || !point || !lattice_origin || !lattice_delta || !lattice_shape || !out)
+    return fail(CITRUS_ERR_ARG, "NULL buffer");
+  for (int a = 0; a < 3; ++a) {
+    J.lat_point[a] = point[a];
+    J.lat_origin[a] = lattice_origin[a];
+    J.lat_delta[a] = lattice_delta[a];
+    J.lat_shape[a] = lattice_shape[a];
+  }
+  J.nrays = (int64_t)lattice_shape[0] * lattice_shape[1] * lattice_shape[2];
+  J.out = out; J.ok = 0; J.oc = J.nrays;
+  return finish(launch_mode<kLattice>(J, reinterpret_cast<hipStream_t>(stream)), "citrus_ray_lattice");
+}
+
+extern "C" int citrus_ray_primary(const float* volume, int R, int C, int S, int nch, const double* box,
+                                  const double* panel, const int* panel_ints, int samples, float* out,
+                                  void* stream) {
+  RayJob J;
+  if (int rc = init_job(J, volume, R, C, S, nch, box)) return rc;
+  if (int rc = set_panel(J, panel, panel_ints)) return rc;
+  if (!volume || !out) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (samples < 1 || samples > 16) return fail(CITRUS_ERR_ARG, "samples per pixel axis must be 1..16");
+  J.samples = samples;
+  J.nrays = (int64_t)J.panel.rows * J.panel.cols;
+  J.out = out; J.ok = 0; J.oc = J.nrays;
+  return finish(launch_mode<kPixels>(J, reinterpret_cast<hipStream_t>(stream)), "citrus_ray_primary");
+}
+
+extern "C" int citrus_ray_oblique(const float* field, int64_t field_k_stride, int R, int C, int S, int nch,
+                                  const double* box, const double* panel, const int* panel_ints,
+                                  const double* omega, int K, float* out, int64_t out_k_stride,
+                                  int64_t out_c_stride, void* stream) {
+  RayJob J;
+  if (int rc = init_job(J, field, R, C, S, nch, box)) return rc;
+  if (int rc = set_panel(J, panel, panel_ints)) return rc;
+  if (!field || !omega || !out || K < 1) return fail(CITRUS_ERR_ARG, "NULL buffer or no directions");
+  J.vol_k_stride = field_k_stride;
+  J.omega = omega;
+  J.K = K;
+  double e2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    J.box_center[a] = J.box.lo[a] + 0.5 * (J.box.hi[a] - J.box.lo[a]);
+    e2 += (J.box.hi[a] - J.box.lo[a]) * (J.box.hi[a] - J.box.lo[a]);
+  }
+  J.half_diag = 0.5 * sqrt(e2) * (1.0 + 1e-6) + 1e-6;
+  J.nrays = (int64_t)J.panel.rows * J.panel.cols;
+  J.out = out; J.ok = out_k_stride; J.oc = out_c_stride;
+  return finish(launch_mode<kOblique>(J, reinterpret_cast<hipStream_t>(stream)), "citrus_ray_oblique");
+}
+
+extern "C" int citrus_ray_downstream(const float* volume, int R, int C, int S, int nch, const double* box,
+                                     const double* panel, const int* panel_ints, const double* omega, int K,
+                                     float* out, void* stream) {
+  RayJob J;
+  if (int rc = init_job(J, volume, R, C, S, nch, box)) return rc;
+  if (int rc = set_panel(J, panel, panel_ints)) return rc;
+  if (!volume || !omega || !out || K < 1) return fail(CITRUS_ERR_ARG, "NULL buffer or no directions");
+  J.omega = omega;
+  J.K = K;
+  double e2 = 0.0;
+  for (int a = 0; a < 3; ++a) e2 += (J.box.hi[a] - J.box.lo[a]) * (J.box.hi[a] - J.box.lo[a]);
+  J.diag = sqrt(e2) * (1.0 + 1e-9) + 1e-9;
+  J.nrays = (int64_t)R * C * S;
+  J.out = out; J.oc = J.nrays; J.ok = (int64_t)nch * J.nrays;
+  return finish(launch_mode<kDownstream>(J, reinterpret_cast<hipStream_t>(stream)), "citrus_ray_downstream");
+}

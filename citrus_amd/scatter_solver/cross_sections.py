@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
+from typing import Optional
 
 import torch
 from torch import Tensor
@@ -20,14 +21,19 @@ from torch import Tensor
 
 @dataclass(frozen=True)
 class MultigroupLibrary:
-    """Multigroup tables of M materials (float64, CPU)."""
+    """Multigroup tables of M materials (float64, CPU), the fields of cross_sections.py:493 the hot path reads."""
 
     group_edges: Tensor        # [G+1] keV, descending
     mu_t: Tensor               # [M, G] cm^2/g
     mu_s_l: Tensor             # [M, G, G, L+1] cm^2/g
-    eta_grid: Tensor           # [E]
-    mu_s_eta: Tensor           # [M, G, G, E] cm^2/g/sr
+    eta_grid: Tensor           # [N_eta] angular-table nodes in eta = cos(scattering angle)
+    mu_s_eta: Tensor           # [M, G, G, N_eta] cm^2/g/sr
     materials: tuple[str, ...]
+    spectrum_fraction: Tensor  # [G] source photons per sr in each group (sum 1)
+    fine_energy: Tensor        # [E] keV
+    fine_mu_t: Tensor          # [M, E] cm^2/g
+    fine_weight: Tensor        # [E] spectrum weight of each fine energy
+    fine_group: Tensor         # [E] int64 group of each fine energy
 
     @property
     def num_groups(self) -> int:
@@ -42,7 +48,24 @@ class MultigroupLibrary:
         return int(self.mu_t.shape[0])
 
 
-def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int = 401) -> MultigroupLibrary:
+def _fine_tables(edges: Tensor, mu_t: Tensor, n: int, spread: float):
+    """Equal-weight fine bins per group whose weighted mean attenuation is the group's (cross_sections.py:1092-1099):
+    (spectrum fraction [G], fine energy [E], fine mu_t [M, E], fine weight [E], fine group [E])."""
+    G = int(mu_t.shape[1])
+    if n < 1:
+        raise ValueError("num_fine_per_group must be >= 1")
+    fraction = torch.ones(G, dtype=torch.float64) / G
+    t = (torch.arange(n, dtype=torch.float64) + 0.5) / n
+    x = 1.0 - 2.0 * torch.arange(n, dtype=torch.float64) / (n - 1) if n > 1 else torch.zeros(1, dtype=torch.float64)
+    fine_energy = torch.cat([edges[g + 1] + t * (edges[g] - edges[g + 1]) for g in range(G)])
+    fine_group = torch.arange(G).repeat_interleave(n)
+    fine_weight = fraction[fine_group] / n
+    fine_mu_t = mu_t[:, fine_group] * (1.0 + float(spread) * x.repeat(G))[None, :]
+    return fraction, fine_energy, fine_mu_t, fine_weight, fine_group
+
+
+def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int = 401, num_fine_per_group: int = 4,
+                      fine_spread: float = 0.0) -> MultigroupLibrary:
     """The reference's two-material closed-form test library (cross_sections.py:1043-1100): kernel
     ``a (1 + b eta) / 4 pi`` with ``a = a_m 0.6^(go-gi)`` (a_m = 0.15, 0.12), ``b = 0.5`` in-group else 0.2, photo-
     electric ``0.05 / 0.2 * 1.5^g``; its Legendre moments are exactly ``a`` and ``a b / 3``."""
@@ -63,11 +86,12 @@ def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int
     mu_pe = torch.tensor([[0.05 * 1.5 ** g for g in range(G)], [0.2 * 1.5 ** g for g in range(G)]],
                          dtype=torch.float64)
     mu_t = mu_pe + mu_s_l[..., 0].sum(dim=2)
-    return MultigroupLibrary(edges, mu_t, mu_s_l, eta, mu_s_eta, ("water", "bone_cortical_icrp"))
+    return MultigroupLibrary(edges, mu_t, mu_s_l, eta, mu_s_eta, ("water", "bone_cortical_icrp"),
+                             *_fine_tables(edges, mu_t, num_fine_per_group, fine_spread))
 
 
 def seeded_library(num_groups: int, legendre_order: int, e_max_kev: float, e_min_kev: float, seed: int,
-                   num_eta: int = 257) -> MultigroupLibrary:
+                   num_eta: int = 257, num_fine_per_group: int = 4, fine_spread: float = 0.15) -> MultigroupLibrary:
     """Seeded synthetic two-material library (water-like, bone-like) of Compton-dominated totals.
 
     Water's total falls from about 0.6 to 0.15 cm^2/g across the groups (highest energy first); bone's is 1.25 to
@@ -97,7 +121,8 @@ def seeded_library(num_groups: int, legendre_order: int, e_max_kev: float, e_min
                 s = float(mu_s * share[j])
                 mu_s_l[m, gi, go] = s * g_hg ** torch.arange(L + 1, dtype=torch.float64)
                 mu_s_eta[m, gi, go] = s / (4 * math.pi) * (1 - g_hg ** 2) / (1 + g_hg ** 2 - 2 * g_hg * eta) ** 1.5
-    return MultigroupLibrary(edges, totals, mu_s_l, eta, mu_s_eta, ("water", "bone"))
+    return MultigroupLibrary(edges, totals, mu_s_l, eta, mu_s_eta, ("water", "bone"),
+                             *_fine_tables(edges, totals, num_fine_per_group, fine_spread))
 
 
 def mu_t_volume(library: MultigroupLibrary, rho: Tensor) -> Tensor:
@@ -108,3 +133,38 @@ def mu_t_volume(library: MultigroupLibrary, rho: Tensor) -> Tensor:
 def transfer_moments_volume(library: MultigroupLibrary, rho: Tensor) -> Tensor:
     """Dense [G_in, G_out, L+1, R, C, S] transfer moments in 1/cm (cross_sections.py:886)."""
     return torch.einsum("mabl,mijk->ablijk", library.mu_s_l.to(rho.device, rho.dtype), rho)
+
+
+def group_transmission(library: MultigroupLibrary, path: Tensor, materials=None, filter_path: Optional[Tensor] = None,
+                       filter_attenuation: Optional[Tensor] = None) -> Tensor:
+    """[G, ...] group transmission from material line integrals ``path [M', ...]`` in g/cm^2 on the HIP kernel
+    (cross_sections.py:797): ``T_g = sum_{e in g} w_e exp(-sum_m fine_mu_t[m, e] path[m] - mu_f[e] P_f)``."""
+    from .. import _hip
+
+    if (filter_path is None) != (filter_attenuation is None):
+        raise ValueError("filter_path and filter_attenuation must be given together")
+    device = path.device
+    Mp = int(path.shape[0])
+    spatial = tuple(path.shape[1:])
+    fine_mu = library.fine_mu_t if materials is None else library.fine_mu_t[[int(m) for m in materials]]
+    if int(fine_mu.shape[0]) != Mp:
+        raise ValueError(f"path has {Mp} materials, the library {int(fine_mu.shape[0])}")
+    paths = path.reshape(Mp, -1).contiguous()
+    _hip.require_gpu_tensor(paths, "path")
+    nrays = int(paths.shape[1])
+    mu = fine_mu.to(device, torch.float32).contiguous()
+    grp = library.fine_group.to(device, torch.int32).contiguous()
+    w = library.fine_weight.to(device, torch.float32).contiguous()
+    fp = fmu = None
+    if filter_path is not None:
+        fp = filter_path.reshape(-1).to(device, torch.float32).contiguous()
+        fmu = filter_attenuation.reshape(-1).to(device, torch.float32).contiguous()
+        if fp.numel() != nrays or fmu.numel() != int(mu.shape[1]):
+            raise ValueError("filter_path must hold one value per ray and filter_attenuation one per fine energy")
+    G = library.num_groups
+    out = torch.empty(G, nrays, device=device, dtype=torch.float32)
+    with torch.cuda.device(device):
+        _hip.check(_hip.library().citrus_group_transmission(
+            paths.data_ptr(), Mp, nrays, mu.data_ptr(), grp.data_ptr(), w.data_ptr(), int(mu.shape[1]), G,
+            _hip.ptr(fp), _hip.ptr(fmu), out.data_ptr(), _hip.stream_of(device)), "citrus_group_transmission")
+    return out.reshape((G,) + spatial)

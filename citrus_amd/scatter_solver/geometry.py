@@ -118,3 +118,146 @@ class Grid:
             return self
         nr, nc, ns = self.shape
         return replace(self, shape=(nr * f, nc * f, ns * f), delta_voxel=self.delta_voxel / f)
+
+
+@dataclass(frozen=True)
+class ConeGeometry:
+    """Cone-beam geometry with a flat or cylindrical panel, circular or helical views (geometry.py:331-765).
+
+    In the view frame the source sits at ``(0, +source_iso_dist, z_shift)``, u along +x_view (channels), v along +z
+    (rows); the flat panel lies in the plane ``y_view = source_iso_dist - source_detector_dist`` and the curved panel on
+    the cylinder of radius ``source_detector_dist`` about the vertical axis through the source. A view at ``angle``
+    rotates the view frame into the recon frame by ``(x, y) = (c x_v + s y_v, -s x_v + c y_v)``.
+    """
+
+    source_iso_dist: float
+    source_detector_dist: float
+    num_det_rows: int
+    num_det_channels: int
+    delta_det_row: float
+    delta_det_channel: float
+    det_row_offset: float = 0.0
+    det_channel_offset: float = 0.0
+    use_curved_detector: bool = False
+    helical_view_angles: tuple[float, ...] = ()
+    helical_z_shifts: tuple[float, ...] = ()
+
+    def __post_init__(self) -> None:
+        angles = tuple(float(a) for a in self.helical_view_angles)
+        shifts = tuple(float(z) for z in self.helical_z_shifts)
+        if len(angles) != len(shifts):
+            raise ValueError("helical_view_angles and helical_z_shifts must have the same length")
+        steps = [b - a for a, b in zip(angles, angles[1:])]
+        if steps and not (all(s > 0 for s in steps) or all(s < 0 for s in steps)):
+            raise ValueError("helical_view_angles must be strictly monotone")
+        if steps and steps[0] < 0:
+            angles, shifts = angles[::-1], shifts[::-1]
+        object.__setattr__(self, "helical_view_angles", angles)
+        object.__setattr__(self, "helical_z_shifts", shifts)
+        object.__setattr__(self, "use_curved_detector", bool(self.use_curved_detector))
+
+    def with_helical_z_shifts(self, angles, z_shifts) -> "ConeGeometry":
+        return replace(self, helical_view_angles=tuple(float(a) for a in angles),
+                       helical_z_shifts=tuple(float(z) for z in z_shifts))
+
+    @property
+    def is_parallel(self) -> bool:
+        return False
+
+    @property
+    def is_helical(self) -> bool:
+        return len(self.helical_z_shifts) > 0 and max(self.helical_z_shifts) > min(self.helical_z_shifts)
+
+    def z_shift(self, angle: float) -> float:
+        """z shift of source and panel, linear in the helical table, constant outside it."""
+        angles, shifts = self.helical_view_angles, self.helical_z_shifts
+        if not angles:
+            return 0.0
+        a = float(angle)
+        if a <= angles[0]:
+            return shifts[0]
+        if a >= angles[-1]:
+            return shifts[-1]
+        import bisect
+
+        j = bisect.bisect_right(angles, a) - 1
+        t = (a - angles[j]) / (angles[j + 1] - angles[j])
+        return shifts[j] + t * (shifts[j + 1] - shifts[j])
+
+    @property
+    def magnification(self) -> float:
+        return self.source_detector_dist / self.source_iso_dist
+
+    @property
+    def panel_y_view(self) -> float:
+        return self.source_iso_dist - self.source_detector_dist
+
+    @property
+    def image_shape(self) -> tuple[int, int]:
+        return (self.num_det_rows, self.num_det_channels)
+
+    @property
+    def pixel_area(self) -> float:
+        return self.delta_det_row * self.delta_det_channel
+
+    def pixel_uv(self, device=None, dtype=torch.float32) -> tuple[Tensor, Tensor]:
+        """u [channels] and v [rows] of the pixel centres in cm."""
+        nc, nr = self.num_det_channels, self.num_det_rows
+        u = (torch.arange(nc, device=device, dtype=dtype) - (nc - 1) / 2) * self.delta_det_channel \
+            - self.det_channel_offset
+        v = (torch.arange(nr, device=device, dtype=dtype) - (nr - 1) / 2) * self.delta_det_row - self.det_row_offset
+        return u, v
+
+    def source_position(self, angle: float) -> tuple[float, float, float]:
+        c, s = math.cos(angle), math.sin(angle)
+        return (s * self.source_iso_dist, c * self.source_iso_dist, self.z_shift(angle))
+
+    def panel_center(self, angle: float) -> tuple[float, float, float]:
+        c, s = math.cos(angle), math.sin(angle)
+        yv = self.panel_y_view
+        return (s * yv, c * yv, self.z_shift(angle))
+
+    def panel_axes(self, angle: float):
+        """(u_hat, v_hat, n_hat) of the panel at its centre in the recon frame; n_hat points toward the source."""
+        c, s = math.cos(angle), math.sin(angle)
+        return (c, -s, 0.0), (0.0, 0.0, 1.0), (s, c, 0.0)
+
+    def panel_points(self, u: Tensor, v: Tensor, angle: float) -> Tensor:
+        """Recon-frame positions of panel points (u, v); on a curved panel the point at azimuth u/SDD."""
+        device, dtype = u.device, u.dtype
+        u, v = torch.broadcast_tensors(u, v.to(device, dtype))
+        uh, vh, nh = (torch.tensor(a, device=device, dtype=dtype) for a in self.panel_axes(angle))
+        pc = torch.tensor(self.panel_center(angle), device=device, dtype=dtype)
+        if not self.use_curved_detector:
+            return pc + u[..., None] * uh + v[..., None] * vh
+        R = float(self.source_detector_dist)
+        gamma = u / R
+        return (pc + (R * torch.sin(gamma))[..., None] * uh + (2.0 * R * torch.sin(0.5 * gamma) ** 2)[..., None] * nh
+                + (v * torch.cos(gamma))[..., None] * vh)
+
+    def pixel_centers(self, angle: float, device=None, dtype=torch.float32) -> Tensor:
+        """[rows, channels, 3] pixel-centre positions in cm."""
+        u, v = self.pixel_uv(device, dtype)
+        return self.panel_points(u[None, :], v[:, None], angle)
+
+    def panel_distance_along(self, points: Tensor, omega: Tensor, angle: float) -> Tensor:
+        """Signed distance t with ``points + t omega`` on the panel surface (larger root on the cylinder)."""
+        if self.use_curved_detector:
+            sp = torch.tensor(self.source_position(angle), device=points.device, dtype=points.dtype)
+            d = (points - sp)[..., :2]
+            w = omega.to(points.dtype)[..., :2]
+            a = (w * w).sum(-1)
+            b = (d * w).sum(-1)
+            c = (d * d).sum(-1) - float(self.source_detector_dist) ** 2
+            a, b, c = torch.broadcast_tensors(a, b, c)
+            disc = b * b - a * c
+            root = torch.sqrt(disc.clamp_min(0.0))
+            safe_a = torch.where(a > 0, a, torch.ones_like(a))
+            pos = b >= 0
+            den = torch.where(pos, b + root, safe_a)
+            num = torch.where(pos, -c, root - b)
+            t = num / torch.where(den != 0, den, torch.ones_like(den))
+            return torch.where((a > 0) & (disc >= 0) & (den != 0), t, torch.full_like(t, math.inf))
+        n = torch.tensor(self.panel_axes(angle)[2], device=points.device, dtype=points.dtype)
+        pc = torch.tensor(self.panel_center(angle), device=points.device, dtype=points.dtype)
+        return ((pc - points) * n).sum(-1) / (omega * n).sum(-1)

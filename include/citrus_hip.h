@@ -101,6 +101,79 @@ int citrus_scatter_moments_factored(const float* rho, int num_materials, const f
                                     int downscatter_only, const float* phi, float* T, int G, int nsh, int64_t N,
                                     int V, void* stream);
 
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Ray traversal (Siddon / Amanatides-Woo): exact line integrals of a channel-last field [R][C][S][nch] (nch <= 16).
+ *
+ * box = {lo_x, lo_y, lo_z, dx, dy, dz} (cm, float64) of the traced grid; axes x = cols, y = rows, z = slices.
+ * box, panel, panel_ints, point and lattice_* are HOST parameter blocks (read before the launch); volume, field,
+ * p0, p1, omega, out and the seq_* arrays are DEVICE buffers.
+ * panel = {source[3], panel_center[3], u_hat[3], v_hat[3], n_hat[3], SDD, row pitch, channel pitch, row offset,
+ * channel offset} (float64, recon frame) and panel_ints = {curved, rows, channels}.
+ * Replaces the reference's `_ray_kernel` MODE 0-3 and `_downstream_shared_kernel`
+ * (citrus/scatter_solver/kernels/ray_tracing_kernels.py:330-639) and `line_integrals` / `_trace_chunk`
+ * (citrus/scatter_solver/ray_tracing.py:141-284), behind the `Projectors` protocol (ray_tracing.py:27-115).
+ * --------------------------------------------------------------------------------------------------------------- */
+
+/* Integrals along the segments p0[i] -> p1[i] ([n][3] float64; p0_shared: one start point for every ray) into
+ * out[c][i]. Counterpart of line_integrals (ray_tracing.py:205) and SiddonProjectors.upstream_to_points (:426). */
+int citrus_ray_segments(const float* volume, int R, int C, int S, int nch, const double* box, const double* p0,
+                        const double* p1, int p0_shared, int64_t n, float* out, void* stream);
+
+/* Voxel sequence of every segment: flat voxel indices with nonzero chord, in traversal order, and their parameter
+ * lengths (up to max_len per ray; seq_count holds the full count). Test and diagnostic entry point. */
+int citrus_ray_trace_indices(int R, int C, int S, const double* box, const double* p0, const double* p1, int64_t n,
+                             int max_len, int32_t* seq_idx, float* seq_len, int32_t* seq_count, void* stream);
+
+/* Integrals from `point` to every point origin + (j dx, i dy, k dz) of a lattice of shape (rows, cols, slices) into
+ * out[c][(i*cols + j)*slices + k]: SiddonProjectors.upstream_cumulative (ray_tracing.py:405) when the lattice is the
+ * voxel centres. */
+int citrus_ray_lattice(const float* volume, int R, int C, int S, int nch, const double* box, const double* point,
+                       const double* lattice_origin, const double* lattice_delta, const int* lattice_shape,
+                       float* out, void* stream);
+
+/* Focal spot to every pixel centre, out[c][row][col]; samples > 1 traces the samples^2 midpoint sub-rays of each
+ * pixel and returns -log(mean(exp(-tau))). SiddonProjectors.primary_project (ray_tracing.py:447). */
+int citrus_ray_primary(const float* volume, int R, int C, int S, int nch, const double* box, const double* panel,
+                       const int* panel_ints, int samples, float* out, void* stream);
+
+/* Line through every pixel centre along omega[k] ([K][3]), clipped to the box, of the field of direction k
+ * (field + k * field_k_stride; stride 0 shares one field) into out[k * out_k_stride + c * out_c_stride + pixel].
+ * SiddonProjectors.oblique_project (ray_tracing.py:548). */
+int citrus_ray_oblique(const float* field, int64_t field_k_stride, int R, int C, int S, int nch, const double* box,
+                       const double* panel, const int* panel_ints, const double* omega, int K, float* out,
+                       int64_t out_k_stride, int64_t out_c_stride, void* stream);
+
+/* From every voxel centre to the panel along omega[k] into out[k][c][voxel]; +inf where omega . (-n_hat) <= 0.
+ * SiddonProjectors.downstream_cumulative (ray_tracing.py:501). */
+int citrus_ray_downstream(const float* volume, int R, int C, int S, int nch, const double* box, const double* panel,
+                          const int* panel_ints, const double* omega, int K, float* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Group transmission and uncollided fluence.
+ * --------------------------------------------------------------------------------------------------------------- */
+
+/* T[g][r] = sum_{e: fine_group[e] = g} fine_weight[e] exp(-sum_m fine_mu[m][e] paths[m][r] - filter_mu[e]
+ * filter_path[r]) for M <= 8 path channels, G <= 32 groups (filter_path and filter_mu both NULL or both given).
+ * Replaces `_group_transmission_kernel` / `group_transmission` (citrus/scatter_solver/kernels/
+ * transmission_kernels.py:190-275, citrus/scatter_solver/cross_sections.py:797-878). All pointers are device. */
+int citrus_group_transmission(const float* paths, int M, int64_t nrays, const float* fine_mu, const int* fine_group,
+                              const float* fine_weight, int E, int G, const float* filter_path,
+                              const float* filter_mu, float* out, void* stream);
+
+/* Uncollided fluence phi[g][point] = b(Omega) / |r - r0|^2 T_g(paths[:, point], filter path) at the lattice points
+ * r = origin + (j dx, i dy, k dz), lattice = {ox, oy, oz, dx, dy, dz} and lattice_shape = {rows, cols, slices}
+ * (host). source = the 20 values of the ray-traversal panel block, then {field margin rows, field margin channels,
+ * filter density} (host, 23 doubles); source_ints = {curved, rows, channels, off-focal nodes K <= 16} (host);
+ * off_focal = {K distances, K fractions} (host, or NULL for K = 0); profile and thickness are device [rows][channels]
+ * images or NULL. Counterpart of uncollided_from_path_lengths (citrus/scatter_solver/uncollided.py:160-197) with
+ * SourceModel.angular_weight and filter_path (citrus/scatter_solver/source.py:289-446). */
+int citrus_uncollided_fluence(const float* paths, int M, const double* lattice, const int* lattice_shape,
+                              const float* fine_mu, const int* fine_group, const float* fine_weight, int E, int G,
+                              const double* source, const int* source_ints, const double* off_focal,
+                              const float* profile, const float* thickness, const float* filter_mu, float* out,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

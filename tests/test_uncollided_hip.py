@@ -1,0 +1,100 @@
+"""Group transmission and uncollided fluence on the HIP path vs closed forms and the FP64 oracle (GPU).
+
+Reference tests: vacuum 1/r^2 and polychromatic Beer's law (tests/test_uncollided.py:69-137), collimation (:166)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from citrus_amd.scatter_solver.cross_sections import group_transmission, seeded_library, synthetic_library
+from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
+from citrus_amd.scatter_solver.source import SourceModel, bowtie_thickness
+from citrus_amd.scatter_solver.uncollided import uncollided_fluence
+from oracle import rays as orays
+from oracle import uncollided as ounc
+from tests.conftest import gpu_device
+from tests.helpers import F32_REL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib_np(library):
+    return (library.fine_mu_t.numpy(), library.fine_group.numpy(), library.fine_weight.numpy())
+
+
+@pytest.mark.parametrize("with_filter", [False, True])
+def test_group_transmission_matches_oracle(with_filter):
+    dev = gpu_device()
+    library = seeded_library(16, 3, 140.0, 20.0, seed=4)
+    rng = np.random.default_rng(0)
+    paths = rng.random((2, 3, 50)) * 4.0
+    fp = rng.random((3, 50)) * 2.0 if with_filter else None
+    fmu = rng.random(library.fine_energy.numel()) * 0.5 if with_filter else None
+    got = group_transmission(library, torch.from_numpy(paths).float().to(dev),
+                             filter_path=None if fp is None else torch.from_numpy(fp).to(dev),
+                             filter_attenuation=None if fmu is None else torch.from_numpy(fmu))
+    ref = ounc.group_transmission(paths.reshape(2, -1), *_lib_np(library), 16,
+                                  None if fp is None else fp.reshape(-1), fmu)
+    assert got.shape == (16, 3, 50)
+    assert rel_err(got.reshape(16, -1), ref) < F32_REL
+
+
+def test_vacuum_inverse_square_and_beer_law():
+    dev = gpu_device()
+    library = synthetic_library(4, 3, num_fine_per_group=3, fine_spread=0.4)
+    grid = Grid((10, 12, 8), 0.5)
+    geo = ConeGeometry(60.0, 100.0, 64, 64, 0.5, 0.5)  # field covers the whole grid
+    src = SourceModel()
+    angle = 0.4
+    centers = grid.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    r2 = ((centers - orays.source_position(60.0, angle)) ** 2).sum(1)
+    vac = uncollided_fluence(library, torch.zeros(2, *grid.shape, device=dev), grid, geo, angle, src)
+    exact = library.spectrum_fraction.numpy()[:, None] / r2[None]
+    assert rel_err(vac.reshape(4, -1), exact) < F32_REL
+    rho = torch.zeros(2, *grid.shape, dtype=torch.float64)
+    rho[0] = 1.0
+    phi = uncollided_fluence(library, rho.to(dev, torch.float32), grid, geo, angle, src)
+    lo = np.array(grid.lower_corner())
+    hi = lo + np.array(grid.extent)
+    L = np.array([orays.box_chord(orays.source_position(60.0, angle), c, lo, hi) for c in centers])
+    mu_w, grp, w = library.fine_mu_t.numpy()[0], library.fine_group.numpy(), library.fine_weight.numpy()
+    beer = np.zeros((4, centers.shape[0]))
+    for e in range(mu_w.size):
+        beer[grp[e]] += w[e] * np.exp(-mu_w[e] * L)
+    assert rel_err(phi.reshape(4, -1), beer / r2[None]) < F32_REL
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_uncollided_with_profile_filter_offfocal_matches_oracle(curved):
+    dev = gpu_device()
+    library = seeded_library(8, 3, 120.0, 20.0, seed=2)
+    grid = Grid((14, 16, 12), 0.5)
+    geo = ConeGeometry(100.0, 150.0, 10, 12, 0.6, 0.8, use_curved_detector=curved)  # narrow field: edges cut
+    rng = np.random.default_rng(3)
+    profile = torch.from_numpy(0.5 + rng.random((10, 12)))
+    thick = bowtie_thickness(geo, 2.0)
+    fmu = torch.from_numpy(rng.random(library.fine_energy.numel()) * 0.3 + 0.1)
+    src = SourceModel(emission_profile=profile, field_margin_channels_cm=0.3, field_margin_rows_cm=0.1,
+                      off_focal_distance_cm=(0.0, 1.0, 3.0), off_focal_fraction=(0.05, 0.03, 0.01),
+                      filter_thickness_cm=thick, filter_density=2.7, filter_attenuation_table=fmu)
+    rho = torch.from_numpy(rng.random((2, *grid.shape)) * 1.2)
+    angle = 1.1
+    phi = uncollided_fluence(library, rho.to(dev, torch.float32), grid, geo, angle, src, materials=None)
+    centers = grid.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    lo = np.array(grid.lower_corner())
+    n = (grid.num_cols, grid.num_rows, grid.num_slices)
+    s0 = orays.source_position(100.0, angle)
+    paths = orays.line_integrals(rho.float().double().numpy(), np.broadcast_to(s0, centers.shape), centers, lo,
+                                 np.array(grid.deltas), n)
+    ref = ounc.uncollided_fluence(paths, centers, *_lib_np(library), 8, 100.0, 150.0, 10, 12, 0.6, 0.8, angle,
+                                  curved, profile=profile.numpy().astype(np.float32).astype(np.float64),
+                                  thickness=np.broadcast_to(thick.numpy().astype(np.float32), (10, 12)).astype(
+                                      np.float64),
+                                  filter_density=2.7, filter_mu=fmu.numpy(), margin_ch=0.3, margin_row=0.1,
+                                  off_dist=np.array([0.0, 1.0, 3.0]), off_frac=np.array([0.05, 0.03, 0.01]))
+    assert (ref > 0).mean() < 0.999  # the field edge and off-focal tail are exercised
+    assert rel_err(phi.reshape(8, -1), ref) < F32_REL
