@@ -31,7 +31,16 @@
 namespace citrus {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 128;
+
+// Native 4-wide float vector for register-resident prefetch buffers (HIP's float4 wrapper is copied through memory,
+// which keeps arrays of it out of registers).
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Workgroup barrier that orders LDS accesses only: outstanding global loads stay in flight across it (a full
+// __syncthreads also waits for them). Every cross-thread exchange in the sweep kernel goes through LDS.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 
 // Cells of a B^3 block grouped by diagonal plane a + b + c = p, in sweep-local coordinates.
 template <int B>
@@ -59,14 +68,11 @@ struct PlaneTable {
 };
 
 __constant__ PlaneTable<4> kPlanes4 = PlaneTable<4>();
-__constant__ PlaneTable<8> kPlanes8 = PlaneTable<8>();
 
 template <int B>
 __device__ __forceinline__ const PlaneTable<B>& planes();
 template <>
 __device__ __forceinline__ const PlaneTable<4>& planes<4>() { return kPlanes4; }
-template <>
-__device__ __forceinline__ const PlaneTable<8>& planes<8>() { return kPlanes8; }
 
 struct LaunchArgs {
   int R, C, S, N;
@@ -93,21 +99,27 @@ struct ReduceShape {
   static constexpr int kChunks = (NSH + kPer - 1) / kPer;
 };
 
-template <int NSH, int B>
-__global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
+template <int NSH, int B, bool HAS_T, bool HAS_SRC>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void ld_sweep_kernel(LaunchArgs a) {
   constexpr int BB = B * B;
-  constexpr PlaneTable<B> host_table{};
-  constexpr int PMAX = host_table.max_plane();
+  constexpr PlaneTable<B> ptab{};
+  constexpr int PMAX = ptab.max_plane();
+  constexpr int NPL = PlaneTable<B>::kPlanes;
   constexpr int KPER = ReduceShape<NSH>::kPer;
   constexpr int KCH = ReduceShape<NSH>::kChunks;
+  constexpr int TPT = (PMAX * NSH + kThreads - 1) / kThreads;  // float4 of T staged per thread and plane
+  constexpr int RPT = (PMAX * KCH + kThreads - 1) / kThreads;  // reduction items per thread and plane
   const PlaneTable<B>& tab = planes<B>();
 
   extern __shared__ float lds[];
   const int dc = a.dc;
-  float* face = lds;                           // [3][dc][BB][3]
-  float* tpl = face + 9 * dc * BB;             // [PMAX][NSH][4]
-  float* psl = tpl + PMAX * NSH * 4;           // [PMAX][dc][4]
-  float* ywl = psl + PMAX * dc * 4;            // [dc][NSH]
+  float* face = lds;                            // [3 axes][3 components][BB positions][dc]
+  float* tpl = face + 9 * dc * BB;              // [PMAX][NSH + 1][4]
+  float* psl = tpl + PMAX * (NSH + 1) * 4;      // [PMAX][dc][4]
+  float* ywl = psl + PMAX * dc * 4;             // [dc][NSH]
+  int* slot_cell = reinterpret_cast<int*>(ywl + dc * NSH);      // [B^3] flat cell of each slot, or -1
+  int* slot_pos = slot_cell + B * B * B;                         // [B^3] local (x, y, z) of each slot, packed
+  float* tmu = reinterpret_cast<float*>(slot_pos + B * B * B);  // [2][PMAX] mu_t of the staged planes
 
   const int tid = threadIdx.x;
   const int task = blockIdx.x % a.ntask;
@@ -126,8 +138,22 @@ __global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
 
   const int64_t face_stride = (int64_t)dc * BB * 3;  // floats per (block, axis)
   float* gface_v = a.faces + (int64_t)v * a.fview + ((int64_t)slot * a.G + g) * a.nblocks * 3 * face_stride;
+  const float* mu_g = a.mu + (int64_t)g * a.N;
+  const float* T_g = HAS_T ? a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4 : nullptr;
+  float* phi_g = a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
+  float* src_v = HAS_SRC ? a.src + (int64_t)v * a.sview : nullptr;
 
-  // ---- direction tables ---------------------------------------------------------------------------------------
+  // ---- per-task tables in LDS: slots, direction weights, incoming faces ------------------------------------------
+  for (int sl = tid; sl < B * B * B; sl += kThreads) {
+    const int packed = tab.cell[sl];
+    const int pa = packed & 0xff, pb = (packed >> 8) & 0xff, pc = (packed >> 16) & 0xff;
+    const int lx = sx > 0 ? pa : B - 1 - pa;
+    const int ly = sy > 0 ? pb : B - 1 - pb;
+    const int lz = sz > 0 ? pc : B - 1 - pc;
+    const int x = x0 + lx, yy = y0 + ly, z = z0 + lz;
+    slot_cell[sl] = (x >= a.C || yy >= a.R || z >= a.S) ? -1 : (yy * a.C + x) * a.S + z;
+    slot_pos[sl] = lx | (ly << 8) | (lz << 16);
+  }
   const int mbase = o * a.D8 + a.d0;
   for (int i = tid; i < dc * NSH; i += kThreads) ywl[i] = a.Yw[(int64_t)(mbase + i / NSH) * NSH + i % NSH];
   const int cpr = kThreads / dc;  // cells per round of the solve phase
@@ -135,12 +161,10 @@ __global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
   const bool solver = tid < cpr * dc;
   float y[NSH];
 #pragma unroll
-  for (int n = 0; n < NSH; ++n) y[n] = a.Y[(int64_t)(mbase + my_d) * NSH + n];
+  for (int n = 0; n < NSH; ++n) y[n] = HAS_T ? a.Y[(int64_t)(mbase + my_d) * NSH + n] : 0.0f;
   const float cx = a.cdir[(mbase + my_d) * 3 + 0];
   const float cy = a.cdir[(mbase + my_d) * 3 + 1];
   const float cz = a.cdir[(mbase + my_d) * 3 + 2];
-
-  // ---- incoming faces: the upstream block's outgoing faces, or vacuum -----------------------------------------
   {
     const int ub[3][3] = {{bi, bj - sx, bk}, {bi - sy, bj, bk}, {bi, bj, bk - sz}};
     for (int ax = 0; ax < 3; ++ax) {
@@ -155,48 +179,84 @@ __global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
       }
     }
   }
+  __syncthreads();
 
-  const float* mu_g = a.mu + (int64_t)g * a.N;
-  const float* T_g = a.T ? a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4 : nullptr;
-  float* phi_g = a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
-  float* src_v = a.src ? a.src + (int64_t)v * a.sview : nullptr;
-
-  // cell of plane slot: physical flat index or -1 if outside the grid
-  auto cell_of = [&](int packed, int& lx, int& ly, int& lz) -> int {
-    const int pa = packed & 0xff, pb = (packed >> 8) & 0xff, pc = (packed >> 16) & 0xff;
-    lx = sx > 0 ? pa : B - 1 - pa;
-    ly = sy > 0 ? pb : B - 1 - pb;
-    lz = sz > 0 ? pc : B - 1 - pc;
-    const int x = x0 + lx, yy = y0 + ly, z = z0 + lz;
-    if (x >= a.C || yy >= a.R || z >= a.S) return -1;
-    return (yy * a.C + x) * a.S + z;
+  // ---- cell planes --------------------------------------------------------------------------------------------
+  // Every global load below is unconditional (clamped to a valid address, the value discarded when unused), issued
+  // in a fixed order, and the plane loop is fully unrolled, so the compiler counts outstanding loads exactly: the
+  // moments T and mu_t of plane p+2 and the old fluence of plane p are in flight during the solve of plane p, and
+  // the barriers (lds_barrier) order LDS only.
+  const float4* T4 = reinterpret_cast<const float4*>(T_g);
+  float4* tpl4 = reinterpret_cast<float4*>(tpl);
+  struct Staged {  // one cell plane's prefetched inputs, held in registers
+    f4 t[TPT];
+    float mu;
   };
+  f4 ppre[RPT][KPER];
+  const f4* T4v = reinterpret_cast<const f4*>(T_g);
 
-  for (int p = 0; p < PlaneTable<B>::kPlanes; ++p) {
-    const int pbeg = tab.offset[p];
-    const int P = tab.offset[p + 1] - pbeg;
-    // (1) stage T of the plane's cells
-    if (T_g) {
-      for (int i = tid; i < P * NSH; i += kThreads) {
-        const int ci = i / NSH, n = i % NSH;
-        int lx, ly, lz;
-        const int cell = cell_of(tab.cell[pbeg + ci], lx, ly, lz);
-        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (cell >= 0) t = reinterpret_cast<const float4*>(T_g + (int64_t)cell * NSH * 4)[n];
-        reinterpret_cast<float4*>(tpl)[ci * NSH + n] = t;
+  auto load_plane = [&](const int pl) __attribute__((always_inline)) -> Staged {
+    Staged st;
+    const int pb = ptab.offset[pl];
+    const int P = ptab.offset[pl + 1] - pb;
+    {
+      const int c = slot_cell[pb + (tid < P ? tid : P - 1)];
+      st.mu = mu_g[c >= 0 ? c : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < TPT; ++r) {
+      if (HAS_T) {
+        int i = tid + r * kThreads;
+        i = i < P * NSH ? i : P * NSH - 1;
+        const int ci = i / NSH, n = i - ci * NSH;
+        const int c = slot_cell[pb + ci];
+        st.t[r] = T4v[(int64_t)(c >= 0 ? c : 0) * NSH + n];
+      } else {
+        st.t[r] = f4{0.f, 0.f, 0.f, 0.f};
       }
     }
-    __syncthreads();
-    // (2) solve every (cell, direction) pair of the plane
+    return st;
+  };
+
+  // stages plane p from `cur`, runs it, and returns the prefetched inputs of plane p + 2
+  auto plane = [&](const int p, const Staged cur) __attribute__((always_inline)) -> Staged {
+    const int which = p & 1;
+    const int pbeg = ptab.offset[p];
+    const int P = ptab.offset[p + 1] - pbeg;
+    // (1) stage this plane's mu_t and T in LDS
+    if (tid < P) tmu[which * PMAX + tid] = cur.mu;
+    if (HAS_T) {
+#pragma unroll
+      for (int r = 0; r < TPT; ++r) {
+        const int i = tid + r * kThreads;
+        if (i < P * NSH) reinterpret_cast<f4*>(tpl4)[i + i / NSH] = cur.t[r];  // cell stride NSH + 1: no conflicts
+      }
+    }
+    // (2) request the fluence moments this plane adds to, then the next-but-one plane
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      int i = tid + r * kThreads;
+      i = i < P * KCH ? i : P * KCH - 1;
+      const int ci = i / KCH, n0 = (i - ci * KCH) * KPER;
+      const int c = slot_cell[pbeg + ci];
+      const f4* src = reinterpret_cast<const f4*>(phi_g + (int64_t)(c >= 0 ? c : 0) * NSH * 4);
+#pragma unroll
+      for (int j = 0; j < KPER; ++j) ppre[r][j] = src[(n0 + j < NSH) ? n0 + j : NSH - 1];
+    }
+    // (the last two planes reload the final plane: a few wasted loads instead of a select between aggregates)
+    const Staged next = load_plane(p + 2 < NPL ? p + 2 : NPL - 1);
+    lds_barrier();
+    // (3) solve every (cell, direction) pair of the plane
     if (solver) {
       for (int ci = tid / dc; ci < P; ci += cpr) {
-        int lx, ly, lz;
-        const int cell = cell_of(tab.cell[pbeg + ci], lx, ly, lz);
+        const int cell = slot_cell[pbeg + ci];
+        const int packed = slot_pos[pbeg + ci];
+        const int lx = packed & 0xff, ly = (packed >> 8) & 0xff, lz = (packed >> 16) & 0xff;
         float4 pout = make_float4(0.f, 0.f, 0.f, 0.f);
         if (cell >= 0) {
           float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-          if (T_g) {
-            const float4* t4 = reinterpret_cast<const float4*>(tpl) + ci * NSH;
+          if (HAS_T) {
+            const float4* t4 = tpl4 + ci * (NSH + 1);
 #pragma unroll
             for (int n = 0; n < NSH; ++n) {
               const float4 t = t4[n];
@@ -207,19 +267,21 @@ __global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
             }
           }
           float4* sp = nullptr;
-          if (src_v) {
+          if (HAS_SRC) {
             sp = reinterpret_cast<float4*>(src_v + (((int64_t)(o * dc + my_d) * a.N + cell) * a.G + g) * 4);
             const float4 q = *sp;
             s0 += q.x; s1 += q.y; s2 += q.z; s3 += q.w;
           }
-          float* fx = face + 0 * face_stride + (my_d * BB + ly * B + lz) * 3;
-          float* fy = face + 1 * face_stride + (my_d * BB + lx * B + lz) * 3;
-          float* fz = face + 2 * face_stride + (my_d * BB + lx * B + ly) * 3;
-          // upstream faces: [0] face average psi_0 + s_a psi_a, [1], [2] transverse moments (see face writes below)
-          const float pbx = fx[0], xy = fx[1], xz = fx[2];
-          const float pby = fy[0], yx = fy[1], yz = fy[2];
-          const float pbz = fz[0], zx = fz[1], zy = fz[2];
-          const float mu = mu_g[cell];
+          // face arrays are [component][position][direction]: the lanes of one cell read consecutive words;
+          // component 0 is the face average psi_0 + s_a psi_a, 1 and 2 the transverse moments
+          const int cs = BB * dc;
+          float* fx = face + 0 * face_stride + (ly * B + lz) * dc + my_d;
+          float* fy = face + 1 * face_stride + (lx * B + lz) * dc + my_d;
+          float* fz = face + 2 * face_stride + (lx * B + ly) * dc + my_d;
+          const float pbx = fx[0], xy = fx[cs], xz = fx[2 * cs];
+          const float pby = fy[0], yx = fy[cs], yz = fy[2 * cs];
+          const float pbz = fz[0], zx = fz[cs], zy = fz[2 * cs];
+          const float mu = tmu[which * PMAX + ci];
           const float r0 = s0 + (cx * pbx + cy * pby + cz * pbz);
           const float rx = s1 - 3.0f * fsx * cx * pbx + (cy * yx + cz * zx);
           const float ry = s2 - 3.0f * fsy * cy * pby + (cx * xy + cz * zy);
@@ -234,22 +296,23 @@ __global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
           const float px = (rx + 3.0f * fsx * cx * p0) / dx;
           const float py = (ry + 3.0f * fsy * cy * p0) / dy;
           const float pz = (rz + 3.0f * fsz * cz * p0) / dz;
-          fx[0] = p0 + fsx * px; fx[1] = py; fx[2] = pz;
-          fy[0] = p0 + fsy * py; fy[1] = px; fy[2] = pz;
-          fz[0] = p0 + fsz * pz; fz[1] = px; fz[2] = py;
+          fx[0] = p0 + fsx * px; fx[cs] = py; fx[2 * cs] = pz;
+          fy[0] = p0 + fsy * py; fy[cs] = px; fy[2 * cs] = pz;
+          fz[0] = p0 + fsz * pz; fz[cs] = px; fz[2 * cs] = py;
           pout = make_float4(p0, px, py, pz);
-          if (a.write_psi) *sp = pout;
+          if (HAS_SRC && a.write_psi) *sp = pout;
         }
         reinterpret_cast<float4*>(psl)[ci * dc + my_d] = pout;
       }
     }
-    __syncthreads();
-    // (3) reduce the plane's angular fluence to moments and add them into phi
-    for (int i = tid; i < P * KCH; i += kThreads) {
-      const int ci = i / KCH, n0 = (i % KCH) * KPER;
-      int lx, ly, lz;
-      const int cell = cell_of(tab.cell[pbeg + ci], lx, ly, lz);
-      if (cell < 0) continue;
+    lds_barrier();
+    // (4) reduce the plane's angular fluence to moments and add them into phi
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int i = tid + r * kThreads;
+      if (i >= P * KCH) continue;
+      const int ci = i / KCH, n0 = (i - ci * KCH) * KPER;
+      const int cell = slot_cell[pbeg + ci];
       float4 acc[KPER];
 #pragma unroll
       for (int j = 0; j < KPER; ++j) acc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -258,31 +321,38 @@ __global__ __launch_bounds__(kThreads) void ld_sweep_kernel(LaunchArgs a) {
         const float4 q = ps[d];
 #pragma unroll
         for (int j = 0; j < KPER; ++j) {
-          if (n0 + j < NSH) {
-            const float w = ywl[d * NSH + n0 + j];
-            acc[j].x = fmaf(w, q.x, acc[j].x);
-            acc[j].y = fmaf(w, q.y, acc[j].y);
-            acc[j].z = fmaf(w, q.z, acc[j].z);
-            acc[j].w = fmaf(w, q.w, acc[j].w);
-          }
+          const float w = (n0 + j < NSH) ? ywl[d * NSH + n0 + j] : 0.0f;
+          acc[j].x = fmaf(w, q.x, acc[j].x);
+          acc[j].y = fmaf(w, q.y, acc[j].y);
+          acc[j].z = fmaf(w, q.z, acc[j].z);
+          acc[j].w = fmaf(w, q.w, acc[j].w);
         }
       }
-      float4* dst = reinterpret_cast<float4*>(phi_g + (int64_t)cell * NSH * 4);
+      if (cell >= 0) {
+        float4* dst = reinterpret_cast<float4*>(phi_g + (int64_t)cell * NSH * 4);
 #pragma unroll
-      for (int j = 0; j < KPER; ++j) {
-        if (n0 + j < NSH) {
-          if (first_visit) {
-            dst[n0 + j] = acc[j];
-          } else {
-            float4 old = dst[n0 + j];
-            old.x += acc[j].x; old.y += acc[j].y; old.z += acc[j].z; old.w += acc[j].w;
-            dst[n0 + j] = old;
+        for (int j = 0; j < KPER; ++j) {
+          if (n0 + j < NSH) {
+            float4 val = acc[j];
+            if (!first_visit) {
+              val.x += ppre[r][j].x; val.y += ppre[r][j].y; val.z += ppre[r][j].z; val.w += ppre[r][j].w;
+            }
+            dst[n0 + j] = val;
           }
         }
       }
     }
-    // no barrier needed here: the next plane's stage phase writes tpl only, and its solve phase (which overwrites
-    // psl) starts after the next barrier
+    // the next plane writes tpl and tmu (of the other parity) before its first barrier: tpl was last read before the
+    // barrier above, and psl, which the reduction reads, is next written after the next barrier
+    return next;
+  };
+
+  Staged sa = load_plane(0);
+  Staged sb = load_plane(NPL > 1 ? 1 : 0);
+#pragma unroll
+  for (int p = 0; p < NPL; p += 2) {
+    sa = plane(p, sa);
+    if (p + 1 < NPL) sb = plane(p + 1, sb);
   }
   __syncthreads();
 
@@ -306,7 +376,8 @@ constexpr int plane_max() {
 
 template <int NSH, int B>
 size_t lds_bytes(int dc) {
-  return sizeof(float) * (size_t)(9 * dc * B * B + plane_max<B>() * NSH * 4 + plane_max<B>() * dc * 4 + dc * NSH);
+  return sizeof(float) * (size_t)(9 * dc * B * B + plane_max<B>() * (NSH + 1) * 4 + plane_max<B>() * dc * 4 +
+                                  dc * NSH + 2 * B * B * B + 2 * plane_max<B>());
 }
 
 constexpr size_t kMaxLds = 64 * 1024;
@@ -315,7 +386,12 @@ template <int NSH, int B>
 hipError_t launch_one(const LaunchArgs& la, int nwg, hipStream_t stream) {
   const size_t lds = lds_bytes<NSH, B>(la.dc);
   if (lds > kMaxLds) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((ld_sweep_kernel<NSH, B>), dim3(nwg), dim3(kThreads), lds, stream, la);
+  if (la.T && !la.src)
+    hipLaunchKernelGGL((ld_sweep_kernel<NSH, B, true, false>), dim3(nwg), dim3(kThreads), lds, stream, la);
+  else if (la.T)
+    hipLaunchKernelGGL((ld_sweep_kernel<NSH, B, true, true>), dim3(nwg), dim3(kThreads), lds, stream, la);
+  else
+    hipLaunchKernelGGL((ld_sweep_kernel<NSH, B, false, true>), dim3(nwg), dim3(kThreads), lds, stream, la);
   return hipGetLastError();
 }
 
@@ -457,7 +533,7 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
     la.first = (!A.accumulate && plan->launch_pass[l] == 0) ? 1 : 0;
     const int64_t nwg = (int64_t)la.ntask * A.G * A.V;
     if (nwg > 0x7fffffff) return fail(CITRUS_ERR_ARG, "too many workgroups in one launch");
-    hipError_t e = plan->B == 4 ? launch_nsh<4>(A.nsh, la, (int)nwg, s) : launch_nsh<8>(A.nsh, la, (int)nwg, s);
+    hipError_t e = launch_nsh<4>(A.nsh, la, (int)nwg, s);
     if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
     if (e == hipErrorInvalidConfiguration) return fail(CITRUS_ERR_UNSUPPORTED, "chunk needs more than 64 KiB of LDS");
     if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));

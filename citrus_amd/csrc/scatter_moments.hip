@@ -1,9 +1,10 @@
 // Scattering-source moments T[go][n][k] = sum_gi mu_s_l[gi][go][l(n)] phi[gi][n][k] for every cell (gfx950).
 //
-// One thread per (view, cell, moment n, LD moment k): consecutive threads walk the contiguous [n][k] of a cell, so
-// every read of phi[gi] and write of T[go] is a coalesced stream and phi is read from HBM once. Output groups are
-// accumulated in registers in chunks of 8; the transfer coefficient of a (gi, go, l) triple is read from the dense
-// volume or formed from the densities and the per-material table held in LDS.
+// A workgroup owns kCells cells of one view. It first forms each cell's transfer matrices mu[l][gi][go] for the
+// degrees l <= L in LDS (from the dense volume, or from the densities and the per-material table), once per cell
+// instead of once per moment; then one thread per (cell, n, k) element keeps phi[.][n][k] of every input group in
+// registers and accumulates all output groups from the matrix of its degree. phi and T are streamed once, coalesced
+// along the contiguous [n][k] of each cell.
 
 #include <hip/hip_runtime.h>
 
@@ -16,76 +17,112 @@
 namespace citrus {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kGoChunk = 8;
+constexpr int kCells = 2;
 constexpr int kMaxMaterials = 8;
 
 __device__ __forceinline__ int degree_of(int n) {
-  int l = (int)sqrtf((float)n);
-  while (l * l > n) --l;
+  int l = 0;
   while ((l + 1) * (l + 1) <= n) ++l;
   return l;
 }
 
-template <bool FACTORED>
-__global__ __launch_bounds__(kThreads) void scatter_moments_kernel(const float* __restrict__ transfer,
-                                                                   const float* __restrict__ rho, int M,
-                                                                   int Lt, const float* __restrict__ phi,
-                                                                   float* __restrict__ T, int G, int nsh,
-                                                                   int64_t N, int64_t total, int lower_only) {
-  extern __shared__ float sigma_l[];  // factored: [M][G][G][Lt+1]
+// LDS: mu [kCells][L+1][G][G], then the material table [M][G][G][Lt+1] (factored form)
+template <int GMAX, bool FACTORED>
+__global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
+    const float* __restrict__ transfer, const float* __restrict__ rho, int M, int Lt, const float* __restrict__ phi,
+    float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only) {
+  extern __shared__ float lds[];
+  const int per_cell = (L + 1) * G * G;
+  float* mu = lds;
+  float* table = lds + kCells * per_cell;
+  const int nthreads = blockDim.x;
+  const int tid = threadIdx.x;
+  const int cells_per_view = (N + kCells - 1) / kCells;
+  const int v = blockIdx.x / cells_per_view;
+  const int c0 = (blockIdx.x % cells_per_view) * kCells;
   if (FACTORED) {
-    const int ns = M * G * G * (Lt + 1);
-    for (int i = threadIdx.x; i < ns; i += blockDim.x) sigma_l[i] = transfer[i];
+    const int nt = M * G * G * (Lt + 1);
+    for (int i = tid; i < nt; i += nthreads) table[i] = transfer[i];
     __syncthreads();
   }
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  const int64_t per_view = N * nsh * 4;
-  const int64_t v = t / per_view;
-  const int64_t r = t - v * per_view;  // (cell, n, k) within the view
-  const int64_t cell = r / (nsh * 4);
-  const int n = (int)((r / 4) % nsh);
-  const int l = degree_of(n);
-  const float* phi_v = phi + v * G * per_view;
-  float* T_v = T + v * G * per_view;
-  float rho_c[kMaxMaterials];
-  if (FACTORED) {
-    for (int m = 0; m < M; ++m) rho_c[m] = rho[(int64_t)m * N + cell];
-  }
-  for (int go0 = 0; go0 < G; go0 += kGoChunk) {
-    float acc[kGoChunk];
+  // per-cell transfer matrices
+  for (int i = tid; i < kCells * per_cell; i += nthreads) {
+    const int cc = i / per_cell;
+    const int r = i - cc * per_cell;
+    const int l = r / (G * G);
+    const int gi = (r / G) % G;
+    const int go = r % G;
+    const int cell = c0 + cc;
+    float m_val = 0.0f;
+    if (cell < N && !(lower_only && gi > go)) {
+      if (FACTORED) {
 #pragma unroll
-    for (int j = 0; j < kGoChunk; ++j) acc[j] = 0.0f;
-    const int gi_end = lower_only ? (go0 + kGoChunk < G ? go0 + kGoChunk : G) : G;
-    for (int gi = 0; gi < gi_end; ++gi) {
-      const float p = phi_v[(int64_t)gi * per_view + r];
-#pragma unroll
-      for (int j = 0; j < kGoChunk; ++j) {
-        const int go = go0 + j;
-        if (go >= G || (lower_only && gi > go)) continue;
-        float mu;
-        if (FACTORED) {
-          mu = 0.0f;
-          for (int m = 0; m < M; ++m) mu = fmaf(rho_c[m], sigma_l[((m * G + gi) * G + go) * (Lt + 1) + l], mu);
-        } else {
-          mu = transfer[(((int64_t)gi * G + go) * (Lt + 1) + l) * N + cell];
-        }
-        acc[j] = fmaf(mu, p, acc[j]);
+        for (int m = 0; m < kMaxMaterials; ++m)
+          if (m < M) m_val = fmaf(rho[(int64_t)m * N + cell], table[((m * G + gi) * G + go) * (Lt + 1) + l], m_val);
+      } else {
+        m_val = transfer[(((int64_t)gi * G + go) * (Lt + 1) + l) * N + cell];
       }
     }
+    mu[i] = m_val;
+  }
+  __syncthreads();
+  const int elems = nsh * 4;
+  const int cc = tid / elems;
+  const int r = tid - cc * elems;
+  const int cell = c0 + cc;
+  if (cc >= kCells || cell >= N) return;
+  const int l = degree_of(r / 4);
+  const int64_t view_stride = (int64_t)G * N * elems;
+  const int64_t group_stride = (int64_t)N * elems;
+  const int64_t off = (int64_t)v * view_stride + (int64_t)cell * elems + r;
+  float p[GMAX];
 #pragma unroll
-    for (int j = 0; j < kGoChunk; ++j)
-      if (go0 + j < G) T_v[(int64_t)(go0 + j) * per_view + r] = acc[j];
+  for (int gi = 0; gi < GMAX; ++gi) p[gi] = gi < G ? phi[off + gi * group_stride] : 0.0f;
+  const float* mcell = mu + cc * per_cell + l * G * G;
+#pragma unroll
+  for (int go = 0; go < GMAX; ++go) {
+    if (go >= G) break;
+    float acc = 0.0f;
+#pragma unroll
+    for (int gi = 0; gi < GMAX; ++gi)
+      if (gi < G && !(lower_only && gi > go)) acc = fmaf(mcell[gi * G + go], p[gi], acc);
+    T[off + go * group_stride] = acc;
   }
 }
 
-int check(int G, int nsh, int Lt, int64_t N, int V) {
-  if (G < 1 || N < 1 || V < 1) return fail(CITRUS_ERR_ARG, "G, N and V must be >= 1");
-  int L = 0;
+int check(int G, int nsh, int Lt, int64_t N, int V, int& L) {
+  if (G < 1 || G > 32) return fail(CITRUS_ERR_ARG, "1..32 groups supported");
+  if (N < 1 || V < 1 || N > 0x7fffffff) return fail(CITRUS_ERR_ARG, "N and V must be >= 1 (N < 2^31)");
+  L = 0;
   while ((L + 1) * (L + 1) < nsh) ++L;
-  if ((L + 1) * (L + 1) != nsh) return fail(CITRUS_ERR_ARG, "nsh must be a square (L+1)^2");
+  if ((L + 1) * (L + 1) != nsh || nsh > 64) return fail(CITRUS_ERR_ARG, "nsh must be (L+1)^2 with L <= 7");
   if (Lt < L) return fail(CITRUS_ERR_ARG, "transfer Legendre order below the moments' degree");
+  return CITRUS_OK;
+}
+
+template <bool FACTORED>
+int launch(const float* transfer, const float* rho, int M, int Lt, const float* phi, float* T, int G, int nsh,
+           int64_t N, int V, int lower_only, void* stream) {
+  int L = 0;
+  if (int rc = check(G, nsh, Lt, N, V, L)) return rc;
+  const size_t lds = sizeof(float) * ((size_t)kCells * (L + 1) * G * G + (FACTORED ? (size_t)M * G * G * (Lt + 1) : 0));
+  if (lds > 64 * 1024) return fail(CITRUS_ERR_UNSUPPORTED, "transfer tables exceed 64 KiB of LDS");
+  const int threads = kCells * nsh * 4;
+  const int64_t blocks = (int64_t)V * ((N + kCells - 1) / kCells);
+  if (blocks > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many cells");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)blocks), block((unsigned)threads);
+  if (G <= 8)
+    hipLaunchKernelGGL((scatter_moments_kernel<8, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
+                       nsh, L, (int)N, V, lower_only);
+  else if (G <= 16)
+    hipLaunchKernelGGL((scatter_moments_kernel<16, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
+                       nsh, L, (int)N, V, lower_only);
+  else
+    hipLaunchKernelGGL((scatter_moments_kernel<32, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
+                       nsh, L, (int)N, V, lower_only);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("scatter moments: ") + hipGetErrorString(e));
   return CITRUS_OK;
 }
 
@@ -95,32 +132,15 @@ int check(int G, int nsh, int Lt, int64_t N, int V) {
 extern "C" int citrus_scatter_moments_dense(const float* transfer, int Lt, const float* phi, float* T, int G, int nsh,
                                             int64_t N, int V, void* stream) {
   using namespace citrus;
-  if (int rc = check(G, nsh, Lt, N, V)) return rc;
   if (!transfer || !phi || !T) return fail(CITRUS_ERR_ARG, "NULL buffer");
-  const int64_t total = (int64_t)V * N * nsh * 4;
-  const int64_t blocks = (total + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(scatter_moments_kernel<false>, dim3((unsigned)blocks), dim3(kThreads), 0,
-                     reinterpret_cast<hipStream_t>(stream), transfer, nullptr, 0, Lt, phi, T, G, nsh, N, total, 0);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("scatter moments: ") + hipGetErrorString(e));
-  return CITRUS_OK;
+  return launch<false>(transfer, nullptr, 0, Lt, phi, T, G, nsh, N, V, 0, stream);
 }
 
 extern "C" int citrus_scatter_moments_factored(const float* rho, int num_materials, const float* sigma, int Lt,
                                                int downscatter_only, const float* phi, float* T, int G, int nsh,
                                                int64_t N, int V, void* stream) {
   using namespace citrus;
-  if (int rc = check(G, nsh, Lt, N, V)) return rc;
   if (!rho || !sigma || !phi || !T) return fail(CITRUS_ERR_ARG, "NULL buffer");
   if (num_materials < 1 || num_materials > kMaxMaterials) return fail(CITRUS_ERR_ARG, "1..8 materials supported");
-  const size_t lds = sizeof(float) * (size_t)num_materials * G * G * (Lt + 1);
-  if (lds > 64 * 1024) return fail(CITRUS_ERR_UNSUPPORTED, "transfer table exceeds 64 KiB of LDS");
-  const int64_t total = (int64_t)V * N * nsh * 4;
-  const int64_t blocks = (total + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(scatter_moments_kernel<true>, dim3((unsigned)blocks), dim3(kThreads), lds,
-                     reinterpret_cast<hipStream_t>(stream), sigma, rho, num_materials, Lt, phi, T, G, nsh, N, total,
-                     downscatter_only ? 1 : 0);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("scatter moments: ") + hipGetErrorString(e));
-  return CITRUS_OK;
+  return launch<true>(sigma, rho, num_materials, Lt, phi, T, G, nsh, N, V, downscatter_only ? 1 : 0, stream);
 }

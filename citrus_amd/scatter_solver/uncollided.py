@@ -79,6 +79,8 @@ def uncollided_fluence(library: MultigroupLibrary, rho: Tensor, grid: Grid, geom
     point_grid = grid if points is None else points
     present = present_materials(rho) if isinstance(materials, str) else (None if materials is None
                                                                         else [int(m) for m in materials])
+    if present is not None and not present:  # vacuum: trace one all-zero map (zero path, no attenuation)
+        present = [0]
     traced = rho if present is None else rho.index_select(0, torch.tensor(present, device=rho.device))
     paths = uncollided_path_lengths(projectors, traced.contiguous(), grid, geometry, angle, point_grid)
     return uncollided_from_path_lengths(library, paths, present, point_grid, geometry, angle, source)

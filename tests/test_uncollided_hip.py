@@ -96,5 +96,8 @@ def test_uncollided_with_profile_filter_offfocal_matches_oracle(curved):
                                       np.float64),
                                   filter_density=2.7, filter_mu=fmu.numpy(), margin_ch=0.3, margin_row=0.1,
                                   off_dist=np.array([0.0, 1.0, 3.0]), off_frac=np.array([0.05, 0.03, 0.01]))
-    assert (ref > 0).mean() < 0.999  # the field edge and off-focal tail are exercised
+    d = centers - orays.source_position(100.0, angle)
+    fwd, u, v = ounc.panel_uv(d, 100.0, 150.0, angle, curved)
+    fw = ounc.field_weight(u, v, 10, 12, 0.6, 0.8, 0.3, 0.1, np.array([0.0, 1.0, 3.0]), np.array([0.05, 0.03, 0.01]))
+    assert (fw == 1).any() and ((fw < 1) & (fw > 0)).any()  # inside the field and in the off-focal tail
     assert rel_err(phi.reshape(8, -1), ref) < F32_REL
