@@ -22,6 +22,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -369,6 +370,325 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// Matrix-core variant for the moment-source sweep (T given, no explicit source), B = 4, NSH <= 36, dc <= 32.
+//
+// One wave sweeps one (view, octant, group, block) task. Both contractions of a cell plane run on
+// v_mfma_f32_16x16x4_f32 in full FP32, four cells (16 rows: cell x LD component) per M-tile:
+//   source expansion  S[(cell,k)][d] = sum_n T[(cell,k)][n] Y[d][n]        (K = SH moments, N = directions)
+//   moment reduction  phi[(cell,k)][n] += sum_d psi[(cell,k)][d] Yw[d][n]  (K = directions, N = SH moments)
+// Operand layouts of the 16x16x4 f32 instruction (lane l): A[i = l%16][k = l/16], B[k = l/16][j = l%16],
+// D[i = 4(l/16) + r][j = l%16] in result register r. So an A fragment of T is one scalar load from the cell-major
+// moments, the expansion leaves all four LD components of one (cell, direction) in one lane for the closed-form solve,
+// and the reduction leaves phi[cell][n][0..3], one float4 of the moment array, in one lane. Only the angular fluence
+// changes layout between the two (through a 2.2 KB LDS tile). The whole task is one wave, so cell planes need no
+// barrier: LDS accesses of one wave execute in program order.
+// ---------------------------------------------------------------------------------------------------------------------
+
+constexpr int kMfmaPsiStride = 136;
+// Buffer-resource word 3 of gfx9-family raw buffers; an access at kDropOffset or beyond num_records is discarded.
+constexpr int kBufferWord3 = 0x00020000;
+constexpr int kDropOffset = 0x7ffffff0;
+typedef unsigned u4 __attribute__((ext_vector_type(4)));  // floats per cell row of the psi tile ([d][k], 32 directions + 8): == 8 mod 32
+
+__device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+template <int NSH, int NT>
+__global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
+  constexpr int B = 4, BB = 16;
+  constexpr PlaneTable<B> ptab{};
+  constexpr int NPL = PlaneTable<B>::kPlanes;
+  constexpr int KS = (NSH + 3) / 4;     // k-steps of the expansion
+  constexpr int NT2 = (NSH + 15) / 16;  // moment tiles of the reduction
+  constexpr int KSD = NT * 4;           // k-steps of the reduction (directions, 4 per step)
+  constexpr int PS = kMfmaPsiStride;
+  const PlaneTable<B>& tab = planes<B>();
+
+  extern __shared__ float lds[];
+  const int dc = a.dc;
+  const int face_stride = dc * BB * 3;                       // floats per axis: [3 components][BB positions][dc]
+  float* face = lds;                                         // [3 axes] of the above
+  float* psl = face + 3 * face_stride;                       // [4 cells][PS]
+  int* slot_cell = reinterpret_cast<int*>(psl + 4 * PS);     // [64] flat cell of each slot, or -1
+  int* slot_pos = slot_cell + 64;                            // [64] local (x, y, z) of each slot, packed
+
+  const int lane = threadIdx.x;
+  const int q = lane >> 4, r16 = lane & 15;
+  const int acell = r16 >> 2, acomp = r16 & 3;  // A-fragment row: (cell of the M-tile, LD component)
+  const int task = blockIdx.x % a.ntask;
+  const int rest = blockIdx.x / a.ntask;
+  const int g = rest % a.G;
+  const int v = rest / a.G;
+  const int4 e = a.tasks[task];
+  const int o = e.x & 7;
+  const bool first_visit = a.first && ((e.x >> 3) & 1);
+  const int slot = (e.x >> 4) & 1;
+  const int bi = e.y, bj = e.z, bk = e.w;
+  const int sx = (o & 4) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 1) ? -1 : 1;
+  const float fsx = (float)sx, fsy = (float)sy, fsz = (float)sz;
+  const int blk = (bi * a.nbc + bj) * a.nbs + bk;
+  const int x0 = bj * B, y0 = bi * B, z0 = bk * B;
+
+  float* gface_v = a.faces + (int64_t)v * a.fview + ((int64_t)slot * a.G + g) * a.nblocks * 3 * face_stride;
+  const float* mu_g = a.mu + (int64_t)g * a.N;
+  const float* T_g = a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
+  float* phi_g = a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
+  const __amdgpu_buffer_rsrc_t phi_rsrc = __builtin_amdgcn_make_buffer_rsrc(phi_g, 0, a.N * NSH * 16, kBufferWord3);
+
+  // ---- slot tables, incoming faces (requested first), direction fragments -----------------------------------------
+  {
+    const int packed = tab.cell[lane];
+    const int pa = packed & 0xff, pb = (packed >> 8) & 0xff, pc = (packed >> 16) & 0xff;
+    const int lx = sx > 0 ? pa : B - 1 - pa;
+    const int ly = sy > 0 ? pb : B - 1 - pb;
+    const int lz = sz > 0 ? pc : B - 1 - pc;
+    const int x = x0 + lx, yy = y0 + ly, z = z0 + lz;
+    slot_cell[lane] = (x >= a.C || yy >= a.R || z >= a.S) ? -1 : (yy * a.C + x) * a.S + z;
+    slot_pos[lane] = lx | (ly << 8) | (lz << 16);
+  }
+  // incoming faces: every load is issued before the first is used (a block without an upstream neighbour along an
+  // axis reads its own, valid, face entries and stores zeros instead)
+  constexpr int FPL = 3 * NT;  // float4 per lane and axis: face_stride / 4 = 12 dc <= 192 NT
+  const int n4 = face_stride / 4;
+  {
+    const int ub[3][3] = {{bi, bj - sx, bk}, {bi - sy, bj, bk}, {bi, bj, bk - sz}};
+    f4 fin[3][FPL];
+    bool inside[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const int ui = ub[ax][0], uj = ub[ax][1], uk = ub[ax][2];
+      inside[ax] = ui >= 0 && ui < a.nbr && uj >= 0 && uj < a.nbc && uk >= 0 && uk < a.nbs;
+      const int ublk = inside[ax] ? (ui * a.nbc + uj) * a.nbs + uk : blk;
+      const f4* srcf = reinterpret_cast<const f4*>(gface_v + ((int64_t)ublk * 3 + ax) * face_stride);
+#pragma unroll
+      for (int r = 0; r < FPL; ++r) {
+        const int i = lane + 64 * r;
+        fin[ax][r] = srcf[i < n4 ? i : n4 - 1];
+      }
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      f4* dst = reinterpret_cast<f4*>(face + ax * face_stride);
+#pragma unroll
+      for (int r = 0; r < FPL; ++r) {
+        const int i = lane + 64 * r;
+        if (i < n4) dst[i] = inside[ax] ? fin[ax][r] : f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  }
+  const int mbase = o * a.D8 + a.d0;
+  float yf[NT][KS];  // expansion B fragments: Y[d = 16 nt + l%16][n = 4 s + l/16]
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int d = nt * 16 + r16;
+    const int dd = d < dc ? d : dc - 1;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int n = 4 * s + q;
+      const float val = a.Y[(int64_t)(mbase + dd) * NSH + (n < NSH ? n : NSH - 1)];
+      yf[nt][s] = (d < dc && n < NSH) ? val : 0.0f;
+    }
+  }
+  float ywf[NT2][KSD];  // reduction B fragments: Yw[d = 4 s + l/16][n = 16 t + l%16]
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) {
+    const int n = t * 16 + r16;
+#pragma unroll
+    for (int s = 0; s < KSD; ++s) {
+      const int d = 4 * s + q;
+      const float val = a.Yw[(int64_t)(mbase + (d < dc ? d : dc - 1)) * NSH + (n < NSH ? n : NSH - 1)];
+      ywf[t][s] = (d < dc && n < NSH) ? val : 0.0f;
+    }
+  }
+  float cxs[NT], cys[NT], czs[NT];  // direction cosines of the D-layout direction d = 16 nt + l%16
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int d = nt * 16 + r16;
+    const int m = mbase + (d < dc ? d : dc - 1);
+    cxs[nt] = a.cdir[m * 3 + 0];
+    cys[nt] = a.cdir[m * 3 + 1];
+    czs[nt] = a.cdir[m * 3 + 2];
+  }
+
+  // ---- cell planes --------------------------------------------------------------------------------------------
+  // Loads are unconditional (addresses clamped, unused values discarded) and the plane loop is fully unrolled, so the
+  // T fragments and mu_t of plane p+1 and the old moments of plane p stay in flight while plane p is computed.
+  struct Staged {
+    float t[3][KS];
+    float mu[3];
+  };
+  auto load_plane = [&](const int pl) __attribute__((always_inline)) -> Staged {
+    Staged st;
+    const int pb = ptab.offset[pl];
+    const int P = ptab.offset[pl + 1] - pb;
+#pragma unroll
+    for (int mt = 0; mt < 3; ++mt) {
+      if (mt * 4 >= P) break;
+      const int sa = 4 * mt + acell;
+      const int ca = slot_cell[pb + (sa < P ? sa : P - 1)];
+      const float* tp = T_g + (int64_t)(ca >= 0 ? ca : 0) * NSH * 4 + acomp;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int n = 4 * s + q;
+        st.t[mt][s] = tp[(n < NSH ? n : NSH - 1) * 4];
+      }
+      const int sd = 4 * mt + q;
+      const int cd = slot_cell[pb + (sd < P ? sd : P - 1)];
+      st.mu[mt] = mu_g[cd >= 0 ? cd : 0];
+    }
+    return st;
+  };
+
+  auto plane = [&](const int p, const Staged cur) __attribute__((always_inline)) -> Staged {
+    const int pb = ptab.offset[p];
+    const int P = ptab.offset[p + 1] - pb;
+    const Staged next = load_plane(p + 1 < NPL ? p + 1 : NPL - 1);
+    f4 pold[3][NT2];
+#pragma unroll
+    for (int mt = 0; mt < 3; ++mt) {
+      if (mt * 4 >= P) break;
+      const int sd = 4 * mt + q;
+      const int cd = slot_cell[pb + (sd < P ? sd : P - 1)];
+      const f4* src = reinterpret_cast<const f4*>(phi_g + (int64_t)(cd >= 0 ? cd : 0) * NSH * 4);
+#pragma unroll
+      for (int t = 0; t < NT2; ++t) {
+        const int n = t * 16 + r16;
+        pold[mt][t] = src[n < NSH ? n : NSH - 1];
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 3; ++mt) {
+      if (mt * 4 >= P) break;
+      // (1) source expansion
+      f4 sacc[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        sacc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) sacc[nt] = mfma4(cur.t[mt][s], yf[nt][s], sacc[nt]);
+      }
+      // (2) closed-form LD solve of (cell 4 mt + l/16, direction 16 nt + l%16)
+      const int sd = 4 * mt + q;
+      const int slot_d = pb + (sd < P ? sd : P - 1);
+      const bool cvalid = sd < P && slot_cell[slot_d] >= 0;
+      const int packed = slot_pos[slot_d];
+      const int lx = packed & 0xff, ly = (packed >> 8) & 0xff, lz = (packed >> 16) & 0xff;
+      const float mu = cur.mu[mt];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int d = nt * 16 + r16;
+        const bool valid = cvalid && d < dc;
+        const int dd = d < dc ? d : dc - 1;
+        const float cx = cxs[nt], cy = cys[nt], cz = czs[nt];
+        // face arrays are [component][position][direction]; component 0 is the face average psi_0 + s_a psi_a,
+        // 1 and 2 the transverse moments
+        const int cs = BB * dc;
+        float* fx = face + 0 * face_stride + (ly * B + lz) * dc + dd;
+        float* fy = face + 1 * face_stride + (lx * B + lz) * dc + dd;
+        float* fz = face + 2 * face_stride + (lx * B + ly) * dc + dd;
+        const float pbx = fx[0], xy = fx[cs], xz = fx[2 * cs];
+        const float pby = fy[0], yx = fy[cs], yz = fy[2 * cs];
+        const float pbz = fz[0], zx = fz[cs], zy = fz[2 * cs];
+        const f4 sv = sacc[nt];
+        const float r0 = sv.x + (cx * pbx + cy * pby + cz * pbz);
+        const float rx = sv.y - 3.0f * fsx * cx * pbx + (cy * yx + cz * zx);
+        const float ry = sv.z - 3.0f * fsy * cy * pby + (cx * xy + cz * zy);
+        const float rz = sv.w - 3.0f * fsz * cz * pbz + (cx * xz + cy * yz);
+        const float csum = cx + cy + cz;
+        // hardware reciprocals (1 ulp) in place of IEEE divisions: four per solve instead of seven divisions
+        const float idx = __builtin_amdgcn_rcpf(3.0f * cx + (csum - cx) + mu);
+        const float idy = __builtin_amdgcn_rcpf(3.0f * cy + (csum - cy) + mu);
+        const float idz = __builtin_amdgcn_rcpf(3.0f * cz + (csum - cz) + mu);
+        const float qx = cx * idx, qy = cy * idy, qz = cz * idz;
+        const float iden = __builtin_amdgcn_rcpf(csum + mu + 3.0f * (cx * qx + cy * qy + cz * qz));
+        const float p0 = (r0 - (fsx * qx * rx + fsy * qy * ry + fsz * qz * rz)) * iden;
+        const float px = (rx + 3.0f * fsx * cx * p0) * idx;
+        const float py = (ry + 3.0f * fsy * cy * p0) * idy;
+        const float pz = (rz + 3.0f * fsz * cz * p0) * idz;
+        if (valid) {
+          fx[0] = p0 + fsx * px; fx[cs] = py; fx[2 * cs] = pz;
+          fy[0] = p0 + fsy * py; fy[cs] = px; fy[2 * cs] = pz;
+          fz[0] = p0 + fsz * pz; fz[cs] = px; fz[2 * cs] = py;
+        }
+        // padding rows and directions contribute exact zeros to the reduction
+        const f4 psi = valid ? f4{p0, px, py, pz} : f4{0.f, 0.f, 0.f, 0.f};
+        *reinterpret_cast<f4*>(psl + q * PS + d * 4) = psi;
+      }
+      // (3) moment reduction over the chunk's directions, added into phi
+      f4 facc[NT2];
+#pragma unroll
+      for (int t = 0; t < NT2; ++t) facc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < KSD; ++s) {
+        if (4 * s >= dc) break;  // wave-uniform
+        const float av = psl[acell * PS + (4 * s + q) * 4 + acomp];
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) facc[t] = mfma4(av, ywf[t][s], facc[t]);
+      }
+      // buffer stores: rows of padding cells and moments n >= NSH get an out-of-range offset and are dropped by the
+      // hardware, so no branch (and no wait for the loads in flight) is needed
+      const int cbyte = slot_cell[slot_d] * (NSH * 16);
+#pragma unroll
+      for (int t = 0; t < NT2; ++t) {
+        const int n = t * 16 + r16;
+        const f4 val = first_visit ? facc[t] : facc[t] + pold[mt][t];
+        const int off = (cvalid && n < NSH) ? cbyte + n * 16 : kDropOffset;
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, val), phi_rsrc, off, 0, 0);
+      }
+    }
+    return next;
+  };
+
+  Staged cur = load_plane(0);
+#pragma unroll
+  for (int p = 0; p < NPL; ++p) cur = plane(p, cur);
+
+  // ---- outgoing faces for the downstream blocks ---------------------------------------------------------------
+  {
+    const int db[3][3] = {{bi, bj + sx, bk}, {bi + sy, bj, bk}, {bi, bj, bk + sz}};
+    for (int ax = 0; ax < 3; ++ax) {
+      const int di = db[ax][0], dj = db[ax][1], dk = db[ax][2];
+      if (!(di >= 0 && di < a.nbr && dj >= 0 && dj < a.nbc && dk >= 0 && dk < a.nbs)) continue;
+      f4* dst = reinterpret_cast<f4*>(gface_v + ((int64_t)blk * 3 + ax) * face_stride);
+      const f4* srcf = reinterpret_cast<const f4*>(face + ax * face_stride);
+      for (int i = lane; i < n4; i += 64) dst[i] = srcf[i];
+    }
+  }
+}
+
+size_t mfma_lds_bytes(int dc) { return sizeof(float) * (size_t)(9 * dc * 16 + 4 * kMfmaPsiStride + 128); }
+
+template <int NSH>
+hipError_t launch_mfma_nsh(const LaunchArgs& la, int nwg, hipStream_t stream) {
+  const size_t lds = mfma_lds_bytes(la.dc);
+  if (la.dc <= 16)
+    hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 1>), dim3(nwg), dim3(64), lds, stream, la);
+  else
+    hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 2>), dim3(nwg), dim3(64), lds, stream, la);
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma(int nsh, const LaunchArgs& la, int nwg, hipStream_t stream) {
+  switch (nsh) {
+    case 1: return launch_mfma_nsh<1>(la, nwg, stream);
+    case 4: return launch_mfma_nsh<4>(la, nwg, stream);
+    case 9: return launch_mfma_nsh<9>(la, nwg, stream);
+    case 16: return launch_mfma_nsh<16>(la, nwg, stream);
+    case 25: return launch_mfma_nsh<25>(la, nwg, stream);
+    case 36: return launch_mfma_nsh<36>(la, nwg, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// The matrix-core kernel serves the moment-source sweep unless CITRUS_SWEEP_KERNEL=vector selects the vector kernel
+// (both give the same results to FP32 rounding; the switch exists to compare them).
+bool use_mfma(const LaunchArgs& la, int nsh) {
+  const char* sel = std::getenv("CITRUS_SWEEP_KERNEL");
+  const bool vector_only = sel && std::string(sel) == "vector";
+  return !vector_only && la.T && !la.src && nsh <= 36 && la.dc <= 32 && !la.write_psi &&
+         (int64_t)la.N * nsh * 16 < kDropOffset;  // moments of one (view, group) addressable by a buffer resource
+}
+
 template <int B>
 constexpr int plane_max() {
   return PlaneTable<B>().max_plane();
@@ -533,7 +853,7 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
     la.first = (!A.accumulate && plan->launch_pass[l] == 0) ? 1 : 0;
     const int64_t nwg = (int64_t)la.ntask * A.G * A.V;
     if (nwg > 0x7fffffff) return fail(CITRUS_ERR_ARG, "too many workgroups in one launch");
-    hipError_t e = launch_nsh<4>(A.nsh, la, (int)nwg, s);
+    hipError_t e = use_mfma(la, A.nsh) ? launch_mfma(A.nsh, la, (int)nwg, s) : launch_nsh<4>(A.nsh, la, (int)nwg, s);
     if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
     if (e == hipErrorInvalidConfiguration) return fail(CITRUS_ERR_UNSUPPORTED, "chunk needs more than 64 KiB of LDS");
     if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));

@@ -148,9 +148,13 @@ def test_source_called_once_per_chunk_and_cell_major(chunk):
         assert torch.equal(index, ctx.direction_indices(d0, min(d0 + dc, 4)))
 
 
-@pytest.mark.parametrize("shape,G,order,L,chunk", [((5, 7, 6), 3, 8, 3, None), ((9, 4, 6), 2, 12, 5, 7)])
-def test_fused_moment_source_matches_oracle(shape, G, order, L, chunk):
-    """Source expanded from SH moments inside the kernel vs the oracle's explicit expansion."""
+@pytest.mark.parametrize("kernel", ["mfma", "vector"])
+@pytest.mark.parametrize("shape,G,order,L,chunk", [((5, 7, 6), 3, 8, 3, None), ((9, 4, 6), 2, 12, 5, 7),
+                                                   ((8, 12, 7), 2, 12, 5, None), ((4, 5, 9), 2, 4, 1, 2)])
+def test_fused_moment_source_matches_oracle(shape, G, order, L, chunk, kernel, monkeypatch):
+    """Source expanded from SH moments inside the kernel vs the oracle's explicit expansion, on the matrix-core
+    kernel (ragged blocks, 10 and 21 directions per octant, chunks of 7 and 2) and on the vector kernel."""
+    monkeypatch.setenv("CITRUS_SWEEP_KERNEL", kernel)
     dev = gpu_device()
     grid = Grid(shape, 0.5)
     q = level_symmetric_quadrature(order)
