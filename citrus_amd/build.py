@@ -21,9 +21,11 @@ LIBRARY = PACKAGE / "libcitrus_hip.so"
 ARCH = "gfx950"
 
 # Packed FP32 instructions are disabled for every device pass (the host pass warns that it does not know the feature).
+# Matrix-core accumulators live in ordinary VGPRs (no copies through the accumulation registers).
 COMMON_FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-    "-packed-fp32-ops", "-Wno-unused-command-line-argument", f"-I{ROOT / 'include'}", f"-I{CSRC}",
+    "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form", "-Wno-unused-command-line-argument",
+    f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
 
 

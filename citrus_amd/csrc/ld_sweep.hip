@@ -383,34 +383,55 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
 // and the reduction leaves phi[cell][n][0..3], one float4 of the moment array, in one lane. Only the angular fluence
 // changes layout between the two (through a 2.2 KB LDS tile). The whole task is one wave, so cell planes need no
 // barrier: LDS accesses of one wave execute in program order.
+//
+// The kernel body has no branches: moments are read and written through buffer resources, whose out-of-range
+// accesses return zero or are dropped, and lanes without a cell or direction send their face updates to a private
+// scratch row of LDS. The fully unrolled plane loop is then a single block the compiler can schedule across, so the
+// matrix-core chains of one M-tile overlap the solve of another and the loads of the next plane.
+// Face arrays (LDS and the inter-block buffer) are [axis][position][component][direction].
 // ---------------------------------------------------------------------------------------------------------------------
 
-constexpr int kMfmaPsiStride = 136;
-// Buffer-resource word 3 of gfx9-family raw buffers; an access at kDropOffset or beyond num_records is discarded.
+constexpr int kMfmaPsiStride = 136;  // floats per cell row of the psi tile ([d][k], 32 directions + 8): == 8 mod 32
+// Buffer-resource word 3 of gfx9-family raw buffers; accesses at kDrop or beyond num_records (< kDrop) read zero and
+// write nothing.
 constexpr int kBufferWord3 = 0x00020000;
-constexpr int kDropOffset = 0x7ffffff0;
-typedef unsigned u4 __attribute__((ext_vector_type(4)));  // floats per cell row of the psi tile ([d][k], 32 directions + 8): == 8 mod 32
+constexpr unsigned kDrop = 0x80000000u;
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
-template <int NSH, int NT>
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+__device__ __forceinline__ f4 buf_load4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+
+__device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, unsigned off, f4 v) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
+}
+
+template <int NSH, int DC4>
 __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
-  constexpr int B = 4, BB = 16;
+  constexpr int B = 4;
   constexpr PlaneTable<B> ptab{};
   constexpr int NPL = PlaneTable<B>::kPlanes;
+  constexpr int NT = (DC4 + 3) / 4;     // direction tiles of the expansion
   constexpr int KS = (NSH + 3) / 4;     // k-steps of the expansion
   constexpr int NT2 = (NSH + 15) / 16;  // moment tiles of the reduction
-  constexpr int KSD = NT * 4;           // k-steps of the reduction (directions, 4 per step)
   constexpr int PS = kMfmaPsiStride;
+  constexpr unsigned CB = NSH * 16;     // bytes of one cell's moments
   const PlaneTable<B>& tab = planes<B>();
 
   extern __shared__ float lds[];
   const int dc = a.dc;
-  const int face_stride = dc * BB * 3;                       // floats per axis: [3 components][BB positions][dc]
+  const int face_stride = 48 * dc;                           // floats per axis: [16 positions][3 components][dc]
   float* face = lds;                                         // [3 axes] of the above
   float* psl = face + 3 * face_stride;                       // [4 cells][PS]
   int* slot_cell = reinterpret_cast<int*>(psl + 4 * PS);     // [64] flat cell of each slot, or -1
   int* slot_pos = slot_cell + 64;                            // [64] local (x, y, z) of each slot, packed
+  float* scratch_row = reinterpret_cast<float*>(slot_pos + 64);  // [2 dc + 64] face updates of padding lanes
 
   const int lane = threadIdx.x;
   const int q = lane >> 4, r16 = lane & 15;
@@ -430,12 +451,14 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   const int x0 = bj * B, y0 = bi * B, z0 = bk * B;
 
   float* gface_v = a.faces + (int64_t)v * a.fview + ((int64_t)slot * a.G + g) * a.nblocks * 3 * face_stride;
-  const float* mu_g = a.mu + (int64_t)g * a.N;
-  const float* T_g = a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
-  float* phi_g = a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
-  const __amdgpu_buffer_rsrc_t phi_rsrc = __builtin_amdgcn_make_buffer_rsrc(phi_g, 0, a.N * NSH * 16, kBufferWord3);
+  const __amdgpu_buffer_rsrc_t mu_r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.mu + (int64_t)g * a.N), 0, a.N * 4, kBufferWord3);
+  const __amdgpu_buffer_rsrc_t T_r = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4), 0, a.N * (int)CB, kBufferWord3);
+  const __amdgpu_buffer_rsrc_t phi_r = __builtin_amdgcn_make_buffer_rsrc(
+      a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4, 0, a.N * (int)CB, kBufferWord3);
 
-  // ---- slot tables, incoming faces (requested first), direction fragments -----------------------------------------
+  // ---- slot tables, incoming faces (all requested before the first is used), direction fragments ------------------
   {
     const int packed = tab.cell[lane];
     const int pa = packed & 0xff, pb = (packed >> 8) & 0xff, pc = (packed >> 16) & 0xff;
@@ -446,11 +469,10 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
     slot_cell[lane] = (x >= a.C || yy >= a.R || z >= a.S) ? -1 : (yy * a.C + x) * a.S + z;
     slot_pos[lane] = lx | (ly << 8) | (lz << 16);
   }
-  // incoming faces: every load is issued before the first is used (a block without an upstream neighbour along an
-  // axis reads its own, valid, face entries and stores zeros instead)
   constexpr int FPL = 3 * NT;  // float4 per lane and axis: face_stride / 4 = 12 dc <= 192 NT
   const int n4 = face_stride / 4;
   {
+    // a block without an upstream neighbour along an axis reads its own (valid) entries and stores zeros instead
     const int ub[3][3] = {{bi, bj - sx, bk}, {bi - sy, bj, bk}, {bi, bj, bk - sz}};
     f4 fin[3][FPL];
     bool inside[3];
@@ -489,12 +511,12 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       yf[nt][s] = (d < dc && n < NSH) ? val : 0.0f;
     }
   }
-  float ywf[NT2][KSD];  // reduction B fragments: Yw[d = 4 s + l/16][n = 16 t + l%16]
+  float ywf[NT2][DC4];  // reduction B fragments: Yw[d = 4 s + l/16][n = 16 t + l%16]
 #pragma unroll
   for (int t = 0; t < NT2; ++t) {
     const int n = t * 16 + r16;
 #pragma unroll
-    for (int s = 0; s < KSD; ++s) {
+    for (int s = 0; s < DC4; ++s) {
       const int d = 4 * s + q;
       const float val = a.Yw[(int64_t)(mbase + (d < dc ? d : dc - 1)) * NSH + (n < NSH ? n : NSH - 1)];
       ywf[t][s] = (d < dc && n < NSH) ? val : 0.0f;
@@ -511,8 +533,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   }
 
   // ---- cell planes --------------------------------------------------------------------------------------------
-  // Loads are unconditional (addresses clamped, unused values discarded) and the plane loop is fully unrolled, so the
-  // T fragments and mu_t of plane p+1 and the old moments of plane p stay in flight while plane p is computed.
+  // The T fragments and mu_t of plane p+1 and the old moments of plane p are in flight while plane p is computed.
   struct Staged {
     float t[3][KS];
     float mu[3];
@@ -525,16 +546,16 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
     for (int mt = 0; mt < 3; ++mt) {
       if (mt * 4 >= P) break;
       const int sa = 4 * mt + acell;
-      const int ca = slot_cell[pb + (sa < P ? sa : P - 1)];
-      const float* tp = T_g + (int64_t)(ca >= 0 ? ca : 0) * NSH * 4 + acomp;
+      const int ca = sa < P ? slot_cell[pb + (sa < P ? sa : P - 1)] : -1;
+      const unsigned ta = ca >= 0 ? (unsigned)ca * CB + 4u * acomp : kDrop;
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
         const int n = 4 * s + q;
-        st.t[mt][s] = tp[(n < NSH ? n : NSH - 1) * 4];
+        st.t[mt][s] = buf_load(T_r, n < NSH ? ta + 16u * n : kDrop);
       }
       const int sd = 4 * mt + q;
-      const int cd = slot_cell[pb + (sd < P ? sd : P - 1)];
-      st.mu[mt] = mu_g[cd >= 0 ? cd : 0];
+      const int cd = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
+      st.mu[mt] = buf_load(mu_r, cd >= 0 ? 4u * cd : kDrop);
     }
     return st;
   };
@@ -542,18 +563,20 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   auto plane = [&](const int p, const Staged cur) __attribute__((always_inline)) -> Staged {
     const int pb = ptab.offset[p];
     const int P = ptab.offset[p + 1] - pb;
-    const Staged next = load_plane(p + 1 < NPL ? p + 1 : NPL - 1);
-    f4 pold[3][NT2];
+    Staged next = cur;
+    if (p + 1 < NPL) next = load_plane(p + 1);
+    f4 pold[3][NT2];  // moments this plane adds to (zero on first visits: those loads are dropped)
+    int cells[3];
 #pragma unroll
     for (int mt = 0; mt < 3; ++mt) {
       if (mt * 4 >= P) break;
       const int sd = 4 * mt + q;
-      const int cd = slot_cell[pb + (sd < P ? sd : P - 1)];
-      const f4* src = reinterpret_cast<const f4*>(phi_g + (int64_t)(cd >= 0 ? cd : 0) * NSH * 4);
+      cells[mt] = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
 #pragma unroll
       for (int t = 0; t < NT2; ++t) {
         const int n = t * 16 + r16;
-        pold[mt][t] = src[n < NSH ? n : NSH - 1];
+        const bool use = !first_visit && cells[mt] >= 0 && n < NSH;
+        pold[mt][t] = buf_load4(phi_r, use ? (unsigned)cells[mt] * CB + 16u * n : kDrop);
       }
     }
 #pragma unroll
@@ -569,26 +592,23 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       }
       // (2) closed-form LD solve of (cell 4 mt + l/16, direction 16 nt + l%16)
       const int sd = 4 * mt + q;
-      const int slot_d = pb + (sd < P ? sd : P - 1);
-      const bool cvalid = sd < P && slot_cell[slot_d] >= 0;
-      const int packed = slot_pos[slot_d];
+      const bool cvalid = cells[mt] >= 0;
+      const int packed = slot_pos[pb + (sd < P ? sd : P - 1)];
       const int lx = packed & 0xff, ly = (packed >> 8) & 0xff, lz = (packed >> 16) & 0xff;
       const float mu = cur.mu[mt];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
         const int d = nt * 16 + r16;
         const bool valid = cvalid && d < dc;
-        const int dd = d < dc ? d : dc - 1;
+        // component c of a face entry sits at +c*dc; component 0 is the face average psi_0 + s_a psi_a, 1 and 2 the
+        // transverse moments
+        float* fx = valid ? face + 0 * face_stride + (ly * B + lz) * 3 * dc + d : scratch_row + lane;
+        float* fy = valid ? face + 1 * face_stride + (lx * B + lz) * 3 * dc + d : scratch_row + lane;
+        float* fz = valid ? face + 2 * face_stride + (lx * B + ly) * 3 * dc + d : scratch_row + lane;
+        const float pbx = fx[0], xy = fx[dc], xz = fx[2 * dc];
+        const float pby = fy[0], yx = fy[dc], yz = fy[2 * dc];
+        const float pbz = fz[0], zx = fz[dc], zy = fz[2 * dc];
         const float cx = cxs[nt], cy = cys[nt], cz = czs[nt];
-        // face arrays are [component][position][direction]; component 0 is the face average psi_0 + s_a psi_a,
-        // 1 and 2 the transverse moments
-        const int cs = BB * dc;
-        float* fx = face + 0 * face_stride + (ly * B + lz) * dc + dd;
-        float* fy = face + 1 * face_stride + (lx * B + lz) * dc + dd;
-        float* fz = face + 2 * face_stride + (lx * B + ly) * dc + dd;
-        const float pbx = fx[0], xy = fx[cs], xz = fx[2 * cs];
-        const float pby = fy[0], yx = fy[cs], yz = fy[2 * cs];
-        const float pbz = fz[0], zx = fz[cs], zy = fz[2 * cs];
         const f4 sv = sacc[nt];
         const float r0 = sv.x + (cx * pbx + cy * pby + cz * pbz);
         const float rx = sv.y - 3.0f * fsx * cx * pbx + (cy * yx + cz * zx);
@@ -605,11 +625,9 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
         const float px = (rx + 3.0f * fsx * cx * p0) * idx;
         const float py = (ry + 3.0f * fsy * cy * p0) * idy;
         const float pz = (rz + 3.0f * fsz * cz * p0) * idz;
-        if (valid) {
-          fx[0] = p0 + fsx * px; fx[cs] = py; fx[2 * cs] = pz;
-          fy[0] = p0 + fsy * py; fy[cs] = px; fy[2 * cs] = pz;
-          fz[0] = p0 + fsz * pz; fz[cs] = px; fz[2 * cs] = py;
-        }
+        fx[0] = p0 + fsx * px; fx[dc] = py; fx[2 * dc] = pz;
+        fy[0] = p0 + fsy * py; fy[dc] = px; fy[2 * dc] = pz;
+        fz[0] = p0 + fsz * pz; fz[dc] = px; fz[2 * dc] = py;
         // padding rows and directions contribute exact zeros to the reduction
         const f4 psi = valid ? f4{p0, px, py, pz} : f4{0.f, 0.f, 0.f, 0.f};
         *reinterpret_cast<f4*>(psl + q * PS + d * 4) = psi;
@@ -619,21 +637,15 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
 #pragma unroll
       for (int t = 0; t < NT2; ++t) facc[t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < KSD; ++s) {
-        if (4 * s >= dc) break;  // wave-uniform
+      for (int s = 0; s < DC4; ++s) {
         const float av = psl[acell * PS + (4 * s + q) * 4 + acomp];
 #pragma unroll
         for (int t = 0; t < NT2; ++t) facc[t] = mfma4(av, ywf[t][s], facc[t]);
       }
-      // buffer stores: rows of padding cells and moments n >= NSH get an out-of-range offset and are dropped by the
-      // hardware, so no branch (and no wait for the loads in flight) is needed
-      const int cbyte = slot_cell[slot_d] * (NSH * 16);
 #pragma unroll
       for (int t = 0; t < NT2; ++t) {
         const int n = t * 16 + r16;
-        const f4 val = first_visit ? facc[t] : facc[t] + pold[mt][t];
-        const int off = (cvalid && n < NSH) ? cbyte + n * 16 : kDropOffset;
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, val), phi_rsrc, off, 0, 0);
+        buf_store4(phi_r, (cvalid && n < NSH) ? (unsigned)cells[mt] * CB + 16u * n : kDrop, facc[t] + pold[mt][t]);
       }
     }
     return next;
@@ -656,15 +668,23 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   }
 }
 
-size_t mfma_lds_bytes(int dc) { return sizeof(float) * (size_t)(9 * dc * 16 + 4 * kMfmaPsiStride + 128); }
+size_t mfma_lds_bytes(int dc) { return sizeof(float) * (size_t)(144 * dc + 4 * kMfmaPsiStride + 128 + 2 * dc + 64); }
 
 template <int NSH>
 hipError_t launch_mfma_nsh(const LaunchArgs& la, int nwg, hipStream_t stream) {
   const size_t lds = mfma_lds_bytes(la.dc);
-  if (la.dc <= 16)
-    hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 1>), dim3(nwg), dim3(64), lds, stream, la);
-  else
-    hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 2>), dim3(nwg), dim3(64), lds, stream, la);
+  const dim3 grid(nwg), block(64);
+  switch ((la.dc + 3) / 4) {
+    case 1: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 1>), grid, block, lds, stream, la); break;
+    case 2: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 2>), grid, block, lds, stream, la); break;
+    case 3: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 3>), grid, block, lds, stream, la); break;
+    case 4: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 4>), grid, block, lds, stream, la); break;
+    case 5: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 5>), grid, block, lds, stream, la); break;
+    case 6: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 6>), grid, block, lds, stream, la); break;
+    case 7: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 7>), grid, block, lds, stream, la); break;
+    case 8: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 8>), grid, block, lds, stream, la); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -681,12 +701,13 @@ hipError_t launch_mfma(int nsh, const LaunchArgs& la, int nwg, hipStream_t strea
 }
 
 // The matrix-core kernel serves the moment-source sweep unless CITRUS_SWEEP_KERNEL=vector selects the vector kernel
-// (both give the same results to FP32 rounding; the switch exists to compare them).
+// (both give the same results to FP32 rounding; the switch exists to compare them). Its face-buffer layout differs
+// from the vector kernel's, so the choice is made once per sweep.
 bool use_mfma(const LaunchArgs& la, int nsh) {
   const char* sel = std::getenv("CITRUS_SWEEP_KERNEL");
   const bool vector_only = sel && std::string(sel) == "vector";
   return !vector_only && la.T && !la.src && nsh <= 36 && la.dc <= 32 && !la.write_psi &&
-         (int64_t)la.N * nsh * 16 < kDropOffset;  // moments of one (view, group) addressable by a buffer resource
+         (int64_t)la.N * nsh * 16 < (int64_t)kDrop;  // moments of one (view, group) addressable by a buffer resource
 }
 
 template <int B>
@@ -847,13 +868,15 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   la.sview = (int64_t)8 * A.dc * N * A.G * 4;
   la.fview = (int64_t)2 * A.G * plan->nblocks * 3 * (int64_t)A.dc * plan->B * plan->B * 3;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  la.ntask = plan->launches.empty() ? 0 : plan->launches[0].second;
+  const bool mfma = use_mfma(la, A.nsh);
   for (size_t l = 0; l < plan->launches.size(); ++l) {
     la.tasks = plan->d_tasks + plan->launches[l].first;
     la.ntask = plan->launches[l].second;
     la.first = (!A.accumulate && plan->launch_pass[l] == 0) ? 1 : 0;
     const int64_t nwg = (int64_t)la.ntask * A.G * A.V;
     if (nwg > 0x7fffffff) return fail(CITRUS_ERR_ARG, "too many workgroups in one launch");
-    hipError_t e = use_mfma(la, A.nsh) ? launch_mfma(A.nsh, la, (int)nwg, s) : launch_nsh<4>(A.nsh, la, (int)nwg, s);
+    hipError_t e = mfma ? launch_mfma(A.nsh, la, (int)nwg, s) : launch_nsh<4>(A.nsh, la, (int)nwg, s);
     if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
     if (e == hipErrorInvalidConfiguration) return fail(CITRUS_ERR_UNSUPPORTED, "chunk needs more than 64 KiB of LDS");
     if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
