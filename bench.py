@@ -86,25 +86,56 @@ def cpu_baseline(args, ctx, library, rho, coarse):
     Y = oq.real_spherical_harmonics(5, om)
     rho_np = rho.numpy().astype(np.float32)
     mu_t = np.einsum("mg,mijk->gijk", library.mu_t.numpy(), rho_np).astype(np.float32)
-    mu_s = np.einsum("mabl,mijk->ablijk", library.mu_s_l.numpy(), rho_np).astype(np.float32)
+    R = coarse.shape[0]
+    rows = max(1, R // 8)  # the scattering-moment pass is timed on the first R/8 cell rows and scaled by R/rows
+    mu_s = np.einsum("mabl,mijk->ablijk", library.mu_s_l.numpy(), rho_np[:, :rows]).astype(np.float32)
     rng = np.random.default_rng(7)
     phi = (rng.standard_normal((G, nsh, 4) + coarse.shape) * 0.05).astype(np.float32)
     phi[:, 0, 0] = np.abs(phi[:, 0, 0]) + 1.0
+    from threadpoolctl import threadpool_limits
+
+    limiter = threadpool_limits(limits=1)  # one host thread for every numpy kernel: cores = 1
     t0 = time.perf_counter()
-    T = scatter_moments(mu_s, phi).astype(np.float32)  # whole-view work: counted once per view
-    t_moments = time.perf_counter() - t0
+    T_rows = scatter_moments(mu_s, phi[:, :, :, :rows]).astype(np.float32)
+    t_moments = (time.perf_counter() - t0) * R / rows
+    T = np.concatenate([T_rows] * (R // rows) + [T_rows[:, :, :, : R % rows]], axis=3)  # sample: cell rows repeated
     t0 = time.perf_counter()
     src = np.tensordot(Y[sel].astype(np.float32), T, axes=([1], [1])).astype(np.float32)  # [Msel, G, 4, R, C, S]
     psi = osw.sweep(coarse.shape, coarse.deltas, om[sel], mu_t, src, dtype=np.float32)
     out = np.tensordot((w[sel, None] * Y[sel]).astype(np.float32), psi, axes=([0], [0]))
     t_dirs = time.perf_counter() - t0
+    limiter.unregister()
     M = ctx.num_directions
     per_view = t_moments + t_dirs * M / sel.size
     del out
     return {"value": 1.0 / per_view, "unit": "view-sweeps/s", "cores": 1, "kind": "port",
-            "sample": f"{sel.size} of {M} directions x {G} groups x {coarse.shape[0]}^3 cells "
-                      f"(source expansion + LD sweep + moment reduction, numpy float32) plus one full scattering-"
-                      f"moment pass; per-view time = moments {t_moments:.1f} s + {t_dirs:.1f} s x {M}/{sel.size}"}
+            "sample": f"{sel.size} of {M} directions x {G} groups x {R}^3 cells (source expansion + LD sweep + "
+                      f"moment reduction, numpy float32, one thread) plus the scattering-moment pass on {rows} of {R} "
+                      f"cell rows; per-view time = moments {t_moments:.1f} s (scaled) + {t_dirs:.1f} s x "
+                      f"{M}/{sel.size}"}
+
+
+def sweep_launches(grid_cells: int, block: int = 4) -> int:
+    """Launches of one view-sweep: K + 1 block planes per octant pair (K = sum of blocks - 1 over the axes), the
+    middle plane split in two when K is even, four octant pairs."""
+    K = 3 * ((grid_cells + block - 1) // block - 1)
+    return 4 * (K + 1 + (1 if K % 2 == 0 else 0))
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this workload, or None.
+
+    The summary lists FETCH_SIZE and WRITE_SIZE per dispatch scaled by the same counters' reading of a plain copy of
+    known size (its `traffic_bytes_per_launch` line)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ld_sweep_mfma_64cube_s12_p5_g16.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        return None
+    return None
 
 
 def main():
@@ -154,10 +185,13 @@ def main():
     V = args.views
     G, nsh = 16, ctx.num_moments
     N = ctx.num_cells
-    # compulsory bytes of one view-sweep: moments T in and phi out (float32) plus mu_t
+    # compulsory bytes of one view-sweep in the sweep kernel: moments T in and phi out (float32) plus mu_t; the
+    # launches of a sweep split it by block planes, so bytes per launch / launch time averages to the same ratio
     alg_bytes = 2 * G * N * nsh * 4 * 4 + G * N * 4
+    launches = sweep_launches(coarse.shape[0])
     achieved = alg_bytes * V / (sweep_ms * 1e-3) / 1e9
-    launches = 4 * 47  # block planes of 4 octant pairs at 64^3 / 4^3 blocks (46 + one split per pair)
+    kernel = f"ld_sweep_mfma_kernel<{nsh}, {(ctx.directions_per_octant + 3) // 4}>"
+    traffic = profiled_traffic(kernel)
     result = {
         "metric": METRIC,
         "value": V * args.steps / (total_ms * 1e-3),
@@ -176,9 +210,11 @@ def main():
                    "directions": ctx.num_directions, "moments": nsh, "views_per_launch": V,
                    "sweep_ms_per_view": sweep_ms / V, "wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ld_sweep_kernel<36,4>", "alg_bytes_per_view_sweep": alg_bytes,
-                     "launches_per_view_sweep": launches},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kernel, "alg_bytes_per_view_sweep": alg_bytes,
+                     "alg_bytes_per_launch": alg_bytes * V / launches,
+                     "avg_launch_ms": sweep_ms / launches, "launches_per_view_sweep": launches,
+                     "traffic_source": "profiles/ld_sweep_mfma_64cube_s12_p5_g16.txt" if traffic else None},
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args, ctx, library, rho, coarse)
