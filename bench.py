@@ -122,8 +122,9 @@ def sweep_launches(grid_cells: int, block: int = 4) -> int:
     return 4 * (K + 1 + (1 if K % 2 == 0 else 0))
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this workload, or None.
+def profiled_traffic(kernel: str, views: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this workload, or None (also
+    when the summary was taken at another number of views per launch).
 
     The summary lists FETCH_SIZE and WRITE_SIZE per dispatch scaled by the same counters' reading of a plain copy of
     known size (its `traffic_bytes_per_launch` line)."""
@@ -131,7 +132,7 @@ def profiled_traffic(kernel: str):
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                if line.startswith("traffic_bytes_per_launch") and kernel in line and f"{views} views" in line:
                     return float(line.split("=")[1].split()[0])
     except OSError:
         return None
@@ -191,7 +192,7 @@ def main():
     launches = sweep_launches(coarse.shape[0])
     achieved = alg_bytes * V / (sweep_ms * 1e-3) / 1e9
     kernel = f"ld_sweep_mfma_kernel<{nsh}, {(ctx.directions_per_octant + 3) // 4}>"
-    traffic = profiled_traffic(kernel)
+    traffic = profiled_traffic(kernel, V)
     result = {
         "metric": METRIC,
         "value": V * args.steps / (total_ms * 1e-3),

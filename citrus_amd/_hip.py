@@ -36,6 +36,19 @@ class SweepArgs(ctypes.Structure):
     ]
 
 
+class FcArgs(ctypes.Structure):
+    """Mirror of ``CitrusFcArgs`` (include/citrus_hip.h)."""
+
+    _fields_ = [
+        ("M", c_int), ("G", c_int), ("n_eta", c_int), ("n_map", c_int),
+        ("table", c_void), ("eta_map", c_void),
+        ("n_slots", c_int), ("slot_material", c_void), ("slot_density", c_void), ("phi_uc", c_void),
+        ("R", c_int), ("C", c_int), ("S", c_int),
+        ("lattice", ctypes.c_double * 6), ("focal", ctypes.c_double * 3), ("parallel", c_int),
+        ("omegas", c_void), ("K", c_int),
+    ]
+
+
 _SIGNATURES = {
     "citrus_last_error": (ctypes.c_char_p, []),
     "citrus_abi_version": (c_int, []),
@@ -62,6 +75,9 @@ _SIGNATURES = {
                                           c_void, c_void]),
     "citrus_uncollided_fluence": (c_int, [c_void, c_int, c_void, c_void, c_void, c_void, c_void, c_int, c_int,
                                           c_void, c_void, c_void, c_void, c_void, c_void, c_void, c_void]),
+    "citrus_fc_source": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
+    "citrus_fc_sums": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void]),
+    "citrus_fc_moments": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_int, c_void, c_void]),
 }
 
 

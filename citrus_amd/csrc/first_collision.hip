@@ -1,0 +1,386 @@
+// First-collision source at the sweep directions, its sampling-correction sums and its LD cell moments (gfx950).
+//
+//   S_FC,g(r, w) = sum_{g' <= g} sum_m rho_m(r) mu_s,eta^{g'->g}(Omega_UC(r) . w) phi_UC,g'(r)
+//
+// (citrus/scatter_solver/uncollided.py:200-270, the transfer table interpolated linearly in eta at the position of
+// cross_sections.py:905-945, uniform or through an adaptive map). Materials enter per point through up to four slots
+// (material index, density), so a point costs what its nonzero materials cost. The table is node-major
+// [n_eta][M][G][GP] (GP = 4, 8, 16 or 32, the kernel's group count; entries below the diagonal zeroed by the host): one point's interpolation
+// at one eta reads two contiguous node rows as float4, which neighbouring points share in L1.
+//
+// Three kernels share the evaluation:
+//   source   thread per (point, direction) -> out [K][G][P]                               (uncollided.py:200)
+//   sums     thread per point, all directions -> out [5][G][P] = I0, I1, I2 (sum_k w_k eta^j S_k, FP32 in direction
+//            order) and the exact moments E0, E1 (uncollided.py:357-384, :538-608; uncollided_kernels.py:637)
+//   moments  thread per (cell, direction): corrected samples S scale (1 + beta eta) at the f^3 sub-points of a cell,
+//            restricted to LD moments mean and mean(S u_a)/mean(u_a^2) (uncollided.py:311-352, :420-433, :464-535),
+//            written [K][G][4][cells], cell-major [K][cells][G][4] or group-major [G][cells][K][4].
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <type_traits>
+
+#include "citrus_hip.h"
+#include "common.h"
+
+namespace citrus {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int kMaxSlots = 4;
+constexpr int kBlock = 256;
+
+struct FcDev {
+  int M, G, GP, n_eta, n_map, n_slots;
+  const float* table;
+  const float* eta_map;
+  const int* slot_material;
+  const float* slot_density;
+  const float* phi_uc;
+  int R, C, S, P;
+  double x0, y0, z0, dx, dy, dz;
+  double fx, fy, fz;
+  int parallel;
+  const float* omegas;
+  int K;
+};
+
+// padded g_out width of the table rows: the kernel's compile-time group count
+int group_pad(int G) { return G <= 4 ? 4 : G <= 8 ? 8 : G <= 16 ? 16 : 32; }
+
+FcDev make_dev(const CitrusFcArgs& a) {
+  FcDev d{};
+  d.M = a.M; d.G = a.G; d.GP = group_pad(a.G); d.n_eta = a.n_eta; d.n_map = a.eta_map ? a.n_map : 0;
+  d.n_slots = a.n_slots; d.table = a.table; d.eta_map = a.eta_map; d.slot_material = a.slot_material;
+  d.slot_density = a.slot_density; d.phi_uc = a.phi_uc; d.R = a.R; d.C = a.C; d.S = a.S; d.P = a.R * a.C * a.S;
+  d.x0 = a.lattice[0]; d.y0 = a.lattice[1]; d.z0 = a.lattice[2];
+  d.dx = a.lattice[3]; d.dy = a.lattice[4]; d.dz = a.lattice[5];
+  d.fx = a.focal[0]; d.fy = a.focal[1]; d.fz = a.focal[2]; d.parallel = a.parallel;
+  d.omegas = a.omegas; d.K = a.K;
+  return d;
+}
+
+// Omega_UC of point p = (r, c, s): (x - focal)/|x - focal| formed in FP64, used in FP32
+__device__ __forceinline__ void primary_direction(const FcDev& d, int r, int c, int s, float& ux, float& uy, float& uz) {
+  if (d.parallel) {
+    ux = (float)d.fx; uy = (float)d.fy; uz = (float)d.fz;
+    return;
+  }
+  const double px = d.x0 + c * d.dx - d.fx, py = d.y0 + r * d.dy - d.fy, pz = d.z0 + s * d.dz - d.fz;
+  const double inv = 1.0 / sqrt(px * px + py * py + pz * pz);
+  ux = (float)(px * inv); uy = (float)(py * inv); uz = (float)(pz * inv);
+}
+
+// lower node and weight of the upper node (cross_sections.py:905-945), in FP32 like the reference's working dtype
+__device__ __forceinline__ void table_position(const FcDev& d, float eta, int& i0, float& w1) {
+  eta = fminf(fmaxf(eta, -1.0f), 1.0f);
+  const int n = d.n_eta;
+  float pos;
+  if (d.n_map == 0) {
+    pos = (eta + 1.0f) / 2.0f * (float)(n - 1);
+  } else {
+    const int K = d.n_map;
+    const float q = sqrtf(fmaxf((1.0f - eta) * 0.5f, 0.0f));
+    const float p = q * (float)(K - 1);
+    const float k0f = fminf(fmaxf(floorf(p), 0.0f), (float)(K - 2));
+    const float a = fminf(fmaxf(p - k0f, 0.0f), 1.0f);
+    const int k0 = (int)k0f;
+    const float u0 = d.eta_map[k0];
+    pos = (u0 + (d.eta_map[k0 + 1] - u0) * a) * (float)(n - 1);
+  }
+  int i = (int)floorf(pos);
+  i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+  i0 = i;
+  w1 = fminf(fmaxf(pos - (float)i, 0.0f), 1.0f);
+}
+
+// per-point data kept in registers: phi_UC of every group; the slots (material, density) are read per use (L1)
+template <int GMAX>
+struct PointData {
+  int p;
+  float ph[GMAX];
+};
+
+template <int GMAX>
+__device__ __forceinline__ void load_point(const FcDev& d, int p, PointData<GMAX>& pd) {
+  pd.p = p;
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) pd.ph[g] = g < d.G ? d.phi_uc[(int64_t)g * d.P + p] : 0.0f;
+}
+
+// S[go] at one eta; entries go < g' of the table are zero (downscatter only)
+template <int GMAX>
+__device__ __forceinline__ void eval_source(const FcDev& d, const PointData<GMAX>& pd, float eta, float S[GMAX]) {
+  int i0;
+  float w1;
+  table_position(d, eta, i0, w1);
+  constexpr int NCH = GMAX / 4;
+  f4 acc[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) acc[ch] = f4{0.f, 0.f, 0.f, 0.f};
+  // rows are GMAX wide (the host pads g_out to the kernel's group count); rows g' >= G are clamped and carry phi 0
+  const int64_t node = (int64_t)d.M * d.G * GMAX;
+  for (int s = 0; s < d.n_slots; ++s) {
+    const float rho = d.slot_density[(int64_t)s * d.P + pd.p];
+    const float* t0 = d.table + (int64_t)i0 * node + (int64_t)d.slot_material[(int64_t)s * d.P + pd.p] * d.G * GMAX;
+#pragma unroll
+    for (int gi = 0; gi < GMAX; ++gi) {
+      const float cv = rho * pd.ph[gi];
+      const int row = gi < d.G ? gi : d.G - 1;
+      const f4* r0 = reinterpret_cast<const f4*>(t0 + row * GMAX);
+      const f4* r1 = reinterpret_cast<const f4*>(t0 + node + row * GMAX);
+#pragma unroll
+      for (int ch = gi / 4; ch < NCH; ++ch) {
+        const f4 a = r0[ch];
+        const f4 b = r1[ch];
+        acc[ch] += cv * (a + w1 * (b - a));
+      }
+    }
+  }
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    S[4 * ch + 0] = acc[ch].x;
+    S[4 * ch + 1] = acc[ch].y;
+    S[4 * ch + 2] = acc[ch].z;
+    S[4 * ch + 3] = acc[ch].w;
+  }
+}
+
+template <int GMAX>
+__global__ __launch_bounds__(kBlock) void fc_source_kernel(FcDev d, float* __restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (int64_t)d.P * d.K) return;
+  const int p = (int)(idx % d.P);
+  const int k = (int)(idx / d.P);
+  const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
+  float ux, uy, uz;
+  primary_direction(d, r, c, s, ux, uy, uz);
+  PointData<GMAX> pd;
+  load_point(d, p, pd);
+  const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
+  float S[GMAX];
+  eval_source(d, pd, eta, S);
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g)
+    if (g < d.G) out[((int64_t)k * d.G + g) * d.P + p] = S[g];
+}
+
+template <int GMAX>
+__global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* __restrict__ weights,
+                                                         const float* __restrict__ mu_s_l, int lp1,
+                                                         float* __restrict__ out) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= d.P) return;
+  const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
+  float ux, uy, uz;
+  primary_direction(d, r, c, s, ux, uy, uz);
+  PointData<GMAX> pd;
+  load_point(d, p, pd);
+  float I0[GMAX], I1[GMAX], I2[GMAX];
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) I0[g] = I1[g] = I2[g] = 0.0f;
+  for (int k = 0; k < d.K; ++k) {
+    const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
+    float S[GMAX];
+    eval_source(d, pd, eta, S);
+    const float w = weights[k];
+    const float we = w * eta;
+    const float wee = we * eta;
+#pragma unroll
+    for (int g = 0; g < GMAX; ++g) {
+      I0[g] = fmaf(w, S[g], I0[g]);
+      I1[g] = fmaf(we, S[g], I1[g]);
+      I2[g] = fmaf(wee, S[g], I2[g]);
+    }
+  }
+  // exact Legendre moments l = 0, 1 of the same source (no quadrature)
+  float E0[GMAX], E1[GMAX];
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) E0[g] = E1[g] = 0.0f;
+  for (int sl = 0; sl < d.n_slots; ++sl) {
+    const float rho = d.slot_density[(int64_t)sl * d.P + p];
+    const float* ms = mu_s_l + (int64_t)d.slot_material[(int64_t)sl * d.P + p] * d.G * d.G * lp1;
+#pragma unroll
+    for (int gi = 0; gi < GMAX; ++gi) {
+      const float cv = rho * pd.ph[gi];  // 0 for g' >= G
+#pragma unroll
+      for (int go = gi; go < GMAX; ++go) {
+        const int a = (gi < d.G ? gi : d.G - 1) * d.G + (go < d.G ? go : d.G - 1);
+        E0[go] = fmaf(cv, ms[a * lp1 + 0], E0[go]);
+        E1[go] = fmaf(cv, ms[a * lp1 + 1], E1[go]);
+      }
+    }
+  }
+  const int64_t plane = (int64_t)d.G * d.P;
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) {
+    if (g >= d.G) continue;
+    const int64_t o = (int64_t)g * d.P + p;
+    out[o] = I0[g];
+    out[plane + o] = I1[g];
+    out[2 * plane + o] = I2[g];
+    out[3 * plane + o] = E0[g];
+    out[4 * plane + o] = E1[g];
+  }
+}
+
+template <int GMAX>
+__global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float* __restrict__ scale,
+                                                            const float* __restrict__ beta, int f, int layout,
+                                                            float* __restrict__ out) {
+  const int Rc = d.R / f, Cc = d.C / f, Sc = d.S / f;
+  const int Nc = Rc * Cc * Sc;
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (int64_t)Nc * d.K) return;
+  // group-major output runs directions fastest, the other layouts cells fastest (coalesced stores either way)
+  const int cell = layout == 2 ? (int)(idx / d.K) : (int)(idx % Nc);
+  const int k = layout == 2 ? (int)(idx % d.K) : (int)(idx / Nc);
+  const int rc = cell / (Cc * Sc), cc = (cell / Sc) % Cc, sc = cell % Sc;
+  const float ox = d.omegas[3 * k], oy = d.omegas[3 * k + 1], oz = d.omegas[3 * k + 2];
+  f4 acc[GMAX];  // per group: sum of v, v u_x, v u_y, v u_z
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) acc[g] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int jr = 0; jr < f; ++jr)
+    for (int jc = 0; jc < f; ++jc)
+      for (int js = 0; js < f; ++js) {
+        const int r = rc * f + jr, c = cc * f + jc, s = sc * f + js;
+        const int p = (r * d.C + c) * d.S + s;
+        float ux, uy, uz;
+        primary_direction(d, r, c, s, ux, uy, uz);
+        PointData<GMAX> pd;
+        load_point(d, p, pd);
+        const float eta = ux * ox + uy * oy + uz * oz;
+        float S[GMAX];
+        eval_source(d, pd, eta, S);
+        // local coordinates u_j = (2j + 1)/f - 1 (x along cols, y along rows, z along slices)
+        const float u_x = (2.0f * jc + 1.0f) / f - 1.0f;
+        const float u_y = (2.0f * jr + 1.0f) / f - 1.0f;
+        const float u_z = (2.0f * js + 1.0f) / f - 1.0f;
+        const bool corrected = scale != nullptr;
+#pragma unroll
+        for (int g = 0; g < GMAX; ++g) {
+          const int gg = g < d.G ? g : d.G - 1;  // groups past G carry S = 0; their loads are clamped
+          float v = S[g];
+          const float sc = corrected ? scale[(int64_t)gg * d.P + p] : 1.0f;
+          const float be = corrected ? beta[(int64_t)gg * d.P + p] : 0.0f;
+          v = v * sc * (1.0f + be * eta);
+          acc[g] += v * f4{1.0f, u_x, u_y, u_z};
+        }
+      }
+  const float n3 = (float)(f * f * f);
+  float slope = 0.0f;  // 1 / (n3 mean(u^2)) along one axis, 0 for f = 1
+  if (f > 1) {
+    float u2 = 0.0f;
+    for (int j = 0; j < f; ++j) {
+      const float u = (2.0f * j + 1.0f) / f - 1.0f;
+      u2 += u * u;
+    }
+    slope = 1.0f / (n3 * (u2 / f));
+  }
+  // element (g, moment j) sits at base + g gstride + j jstride
+  int64_t base, gstride, jstride;
+  if (layout == 0) {
+    base = (int64_t)k * d.G * 4 * Nc + cell; gstride = 4 * (int64_t)Nc; jstride = Nc;
+  } else if (layout == 1) {
+    base = ((int64_t)k * Nc + cell) * d.G * 4; gstride = 4; jstride = 1;
+  } else {
+    base = ((int64_t)cell * d.K + k) * 4; gstride = (int64_t)Nc * d.K * 4; jstride = 1;
+  }
+#pragma unroll
+  for (int g = 0; g < GMAX; ++g) {
+    if (g >= d.G) continue;
+    const f4 m = f4{acc[g].x / n3, acc[g].y * slope, acc[g].z * slope, acc[g].w * slope};
+    float* o = out + base + g * gstride;
+    if (jstride == 1) {
+      *reinterpret_cast<f4*>(o) = m;
+    } else {
+      o[0] = m.x; o[jstride] = m.y; o[2 * jstride] = m.z; o[3 * jstride] = m.w;
+    }
+  }
+}
+
+int check_args(const CitrusFcArgs* a) {
+  if (!a) return fail(CITRUS_ERR_ARG, "args is NULL");
+  if (a->G < 1 || a->G > 32) return fail(CITRUS_ERR_ARG, "1..32 groups supported");
+  if (a->M < 1 || a->n_eta < 2 || a->K < 0) return fail(CITRUS_ERR_ARG, "M >= 1, n_eta >= 2 and K >= 0 required");
+  if (a->n_slots < 1 || a->n_slots > kMaxSlots) return fail(CITRUS_ERR_ARG, "1..4 material slots per point supported");
+  if (a->eta_map && a->n_map < 2) return fail(CITRUS_ERR_ARG, "an adaptive map needs at least 2 knots");
+  if (a->R < 1 || a->C < 1 || a->S < 1 || (int64_t)a->R * a->C * a->S > 0x7fffffff)
+    return fail(CITRUS_ERR_ARG, "point grid must be non-empty and below 2^31 points");
+  if (!a->table || !a->slot_material || !a->slot_density || !a->phi_uc || (a->K > 0 && !a->omegas))
+    return fail(CITRUS_ERR_ARG, "NULL table or buffer");
+  return CITRUS_OK;
+}
+
+template <typename Launch>
+int dispatch_groups(int G, Launch&& launch) {
+  if (G <= 4) return launch(std::integral_constant<int, 4>{});
+  if (G <= 8) return launch(std::integral_constant<int, 8>{});
+  if (G <= 16) return launch(std::integral_constant<int, 16>{});
+  return launch(std::integral_constant<int, 32>{});
+}
+
+int finish(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return CITRUS_OK;
+}
+
+}  // namespace
+}  // namespace citrus
+
+extern "C" int citrus_fc_source(const CitrusFcArgs* args, float* out, void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!out) return fail(CITRUS_ERR_ARG, "NULL output");
+  const FcDev d = make_dev(*args);
+  const int64_t n = (int64_t)d.P * d.K;
+  if (n == 0) return CITRUS_OK;
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dispatch_groups(d.G, [&](auto gm) {
+    hipLaunchKernelGGL((fc_source_kernel<decltype(gm)::value>), grid, block, 0, s, d, out);
+    return 0;
+  });
+  return finish("first-collision source");
+}
+
+extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1,
+                              float* out, void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!weights || !mu_s_l || !out) return fail(CITRUS_ERR_ARG, "NULL weights, moments or output");
+  if (lp1 < 2) return fail(CITRUS_ERR_ARG, "the exact moments need Legendre order >= 1");
+  const FcDev d = make_dev(*args);
+  const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock)), block(kBlock);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dispatch_groups(d.G, [&](auto gm) {
+    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), grid, block, 0, s, d, weights, mu_s_l, lp1, out);
+    return 0;
+  });
+  return finish("first-collision sums");
+}
+
+extern "C" int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float* beta, int factor,
+                                 int layout, float* out, void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!out) return fail(CITRUS_ERR_ARG, "NULL output");
+  if ((scale == nullptr) != (beta == nullptr)) return fail(CITRUS_ERR_ARG, "scale and beta must be given together");
+  if (factor < 1 || args->R % factor || args->C % factor || args->S % factor)
+    return fail(CITRUS_ERR_ARG, "the point grid must be divisible by the factor");
+  if (layout < 0 || layout > 2) return fail(CITRUS_ERR_ARG, "layout must be 0, 1 or 2");
+  const FcDev d = make_dev(*args);
+  const int64_t n = (int64_t)(d.P / (factor * factor * factor)) * d.K;
+  if (n == 0) return CITRUS_OK;
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dispatch_groups(d.G, [&](auto gm) {
+    hipLaunchKernelGGL((fc_moments_kernel<decltype(gm)::value>), grid, block, 0, s, d, scale, beta, factor, layout,
+                       out);
+    return 0;
+  });
+  return finish("first-collision moments");
+}

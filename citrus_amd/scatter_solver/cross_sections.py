@@ -34,6 +34,7 @@ class MultigroupLibrary:
     fine_mu_t: Tensor          # [M, E] cm^2/g
     fine_weight: Tensor        # [E] spectrum weight of each fine energy
     fine_group: Tensor         # [E] int64 group of each fine energy
+    eta_map: Optional[Tensor] = None  # [K] node map of an adaptive angular table, None for a uniform one
 
     @property
     def num_groups(self) -> int:
@@ -64,14 +65,47 @@ def _fine_tables(edges: Tensor, mu_t: Tensor, n: int, spread: float):
     return fraction, fine_energy, fine_mu_t, fine_weight, fine_group
 
 
+def synthetic_angular_map(num_knots: int = 512) -> Tensor:
+    """[K] adaptive node placement dense near eta = 1 (cross_sections.py:1025-1040):
+    ``u(q) = 1 - 0.6 log(1 + q/0.02)/log(51) - 0.4 q^2`` at ``q_k = k/(K-1)``, strictly decreasing from 1 to 0."""
+    q = torch.arange(num_knots, dtype=torch.float64) / (num_knots - 1)
+    u = 1.0 - 0.6 * torch.log1p(q / 0.02) / math.log(51.0) - 0.4 * q * q
+    u[0], u[-1] = 1.0, 0.0
+    return u
+
+
+def angular_table_nodes(num_nodes: int, eta_map: Optional[Tensor] = None) -> Tensor:
+    """[N] float64 node cosines, ascending from -1 to 1 (cross_sections.py:948-973): equally spaced, or node j at the
+    map coordinate ``u = j/(N-1)``, ``eta_j = 1 - 2 q_j^2`` with q_j from the inverse of the piecewise-linear map."""
+    n = int(num_nodes)
+    if eta_map is None:
+        return torch.linspace(-1.0, 1.0, n, dtype=torch.float64)
+    uk = eta_map.detach().to("cpu", torch.float64)
+    K = uk.numel()
+    if not bool((uk[1:] < uk[:-1]).all()):
+        raise ValueError("eta_map must decrease strictly from the first knot (eta = 1) to the last (eta = -1)")
+    qk = torch.arange(K, dtype=torch.float64) / (K - 1)
+    au, aq = uk.flip(0), qk.flip(0)
+    u = torch.arange(n, dtype=torch.float64) / (n - 1)
+    j = torch.searchsorted(au, u, right=True).clamp(1, K - 1)
+    q = aq[j - 1] + (aq[j] - aq[j - 1]) * (u - au[j - 1]) / (au[j] - au[j - 1])
+    eta = 1.0 - 2.0 * q * q
+    eta[0], eta[-1] = -1.0, 1.0
+    return eta
+
+
 def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int = 401, num_fine_per_group: int = 4,
-                      fine_spread: float = 0.0) -> MultigroupLibrary:
+                      fine_spread: float = 0.0, angular_table_grid: str = "uniform") -> MultigroupLibrary:
     """The reference's two-material closed-form test library (cross_sections.py:1043-1100): kernel
     ``a (1 + b eta) / 4 pi`` with ``a = a_m 0.6^(go-gi)`` (a_m = 0.15, 0.12), ``b = 0.5`` in-group else 0.2, photo-
-    electric ``0.05 / 0.2 * 1.5^g``; its Legendre moments are exactly ``a`` and ``a b / 3``."""
+    electric ``0.05 / 0.2 * 1.5^g``; its Legendre moments are exactly ``a`` and ``a b / 3``. ``angular_table_grid``
+    "adaptive" places the table nodes by ``synthetic_angular_map``."""
     G, L = num_groups, legendre_order
     edges = torch.linspace(125.0, 10.0, G + 1, dtype=torch.float64)
-    eta = torch.linspace(-1.0, 1.0, num_eta, dtype=torch.float64)
+    if angular_table_grid not in ("uniform", "adaptive"):
+        raise ValueError(f"angular_table_grid must be 'uniform' or 'adaptive', got {angular_table_grid!r}")
+    eta_map = synthetic_angular_map() if angular_table_grid == "adaptive" else None
+    eta = angular_table_nodes(num_eta, eta_map)
     mu_s_eta = torch.zeros(2, G, G, num_eta, dtype=torch.float64)
     mu_s_l = torch.zeros(2, G, G, L + 1, dtype=torch.float64)
     for m, am in enumerate((0.15, 0.12)):
@@ -87,7 +121,7 @@ def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int
                          dtype=torch.float64)
     mu_t = mu_pe + mu_s_l[..., 0].sum(dim=2)
     return MultigroupLibrary(edges, mu_t, mu_s_l, eta, mu_s_eta, ("water", "bone_cortical_icrp"),
-                             *_fine_tables(edges, mu_t, num_fine_per_group, fine_spread))
+                             *_fine_tables(edges, mu_t, num_fine_per_group, fine_spread), eta_map=eta_map)
 
 
 def seeded_library(num_groups: int, legendre_order: int, e_max_kev: float, e_min_kev: float, seed: int,

@@ -9,8 +9,11 @@ fused launch (``citrus_uncollided_fluence``).
 
 from __future__ import annotations
 
+import ctypes
+from dataclasses import dataclass
 from typing import Optional, Sequence, Union
 
+import numpy as np
 import torch
 from torch import Tensor
 
@@ -54,8 +57,6 @@ def uncollided_from_path_lengths(library: MultigroupLibrary, path_lengths: Tenso
     values, ints, table = source.host_block(geometry, angle)
     x0, y0, z0 = points.lower_corner()
     dx, dy, dz = points.deltas
-    import numpy as np
-
     lattice = np.array([x0 + 0.5 * dx, y0 + 0.5 * dy, z0 + 0.5 * dz, dx, dy, dz], dtype=np.float64)
     shape = np.array(points.shape, dtype=np.int32)
     G = library.num_groups
@@ -84,3 +85,259 @@ def uncollided_fluence(library: MultigroupLibrary, rho: Tensor, grid: Grid, geom
     traced = rho if present is None else rho.index_select(0, torch.tensor(present, device=rho.device))
     paths = uncollided_path_lengths(projectors, traced.contiguous(), grid, geometry, angle, point_grid)
     return uncollided_from_path_lengths(library, paths, present, point_grid, geometry, angle, source)
+
+
+# ==================================================================================================================
+# first-collision source at the sweep directions (uncollided.py:200-656 of the reference)
+# ==================================================================================================================
+
+def primary_directions_to_voxels(grid: Grid, geometry: ConeGeometry, angle: float, device=None,
+                                 dtype=torch.float32) -> Tensor:
+    """[rows, cols, slices, 3] unit vectors ``(r - r0)/|r - r0|`` from the focal spot to the voxel centres
+    (uncollided.py:74-96), formed in float64."""
+    r = grid.center_coordinates(device, torch.float64)
+    r0 = torch.tensor(geometry.source_position(angle), device=device, dtype=torch.float64)
+    d = r - r0
+    return (d / torch.linalg.norm(d, dim=-1, keepdim=True)).to(dtype)
+
+
+_FC_TABLES: dict = {}
+
+
+def _fc_tables(library: MultigroupLibrary, device: torch.device):
+    """Device tables of the first-collision kernels, cached per library and device: the node-major, downscatter-masked
+    eta table [n_eta][M][G][GP] (GP from the kernel's group count), mu_s_l [M][G][G][L+1] and the adaptive map."""
+    key = (id(library), str(device))
+    hit = _FC_TABLES.get(key)
+    if hit is not None and hit[0] is library:
+        return hit[1]
+    M, G = library.num_materials, library.num_groups
+    gp = 4 if G <= 4 else 8 if G <= 8 else 16 if G <= 16 else 32
+    down = torch.triu(torch.ones(G, G, dtype=torch.float64))  # [g_in, g_out], 1 where g_out >= g_in
+    masked = library.mu_s_eta * down[None, :, :, None]  # [M, Gi, Go, N]
+    table = torch.zeros(library.mu_s_eta.shape[-1], M, G, gp, dtype=torch.float64)
+    table[..., :G] = masked.permute(3, 0, 1, 2)
+    tables = (table.to(device, torch.float32).contiguous(),
+              library.mu_s_l.to(device, torch.float32).contiguous(),
+              None if library.eta_map is None else library.eta_map.to(device, torch.float32).contiguous())
+    _FC_TABLES[key] = (library, tables)
+    return tables
+
+
+def _material_slots(rho: Tensor):
+    """(slot material int32 [n][P], slot density float32 [n][P]) with n = the largest count of nonzero materials at
+    one point (at least 1, at most 4): every material when M <= 4, else each point's nonzero ones (the reference's
+    MaterialPairs, cross_sections.py:656, generalised)."""
+    M = int(rho.shape[0])
+    flat = rho.reshape(M, -1).to(torch.float32)
+    if M <= 4:
+        mats = torch.arange(M, device=rho.device, dtype=torch.int32)[:, None].expand(M, flat.shape[1]).contiguous()
+        return mats, flat.contiguous()
+    n = int((flat != 0).sum(0).max())
+    if n > 4:
+        raise ValueError(f"{n} nonzero materials at one point; the first-collision kernels take at most 4")
+    n = max(n, 1)
+    dens, idx = torch.topk(flat.abs(), n, dim=0)
+    dens = torch.gather(flat, 0, idx)
+    return idx.to(torch.int32).contiguous(), dens.contiguous()
+
+
+def _fc_args(library, rho, phi_uncollided, grid: Grid, geometry, angle, omegas):
+    device = phi_uncollided.device
+    _hip.require_gpu_tensor(phi_uncollided, "phi_uncollided")
+    if tuple(phi_uncollided.shape[1:]) != tuple(grid.shape) or tuple(rho.shape[1:]) != tuple(grid.shape):
+        raise ValueError(f"rho {tuple(rho.shape)} and phi_uncollided {tuple(phi_uncollided.shape)} must lie on "
+                         f"the grid {grid.shape}")
+    if int(phi_uncollided.shape[0]) != library.num_groups or int(rho.shape[0]) != library.num_materials:
+        raise ValueError("rho / phi_uncollided disagree with the library's materials / groups")
+    table, mu_s_l, emap = _fc_tables(library, device)
+    mats, dens = _material_slots(rho.to(device))
+    om = omegas.reshape(-1, 3).to(device, torch.float32).contiguous()
+    a = _hip.FcArgs()
+    a.M, a.G, a.n_eta = library.num_materials, library.num_groups, int(library.mu_s_eta.shape[-1])
+    a.n_map = 0 if emap is None else int(emap.numel())
+    a.table, a.eta_map = table.data_ptr(), _hip.ptr(emap)
+    a.n_slots, a.slot_material, a.slot_density = int(mats.shape[0]), mats.data_ptr(), dens.data_ptr()
+    a.phi_uc = phi_uncollided.data_ptr()
+    a.R, a.C, a.S = grid.shape
+    x, y, z = grid.axis_coordinates(dtype=torch.float64)
+    a.lattice[:] = [float(x[0]), float(y[0]), float(z[0]), *grid.deltas]
+    a.focal[:] = list(geometry.source_position(angle))
+    a.parallel = 0
+    a.omegas, a.K = om.data_ptr(), int(om.shape[0])
+    keep = (table, mu_s_l, emap, mats, dens, om)  # alive until the launch is queued
+    return a, keep, mu_s_l
+
+
+def first_collision_source_at_directions(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
+                                         geometry: ConeGeometry, angle: float, omegas: Tensor, chunk: int = 4,
+                                         backend: str = "auto", pairs=None) -> Tensor:
+    """[K, G, rows, cols, slices] first-collision source at the voxel centres of ``grid`` for each direction
+    (uncollided.py:200), one HIP launch (``citrus_fc_source``). ``chunk`` and ``pairs`` are accepted for the
+    reference's signature; the kernel takes every direction at once and forms material slots itself."""
+    if backend not in ("auto", "hip"):
+        raise ValueError(f"backend must be 'auto' or 'hip', got {backend!r}")
+    a, keep, _ = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
+    out = torch.empty((a.K, library.num_groups) + tuple(grid.shape), device=phi_uncollided.device,
+                      dtype=torch.float32)
+    with torch.cuda.device(phi_uncollided.device):
+        _hip.check(_hip.library().citrus_fc_source(ctypes.byref(a), out.data_ptr(),
+                                                   _hip.stream_of(phi_uncollided.device)), "citrus_fc_source")
+    del keep
+    return out
+
+
+def first_collision_exact_moments(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor,
+                                  l_max: int = 1) -> Tensor:
+    """[l_max+1, G, R, C, S] ``M_l[g] = sum_{g' <= g} sum_m rho_m mu_s_l^{g'->g} phi_UC,g'`` (uncollided.py:357)."""
+    if library.legendre_order < l_max:
+        raise ValueError(f"library Legendre order {library.legendre_order} < {l_max}")
+    device, dtype = phi_uncollided.device, phi_uncollided.dtype
+    G = phi_uncollided.shape[0]
+    transfer = library.mu_s_l[..., : l_max + 1].to(device=device, dtype=dtype)
+    down = torch.triu(torch.ones(G, G, device=device, dtype=dtype))
+    return torch.einsum("mabl,ab,m...,a...->lb...", transfer, down, rho.to(device=device, dtype=dtype),
+                        phi_uncollided)
+
+
+@dataclass(frozen=True, eq=False)
+class FirstCollisionSamplingCorrection:
+    """Per-point, per-group correction ``S <- scale S (1 + beta omega . Omega_UC)`` (uncollided.py:387-433).
+
+    Attributes: scale [G, R, C, S], beta [G, R, C, S] in [-1, 1], omega_uncollided [R, C, S, 3]."""
+
+    scale: Tensor
+    beta: Tensor
+    omega_uncollided: Tensor
+
+    def apply(self, source: Tensor, omegas: Tensor) -> Tensor:
+        """[K, G, R, C, S] corrected source from the samples at ``omegas`` [K, 3]."""
+        directions = omegas.to(device=source.device, dtype=source.dtype).reshape(-1, 3)
+        eta = torch.einsum("rcsx,kx->krcs", self.omega_uncollided.to(source.dtype), directions)
+        return source * self.scale * (1.0 + self.beta[None] * eta[:, None])
+
+
+def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights) -> Tensor:
+    """[5, G, R, C, S] float32 I0, I1, I2, E0, E1 from one launch of ``citrus_fc_sums``."""
+    if library.legendre_order < 1:
+        raise ValueError("the sampling correction needs a library of Legendre order >= 1")
+    a, keep, mu_s_l = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
+    w = weights.reshape(-1).to(phi_uncollided.device, torch.float32).contiguous()
+    if int(w.numel()) != a.K:
+        raise ValueError("weights and omegas disagree in length")
+    out = torch.empty((5, library.num_groups) + tuple(grid.shape), device=phi_uncollided.device, dtype=torch.float32)
+    with torch.cuda.device(phi_uncollided.device):
+        _hip.check(_hip.library().citrus_fc_sums(ctypes.byref(a), w.data_ptr(), mu_s_l.data_ptr(),
+                                                 int(mu_s_l.shape[-1]), out.data_ptr(),
+                                                 _hip.stream_of(phi_uncollided.device)), "citrus_fc_sums")
+    del keep
+    return out
+
+
+def first_collision_quadrature_sums(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
+                                    geometry: ConeGeometry, angle: float, omegas: Tensor, weights: Tensor,
+                                    omega_uncollided: Optional[Tensor] = None, chunk: int = 4, batch: int = 8,
+                                    pairs=None) -> Tensor:
+    """[3, G, R, C, S] float64 ``I_j = sum_k w_k (omega_k . Omega_UC)^j S_k`` (uncollided.py:538), accumulated in
+    float32 over all directions in one kernel like the reference's fused path (uncollided_kernels.py:637)."""
+    return _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights)[:3].double()
+
+
+def first_collision_correction_factors(I: Tensor, E: Tensor) -> tuple[Tensor, Tensor]:
+    """(scale, beta) in float64 from the sums and exact moments (uncollided.py:611-640): ``c = E1/E0``,
+    ``beta = (c I0 - I1)/(I2 - c I1)`` clamped to [-1, 1] (0 where E0 <= 0 or the denominator is below 1e-12 |I0|),
+    ``scale = E0/(I0 + beta I1)`` (1 where that denominator is not positive)."""
+    I = I.double()
+    E = E.to(device=I.device, dtype=torch.float64)
+    I0, I1, I2 = I[0], I[1], I[2]
+    E0, E1 = E[0], E[1]
+    pos = E0 > 0
+    c = torch.where(pos, E1 / torch.where(pos, E0, torch.ones_like(E0)), torch.zeros_like(E0))
+    den = I2 - c * I1
+    ok = pos & (den.abs() > 1e-12 * I0.abs())
+    beta = torch.where(ok, (c * I0 - I1) / torch.where(ok, den, torch.ones_like(den)), torch.zeros_like(den))
+    beta = beta.clamp(-1.0, 1.0)
+    d0 = I0 + beta * I1
+    scale = torch.where(d0 > 0, E0 / torch.where(d0 > 0, d0, torch.ones_like(d0)), torch.ones_like(d0))
+    return scale, beta
+
+
+def first_collision_sampling_correction(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
+                                        geometry: ConeGeometry, angle: float, omegas: Tensor, weights: Tensor,
+                                        chunk: int = 4, batch: int = 8,
+                                        pairs=None) -> FirstCollisionSamplingCorrection:
+    """Correction that makes the direction quadrature integrate the sampled source exactly for l = 0, 1
+    (uncollided.py:436-461): sums and exact moments from one ``citrus_fc_sums`` launch (float32), factors in
+    float64, returned in float32."""
+    sums = _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights).double()
+    scale, beta = first_collision_correction_factors(sums[:3], sums[3:])
+    ou = primary_directions_to_voxels(grid, geometry, angle, phi_uncollided.device, torch.float32)
+    return FirstCollisionSamplingCorrection(scale=scale.float().contiguous(), beta=beta.float().contiguous(),
+                                            omega_uncollided=ou)
+
+
+_FC_LAYOUTS = {"moment_major": 0, "cell_major": 1, "group_major": 2}
+
+
+def first_collision_moments_at_directions(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
+                                          geometry: ConeGeometry, angle: float, omegas: Tensor,
+                                          correction: Optional[FirstCollisionSamplingCorrection], factor: int,
+                                          chunk: int = 4, pairs=None, out: Optional[Tensor] = None,
+                                          cell_major: bool = False, layout: Optional[str] = None) -> Tensor:
+    """LD moments of the corrected first-collision source on the cells of the ``factor``-coarsened grid
+    (uncollided.py:464-535), one ``citrus_fc_moments`` launch: [K, G, 4, Rc, Cc, Sc], or cell-major
+    [K, Rc, Cc, Sc, G, 4] (the sweep's source chunk) with ``cell_major``; ``layout="group_major"`` gives
+    [G, Rc, Cc, Sc, K, 4]. ``correction`` None leaves the samples uncorrected."""
+    f = int(factor)
+    if f < 1 or any(n % f for n in grid.shape):
+        raise ValueError(f"grid shape {grid.shape} not divisible by the factor {factor}")
+    name = layout or ("cell_major" if cell_major else "moment_major")
+    if name not in _FC_LAYOUTS:
+        raise ValueError(f"layout must be one of {sorted(_FC_LAYOUTS)}, got {layout!r}")
+    a, keep, _ = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
+    G, K = library.num_groups, a.K
+    Rc, Cc, Sc = (n // f for n in grid.shape)
+    shape = {"moment_major": (K, G, 4, Rc, Cc, Sc), "cell_major": (K, Rc, Cc, Sc, G, 4),
+             "group_major": (G, Rc, Cc, Sc, K, 4)}[name]
+    if out is None:
+        out = torch.empty(shape, device=phi_uncollided.device, dtype=torch.float32)
+    elif tuple(out.shape) != shape or not out.is_contiguous() or out.dtype != torch.float32:
+        raise ValueError(f"out must be a contiguous float32 tensor of shape {shape}")
+    scale = beta = None
+    if correction is not None:
+        scale = correction.scale.to(phi_uncollided.device, torch.float32).contiguous()
+        beta = correction.beta.to(phi_uncollided.device, torch.float32).contiguous()
+    with torch.cuda.device(phi_uncollided.device):
+        _hip.check(_hip.library().citrus_fc_moments(ctypes.byref(a), _hip.ptr(scale), _hip.ptr(beta), f,
+                                                    _FC_LAYOUTS[name], out.data_ptr(),
+                                                    _hip.stream_of(phi_uncollided.device)), "citrus_fc_moments")
+    del keep
+    return out
+
+
+def restrict_to_linear_discontinuous_moments(field: Tensor, grid: Grid, factor: int) -> Tensor:
+    """[..., 4, R/f, C/f, S/f] LD moments [0, x, y, z] of a field on the refined grid (uncollided.py:311-352):
+    sub-point coordinates ``u_j = (2j + 1)/f - 1``, mean and ``mean(S u_a)/mean(u_a^2)``."""
+    f = int(factor)
+    if any(n % f for n in grid.shape):
+        raise ValueError(f"grid shape {grid.shape} not divisible by the factor {factor}")
+    if tuple(field.shape[-3:]) != tuple(grid.shape):
+        raise ValueError(f"field spatial shape {tuple(field.shape[-3:])} != grid shape {grid.shape}")
+    lead = field.shape[:-3]
+    R, C, S = (n // f for n in grid.shape)
+    x = field.reshape(*lead, R, f, C, f, S, f)
+    nl = len(lead)
+    ar, ac, as_ = nl + 1, nl + 3, nl + 5
+    mean = x.mean(dim=(ar, ac, as_))
+    if f == 1:
+        zero = torch.zeros_like(mean)
+        return torch.stack([mean, zero, zero, zero], dim=-4)
+    u = (2 * torch.arange(f, device=field.device, dtype=field.dtype) + 1) / f - 1
+    scale = 1.0 / (u * u).mean()
+
+    def slope(axis: int) -> Tensor:
+        shape = [1] * x.dim()
+        shape[axis] = f
+        return (x * u.reshape(shape)).mean(dim=(ar, ac, as_)) * scale
+
+    return torch.stack([mean, slope(ac), slope(ar), slope(as_)], dim=-4)

@@ -174,6 +174,48 @@ int citrus_uncollided_fluence(const float* paths, int M, const double* lattice, 
                               const float* profile, const float* thickness, const float* filter_mu, float* out,
                               void* stream);
 
+/* ---- first-collision source (citrus/scatter_solver/uncollided.py:200-656; uncollided_kernels.py:404-700) ---------
+ * Shared inputs of the three entry points below. Points form the lattice r = lattice[0:3] + (c dx, r dy, s dz) of
+ * the R x C x S grid (flat (r C + c) S + s). Each point carries up to 4 material slots (index into the library,
+ * density in g/cm^3; unused slots density 0), the counterpart of the reference's MaterialPairs
+ * (cross_sections.py:656). The transfer table is node-major [n_eta][M][G][GP] with GP = 4 for G <= 4, 8 for G <= 8,
+ * 16 for G <= 16 and 32 otherwise (padding entries 0), and the entries g_out < g_in zeroed (downscatter only,
+ * uncollided.py:258); eta_map, when given, is the adaptive node map of the table (cross_sections.py:905-945).
+ * Pointers are device unless marked host. */
+typedef struct CitrusFcArgs {
+  int M, G, n_eta, n_map;       /* materials, groups, table nodes, map knots (ignored without eta_map) */
+  const float* table;           /* [n_eta][M][G][GP] per-steradian transfer in cm^2/g/sr */
+  const float* eta_map;         /* [n_map] or NULL (uniform table) */
+  int n_slots;                  /* material slots per point, 1..4 */
+  const int* slot_material;     /* [n_slots][P] */
+  const float* slot_density;    /* [n_slots][P] */
+  const float* phi_uc;          /* [G][P] uncollided fluence */
+  int R, C, S;                  /* point grid */
+  double lattice[6];            /* host values: first point centre (x, y, z), pitches (dx, dy, dz) in cm */
+  double focal[3];              /* host values: focal spot, or the beam direction when parallel != 0 */
+  int parallel;
+  const float* omegas;          /* [K][3] unit directions */
+  int K;
+} CitrusFcArgs;
+
+/* S_FC at every point and direction: out [K][G][P] (first_collision_source_at_directions, uncollided.py:200). */
+int citrus_fc_source(const CitrusFcArgs* args, float* out, void* stream);
+
+/* Sampling-correction inputs per point: out [5][G][P] = quadrature sums I0, I1, I2 (sum_k w_k eta_k^j S_k, float32
+ * in direction order) and the exact moments E0, E1 = sum_{g'<=g} sum_m rho_m mu_s_l^{g'->g}[l] phi_UC,g'
+ * (first_collision_quadrature_sums and first_collision_exact_moments, uncollided.py:357-384, :538-608; the fused
+ * first_collision_sums of uncollided_kernels.py:637). weights [K]; mu_s_l [M][G][G][lp1], lp1 >= 2. */
+int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1, float* out,
+                   void* stream);
+
+/* LD moments of the corrected first-collision source on the cells of the factor-coarsened point grid
+ * (first_collision_moments_at_directions, uncollided.py:464-535): samples S scale (1 + beta eta) (scale, beta
+ * [G][P], or both NULL for no correction) at the f^3 sub-points of each cell, restricted to [mean, x, y, z] slopes
+ * (restrict_to_linear_discontinuous_moments, uncollided.py:311). layout 0: out [K][G][4][cells]; 1: cell-major
+ * [K][cells][G][4] (the sweep's source chunk); 2: group-major [G][cells][K][4]. */
+int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float* beta, int factor, int layout,
+                      float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

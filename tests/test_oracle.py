@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import checks, ld, quadrature as oq, scattering, sweep as osw
+from oracle import first_collision as ofc
 
 
 def _octant_set(base=(0.3, 0.5, 0.81), replace=None):
@@ -233,3 +234,130 @@ def test_source_iteration_fixed_point():
     fp = osw.moments(osw.sweep(shape, deltas, om, mu_t, osw.sources_from_moments(T, Y)), w, Y) \
         + osw.moments(osw.sweep(shape, deltas, om, mu_t, fc), w, Y)
     assert np.linalg.norm(fp - phi) / np.linalg.norm(phi) < 1e-7
+
+
+# ---- first-collision source, sampling correction, LD restriction ---------------------------------------------------
+
+def _random_points(rng, P):
+    pts = rng.uniform(-3.0, 3.0, (P, 3))
+    return pts, ofc.primary_directions(pts, np.array([0.0, 100.0, 0.5]))
+
+
+def test_fc_source_hand_interpolation():
+    """Per-element np.interp of a random (non-linear) table, the hand evaluation of test_uncollided.py:181-213."""
+    rng = np.random.default_rng(3)
+    M, G, N, P, K = 2, 4, 37, 6, 5
+    eta_nodes = np.linspace(-1.0, 1.0, N)
+    table = rng.random((M, G, G, N))
+    rho = rng.random((M, P)) + 0.1
+    phi = rng.random((G, P))
+    _, omega_uc = _random_points(rng, P)
+    om = rng.standard_normal((K, 3))
+    om /= np.linalg.norm(om, axis=1, keepdims=True)
+    S = ofc.source_at_directions(table, None, rho, phi, omega_uc, om)
+    for k in range(K):
+        for p in range(P):
+            eta = float(omega_uc[p] @ om[k])
+            for go in range(G):
+                ref = sum(rho[m, p] * np.interp(eta, eta_nodes, table[m, gi, go]) * phi[gi, p]
+                          for gi in range(go + 1) for m in range(M))
+                assert abs(S[k, go, p] - ref) < 1e-13 * max(1.0, abs(ref))
+
+
+def test_fc_source_synthetic_kernel_closed_form():
+    """The synthetic kernel a (1 + b eta) / 4 pi is linear in eta, so a uniform table reproduces it exactly."""
+    from citrus_amd.scatter_solver.cross_sections import synthetic_library
+
+    lib = synthetic_library(5, 3, num_eta=11)
+    rng = np.random.default_rng(4)
+    P, K, G = 9, 7, 5
+    rho = rng.random((2, P))
+    phi = rng.random((G, P))
+    _, omega_uc = _random_points(rng, P)
+    om = rng.standard_normal((K, 3))
+    om /= np.linalg.norm(om, axis=1, keepdims=True)
+    S = ofc.source_at_directions(lib.mu_s_eta.numpy(), None, rho, phi, omega_uc, om)
+    eta = om @ omega_uc.T  # [K, P]
+    ref = np.zeros_like(S)
+    for m, am in enumerate((0.15, 0.12)):
+        for gi in range(G):
+            for go in range(gi, G):
+                a, b = am * 0.6 ** (go - gi), (0.5 if go == gi else 0.2)
+                ref[:, go] += rho[m] * a * (1 + b * eta) / (4 * math.pi) * phi[gi]
+    assert np.abs(S - ref).max() < 1e-14
+
+
+def test_fc_exact_moments_match_sphere_integral():
+    """E_l = int P_l(w . Omega_UC) S dOmega for the synthetic kernel, by a Gauss-Legendre x uniform-azimuth rule that
+    integrates its degree-2 integrands exactly."""
+    from citrus_amd.scatter_solver.cross_sections import synthetic_library
+
+    lib = synthetic_library(4, 3, num_eta=21)
+    rng = np.random.default_rng(5)
+    P, G = 4, 4
+    rho = rng.random((2, P))
+    phi = rng.random((G, P))
+    _, omega_uc = _random_points(rng, P)
+    x, wx = np.polynomial.legendre.leggauss(6)
+    na = 12
+    az = (np.arange(na) + 0.5) * 2 * math.pi / na
+    om = np.array([[math.sqrt(1 - mu * mu) * math.cos(a), math.sqrt(1 - mu * mu) * math.sin(a), mu]
+                   for mu in x for a in az])
+    w = np.repeat(wx * 2 * math.pi / na, na)
+    S = ofc.source_at_directions(lib.mu_s_eta.numpy(), None, rho, phi, omega_uc, om)
+    E = ofc.exact_moments(lib.mu_s_l.numpy(), rho, phi, 1)
+    I = ofc.quadrature_sums(S, omega_uc, om, w)
+    assert np.allclose(I[0], E[0], rtol=1e-12, atol=0)
+    assert np.allclose(I[1], E[1], rtol=1e-12, atol=1e-15)
+
+
+def test_fc_adaptive_table_positions_land_on_nodes():
+    emap = ofc.synthetic_angular_map()
+    nodes = ofc.table_nodes(65, emap)
+    assert nodes[0] == -1.0 and nodes[-1] == 1.0 and np.all(np.diff(nodes) > 0)
+    i0, w1 = ofc.table_position(nodes, 65, emap)
+    pos = i0 + w1
+    assert np.abs(pos - np.arange(65)).max() < 1e-9
+    mid = 0.5 * (nodes[:-1] + nodes[1:])  # strictly between neighbours
+    i0m, w1m = ofc.table_position(mid, 65, emap)
+    assert np.all(i0m == np.arange(64)) and np.all((w1m > 0) & (w1m < 1))
+
+
+@pytest.mark.parametrize("shape,factor", [((4, 6, 8), 2), ((6, 6, 3), 3), ((4, 4, 4), 4)])
+def test_fc_restriction_reproduces_linear_fields(shape, factor):
+    rng = np.random.default_rng(6)
+    Rc, Cc, Sc = (n // factor for n in shape)
+    coef = rng.standard_normal((2, 3, 4, Rc, Cc, Sc))
+    u = (2.0 * np.arange(factor) + 1.0) / factor - 1.0
+    uy = np.tile(u, Rc)[:, None, None]
+    ux = np.tile(u, Cc)[None, :, None]
+    uz = np.tile(u, Sc)[None, None, :]
+
+    def up(t):
+        return t.repeat(factor, -3).repeat(factor, -2).repeat(factor, -1)
+
+    field = up(coef[:, :, 0]) + up(coef[:, :, 1]) * ux + up(coef[:, :, 2]) * uy + up(coef[:, :, 3]) * uz
+    mom = ofc.restrict_ld(field, factor)
+    assert np.abs(mom - coef).max() < 1e-12
+    assert np.allclose(mom[:, :, 0].sum((-3, -2, -1)) * factor ** 3, field.sum((-3, -2, -1)))
+
+
+def test_fc_correction_conservation_identities():
+    """Corrected samples integrate to E0 exactly, and to E1 about Omega_UC wherever beta is not clamped."""
+    rng = np.random.default_rng(7)
+    G, P, K = 3, 50, 24
+    _, omega_uc = _random_points(rng, P)
+    om = rng.standard_normal((K, 3))
+    om /= np.linalg.norm(om, axis=1, keepdims=True)
+    w = rng.random(K) * 4 * math.pi / K * 2
+    S = rng.random((K, G, P)) + 0.05
+    I = ofc.quadrature_sums(S, omega_uc, om, w)
+    E = np.stack([I[0] * (1 + 0.3 * rng.standard_normal((G, P))), I[1] + 0.05 * I[0] * rng.standard_normal((G, P))])
+    E[0] = np.abs(E[0])
+    scale, beta = ofc.correction_factors(I, E)
+    Sc = ofc.apply_correction(S, scale, beta, omega_uc, om)
+    J = ofc.quadrature_sums(Sc, omega_uc, om, w)
+    assert np.allclose(J[0], E[0], rtol=1e-12)
+    free = np.abs(beta) < 1
+    assert free.mean() > 0.3 and (~free).any()
+    assert np.allclose(J[1][free], E[1][free], rtol=1e-10, atol=1e-12)
