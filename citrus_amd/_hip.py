@@ -78,6 +78,9 @@ _SIGNATURES = {
     "citrus_fc_source": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
     "citrus_fc_sums": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void]),
     "citrus_fc_moments": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_int, c_void, c_void]),
+    "citrus_attenuated_emission": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_void, c_int, c_void, c_void,
+                                           c_void, c_void, c_void]),
+    "citrus_box_matrix": (c_int, [c_void, c_int, c_int, c_void, c_void]),
 }
 
 

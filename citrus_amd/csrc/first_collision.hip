@@ -301,6 +301,74 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
   }
 }
 
+// Attenuated total emission of the collapsed-cone image at the quadrature points (uncollided_kernels.py:713-830,
+// scatter_image.py:829-875): e_g = (S_FC,g(point, w) + S_MC,g(cell, w)) exp(-tau_j,g) with the first-order in-cell
+// optical depth tau_j = max(0, tau_cell - mu_cell (w . (r_j - r_cell))) (tau_cell exactly for f = 1). Thread per
+// (point, direction), blockIdx.y = direction; each block also reduces sum_g e and h sum_g e over its points into
+// partial [K][blocks][2] (no atomics), the sums of the footprint's emission-weighted mean height.
+template <int GMAX>
+__global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* __restrict__ tau,
+                                                          const float* __restrict__ mu_c,
+                                                          const float* __restrict__ multiple, int f,
+                                                          const float* __restrict__ heights,
+                                                          const float* __restrict__ signal_weight,
+                                                          float* __restrict__ out, float* __restrict__ partial) {
+  __shared__ float red[2][kBlock];
+  const int k = blockIdx.y;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const int Rc = d.R / f, Cc = d.C / f, Sc = d.S / f;
+  const int Nc = Rc * Cc * Sc;
+  float esum = 0.0f, ehsum = 0.0f;
+  if (p < d.P) {
+    const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
+    const int cell = ((r / f) * Cc + (c / f)) * Sc + (s / f);
+    const float ox = d.omegas[3 * k], oy = d.omegas[3 * k + 1], oz = d.omegas[3 * k + 2];
+    float ux, uy, uz;
+    primary_direction(d, r, c, s, ux, uy, uz);
+    PointData<GMAX> pd;
+    load_point(d, p, pd);
+    float S[GMAX];
+    eval_source(d, pd, ux * ox + uy * oy + uz * oz, S);
+    // offset of the point from its cell centre along w (centred sub-point index times the pitch)
+    const float half = 0.5f * (float)(f - 1);
+    const float off = f > 1 ? ox * (((float)(c % f) - half) * (float)d.dx) + oy * (((float)(r % f) - half) * (float)d.dy)
+                                  + oz * (((float)(s % f) - half) * (float)d.dz)
+                            : 0.0f;
+    float sig = 0.0f;
+#pragma unroll
+    for (int g = 0; g < GMAX; ++g) {
+      if (g >= d.G) continue;
+      const int64_t kg = (int64_t)k * d.G + g;
+      float t = tau[kg * Nc + cell];
+      if (f > 1) t = fmaxf(t - mu_c[(int64_t)g * Nc + cell] * off, 0.0f);
+      const float e = (S[g] + multiple[kg * Nc + cell]) * expf(-t);
+      esum += e;
+      if (signal_weight) {
+        sig = fmaf(signal_weight[kg], e, sig);
+      } else {
+        out[kg * d.P + p] = e;
+      }
+    }
+    if (signal_weight) out[(int64_t)k * d.P + p] = sig;
+    ehsum = esum * (heights ? heights[p] : 0.0f);
+  }
+  red[0][threadIdx.x] = esum;
+  red[1][threadIdx.x] = ehsum;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float* o = partial + ((int64_t)k * gridDim.x + blockIdx.x) * 2;
+    o[0] = red[0][0];
+    o[1] = red[1][0];
+  }
+}
+
 int check_args(const CitrusFcArgs* a) {
   if (!a) return fail(CITRUS_ERR_ARG, "args is NULL");
   if (a->G < 1 || a->G > 32) return fail(CITRUS_ERR_ARG, "1..32 groups supported");
@@ -383,4 +451,25 @@ extern "C" int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, c
     return 0;
   });
   return finish("first-collision moments");
+}
+
+extern "C" int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, const float* mu_t_coarse,
+                                          const float* multiple, int factor, const float* heights,
+                                          const float* signal_weight, float* out, float* partial, void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!tau || !mu_t_coarse || !multiple || !out || !partial) return fail(CITRUS_ERR_ARG, "NULL input or output");
+  if (factor < 1 || args->R % factor || args->C % factor || args->S % factor)
+    return fail(CITRUS_ERR_ARG, "the point grid must be divisible by the factor");
+  if (args->K > 65535) return fail(CITRUS_ERR_ARG, "at most 65535 directions per launch");
+  const FcDev d = make_dev(*args);
+  if (d.K == 0) return CITRUS_OK;
+  const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock), (unsigned)d.K), block(kBlock);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dispatch_groups(d.G, [&](auto gm) {
+    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value>), grid, block, 0, s, d, tau, mu_t_coarse, multiple,
+                       factor, heights, signal_weight, out, partial);
+    return 0;
+  });
+  return finish("attenuated emission");
 }

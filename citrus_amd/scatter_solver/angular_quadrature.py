@@ -152,6 +152,104 @@ def level_symmetric_quadrature(order: int, device: _DeviceLike = None, dtype: to
     return Quadrature.from_directions(omega.to(device=device, dtype=dtype), weight.to(device=device, dtype=dtype))
 
 
+# ---- cone sets of the collapsed-cone scatter image (angular_quadrature.py:229-420) ---------------------------------
+
+def _unit_vector(v) -> Tensor:
+    t = torch.as_tensor(v, dtype=torch.float64).detach().to("cpu").reshape(-1)
+    if t.numel() != 3:
+        raise ValueError("axis must have 3 components")
+    n = torch.linalg.norm(t)
+    if not bool(n > 0):
+        raise ValueError("axis must be non-zero")
+    return t / n
+
+
+def _local_frame(a: Tensor) -> tuple[Tensor, Tensor, Tensor]:
+    """Right-handed frame (e1, e2, a) about a unit axis: the helper is the coordinate axis least aligned with a (lowest
+    index on ties), ``e1 = normalize(helper - (helper . a) a)``, ``e2 = a x e1`` (angular_quadrature.py:248)."""
+    helper = torch.zeros(3, dtype=torch.float64)
+    helper[int(torch.argmin(a.abs()))] = 1.0
+    e1 = helper - (helper @ a) * a
+    e1 = e1 / torch.linalg.norm(e1)
+    return e1, torch.linalg.cross(a, e1), a
+
+
+def _cone_polar(num_polar: int, max_polar_angle: float) -> tuple[Tensor, Tensor]:
+    """(cos theta, weights) [P] of the Gauss-Legendre nodes on [cos(max), 1], near-axis first."""
+    cos_max = math.cos(max_polar_angle)
+    x, w = np.polynomial.legendre.leggauss(num_polar)
+    half = 0.5 * (1.0 - cos_max)
+    return (torch.as_tensor(cos_max + half * (x + 1.0), dtype=torch.float64).flip(0),
+            torch.as_tensor(w * half, dtype=torch.float64).flip(0))
+
+
+def _cone_cell_widths(a: Tensor, u: Tensor, v: Tensor, num_polar: int, num_azimuthal: int, max_polar_angle: float,
+                      subsamples: int) -> Tensor:
+    """[M, 2] gnomonic widths of every cone cell (angular_quadrature.py:375): node (i, j) owns the cell between the
+    cumulative-weight edges in cos(theta) and the azimuths [j dphi, (j+1) dphi]; the width is sqrt(12) times the
+    standard deviation of ``(d.u / d.a, d.v / d.a)`` over an s x s equal-area lattice of the cell."""
+    s = int(subsamples)
+    frac = (torch.arange(s, dtype=torch.float64) + 0.5) / s
+    cos_max = math.cos(max_polar_angle)
+    _, pw = _cone_polar(num_polar, max_polar_angle)
+    edges = torch.cat([torch.ones(1, dtype=torch.float64), 1.0 - torch.cumsum(pw, 0)])
+    edges[-1] = cos_max
+    step = 2.0 * math.pi / num_azimuthal
+    e1, e2, _ = _local_frame(a)
+    cs = edges[1:, None] + (edges[:-1] - edges[1:])[:, None] * frac[None, :]  # [P, s]
+    az = (torch.arange(num_azimuthal, dtype=torch.float64)[:, None] + frac[None, :]) * step  # [A, s]
+    sn = torch.sqrt((1.0 - cs * cs).clamp_min(0.0))
+    lateral = torch.cos(az)[..., None] * e1 + torch.sin(az)[..., None] * e2  # [A, s, 3]
+    d = sn[:, None, :, None, None] * lateral[None, :, None, :, :] + cs[:, None, :, None, None] * a
+    nc = (d @ a).clamp_min(1e-6)
+    alpha, beta = (d @ u) / nc, (d @ v) / nc
+    n = float(s * s)
+
+    def width(x: Tensor) -> Tensor:
+        mean = x.sum(dim=(2, 3)) / n
+        var = ((x * x).sum(dim=(2, 3)) / n - mean * mean).clamp_min(0.0)
+        return torch.sqrt(12.0 * var).reshape(-1)
+
+    return torch.stack([width(alpha), width(beta)], dim=-1)
+
+
+def hemisphere_quadrature(axis, num_polar: int, num_azimuthal: int, max_polar_angle: float = math.pi / 2,
+                          device: _DeviceLike = None, dtype: torch.dtype = torch.float64, *,
+                          tangent_axes: Optional[tuple] = None, subsamples: int = 16) -> Quadrature:
+    """Product set on the cone of half-angle ``max_polar_angle`` about ``axis`` (angular_quadrature.py:309):
+    Gauss-Legendre in cos(theta) on [cos(max), 1] times equal azimuths ``(j + 1/2) 2 pi / A`` in ``_local_frame``,
+    flat index ``i A + j`` with i = 0 nearest the axis; the weights sum to ``2 pi (1 - cos(max))``. With
+    ``tangent_axes = (u, v)`` the gnomonic cell widths of the finite-cone footprint are filled."""
+    if not (0.0 < max_polar_angle <= math.pi):
+        raise ValueError("max_polar_angle must be in (0, pi]")
+    if num_polar < 1 or num_azimuthal < 1:
+        raise ValueError("num_polar and num_azimuthal must be >= 1")
+    if device is None and isinstance(axis, Tensor):
+        device = axis.device
+    a = _unit_vector(axis)
+    cos_t, pw = _cone_polar(num_polar, max_polar_angle)
+    step = 2.0 * math.pi / num_azimuthal
+    phi = (torch.arange(num_azimuthal, dtype=torch.float64) + 0.5) * step
+    e1, e2, _ = _local_frame(a)
+    sin_t = torch.sqrt((1.0 - cos_t * cos_t).clamp_min(0.0))
+    local = ((sin_t[:, None] * torch.cos(phi)[None, :])[..., None] * e1
+             + (sin_t[:, None] * torch.sin(phi)[None, :])[..., None] * e2 + cos_t[:, None, None] * a)
+    omega = local.reshape(-1, 3)
+    omega = omega / torch.linalg.norm(omega, dim=-1, keepdim=True)
+    weight = (pw[:, None] * step).expand(num_polar, num_azimuthal).reshape(-1)
+    q = Quadrature.from_directions(omega.to(device=device, dtype=dtype), weight.to(device=device, dtype=dtype))
+    if tangent_axes is None:
+        return q
+    u = _unit_vector(tangent_axes[0])
+    u = u - (u @ a) * a
+    u = u / torch.linalg.norm(u)
+    v = _unit_vector(tangent_axes[1])
+    v = v - (v @ a) * a - (v @ u) * u
+    v = v / torch.linalg.norm(v)
+    width = _cone_cell_widths(a, u, v, num_polar, num_azimuthal, max_polar_angle, subsamples)
+    return Quadrature(omega=q.omega, weight=q.weight, octant=q.octant, cell_width=width.to(device=device, dtype=dtype))
+
+
 def legendre_polynomials(L: int, x: Tensor) -> Tensor:
     """P_0..P_L at x by Bonnet's recurrence, [..., L+1]."""
     if L < 0:

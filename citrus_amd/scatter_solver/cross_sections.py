@@ -35,6 +35,12 @@ class MultigroupLibrary:
     fine_weight: Tensor        # [E] spectrum weight of each fine energy
     fine_group: Tensor         # [E] int64 group of each fine energy
     eta_map: Optional[Tensor] = None  # [K] node map of an adaptive angular table, None for a uniform one
+    group_energy: Optional[Tensor] = None  # [G] keV, the group-edge midpoints when not given
+
+    def __post_init__(self) -> None:
+        if self.group_energy is None:
+            e = self.group_edges.to(torch.float64)
+            object.__setattr__(self, "group_energy", 0.5 * (e[:-1] + e[1:]))
 
     @property
     def num_groups(self) -> int:

@@ -235,6 +235,24 @@ class ConeGeometry:
         return (pc + (R * torch.sin(gamma))[..., None] * uh + (2.0 * R * torch.sin(0.5 * gamma) ** 2)[..., None] * nh
                 + (v * torch.cos(gamma))[..., None] * vh)
 
+    def panel_half_extent(self) -> tuple[float, float]:
+        """Half widths (u, v) of the active panel area, edge to edge, in cm (geometry.py:482)."""
+        return (self.num_det_channels * self.delta_det_channel / 2, self.num_det_rows * self.delta_det_row / 2)
+
+    def local_panel_axes(self, points: Tensor, angle: float) -> tuple[Tensor, Tensor, Tensor]:
+        """(u_hat, v_hat, n_hat) [..., 3] of the panel surface at panel points, n_hat toward the source
+        (geometry.py:624): the view's axes on a flat panel; on a curved one the horizontal normal toward the cylinder
+        axis through the source, the vertical v_hat and ``u_hat = n_hat x v_hat``."""
+        device, dtype = points.device, points.dtype
+        u_hat, v_hat, n_hat = (torch.tensor(a, device=device, dtype=dtype).expand_as(points)
+                               for a in self.panel_axes(angle))
+        if not self.use_curved_detector:
+            return u_hat, v_hat, n_hat
+        to_axis = torch.tensor(self.source_position(angle), device=device, dtype=dtype) - points
+        to_axis = to_axis - (to_axis * v_hat).sum(-1, keepdim=True) * v_hat
+        n_local = to_axis / torch.linalg.norm(to_axis, dim=-1, keepdim=True)
+        return torch.linalg.cross(n_local, v_hat, dim=-1), v_hat, n_local
+
     def pixel_centers(self, angle: float, device=None, dtype=torch.float32) -> Tensor:
         """[rows, channels, 3] pixel-centre positions in cm."""
         u, v = self.pixel_uv(device, dtype)

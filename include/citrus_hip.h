@@ -216,6 +216,23 @@ int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* 
 int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float* beta, int factor, int layout,
                       float* out, void* stream);
 
+/* ---- collapsed-cone scatter image (citrus/scatter_solver/scatter_image.py:829-1323) ----------------------------
+ * Attenuated total emission at the quadrature points of one direction chunk, the fused form of
+ * first_collision_source_at_directions + the cell's multiple-collision source + supervoxel_subpoint_emission
+ * (uncollided_kernels.py:713, attenuated_emission at :1138): e_g = (S_FC,g + S_MC,g) exp(-tau_j,g) with
+ * tau_j = max(0, tau_cell - mu_cell w . (r_j - r_cell)). args: the first-collision inputs on the quadrature grid,
+ * omegas = the chunk's K directions; tau and multiple [K][G][cells] of the factor-coarsened grid; mu_t_coarse
+ * [G][cells]; heights [P] panel heights of the points (or NULL: 0); signal_weight [K][G] (or NULL). out [K][G][P], or
+ * [K][P] = sum_g signal_weight e_g with signal_weight. partial [K][ceil(P / 256)][2]: per block of 256 points the sums
+ * of sum_g e_g and h sum_g e_g (unweighted), which the caller reduces. */
+int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, const float* mu_t_coarse,
+                               const float* multiple, int factor, const float* heights, const float* signal_weight,
+                               float* out, float* partial, void* stream);
+
+/* Footprint box-filter matrices out [k][N][N], B[c][j] = |[j, j+1] n W_c| / |W_c|, W_c the window of width
+ * widths[m] (pixels) centred on pixel c clipped to [0, N] (box_matrix, scatter_image.py:952). */
+int citrus_box_matrix(const float* widths, int k, int N, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,145 @@
+"""Collapsed-cone scatter image, primary image and SPR on the HIP kernels vs the FP64 oracle (GPU).
+
+Reference tests mirrored: single-voxel bookkeeping (tests/test_scatter_image.py:131-167), the combined image of the
+first-collision and multiple-collision sources (:729-787), the footprint (tests/test_footprint.py) and the primary
+closed forms. FP32 bar ``F32_REL = 1e-5`` of the field maximum."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from citrus_amd.scatter_solver.angular_quadrature import Quadrature, hemisphere_quadrature, real_spherical_harmonics
+from citrus_amd.scatter_solver.cross_sections import seeded_library, synthetic_library
+from citrus_amd.scatter_solver.detector import energy_weighted_response, ideal_counting_response, no_grid
+from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
+from citrus_amd.scatter_solver.ray_tracing import HipProjectors
+from citrus_amd.scatter_solver.scatter_image import (
+    box_matrix, combined_scatter_image, make_direction_context, panel_direction_set, panel_height_field,
+    primary_image, scatter_fraction, scatter_image_from_source)
+from citrus_amd.scatter_solver.source import SourceModel
+from oracle import first_collision as ofc
+from oracle import image as oim
+from oracle import rays as orays
+from tests.conftest import gpu_device
+from tests.helpers import F32_REL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def test_box_matrix_kernel_matches_oracle():
+    dev = gpu_device()
+    w = torch.tensor([0.2, 1.0, 1.7, 3.3, 9.0, 40.0])
+    got = box_matrix(23, w.to(dev))
+    assert rel_err(got, oim.box_matrix(23, w.numpy())) < 1e-6
+
+
+@pytest.mark.parametrize("angle", [0.0, math.pi / 2])
+def test_single_voxel_bookkeeping_on_gpu(angle):
+    dev = gpu_device()
+    grid = Grid((3, 3, 3), 1.0)
+    geometry = ConeGeometry(20.0, 30.0, 40, 40, 1.0, 1.0)
+    E0 = torch.tensor([2.0, 0.5], device=dev)
+    field = torch.zeros(2, *grid.shape, device=dev)
+    field[:, 1, 1, 1] = E0
+    n_hat = torch.tensor(geometry.panel_axes(angle)[2], dtype=torch.float64)
+    q = hemisphere_quadrature(-n_hat, 6, 12, math.radians(55.0), dtype=torch.float32)
+    lib = synthetic_library(2, 1)
+    out = scatter_image_from_source(lambda d: field.expand(d.shape[0], -1, -1, -1, -1), grid, geometry, angle, q,
+                                    torch.zeros_like(field), HipProjectors(), energy_weighted_response(), no_grid(),
+                                    lib, chunk=16, footprint=False)
+    cos = q.omega.double() @ -n_hat
+    expected = E0.double().cpu() * (q.weight.double() / cos).sum()
+    assert torch.allclose(out["fluence"].sum((1, 2)).double().cpu() * geometry.pixel_area, expected, rtol=1e-5)
+    planar = E0.double().cpu() * q.weight.double().sum()
+    assert torch.allclose(out["planar_fluence"].sum((1, 2)).double().cpu() * geometry.pixel_area, planar, rtol=1e-5)
+    sig = float(out["signal"].sum()) * geometry.pixel_area
+    assert sig == pytest.approx(float((planar * lib.group_energy).sum()), rel=1e-5)
+
+
+def _image_problem(curved, q_factor, seed=5):
+    gen = torch.Generator().manual_seed(seed)
+    library = seeded_library(3, 2, 120.0, 20.0, seed=21, num_eta=65)
+    coarse = Grid((4, 5, 3), 0.8)
+    gq = Grid(tuple(n * q_factor for n in coarse.shape), 0.8 / q_factor)
+    rho = torch.rand(2, *gq.shape, generator=gen, dtype=torch.float64) * 1.2
+    phi = torch.rand(3, *gq.shape, generator=gen, dtype=torch.float64) * 0.01 + 0.001
+    T = torch.randn(3, 9, *coarse.shape, generator=gen, dtype=torch.float64) * 0.002
+    T[:, 0] = T[:, 0].abs() + 0.01
+    mu_c = torch.rand(3, *coarse.shape, generator=gen, dtype=torch.float64) * 0.3 + 0.1
+    geometry = ConeGeometry(40.0, 60.0, 12, 14, 0.6, 0.6, use_curved_detector=curved)
+    return library, coarse, gq, rho, phi, T, mu_c, geometry
+
+
+@pytest.mark.parametrize("curved,q_factor", [(False, 1), (False, 2), (True, 1)])
+def test_combined_scatter_image_matches_oracle(curved, q_factor):
+    dev = gpu_device()
+    library, coarse, gq, rho, phi, T, mu_c, geometry = _image_problem(curved, q_factor)
+    angle = 0.35
+    quad = panel_direction_set(geometry, angle, (3, 6), max_polar_angle=gq)
+    det, grid_response = energy_weighted_response(), no_grid()
+    ctx = make_direction_context(quad, geometry, angle, det, grid_response, library, dev, torch.float32, chunk=5)
+    f32 = lambda t: t.float().to(dev)
+    out = combined_scatter_image(library, f32(rho), f32(phi), gq, q_factor, f32(T), 2, geometry, angle, quad,
+                                 f32(mu_c), HipProjectors(), det, grid_response, context=ctx, grid_coarse=coarse)
+    sig_only = combined_scatter_image(library, f32(rho), f32(phi), gq, q_factor, f32(T), 2, geometry, angle, quad,
+                                      f32(mu_c), HipProjectors(), det, grid_response, context=ctx,
+                                      grid_coarse=coarse, signal_only=True)
+    # oracle on the same (float32-rounded) inputs and direction tables
+    d64 = lambda t: t.float().double().cpu().numpy()
+    om = ctx.omega.double().cpu().numpy()
+    pts = gq.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    ou = ofc.primary_directions(pts, np.array(geometry.source_position(angle)))
+    S = ofc.source_at_directions(library.mu_s_eta.numpy(), None, d64(rho).reshape(2, -1), d64(phi).reshape(3, -1),
+                                 ou, om).reshape(om.shape[0], 3, *gq.shape)
+    Y = real_spherical_harmonics(2, torch.from_numpy(om)).numpy()
+    pixels = geometry.pixel_centers(angle, dtype=torch.float64).numpy()
+    heights = panel_height_field(gq, geometry, angle, dtype=torch.float64).numpy()
+    _, _, n_hat = geometry.panel_axes(angle)
+    pc = geometry.panel_center(angle)
+    lo_, ext = gq.lower_corner(), gq.extent
+    hs = [sum((lo_[a] + c[a] * ext[a] - pc[a]) * n_hat[a] for a in range(3))
+          for c in [(i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]]
+    fl, pl, sg = oim.collapsed_cone_image(
+        lambda k: S[k], d64(T), Y, d64(mu_c), np.array(gq.lower_corner()), gq.deltas, gq.shape, q_factor, om,
+        ctx.weight.double().cpu().numpy(), ctx.cell_width.double().cpu().numpy(), pixels,
+        (geometry.delta_det_channel, geometry.delta_det_row), heights, 0.5 * (min(hs) + max(hs)),
+        ctx.signal_weight.double().cpu().numpy(), ctx.cos_theta0.double().cpu().numpy(), geometry.source_iso_dist,
+        geometry.source_detector_dist, angle, curved, footprint=True,
+        planar_weight=None if ctx.planar_weight is None else ctx.planar_weight.double().cpu().numpy())
+    assert rel_err(out["fluence"], fl) < F32_REL
+    assert rel_err(out["planar_fluence"], pl) < F32_REL
+    assert rel_err(out["signal_per_group"], sg) < F32_REL
+    assert rel_err(sig_only["signal"], sg.sum(0)) < F32_REL
+    assert float(out["fluence"].min()) > 0.0
+
+
+def test_primary_image_and_spr():
+    dev = gpu_device()
+    lib = synthetic_library(3, 1, num_fine_per_group=3, fine_spread=0.3)
+    grid = Grid((6, 8, 5), 0.5)
+    geometry = ConeGeometry(40.0, 70.0, 10, 12, 0.8, 0.8)
+    angle = 0.3
+    rho = torch.zeros(2, *grid.shape, dtype=torch.float64)
+    rho[0] = 1.0
+    rho[1, 2:4, 3:6, 1:4] = 1.5
+    out = primary_image(lib, rho.float().to(dev), grid, geometry, angle, SourceModel(), HipProjectors(),
+                        ideal_counting_response(), no_grid())
+    pixels = orays.pixel_centers(40.0, 70.0, 10, 12, 0.8, 0.8, angle, False)[:, :, 0, 0]
+    src = orays.source_position(40.0, angle)
+    lo = np.array(grid.lower_corner())
+    n = (grid.num_cols, grid.num_rows, grid.num_slices)
+    path = orays.line_integrals(rho.numpy(), np.broadcast_to(src, (120, 3)), pixels.reshape(-1, 3), lo,
+                                np.array(grid.deltas), n).reshape(2, 10, 12)
+    _, _, _, nh = orays.panel_frame(40.0, 70.0, angle)
+    counting = lambda e, th: np.ones((len(e),) + th.shape)
+    fl, pl, sg = oim.primary_flat(path, lib.fine_mu_t.numpy(), lib.fine_group.numpy(), lib.fine_weight.numpy(), 3,
+                                  pixels, src, nh, lib.group_energy.numpy(), counting)
+    assert rel_err(out["fluence"], fl) < F32_REL
+    assert rel_err(out["planar_fluence"], pl) < F32_REL
+    assert rel_err(out["signal"], sg.sum(0)) < F32_REL
+    spr = scatter_fraction(0.3 * out["signal"], out["signal"])
+    assert torch.allclose(spr, torch.full_like(spr, 0.3 / 1.3), rtol=1e-6)

@@ -14,6 +14,8 @@ import pytest
 
 from oracle import checks, ld, quadrature as oq, scattering, sweep as osw
 from oracle import first_collision as ofc
+from oracle import image as oim
+from oracle import rays as ormod
 
 
 def _octant_set(base=(0.3, 0.5, 0.81), replace=None):
@@ -361,3 +363,106 @@ def test_fc_correction_conservation_identities():
     free = np.abs(beta) < 1
     assert free.mean() > 0.3 and (~free).any()
     assert np.allclose(J[1][free], E[1][free], rtol=1e-10, atol=1e-12)
+
+
+# ---- collapsed-cone image and primary ------------------------------------------------------------------------------
+
+def test_image_box_matrix_properties():
+    B = oim.box_matrix(5, [2.0, 3.0, 0.5, 1.0])
+    assert np.allclose(B.sum(-1), 1.0)
+    assert np.allclose(B[2], np.eye(5)) and np.allclose(B[3], np.eye(5))  # width <= 1: unchanged
+    assert np.allclose(B[0, 2], [0, 0.25, 0.5, 0.25, 0])  # window [1.5, 3.5]
+    assert np.allclose(B[1, 0], [0.5, 0.5, 0, 0, 0])  # [-1, 2] clipped to [0, 2]
+
+
+def _flat_panel(sid, sdd, rows, cols, pitch, angle):
+    return ormod.pixel_centers(sid, sdd, rows, cols, pitch, pitch, angle, False)[:, :, 0, 0]
+
+
+@pytest.mark.parametrize("angle", [0.0, math.pi / 2])
+def test_image_single_voxel_bookkeeping(angle):
+    """sum_pixels A I = sum_k w_k E V / cos(theta_k) and planar sum_k w_k E V for a pixel-aligned unit cube in vacuum
+    (the reference's test_scatter_image.py:131-167)."""
+    from citrus_amd.scatter_solver.angular_quadrature import hemisphere_quadrature
+
+    sid, sdd = 20.0, 30.0
+    pixels = _flat_panel(sid, sdd, 40, 40, 1.0, angle)
+    _, _, _, nh = ormod.panel_frame(sid, sdd, angle)
+    q = hemisphere_quadrature(-nh, 6, 12, math.radians(55.0))
+    om, w = q.omega.numpy(), q.weight.numpy()
+    G, E0 = 2, np.array([2.0, 0.5])
+    T = np.zeros((G, 1, 3, 3, 3))
+    T[:, 0, 1, 1, 1] = E0 * math.sqrt(4 * math.pi)  # Y_0 T_0 = E in the centre voxel
+    Y = np.full((om.shape[0], 1), 1 / math.sqrt(4 * math.pi))
+    lo = np.array([-1.5, -1.5, -1.5])
+    cos0 = om @ (-nh)
+    fl, pl, _ = oim.collapsed_cone_image(lambda k: np.zeros((G, 3, 3, 3)), T, Y, np.zeros((G, 3, 3, 3)), lo,
+                                         (1.0, 1.0, 1.0), (3, 3, 3), 1, om, w, None, pixels, (1.0, 1.0),
+                                         np.zeros((3, 3, 3)), 0.0, np.zeros((om.shape[0], G)), cos0, sid, sdd, angle,
+                                         False, footprint=False)
+    assert np.allclose(fl.sum((1, 2)), E0 * (w / cos0).sum(), rtol=1e-9)
+    assert np.allclose(pl.sum((1, 2)), E0 * w.sum(), rtol=1e-9)
+
+
+def test_image_converges_to_brute_force():
+    """Collapsed-cone fluence approaches the point-to-point volume integral sum E exp(-tau)/r^2 dV as the cone set is
+    refined (test_scatter_image.py:75-127)."""
+    from citrus_amd.scatter_solver.angular_quadrature import hemisphere_quadrature, real_spherical_harmonics
+
+    sid, sdd, angle = 50.0, 60.0, 0.0
+    shape, d = (4, 4, 4), 1.0
+    lo = np.array([-2.0, -2.0, -2.0])
+    pixels = _flat_panel(sid, sdd, 8, 8, 2.0, angle)
+    _, _, _, nh = ormod.panel_frame(sid, sdd, angle)
+    x = lo[0] + d * (np.arange(4) + 0.5)
+    mu = np.broadcast_to(0.06 * (1.0 + 0.1 * x[None, None, :, None]), (1, 4, 4, 4)).copy()  # [G=1, R, C, S]
+    E = (1.0 + 0.2 * x[None, :, None] + 0.1 * x[None, None, :]) * np.ones((4, 1, 1))  # [R, C, S]
+    # brute force: 4^3 sub-points per voxel to every pixel centre
+    off = (np.arange(4) + 0.5) / 4 - 0.5
+    sub = np.stack(np.meshgrid(off, off, off, indexing="ij"), -1).reshape(-1, 3)
+    cen = np.stack(np.meshgrid(x, x, x, indexing="ij"), -1)  # [ix(y), iy(x), iz] -> need (x, y, z) per [r, c, s]
+    centres = np.stack([cen[..., 1], cen[..., 0], cen[..., 2]], -1).reshape(-1, 3)  # r -> y, c -> x
+    pts = (centres[:, None] + sub[None] * d).reshape(-1, 3)
+    Ev = np.repeat(E.reshape(-1), 64)
+    ref = np.zeros(64)
+    for i, p in enumerate(pixels.reshape(-1, 3)):
+        tau = ormod.line_integrals(mu, pts, np.broadcast_to(p, pts.shape), lo, np.array([d, d, d]), (4, 4, 4))[0]
+        r2 = ((p - pts) ** 2).sum(-1)
+        ref[i] = (Ev * np.exp(-tau) / r2).sum() * d ** 3 / 64
+    errs = []
+    for P, A in ((4, 8), (8, 16), (16, 32)):
+        q = hemisphere_quadrature(-nh, P, A, math.radians(60.0))
+        om, w = q.omega.numpy(), q.weight.numpy()
+        T = (E * math.sqrt(4 * math.pi))[None, None]  # isotropic: only the l = 0 moment
+        Y = real_spherical_harmonics(0, q.omega).numpy()
+        fl, _, _ = oim.collapsed_cone_image(lambda k: np.zeros((1, 4, 4, 4)), T, Y, mu, lo, (d, d, d), shape, 1, om,
+                                            w, None, pixels, (2.0, 2.0), np.zeros(shape), 0.0,
+                                            np.zeros((om.shape[0], 1)), om @ (-nh), sid, sdd, angle, False,
+                                            footprint=False)
+        errs.append(float(np.linalg.norm(fl.reshape(-1) - ref) / np.linalg.norm(ref)))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 0.05, errs
+
+
+def test_primary_vacuum_and_slab_closed_forms():
+    from citrus_amd.scatter_solver.cross_sections import synthetic_library
+
+    lib = synthetic_library(3, 1, num_fine_per_group=2, fine_spread=0.3)
+    sid, sdd, angle = 40.0, 70.0, 0.3
+    pixels = _flat_panel(sid, sdd, 5, 6, 0.8, angle)
+    src = ormod.source_position(sid, angle)
+    _, _, _, nh = ormod.panel_frame(sid, sdd, angle)
+    fm, fg, fw = lib.fine_mu_t.numpy(), lib.fine_group.numpy(), lib.fine_weight.numpy()
+    energy = lib.group_energy.numpy()
+    counting = lambda e, th: np.ones((len(e),) + th.shape)
+    fl, pl, sg = oim.primary_flat(np.zeros((2, 5, 6)), fm, fg, fw, 3, pixels, src, nh, energy, counting)
+    r2 = ((pixels - src) ** 2).sum(-1)
+    assert np.allclose(fl, lib.spectrum_fraction.numpy()[:, None, None] / r2[None], rtol=1e-14)
+    assert np.allclose(sg, pl) and np.all(pl <= fl)
+    L = 2.5  # uniform water path in g/cm^2
+    fl2, _, _ = oim.primary_flat(np.stack([np.full((5, 6), L), np.zeros((5, 6))]), fm, fg, fw, 3, pixels, src, nh,
+                                 energy, counting)
+    beer = np.zeros(3)
+    np.add.at(beer, fg, fw * np.exp(-fm[0] * L))
+    assert np.allclose(fl2, beer[:, None, None] / r2[None], rtol=1e-13)
+    S = np.array([1.0, 0.0, 2.0])
+    assert np.allclose(oim.scatter_fraction(S, np.array([1.0, 0.0, 6.0])), [0.5, 0.0, 0.25])
