@@ -1,0 +1,134 @@
+"""The CITRUS scatter step of one view on the HIP kernels.
+
+Keeps the interface of ``citrus/citrus.py:70-366``: ``CitrusState`` (the caller-owned accumulator Phi of the collided
+moments of a view), ``uncollided_operator``, ``first_collision_operator``, ``accumulator_update_operator``
+(``Phi_new = phi^(1) + K Phi`` in one sweep of the summed first-collision and scattering sources, plus optional extra
+sweeps ``Phi_new <- Phi_new + K (Phi_new - Phi_old)``), ``project_scatter``, ``citrus_step`` and
+``citrus_view_step`` (the uncollided fluence, the accumulator update, the signal-only scatter image and the primary).
+With the densities held fixed, N steps from an empty state equal N source iterations.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import torch
+from torch import Tensor
+
+from .scatter_solver.angular_quadrature import Quadrature
+from .scatter_solver.geometry import Grid
+from .scatter_solver.scatter_solver import MaterialFields, ScatterSolver
+from .scatter_solver.source_iteration import summed_source_function
+
+
+@dataclass
+class CitrusState:
+    """Caller-owned state of one view (citrus.py:70): angle, accumulator Phi [G, (L+1)^2, 4, Rc, Cc, Sc] (None is
+    zero), cached cone set, step count and the relative change of Phi's scalar-flux cell means per step."""
+
+    angle: Optional[float] = None
+    collided_moments: Optional[Tensor] = None
+    cone_quadrature: Optional[Quadrature] = None
+    num_steps: int = 0
+    history: list = field(default_factory=list)
+
+
+def _fields(solver: ScatterSolver, rho, grid: Grid, fields: Optional[MaterialFields]) -> MaterialFields:
+    return solver.material_fields(rho, grid) if fields is None else fields
+
+
+def _check_accumulator(solver: ScatterSolver, fields: MaterialFields, phi: Tensor) -> None:
+    ctx = solver.sweep_context(fields.grid_coarse)
+    want = (solver.num_groups, ctx.num_moments, 4) + tuple(fields.grid_coarse.shape)
+    if tuple(phi.shape) != want:
+        raise ValueError(f"collided moments must be {want}, got {tuple(phi.shape)}")
+
+
+def _check_count(name: str, count) -> None:
+    if isinstance(count, bool) or not isinstance(count, int) or count < 1:
+        raise ValueError(f"{name} must be an integer >= 1, got {count!r}")
+
+
+def uncollided_operator(solver: ScatterSolver, rho, grid: Grid, angle: float,
+                        fields: Optional[MaterialFields] = None) -> Tensor:
+    """[G, Rq, Cq, Sq] uncollided fluence G_1 at the quadrature points (citrus.py:108)."""
+    return solver.uncollided_fluence(_fields(solver, rho, grid, fields), angle)
+
+
+def first_collision_operator(solver: ScatterSolver, rho, grid: Grid, angle: float, phi_uncollided: Tensor,
+                             fields: Optional[MaterialFields] = None) -> Tensor:
+    """Once-collided moments phi^(1): one sweep of the first-collision source (citrus.py:126)."""
+    fields = _fields(solver, rho, grid, fields)
+    return solver.transport_sweep(fields, solver.first_collision_moment_source(fields, angle, phi_uncollided))
+
+
+def accumulator_update_operator(solver: ScatterSolver, rho, grid: Grid, angle: float, phi_uncollided: Tensor,
+                                collided_moments: Optional[Tensor], fields: Optional[MaterialFields] = None, *,
+                                sweeps: int = 1) -> Tensor:
+    """New accumulator ``phi^(1) + K Phi`` in one sweep of the summed sources (citrus.py:192-247); every further
+    sweep adds K applied to the last change. The scattering moments are expanded inside the sweep kernel and the
+    first-collision moments enter as the cell-major chunk source of the same sweep."""
+    _check_count("sweeps", sweeps)
+    fields = _fields(solver, rho, grid, fields)
+    ctx = solver.sweep_context(fields.grid_coarse)
+    fc = solver.first_collision_moment_source(fields, angle, phi_uncollided, ctx)
+    if collided_moments is None:
+        previous = None
+        collided = solver.transport_sweep(fields, fc)
+    else:
+        _check_accumulator(solver, fields, collided_moments)
+        previous = collided_moments.to(solver.device, solver.dtype)
+        source = summed_source_function(solver.scatter_moment_source(fields, previous, ctx), fc)
+        collided = solver.transport_sweep(fields, source)
+        del source
+    fc = None
+    for _ in range(int(sweeps) - 1):
+        change = collided if previous is None else collided - previous
+        source = solver.scatter_moment_source(fields, change, ctx)
+        previous = collided
+        collided = solver.transport_sweep(fields, source).add_(previous)
+        del source, change
+    return collided
+
+
+def project_scatter(solver: ScatterSolver, rho, grid: Grid, angle: float, phi_uncollided: Tensor,
+                    collided_moments: Tensor, fields: Optional[MaterialFields] = None,
+                    quadrature: Optional[Quadrature] = None, signal_only: bool = False) -> dict:
+    """Total scatter image of one view in one collapsed-cone pass (citrus.py:263)."""
+    fields = _fields(solver, rho, grid, fields)
+    T = solver.multiple_collision_source_moments(fields, collided_moments.to(solver.device, solver.dtype))
+    return solver.scatter_image(fields, angle, phi_uncollided, T, quadrature, signal_only=signal_only)
+
+
+def citrus_view_step(solver: ScatterSolver, state: CitrusState, fields: MaterialFields, angle: float, *,
+                     primary: bool = True, sweeps: int = 1):
+    """One CITRUS scatter step of one view (citrus.py:320-366): (scatter signal [Rd, Cd], primary signal [Rd, Cd] or
+    None, new state). Raises ``ValueError`` if ``angle`` differs from ``state.angle``."""
+    if state.angle is not None and abs(float(state.angle) - float(angle)) > 1e-12:
+        raise ValueError(f"state belongs to angle {state.angle}, got {angle}")
+    grid = fields.grid_fine
+    cone = state.cone_quadrature if state.cone_quadrature is not None else solver.cone_quadrature(grid, angle)
+    phi_uc = uncollided_operator(solver, None, grid, angle, fields)
+    new = accumulator_update_operator(solver, None, grid, angle, phi_uc, state.collided_moments, fields,
+                                      sweeps=sweeps)
+    scatter = project_scatter(solver, None, grid, angle, phi_uc, new, fields, cone, signal_only=True)["signal"]
+    del phi_uc
+    primary_signal = solver.primary(fields, angle)["signal"] if primary else None
+    if state.collided_moments is None:
+        n = float(torch.linalg.vector_norm(new[:, 0, 0].double()))
+        change = 1.0 if n > 0 else 0.0
+    else:
+        prev = state.collided_moments[:, 0, 0].to(new)
+        n, d = torch.stack([torch.linalg.vector_norm(new[:, 0, 0].double()),
+                            torch.linalg.vector_norm((new[:, 0, 0] - prev).double())]).tolist()
+        change = d / n if n > 0 else 0.0
+    new_state = CitrusState(angle=float(angle), collided_moments=new, cone_quadrature=cone,
+                            num_steps=state.num_steps + 1, history=state.history + [change])
+    return scatter, primary_signal, new_state
+
+
+def citrus_step(solver: ScatterSolver, state: CitrusState, rho, grid: Grid, angle: float,
+                fields: Optional[MaterialFields] = None):
+    """``citrus_view_step`` from the densities (citrus.py:293)."""
+    return citrus_view_step(solver, state, _fields(solver, rho, grid, fields), angle)

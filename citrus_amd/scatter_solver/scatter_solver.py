@@ -1,0 +1,246 @@
+"""Per-view scatter solver on the HIP kernels.
+
+Keeps the interface of ``ScatterSolver`` and ``MaterialFields`` (``citrus/scatter_solver/scatter_solver.py:155-1000``)
+for the operators of the per-view transport path: the transport and quadrature grids, the sweep and cone direction
+sets, the density fields, the uncollided fluence, the first-collision moment source with its sampling correction, the
+scattering-moment source, the transport sweep, source iteration, the multiple-collision source moments, the scatter
+image and the primary. Densities are coarsened by integer block averages (the conservative resampling of
+geometry.py:304 when the grids nest); library building and the attenuation-to-density models are outside this build.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+from torch import Tensor
+
+from .angular_quadrature import Quadrature, level_symmetric_quadrature, product_quadrature
+from .config import SolverConfig
+from .cross_sections import MultigroupLibrary, mu_t_volume
+from .detector import DetectorResponse, GridResponse, ideal_counting_response, no_grid
+from .geometry import ConeGeometry, Grid
+from .ray_tracing import HipProjectors
+from .scatter_image import combined_scatter_image, make_direction_context, panel_direction_set, primary_image
+from .source import SourceModel
+from .source_iteration import (MaterialTransfer, scatter_moments, scatter_source_function, source_iteration,
+                               summed_source_function)
+from .transport_sweep import SweepContext, make_sweep_context, sweep
+from .uncollided import (first_collision_moments_at_directions, first_collision_sampling_correction,
+                         present_materials, uncollided_fluence)
+
+
+def coarsen_density(rho_fine: Tensor, grid_fine: Grid, grid_coarse: Grid) -> Tensor:
+    """[M, Rc, Cc, Sc] cell-averaged densities of nested grids (scatter_solver.py:115): each coarse cell is the mean of
+    its f^3 fine voxels."""
+    if grid_coarse == grid_fine:
+        return rho_fine
+    fs = [c // n if n and c % n == 0 else 0 for c, n in zip(grid_fine.shape, grid_coarse.shape)]
+    f = fs[0]
+    if f < 1 or any(x != f for x in fs) or abs(grid_coarse.delta_voxel - f * grid_fine.delta_voxel) > 1e-9 * f:
+        raise ValueError(f"grid {grid_coarse.shape} does not nest the fine grid {grid_fine.shape} by an integer factor")
+    M = int(rho_fine.shape[0])
+    Rc, Cc, Sc = grid_coarse.shape
+    return rho_fine.reshape(M, Rc, f, Cc, f, Sc, f).mean(dim=(2, 4, 6))
+
+
+@dataclass
+class MaterialFields:
+    """Density fields of a volume on the fine, transport and quadrature grids (scatter_solver.py:155)."""
+
+    grid_fine: Grid
+    grid_coarse: Grid
+    grid_quadrature: Grid
+    q: int
+    rho_fine: Tensor
+    rho_coarse: Tensor
+    rho_quadrature: Tensor
+    mu_t_coarse: Tensor
+    transfer_coarse: MaterialTransfer
+    traced_materials: Optional[tuple] = None
+
+    @property
+    def num_materials(self) -> int:
+        return int(self.rho_coarse.shape[0])
+
+    @property
+    def mu_s_l_coarse(self) -> MaterialTransfer:
+        """The transfer of the transport cells in the factored form ``rho_m sigma_m`` (scatter_moments accepts it)."""
+        return self.transfer_coarse
+
+
+class ScatterSolver:
+    """Per-view scatter solver (scatter_solver.py:314) on one GPU, float32."""
+
+    def __init__(self, library: MultigroupLibrary, config: SolverConfig, geometry: ConeGeometry, projectors=None,
+                 detector: Optional[DetectorResponse] = None, grid_response: Optional[GridResponse] = None,
+                 source: Optional[SourceModel] = None, device=None, dtype: torch.dtype = torch.float32) -> None:
+        if dtype != torch.float32:
+            raise ValueError("the HIP path computes in float32")
+        self.library = library
+        self.config = config.validate()
+        self.geometry = geometry
+        self.projectors = projectors if projectors is not None else HipProjectors()
+        self.detector = detector if detector is not None else ideal_counting_response()
+        self.grid_response = grid_response if grid_response is not None else no_grid()
+        self.source = source if source is not None else SourceModel()
+        self.device = torch.device(device if device is not None else "cuda")
+        self.dtype = dtype
+        self.L = int(config.legendre_order)
+        if library.legendre_order < self.L:
+            raise ValueError(f"library Legendre order {library.legendre_order} < {self.L}")
+        self._transport_grids: dict = {}
+        self._sweep_contexts: dict = {}
+        self._cone_quadratures: dict = {}
+        self._direction_contexts: dict = {}
+        self._sigma = library.mu_s_l[..., : self.L + 1].to(self.device, torch.float32).contiguous()
+
+    # ---- grids and direction sets ----------------------------------------------------------------------------
+    def responses(self) -> tuple:
+        return self.detector, self.grid_response
+
+    @property
+    def num_groups(self) -> int:
+        return self.library.num_groups
+
+    def quadrature_points(self, grid_fine: Grid) -> int:
+        return self.config.quadrature_points(max(grid_fine.deltas))
+
+    def transport_grid(self, grid_fine: Grid) -> Grid:
+        g = self._transport_grids.get(grid_fine)
+        if g is None:
+            g = self._transport_grids[grid_fine] = Grid.covering(grid_fine, self.config.transport_cell_cm)
+        return g
+
+    def quadrature_grid(self, grid_fine: Grid) -> Grid:
+        return self.transport_grid(grid_fine).refine(self.quadrature_points(grid_fine))
+
+    def sweep_quadrature(self) -> Quadrature:
+        c = self.config
+        if c.sweep_quadrature == "level_symmetric":
+            return level_symmetric_quadrature(c.sn_order)
+        return product_quadrature(c.num_polar, c.num_azimuthal)
+
+    def sweep_context(self, grid_coarse: Grid) -> SweepContext:
+        ctx = self._sweep_contexts.get(grid_coarse)
+        if ctx is None:
+            ctx = self._sweep_contexts[grid_coarse] = make_sweep_context(grid_coarse, self.sweep_quadrature(), self.L,
+                                                                         device=self.device)
+        return ctx
+
+    def cone_quadrature(self, grid: Grid, angle: float) -> Quadrature:
+        key = (grid, float(angle))
+        q = self._cone_quadratures.get(key)
+        if q is None:
+            q = self._cone_quadratures[key] = panel_direction_set(
+                self.geometry, angle, self.config, max_polar_angle=self.transport_grid(grid), device=self.device,
+                dtype=self.dtype)
+        return q
+
+    def direction_context(self, fields: MaterialFields, angle: float, quadrature: Optional[Quadrature] = None):
+        quadrature = self.cone_quadrature(fields.grid_fine, angle) if quadrature is None else quadrature
+        key = (id(quadrature), float(angle))
+        hit = self._direction_contexts.get(key)
+        if hit is not None and hit.quadrature is quadrature:
+            return hit
+        ctx = make_direction_context(quadrature, self.geometry, angle, self.detector, self.grid_response,
+                                     self.library, self.device, self.dtype, chunk=self.config.direction_chunk)
+        self._direction_contexts[key] = ctx
+        return ctx
+
+    def sweep_direction_chunk(self, fields: MaterialFields) -> int:
+        """Directions per octant per sweep chunk: all of them when the cell-major source chunk
+        [8 dc, N, G, 4] fits ``config.sweep_buffer_bytes``, else the largest that fits (at least 1)."""
+        ctx = self.sweep_context(fields.grid_coarse)
+        per_dir = 8 * ctx.num_cells * self.num_groups * 4 * 4
+        return max(1, min(ctx.directions_per_octant, int(self.config.sweep_buffer_bytes // per_dir), 32))
+
+    # ---- fields ----------------------------------------------------------------------------------------------
+    def material_fields(self, rho_fine: Tensor, grid_fine: Grid) -> MaterialFields:
+        """Density fields of fine densities on the transport and quadrature grids (scatter_solver.py:658)."""
+        if tuple(rho_fine.shape[1:]) != tuple(grid_fine.shape):
+            raise ValueError(f"rho spatial shape {tuple(rho_fine.shape[1:])} != grid shape {grid_fine.shape}")
+        if int(rho_fine.shape[0]) != self.library.num_materials:
+            raise ValueError(f"rho has {int(rho_fine.shape[0])} materials, the library {self.library.num_materials}")
+        rho = rho_fine.to(self.device, torch.float32).contiguous()
+        coarse = self.transport_grid(grid_fine)
+        q = self.quadrature_points(grid_fine)
+        gq = coarse.refine(q)
+        rho_c = coarsen_density(rho, grid_fine, coarse).contiguous()
+        rho_q = rho_c if gq == coarse else coarsen_density(rho, grid_fine, gq).contiguous()
+        traced = present_materials(rho_q)
+        return MaterialFields(grid_fine=grid_fine, grid_coarse=coarse, grid_quadrature=gq, q=q, rho_fine=rho,
+                              rho_coarse=rho_c, rho_quadrature=rho_q,
+                              mu_t_coarse=mu_t_volume(self.library, rho_c).contiguous(),
+                              transfer_coarse=MaterialTransfer(rho_c, self._sigma),
+                              traced_materials=tuple(traced) if traced else (0,))
+
+    # ---- operators -------------------------------------------------------------------------------------------
+    def uncollided_fluence(self, fields: MaterialFields, angle: float) -> Tensor:
+        """[G, Rq, Cq, Sq] uncollided fluence at the quadrature points (scatter_solver.py:745)."""
+        t = self.config.uncollided_trace_grid
+        rho, grid = ((fields.rho_fine, fields.grid_fine) if t == "fine" else
+                     (fields.rho_quadrature, fields.grid_quadrature) if t == "quadrature" else
+                     (fields.rho_coarse, fields.grid_coarse))
+        return uncollided_fluence(self.library, rho, grid, self.geometry, angle, self.source, self.projectors,
+                                  points=fields.grid_quadrature, materials=fields.traced_materials)
+
+    def first_collision_sampling_correction(self, fields: MaterialFields, angle: float, phi_uncollided: Tensor,
+                                            sweep_context: Optional[SweepContext] = None):
+        ctx = self.sweep_context(fields.grid_coarse) if sweep_context is None else sweep_context
+        return first_collision_sampling_correction(self.library, fields.rho_quadrature, phi_uncollided,
+                                                   fields.grid_quadrature, self.geometry, angle,
+                                                   ctx.quadrature.omega, ctx.quadrature.weight)
+
+    def first_collision_moment_source(self, fields: MaterialFields, angle: float, phi_uncollided: Tensor,
+                                      sweep_context: Optional[SweepContext] = None):
+        """Cell-major sweep source of the corrected, restricted first-collision source (scatter_solver.py:774)."""
+        ctx = self.sweep_context(fields.grid_coarse) if sweep_context is None else sweep_context
+        omega = ctx.quadrature.omega.to(self.device, torch.float32)
+        correction = self.first_collision_sampling_correction(fields, angle, phi_uncollided, ctx)
+
+        def source_function(direction_index: Tensor, out: Optional[Tensor] = None) -> Tensor:
+            directions = omega.index_select(0, direction_index.to(omega.device))
+            return first_collision_moments_at_directions(self.library, fields.rho_quadrature, phi_uncollided,
+                                                         fields.grid_quadrature, self.geometry, angle, directions,
+                                                         correction, fields.q, out=out, cell_major=True)
+
+        source_function.cell_major = True
+        return source_function
+
+    def scatter_moment_source(self, fields: MaterialFields, phi: Tensor,
+                              sweep_context: Optional[SweepContext] = None):
+        """Scattering source of collided moments, ``T = scatter_moments(mu_s_l, phi)`` (scatter_solver.py:850)."""
+        ctx = self.sweep_context(fields.grid_coarse) if sweep_context is None else sweep_context
+        return scatter_source_function(ctx, scatter_moments(fields.mu_s_l_coarse, phi))
+
+    def transport_sweep(self, fields: MaterialFields, source_function) -> Tensor:
+        return sweep(self.sweep_context(fields.grid_coarse), fields.mu_t_coarse, source_function,
+                     direction_chunk=self.sweep_direction_chunk(fields))
+
+    def run_source_iteration(self, fields: MaterialFields, first_collision_source_function):
+        c = self.config
+        return source_iteration(self.sweep_context(fields.grid_coarse), fields.mu_t_coarse, fields.mu_s_l_coarse,
+                                first_collision_source_function, tolerance=c.source_iteration_tolerance,
+                                max_iterations=c.max_source_iterations,
+                                direction_chunk=self.sweep_direction_chunk(fields))
+
+    def multiple_collision_source_moments(self, fields: MaterialFields, phi_collided: Tensor) -> Tensor:
+        """[G, (L+1)^2, Rc, Cc, Sc] cell-mean moments T of the multiple-collision source (scatter_solver.py:899)."""
+        return scatter_moments(fields.mu_s_l_coarse, phi_collided)[:, :, 0]
+
+    def scatter_image(self, fields: MaterialFields, angle: float, phi_uncollided: Tensor,
+                      scatter_source_moments: Tensor, quadrature: Optional[Quadrature] = None, context=None,
+                      signal_only: bool = False) -> dict:
+        quadrature = self.cone_quadrature(fields.grid_fine, angle) if quadrature is None else quadrature
+        context = self.direction_context(fields, angle, quadrature) if context is None else context
+        return combined_scatter_image(self.library, fields.rho_quadrature, phi_uncollided, fields.grid_quadrature,
+                                      fields.q, scatter_source_moments, self.L, self.geometry, angle, quadrature,
+                                      fields.mu_t_coarse, self.projectors, self.detector, self.grid_response,
+                                      context, chunk=self.config.direction_chunk, grid_coarse=fields.grid_coarse,
+                                      signal_only=signal_only)
+
+    def primary(self, fields: MaterialFields, angle: float) -> dict:
+        return primary_image(self.library, fields.rho_fine, fields.grid_fine, self.geometry, angle, self.source,
+                             self.projectors, self.detector, self.grid_response)

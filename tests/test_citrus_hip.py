@@ -1,0 +1,117 @@
+"""The CITRUS view step on the HIP kernels (GPU).
+
+Reference tests mirrored: N CITRUS steps from an empty state equal N source-iteration sweeps
+(tests/test_citrus_step.py:43); plus the accumulator of two steps against the FP64 oracle's restatement of the whole
+step (uncollided fluence, corrected first-collision moments, sweeps of the summed sources). FP32 bar ``F32_REL``."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from citrus_amd.citrus import CitrusState, citrus_view_step, first_collision_operator, uncollided_operator
+from citrus_amd.scatter_solver.config import SolverConfig
+from citrus_amd.scatter_solver.cross_sections import seeded_library
+from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
+from citrus_amd.scatter_solver.scatter_solver import ScatterSolver
+from citrus_amd.utilities import ellipsoid_phantom
+from oracle import first_collision as ofc
+from oracle import quadrature as oq
+from oracle import rays as orays
+from oracle import scattering as osc
+from oracle import sweep as osw
+from oracle import uncollided as ounc
+from tests.conftest import gpu_device
+from tests.helpers import F32_REL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(dev):
+    library = seeded_library(4, 3, 120.0, 20.0, seed=31, num_eta=129)
+    config = SolverConfig(legendre_order=3, sweep_quadrature="level_symmetric", sn_order=4, transport_cell_cm=1.0,
+                          quadrature_point_spacing_cm=0.5, cone_num_polar=3, cone_num_azimuthal=6,
+                          max_source_iterations=3, source_iteration_tolerance=1e-12)
+    geometry = ConeGeometry(30.0, 50.0, 16, 16, 0.5, 0.5)
+    solver = ScatterSolver(library, config, geometry, device=dev)
+    grid = Grid((8, 8, 8), 0.5)
+    rho = ellipsoid_phantom(grid, seed=3).to(dev)
+    return solver, grid, rho
+
+
+def test_citrus_steps_equal_source_iteration():
+    dev = gpu_device()
+    solver, grid, rho = _solver(dev)
+    fields = solver.material_fields(rho, grid)
+    assert fields.q == 2 and fields.grid_coarse.shape == (4, 4, 4)
+    angle = 0.4
+    state = CitrusState()
+    signals = []
+    for _ in range(3):
+        scatter, primary, state = citrus_view_step(solver, state, fields, angle)
+        signals.append(scatter)
+    assert state.num_steps == 3 and len(state.history) == 3 and state.history[0] == 1.0
+    assert state.history[1] > state.history[2] > 0.0
+    phi_uc = uncollided_operator(solver, None, grid, angle, fields)
+    result = solver.run_source_iteration(fields, solver.first_collision_moment_source(fields, angle, phi_uc))
+    assert result.num_iterations == 3
+    assert rel_err(state.collided_moments, result.phi_total) < F32_REL
+    assert bool((primary > 0).all()) and bool((signals[-1] > 0).all())
+    assert float(signals[-1].sum()) > float(signals[0].sum())  # more collision orders, more scatter
+    with pytest.raises(ValueError):
+        citrus_view_step(solver, state, fields, angle + 0.1)
+
+
+def test_two_view_steps_match_oracle():
+    dev = gpu_device()
+    solver, grid, rho = _solver(dev)
+    fields = solver.material_fields(rho, grid)
+    angle = 0.9
+    _, _, s1 = citrus_view_step(solver, CitrusState(), fields, angle, primary=False)
+    _, _, s2 = citrus_view_step(solver, s1, fields, angle, primary=False)
+    lib, geo = solver.library, solver.geometry
+    G = lib.num_groups
+    gq, gc = fields.grid_quadrature, fields.grid_coarse
+    rho_q = fields.rho_quadrature.double().cpu().numpy().reshape(2, -1)
+    rho_c = fields.rho_coarse.double().cpu().numpy()
+    # uncollided fluence at the quadrature points, traced on the quadrature grid
+    pts = gq.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    src = orays.source_position(geo.source_iso_dist, angle)
+    paths = orays.line_integrals(rho_q.reshape(2, *gq.shape), np.broadcast_to(src, pts.shape), pts,
+                                 np.array(gq.lower_corner()), np.array(gq.deltas),
+                                 (gq.num_cols, gq.num_rows, gq.num_slices))
+    phi_uc = ounc.uncollided_fluence(paths, pts, lib.fine_mu_t.numpy(), lib.fine_group.numpy(),
+                                     lib.fine_weight.numpy(), G, geo.source_iso_dist, geo.source_detector_dist,
+                                     geo.num_det_rows, geo.num_det_channels, geo.delta_det_row,
+                                     geo.delta_det_channel, angle, False)
+    # corrected, restricted first-collision source at the sweep directions
+    om, w = oq.level_symmetric_quadrature(4)
+    ou = ofc.primary_directions(pts, src)
+    S = ofc.source_at_directions(lib.mu_s_eta.numpy(), None, rho_q, phi_uc, ou, om)
+    I = ofc.quadrature_sums(S, ou, om, w)
+    E = ofc.exact_moments(lib.mu_s_l.numpy(), rho_q, phi_uc, 1)
+    scale, beta = ofc.correction_factors(I, E)
+    fc = ofc.restrict_ld(ofc.apply_correction(S, scale, beta, ou, om).reshape(om.shape[0], G, *gq.shape), 2)
+    mu_c = np.einsum("mg,mijk->gijk", lib.mu_t.numpy(), rho_c)
+    Y = oq.real_spherical_harmonics(3, om)
+    psi1 = osw.sweep(gc.shape, gc.deltas, om, mu_c, fc)
+    phi1 = osw.moments(psi1, w, Y)
+    assert rel_err(s1.collided_moments, phi1) < F32_REL
+    mu_s = np.einsum("mabl,mijk->ablijk", lib.mu_s_l.numpy()[..., :4], rho_c)
+    T = osc.scatter_moments(mu_s, phi1)
+    psi2 = osw.sweep(gc.shape, gc.deltas, om, mu_c, fc + osw.sources_from_moments(T, Y))
+    phi2 = osw.moments(psi2, w, Y)
+    assert rel_err(s2.collided_moments, phi2) < F32_REL
+
+
+def test_first_collision_operator_is_the_first_step():
+    dev = gpu_device()
+    solver, grid, rho = _solver(dev)
+    fields = solver.material_fields(rho, grid)
+    phi_uc = uncollided_operator(solver, None, grid, 0.2, fields)
+    phi1 = first_collision_operator(solver, None, grid, 0.2, phi_uc, fields)
+    _, _, s1 = citrus_view_step(solver, CitrusState(), fields, 0.2, primary=False)
+    assert rel_err(s1.collided_moments, phi1) < 1e-6

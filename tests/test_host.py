@@ -1,0 +1,87 @@
+"""Host-side setup of the view step (CPU): configuration, cone direction sets, detector tables, density coarsening."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from citrus_amd.scatter_solver.angular_quadrature import hemisphere_quadrature
+from citrus_amd.scatter_solver.config import SolverConfig
+from citrus_amd.scatter_solver.detector import (absorber_response, detector_angle, energy_weighted_response,
+                                                ideal_counting_response, no_grid)
+from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
+from citrus_amd.scatter_solver.scatter_image import covering_polar_angle, panel_direction_set
+from citrus_amd.scatter_solver.scatter_solver import coarsen_density
+
+
+def test_config_validation_and_quadrature_points():
+    c = SolverConfig(transport_cell_cm=0.4, quadrature_point_spacing_cm=0.5)
+    assert c.quadrature_points(0.2) == 1  # configs[0]: q = min(max(1, round(0.8)), max(1, round(2))) = 1
+    assert SolverConfig(transport_cell_cm=1.0, quadrature_point_spacing_cm=0.5).quadrature_points(0.5) == 2
+    for bad in (dict(legendre_order=8), dict(sweep_quadrature="gauss"), dict(uncollided_trace_grid="medium"),
+                dict(max_source_iterations=0), dict(direction_chunk=0)):
+        with pytest.raises(ValueError):
+            SolverConfig(**bad).validate()
+
+
+def test_cone_set_weights_and_axis():
+    axis = torch.tensor([0.3, -0.9, 0.2], dtype=torch.float64)
+    q = hemisphere_quadrature(axis, 5, 10, 0.7, tangent_axes=([1.0, 0.0, 0.0], [0.0, 0.0, 1.0]))
+    a = axis / axis.norm()
+    assert abs(float(q.weight.sum()) - 2 * math.pi * (1 - math.cos(0.7))) < 1e-12
+    cosines = q.omega @ a
+    assert float(cosines.min()) > math.cos(0.7) and torch.allclose(q.omega.norm(dim=-1), torch.ones(50, dtype=torch.float64))
+    assert q.cell_width.shape == (50, 2) and bool((q.cell_width > 0).all())
+    # a cell's gnomonic width is about its angular size: wider cells far from the axis
+    assert float(q.cell_width[-1].max()) > float(q.cell_width[0].max())
+
+
+def test_covering_polar_angle_contains_the_box():
+    grid = Grid((6, 8, 5), 0.7)
+    geometry = ConeGeometry(40.0, 60.0, 20, 24, 0.8, 0.8)
+    angle = 0.6
+    cover = covering_polar_angle(grid, geometry, angle)
+    g = torch.Generator().manual_seed(0)
+    lo, ext = torch.tensor(grid.lower_corner(), dtype=torch.float64), torch.tensor(grid.extent, dtype=torch.float64)
+    r = lo + torch.rand(300, 3, generator=g, dtype=torch.float64) * ext
+    uh, vh, nh = (torch.tensor(a, dtype=torch.float64) for a in geometry.panel_axes(angle))
+    pc = torch.tensor(geometry.panel_center(angle), dtype=torch.float64)
+    hu, hv = geometry.panel_half_extent()
+    uv = (2 * torch.rand(300, 2, generator=g, dtype=torch.float64) - 1) * torch.tensor([hu, hv], dtype=torch.float64)
+    p = pc + uv[:, :1] * uh + uv[:, 1:] * vh
+    d = p[:, None] - r[None]
+    theta = torch.arccos((d @ -nh) / torch.linalg.norm(d, dim=-1))
+    assert float(theta.max()) < cover
+    q = panel_direction_set(geometry, angle, (4, 8), max_polar_angle=grid, dtype=torch.float64)
+    assert float((q.omega @ -nh).min()) >= math.cos(cover) - 1e-12
+
+
+def test_detector_tables():
+    e = torch.tensor([30.0, 60.0])
+    th = torch.tensor([[0.0, 0.5], [1.0, 1.4]])
+    assert torch.equal(ideal_counting_response().at(e, th), torch.ones(2, 2, 2))
+    assert torch.allclose(energy_weighted_response().at(e, th), e[:, None, None].expand(2, 2, 2))
+    mu = lambda en: 5.0 * (30.0 / en) ** 2.5
+    d = absorber_response(0.05, 4.5, mu, energies=np.array([20.0, 30.0, 60.0, 100.0]),
+                          thetas=np.radians(np.linspace(0, 80, 81)))
+    got = d.at(torch.tensor([30.0], dtype=torch.float64), torch.tensor([math.radians(40.0)], dtype=torch.float64))
+    exact = 30.0 * (1 - math.exp(-mu(30.0) * 4.5 * 0.05 / math.cos(math.radians(40.0))))
+    assert abs(float(got) - exact) < 1e-12  # on the nodes the table is exact
+    assert torch.equal(no_grid().at_energy(e, "cpu", torch.float64), torch.ones(2, dtype=torch.float64))
+    geo = ConeGeometry(40.0, 60.0, 4, 4, 1.0, 1.0)
+    nh = torch.tensor(geo.panel_axes(0.3)[2], dtype=torch.float64)
+    assert float(detector_angle(-nh[None], geo, 0.3)) < 1e-7
+
+
+def test_coarsen_density_block_means():
+    fine = Grid((4, 6, 2), 0.5)
+    coarse = Grid((2, 3, 1), 1.0)
+    rho = torch.arange(2 * 48, dtype=torch.float64).reshape(2, 4, 6, 2)
+    c = coarsen_density(rho, fine, coarse)
+    assert c.shape == (2, 2, 3, 1) and float(c[1, 1, 2, 0]) == float(rho[1, 2:4, 4:6, 0:2].mean())
+    assert torch.allclose(c.sum() * 8, rho.sum())
+    with pytest.raises(ValueError):
+        coarsen_density(rho, fine, Grid((3, 3, 1), 1.0))
