@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -32,7 +32,7 @@ class SweepArgs(ctypes.Structure):
         ("G", c_int), ("nsh", c_int), ("D8", c_int), ("d0", c_int), ("dc", c_int), ("V", c_int),
         ("accumulate", c_int), ("write_psi", c_int),
         ("mu_t", c_void), ("T", c_void), ("source", c_void), ("phi", c_void),
-        ("Y", c_void), ("Yw", c_void), ("cdir", c_void), ("faces", c_void),
+        ("Y", c_void), ("Yw", c_void), ("cdir", c_void), ("faces", c_void), ("source_layout", c_int),
     ]
 
 

@@ -81,6 +81,8 @@ struct LaunchArgs {
   int G, D8, d0, dc, V;
   int first;      // this launch belongs to the first pass and the first chunk: first visits store phi
   int write_psi;  // psi overwrites the explicit source in place
+  int src_layout;  // 0: cell-major chunk [8 dc][N][G][4]; 1: group-major [G][N][M][4], M = 8 D8
+  int M;
   int ntask;
   const int4* tasks;  // (octant | first_in_pair << 3 | slot << 4, block row, block col, block slice)
   const float* mu;
@@ -269,7 +271,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
           }
           float4* sp = nullptr;
           if (HAS_SRC) {
-            sp = reinterpret_cast<float4*>(src_v + (((int64_t)(o * dc + my_d) * a.N + cell) * a.G + g) * 4);
+            sp = reinterpret_cast<float4*>(
+                src_v + (a.src_layout == 0 ? (((int64_t)(o * dc + my_d) * a.N + cell) * a.G + g) * 4
+                                           : (((int64_t)g * a.N + cell) * a.M + (o * a.D8 + a.d0 + my_d)) * 4));
             const float4 q = *sp;
             s0 += q.x; s1 += q.y; s2 += q.z; s3 += q.w;
           }
@@ -371,7 +375,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// Matrix-core variant for the moment-source sweep (T given, no explicit source), B = 4, NSH <= 36, dc <= 32.
+// Matrix-core variant for the moment-source sweep (T given, plus an optional group-major explicit source), B = 4,
+// NSH <= 36, dc <= 32.
 //
 // One wave sweeps one (view, octant, group, block) task. Both contractions of a cell plane run on
 // v_mfma_f32_16x16x4_f32 in full FP32, four cells (16 rows: cell x LD component) per M-tile:
@@ -412,7 +417,7 @@ __device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, unsigned of
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
 }
 
-template <int NSH, int DC4>
+template <int NSH, int DC4, bool HAS_SRC>
 __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   constexpr int B = 4;
   constexpr PlaneTable<B> ptab{};
@@ -457,6 +462,10 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       const_cast<float*>(a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4), 0, a.N * (int)CB, kBufferWord3);
   const __amdgpu_buffer_rsrc_t phi_r = __builtin_amdgcn_make_buffer_rsrc(
       a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4, 0, a.N * (int)CB, kBufferWord3);
+  // explicit source, group-major [G][N][M][4]: the 16 directions of one cell in a D tile are 256 contiguous bytes
+  const __amdgpu_buffer_rsrc_t src_r = __builtin_amdgcn_make_buffer_rsrc(
+      HAS_SRC ? a.src + (int64_t)v * a.sview + (int64_t)g * a.N * a.M * 4 : a.phi, 0, HAS_SRC ? a.N * a.M * 16 : 0,
+      kBufferWord3);
 
   // ---- slot tables, incoming faces (all requested before the first is used), direction fragments ------------------
   {
@@ -579,6 +588,19 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
         pold[mt][t] = buf_load4(phi_r, use ? (unsigned)cells[mt] * CB + 16u * n : kDrop);
       }
     }
+    f4 srcv[3][NT];  // explicit source of (cell 4 mt + l/16, direction 16 nt + l%16)
+    if (HAS_SRC) {
+#pragma unroll
+      for (int mt = 0; mt < 3; ++mt) {
+        if (mt * 4 >= P) break;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int d = nt * 16 + r16;
+          const bool use = cells[mt] >= 0 && d < dc;
+          srcv[mt][nt] = buf_load4(src_r, use ? ((unsigned)cells[mt] * a.M + (unsigned)(mbase + d)) * 16u : kDrop);
+        }
+      }
+    }
 #pragma unroll
     for (int mt = 0; mt < 3; ++mt) {
       if (mt * 4 >= P) break;
@@ -609,7 +631,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
         const float pby = fy[0], yx = fy[dc], yz = fy[2 * dc];
         const float pbz = fz[0], zx = fz[dc], zy = fz[2 * dc];
         const float cx = cxs[nt], cy = cys[nt], cz = czs[nt];
-        const f4 sv = sacc[nt];
+        const f4 sv = HAS_SRC ? sacc[nt] + srcv[mt][nt] : sacc[nt];
         const float r0 = sv.x + (cx * pbx + cy * pby + cz * pbz);
         const float rx = sv.y - 3.0f * fsx * cx * pbx + (cy * yx + cz * zx);
         const float ry = sv.z - 3.0f * fsy * cy * pby + (cx * xy + cz * zy);
@@ -670,22 +692,27 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
 
 size_t mfma_lds_bytes(int dc) { return sizeof(float) * (size_t)(144 * dc + 4 * kMfmaPsiStride + 128 + 2 * dc + 64); }
 
-template <int NSH>
-hipError_t launch_mfma_nsh(const LaunchArgs& la, int nwg, hipStream_t stream) {
+template <int NSH, bool HAS_SRC>
+hipError_t launch_mfma_src(const LaunchArgs& la, int nwg, hipStream_t stream) {
   const size_t lds = mfma_lds_bytes(la.dc);
   const dim3 grid(nwg), block(64);
   switch ((la.dc + 3) / 4) {
-    case 1: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 1>), grid, block, lds, stream, la); break;
-    case 2: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 2>), grid, block, lds, stream, la); break;
-    case 3: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 3>), grid, block, lds, stream, la); break;
-    case 4: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 4>), grid, block, lds, stream, la); break;
-    case 5: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 5>), grid, block, lds, stream, la); break;
-    case 6: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 6>), grid, block, lds, stream, la); break;
-    case 7: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 7>), grid, block, lds, stream, la); break;
-    case 8: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 8>), grid, block, lds, stream, la); break;
+    case 1: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 1, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 2: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 2, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 3: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 3, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 4: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 4, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 5: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 5, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 6: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 6, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 7: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 7, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 8: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 8, HAS_SRC>), grid, block, lds, stream, la); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int NSH>
+hipError_t launch_mfma_nsh(const LaunchArgs& la, int nwg, hipStream_t stream) {
+  return la.src ? launch_mfma_src<NSH, true>(la, nwg, stream) : launch_mfma_src<NSH, false>(la, nwg, stream);
 }
 
 hipError_t launch_mfma(int nsh, const LaunchArgs& la, int nwg, hipStream_t stream) {
@@ -706,8 +733,9 @@ hipError_t launch_mfma(int nsh, const LaunchArgs& la, int nwg, hipStream_t strea
 bool use_mfma(const LaunchArgs& la, int nsh) {
   const char* sel = std::getenv("CITRUS_SWEEP_KERNEL");
   const bool vector_only = sel && std::string(sel) == "vector";
-  return !vector_only && la.T && !la.src && nsh <= 36 && la.dc <= 32 && !la.write_psi &&
-         (int64_t)la.N * nsh * 16 < (int64_t)kDrop;  // moments of one (view, group) addressable by a buffer resource
+  // moments of one (view, group), and its group-major source, addressable by a buffer resource
+  return !vector_only && la.T && (!la.src || la.src_layout == 1) && nsh <= 36 && la.dc <= 32 && !la.write_psi &&
+         (int64_t)la.N * nsh * 16 < (int64_t)kDrop && (!la.src || (int64_t)la.N * la.M * 16 < (int64_t)kDrop);
 }
 
 template <int B>
@@ -856,6 +884,8 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   if (!A.mu_t || !A.phi || !A.Y || !A.Yw || !A.cdir || !A.faces) return fail(CITRUS_ERR_ARG, "NULL table or buffer");
   if (!A.T && !A.source) return fail(CITRUS_ERR_ARG, "need T moments and/or an explicit source");
   if (A.write_psi && !A.source) return fail(CITRUS_ERR_ARG, "write_psi needs the explicit source buffer");
+  if (A.source_layout != 0 && A.source_layout != 1) return fail(CITRUS_ERR_ARG, "source_layout must be 0 or 1");
+  if (A.source_layout == 1 && A.write_psi) return fail(CITRUS_ERR_ARG, "write_psi needs the cell-major layout");
   const int N = plan->R * plan->C * plan->S;
   LaunchArgs la{};
   la.R = plan->R; la.C = plan->C; la.S = plan->S; la.N = N;
@@ -865,7 +895,9 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   la.mu = A.mu_t; la.T = A.T; la.src = A.source; la.phi = A.phi;
   la.Y = A.Y; la.Yw = A.Yw; la.cdir = A.cdir; la.faces = A.faces;
   la.tview = (int64_t)A.G * N * A.nsh * 4;
-  la.sview = (int64_t)8 * A.dc * N * A.G * 4;
+  la.src_layout = A.source_layout;
+  la.M = 8 * A.D8;
+  la.sview = A.source_layout == 0 ? (int64_t)8 * A.dc * N * A.G * 4 : (int64_t)A.G * N * la.M * 4;
   la.fview = (int64_t)2 * A.G * plan->nblocks * 3 * (int64_t)A.dc * plan->B * plan->B * 3;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   la.ntask = plan->launches.empty() ? 0 : plan->launches[0].second;

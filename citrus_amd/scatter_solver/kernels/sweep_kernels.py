@@ -51,7 +51,7 @@ class SweepSchedule:
 
     def run(self, *, mu: Tensor, phi: Tensor, Y: Tensor, Yw: Tensor, cdir: Tensor, G: int, nsh: int, D8: int,
             d0: int, dc: int, V: int = 1, T: Optional[Tensor] = None, source: Optional[Tensor] = None,
-            accumulate: bool = False, write_psi: bool = False) -> None:
+            accumulate: bool = False, write_psi: bool = False, source_layout: int = 0) -> None:
         """Enqueue one chunk sweep on torch's current stream (no host synchronization)."""
         N = self.shape[0] * self.shape[1] * self.shape[2]
         for name, t in (("mu", mu), ("phi", phi), ("Y", Y), ("Yw", Yw), ("cdir", cdir)):
@@ -64,15 +64,17 @@ class SweepSchedule:
                 raise ValueError("T size does not match [V, G, N, nsh, 4]")
         if source is not None:
             _hip.require_gpu_tensor(source, "source")
-            if source.numel() != V * 8 * dc * N * G * 4:
+            if source_layout == 0 and source.numel() != V * 8 * dc * N * G * 4:
                 raise ValueError("source size does not match [V, 8 dc, N, G, 4]")
+            if source_layout == 1 and source.numel() != V * G * N * 8 * D8 * 4:
+                raise ValueError("group-major source size does not match [V, G, N, 8 D8, 4]")
         if Y.shape != (8 * D8, nsh) or Yw.shape != (8 * D8, nsh) or cdir.shape != (8 * D8, 3):
             raise ValueError("direction tables do not match the quadrature")
         faces = self.faces(G, dc, V)
         args = _hip.SweepArgs(G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V, accumulate=int(accumulate),
                               write_psi=int(write_psi), mu_t=mu.data_ptr(), T=_hip.ptr(T), source=_hip.ptr(source),
                               phi=phi.data_ptr(), Y=Y.data_ptr(), Yw=Yw.data_ptr(), cdir=cdir.data_ptr(),
-                              faces=faces.data_ptr())
+                              faces=faces.data_ptr(), source_layout=int(source_layout))
         lib = _hip.library()
         with torch.cuda.device(self.device):
             _hip.check(lib.citrus_sweep_run(self.handle, ctypes.byref(args), _hip.stream_of(self.device)),

@@ -206,7 +206,17 @@ class ScatterSolver:
                                                          fields.grid_quadrature, self.geometry, angle, directions,
                                                          correction, fields.q, out=out, cell_major=True)
 
+        def group_major() -> Tensor:
+            """[G, Rc, Cc, Sc, M, 4] moments of every sweep direction (reordered), one launch, kept for the view."""
+            if cache.get("all") is None:
+                cache["all"] = first_collision_moments_at_directions(
+                    self.library, fields.rho_quadrature, phi_uncollided, fields.grid_quadrature, self.geometry, angle,
+                    omega, correction, fields.q, layout="group_major")
+            return cache["all"]
+
+        cache: dict = {}
         source_function.cell_major = True
+        source_function.hip_group_major = group_major
         return source_function
 
     def scatter_moment_source(self, fields: MaterialFields, phi: Tensor,

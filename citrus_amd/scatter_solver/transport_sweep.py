@@ -259,9 +259,22 @@ def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: Sourc
             raise ValueError(f"moment source must carry contiguous [G, R, C, S, {nsh}, 4] moments on {device}")
     phi_store = torch.empty(G, R, C, S, nsh, 4, device=device, dtype=torch.float32)
     psi_out = torch.empty(context.num_directions, G, 4, N, device=device, dtype=torch.float32) if return_psi else None
+    # an explicit source that can hand over all directions at once, group-major [G, R, C, S, 8 D8, 4], is read by the
+    # kernel in place (no per-chunk cell-major buffer)
+    explicit = source_function if T is None else extra
+    group_major = None
+    if explicit is not None and not return_psi and hasattr(explicit, "hip_group_major"):
+        group_major = explicit.hip_group_major()
+        want = (G, R, C, S, context.num_directions, 4)
+        if tuple(group_major.shape) != want or not group_major.is_contiguous() or group_major.device != device:
+            raise ValueError(f"hip_group_major must return a contiguous {want} tensor on {device}")
     for d0 in range(0, D8, Dc):
         d1 = min(d0 + Dc, D8)
         dc = d1 - d0
+        if group_major is not None:
+            hip.schedule.run(mu=mu, phi=phi_store, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0,
+                             dc=dc, T=T, source=group_major, accumulate=d0 > 0, source_layout=1)
+            continue
         buf = None
         if T is None or extra is not None or return_psi:
             buf = torch.empty(8 * dc, N, G, 4, device=device, dtype=torch.float32)

@@ -68,12 +68,15 @@ typedef struct CitrusSweepArgs {
   int write_psi;    /* 1: the angular-fluence moments overwrite `source` in place (needs source) */
   const float* mu_t;     /* [G][N] */
   const float* T;        /* [V][G][N][nsh][4] scattering-source moments, or NULL */
-  float* source;         /* [V][8*dc][N][G][4] explicit source chunk, or NULL (in/out when write_psi) */
+  float* source;         /* explicit source in source_layout, or NULL (in/out when write_psi) */
   float* phi;            /* [V][G][N][nsh][4] fluence moments (output) */
   const float* Y;        /* [8*D8][nsh] */
   const float* Yw;       /* [8*D8][nsh] */
   const float* cdir;     /* [8*D8][3] */
   float* faces;          /* scratch of citrus_sweep_face_floats(plan, G, dc, V) floats */
+  int source_layout;     /* 0: cell-major chunk [V][8*dc][N][G][4] (the reference's sweep buffer);
+                          * 1: group-major [V][G][N][8*D8][4] of every direction, indexed by the reordered direction
+                          *    m = o*D8 + d (not with write_psi) */
 } CitrusSweepArgs;
 
 /* Sweep one direction chunk of every octant, all groups and views, through the grid with vacuum boundaries, and

@@ -203,3 +203,42 @@ def test_backend_errors():
         sweep(ctx, mu.double(), direction_source_function(src.double()))
     with pytest.raises(ValueError):
         make_sweep_context(grid, q, 1, device=dev, backend="bogus")
+
+
+@pytest.mark.parametrize("kernel", ["mfma", "vector"])
+@pytest.mark.parametrize("chunk", [None, 3])
+def test_group_major_explicit_source_with_moments(kernel, chunk, monkeypatch):
+    """Moment source plus an explicit source handed over group-major for every direction (the CITRUS step's summed
+    source) equals the cell-major chunk path and the oracle, on both kernels and in direction chunks."""
+    from citrus_amd.scatter_solver.source_iteration import summed_source_function
+
+    monkeypatch.setenv("CITRUS_SWEEP_KERNEL", kernel)
+    dev = gpu_device()
+    shape, G, L = (6, 5, 7), 2, 3
+    grid = Grid(shape, 0.5)
+    q = level_symmetric_quadrature(8)
+    ctx = make_sweep_context(grid, q, L, device=dev)
+    gen = torch.Generator().manual_seed(17)
+    mu = (torch.rand(G, *shape, generator=gen) + 0.3).to(dev)
+    T = (torch.randn(G, ctx.num_moments, 4, *shape, generator=gen) * 0.2).to(dev)
+    extra_cm = torch.randn(ctx.num_directions, *shape, G, 4, generator=gen).to(dev)  # [M, R, C, S, G, 4] reordered
+
+    def plain(index, out=None):
+        return out.copy_(extra_cm.index_select(0, index))
+
+    plain.cell_major = True
+
+    def grouped(index, out=None):
+        return out.copy_(extra_cm.index_select(0, index))
+
+    grouped.cell_major = True
+    grouped.hip_group_major = lambda: extra_cm.permute(4, 1, 2, 3, 0, 5).contiguous()
+    phi_gm = sweep(ctx, mu, summed_source_function(scatter_source_function(ctx, T), grouped), direction_chunk=chunk)
+    phi_cm = sweep(ctx, mu, summed_source_function(scatter_source_function(ctx, T), plain), direction_chunk=chunk)
+    assert rel_err(phi_gm, phi_cm) < F32_REL
+    om, w = q.omega[ctx.permutation.cpu()].numpy(), q.weight[ctx.permutation.cpu()].numpy()
+    Y = oq.real_spherical_harmonics(L, om)
+    src = osw.sources_from_moments(T.double().cpu().numpy(), Y) + \
+        extra_cm.double().cpu().permute(0, 4, 5, 1, 2, 3).numpy()
+    psi = osw.sweep(shape, grid.deltas, om, mu.double().cpu().numpy(), src)
+    assert rel_err(phi_gm, osw.moments(psi, w, Y)) < F32_REL
