@@ -1,12 +1,15 @@
 #!/usr/bin/env python
-"""Benchmark of the per-view transport step on one MI355X: scatter view-sweeps/s at 64^3 x 16 groups x S12 (P5).
+"""Benchmark of the per-view transport step on one MI355X: CITRUS scatter view steps/s at 64^3 x 16 groups x S12.
 
-Workload (the batched circular-scan configuration): a seeded 256^3 ellipsoid phantom at 0.08 cm, coarsened to the
-64^3 transport grid (0.32 cm cells); 16 groups over 140-20 keV from a seeded two-material library with P5 transfer;
-the S12 level-symmetric set (168 directions); V views per launch sequence. One step of one view is the CITRUS
-accumulator update's sweep: scattering-source moments T = K Phi_prev of the view's accumulator (HIP), then one LD
-sweep of all 168 directions x 16 groups x 64^3 cells whose source is expanded from T in the kernel and whose angular
-fluence is reduced to the new P5 moments in the kernel. Inputs are resident in HBM before the timed region.
+Workload (the batched circular-scan configuration): a seeded 256^3 ellipsoid phantom at 0.08 cm (water cylinder, bone
+and air ellipsoids), the 64^3 transport grid of 0.32 cm cells (one quadrature point per cell); 16 groups over 140-20
+keV from a seeded two-material P5 library; the S12 level-symmetric sweep set (168 directions); the 16 x 32 cone set of
+the scatter image on a 256 x 256 curved panel (0.12 cm pixels, SID 100 cm, SDD 150 cm); V views of a 64-view circle,
+each with its own CITRUS state. One view step is the CITRUS step of one view: the uncollided trace, the corrected
+first-collision source, one sweep of the summed first-collision and scattering sources (the accumulator update), the
+signal-only collapsed-cone scatter image and the primary. `value` counts completed view steps per second with every
+input resident in HBM; the sweep-only rate (scattering moments plus one fused sweep per view, V views per launch
+sequence) and the sweep kernel's roofline are reported beside it.
 
 Prints one JSON line (rank 0) with the metric, the roofline of the sweep kernel (HIP events on the launch stream)
 and the CPU baseline (the FP64 oracle's algorithm run in float32 on a sample of the same workload, on the host).
@@ -16,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -29,7 +33,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--views", type=int, default=4, help="views per launch sequence")
+    ap.add_argument("--views", type=int, default=4, help="views per step (each with its own CITRUS state)")
+    ap.add_argument("--sweep-only", action="store_true", help="time the sweep-only step alone")
     ap.add_argument("--cells", type=int, default=64, help="transport cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-directions", type=int, default=8, help="directions in the CPU-baseline sample")
@@ -139,6 +144,27 @@ def profiled_traffic(kernel: str, views: int):
     return None
 
 
+def build_view_solver(args, device, rho_fine):
+    """Solver, material fields and view angles of the view-step workload (the grids of ``build_problem``)."""
+    from citrus_amd.scatter_solver.config import SolverConfig
+    from citrus_amd.scatter_solver.cross_sections import seeded_library
+    from citrus_amd.scatter_solver.detector import energy_weighted_response
+    from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
+    from citrus_amd.scatter_solver.scatter_solver import ScatterSolver
+
+    n = args.cells
+    fine = Grid((4 * n, 4 * n, 4 * n), 0.08 * 64 / n)
+    config = SolverConfig(legendre_order=5, sweep_quadrature="level_symmetric", sn_order=12,
+                          transport_cell_cm=0.32 * 64 / n, quadrature_point_spacing_cm=0.5, cone_num_polar=16,
+                          cone_num_azimuthal=32, direction_chunk=8)
+    geometry = ConeGeometry(100.0, 150.0, 256, 256, 0.12, 0.12, use_curved_detector=True)
+    solver = ScatterSolver(seeded_library(16, 5, 140.0, 20.0, seed=99), config, geometry,
+                           detector=energy_weighted_response(), device=device)
+    fields = solver.material_fields(rho_fine, fine)
+    angles = [2.0 * math.pi * k / 64 for k in range(args.views)]
+    return solver, fields, angles
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -149,41 +175,77 @@ def main():
         sys.exit(2)
     import torch
 
+    from citrus_amd.citrus import CitrusState, citrus_view_step
+    from citrus_amd.scatter_solver.geometry import Grid
     from citrus_amd.scatter_solver.source_iteration import scatter_moments_views
     from citrus_amd.scatter_solver.transport_sweep import sweep_views
+    from citrus_amd.utilities import ellipsoid_phantom
 
     device = torch.device("cuda:0")
-    ctx, mu_t, transfer, phi_a, library, rho, coarse = build_problem(args, device)
+    V = args.views
+    stream = torch.cuda.current_stream(device)
+    n = args.cells
+    rho_fine = ellipsoid_phantom(Grid((4 * n, 4 * n, 4 * n), 0.08 * 64 / n), seed=1234)
+
+    # ---- CITRUS view steps (the metric) --------------------------------------------------------------------
+    view = None
+    if not args.sweep_only:
+        solver, fields, angles = build_view_solver(args, device, rho_fine.to(device, torch.float32))
+        states = [CitrusState() for _ in range(V)]
+
+        def view_step():
+            out = []
+            for v in range(V):
+                scatter, primary, states[v] = citrus_view_step(solver, states[v], fields, angles[v])
+                out.append((scatter, primary))
+            return out
+
+        for _ in range(max(1, args.warmup)):  # the first step of a view has no accumulator yet
+            view_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(args.steps):
+            images = view_step()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        view = {"wall": time.perf_counter() - t0, "ms": start.elapsed_time(stop)}
+        scatter, primary = images[0]
+        spr = float((scatter.sum() / primary.sum()).item())
+        assert all(bool(torch.isfinite(s.collided_moments).all()) for s in states), "non-finite accumulator"
+        view["spr"] = spr
+        view["history"] = [round(h, 6) for h in states[0].history]
+        del solver, fields, states, images
+        torch.cuda.empty_cache()
+
+    # ---- sweep only: scattering moments + one fused sweep per view, V views per launch sequence -------------
+    args_sweep = argparse.Namespace(**vars(args))
+    ctx, mu_t, transfer, phi_a, library, rho, coarse = build_problem(args_sweep, device)
     phi_b = torch.empty_like(phi_a)
     T = torch.empty_like(phi_a)
-
-    def step(src, dst):
-        scatter_moments_views(transfer, src, out=T)
-        sweep_views(ctx, mu_t, T, out=dst)
-
-    for _ in range(args.warmup):
-        step(phi_a, phi_b)
+    for _ in range(1):
+        scatter_moments_views(transfer, phi_a, out=T)
+        sweep_views(ctx, mu_t, T, out=phi_b)
         phi_a, phi_b = phi_b, phi_a
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream(device)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    sweep_steps = 2 if view is not None else args.steps
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(sweep_steps)]
+    s_start, s_stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for s in range(args.steps):
+    s_start.record(stream)
+    for s in range(sweep_steps):
         scatter_moments_views(transfer, phi_a, out=T)
         ev[s][0].record(stream)
         sweep_views(ctx, mu_t, T, out=phi_b)
         ev[s][1].record(stream)
         phi_a, phi_b = phi_b, phi_a
-    stop.record(stream)
+    s_stop.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    total_ms = start.elapsed_time(stop)
-    sweep_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps  # per step = V view-sweeps
+    sweep_wall = time.perf_counter() - t0
+    sweep_total_ms = s_start.elapsed_time(s_stop)
+    sweep_ms = sum(a.elapsed_time(b) for a, b in ev) / sweep_steps  # per step = V view-sweeps
     assert bool(torch.isfinite(phi_a).all()), "non-finite moments"
-    V = args.views
     G, nsh = 16, ctx.num_moments
     N = ctx.num_cells
     # compulsory bytes of one view-sweep in the sweep kernel: moments T in and phi out (float32) plus mu_t; the
@@ -193,23 +255,34 @@ def main():
     achieved = alg_bytes * V / (sweep_ms * 1e-3) / 1e9
     kernel = f"ld_sweep_mfma_kernel<{nsh}, {(ctx.directions_per_octant + 3) // 4}>"
     traffic = profiled_traffic(kernel, V)
+    sweep_rate = V * sweep_steps / (sweep_total_ms * 1e-3)
+    if view is not None:
+        value, ms_per_step, wall = V * args.steps / (view["ms"] * 1e-3), view["ms"] / args.steps, view["wall"]
+        workload = ("CITRUS view step: uncollided trace + corrected first-collision source + one fused sweep of the "
+                    "summed sources (S12, P5) + signal-only collapsed-cone image (16x32 cone directions, 256x256 "
+                    "curved panel) + primary")
+    else:
+        value, ms_per_step, wall = sweep_rate, sweep_total_ms / sweep_steps, sweep_wall
+        workload = "sweep only: HIP scattering moments + fused LD sweep (S12, P5)"
     result = {
         "metric": METRIC,
-        "value": V * args.steps / (total_ms * 1e-3),
+        "value": value,
         "unit": "view-sweeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": total_ms / args.steps,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded ellipsoid phantom, seeded 2-material P5 library)",
-        "config": {"workload": "plain view-sweep step: HIP scattering moments + fused LD sweep (S12, P5)",
+        "data": "synthetic (seeded 256^3 ellipsoid phantom, seeded 2-material P5 library)",
+        "config": {"workload": workload,
                    "grid": f"{coarse.shape[0]}^3 cells at {coarse.delta_voxel:.2f} cm", "groups": G,
-                   "directions": ctx.num_directions, "moments": nsh, "views_per_launch": V,
-                   "sweep_ms_per_view": sweep_ms / V, "wall_s": wall},
+                   "directions": ctx.num_directions, "moments": nsh, "views_per_step": V,
+                   "sweep_only_view_sweeps_per_s": sweep_rate, "sweep_ms_per_view": sweep_ms / V,
+                   "wall_s": wall,
+                   **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "alg_bytes_per_view_sweep": alg_bytes,

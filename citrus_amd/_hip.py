@@ -71,6 +71,10 @@ _SIGNATURES = {
                                    c_void, c_i64, c_i64, c_void]),
     "citrus_ray_downstream": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_void, c_int,
                                       c_void, c_void]),
+    "citrus_ray_crossing_sequence_bytes": (c_i64, [c_int, c_int, c_int]),
+    "citrus_ray_crossing_sequences": (c_int, [c_int, c_int, c_int, c_void, c_void, c_int, c_void, c_void]),
+    "citrus_ray_downstream_shared": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_void,
+                                             c_int, c_void, c_void, c_int, c_void, c_void]),
     "citrus_group_transmission": (c_int, [c_void, c_int, c_i64, c_void, c_void, c_void, c_int, c_int, c_void, c_void,
                                           c_void, c_void]),
     "citrus_uncollided_fluence": (c_int, [c_void, c_int, c_void, c_void, c_void, c_void, c_void, c_int, c_int,

@@ -368,6 +368,177 @@ int set_panel(RayJob& J, const double* panel20, const int* panel_ints) {
   return CITRUS_OK;
 }
 
+// ---- downstream optical depth from a shared crossing sequence (ray_tracing_kernels.py:518-639) ---------------------
+// Rays that start at voxel centres along one direction are translates of each other by whole voxels, so they cross
+// the same sequence of (voxel offset, chord) until they leave the grid. One thread per direction traverses it once
+// from the centre of a voxel of an unbounded lattice (float64, the plane-stepping and tie rules of trace()) into a
+// record; every voxel then sums its field over the first samples of the record that stay inside the grid, cut at its
+// own panel distance.
+//
+// Record of one direction, in 32-bit words (kSeqHeader + 3 max_len + 1 + (C + 1) + (R + 1) + (S + 1), rounded up to
+// a multiple of 4): {count, 0, 0, 0}, flat voxel offset [max_len] (int), chord [max_len] (cm, float), prefix
+// [max_len + 1] (float, sum of the chords before each sample), then per axis x, y, z the table first[m] (int,
+// m = 0..n_axis) of the first sample whose offset along that axis has magnitude >= m (count when none).
+// A voxel at index j along a moving axis leaves the grid at sample first[n - j] (positive step) or first[j + 1]
+// (negative step); its in-grid samples are [0, min over the axes).
+
+constexpr int kSeqHeader = 4;
+
+__host__ __device__ inline int seq_max_len(int R, int C, int S) { return R + C + S + 3; }
+
+__host__ __device__ inline int64_t seq_record_words(int R, int C, int S) {
+  const int64_t L = seq_max_len(R, C, S);
+  const int64_t w = kSeqHeader + 3 * L + 1 + (int64_t)(C + 1) + (R + 1) + (S + 1);
+  return (w + 3) & ~(int64_t)3;
+}
+
+__global__ void crossing_sequence_kernel(Box B, const double* __restrict__ omega, int K, int* __restrict__ records,
+                                         int64_t words) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const int R = B.n[1], C = B.n[0], S = B.n[2];
+  const int max_len = seq_max_len(R, C, S);
+  int* rec = records + (int64_t)k * words;
+  int* flat = rec + kSeqHeader;
+  float* chord = reinterpret_cast<float*>(flat + max_len);
+  float* prefix = chord + max_len;
+  int* first[3];
+  first[0] = reinterpret_cast<int*>(prefix + max_len + 1);  // x: C + 1 entries
+  first[1] = first[0] + (C + 1);                            // y: R + 1 entries
+  first[2] = first[1] + (R + 1);                            // z: S + 1 entries
+  const double* w = omega + 3 * k;
+  const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  int idx[3] = {0, 0, 0}, step[3];
+  double an[3], inv[3];
+  bool nz[3], pos[3];
+  for (int a = 0; a < 3; ++a) {
+    nz[a] = w[a] != 0.0;
+    pos[a] = w[a] > 0.0;
+    inv[a] = 1.0 / (nz[a] ? w[a] : 1.0);
+    step[a] = pos[a] ? 1 : (w[a] < 0.0 ? -1 : 0);
+    first[a][0] = 0;
+  }
+  // parameter along w of the next plane crossed on axis a, from the centre of voxel 0 of the lattice
+  auto plane = [&](int a) {
+    return nz[a] ? ((double)(idx[a] + (pos[a] ? 1 : 0)) * B.delta[a] - 0.5 * B.delta[a]) * inv[a] : kBig;
+  };
+  for (int a = 0; a < 3; ++a) an[a] = plane(a);
+  double t = 0.0, cum = 0.0;
+  int count = 0;
+  for (int s = 0; s < max_len; ++s) {
+    const double nxt = fmin(fmin(an[0], an[1]), an[2]);
+    const double seg = nxt - t;
+    if (seg > 0.0 && count < max_len) {
+      flat[count] = (idx[1] * C + idx[0]) * S + idx[2];
+      chord[count] = (float)(seg * wn);
+      prefix[count] = (float)cum;
+      cum += seg * wn;
+      ++count;
+    }
+    const bool sx = (an[0] <= an[1]) && (an[0] <= an[2]);
+    const bool sy = (an[1] < an[0]) && (an[1] <= an[2]);
+    const bool sz = (an[2] < an[0]) && (an[2] < an[1]);
+    const int a = sx ? 0 : (sy ? 1 : 2);
+    if (!(sx || sy || sz) || !nz[a]) break;  // no moving axis (w = 0)
+    idx[a] += step[a];
+    an[a] = plane(a);
+    t = nxt;
+    const int m = abs(idx[a]);
+    first[a][m] = count;  // the next stored sample is the first with this offset magnitude along a
+    if (m >= B.n[a]) break;  // past every voxel of the grid along a: no start voxel reaches further
+  }
+  prefix[count] = (float)cum;
+  for (int a = 0; a < 3; ++a)
+    for (int m = abs(idx[a]) + 1; m <= B.n[a]; ++m) first[a][m] = count;
+  rec[0] = count;
+  rec[1] = rec[2] = rec[3] = 0;
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void downstream_shared_kernel(Box B, Panel P, const float* __restrict__ vol,
+                                                                int nch, const double* __restrict__ omega,
+                                                                const int* __restrict__ records, int64_t words,
+                                                                const float* __restrict__ mix, int nout,
+                                                                float* __restrict__ out) {
+  extern __shared__ int srec[];
+  const int k = blockIdx.y;
+  const int R = B.n[1], C = B.n[0], S = B.n[2];
+  const int64_t N = (int64_t)R * C * S;
+  const int max_len = seq_max_len(R, C, S);
+  const int* rec = records + (int64_t)k * words;
+  for (int s = threadIdx.x; s < (int)words; s += blockDim.x) srec[s] = rec[s];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int* flat = srec + kSeqHeader;
+  const float* chord = reinterpret_cast<const float*>(flat + max_len);
+  const float* prefix = chord + max_len;
+  const int* first_x = reinterpret_cast<const int*>(prefix + max_len + 1);
+  const int* first_y = first_x + (C + 1);
+  const int* first_z = first_y + (R + 1);
+  const int kk = (int)(i % S), jj = (int)((i / S) % C), ii = (int)(i / ((int64_t)S * C));
+  const double* w = omega + 3 * k;
+  const double toward = -(w[0] * P.nh[0] + w[1] * P.nh[1] + w[2] * P.nh[2]);
+  float* o = out + (int64_t)k * nout * N + i;
+  if (!(toward > 0.0)) {
+    for (int g = 0; g < nout; ++g) o[(int64_t)g * N] = INFINITY;
+    return;
+  }
+  const int lx = w[0] > 0.0 ? C - jj : (w[0] < 0.0 ? jj + 1 : C);
+  const int ly = w[1] > 0.0 ? R - ii : (w[1] < 0.0 ? ii + 1 : R);
+  const int lz = w[2] > 0.0 ? S - kk : (w[2] < 0.0 ? kk + 1 : S);
+  const int n_in = min(first_x[lx], min(first_y[ly], first_z[lz]));
+  double p0[3] = {B.lo[0] + ((double)jj + 0.5) * B.delta[0], B.lo[1] + ((double)ii + 0.5) * B.delta[1],
+                  B.lo[2] + ((double)kk + 0.5) * B.delta[2]};
+  const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double tp = fmax(panel_distance(P, p0, w), 0.0) * wn;  // cm to the panel
+  const float* base = vol + i * nch;
+  float sum[CH], comp[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) { sum[c] = 0.0f; comp[c] = 0.0f; }
+  auto add = [&](int s, float len) {
+    const float* v = base + (int64_t)flat[s] * nch;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (c < nch) {  // Kahan-compensated sum
+        const float yv = v[c] * len - comp[c];
+        const float tv = sum[c] + yv;
+        comp[c] = (tv - sum[c]) - yv;
+        sum[c] = tv;
+      }
+    }
+  };
+  if ((double)prefix[n_in] * (1.0 + 1e-5) + 1e-6 < tp) {
+    // the panel lies beyond the grid exit: every in-grid sample counts in full
+#pragma unroll 4
+    for (int s = 0; s < n_in; ++s) add(s, chord[s]);
+  } else {
+    double cum = 0.0;
+    for (int s = 0; s < n_in; ++s) {
+      const double seg = (double)chord[s];
+      if (cum + seg >= tp) {
+        add(s, (float)(tp - cum));
+        break;
+      }
+      add(s, chord[s]);
+      cum += seg;
+    }
+  }
+  if (mix) {  // one output per row of the mix table: sum over the traced channels
+    for (int g = 0; g < nout; ++g) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        if (c < nch) acc = fmaf(mix[g * nch + c], sum[c], acc);
+      o[(int64_t)g * N] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (c < nch) o[(int64_t)c * N] = sum[c];
+  }
+}
+
 int finish(hipError_t e, const char* what) {
   if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_ARG, std::string(what) + ": too many rays or channels");
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -470,4 +641,69 @@ extern "C" int citrus_ray_downstream(const float* volume, int R, int C, int S, i
   J.nrays = (int64_t)R * C * S;
   J.out = out; J.oc = J.nrays; J.ok = (int64_t)nch * J.nrays;
   return finish(launch_mode<kDownstream>(J, reinterpret_cast<hipStream_t>(stream)), "citrus_ray_downstream");
+}
+
+namespace {
+int check_sequence_grid(int R, int C, int S) {
+  if (R < 1 || C < 1 || S < 1) return citrus::fail(CITRUS_ERR_ARG, "empty grid");
+  if (R > 511 || C > 511 || S > 511) return citrus::fail(CITRUS_ERR_UNSUPPORTED, "at most 511 voxels per axis");
+  return CITRUS_OK;
+}
+}  // namespace
+
+extern "C" int64_t citrus_ray_crossing_sequence_bytes(int R, int C, int S) {
+  if (R < 1 || C < 1 || S < 1 || R > 511 || C > 511 || S > 511) return -1;
+  return 4 * citrus::seq_record_words(R, C, S);
+}
+
+extern "C" int citrus_ray_crossing_sequences(int R, int C, int S, const double* box, const double* omega, int K,
+                                             void* sequences, void* stream) {
+  using namespace citrus;
+  if (int rc = check_sequence_grid(R, C, S)) return rc;
+  RayJob J;
+  if (int rc = init_job(J, nullptr, R, C, S, 1, box)) return rc;
+  if (K < 0) return fail(CITRUS_ERR_ARG, "K must be >= 0");
+  if (K == 0) return CITRUS_OK;
+  if (!omega || !sequences) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(crossing_sequence_kernel, dim3((K + 63) / 64), dim3(64), 0, s, J.box, omega, K,
+                     reinterpret_cast<int*>(sequences), seq_record_words(R, C, S));
+  return finish(hipGetLastError(), "citrus_ray_crossing_sequences");
+}
+
+extern "C" int citrus_ray_downstream_shared(const float* volume, int R, int C, int S, int nch, const double* box,
+                                            const double* panel, const int* panel_ints, const double* omega, int K,
+                                            const void* sequences, const float* mix, int nout, float* out,
+                                            void* stream) {
+  using namespace citrus;
+  if (int rc = check_sequence_grid(R, C, S)) return rc;
+  RayJob J;
+  if (int rc = init_job(J, volume, R, C, S, nch, box)) return rc;
+  if (int rc = set_panel(J, panel, panel_ints)) return rc;
+  if (K < 0 || K > 65535) return fail(CITRUS_ERR_ARG, "0..65535 directions per launch");
+  if (mix ? (nout < 1 || nout > 64) : nout != nch) return fail(CITRUS_ERR_ARG, "nout must be nch without a mix table");
+  if (K == 0) return CITRUS_OK;
+  if (!volume || !omega || !out || !sequences) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  const int64_t words = seq_record_words(R, C, S);
+  const int* rec = reinterpret_cast<const int*>(sequences);
+  const int64_t N = (int64_t)R * C * S;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)((N + 255) / 256), (unsigned)K), block(256);
+  const size_t lds = (size_t)words * 4;
+  if (nch <= 1)
+    hipLaunchKernelGGL((downstream_shared_kernel<1>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
+                       words, mix, nout, out);
+  else if (nch <= 2)
+    hipLaunchKernelGGL((downstream_shared_kernel<2>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
+                       words, mix, nout, out);
+  else if (nch <= 4)
+    hipLaunchKernelGGL((downstream_shared_kernel<4>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
+                       words, mix, nout, out);
+  else if (nch <= 8)
+    hipLaunchKernelGGL((downstream_shared_kernel<8>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
+                       words, mix, nout, out);
+  else
+    hipLaunchKernelGGL((downstream_shared_kernel<16>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
+                       words, mix, nout, out);
+  return finish(hipGetLastError(), "citrus_ray_downstream_shared");
 }

@@ -46,6 +46,11 @@ class DetectorResponse:
             raise ValueError(f"response must be [E, T] = [{self.energy_kev.numel()}, {self.theta_rad.numel()}], got "
                              f"{tuple(self.response.shape)}")
 
+    @property
+    def angle_independent(self) -> bool:
+        """True when every energy row of the table is constant in the angle."""
+        return bool((self.response == self.response[:, :1]).all())
+
     def at(self, group_energy: Tensor, theta: Tensor) -> Tensor:
         """[G, ...] response at the group energies and the angles ``theta [...]``, in the dtype of ``theta``."""
         device, dtype = theta.device, theta.dtype

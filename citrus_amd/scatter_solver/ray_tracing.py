@@ -188,6 +188,58 @@ class HipProjectors:
                                                             _hip.stream_of(mu.device)), "citrus_ray_downstream")
         return out
 
+    def crossing_sequences(self, grid: Grid, omegas: Tensor) -> Tensor:
+        """[K, bytes] device records of the voxel crossings of every direction from a voxel centre
+        (``citrus_ray_crossing_sequences``); they depend on the directions and the voxel sizes only."""
+        w = omegas.to(dtype=torch.float64).reshape(-1, 3).contiguous()
+        if w.device.type != "cuda":
+            raise ValueError("crossing sequences are built on the GPU: pass device directions")
+        R, C, S = grid.shape
+        lib = _hip.library()
+        nbytes = int(lib.citrus_ray_crossing_sequence_bytes(R, C, S))
+        if nbytes < 0:
+            raise ValueError(f"crossing sequences need 1..511 voxels per axis, got {grid.shape}")
+        K = int(w.shape[0])
+        records = torch.empty(K, nbytes, device=w.device, dtype=torch.uint8)
+        with torch.cuda.device(w.device):
+            _hip.check(lib.citrus_ray_crossing_sequences(R, C, S, _ptr(_box(grid)), w.data_ptr(), K,
+                                                         records.data_ptr(), _hip.stream_of(w.device)),
+                       "citrus_ray_crossing_sequences")
+        return records
+
+    def downstream_cumulative_mixed(self, density: Tensor, mix: Optional[Tensor], grid: Grid, geometry: ConeGeometry,
+                                    angle: float, omegas: Tensor, sequences: Optional[Tensor] = None) -> Tensor:
+        """``downstream_cumulative`` of the fields ``sum_c mix[g, c] density[c]`` ([K, G, R, C, S]; ``mix`` [G, M],
+        or None for the densities' own integrals) along one shared crossing sequence per direction
+        (``citrus_ray_downstream_shared``): only the M density channels are traced. ``sequences`` are the
+        ``crossing_sequences`` records of these directions on this grid (built here when None)."""
+        M = int(density.shape[0])
+        vol = _channel_last(density, grid)
+        w = omegas.to(device=density.device, dtype=torch.float64).reshape(-1, 3).contiguous()
+        K = int(w.shape[0])
+        R, C, S = grid.shape
+        if mix is not None:
+            mix = mix.to(device=density.device, dtype=torch.float32).contiguous()
+            if mix.dim() != 2 or int(mix.shape[1]) != M:
+                raise ValueError(f"mix must be [G, {M}], got {tuple(mix.shape)}")
+        nout = M if mix is None else int(mix.shape[0])
+        out = torch.empty((K, nout, R, C, S), device=density.device, dtype=torch.float32)
+        if K == 0:
+            return out
+        if sequences is None:
+            sequences = self.crossing_sequences(grid, w)
+        nbytes = int(_hip.library().citrus_ray_crossing_sequence_bytes(R, C, S))
+        if (tuple(sequences.shape) != (K, nbytes) or sequences.dtype != torch.uint8 or not sequences.is_contiguous()
+                or sequences.device != density.device):
+            raise ValueError(f"sequences must be contiguous uint8 [{K}, {nbytes}] on {density.device}")
+        panel, ints = _panel(geometry, angle)
+        with torch.cuda.device(density.device):
+            _hip.check(_hip.library().citrus_ray_downstream_shared(
+                vol.data_ptr(), R, C, S, M, _ptr(_box(grid)), _ptr(panel), _ptr(ints), w.data_ptr(), K,
+                sequences.data_ptr(), _hip.ptr(mix), nout, out.data_ptr(), _hip.stream_of(density.device)),
+                "citrus_ray_downstream_shared")
+        return out
+
     def oblique_project(self, field: Tensor, grid: Grid, geometry: ConeGeometry, angle: float, omegas: Tensor,
                         out: Optional[Tensor] = None) -> Tensor:
         """[K, G, rows, channels] line integrals of the field along each direction through every pixel centre."""

@@ -190,7 +190,10 @@ def make_direction_context(quadrature: Quadrature, geometry: ConeGeometry, angle
     """Precompute the emission-independent part of one view's images (scatter_image.py:542): the panel-facing
     directions (``w . -n_hat > 0`` in the image dtype), their weights, ``cos(theta0)``, the detector response at
     ``theta0`` and the signal weights ``w cos(theta0) D_g(theta0) G_g``; on a curved panel the incidence angle of
-    each direction at each channel's local normal."""
+    each direction at each channel's local normal. ``signal_group_weight`` [K, G] holds the weights by which the
+    groups of one direction collapse into one signal image: ``w cos(theta0) D_g G_g`` on a flat panel and, on a
+    curved panel with an angle-independent detector, ``D_g G_g`` (the channel factor ``planar_weight`` is applied
+    after the footprint blur)."""
     if group_energy is None:
         if library is None:
             raise ValueError("pass library or group_energy")
@@ -221,6 +224,8 @@ def make_direction_context(quadrature: Quadrature, geometry: ConeGeometry, angle
         planar = weight[:, None] * cos_c
         signal = (planar[:, None, :] * detector.at(energy, torch.arccos(cos_c)).permute(1, 0, 2)
                   * grid_t[None, :, None])  # [K, G, Cd]
+        if detector.angle_independent:  # signal = planar[k, c] * D_g G_g: the groups collapse before the image
+            group_weight = (det.t() * grid_t[None, :]).to(dtype).contiguous()  # [K, G]
     else:
         signal = (weight * cos0)[:, None] * det.t() * grid_t[None, :]  # [K, G]
         group_weight = signal.to(dtype).contiguous()
@@ -429,10 +434,13 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
                            mu_t_coarse: Optional[Tensor], projectors, detector: DetectorResponse,
                            grid_response: GridResponse, context: Optional[DirectionSetContext] = None,
                            chunk: int = 8, *, first_collision_chunk: int = 4, footprint: bool = True, pairs=None,
-                           grid_coarse: Optional[Grid] = None, signal_only: bool = False) -> dict:
+                           grid_coarse: Optional[Grid] = None, signal_only: bool = False,
+                           density_coarse: Optional[Tensor] = None) -> dict:
     """Total scatter image (first order plus multiple collision) of one view in one collapsed-cone pass
     (scatter_image.py:1141). ``scatter_source_moments`` is [G, (L+1)^2, Rc, Cc, Sc] cell means of the multiple-
-    collision source moments T, or [G, (L+1)^2, 4, Rc, Cc, Sc] whose means are used."""
+    collision source moments T, or [G, (L+1)^2, 4, Rc, Cc, Sc] whose means are used. With ``density_coarse``
+    [M, Rc, Cc, Sc] (the densities whose ``library.mu_t`` mix is ``mu_t_coarse``) the downstream optical depths come
+    from the M density channels traced along one shared crossing sequence per direction."""
     device = phi_uncollided.device
     rho = rho_quadrature.to(device=device, dtype=torch.float32)
     f = int(q)
@@ -458,7 +466,20 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
         raise ValueError(f"scatter_source_moments has {int(T.shape[1])} moments, expected {(L + 1) ** 2}")
     context = _resolve_context(context, quadrature, geometry, angle, detector, grid_response, library, mu_t_coarse,
                                chunk, None)
+    mix = None
+    if (density_coarse is not None and int(density_coarse.shape[0]) <= 16
+            and hasattr(projectors, "downstream_cumulative_mixed") and hasattr(projectors, "crossing_sequences")):
+        density_coarse = density_coarse.to(device, torch.float32).contiguous()
+        if tuple(density_coarse.shape) != (library.num_materials,) + tuple(grid_coarse.shape):
+            raise ValueError(f"density_coarse must be [{library.num_materials}, *{grid_coarse.shape}], got "
+                             f"{tuple(density_coarse.shape)}")
+        mix = library.mu_t.to(device, torch.float32).t().contiguous()  # [G, M]
     harmonics = context.spherical_harmonics(L)
+    if mix is not None:
+        key = ("crossing_sequences", tuple(grid_coarse.shape), tuple(grid_coarse.deltas))
+        if key not in context._cache:
+            context._cache[key] = projectors.crossing_sequences(grid_coarse, context.omega)
+        sequences = context._cache[key]
     nsh = int(T.shape[1])
     by_cell = T.to(device, torch.float32).transpose(0, 1).reshape(nsh, -1).contiguous()  # [N_sh, G cells]
     Rc, Cc, Sc = grid_coarse.shape
@@ -470,7 +491,11 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
     for i, directions in enumerate(context.omega_chunks):
         k = int(directions.shape[0])
         start, stop = context.chunk_bounds[i]
-        tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, directions)
+        if mix is not None:
+            tau = projectors.downstream_cumulative_mixed(density_coarse, mix, grid_coarse, geometry, angle,
+                                                         directions, sequences[start:stop])
+        else:
+            tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, directions)
         multiple = torch.mm(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
         emission, sums = _attenuated_emission(
             library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse, multiple, f,
@@ -480,8 +505,10 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
         del emission
         if fp is not None:
             image = fp.blur(image, i, fp.mean_height_from_sums(sums))
-        if group_weight is not None:
+        if group_weight is not None and context.planar_weight is None:
             acc.add_(image.sum(dim=(0, 1)))
+        elif group_weight is not None:  # curved panel, angle-independent detector: w cos(theta_c) per channel
+            acc.add_(torch.einsum("kgrc,kc->rc", image, context.planar_weight[start:stop]))
         elif signal_only:  # curved panel: the signal weights vary along the channels
             acc.add_(torch.einsum("kgrc,kgc->rc", image, context.signal_weight[start:stop]))
         else:

@@ -249,7 +249,7 @@ class ScatterSolver:
                                       fields.q, scatter_source_moments, self.L, self.geometry, angle, quadrature,
                                       fields.mu_t_coarse, self.projectors, self.detector, self.grid_response,
                                       context, chunk=self.config.direction_chunk, grid_coarse=fields.grid_coarse,
-                                      signal_only=signal_only)
+                                      signal_only=signal_only, density_coarse=fields.rho_coarse)
 
     def primary(self, fields: MaterialFields, angle: float) -> dict:
         return primary_image(self.library, fields.rho_fine, fields.grid_fine, self.geometry, angle, self.source,

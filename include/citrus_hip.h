@@ -152,6 +152,27 @@ int citrus_ray_oblique(const float* field, int64_t field_k_stride, int R, int C,
 int citrus_ray_downstream(const float* volume, int R, int C, int S, int nch, const double* box, const double* panel,
                           const int* panel_ints, const double* omega, int K, float* out, void* stream);
 
+/* Crossing sequences of K directions ([K][3] float64, device) through the lattice of the grid's voxels, one record of
+ * citrus_ray_crossing_sequence_bytes(R, C, S) bytes per direction into `sequences` (device, 16-byte aligned). Rays
+ * from voxel centres along one direction are whole-voxel translates of each other, so one float64 traversal per
+ * direction (the plane-stepping and tie rules of the other traversals) serves every voxel. Only the voxel sizes of
+ * `box` enter. At most 511 voxels per axis. The records depend on the directions and voxel sizes alone: compute them
+ * once per direction set and reuse them for every field traced along those directions. */
+int64_t citrus_ray_crossing_sequence_bytes(int R, int C, int S);
+int citrus_ray_crossing_sequences(int R, int C, int S, const double* box, const double* omega, int K, void* sequences,
+                                  void* stream);
+
+/* citrus_ray_downstream from the crossing-sequence records of the same K directions (citrus_ray_crossing_sequences):
+ * every voxel sums its field over the samples of the record that stay inside the grid, cut at its own panel distance
+ * (Kahan-compensated float32). With mix ([nout][nch] device, nout <= 64) the outputs are out[k][g][voxel] =
+ * sum_c mix[g][c] * integral of channel c, e.g. the optical depth of every group from the material densities and
+ * mu_t per unit density; without mix, nout == nch and out is as in citrus_ray_downstream. At most 65535 directions
+ * per launch. Same contract as SiddonProjectors.downstream_cumulative (ray_tracing.py:501, shared-sequence kernel
+ * ray_tracing_kernels.py:518-639) up to float32 rounding. */
+int citrus_ray_downstream_shared(const float* volume, int R, int C, int S, int nch, const double* box,
+                                 const double* panel, const int* panel_ints, const double* omega, int K,
+                                 const void* sequences, const float* mix, int nout, float* out, void* stream);
+
 /* ---------------------------------------------------------------------------------------------------------------
  * Group transmission and uncollided fluence.
  * --------------------------------------------------------------------------------------------------------------- */

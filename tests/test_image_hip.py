@@ -115,6 +115,15 @@ def test_combined_scatter_image_matches_oracle(curved, q_factor):
     assert rel_err(out["signal_per_group"], sg) < F32_REL
     assert rel_err(sig_only["signal"], sg.sum(0)) < F32_REL
     assert float(out["fluence"].min()) > 0.0
+    # optical depths from the two density channels along shared crossing sequences: same image as tracing mu_t
+    rho_c = torch.rand(2, *coarse.shape, generator=torch.Generator().manual_seed(9), dtype=torch.float64) + 0.2
+    mu_mix = torch.einsum("mg,mrcs->grcs", library.mu_t.double(), rho_c)
+    args = (library, f32(rho), f32(phi), gq, q_factor, f32(T), 2, geometry, angle, quad, f32(mu_mix), HipProjectors(),
+            det, grid_response)
+    per_group = combined_scatter_image(*args, context=ctx, grid_coarse=coarse, signal_only=True)
+    mixed = combined_scatter_image(*args, context=ctx, grid_coarse=coarse, signal_only=True,
+                                   density_coarse=f32(rho_c))
+    assert rel_err(mixed["signal"], per_group["signal"]) < F32_REL
 
 
 def test_primary_image_and_spr():

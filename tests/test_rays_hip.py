@@ -154,6 +154,74 @@ def test_projector_operations_match_oracle(curved):
         assert rel_err(got_k[k], ref_k) < F32_REL
 
 
+@pytest.mark.parametrize("curved,angle", [(False, 0.7), (True, 0.7), (False, 0.0), (True, 1.5 * math.pi)])
+def test_downstream_shared_sequence_matches_oracle(curved, angle):
+    """One crossing sequence per direction, gathered per voxel and cut at each voxel's panel distance, against the
+    FP64 per-ray traversal; with a [G, M] mix it gives the group optical depths from the density channels."""
+    dev = gpu_device()
+    grid = Grid((12, 14, 10), 0.4, 1.1, 0.9)
+    geo = _geometry(curved)
+    rng = np.random.default_rng(5)
+    rho = rng.random((2, *grid.shape)).astype(np.float32)
+    mix = (rng.random((3, 2)) * 0.4).astype(np.float32)
+    lo, delta, n = _grid_box(grid)
+    hi = lo + delta * np.array(n)
+    centers = grid.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    _, _, _, nh = orays.panel_frame(geo.source_iso_dist, geo.source_detector_dist, angle)
+    w = rng.standard_normal((7, 3))
+    w[:5] -= 2.5 * nh
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    w[5] = -nh  # along the panel normal: axis-aligned at angles that are multiples of pi / 2
+    proj = HipProjectors()
+    rho_t = torch.from_numpy(rho).to(dev)
+    got = proj.downstream_cumulative_mixed(rho_t, None, grid, geo, angle, torch.from_numpy(w)).cpu().double().numpy()
+    got_mix = proj.downstream_cumulative_mixed(rho_t, torch.from_numpy(mix), grid, geo, angle,
+                                               torch.from_numpy(w)).cpu().double().numpy()
+    diag = math.sqrt(((hi - lo) ** 2).sum()) * (1 + 1e-9) + 1e-9
+    for k in range(7):
+        if not (w[k] @ (-nh) > 0):
+            assert np.isinf(got[k]).all() and np.isinf(got_mix[k]).all()
+            continue
+        t = np.clip(orays.panel_distance(centers, w[k], geo.source_iso_dist, geo.source_detector_dist, angle, curved),
+                    0.0, diag)
+        ref = orays.line_integrals(rho, centers, centers + t[:, None] * w[k], lo, delta, n)
+        assert rel_err(got[k].reshape(2, -1), ref) < F32_REL
+        assert rel_err(got_mix[k].reshape(3, -1), mix.astype(np.float64) @ ref) < F32_REL
+    # the group fields traced channel by channel agree with the mixed densities
+    mu = torch.einsum("gm,mrcs->grcs", torch.from_numpy(mix), torch.from_numpy(rho)).contiguous().to(dev)
+    per_group = proj.downstream_cumulative(mu, grid, geo, angle, torch.from_numpy(w[:5])).cpu().double().numpy()
+    assert rel_err(got_mix[:5], per_group) < F32_REL
+    assert proj.downstream_cumulative_mixed(rho_t, None, grid, geo, angle, torch.zeros(0, 3)).shape == (0, 2, 12, 14,
+                                                                                                       10)
+
+
+def test_downstream_shared_sequence_limits():
+    dev = gpu_device()
+    geo = _geometry(False)
+    proj = HipProjectors()
+    big = Grid((1, 600, 1), 0.1)
+    w = torch.tensor([[0.0, 0.0, -1.0]], dtype=torch.float64)
+    with pytest.raises(ValueError, match="511"):
+        proj.downstream_cumulative_mixed(torch.ones(1, 1, 600, 1, device=dev), None, big, geo, 0.0, w)
+    grid = Grid((4, 4, 4), 0.5)
+    with pytest.raises(ValueError):
+        proj.downstream_cumulative_mixed(torch.ones(2, 4, 4, 4, device=dev), torch.ones(3, 4), grid, geo, 0.0, w)
+    seq = proj.crossing_sequences(grid, w.to(dev))
+    with pytest.raises(ValueError, match="sequences"):  # records of another grid
+        proj.downstream_cumulative_mixed(torch.ones(2, 4, 4, 5, device=dev), None, Grid((4, 4, 5), 0.5), geo, 0.0,
+                                         w, seq)
+    rec = seq.cpu().view(torch.int32)[0]
+    assert int(rec[0]) == 4  # straight down the slices of a 4-voxel column: 4 samples of 0.5 cm
+    assert torch.equal(rec[4:8], torch.tensor([0, -1, -2, -3], dtype=torch.int32))
+    nbytes = int(seq.shape[1])
+    lib = __import__("citrus_amd._hip", fromlist=["library"]).library()
+    buf = torch.zeros(1, nbytes, dtype=torch.uint8, device=dev)
+    vol, box, wd = torch.ones(600, device=dev), np.array([0, 0, 0, 0.1, 0.1, 0.1]), w.to(dev)
+    rc = lib.citrus_ray_downstream_shared(vol.data_ptr(), 600, 1, 1, 1, box.ctypes.data, None, None, wd.data_ptr(),
+                                          1, buf.data_ptr(), None, 1, buf.data_ptr(), None)
+    assert rc != 0 and "511" in lib.citrus_last_error().decode()
+
+
 def test_upstream_closed_form_uniform_box():
     """Uniform attenuation: upstream depth = mu * chord of the source-to-centre segment inside the box."""
     dev = gpu_device()
