@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -80,10 +80,11 @@ _SIGNATURES = {
     "citrus_uncollided_fluence": (c_int, [c_void, c_int, c_void, c_void, c_void, c_void, c_void, c_int, c_int,
                                           c_void, c_void, c_void, c_void, c_void, c_void, c_void, c_void]),
     "citrus_fc_source": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
-    "citrus_fc_sums": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void]),
+    "citrus_fc_sums": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void, c_void]),
+    "citrus_fc_correct": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_void, c_void]),
     "citrus_fc_moments": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_int, c_void, c_void]),
-    "citrus_attenuated_emission": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_void, c_int, c_void, c_void,
-                                           c_void, c_void, c_void]),
+    "citrus_attenuated_emission": (c_int, [ctypes.POINTER(FcArgs), c_void, c_int, c_void, c_void, c_void, c_int,
+                                           c_void, c_void, c_void, c_void, c_void]),
     "citrus_box_matrix": (c_int, [c_void, c_int, c_int, c_void, c_void]),
 }
 

@@ -5,13 +5,14 @@ moments of a view), ``uncollided_operator``, ``first_collision_operator``, ``acc
 (``Phi_new = phi^(1) + K Phi`` in one sweep of the summed first-collision and scattering sources, plus optional extra
 sweeps ``Phi_new <- Phi_new + K (Phi_new - Phi_old)``), ``project_scatter``, ``citrus_step`` and
 ``citrus_view_step`` (the uncollided fluence, the accumulator update, the signal-only scatter image and the primary).
-With the densities held fixed, N steps from an empty state equal N source iterations.
+With the densities held fixed, N steps from an empty state equal N source iterations. ``citrus_views_step`` runs the
+steps of several views that share the densities with their accumulator updates batched in one sweep.
 """
 
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Optional
+from typing import Optional, Sequence
 
 import torch
 from torch import Tensor
@@ -19,7 +20,8 @@ from torch import Tensor
 from .scatter_solver.angular_quadrature import Quadrature
 from .scatter_solver.geometry import Grid
 from .scatter_solver.scatter_solver import MaterialFields, ScatterSolver
-from .scatter_solver.source_iteration import summed_source_function
+from .scatter_solver.source_iteration import scatter_moments_views, summed_source_function
+from .scatter_solver.transport_sweep import sweep_views
 
 
 @dataclass
@@ -115,17 +117,69 @@ def citrus_view_step(solver: ScatterSolver, state: CitrusState, fields: Material
     scatter = project_scatter(solver, None, grid, angle, phi_uc, new, fields, cone, signal_only=True)["signal"]
     del phi_uc
     primary_signal = solver.primary(fields, angle)["signal"] if primary else None
+    new_state = CitrusState(angle=float(angle), collided_moments=new, cone_quadrature=cone,
+                            num_steps=state.num_steps + 1, history=state.history + [_change(new, state)])
+    return scatter, primary_signal, new_state
+
+
+def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fields: MaterialFields,
+                      angles: Sequence[float], *, primary: bool = True):
+    """``citrus_view_step`` of V views that share the densities, batched for the GPU: per view the uncollided
+    fluence and the corrected first-collision source (group-major, every sweep direction); the scattering moments of
+    every accumulator; then the V accumulator updates ``Phi_new = G(S_FC + K Phi)`` in one sweep with the views
+    batched in every launch (the reference loops over views one step at a time, citrus.py:1907-1970); then per view
+    the signal-only image and the primary. Returns ([(scatter, primary or None)] per view, [new state] per view);
+    each view's result equals its own ``citrus_view_step`` to float32 rounding."""
+    V = len(states)
+    if V < 1 or len(angles) != V:
+        raise ValueError(f"need one angle per state, got {len(angles)} angles for {V} states")
+    for state, angle in zip(states, angles):
+        if state.angle is not None and abs(float(state.angle) - float(angle)) > 1e-12:
+            raise ValueError(f"state belongs to angle {state.angle}, got {angle}")
+    grid = fields.grid_fine
+    ctx = solver.sweep_context(fields.grid_coarse)
+    G, nsh, M = solver.num_groups, ctx.num_moments, ctx.num_directions
+    R, C, S = fields.grid_coarse.shape
+    device = solver.device
+    cones = [st.cone_quadrature if st.cone_quadrature is not None else solver.cone_quadrature(grid, a)
+             for st, a in zip(states, angles)]
+    phi_uc = [uncollided_operator(solver, None, grid, a, fields) for a in angles]
+    src_shape = (V, G, R, C, S, M) if fields.q == 1 else (V, G, R, C, S, M, 4)
+    source = torch.empty(src_shape, device=device, dtype=torch.float32)
+    for v, a in enumerate(angles):
+        solver.first_collision_group_major(fields, a, phi_uc[v], ctx, out=source[v])
+    T = torch.empty(V, G, R, C, S, nsh, 4, device=device, dtype=torch.float32)
+    for v, st in enumerate(states):
+        if st.collided_moments is None:
+            T[v].zero_()
+        else:
+            _check_accumulator(solver, fields, st.collided_moments)
+            prev = st.collided_moments.to(device, torch.float32).permute(0, 3, 4, 5, 1, 2).contiguous()
+            scatter_moments_views(fields.transfer_coarse, prev[None], out=T[v:v + 1])
+    collided = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
+    del source
+    T = scatter_moments_views(fields.transfer_coarse, collided, out=T)  # multiple-collision source of the images
+    results, new_states = [], []
+    for v, (st, a) in enumerate(zip(states, angles)):
+        new = collided[v].permute(0, 4, 5, 1, 2, 3)
+        image = solver.scatter_image(fields, a, phi_uc[v], T[v].permute(0, 4, 5, 1, 2, 3)[:, :, 0], cones[v],
+                                     signal_only=True)["signal"]
+        primary_signal = solver.primary(fields, a)["signal"] if primary else None
+        results.append((image, primary_signal))
+        new_states.append(CitrusState(angle=float(a), collided_moments=new, cone_quadrature=cones[v],
+                                      num_steps=st.num_steps + 1, history=st.history + [_change(new, st)]))
+    return results, new_states
+
+
+def _change(new: Tensor, state: CitrusState) -> float:
+    """Relative change of the scalar-flux cell means over one step (1 from an empty state with nonzero fluence)."""
     if state.collided_moments is None:
         n = float(torch.linalg.vector_norm(new[:, 0, 0].double()))
-        change = 1.0 if n > 0 else 0.0
-    else:
-        prev = state.collided_moments[:, 0, 0].to(new)
-        n, d = torch.stack([torch.linalg.vector_norm(new[:, 0, 0].double()),
-                            torch.linalg.vector_norm((new[:, 0, 0] - prev).double())]).tolist()
-        change = d / n if n > 0 else 0.0
-    new_state = CitrusState(angle=float(angle), collided_moments=new, cone_quadrature=cone,
-                            num_steps=state.num_steps + 1, history=state.history + [change])
-    return scatter, primary_signal, new_state
+        return 1.0 if n > 0 else 0.0
+    prev = state.collided_moments[:, 0, 0].to(new)
+    n, d = torch.stack([torch.linalg.vector_norm(new[:, 0, 0].double()),
+                        torch.linalg.vector_norm((new[:, 0, 0] - prev).double())]).tolist()
+    return d / n if n > 0 else 0.0
 
 
 def citrus_step(solver: ScatterSolver, state: CitrusState, rho, grid: Grid, angle: float,

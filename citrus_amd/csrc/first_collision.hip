@@ -14,7 +14,8 @@
 //            order) and the exact moments E0, E1 (uncollided.py:357-384, :538-608; uncollided_kernels.py:637)
 //   moments  thread per (cell, direction): corrected samples S scale (1 + beta eta) at the f^3 sub-points of a cell,
 //            restricted to LD moments mean and mean(S u_a)/mean(u_a^2) (uncollided.py:311-352, :420-433, :464-535),
-//            written [K][G][4][cells], cell-major [K][cells][G][4] or group-major [G][cells][K][4].
+//            written [K][G][4][cells], cell-major [K][cells][G][4], group-major [G][cells][K][4] or, when f = 1 (the
+//            slopes vanish), group-major cell means [G][cells][K].
 
 #include <hip/hip_runtime.h>
 
@@ -31,6 +32,7 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxSlots = 4;
 constexpr int kBlock = 256;
+constexpr int kMaxTauChannels = 8;
 
 struct FcDev {
   int M, G, GP, n_eta, n_map, n_slots;
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void fc_source_kernel(FcDev d, float* __res
 template <int GMAX>
 __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* __restrict__ weights,
                                                          const float* __restrict__ mu_s_l, int lp1,
-                                                         float* __restrict__ out) {
+                                                         float* __restrict__ out, float* __restrict__ samples) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= d.P) return;
   const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
@@ -193,6 +195,7 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
       I0[g] = fmaf(w, S[g], I0[g]);
       I1[g] = fmaf(we, S[g], I1[g]);
       I2[g] = fmaf(wee, S[g], I2[g]);
+      if (samples && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[g];
     }
   }
   // exact Legendre moments l = 0, 1 of the same source (no quadrature)
@@ -226,6 +229,24 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
   }
 }
 
+// In-place sampling correction of captured samples [G][P][K]: S scale (1 + beta eta), the operations and order of
+// fc_moments_kernel at one point per cell, so the result equals its cell-mean layout bit for bit.
+__global__ __launch_bounds__(kBlock) void fc_correct_kernel(FcDev d, const float* __restrict__ scale,
+                                                           const float* __restrict__ beta,
+                                                           float* __restrict__ samples) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (g, p, k), k fastest
+  if (idx >= (int64_t)d.G * d.P * d.K) return;
+  const int k = (int)(idx % d.K);
+  const int64_t gp = idx / d.K;
+  const int p = (int)(gp % d.P);
+  const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
+  float ux, uy, uz;
+  primary_direction(d, r, c, s, ux, uy, uz);
+  const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
+  const float v = samples[idx];
+  samples[idx] = v * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+}
+
 template <int GMAX>
 __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float* __restrict__ scale,
                                                             const float* __restrict__ beta, int f, int layout,
@@ -235,8 +256,9 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
   const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= (int64_t)Nc * d.K) return;
   // group-major output runs directions fastest, the other layouts cells fastest (coalesced stores either way)
-  const int cell = layout == 2 ? (int)(idx / d.K) : (int)(idx % Nc);
-  const int k = layout == 2 ? (int)(idx % d.K) : (int)(idx / Nc);
+  const bool dir_fast = layout >= 2;
+  const int cell = dir_fast ? (int)(idx / d.K) : (int)(idx % Nc);
+  const int k = dir_fast ? (int)(idx % d.K) : (int)(idx / Nc);
   const int rc = cell / (Cc * Sc), cc = (cell / Sc) % Cc, sc = cell % Sc;
   const float ox = d.omegas[3 * k], oy = d.omegas[3 * k + 1], oz = d.omegas[3 * k + 2];
   f4 acc[GMAX];  // per group: sum of v, v u_x, v u_y, v u_z
@@ -265,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
           float v = S[g];
           const float sc = corrected ? scale[(int64_t)gg * d.P + p] : 1.0f;
           const float be = corrected ? beta[(int64_t)gg * d.P + p] : 0.0f;
-          v = v * sc * (1.0f + be * eta);
+          v = v * sc * fmaf(be, eta, 1.0f);
           acc[g] += v * f4{1.0f, u_x, u_y, u_z};
         }
       }
@@ -285,15 +307,19 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
     base = (int64_t)k * d.G * 4 * Nc + cell; gstride = 4 * (int64_t)Nc; jstride = Nc;
   } else if (layout == 1) {
     base = ((int64_t)k * Nc + cell) * d.G * 4; gstride = 4; jstride = 1;
-  } else {
+  } else if (layout == 2) {
     base = ((int64_t)cell * d.K + k) * 4; gstride = (int64_t)Nc * d.K * 4; jstride = 1;
+  } else {  // cell means only (f = 1: the slopes are zero)
+    base = (int64_t)cell * d.K + k; gstride = (int64_t)Nc * d.K; jstride = 0;
   }
 #pragma unroll
   for (int g = 0; g < GMAX; ++g) {
     if (g >= d.G) continue;
     const f4 m = f4{acc[g].x / n3, acc[g].y * slope, acc[g].z * slope, acc[g].w * slope};
     float* o = out + base + g * gstride;
-    if (jstride == 1) {
+    if (jstride == 0) {
+      *o = m.x;
+    } else if (jstride == 1) {
       *reinterpret_cast<f4*>(o) = m;
     } else {
       o[0] = m.x; o[jstride] = m.y; o[2 * jstride] = m.z; o[3 * jstride] = m.w;
@@ -307,7 +333,8 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
 // (point, direction), blockIdx.y = direction; each block also reduces sum_g e and h sum_g e over its points into
 // partial [K][blocks][2] (no atomics), the sums of the footprint's emission-weighted mean height.
 template <int GMAX>
-__global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* __restrict__ tau,
+__global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* __restrict__ tau, int tau_channels,
+                                                          const float* __restrict__ tau_mix,
                                                           const float* __restrict__ mu_c,
                                                           const float* __restrict__ multiple, int f,
                                                           const float* __restrict__ heights,
@@ -334,12 +361,29 @@ __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* 
     const float off = f > 1 ? ox * (((float)(c % f) - half) * (float)d.dx) + oy * (((float)(r % f) - half) * (float)d.dy)
                                   + oz * (((float)(s % f) - half) * (float)d.dz)
                             : 0.0f;
+    float tc[kMaxTauChannels];  // optical depths of the traced channels (with a mix table)
+    bool unreachable = false;
+    if (tau_mix) {
+#pragma unroll
+      for (int c = 0; c < kMaxTauChannels; ++c)
+        tc[c] = c < tau_channels ? tau[((int64_t)k * tau_channels + c) * Nc + cell] : 0.0f;
+      unreachable = tc[0] == INFINITY;
+    }
     float sig = 0.0f;
 #pragma unroll
     for (int g = 0; g < GMAX; ++g) {
       if (g >= d.G) continue;
       const int64_t kg = (int64_t)k * d.G + g;
-      float t = tau[kg * Nc + cell];
+      float t;
+      if (tau_mix) {
+        t = 0.0f;
+#pragma unroll
+        for (int c = 0; c < kMaxTauChannels; ++c)
+          if (c < tau_channels) t = fmaf(tau_mix[g * tau_channels + c], tc[c], t);
+        if (unreachable) t = INFINITY;
+      } else {
+        t = tau[kg * Nc + cell];
+      }
       if (f > 1) t = fmaxf(t - mu_c[(int64_t)g * Nc + cell] * off, 0.0f);
       const float e = (S[g] + multiple[kg * Nc + cell]) * expf(-t);
       esum += e;
@@ -416,7 +460,7 @@ extern "C" int citrus_fc_source(const CitrusFcArgs* args, float* out, void* stre
 }
 
 extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1,
-                              float* out, void* stream) {
+                              float* out, float* samples, void* stream) {
   using namespace citrus;
   if (int rc = check_args(args)) return rc;
   if (!weights || !mu_s_l || !out) return fail(CITRUS_ERR_ARG, "NULL weights, moments or output");
@@ -425,10 +469,26 @@ extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, co
   const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock)), block(kBlock);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
-    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), grid, block, 0, s, d, weights, mu_s_l, lp1, out);
+    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), grid, block, 0, s, d, weights, mu_s_l, lp1, out,
+                       samples);
     return 0;
   });
   return finish("first-collision sums");
+}
+
+extern "C" int citrus_fc_correct(const CitrusFcArgs* args, const float* scale, const float* beta, float* samples,
+                                 void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!scale || !beta || !samples) return fail(CITRUS_ERR_ARG, "NULL scale, beta or samples");
+  const FcDev d = make_dev(*args);
+  const int64_t n = (int64_t)d.G * d.P * d.K;
+  if (n == 0) return CITRUS_OK;
+  if ((n + kBlock - 1) / kBlock > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many samples for one launch");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(fc_correct_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, scale,
+                     beta, samples);
+  return finish("first-collision correction");
 }
 
 extern "C" int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float* beta, int factor,
@@ -439,7 +499,8 @@ extern "C" int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, c
   if ((scale == nullptr) != (beta == nullptr)) return fail(CITRUS_ERR_ARG, "scale and beta must be given together");
   if (factor < 1 || args->R % factor || args->C % factor || args->S % factor)
     return fail(CITRUS_ERR_ARG, "the point grid must be divisible by the factor");
-  if (layout < 0 || layout > 2) return fail(CITRUS_ERR_ARG, "layout must be 0, 1 or 2");
+  if (layout < 0 || layout > 3) return fail(CITRUS_ERR_ARG, "layout must be 0, 1, 2 or 3");
+  if (layout == 3 && factor != 1) return fail(CITRUS_ERR_ARG, "layout 3 (cell means) needs factor 1");
   const FcDev d = make_dev(*args);
   const int64_t n = (int64_t)(d.P / (factor * factor * factor)) * d.K;
   if (n == 0) return CITRUS_OK;
@@ -453,12 +514,15 @@ extern "C" int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, c
   return finish("first-collision moments");
 }
 
-extern "C" int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, const float* mu_t_coarse,
-                                          const float* multiple, int factor, const float* heights,
-                                          const float* signal_weight, float* out, float* partial, void* stream) {
+extern "C" int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, int tau_channels,
+                                          const float* tau_mix, const float* mu_t_coarse, const float* multiple,
+                                          int factor, const float* heights, const float* signal_weight, float* out,
+                                          float* partial, void* stream) {
   using namespace citrus;
   if (int rc = check_args(args)) return rc;
   if (!tau || !mu_t_coarse || !multiple || !out || !partial) return fail(CITRUS_ERR_ARG, "NULL input or output");
+  if (tau_mix && (tau_channels < 1 || tau_channels > kMaxTauChannels))
+    return fail(CITRUS_ERR_ARG, "1..8 optical-depth channels with a mix table");
   if (factor < 1 || args->R % factor || args->C % factor || args->S % factor)
     return fail(CITRUS_ERR_ARG, "the point grid must be divisible by the factor");
   if (args->K > 65535) return fail(CITRUS_ERR_ARG, "at most 65535 directions per launch");
@@ -467,8 +531,8 @@ extern "C" int citrus_attenuated_emission(const CitrusFcArgs* args, const float*
   const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock), (unsigned)d.K), block(kBlock);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
-    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value>), grid, block, 0, s, d, tau, mu_t_coarse, multiple,
-                       factor, heights, signal_weight, out, partial);
+    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value>), grid, block, 0, s, d, tau, tau_channels, tau_mix,
+                       mu_t_coarse, multiple, factor, heights, signal_weight, out, partial);
     return 0;
   });
   return finish("attenuated emission");

@@ -81,7 +81,7 @@ struct LaunchArgs {
   int G, D8, d0, dc, V;
   int first;      // this launch belongs to the first pass and the first chunk: first visits store phi
   int write_psi;  // psi overwrites the explicit source in place
-  int src_layout;  // 0: cell-major chunk [8 dc][N][G][4]; 1: group-major [G][N][M][4], M = 8 D8
+  int src_layout;  // 0: cell-major chunk [8 dc][N][G][4]; 1: group-major [G][N][M][4], M = 8 D8; 2: [G][N][M] means
   int M;
   int ntask;
   const int4* tasks;  // (octant | first_in_pair << 3 | slot << 4, block row, block col, block slice)
@@ -270,7 +270,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
             }
           }
           float4* sp = nullptr;
-          if (HAS_SRC) {
+          if (HAS_SRC && a.src_layout == 2) {  // cell means only: the slopes are zero
+            s0 += src_v[((int64_t)g * a.N + cell) * a.M + (o * a.D8 + a.d0 + my_d)];
+          } else if (HAS_SRC) {
             sp = reinterpret_cast<float4*>(
                 src_v + (a.src_layout == 0 ? (((int64_t)(o * dc + my_d) * a.N + cell) * a.G + g) * 4
                                            : (((int64_t)g * a.N + cell) * a.M + (o * a.D8 + a.d0 + my_d)) * 4));
@@ -462,10 +464,12 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       const_cast<float*>(a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4), 0, a.N * (int)CB, kBufferWord3);
   const __amdgpu_buffer_rsrc_t phi_r = __builtin_amdgcn_make_buffer_rsrc(
       a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4, 0, a.N * (int)CB, kBufferWord3);
-  // explicit source, group-major [G][N][M][4]: the 16 directions of one cell in a D tile are 256 contiguous bytes
+  // explicit source, group-major [G][N][M][4] (layout 1: the 16 directions of one cell in a D tile are 256 contiguous
+  // bytes) or cell means [G][N][M] (layout 2, zero slopes: 64 contiguous bytes)
+  const int src_words = a.src_layout == 2 ? 1 : 4;
   const __amdgpu_buffer_rsrc_t src_r = __builtin_amdgcn_make_buffer_rsrc(
-      HAS_SRC ? a.src + (int64_t)v * a.sview + (int64_t)g * a.N * a.M * 4 : a.phi, 0, HAS_SRC ? a.N * a.M * 16 : 0,
-      kBufferWord3);
+      HAS_SRC ? a.src + (int64_t)v * a.sview + (int64_t)g * a.N * a.M * src_words : a.phi, 0,
+      HAS_SRC ? a.N * a.M * 4 * src_words : 0, kBufferWord3);
 
   // ---- slot tables, incoming faces (all requested before the first is used), direction fragments ------------------
   {
@@ -597,7 +601,11 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
         for (int nt = 0; nt < NT; ++nt) {
           const int d = nt * 16 + r16;
           const bool use = cells[mt] >= 0 && d < dc;
-          srcv[mt][nt] = buf_load4(src_r, use ? ((unsigned)cells[mt] * a.M + (unsigned)(mbase + d)) * 16u : kDrop);
+          const unsigned e = (unsigned)cells[mt] * a.M + (unsigned)(mbase + d);
+          if (src_words == 1)
+            srcv[mt][nt] = f4{buf_load(src_r, use ? e * 4u : kDrop), 0.f, 0.f, 0.f};
+          else
+            srcv[mt][nt] = buf_load4(src_r, use ? e * 16u : kDrop);
         }
       }
     }
@@ -734,7 +742,7 @@ bool use_mfma(const LaunchArgs& la, int nsh) {
   const char* sel = std::getenv("CITRUS_SWEEP_KERNEL");
   const bool vector_only = sel && std::string(sel) == "vector";
   // moments of one (view, group), and its group-major source, addressable by a buffer resource
-  return !vector_only && la.T && (!la.src || la.src_layout == 1) && nsh <= 36 && la.dc <= 32 && !la.write_psi &&
+  return !vector_only && la.T && (!la.src || la.src_layout != 0) && nsh <= 36 && la.dc <= 32 && !la.write_psi &&
          (int64_t)la.N * nsh * 16 < (int64_t)kDrop && (!la.src || (int64_t)la.N * la.M * 16 < (int64_t)kDrop);
 }
 
@@ -884,8 +892,8 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   if (!A.mu_t || !A.phi || !A.Y || !A.Yw || !A.cdir || !A.faces) return fail(CITRUS_ERR_ARG, "NULL table or buffer");
   if (!A.T && !A.source) return fail(CITRUS_ERR_ARG, "need T moments and/or an explicit source");
   if (A.write_psi && !A.source) return fail(CITRUS_ERR_ARG, "write_psi needs the explicit source buffer");
-  if (A.source_layout != 0 && A.source_layout != 1) return fail(CITRUS_ERR_ARG, "source_layout must be 0 or 1");
-  if (A.source_layout == 1 && A.write_psi) return fail(CITRUS_ERR_ARG, "write_psi needs the cell-major layout");
+  if (A.source_layout < 0 || A.source_layout > 2) return fail(CITRUS_ERR_ARG, "source_layout must be 0, 1 or 2");
+  if (A.source_layout != 0 && A.write_psi) return fail(CITRUS_ERR_ARG, "write_psi needs the cell-major layout");
   const int N = plan->R * plan->C * plan->S;
   LaunchArgs la{};
   la.R = plan->R; la.C = plan->C; la.S = plan->S; la.N = N;
@@ -897,7 +905,8 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   la.tview = (int64_t)A.G * N * A.nsh * 4;
   la.src_layout = A.source_layout;
   la.M = 8 * A.D8;
-  la.sview = A.source_layout == 0 ? (int64_t)8 * A.dc * N * A.G * 4 : (int64_t)A.G * N * la.M * 4;
+  la.sview = A.source_layout == 0 ? (int64_t)8 * A.dc * N * A.G * 4
+                                   : (int64_t)A.G * N * la.M * (A.source_layout == 1 ? 4 : 1);
   la.fview = (int64_t)2 * A.G * plan->nblocks * 3 * (int64_t)A.dc * plan->B * plan->B * 3;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   la.ntask = plan->launches.empty() ? 0 : plan->launches[0].second;

@@ -68,6 +68,12 @@ class SweepSchedule:
                 raise ValueError("source size does not match [V, 8 dc, N, G, 4]")
             if source_layout == 1 and source.numel() != V * G * N * 8 * D8 * 4:
                 raise ValueError("group-major source size does not match [V, G, N, 8 D8, 4]")
+            if source_layout == 2 and source.numel() != V * G * N * 8 * D8:
+                raise ValueError("group-major cell-mean source size does not match [V, G, N, 8 D8]")
+            if source_layout not in (0, 1, 2):
+                raise ValueError("source_layout must be 0, 1 or 2")
+            if source_layout != 0 and write_psi:
+                raise ValueError("write_psi needs the cell-major source layout")
         if Y.shape != (8 * D8, nsh) or Yw.shape != (8 * D8, nsh) or cdir.shape != (8 * D8, 3):
             raise ValueError("direction tables do not match the quadrature")
         faces = self.faces(G, dc, V)

@@ -379,8 +379,9 @@ def supervoxel_subpoint_emission(source_quadrature: Tensor, tau_coarse: Tensor, 
 
 
 def _attenuated_emission(library, rho, phi_uncollided, grid_q, geometry, angle, directions, tau, mu_c, multiple, f,
-                         heights, signal_weight):
-    """One ``citrus_attenuated_emission`` launch: (emission [k, G or 1, Rq, Cq, Sq], sums [k, 2])."""
+                         heights, signal_weight, tau_mix=None):
+    """One ``citrus_attenuated_emission`` launch: (emission [k, G or 1, Rq, Cq, Sq], sums [k, 2]). ``tau`` is
+    [k, G, cells], or [k, M, cells] of traced densities whose group optical depths are ``tau_mix [G, M]`` mixes."""
     a, keep, _ = _fc_args(library, rho, phi_uncollided, grid_q, geometry, angle, directions)
     k, G, P = a.K, library.num_groups, grid_q.num_voxels
     device = phi_uncollided.device
@@ -392,10 +393,19 @@ def _attenuated_emission(library, rho, phi_uncollided, grid_q, geometry, angle, 
     nb = -(-P // 256)
     partial = torch.empty(k, nb, 2, device=device, dtype=torch.float32)
     h = None if heights is None else heights.to(device, torch.float32).contiguous()
+    channels = G
+    if tau_mix is not None:
+        tau_mix = tau_mix.to(device, torch.float32).contiguous()
+        channels = int(tau_mix.shape[1])
+        if tuple(tau_mix.shape) != (G, channels):
+            raise ValueError(f"tau_mix must be [{G}, channels], got {tuple(tau_mix.shape)}")
+    if int(tau.shape[1]) != channels:
+        raise ValueError(f"tau has {int(tau.shape[1])} channels, expected {channels}")
     with torch.cuda.device(device):
         _hip.check(_hip.library().citrus_attenuated_emission(
-            ctypes.byref(a), tau.data_ptr(), mu_c.data_ptr(), multiple.data_ptr(), int(f), _hip.ptr(h), _hip.ptr(sw),
-            out.data_ptr(), partial.data_ptr(), _hip.stream_of(device)), "citrus_attenuated_emission")
+            ctypes.byref(a), tau.data_ptr(), channels, _hip.ptr(tau_mix), mu_c.data_ptr(), multiple.data_ptr(),
+            int(f), _hip.ptr(h), _hip.ptr(sw), out.data_ptr(), partial.data_ptr(), _hip.stream_of(device)),
+            "citrus_attenuated_emission")
     del keep
     return out, partial.sum(1)
 
@@ -467,7 +477,7 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
     context = _resolve_context(context, quadrature, geometry, angle, detector, grid_response, library, mu_t_coarse,
                                chunk, None)
     mix = None
-    if (density_coarse is not None and int(density_coarse.shape[0]) <= 16
+    if (density_coarse is not None and int(density_coarse.shape[0]) <= 8
             and hasattr(projectors, "downstream_cumulative_mixed") and hasattr(projectors, "crossing_sequences")):
         density_coarse = density_coarse.to(device, torch.float32).contiguous()
         if tuple(density_coarse.shape) != (library.num_materials,) + tuple(grid_coarse.shape):
@@ -491,15 +501,15 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
     for i, directions in enumerate(context.omega_chunks):
         k = int(directions.shape[0])
         start, stop = context.chunk_bounds[i]
-        if mix is not None:
-            tau = projectors.downstream_cumulative_mixed(density_coarse, mix, grid_coarse, geometry, angle,
+        if mix is not None:  # density line integrals; the emission kernel mixes them into group optical depths
+            tau = projectors.downstream_cumulative_mixed(density_coarse, None, grid_coarse, geometry, angle,
                                                          directions, sequences[start:stop])
         else:
             tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, directions)
         multiple = torch.mm(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
         emission, sums = _attenuated_emission(
             library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse, multiple, f,
-            None if fp is None else fp.heights, None if group_weight is None else group_weight[start:stop])
+            None if fp is None else fp.heights, None if group_weight is None else group_weight[start:stop], mix)
         del tau, multiple
         image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, directions)
         del emission
@@ -602,12 +612,16 @@ def primary_from_path_lengths(library: MultigroupLibrary, path: Tensor, material
 
 def primary_image(library: MultigroupLibrary, rho: Tensor, grid: Grid, geometry: ConeGeometry, angle: float,
                   source: SourceModel, projectors, detector: DetectorResponse, grid_response: GridResponse,
-                  samples_per_pixel_axis: Optional[int] = None) -> dict:
-    """Primary image of one view from the density maps of the materials present (:1474)."""
+                  samples_per_pixel_axis: Optional[int] = None, materials: Optional[Sequence[int]] = None) -> dict:
+    """Primary image of one view from the density maps of the materials present (:1474); ``materials`` names them
+    when the caller already knows (else one host synchronization finds them)."""
     if int(rho.shape[0]) != library.num_materials:
         raise ValueError(f"rho has {int(rho.shape[0])} materials, the library {library.num_materials}")
-    present = present_materials(rho) or [0]
-    sel = rho.index_select(0, torch.tensor(present, device=rho.device)).contiguous()
+    present = (list(materials) if materials is not None else present_materials(rho)) or [0]
+    if present == list(range(library.num_materials)) and rho.is_contiguous():
+        sel = rho
+    else:
+        sel = rho.index_select(0, torch.tensor(present, device=rho.device)).contiguous()
     path = primary_path_lengths(projectors, sel, grid, geometry, angle, samples_per_pixel_axis)
     return primary_from_path_lengths(library, path, present, geometry, angle, source, detector, grid_response)
 

@@ -28,7 +28,7 @@ from .source_iteration import (MaterialTransfer, scatter_moments, scatter_source
                                summed_source_function)
 from .transport_sweep import SweepContext, make_sweep_context, sweep
 from .uncollided import (first_collision_moments_at_directions, first_collision_sampling_correction,
-                         present_materials, uncollided_fluence)
+                         present_materials, uncollided_fluence, first_collision_corrected_samples)
 
 
 def coarsen_density(rho_fine: Tensor, grid_fine: Grid, grid_coarse: Grid) -> Tensor:
@@ -59,6 +59,7 @@ class MaterialFields:
     mu_t_coarse: Tensor
     transfer_coarse: MaterialTransfer
     traced_materials: Optional[tuple] = None
+    fine_materials: Optional[tuple] = None
 
     @property
     def num_materials(self) -> int:
@@ -174,7 +175,8 @@ class ScatterSolver:
                               rho_coarse=rho_c, rho_quadrature=rho_q,
                               mu_t_coarse=mu_t_volume(self.library, rho_c).contiguous(),
                               transfer_coarse=MaterialTransfer(rho_c, self._sigma),
-                              traced_materials=tuple(traced) if traced else (0,))
+                              traced_materials=tuple(traced) if traced else (0,),
+                              fine_materials=tuple(present_materials(rho)))
 
     # ---- operators -------------------------------------------------------------------------------------------
     def uncollided_fluence(self, fields: MaterialFields, angle: float) -> Tensor:
@@ -193,12 +195,35 @@ class ScatterSolver:
                                                    fields.grid_quadrature, self.geometry, angle,
                                                    ctx.quadrature.omega, ctx.quadrature.weight)
 
-    def first_collision_moment_source(self, fields: MaterialFields, angle: float, phi_uncollided: Tensor,
-                                      sweep_context: Optional[SweepContext] = None):
-        """Cell-major sweep source of the corrected, restricted first-collision source (scatter_solver.py:774)."""
+    def first_collision_group_major(self, fields: MaterialFields, angle: float, phi_uncollided: Tensor,
+                                    sweep_context: Optional[SweepContext] = None, out: Optional[Tensor] = None):
+        """(source, correction): the corrected, restricted first-collision source of every sweep direction
+        (reordered), group-major [G, Rc, Cc, Sc, M, 4], or its cell means [G, Rc, Cc, Sc, M] when every cell holds
+        one quadrature point (the slopes vanish; then the samples come from the same source evaluation as the
+        correction's sums)."""
         ctx = self.sweep_context(fields.grid_coarse) if sweep_context is None else sweep_context
         omega = ctx.quadrature.omega.to(self.device, torch.float32)
+        if fields.q == 1:
+            return first_collision_corrected_samples(self.library, fields.rho_quadrature, phi_uncollided,
+                                                     fields.grid_quadrature, self.geometry, angle, omega,
+                                                     ctx.quadrature.weight, out=out)
         correction = self.first_collision_sampling_correction(fields, angle, phi_uncollided, ctx)
+        source = first_collision_moments_at_directions(self.library, fields.rho_quadrature, phi_uncollided,
+                                                       fields.grid_quadrature, self.geometry, angle, omega,
+                                                       correction, fields.q, out=out, layout="group_major")
+        return source, correction
+
+    def first_collision_moment_source(self, fields: MaterialFields, angle: float, phi_uncollided: Tensor,
+                                      sweep_context: Optional[SweepContext] = None):
+        """Cell-major sweep source of the corrected, restricted first-collision source (scatter_solver.py:774); its
+        ``hip_group_major`` attribute hands every direction at once to the sweep kernel."""
+        ctx = self.sweep_context(fields.grid_coarse) if sweep_context is None else sweep_context
+        omega = ctx.quadrature.omega.to(self.device, torch.float32)
+        cache: dict = {}
+        if fields.q == 1:
+            cache["all"], correction = self.first_collision_group_major(fields, angle, phi_uncollided, ctx)
+        else:
+            correction = self.first_collision_sampling_correction(fields, angle, phi_uncollided, ctx)
 
         def source_function(direction_index: Tensor, out: Optional[Tensor] = None) -> Tensor:
             directions = omega.index_select(0, direction_index.to(omega.device))
@@ -207,14 +232,14 @@ class ScatterSolver:
                                                          correction, fields.q, out=out, cell_major=True)
 
         def group_major() -> Tensor:
-            """[G, Rc, Cc, Sc, M, 4] moments of every sweep direction (reordered), one launch, kept for the view."""
+            """[G, Rc, Cc, Sc, M, 4] moments of every sweep direction (reordered), one launch, kept for the view;
+            with one quadrature point per cell (zero slopes) the cell means [G, Rc, Cc, Sc, M]."""
             if cache.get("all") is None:
                 cache["all"] = first_collision_moments_at_directions(
                     self.library, fields.rho_quadrature, phi_uncollided, fields.grid_quadrature, self.geometry, angle,
-                    omega, correction, fields.q, layout="group_major")
+                    omega, correction, fields.q, layout="group_major_means" if fields.q == 1 else "group_major")
             return cache["all"]
 
-        cache: dict = {}
         source_function.cell_major = True
         source_function.hip_group_major = group_major
         return source_function
@@ -253,4 +278,4 @@ class ScatterSolver:
 
     def primary(self, fields: MaterialFields, angle: float) -> dict:
         return primary_image(self.library, fields.rho_fine, fields.grid_fine, self.geometry, angle, self.source,
-                             self.projectors, self.detector, self.grid_response)
+                             self.projectors, self.detector, self.grid_response, materials=fields.fine_materials)

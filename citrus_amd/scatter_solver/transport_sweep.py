@@ -259,21 +259,26 @@ def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: Sourc
             raise ValueError(f"moment source must carry contiguous [G, R, C, S, {nsh}, 4] moments on {device}")
     phi_store = torch.empty(G, R, C, S, nsh, 4, device=device, dtype=torch.float32)
     psi_out = torch.empty(context.num_directions, G, 4, N, device=device, dtype=torch.float32) if return_psi else None
-    # an explicit source that can hand over all directions at once, group-major [G, R, C, S, 8 D8, 4], is read by the
-    # kernel in place (no per-chunk cell-major buffer)
+    # an explicit source that can hand over all directions at once, group-major [G, R, C, S, 8 D8, 4] (or its cell
+    # means [G, R, C, S, 8 D8] when the slopes vanish), is read by the kernel in place (no per-chunk buffer)
     explicit = source_function if T is None else extra
     group_major = None
+    group_layout = 1
     if explicit is not None and not return_psi and hasattr(explicit, "hip_group_major"):
         group_major = explicit.hip_group_major()
         want = (G, R, C, S, context.num_directions, 4)
-        if tuple(group_major.shape) != want or not group_major.is_contiguous() or group_major.device != device:
-            raise ValueError(f"hip_group_major must return a contiguous {want} tensor on {device}")
+        if tuple(group_major.shape) == want[:-1]:
+            group_layout = 2
+        elif tuple(group_major.shape) != want:
+            raise ValueError(f"hip_group_major must return a contiguous {want} or {want[:-1]} tensor on {device}")
+        if not group_major.is_contiguous() or group_major.device != device:
+            raise ValueError(f"hip_group_major must return a contiguous {want} or {want[:-1]} tensor on {device}")
     for d0 in range(0, D8, Dc):
         d1 = min(d0 + Dc, D8)
         dc = d1 - d0
         if group_major is not None:
             hip.schedule.run(mu=mu, phi=phi_store, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0,
-                             dc=dc, T=T, source=group_major, accumulate=d0 > 0, source_layout=1)
+                             dc=dc, T=T, source=group_major, accumulate=d0 > 0, source_layout=group_layout)
             continue
         buf = None
         if T is None or extra is not None or return_psi:
@@ -295,7 +300,7 @@ def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: Sourc
 
 
 def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out: Optional[Tensor] = None,
-                direction_chunk: Optional[int] = None) -> Tensor:
+                direction_chunk: Optional[int] = None, source: Optional[Tensor] = None) -> Tensor:
     """Sweep V views that share the material fields, each from its own scattering-source moments, in one launch
     sequence: the batched form of one ``sweep`` per view of an outer iteration (citrus.py:1920-1935).
 
@@ -305,6 +310,9 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         moments: [V, G, R, C, S, nsh, 4] float32 contiguous scattering-source moments (group-major layout).
         out: optional [V, G, R, C, S, nsh, 4] float32 output buffer.
         direction_chunk: directions per octant per launch sequence. Defaults to all.
+        source: optional explicit source added to the moments' expansion, every direction of every view:
+            [V, G, R, C, S, M, 4] LD moments or [V, G, R, C, S, M] cell means (zero slopes), M the reordered
+            directions (the summed source of the CITRUS step, citrus.py:232-236).
 
     Returns:
         [V, G, R, C, S, nsh, 4] fluence moments.
@@ -318,6 +326,18 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     if moments.dim() != 7 or tuple(moments.shape[1:]) != (G, R, C, S, nsh, 4) or not moments.is_contiguous():
         raise ValueError(f"moments must be contiguous [V, {G}, {R}, {C}, {S}, {nsh}, 4]")
     V = int(moments.shape[0])
+    layout = 0
+    if source is not None:
+        M = sweep_context.num_directions
+        base = (V, G, R, C, S, M)
+        if tuple(source.shape) == base + (4,):
+            layout = 1
+        elif tuple(source.shape) == base:
+            layout = 2
+        else:
+            raise ValueError(f"source must be {base + (4,)} or {base}, got {tuple(source.shape)}")
+        if not source.is_contiguous() or source.device != moments.device or source.dtype != torch.float32:
+            raise ValueError("source must be a contiguous float32 tensor on the moments' device")
     if out is None:
         out = torch.empty_like(moments)
     D8 = sweep_context.directions_per_octant
@@ -327,5 +347,5 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     for d0 in range(0, D8, Dc):
         dc = min(d0 + Dc, D8) - d0
         hip.schedule.run(mu=mu, phi=out, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V,
-                         T=moments, accumulate=d0 > 0)
+                         T=moments, accumulate=d0 > 0, source=source, source_layout=layout)
     return out

@@ -217,8 +217,10 @@ class FirstCollisionSamplingCorrection:
         return source * self.scale * (1.0 + self.beta[None] * eta[:, None])
 
 
-def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights) -> Tensor:
-    """[5, G, R, C, S] float32 I0, I1, I2, E0, E1 from one launch of ``citrus_fc_sums``."""
+def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights,
+                 samples: Optional[Tensor] = None) -> Tensor:
+    """[5, G, R, C, S] float32 I0, I1, I2, E0, E1 from one launch of ``citrus_fc_sums`` (which also stores the
+    uncorrected samples [G, R, C, S, K] into ``samples`` when given)."""
     if library.legendre_order < 1:
         raise ValueError("the sampling correction needs a library of Legendre order >= 1")
     a, keep, mu_s_l = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
@@ -226,9 +228,14 @@ def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, we
     if int(w.numel()) != a.K:
         raise ValueError("weights and omegas disagree in length")
     out = torch.empty((5, library.num_groups) + tuple(grid.shape), device=phi_uncollided.device, dtype=torch.float32)
+    if samples is not None:
+        want = (library.num_groups,) + tuple(grid.shape) + (a.K,)
+        if (tuple(samples.shape) != want or samples.dtype != torch.float32 or not samples.is_contiguous()
+                or samples.device != phi_uncollided.device):
+            raise ValueError(f"samples must be a contiguous float32 {want} tensor on {phi_uncollided.device}")
     with torch.cuda.device(phi_uncollided.device):
         _hip.check(_hip.library().citrus_fc_sums(ctypes.byref(a), w.data_ptr(), mu_s_l.data_ptr(),
-                                                 int(mu_s_l.shape[-1]), out.data_ptr(),
+                                                 int(mu_s_l.shape[-1]), out.data_ptr(), _hip.ptr(samples),
                                                  _hip.stream_of(phi_uncollided.device)), "citrus_fc_sums")
     del keep
     return out
@@ -276,7 +283,34 @@ def first_collision_sampling_correction(library: MultigroupLibrary, rho: Tensor,
                                             omega_uncollided=ou)
 
 
-_FC_LAYOUTS = {"moment_major": 0, "cell_major": 1, "group_major": 2}
+def first_collision_corrected_samples(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
+                                      geometry: ConeGeometry, angle: float, omegas: Tensor, weights: Tensor,
+                                      out: Optional[Tensor] = None
+                                      ) -> tuple[Tensor, FirstCollisionSamplingCorrection]:
+    """Corrected first-collision source at every point and sweep direction, group-major [G, R, C, S, K], with its
+    correction, from one evaluation of the source: ``citrus_fc_sums`` stores the samples while it sums them, the
+    factors follow in float64 (uncollided.py:436-461, :611-640) and ``citrus_fc_correct`` applies them in place.
+    With one point per cell this is ``first_collision_moments_at_directions(..., layout="group_major_means")``
+    bit for bit, at the cost of one source evaluation instead of two."""
+    K = int(omegas.reshape(-1, 3).shape[0])
+    shape = (library.num_groups,) + tuple(grid.shape) + (K,)
+    if out is None:
+        out = torch.empty(shape, device=phi_uncollided.device, dtype=torch.float32)
+    sums = _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights, samples=out).double()
+    scale, beta = first_collision_correction_factors(sums[:3], sums[3:])
+    ou = primary_directions_to_voxels(grid, geometry, angle, phi_uncollided.device, torch.float32)
+    correction = FirstCollisionSamplingCorrection(scale=scale.float().contiguous(), beta=beta.float().contiguous(),
+                                                  omega_uncollided=ou)
+    a, keep, _ = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
+    with torch.cuda.device(phi_uncollided.device):
+        _hip.check(_hip.library().citrus_fc_correct(ctypes.byref(a), correction.scale.data_ptr(),
+                                                    correction.beta.data_ptr(), out.data_ptr(),
+                                                    _hip.stream_of(phi_uncollided.device)), "citrus_fc_correct")
+    del keep
+    return out, correction
+
+
+_FC_LAYOUTS = {"moment_major": 0, "cell_major": 1, "group_major": 2, "group_major_means": 3}
 
 
 def first_collision_moments_at_directions(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
@@ -287,7 +321,8 @@ def first_collision_moments_at_directions(library: MultigroupLibrary, rho: Tenso
     """LD moments of the corrected first-collision source on the cells of the ``factor``-coarsened grid
     (uncollided.py:464-535), one ``citrus_fc_moments`` launch: [K, G, 4, Rc, Cc, Sc], or cell-major
     [K, Rc, Cc, Sc, G, 4] (the sweep's source chunk) with ``cell_major``; ``layout="group_major"`` gives
-    [G, Rc, Cc, Sc, K, 4]. ``correction`` None leaves the samples uncorrected."""
+    [G, Rc, Cc, Sc, K, 4] and, for ``factor`` 1 (where the slopes vanish), ``layout="group_major_means"`` the cell
+    means [G, Rc, Cc, Sc, K]. ``correction`` None leaves the samples uncorrected."""
     f = int(factor)
     if f < 1 or any(n % f for n in grid.shape):
         raise ValueError(f"grid shape {grid.shape} not divisible by the factor {factor}")
@@ -298,7 +333,9 @@ def first_collision_moments_at_directions(library: MultigroupLibrary, rho: Tenso
     G, K = library.num_groups, a.K
     Rc, Cc, Sc = (n // f for n in grid.shape)
     shape = {"moment_major": (K, G, 4, Rc, Cc, Sc), "cell_major": (K, Rc, Cc, Sc, G, 4),
-             "group_major": (G, Rc, Cc, Sc, K, 4)}[name]
+             "group_major": (G, Rc, Cc, Sc, K, 4), "group_major_means": (G, Rc, Cc, Sc, K)}[name]
+    if name == "group_major_means" and f != 1:
+        raise ValueError("layout 'group_major_means' needs factor 1 (the LD slopes vanish only there)")
     if out is None:
         out = torch.empty(shape, device=phi_uncollided.device, dtype=torch.float32)
     elif tuple(out.shape) != shape or not out.is_contiguous() or out.dtype != torch.float32:

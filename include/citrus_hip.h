@@ -76,7 +76,8 @@ typedef struct CitrusSweepArgs {
   float* faces;          /* scratch of citrus_sweep_face_floats(plan, G, dc, V) floats */
   int source_layout;     /* 0: cell-major chunk [V][8*dc][N][G][4] (the reference's sweep buffer);
                           * 1: group-major [V][G][N][8*D8][4] of every direction, indexed by the reordered direction
-                          *    m = o*D8 + d (not with write_psi) */
+                          *    m = o*D8 + d (not with write_psi);
+                          * 2: group-major cell means [V][G][N][8*D8], slopes zero (not with write_psi) */
 } CitrusSweepArgs;
 
 /* Sweep one direction chunk of every octant, all groups and views, through the grid with vacuum boundaries, and
@@ -228,15 +229,22 @@ int citrus_fc_source(const CitrusFcArgs* args, float* out, void* stream);
 /* Sampling-correction inputs per point: out [5][G][P] = quadrature sums I0, I1, I2 (sum_k w_k eta_k^j S_k, float32
  * in direction order) and the exact moments E0, E1 = sum_{g'<=g} sum_m rho_m mu_s_l^{g'->g}[l] phi_UC,g'
  * (first_collision_quadrature_sums and first_collision_exact_moments, uncollided.py:357-384, :538-608; the fused
- * first_collision_sums of uncollided_kernels.py:637). weights [K]; mu_s_l [M][G][G][lp1], lp1 >= 2. */
+ * first_collision_sums of uncollided_kernels.py:637). weights [K]; mu_s_l [M][G][G][lp1], lp1 >= 2. samples (or
+ * NULL): also stores the uncorrected S_FC of every point and direction, group-major [G][P][K]. */
 int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1, float* out,
-                   void* stream);
+                   float* samples, void* stream);
+
+/* In place, samples [G][P][K] (from citrus_fc_sums) *= scale (1 + beta eta) with scale, beta [G][P] and eta the
+ * cosine between Omega_UC of the point and omegas[k]. At one point per cell (factor 1) the result equals
+ * citrus_fc_moments layout 3 bit for bit, without a second evaluation of the source (uncollided.py:420-433). */
+int citrus_fc_correct(const CitrusFcArgs* args, const float* scale, const float* beta, float* samples, void* stream);
 
 /* LD moments of the corrected first-collision source on the cells of the factor-coarsened point grid
  * (first_collision_moments_at_directions, uncollided.py:464-535): samples S scale (1 + beta eta) (scale, beta
  * [G][P], or both NULL for no correction) at the f^3 sub-points of each cell, restricted to [mean, x, y, z] slopes
  * (restrict_to_linear_discontinuous_moments, uncollided.py:311). layout 0: out [K][G][4][cells]; 1: cell-major
- * [K][cells][G][4] (the sweep's source chunk); 2: group-major [G][cells][K][4]. */
+ * [K][cells][G][4] (the sweep's source chunk); 2: group-major [G][cells][K][4]; 3 (factor 1 only, where the slopes
+ * are zero): group-major cell means [G][cells][K]. */
 int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float* beta, int factor, int layout,
                       float* out, void* stream);
 
@@ -245,13 +253,15 @@ int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float*
  * first_collision_source_at_directions + the cell's multiple-collision source + supervoxel_subpoint_emission
  * (uncollided_kernels.py:713, attenuated_emission at :1138): e_g = (S_FC,g + S_MC,g) exp(-tau_j,g) with
  * tau_j = max(0, tau_cell - mu_cell w . (r_j - r_cell)). args: the first-collision inputs on the quadrature grid,
- * omegas = the chunk's K directions; tau and multiple [K][G][cells] of the factor-coarsened grid; mu_t_coarse
+ * omegas = the chunk's K directions; multiple [K][G][cells] of the factor-coarsened grid; tau [K][G][cells] with
+ * tau_mix NULL, or [K][tau_channels][cells] (tau_channels <= 8) with tau_mix [G][tau_channels], the cell's optical
+ * depth of group g then being sum_c tau_mix[g][c] tau[k][c][cell] (+inf where tau[k][0][cell] is +inf); mu_t_coarse
  * [G][cells]; heights [P] panel heights of the points (or NULL: 0); signal_weight [K][G] (or NULL). out [K][G][P], or
  * [K][P] = sum_g signal_weight e_g with signal_weight. partial [K][ceil(P / 256)][2]: per block of 256 points the sums
  * of sum_g e_g and h sum_g e_g (unweighted), which the caller reduces. */
-int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, const float* mu_t_coarse,
-                               const float* multiple, int factor, const float* heights, const float* signal_weight,
-                               float* out, float* partial, void* stream);
+int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, int tau_channels, const float* tau_mix,
+                               const float* mu_t_coarse, const float* multiple, int factor, const float* heights,
+                               const float* signal_weight, float* out, float* partial, void* stream);
 
 /* Footprint box-filter matrices out [k][N][N], B[c][j] = |[j, j+1] n W_c| / |W_c|, W_c the window of width
  * widths[m] (pixels) centred on pixel c clipped to [0, N] (box_matrix, scatter_image.py:952). */

@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 import torch
 
-from citrus_amd.citrus import CitrusState, citrus_view_step, first_collision_operator, uncollided_operator
+from citrus_amd.citrus import (CitrusState, citrus_view_step, citrus_views_step, first_collision_operator,
+                               uncollided_operator)
 from citrus_amd.scatter_solver.config import SolverConfig
 from citrus_amd.scatter_solver.cross_sections import seeded_library
 from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
@@ -30,10 +31,10 @@ from tests.helpers import F32_REL, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _solver(dev):
+def _solver(dev, spacing=0.5):
     library = seeded_library(4, 3, 120.0, 20.0, seed=31, num_eta=129)
     config = SolverConfig(legendre_order=3, sweep_quadrature="level_symmetric", sn_order=4, transport_cell_cm=1.0,
-                          quadrature_point_spacing_cm=0.5, cone_num_polar=3, cone_num_azimuthal=6,
+                          quadrature_point_spacing_cm=spacing, cone_num_polar=3, cone_num_azimuthal=6,
                           max_source_iterations=3, source_iteration_tolerance=1e-12)
     geometry = ConeGeometry(30.0, 50.0, 16, 16, 0.5, 0.5)
     solver = ScatterSolver(library, config, geometry, device=dev)
@@ -115,3 +116,29 @@ def test_first_collision_operator_is_the_first_step():
     phi1 = first_collision_operator(solver, None, grid, 0.2, phi_uc, fields)
     _, _, s1 = citrus_view_step(solver, CitrusState(), fields, 0.2, primary=False)
     assert rel_err(s1.collided_moments, phi1) < 1e-6
+
+
+@pytest.mark.parametrize("spacing,q", [(0.5, 2), (1.0, 1)])
+def test_views_step_equals_view_steps(spacing, q):
+    """The batched step of V views (one sweep launch sequence for all accumulator updates) equals each view's own
+    citrus_view_step, over two steps, with one (cell means of the source) and two quadrature points per cell."""
+    dev = gpu_device()
+    solver, grid, rho = _solver(dev, spacing)
+    fields = solver.material_fields(rho, grid)
+    assert fields.q == q
+    angles = [0.3, 1.9, 4.0]
+    batched = [CitrusState() for _ in angles]
+    single = [CitrusState() for _ in angles]
+    for _ in range(2):
+        results, batched = citrus_views_step(solver, batched, fields, angles)
+        for v, a in enumerate(angles):
+            scatter, primary, single[v] = citrus_view_step(solver, single[v], fields, a)
+            assert rel_err(batched[v].collided_moments, single[v].collided_moments) < F32_REL
+            assert rel_err(results[v][0], scatter) < F32_REL
+            assert torch.equal(results[v][1], primary)
+            assert abs(batched[v].history[-1] - single[v].history[-1]) < 1e-5
+    assert [s.num_steps for s in batched] == [2, 2, 2]
+    with pytest.raises(ValueError):
+        citrus_views_step(solver, batched, fields, angles[:2])
+    with pytest.raises(ValueError):
+        citrus_views_step(solver, batched, fields, [a + 0.1 for a in angles])

@@ -18,7 +18,7 @@ from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
 from citrus_amd.scatter_solver.uncollided import (
     first_collision_correction_factors, first_collision_exact_moments, first_collision_moments_at_directions,
     first_collision_quadrature_sums, first_collision_sampling_correction, first_collision_source_at_directions,
-    primary_directions_to_voxels, restrict_to_linear_discontinuous_moments)
+    primary_directions_to_voxels, restrict_to_linear_discontinuous_moments, first_collision_corrected_samples)
 from oracle import first_collision as ofc
 from tests.conftest import gpu_device
 from tests.helpers import F32_REL, rel_err
@@ -134,6 +134,20 @@ def test_fc_moments_match_oracle_in_every_layout(factor):
     gm = first_collision_moments_at_directions(library, rho_d, phi_d, grid, GEOMETRY, ANGLE, q.omega, corr, factor,
                                                layout="group_major")
     assert torch.equal(gm, mm.permute(1, 3, 4, 5, 0, 2).contiguous())
+    if factor == 1:  # one point per cell: zero slopes, and the cell-mean layout holds the means alone
+        assert not bool(mm[:, :, 1:].any())
+        gmm = first_collision_moments_at_directions(library, rho_d, phi_d, grid, GEOMETRY, ANGLE, q.omega, corr,
+                                                    factor, layout="group_major_means")
+        assert torch.equal(gmm, gm[..., 0].contiguous())
+        # samples stored by the sums kernel and corrected in place: the same values from one source evaluation
+        fused, corr2 = first_collision_corrected_samples(library, rho_d, phi_d, grid, GEOMETRY, ANGLE, q.omega,
+                                                         q.weight)
+        assert torch.equal(corr2.scale, corr.scale) and torch.equal(corr2.beta, corr.beta)
+        assert torch.equal(fused, gmm)
+    else:
+        with pytest.raises(ValueError):
+            first_collision_moments_at_directions(library, rho_d, phi_d, grid, GEOMETRY, ANGLE, q.omega, corr, factor,
+                                                  layout="group_major_means")
     # the fused kernel equals correction + restriction applied to the sampled source on the device
     S_dev = first_collision_source_at_directions(library, rho_d, phi_d, grid, GEOMETRY, ANGLE, q.omega)
     two_step = restrict_to_linear_discontinuous_moments(corr.apply(S_dev, q.omega), grid, factor)

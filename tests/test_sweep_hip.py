@@ -207,9 +207,11 @@ def test_backend_errors():
 
 @pytest.mark.parametrize("kernel", ["mfma", "vector"])
 @pytest.mark.parametrize("chunk", [None, 3])
-def test_group_major_explicit_source_with_moments(kernel, chunk, monkeypatch):
+@pytest.mark.parametrize("means", [False, True])
+def test_group_major_explicit_source_with_moments(kernel, chunk, means, monkeypatch):
     """Moment source plus an explicit source handed over group-major for every direction (the CITRUS step's summed
-    source) equals the cell-major chunk path and the oracle, on both kernels and in direction chunks."""
+    source) equals the cell-major chunk path and the oracle, on both kernels and in direction chunks; with zero
+    slopes the source may be handed over as its cell means alone."""
     from citrus_amd.scatter_solver.source_iteration import summed_source_function
 
     monkeypatch.setenv("CITRUS_SWEEP_KERNEL", kernel)
@@ -222,6 +224,8 @@ def test_group_major_explicit_source_with_moments(kernel, chunk, monkeypatch):
     mu = (torch.rand(G, *shape, generator=gen) + 0.3).to(dev)
     T = (torch.randn(G, ctx.num_moments, 4, *shape, generator=gen) * 0.2).to(dev)
     extra_cm = torch.randn(ctx.num_directions, *shape, G, 4, generator=gen).to(dev)  # [M, R, C, S, G, 4] reordered
+    if means:
+        extra_cm[..., 1:] = 0.0
 
     def plain(index, out=None):
         return out.copy_(extra_cm.index_select(0, index))
@@ -232,7 +236,8 @@ def test_group_major_explicit_source_with_moments(kernel, chunk, monkeypatch):
         return out.copy_(extra_cm.index_select(0, index))
 
     grouped.cell_major = True
-    grouped.hip_group_major = lambda: extra_cm.permute(4, 1, 2, 3, 0, 5).contiguous()
+    grouped.hip_group_major = (lambda: extra_cm[..., 0].permute(4, 1, 2, 3, 0).contiguous()) if means else \
+        (lambda: extra_cm.permute(4, 1, 2, 3, 0, 5).contiguous())
     phi_gm = sweep(ctx, mu, summed_source_function(scatter_source_function(ctx, T), grouped), direction_chunk=chunk)
     phi_cm = sweep(ctx, mu, summed_source_function(scatter_source_function(ctx, T), plain), direction_chunk=chunk)
     assert rel_err(phi_gm, phi_cm) < F32_REL
