@@ -27,13 +27,18 @@ from .scatter_solver.transport_sweep import sweep_views
 @dataclass
 class CitrusState:
     """Caller-owned state of one view (citrus.py:70): angle, accumulator Phi [G, (L+1)^2, 4, Rc, Cc, Sc] (None is
-    zero), cached cone set, step count and the relative change of Phi's scalar-flux cell means per step."""
+    zero), cached cone set, step count and the relative change of Phi's scalar-flux cell means per step.
+    ``citrus_views_step`` also keeps the scattering moments T = K Phi it formed for the image ([G, Rc, Cc, Sc,
+    (L+1)^2, 4]) with the fields they belong to: the next step reuses them only for those same fields."""
 
     angle: Optional[float] = None
     collided_moments: Optional[Tensor] = None
     cone_quadrature: Optional[Quadrature] = None
     num_steps: int = 0
     history: list = field(default_factory=list)
+    scatter_source: Optional[Tensor] = field(default=None, repr=False)
+    scatter_source_fields: Optional[object] = field(default=None, repr=False)
+    _batch: Optional[tuple] = field(default=None, repr=False)
 
 
 def _fields(solver: ScatterSolver, rho, grid: Grid, fields: Optional[MaterialFields]) -> MaterialFields:
@@ -148,17 +153,22 @@ def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fiel
     source = torch.empty(src_shape, device=device, dtype=torch.float32)
     for v, a in enumerate(angles):
         solver.first_collision_group_major(fields, a, phi_uc[v], ctx, out=source[v])
-    T = torch.empty(V, G, R, C, S, nsh, 4, device=device, dtype=torch.float32)
-    for v, st in enumerate(states):
-        if st.collided_moments is None:
-            T[v].zero_()
-        else:
-            _check_accumulator(solver, fields, st.collided_moments)
-            prev = st.collided_moments.to(device, torch.float32).permute(0, 3, 4, 5, 1, 2).contiguous()
-            scatter_moments_views(fields.transfer_coarse, prev[None], out=T[v:v + 1])
+    T = _previous_scatter_moments(states, fields)
+    if T is None:
+        T = torch.empty(V, G, R, C, S, nsh, 4, device=device, dtype=torch.float32)
+        for v, st in enumerate(states):
+            if st.collided_moments is None:
+                T[v].zero_()
+            elif st.scatter_source is not None and st.scatter_source_fields is fields:
+                T[v].copy_(st.scatter_source)
+            else:
+                _check_accumulator(solver, fields, st.collided_moments)
+                prev = st.collided_moments.to(device, torch.float32).permute(0, 3, 4, 5, 1, 2).contiguous()
+                scatter_moments_views(fields.transfer_coarse, prev[None], out=T[v:v + 1])
     collided = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
     del source
-    T = scatter_moments_views(fields.transfer_coarse, collided, out=T)  # multiple-collision source of the images
+    # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same
+    T = scatter_moments_views(fields.transfer_coarse, collided)
     results, new_states = [], []
     for v, (st, a) in enumerate(zip(states, angles)):
         new = collided[v].permute(0, 4, 5, 1, 2, 3)
@@ -167,8 +177,22 @@ def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fiel
         primary_signal = solver.primary(fields, a)["signal"] if primary else None
         results.append((image, primary_signal))
         new_states.append(CitrusState(angle=float(a), collided_moments=new, cone_quadrature=cones[v],
-                                      num_steps=st.num_steps + 1, history=st.history + [_change(new, st)]))
+                                      num_steps=st.num_steps + 1, history=st.history + [_change(new, st)],
+                                      scatter_source=T[v], scatter_source_fields=fields, _batch=(T, v)))
     return results, new_states
+
+
+def _previous_scatter_moments(states: Sequence[CitrusState], fields: MaterialFields) -> Optional[Tensor]:
+    """The [V, ...] scattering moments of the previous batched step when every state still holds its slot of it, in
+    order, for these same fields (no copy); else None."""
+    batch = states[0]._batch
+    if batch is None or int(batch[0].shape[0]) != len(states):
+        return None
+    for v, st in enumerate(states):
+        same = st._batch is not None and st._batch[0] is batch[0] and st._batch[1] == v
+        if not same or st.scatter_source_fields is not fields:
+            return None
+    return batch[0]
 
 
 def _change(new: Tensor, state: CitrusState) -> float:

@@ -183,19 +183,33 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
   float I0[GMAX], I1[GMAX], I2[GMAX];
 #pragma unroll
   for (int g = 0; g < GMAX; ++g) I0[g] = I1[g] = I2[g] = 0.0f;
-  for (int k = 0; k < d.K; ++k) {
-    const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
-    float S[GMAX];
-    eval_source(d, pd, eta, S);
-    const float w = weights[k];
-    const float we = w * eta;
-    const float wee = we * eta;
+  // directions in runs of four: with samples and K % 4 == 0 each group's four samples leave as one 16-byte store
+  const bool vec = samples && (d.K % 4) == 0;
+  for (int k0 = 0; k0 < d.K; k0 += 4) {
+    f4 run[GMAX];
 #pragma unroll
-    for (int g = 0; g < GMAX; ++g) {
-      I0[g] = fmaf(w, S[g], I0[g]);
-      I1[g] = fmaf(we, S[g], I1[g]);
-      I2[g] = fmaf(wee, S[g], I2[g]);
-      if (samples && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[g];
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + j;
+      if (k >= d.K) break;
+      const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
+      float S[GMAX];
+      eval_source(d, pd, eta, S);
+      const float w = weights[k];
+      const float we = w * eta;
+      const float wee = we * eta;
+#pragma unroll
+      for (int g = 0; g < GMAX; ++g) {
+        I0[g] = fmaf(w, S[g], I0[g]);
+        I1[g] = fmaf(we, S[g], I1[g]);
+        I2[g] = fmaf(wee, S[g], I2[g]);
+        run[g][j] = S[g];
+        if (samples && !vec && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[g];
+      }
+    }
+    if (vec) {
+#pragma unroll
+      for (int g = 0; g < GMAX; ++g)
+        if (g < d.G) *reinterpret_cast<f4*>(samples + ((int64_t)g * d.P + p) * d.K + k0) = run[g];
     }
   }
   // exact Legendre moments l = 0, 1 of the same source (no quadrature)
@@ -234,17 +248,20 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
 __global__ __launch_bounds__(kBlock) void fc_correct_kernel(FcDev d, const float* __restrict__ scale,
                                                            const float* __restrict__ beta,
                                                            float* __restrict__ samples) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (g, p, k), k fastest
-  if (idx >= (int64_t)d.G * d.P * d.K) return;
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (p, k), k fastest; the groups in a loop
+  const int64_t PK = (int64_t)d.P * d.K;
+  if (idx >= PK) return;
   const int k = (int)(idx % d.K);
-  const int64_t gp = idx / d.K;
-  const int p = (int)(gp % d.P);
+  const int p = (int)(idx / d.K);
   const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
   float ux, uy, uz;
   primary_direction(d, r, c, s, ux, uy, uz);
   const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
-  const float v = samples[idx];
-  samples[idx] = v * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+  for (int g = 0; g < d.G; ++g) {
+    const int64_t gp = (int64_t)g * d.P + p;
+    const float v = samples[g * PK + idx];
+    samples[g * PK + idx] = v * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+  }
 }
 
 template <int GMAX>
@@ -482,7 +499,7 @@ extern "C" int citrus_fc_correct(const CitrusFcArgs* args, const float* scale, c
   if (int rc = check_args(args)) return rc;
   if (!scale || !beta || !samples) return fail(CITRUS_ERR_ARG, "NULL scale, beta or samples");
   const FcDev d = make_dev(*args);
-  const int64_t n = (int64_t)d.G * d.P * d.K;
+  const int64_t n = (int64_t)d.P * d.K;
   if (n == 0) return CITRUS_OK;
   if ((n + kBlock - 1) / kBlock > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many samples for one launch");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

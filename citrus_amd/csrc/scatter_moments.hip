@@ -26,13 +26,19 @@ __device__ __forceinline__ int degree_of(int n) {
   return l;
 }
 
-// LDS: mu [kCells][L+1][G][G], then the material table [M][G][G][Lt+1] (factored form)
+// row stride of one degree's [GMAX][GMAX] matrix in LDS: one word of padding puts the rows of different degrees,
+// which the lanes of one wave read at the same (gi, go), in different banks
+template <int GMAX>
+constexpr int degree_stride() { return GMAX * GMAX + 1; }
+
+// LDS: mu [kCells][L+1][GMAX][GMAX] (degree stride padded), then the material table [M][G][G][Lt+1] (factored form)
 template <int GMAX, bool FACTORED>
 __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
     const float* __restrict__ transfer, const float* __restrict__ rho, int M, int Lt, const float* __restrict__ phi,
     float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only) {
   extern __shared__ float lds[];
-  const int per_cell = (L + 1) * G * G;
+  constexpr int DS = degree_stride<GMAX>();
+  const int per_cell = (L + 1) * DS + 1;  // odd: the two cells' matrices also fall in different banks
   float* mu = lds;
   float* table = lds + kCells * per_cell;
   const int nthreads = blockDim.x;
@@ -45,25 +51,27 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
     for (int i = tid; i < nt; i += nthreads) table[i] = transfer[i];
     __syncthreads();
   }
-  // per-cell transfer matrices
-  for (int i = tid; i < kCells * per_cell; i += nthreads) {
-    const int cc = i / per_cell;
-    const int r = i - cc * per_cell;
-    const int l = r / (G * G);
-    const int gi = (r / G) % G;
-    const int go = r % G;
+  // per-cell transfer matrices, entries (l, gi, go) with compile-time strides
+  const int entries = (L + 1) * GMAX * GMAX;
+  for (int cc = 0; cc < kCells; ++cc) {
     const int cell = c0 + cc;
-    float m_val = 0.0f;
-    if (cell < N && !(lower_only && gi > go)) {
-      if (FACTORED) {
+    for (int r = tid; r < entries; r += nthreads) {
+      const int l = r / (GMAX * GMAX);
+      const int gi = (r / GMAX) % GMAX;
+      const int go = r % GMAX;
+      float m_val = 0.0f;
+      if (cell < N && gi < G && go < G && !(lower_only && gi > go)) {
+        if (FACTORED) {
 #pragma unroll
-        for (int m = 0; m < kMaxMaterials; ++m)
-          if (m < M) m_val = fmaf(rho[(int64_t)m * N + cell], table[((m * G + gi) * G + go) * (Lt + 1) + l], m_val);
-      } else {
-        m_val = transfer[(((int64_t)gi * G + go) * (Lt + 1) + l) * N + cell];
+          for (int m = 0; m < kMaxMaterials; ++m)
+            if (m < M)
+              m_val = fmaf(rho[(int64_t)m * N + cell], table[((m * G + gi) * G + go) * (Lt + 1) + l], m_val);
+        } else {
+          m_val = transfer[(((int64_t)gi * G + go) * (Lt + 1) + l) * N + cell];
+        }
       }
+      mu[cc * per_cell + l * DS + gi * GMAX + go] = m_val;
     }
-    mu[i] = m_val;
   }
   __syncthreads();
   const int elems = nsh * 4;
@@ -78,14 +86,14 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
   float p[GMAX];
 #pragma unroll
   for (int gi = 0; gi < GMAX; ++gi) p[gi] = gi < G ? phi[off + gi * group_stride] : 0.0f;
-  const float* mcell = mu + cc * per_cell + l * G * G;
+  const float* mcell = mu + cc * per_cell + l * DS;
 #pragma unroll
   for (int go = 0; go < GMAX; ++go) {
     if (go >= G) break;
     float acc = 0.0f;
 #pragma unroll
     for (int gi = 0; gi < GMAX; ++gi)
-      if (gi < G && !(lower_only && gi > go)) acc = fmaf(mcell[gi * G + go], p[gi], acc);
+      if (gi < G && !(lower_only && gi > go)) acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
     T[off + go * group_stride] = acc;
   }
 }
@@ -105,7 +113,9 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
            int64_t N, int V, int lower_only, void* stream) {
   int L = 0;
   if (int rc = check(G, nsh, Lt, N, V, L)) return rc;
-  const size_t lds = sizeof(float) * ((size_t)kCells * (L + 1) * G * G + (FACTORED ? (size_t)M * G * G * (Lt + 1) : 0));
+  const int gmax = G <= 8 ? 8 : (G <= 16 ? 16 : 32);
+  const size_t per_cell = (size_t)(L + 1) * (gmax * gmax + 1) + 1;
+  const size_t lds = sizeof(float) * ((size_t)kCells * per_cell + (FACTORED ? (size_t)M * G * G * (Lt + 1) : 0));
   if (lds > 64 * 1024) return fail(CITRUS_ERR_UNSUPPORTED, "transfer tables exceed 64 KiB of LDS");
   const int threads = kCells * nsh * 4;
   const int64_t blocks = (int64_t)V * ((N + kCells - 1) / kCells);

@@ -138,6 +138,13 @@ def test_views_step_equals_view_steps(spacing, q):
             assert torch.equal(results[v][1], primary)
             assert abs(batched[v].history[-1] - single[v].history[-1]) < 1e-5
     assert [s.num_steps for s in batched] == [2, 2, 2]
+    # new densities: the scattering moments kept by the states belong to the old fields and are formed again
+    fields2 = solver.material_fields(rho * 1.1, grid)
+    results, batched = citrus_views_step(solver, batched, fields2, angles)
+    for v, a in enumerate(angles):
+        scatter, _, single[v] = citrus_view_step(solver, single[v], fields2, a)
+        assert rel_err(batched[v].collided_moments, single[v].collided_moments) < F32_REL
+        assert rel_err(results[v][0], scatter) < F32_REL
     with pytest.raises(ValueError):
         citrus_views_step(solver, batched, fields, angles[:2])
     with pytest.raises(ValueError):
