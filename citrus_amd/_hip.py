@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -79,6 +79,23 @@ _SIGNATURES = {
                                           c_void, c_void]),
     "citrus_uncollided_fluence": (c_int, [c_void, c_int, c_void, c_void, c_void, c_void, c_void, c_int, c_int,
                                           c_void, c_void, c_void, c_void, c_void, c_void, c_void, c_void]),
+    "citrus_nn_conv3d_forward": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_void, c_int,
+                                         c_int, c_int, c_void, c_i64, c_void]),
+    "citrus_nn_conv3d_backward_data": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_int, c_int,
+                                               c_void, c_i64, c_void]),
+    "citrus_nn_wgrad_workspace_floats": (c_i64, [c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
+    "citrus_nn_conv3d_backward_weight": (c_int, [c_void, c_i64, c_void, c_i64, c_int, c_int, c_int, c_int, c_int,
+                                                 c_int, c_int, c_void, c_void, c_void, c_void]),
+    "citrus_nn_leaky_backward": (c_int, [c_void, c_i64, c_void, c_i64, c_i64, c_int, c_void]),
+    "citrus_nn_avg_pool2": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_i64, c_void]),
+    "citrus_nn_avg_pool2_backward": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_i64, c_int,
+                                             c_void]),
+    "citrus_nn_upsample2": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_i64, c_void]),
+    "citrus_nn_upsample2_backward": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_i64,
+                                             c_void]),
+    "citrus_nn_adam": (c_int, [c_void, c_void, c_void, c_void, c_void, c_i64, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, c_int, c_void]),
+    "citrus_nn_to_bf16": (c_int, [c_void, c_void, c_i64, c_void]),
     "citrus_fc_source": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
     "citrus_fc_sums": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void, c_void]),
     "citrus_fc_correct": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_void, c_void]),

@@ -267,6 +267,53 @@ int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, int t
  * widths[m] (pixels) centred on pixel c clipped to [0, N] (box_matrix, scatter_image.py:952). */
 int citrus_box_matrix(const float* widths, int k, int N, float* out, void* stream);
 
+/* ---- Phi_{>2} surrogate network ------------------------------------------------------------------------------
+ * No counterpart in the reference: the learned part of this path, a 3D U-Net that predicts the collided moments of
+ * collision order >= 3 from the densities and the second-order moments. Activations are float32 [B][C][D][H][W]
+ * with the channels of a sample contiguous; *_bstride is the element stride between samples (a tensor may be a
+ * channel slice of a larger one). Weights are bfloat16 (uint16_t bit patterns) [Co][Ci][k][k][k], k = 3 or 1, zero
+ * padding (k-1)/2, stride 1; every sum accumulates in float32. The leaky ReLU has slope 0.01 below zero. */
+
+/* y = conv(x, w) + bias (bias may be NULL), then the leaky ReLU when leaky != 0. */
+int citrus_nn_conv3d_forward(const float* x, int64_t x_bstride, int B, int Ci, int D, int H, int W, const uint16_t* w,
+                             const float* bias, int Co, int k, int leaky, float* y, int64_t y_bstride, void* stream);
+
+/* dx = gradient of sum(g * conv(x, w)) with respect to x: g [B][Co] -> dx [B][Ci] (overwritten). */
+int citrus_nn_conv3d_backward_data(const float* g, int64_t g_bstride, int B, int Co, int D, int H, int W,
+                                   const uint16_t* w, int Ci, int k, float* dx, int64_t dx_bstride, void* stream);
+
+/* dw [Co][Ci][k^3] and db [Co] (db may be NULL) of sum(g * conv(x, w)), summed over the samples in a fixed order;
+ * workspace holds citrus_nn_wgrad_workspace_floats(B, Ci, Co, D, H, W, k) floats. */
+int64_t citrus_nn_wgrad_workspace_floats(int B, int Ci, int Co, int D, int H, int W, int k);
+int citrus_nn_conv3d_backward_weight(const float* x, int64_t x_bstride, const float* g, int64_t g_bstride, int B,
+                                     int Ci, int Co, int D, int H, int W, int k, float* dw, float* db,
+                                     float* workspace, void* stream);
+
+/* g *= leaky'(y) elementwise over n_per_sample contiguous elements of each sample (y is the layer's output). */
+int citrus_nn_leaky_backward(float* g, int64_t g_bstride, const float* y, int64_t y_bstride, int64_t n_per_sample,
+                             int B, void* stream);
+
+/* 2x2x2 average pooling of x [B][C][D][H][W] (even sizes) into y; its adjoint adds (accumulate) or writes
+ * g / 8 into every voxel of the block. */
+int citrus_nn_avg_pool2(const float* x, int64_t x_bstride, int B, int C, int D, int H, int W, float* y,
+                        int64_t y_bstride, void* stream);
+int citrus_nn_avg_pool2_backward(const float* g, int64_t g_bstride, int B, int C, int D, int H, int W, float* dx,
+                                 int64_t dx_bstride, int accumulate, void* stream);
+
+/* Nearest 2x upsampling of x [B][C][D][H][W] into y [B][C][2D][2H][2W]; its adjoint sums each 2x2x2 block. */
+int citrus_nn_upsample2(const float* x, int64_t x_bstride, int B, int C, int D, int H, int W, float* y,
+                        int64_t y_bstride, void* stream);
+int citrus_nn_upsample2_backward(const float* g, int64_t g_bstride, int B, int C, int D, int H, int W, float* dx,
+                                 int64_t dx_bstride, void* stream);
+
+/* One bias-corrected Adam step on float32 master weights p (m, v the moment estimates, step >= 1); p_bf16 (or NULL)
+ * receives p rounded to bfloat16 (nearest, ties to even). */
+int citrus_nn_adam(float* p, const float* g, float* m, float* v, uint16_t* p_bf16, int64_t n, float lr, float beta1,
+                   float beta2, float eps, int step, void* stream);
+
+/* y = x rounded to bfloat16 (nearest, ties to even). */
+int citrus_nn_to_bf16(const float* x, uint16_t* y, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

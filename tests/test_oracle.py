@@ -466,3 +466,117 @@ def test_primary_vacuum_and_slab_closed_forms():
     assert np.allclose(fl2, beer[:, None, None] / r2[None], rtol=1e-13)
     S = np.array([1.0, 0.0, 2.0])
     assert np.allclose(oim.scatter_fraction(S, np.array([1.0, 0.0, 6.0])), [0.5, 0.0, 0.25])
+
+
+# ---- surrogate network oracle --------------------------------------------------------------------------------------
+
+def test_surrogate_conv_against_brute_force():
+    from oracle import surrogate as osg
+
+    rng = np.random.default_rng(3)
+    for k in (3, 1):
+        x = rng.standard_normal((2, 3, 4, 5, 3))
+        w = rng.standard_normal((4, 3, k, k, k))
+        b = rng.standard_normal(4)
+        got = osg.conv3d(x, w, b)
+        p = (k - 1) // 2
+        ref = np.zeros_like(got)
+        for bb in range(2):
+            for o in range(4):
+                for z in range(4):
+                    for y in range(5):
+                        for xx in range(3):
+                            acc = b[o]
+                            for i in range(3):
+                                for dz in range(k):
+                                    for dy in range(k):
+                                        for dx in range(k):
+                                            zz, yy, x3 = z + dz - p, y + dy - p, xx + dx - p
+                                            if 0 <= zz < 4 and 0 <= yy < 5 and 0 <= x3 < 3:
+                                                acc += w[o, i, dz, dy, dx] * x[bb, i, zz, yy, x3]
+                            ref[bb, o, z, y, xx] = acc
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-13)
+
+
+def _fd(f, arr, idx, h=1e-6):
+    old = arr[idx]
+    arr[idx] = old + h
+    fp = f()
+    arr[idx] = old - h
+    fm = f()
+    arr[idx] = old
+    return (fp - fm) / (2 * h)
+
+
+def test_surrogate_conv_gradients_finite_differences():
+    from oracle import surrogate as osg
+
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((2, 2, 4, 3, 5))
+    w = rng.standard_normal((3, 2, 3, 3, 3))
+    b = rng.standard_normal(3)
+    g = rng.standard_normal((2, 3, 4, 3, 5))
+    loss = lambda: float((g * osg.conv3d(x, w, b)).sum())
+    dx, dw, db = osg.conv3d_backward(x, w, g)
+    for idx in [(0, 1, 0, 0, 0), (1, 0, 3, 2, 4), (0, 0, 2, 1, 2)]:
+        assert abs(_fd(loss, x, idx) - dx[idx]) < 1e-7 * max(1.0, abs(dx[idx]))
+    for idx in [(0, 0, 0, 0, 0), (2, 1, 1, 2, 0), (1, 1, 2, 2, 2)]:
+        assert abs(_fd(loss, w, idx) - dw[idx]) < 1e-7 * max(1.0, abs(dw[idx]))
+    assert abs(_fd(loss, b, (1,)) - db[1]) < 1e-7 * max(1.0, abs(db[1]))
+
+
+def test_surrogate_pool_upsample_adjoints():
+    from oracle import surrogate as osg
+
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 3, 4, 6, 2))
+    y = rng.standard_normal((2, 3, 2, 3, 1))
+    assert math.isclose(float((osg.avg_pool2(x) * y).sum()), float((x * osg.avg_pool2_backward(y)).sum()),
+                        rel_tol=1e-12)
+    assert math.isclose(float((osg.upsample2(y) * x).sum()), float((y * osg.upsample2_backward(x)).sum()),
+                        rel_tol=1e-12)
+    assert np.allclose(osg.avg_pool2(osg.upsample2(y)), y)
+
+
+def test_surrogate_network_gradient_finite_differences():
+    from oracle import surrogate as osg
+
+    rng = np.random.default_rng(6)
+    shapes = osg.layer_shapes(2, 3, 2)
+    params = {n: (rng.standard_normal((co, ci, k, k, k)) * 0.4, rng.standard_normal(co) * 0.1)
+              for n, (co, ci, k) in shapes.items()}
+    x = rng.standard_normal((1, 2, 4, 4, 4))
+    target = rng.standard_normal((1, 3, 4, 4, 4))
+
+    def loss():
+        return osg.mse(osg.forward(params, x)[0], target)[0]
+
+    out, cache = osg.forward(params, x)
+    _, g = osg.mse(out, target)
+    grads = osg.backward(params, cache, g)
+    for name, idx in [("e0a", (1, 0, 0, 1, 2)), ("b1", (3, 2, 1, 1, 1)), ("d1a", (1, 5, 2, 0, 1)),
+                      ("d0a", (0, 4, 1, 1, 0)), ("head", (2, 1, 0, 0, 0)), ("e1b", (0, 3, 2, 2, 2))]:
+        fd = _fd(loss, params[name][0], idx)
+        assert abs(fd - grads[name][0][idx]) < 1e-6 * max(1e-3, abs(fd)), (name, fd, grads[name][0][idx])
+    fd = _fd(loss, params["d1b"][1], (2,))
+    assert abs(fd - grads["d1b"][1][2]) < 1e-6 * max(1e-3, abs(fd))
+
+
+def test_surrogate_adam_and_bf16_rounding():
+    import torch
+
+    from oracle import surrogate as osg
+
+    p, g = np.array([1.0, -2.0, 0.5]), np.array([0.3, -0.01, 0.0])
+    p1, m, v = osg.adam(p, g, np.zeros(3), np.zeros(3), 1, 0.1)
+    # first step: m_hat = g, v_hat = g^2, so the step is lr * sign(g) up to eps
+    assert np.allclose(p1, p - 0.1 * g / (np.abs(g) + 1e-8))
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
+    vals = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -3.14159, 1e-20, 65504.0, 0.0], dtype=np.float32)
+    want = np.array([1.0, 1.0, 1.0 + 2.0 ** -6, -3.140625, None, 65536.0, 0.0], dtype=object)
+    got = osg.bf16_round(vals)
+    for gv, wv in zip(got, want):
+        if wv is not None:
+            assert gv == wv
+    ref = torch.from_numpy(np.random.default_rng(7).standard_normal(1000).astype(np.float32))
+    assert np.array_equal(osg.bf16_round(ref.numpy()), ref.to(torch.bfloat16).double().numpy())
