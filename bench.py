@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--sweep-only", action="store_true", help="time the sweep-only step alone")
     ap.add_argument("--cells", type=int, default=64, help="transport cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
     ap.add_argument("--cpu-directions", type=int, default=8, help="directions in the CPU-baseline sample")
     return ap.parse_args()
 
@@ -166,6 +167,67 @@ def build_view_solver(args, device, rho_fine):
     return solver, fields, angles
 
 
+SURROGATE_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "citrus_amd", "weights",
+                                 "surrogate_64cube_s12_p5_g16.npz")
+
+
+def learned_vs_plain(solver, fields, angles, device):
+    """Converged scatter images of the step's views from scratch: the plain source iteration (stopping at the
+    configured tolerance) against the learned form (two sweeps, the surrogate's Phi_{>2}, one judging sweep at a 1%
+    threshold, plain fallback), both batched over the views and timed with the uncollided and first-collision work
+    and the signal images; errors against a source iteration converged to 1e-5."""
+    import torch
+
+    from citrus_amd.surrogate import UNet3D, collision_orders_views, learned_views, plain_views
+
+    if not os.path.exists(SURROGATE_WEIGHTS):
+        return {"error": f"surrogate weights missing ({SURROGATE_WEIGHTS})"}
+    net = UNet3D.load(SURROGATE_WEIGHTS, device)
+    cones = [solver.cone_quadrature(fields.grid_fine, a) for a in angles]
+
+    def images(moments):
+        out = []
+        for a, cone, m in zip(angles, cones, moments):
+            phi_uc = solver.uncollided_fluence(fields, a)
+            T = solver.multiple_collision_source_moments(fields, m)
+            out.append(solver.scatter_image(fields, a, phi_uc, T, cone, signal_only=True)["signal"])
+        return out
+
+    ref = [o.total for o in collision_orders_views(solver, fields, angles, tolerance=1e-5)]
+    ref_images = images(ref)
+
+    def errors(moments, imgs):
+        img = [float(torch.linalg.vector_norm(i - r) / torch.linalg.vector_norm(r)) for i, r in zip(imgs, ref_images)]
+        flux = [float(torch.linalg.vector_norm((m - x)[:, 0, 0]) / torch.linalg.vector_norm(x[:, 0, 0]))
+                for m, x in zip(moments, ref)]
+        return max(img), max(flux)
+
+    tol = solver.config.source_iteration_tolerance
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain, plain_sweeps = plain_views(solver, fields, angles, tol)
+    plain_images = images(plain)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lv = learned_views(solver, fields, angles, net, threshold=0.01, fallback_tolerance=tol)
+    learned_images = images(lv.collided_moments)
+    torch.cuda.synchronize()
+    t_learned = time.perf_counter() - t0
+    plain_img, plain_flux = errors(plain, plain_images)
+    learned_img, learned_flux = errors(lv.collided_moments, learned_images)
+    V = len(angles)
+    return {"plain_views_per_s": V / t_plain, "learned_views_per_s": V / t_learned,
+            "learned_speedup": t_plain / t_learned, "plain_sweeps": plain_sweeps, "learned_sweeps": lv.sweeps,
+            "learned_accepted": sum(lv.accepted), "judging_residuals": [round(r, 5) for r in lv.residuals],
+            "plain_tolerance": tol, "plain_image_error": plain_img, "learned_image_error": learned_img,
+            "plain_flux_error": plain_flux, "learned_flux_error": learned_flux,
+            "surrogate_parameters": net.num_parameters(),
+            "note": "errors: max over views of the relative L2 error of the signal-only scatter image (the SPR "
+                    "numerator) and of the scalar-flux cell means, against a source iteration converged to 1e-5"}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -214,7 +276,14 @@ def main():
         assert all(bool(torch.isfinite(s.collided_moments).all()) for s in states), "non-finite accumulator"
         view["spr"] = spr
         view["history"] = [round(h, 6) for h in states[0].history]
-        del solver, fields, states, images
+        del states, images
+        torch.cuda.empty_cache()
+        if not args.no_learned:
+            try:
+                view["learned"] = learned_vs_plain(solver, fields, angles, device)
+            except Exception as e:  # reported, not fatal: the view-step rate above is the metric
+                view["learned"] = {"error": f"{type(e).__name__}: {e}"}
+        del solver, fields
         torch.cuda.empty_cache()
 
     # ---- sweep only: scattering moments + one fused sweep per view, V views per launch sequence -------------
@@ -280,7 +349,8 @@ def main():
                    "directions": ctx.num_directions, "moments": nsh, "views_per_step": V,
                    "sweep_only_view_sweeps_per_s": sweep_rate, "sweep_ms_per_view": sweep_ms / V,
                    "wall_s": wall,
-                   **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {})},
+                   **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),
+                   **({"learned_vs_plain": view["learned"]} if view and "learned" in view else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "alg_bytes_per_view_sweep": alg_bytes,

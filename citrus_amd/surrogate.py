@@ -10,9 +10,9 @@ means (the quantity the source iteration's stopping rule measures, source_iterat
 threshold, else the plain source iteration runs instead.
 
 Network (``UNet3D``): two pooling levels, base width c, leaky ReLU (slope 0.01), a 1x1x1 head; inputs are the M
-material densities of the transport cells and the P1 cell means of ``phi^(2)`` (4 moments x G groups) divided by the
-sample's scale (RMS of the scalar-flux cell means of ``phi^(2)``); outputs the P1 cell means of ``Phi_{>2}`` in the
-same scale (the transport problem is linear in the source, so the scale carries through). The LD slopes and the
+material densities of the transport cells and the P1 cell means of ``phi^(2)`` (4 moments x G groups), optionally
+also those of ``phi^(1)``, divided by the sample's scale (RMS of the scalar-flux cell means of ``phi^(2)``); outputs
+the P1 cell means of ``Phi_{>2}`` in the same scale (the transport problem is linear in the source, so the scale carries through). The LD slopes and the
 moments l >= 2 of ``Phi_{>2}`` are left to the judging sweep, which forms them from the predicted part.
 
 Every layer runs on the library's kernels (``citrus_nn_*``): bfloat16 weights, float32 activations and accumulation;
@@ -293,10 +293,22 @@ class UNet3D:
         return net
 
 
-def mse_gradient(pred: Tensor, target: Tensor) -> tuple[float, Tensor]:
-    """(mean squared error, its gradient 2 (pred - target) / n)."""
+def mse_gradient(pred: Tensor, target: Tensor, channel_weight: Optional[Tensor] = None) -> tuple[float, Tensor]:
+    """(mean squared error, its gradient 2 (pred - target) / n); with ``channel_weight`` [C] each channel's squared
+    error is weighted."""
     d = pred - target
-    return float((d * d).mean()), d * (2.0 / d.numel())
+    if channel_weight is None:
+        return float((d * d).mean()), d * (2.0 / d.numel())
+    w = channel_weight.to(d.device, d.dtype).reshape(1, -1, *([1] * (d.dim() - 2)))
+    return float((w * d * d).mean()), w * d * (2.0 / d.numel())
+
+
+def scalar_flux_weight(groups: int, weight: float) -> Tensor:
+    """[4 G] channel weights of the P1 outputs: ``weight`` on the scalar-flux channel (l = 0) of every group, 1 on
+    the three l = 1 channels."""
+    w = torch.ones(groups, 4)
+    w[:, 0] = weight
+    return w.reshape(-1)
 
 
 # ---- features and targets from the solver ----------------------------------------------------------------------
@@ -313,9 +325,24 @@ def sample_scale(phi2: Tensor) -> float:
     return s if s > 0 else 1.0
 
 
-def features(rho_coarse: Tensor, phi2: Tensor, scale: float) -> Tensor:
-    """[M + 4 G, R, C, S] network input: densities and the scaled P1 cell means of phi^(2)."""
-    return torch.cat([rho_coarse.to(phi2.device, torch.float32), p1_means(phi2).float() / scale], dim=0)
+def features(rho_coarse: Tensor, phi2: Tensor, scale: float, phi1: Optional[Tensor] = None) -> Tensor:
+    """[M + 4 G (+ 4 G), R, C, S] network input: densities, the scaled P1 cell means of phi^(2) and, when given,
+    those of phi^(1)."""
+    parts = [rho_coarse.to(phi2.device, torch.float32), p1_means(phi2).float() / scale]
+    if phi1 is not None:
+        parts.append(p1_means(phi1).float() / scale)
+    return torch.cat(parts, dim=0)
+
+
+def _uses_phi1(net: "UNet3D", fields) -> bool:
+    """Whether the network's input carries phi^(1) besides phi^(2) (told by its input width)."""
+    M = int(fields.rho_coarse.shape[0])
+    extra = net.cin - M
+    if extra == net.cout:
+        return False
+    if extra == 2 * net.cout:
+        return True
+    raise ValueError(f"network input width {net.cin} fits neither {M + net.cout} nor {M + 2 * net.cout}")
 
 
 @dataclass
@@ -353,17 +380,19 @@ def collision_orders(solver, fields, angle: float, tolerance: float = 1e-4, max_
     return CollisionOrders(phi1=phi1, phi2=phi2, total=total, sweeps=sweeps)
 
 
-def training_pair(solver, fields, angle: float, tolerance: float = 1e-4):
-    """(input [M + 4G, R, C, S], target [4G, R, C, S]) of one view: the scaled P1 cell means of Phi_{>2}."""
+def training_pair(solver, fields, angle: float, tolerance: float = 1e-4, with_phi1: bool = False):
+    """(input [M + 4G (+ 4G), R, C, S], target [4G, R, C, S]) of one view: the scaled P1 cell means of Phi_{>2}."""
     orders = collision_orders(solver, fields, angle, tolerance)
     s = sample_scale(orders.phi2)
     higher = orders.total - orders.phi1 - orders.phi2
-    return features(fields.rho_coarse, orders.phi2, s), p1_means(higher).float() / s
+    return (features(fields.rho_coarse, orders.phi2, s, orders.phi1 if with_phi1 else None),
+            p1_means(higher).float() / s)
 
 
 def train(net: UNet3D, inputs: Sequence[Tensor], targets: Sequence[Tensor], epochs: int, lr: float = 1e-3,
-          batch: int = 1, seed: int = 0) -> list[float]:
-    """Adam on the mean squared error over the pairs, shuffled per epoch (seeded); returns the mean loss per epoch."""
+          batch: int = 1, seed: int = 0, channel_weight: Optional[Tensor] = None) -> list[float]:
+    """Adam on the (channel-weighted) mean squared error over the pairs, shuffled per epoch (seeded); returns the
+    mean loss per epoch."""
     gen = torch.Generator().manual_seed(seed)
     n = len(inputs)
     history = []
@@ -375,7 +404,7 @@ def train(net: UNet3D, inputs: Sequence[Tensor], targets: Sequence[Tensor], epoc
             x = torch.stack([inputs[j] for j in idx]).to(net.device)
             y = torch.stack([targets[j] for j in idx]).to(net.device)
             out, cache = net.forward(x, keep=True)
-            loss, g = mse_gradient(out, y)
+            loss, g = mse_gradient(out, y, channel_weight)
             net.adam_step(net.backward(cache, g), lr)
             total += loss * len(idx)
         history.append(total / n)
@@ -415,7 +444,7 @@ def learned_collided_moments(solver, fields, angle: float, net: UNet3D, threshol
     phi1 = solver.transport_sweep(fields, fc)
     phi2 = solver.transport_sweep(fields, solver.scatter_moment_source(fields, phi1, ctx))
     s = sample_scale(phi2)
-    x = features(fields.rho_coarse, phi2, s)[None]
+    x = features(fields.rho_coarse, phi2, s, phi1 if _uses_phi1(net, fields) else None)[None]
     pred = net.forward(x)[0]
     estimate = _embed_p1(pred, phi1, s).add_(phi1).add_(phi2)
     del pred, x, phi2
@@ -435,3 +464,118 @@ def learned_collided_moments(solver, fields, angle: float, net: UNet3D, threshol
                              max_iterations=max(1, solver.config.max_source_iterations),
                              direction_chunk=solver.sweep_direction_chunk(fields))
     return LearnedResult(plain.phi_total, residual, False, 3 + plain.num_iterations)
+
+
+# ---- several views of one volume at once (batched sweeps) ---------------------------------------------------------
+
+def _views_setup(solver, fields, angles):
+    ctx = solver.sweep_context(fields.grid_coarse)
+    V = len(angles)
+    G, nsh, M = solver.num_groups, ctx.num_moments, ctx.num_directions
+    R, C, S = fields.grid_coarse.shape
+    dev = solver.device
+    shape = (V, G, R, C, S, M) if fields.q == 1 else (V, G, R, C, S, M, 4)
+    source = torch.empty(shape, device=dev, dtype=torch.float32)
+    for v, a in enumerate(angles):
+        phi_uc = solver.uncollided_fluence(fields, a)
+        solver.first_collision_group_major(fields, a, phi_uc, ctx, out=source[v])
+        del phi_uc
+    T = torch.zeros(V, G, R, C, S, nsh, 4, device=dev, dtype=torch.float32)
+    return ctx, source, T
+
+
+def _api(phi_v: Tensor) -> Tensor:
+    """[G, nsh, 4, R, C, S] view of one view's [G, R, C, S, nsh, 4] moments."""
+    return phi_v.permute(0, 4, 5, 1, 2, 3)
+
+
+def _scalar_norms_views(*phis: Tensor) -> list:
+    """Per view, float64 norms of the scalar-flux cell means of [V, G, R, C, S, nsh, 4] moments (one sync)."""
+    rows = [torch.linalg.vector_norm(p[..., 0, 0].reshape(p.shape[0], -1).double(), dim=1) for p in phis]
+    return torch.stack(rows).tolist()
+
+
+def collision_orders_views(solver, fields, angles: Sequence[float], tolerance: float = 1e-4,
+                           max_sweeps: int = 200) -> list[CollisionOrders]:
+    """``collision_orders`` of several views of one volume, each collision order of all views in one batched sweep
+    (``sweep_views``); stops when every view's order norm is below ``tolerance`` of its total."""
+    from .scatter_solver.source_iteration import scatter_moments_views
+    from .scatter_solver.transport_sweep import sweep_views
+
+    ctx, source, T = _views_setup(solver, fields, angles)
+    phi1 = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
+    del source
+    total = phi1.clone()
+    phi_n, phi2, sweeps = phi1, None, 1
+    while sweeps < max_sweeps:
+        T = scatter_moments_views(fields.transfer_coarse, phi_n, out=T)
+        phi_n = sweep_views(ctx, fields.mu_t_coarse, T)
+        sweeps += 1
+        if phi2 is None:
+            phi2 = phi_n
+        total.add_(phi_n)
+        n, t = _scalar_norms_views(phi_n, total)
+        if all(a <= tolerance * b for a, b in zip(n, t)):
+            break
+    return [CollisionOrders(phi1=_api(phi1[v]), phi2=_api(phi2[v]), total=_api(total[v]), sweeps=sweeps)
+            for v in range(len(angles))]
+
+
+def plain_views(solver, fields, angles: Sequence[float], tolerance: float, max_sweeps: int = 200):
+    """Plain source iteration of several views (batched sweeps) until every view's last collision order is below
+    ``tolerance`` of its total: ([G, nsh, 4, R, C, S] per view, sweeps)."""
+    orders = collision_orders_views(solver, fields, angles, tolerance, max_sweeps)
+    return [o.total for o in orders], orders[0].sweeps
+
+
+@dataclass
+class LearnedViews:
+    """The learned form of several views: collided moments per view, the judging sweep's relative change per view,
+    which views it accepted (the others hold the plain source iteration's result) and the sweeps spent."""
+
+    collided_moments: list
+    residuals: list
+    accepted: list
+    sweeps: int
+
+
+def learned_views(solver, fields, angles: Sequence[float], net: UNet3D, threshold: float = 0.01,
+                  fallback_tolerance: float = 1e-3) -> LearnedViews:
+    """``learned_collided_moments`` of several views of one volume with batched sweeps and one batched inference:
+    phi^(1) and phi^(2) of every view (two batched sweeps), the network on all views at once, one batched judging
+    sweep; views whose relative change exceeds ``threshold`` are recomputed by the plain source iteration."""
+    from .scatter_solver.source_iteration import scatter_moments_views
+    from .scatter_solver.transport_sweep import sweep_views
+
+    V = len(angles)
+    ctx, source, T = _views_setup(solver, fields, angles)
+    phi1 = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
+    T = scatter_moments_views(fields.transfer_coarse, phi1, out=T)
+    phi2 = sweep_views(ctx, fields.mu_t_coarse, T)
+    scales = [sample_scale(_api(phi2[v])) for v in range(V)]
+    with1 = _uses_phi1(net, fields)
+    x = torch.stack([features(fields.rho_coarse, _api(phi2[v]), scales[v], _api(phi1[v]) if with1 else None)
+                     for v in range(V)])
+    pred = net.forward(x)
+    G = solver.num_groups
+    estimate = phi1.add_(phi2)  # phi1 + phi2, in place
+    del phi2
+    for v in range(V):  # predicted P1 cell means of Phi_{>2}: moments 0..3, LD component 0
+        est = estimate[v]
+        est[..., :4, 0] += (pred[v].reshape(G, 4, *est.shape[1:4]) * scales[v]).permute(0, 2, 3, 4, 1)
+    del pred, x
+    T = scatter_moments_views(fields.transfer_coarse, estimate, out=T)
+    judged = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
+    n, d = _scalar_norms_views(judged, judged - estimate)
+    residuals = [di / ni if ni > 0 else 0.0 for ni, di in zip(n, d)]
+    accepted = [r <= threshold for r in residuals]
+    result = [_api(judged[v]) for v in range(V)]
+    sweeps = 3
+    rejected = [v for v in range(V) if not accepted[v]]
+    if rejected:
+        del estimate, source, T
+        totals, extra = plain_views(solver, fields, [angles[v] for v in rejected], fallback_tolerance)
+        for v, tot in zip(rejected, totals):
+            result[v] = tot
+        sweeps += extra
+    return LearnedViews(result, residuals, accepted, sweeps)

@@ -226,3 +226,34 @@ def test_trained_surrogate_gains_over_two_orders_and_falls_back():
     forced = learned_collided_moments(solver, fields, 0.7, net, threshold=0.0, fallback_tolerance=1e-4)
     assert not forced.accepted and forced.sweeps > 3
     assert rel_err(forced.collided_moments[:, 0, 0], orders.total[:, 0, 0]) < 1e-3
+
+
+def test_batched_views_equal_single_views():
+    """Collision orders and the learned form of several views in batched sweeps equal the per-view functions."""
+    from citrus_amd.scatter_solver.geometry import Grid
+    from citrus_amd.surrogate import collision_orders_views, learned_views, plain_views
+    from citrus_amd.utilities import ellipsoid_phantom
+
+    dev = gpu_device()
+    solver = _small_solver(dev)
+    grid = Grid((16, 16, 16), 1.0)
+    fields = solver.material_fields(ellipsoid_phantom(grid, seed=77).to(dev), grid)
+    angles = [0.2, 2.5]
+    batched = collision_orders_views(solver, fields, angles, tolerance=1e-4)
+    for o, a in zip(batched, angles):
+        single = collision_orders(solver, fields, a, tolerance=1e-4)
+        assert rel_err(o.phi1, single.phi1) < F32_REL and rel_err(o.phi2, single.phi2) < F32_REL
+        assert rel_err(o.total, single.total) < 1e-4  # the batch stops when its slowest view has converged
+    totals, sweeps = plain_views(solver, fields, angles, 1e-4)
+    assert sweeps == batched[0].sweeps and all(torch.equal(t, o.total) for t, o in zip(totals, batched))
+    G = solver.num_groups
+    net = UNet3D.create(2 + 4 * G, 4 * G, width=4, device=dev, seed=2)
+    lv = learned_views(solver, fields, angles, net, threshold=1.0)
+    assert lv.accepted == [True, True] and lv.sweeps == 3
+    for v, a in enumerate(angles):
+        single = learned_collided_moments(solver, fields, a, net, threshold=1.0)
+        assert rel_err(lv.collided_moments[v], single.collided_moments) < F32_REL
+        assert abs(lv.residuals[v] - single.residual) < 1e-5
+    forced = learned_views(solver, fields, angles, net, threshold=0.0, fallback_tolerance=1e-4)
+    assert forced.accepted == [False, False] and forced.sweeps == 3 + batched[0].sweeps
+    assert all(torch.equal(m, o.total) for m, o in zip(forced.collided_moments, batched))
