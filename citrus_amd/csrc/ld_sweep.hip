@@ -94,6 +94,9 @@ struct LaunchArgs {
   const float* cdir;
   float* faces;
   int64_t tview, sview, fview;
+  const uint16_t* T16;     // bfloat16 operand of the source expansion [V][G][N][4][KP] (or NULL)
+  const float* tinv;       // [V][G] scale undoing the operand's per-group power of two
+  int64_t t16view;         // elements per view of T16
 };
 
 template <int NSH>
@@ -407,6 +410,19 @@ typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
+typedef short s4 __attribute__((ext_vector_type(4)));  // four bfloat16 bit patterns
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
+
+// D += A B with A [16][16] and B [16][16] bfloat16 (lane l: A[l%16][4(l/16)..+3], B[4(l/16)..+3][l%16]), fp32 D
+__device__ __forceinline__ f4 mfma_bf16(s4 a, s4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ short bf16_bits(float f) {  // round to nearest even
+  const unsigned b = __float_as_uint(f);
+  return (short)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
 __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
@@ -415,17 +431,23 @@ __device__ __forceinline__ f4 buf_load4(__amdgpu_buffer_rsrc_t r, unsigned off) 
   return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
 }
 
+__device__ __forceinline__ s4 buf_load_s4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(s4, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
 __device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, unsigned off, f4 v) {
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
 }
 
-template <int NSH, int DC4, bool HAS_SRC>
+template <int NSH, int DC4, bool HAS_SRC, bool BF16>
 __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   constexpr int B = 4;
   constexpr PlaneTable<B> ptab{};
   constexpr int NPL = PlaneTable<B>::kPlanes;
   constexpr int NT = (DC4 + 3) / 4;     // direction tiles of the expansion
-  constexpr int KS = (NSH + 3) / 4;     // k-steps of the expansion
+  constexpr int KS = BF16 ? 1 : (NSH + 3) / 4;  // k-steps of the float32 expansion
+  constexpr int KB = BF16 ? (NSH + 15) / 16 : 1;  // k-steps of the bfloat16 expansion (16 moments each)
+  constexpr int KP = 16 * ((NSH + 15) / 16);      // moments per row of the bfloat16 operand
   constexpr int NT2 = (NSH + 15) / 16;  // moment tiles of the reduction
   constexpr int PS = kMfmaPsiStride;
   constexpr unsigned CB = NSH * 16;     // bytes of one cell's moments
@@ -460,8 +482,14 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   float* gface_v = a.faces + (int64_t)v * a.fview + ((int64_t)slot * a.G + g) * a.nblocks * 3 * face_stride;
   const __amdgpu_buffer_rsrc_t mu_r =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.mu + (int64_t)g * a.N), 0, a.N * 4, kBufferWord3);
-  const __amdgpu_buffer_rsrc_t T_r = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4), 0, a.N * (int)CB, kBufferWord3);
+  const __amdgpu_buffer_rsrc_t T_r =
+      BF16 ? __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(a.T16 + (int64_t)v * a.t16view +
+                                                                      (int64_t)g * a.N * 4 * KP),
+                                               0, a.N * 8 * KP, kBufferWord3)
+           : __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.T + (int64_t)v * a.tview +
+                                                                   (int64_t)g * a.N * NSH * 4),
+                                               0, a.N * (int)CB, kBufferWord3);
+  const float tinv = BF16 ? a.tinv[v * a.G + g] : 1.0f;
   const __amdgpu_buffer_rsrc_t phi_r = __builtin_amdgcn_make_buffer_rsrc(
       a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4, 0, a.N * (int)CB, kBufferWord3);
   // explicit source, group-major [G][N][M][4] (layout 1: the 16 directions of one cell in a D tile are 256 contiguous
@@ -513,15 +541,28 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   }
   const int mbase = o * a.D8 + a.d0;
   float yf[NT][KS];  // expansion B fragments: Y[d = 16 nt + l%16][n = 4 s + l/16]
+  s4 yb[NT][KB];     // bfloat16 expansion B fragments: Y[d = 16 nt + l%16][n = 16 s + 4 (l/16) + 0..3]
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
     const int d = nt * 16 + r16;
     const int dd = d < dc ? d : dc - 1;
+    if (!BF16) {
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int n = 4 * s + q;
-      const float val = a.Y[(int64_t)(mbase + dd) * NSH + (n < NSH ? n : NSH - 1)];
-      yf[nt][s] = (d < dc && n < NSH) ? val : 0.0f;
+      for (int s = 0; s < KS; ++s) {
+        const int n = 4 * s + q;
+        const float val = a.Y[(int64_t)(mbase + dd) * NSH + (n < NSH ? n : NSH - 1)];
+        yf[nt][s] = (d < dc && n < NSH) ? val : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < KB; ++s) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = 16 * s + 4 * q + j;
+          const float val = a.Y[(int64_t)(mbase + dd) * NSH + (n < NSH ? n : NSH - 1)];
+          yb[nt][s][j] = bf16_bits((d < dc && n < NSH) ? val : 0.0f);
+        }
+      }
     }
   }
   float ywf[NT2][DC4];  // reduction B fragments: Yw[d = 4 s + l/16][n = 16 t + l%16]
@@ -549,6 +590,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   // The T fragments and mu_t of plane p+1 and the old moments of plane p are in flight while plane p is computed.
   struct Staged {
     float t[3][KS];
+    s4 tb[3][KB];
     float mu[3];
   };
   auto load_plane = [&](const int pl) __attribute__((always_inline)) -> Staged {
@@ -560,11 +602,17 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       if (mt * 4 >= P) break;
       const int sa = 4 * mt + acell;
       const int ca = sa < P ? slot_cell[pb + (sa < P ? sa : P - 1)] : -1;
-      const unsigned ta = ca >= 0 ? (unsigned)ca * CB + 4u * acomp : kDrop;
+      if (!BF16) {
+        const unsigned ta = ca >= 0 ? (unsigned)ca * CB + 4u * acomp : kDrop;
 #pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        const int n = 4 * s + q;
-        st.t[mt][s] = buf_load(T_r, n < NSH ? ta + 16u * n : kDrop);
+        for (int s = 0; s < KS; ++s) {
+          const int n = 4 * s + q;
+          st.t[mt][s] = buf_load(T_r, n < NSH ? ta + 16u * n : kDrop);
+        }
+      } else {  // row (cell, component): KP contiguous bfloat16 moments, four per lane and k-step
+        const unsigned tb = ca >= 0 ? ((unsigned)ca * 4u + (unsigned)acomp) * (2u * KP) + 8u * q : kDrop;
+#pragma unroll
+        for (int s = 0; s < KB; ++s) st.tb[mt][s] = buf_load_s4(T_r, ca >= 0 ? tb + 32u * s : kDrop);
       }
       const int sd = 4 * mt + q;
       const int cd = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
@@ -617,8 +665,14 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
         sacc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+        if (!BF16) {
 #pragma unroll
-        for (int s = 0; s < KS; ++s) sacc[nt] = mfma4(cur.t[mt][s], yf[nt][s], sacc[nt]);
+          for (int s = 0; s < KS; ++s) sacc[nt] = mfma4(cur.t[mt][s], yf[nt][s], sacc[nt]);
+        } else {
+#pragma unroll
+          for (int s = 0; s < KB; ++s) sacc[nt] = mfma_bf16(cur.tb[mt][s], yb[nt][s], sacc[nt]);
+          sacc[nt] *= tinv;
+        }
       }
       // (2) closed-form LD solve of (cell 4 mt + l/16, direction 16 nt + l%16)
       const int sd = 4 * mt + q;
@@ -700,19 +754,19 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
 
 size_t mfma_lds_bytes(int dc) { return sizeof(float) * (size_t)(144 * dc + 4 * kMfmaPsiStride + 128 + 2 * dc + 64); }
 
-template <int NSH, bool HAS_SRC>
+template <int NSH, bool HAS_SRC, bool BF16>
 hipError_t launch_mfma_src(const LaunchArgs& la, int nwg, hipStream_t stream) {
   const size_t lds = mfma_lds_bytes(la.dc);
   const dim3 grid(nwg), block(64);
   switch ((la.dc + 3) / 4) {
-    case 1: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 1, HAS_SRC>), grid, block, lds, stream, la); break;
-    case 2: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 2, HAS_SRC>), grid, block, lds, stream, la); break;
-    case 3: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 3, HAS_SRC>), grid, block, lds, stream, la); break;
-    case 4: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 4, HAS_SRC>), grid, block, lds, stream, la); break;
-    case 5: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 5, HAS_SRC>), grid, block, lds, stream, la); break;
-    case 6: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 6, HAS_SRC>), grid, block, lds, stream, la); break;
-    case 7: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 7, HAS_SRC>), grid, block, lds, stream, la); break;
-    case 8: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 8, HAS_SRC>), grid, block, lds, stream, la); break;
+    case 1: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 1, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
+    case 2: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 2, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
+    case 3: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 3, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
+    case 4: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 4, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
+    case 5: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 5, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
+    case 6: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 6, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
+    case 7: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 7, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
+    case 8: hipLaunchKernelGGL((ld_sweep_mfma_kernel<NSH, 8, HAS_SRC, BF16>), grid, block, lds, stream, la); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -720,10 +774,23 @@ hipError_t launch_mfma_src(const LaunchArgs& la, int nwg, hipStream_t stream) {
 
 template <int NSH>
 hipError_t launch_mfma_nsh(const LaunchArgs& la, int nwg, hipStream_t stream) {
-  return la.src ? launch_mfma_src<NSH, true>(la, nwg, stream) : launch_mfma_src<NSH, false>(la, nwg, stream);
+  return la.src ? launch_mfma_src<NSH, true, false>(la, nwg, stream)
+                : launch_mfma_src<NSH, false, false>(la, nwg, stream);
+}
+
+// the bfloat16 expansion is built for P3 and P5 (16 and 36 moments)
+template <int NSH>
+hipError_t launch_mfma_bf16(const LaunchArgs& la, int nwg, hipStream_t stream) {
+  return la.src ? launch_mfma_src<NSH, true, true>(la, nwg, stream)
+                : launch_mfma_src<NSH, false, true>(la, nwg, stream);
 }
 
 hipError_t launch_mfma(int nsh, const LaunchArgs& la, int nwg, hipStream_t stream) {
+  if (la.T16) {
+    if (nsh == 16) return launch_mfma_bf16<16>(la, nwg, stream);
+    if (nsh == 36) return launch_mfma_bf16<36>(la, nwg, stream);
+    return hipErrorInvalidValue;
+  }
   switch (nsh) {
     case 1: return launch_mfma_nsh<1>(la, nwg, stream);
     case 4: return launch_mfma_nsh<4>(la, nwg, stream);
@@ -742,7 +809,8 @@ bool use_mfma(const LaunchArgs& la, int nsh) {
   const char* sel = std::getenv("CITRUS_SWEEP_KERNEL");
   const bool vector_only = sel && std::string(sel) == "vector";
   // moments of one (view, group), and its group-major source, addressable by a buffer resource
-  return !vector_only && la.T && (!la.src || la.src_layout != 0) && nsh <= 36 && la.dc <= 32 && !la.write_psi &&
+  return !vector_only && (la.T || la.T16) && (!la.src || la.src_layout != 0) && nsh <= 36 && la.dc <= 32 &&
+         !la.write_psi &&
          (int64_t)la.N * nsh * 16 < (int64_t)kDrop && (!la.src || (int64_t)la.N * la.M * 16 < (int64_t)kDrop);
 }
 
@@ -890,9 +958,12 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   if (A.dc < 1 || A.dc > 32) return fail(CITRUS_ERR_ARG, "dc must be in 1..32");
   if (A.d0 < 0 || A.d0 + A.dc > A.D8) return fail(CITRUS_ERR_ARG, "chunk [d0, d0+dc) outside the octant");
   if (!A.mu_t || !A.phi || !A.Y || !A.Yw || !A.cdir || !A.faces) return fail(CITRUS_ERR_ARG, "NULL table or buffer");
-  if (!A.T && !A.source) return fail(CITRUS_ERR_ARG, "need T moments and/or an explicit source");
+  if (!A.T && !A.T_bf16 && !A.source) return fail(CITRUS_ERR_ARG, "need T moments and/or an explicit source");
   if (A.write_psi && !A.source) return fail(CITRUS_ERR_ARG, "write_psi needs the explicit source buffer");
   if (A.source_layout < 0 || A.source_layout > 2) return fail(CITRUS_ERR_ARG, "source_layout must be 0, 1 or 2");
+  if (A.T_bf16 && (!A.T_inv_scale || A.T)) return fail(CITRUS_ERR_ARG, "T_bf16 needs T_inv_scale and no float32 T");
+  if (A.T_bf16 && A.nsh != 16 && A.nsh != 36)
+    return fail(CITRUS_ERR_UNSUPPORTED, "the bfloat16 contraction is built for P3 and P5 (nsh 16 and 36)");
   if (A.source_layout != 0 && A.write_psi) return fail(CITRUS_ERR_ARG, "write_psi needs the cell-major layout");
   const int N = plan->R * plan->C * plan->S;
   LaunchArgs la{};
@@ -908,9 +979,13 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   la.sview = A.source_layout == 0 ? (int64_t)8 * A.dc * N * A.G * 4
                                    : (int64_t)A.G * N * la.M * (A.source_layout == 1 ? 4 : 1);
   la.fview = (int64_t)2 * A.G * plan->nblocks * 3 * (int64_t)A.dc * plan->B * plan->B * 3;
+  la.T16 = reinterpret_cast<const uint16_t*>(A.T_bf16);
+  la.tinv = A.T_inv_scale;
+  la.t16view = (int64_t)A.G * N * 4 * ((A.nsh + 15) / 16 * 16);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   la.ntask = plan->launches.empty() ? 0 : plan->launches[0].second;
   const bool mfma = use_mfma(la, A.nsh);
+  if (la.T16 && !mfma) return fail(CITRUS_ERR_UNSUPPORTED, "the bfloat16 contraction runs on the matrix-core kernel only");
   for (size_t l = 0; l < plan->launches.size(); ++l) {
     la.tasks = plan->d_tasks + plan->launches[l].first;
     la.ntask = plan->launches[l].second;

@@ -136,8 +136,90 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
   return CITRUS_OK;
 }
 
+// ---- bfloat16 operand of the source contraction -------------------------------------------------------------------
+// T [V][G][N][nsh][4] float32 -> T16 [V][G][N][4][KP] bfloat16 (KP = nsh rounded up to 16, zero padded), each
+// (view, group) scaled by a power of two 2^-e with max|T| in [2^e, 2^(e+1)), so every group's largest value is in
+// [1, 2); inv_scale[v][g] = 2^e undoes it after the contraction.
+
+constexpr int kAmaxBlock = 256;
+
+__global__ __launch_bounds__(kAmaxBlock) void group_amax_kernel(const float* __restrict__ T, int64_t per_group,
+                                                                int blocks_per_group, float* __restrict__ partial) {
+  __shared__ float red[kAmaxBlock];
+  const int vg = blockIdx.y, b = blockIdx.x;
+  const int64_t chunk = (per_group + blocks_per_group - 1) / blocks_per_group;
+  const int64_t lo = (int64_t)b * chunk, hi = lo + chunk < per_group ? lo + chunk : per_group;
+  const float* src = T + (int64_t)vg * per_group;
+  float m = 0.0f;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kAmaxBlock) m = fmaxf(m, fabsf(src[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kAmaxBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(int64_t)vg * blocks_per_group + b] = red[0];
+}
+
+__global__ void group_scale_kernel(const float* __restrict__ partial, int blocks_per_group, int n,
+                                   float* __restrict__ scale, float* __restrict__ inv_scale) {
+  const int vg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (vg >= n) return;
+  float m = 0.0f;
+  for (int b = 0; b < blocks_per_group; ++b) m = fmaxf(m, partial[(int64_t)vg * blocks_per_group + b]);
+  const int e = m > 0.0f ? ilogbf(m) : 0;
+  scale[vg] = ldexpf(1.0f, -e);
+  inv_scale[vg] = ldexpf(1.0f, e);
+}
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+__global__ void to_bf16_operand_kernel(const float* __restrict__ T, int nsh, int kp, int64_t N, int64_t n_vg,
+                                       const float* __restrict__ scale, uint16_t* __restrict__ T16) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (vg, cell, comp, n), n fastest
+  const int64_t total = n_vg * N * 4 * kp;
+  if (i >= total) return;
+  const int n = (int)(i % kp);
+  const int comp = (int)((i / kp) % 4);
+  const int64_t cell_vg = i / (4 * (int64_t)kp);  // vg * N + cell
+  const int64_t vg = cell_vg / N;
+  T16[i] = n < nsh ? bf16_rne(T[(cell_vg * nsh + n) * 4 + comp] * scale[vg]) : (uint16_t)0;
+}
+
 }  // namespace
 }  // namespace citrus
+
+extern "C" int64_t citrus_bf16_operand_workspace_floats(int G, int V) {
+  if (G < 1 || V < 1) return -1;
+  return (int64_t)G * V * (64 + 1);
+}
+
+extern "C" int citrus_bf16_operand(const float* T, int G, int nsh, int64_t N, int V, uint16_t* T16, float* inv_scale,
+                                   float* workspace, void* stream) {
+  using namespace citrus;
+  if (!T || !T16 || !inv_scale || !workspace) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (G < 1 || V < 1 || N < 1 || nsh < 1 || nsh > 64) return fail(CITRUS_ERR_ARG, "bad sizes");
+  const int kp = (nsh + 15) / 16 * 16;
+  const int blocks_per_group = 64;
+  const int n_vg = G * V;
+  float* partial = workspace;
+  float* scale = workspace + (int64_t)n_vg * blocks_per_group;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(group_amax_kernel, dim3(blocks_per_group, n_vg), dim3(kAmaxBlock), 0, s, T, N * nsh * 4,
+                     blocks_per_group, partial);
+  hipLaunchKernelGGL(group_scale_kernel, dim3((n_vg + 63) / 64), dim3(64), 0, s, partial, blocks_per_group, n_vg,
+                     scale, inv_scale);
+  const int64_t total = (int64_t)n_vg * N * 4 * kp;
+  if ((total + 255) / 256 > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "operand too large");
+  hipLaunchKernelGGL(to_bf16_operand_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, T, nsh, kp, N,
+                     (int64_t)n_vg, scale, T16);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("bf16 operand: ") + hipGetErrorString(e));
+  return CITRUS_OK;
+}
 
 extern "C" int citrus_scatter_moments_dense(const float* transfer, int Lt, const float* phi, float* T, int G, int nsh,
                                             int64_t N, int V, void* stream) {

@@ -51,8 +51,10 @@ class SweepSchedule:
 
     def run(self, *, mu: Tensor, phi: Tensor, Y: Tensor, Yw: Tensor, cdir: Tensor, G: int, nsh: int, D8: int,
             d0: int, dc: int, V: int = 1, T: Optional[Tensor] = None, source: Optional[Tensor] = None,
-            accumulate: bool = False, write_psi: bool = False, source_layout: int = 0) -> None:
-        """Enqueue one chunk sweep on torch's current stream (no host synchronization)."""
+            accumulate: bool = False, write_psi: bool = False, source_layout: int = 0,
+            T16: Optional[Tensor] = None, T_inv_scale: Optional[Tensor] = None) -> None:
+        """Enqueue one chunk sweep on torch's current stream (no host synchronization). ``T16``/``T_inv_scale`` (the
+        bfloat16 operand of ``bf16_operand``) replace ``T`` for the bfloat16 source contraction."""
         N = self.shape[0] * self.shape[1] * self.shape[2]
         for name, t in (("mu", mu), ("phi", phi), ("Y", Y), ("Yw", Yw), ("cdir", cdir)):
             _hip.require_gpu_tensor(t, name)
@@ -62,6 +64,16 @@ class SweepSchedule:
             _hip.require_gpu_tensor(T, "T")
             if T.numel() != V * G * N * nsh * 4:
                 raise ValueError("T size does not match [V, G, N, nsh, 4]")
+        if T16 is not None:
+            kp = (nsh + 15) // 16 * 16
+            if T is not None or T_inv_scale is None:
+                raise ValueError("T16 replaces T and needs T_inv_scale")
+            if T16.dtype != torch.bfloat16 or T16.device.type != "cuda" or not T16.is_contiguous() \
+                    or T16.numel() != V * G * N * 4 * kp:
+                raise ValueError(f"T16 must be contiguous bfloat16 [V, G, N, 4, {kp}] on the GPU")
+            _hip.require_gpu_tensor(T_inv_scale, "T_inv_scale")
+            if T_inv_scale.numel() != V * G:
+                raise ValueError("T_inv_scale must hold V * G scales")
         if source is not None:
             _hip.require_gpu_tensor(source, "source")
             if source_layout == 0 and source.numel() != V * 8 * dc * N * G * 4:
@@ -80,7 +92,8 @@ class SweepSchedule:
         args = _hip.SweepArgs(G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V, accumulate=int(accumulate),
                               write_psi=int(write_psi), mu_t=mu.data_ptr(), T=_hip.ptr(T), source=_hip.ptr(source),
                               phi=phi.data_ptr(), Y=Y.data_ptr(), Yw=Yw.data_ptr(), cdir=cdir.data_ptr(),
-                              faces=faces.data_ptr(), source_layout=int(source_layout))
+                              faces=faces.data_ptr(), source_layout=int(source_layout), T_bf16=_hip.ptr(T16),
+                              T_inv_scale=_hip.ptr(T_inv_scale))
         lib = _hip.library()
         with torch.cuda.device(self.device):
             _hip.check(lib.citrus_sweep_run(self.handle, ctypes.byref(args), _hip.stream_of(self.device)),

@@ -175,6 +175,27 @@ def scatter_moments_views(transfer: MaterialTransfer, phi_views: Tensor, out: Op
     return out
 
 
+def bf16_operand(moments: Tensor) -> tuple[Tensor, Tensor]:
+    """(T16 [V, G, N, 4, KP] bfloat16, inv_scale [V, G] float32): the scattering-source moments [V, G, R, C, S, nsh, 4]
+    as the bfloat16 operand of the source contraction (``citrus_bf16_operand``), each (view, group) scaled by the
+    power of two that puts its largest magnitude in [1, 2)."""
+    V, G, R, C, S, nsh, four = (int(v) for v in moments.shape)
+    _hip.require_gpu_tensor(moments, "moments")
+    if not moments.is_contiguous() or four != 4:
+        raise ValueError("moments must be contiguous [V, G, R, C, S, nsh, 4]")
+    N = R * C * S
+    kp = (nsh + 15) // 16 * 16
+    T16 = torch.empty(V, G, N, 4, kp, device=moments.device, dtype=torch.bfloat16)
+    inv = torch.empty(V, G, device=moments.device, dtype=torch.float32)
+    lib = _hip.library()
+    ws = torch.empty(int(lib.citrus_bf16_operand_workspace_floats(G, V)), device=moments.device,
+                     dtype=torch.float32)
+    with torch.cuda.device(moments.device):
+        _hip.check(lib.citrus_bf16_operand(moments.data_ptr(), G, nsh, N, V, T16.data_ptr(), inv.data_ptr(),
+                                           ws.data_ptr(), _hip.stream_of(moments.device)), "citrus_bf16_operand")
+    return T16, inv
+
+
 def multiple_collision_source_moments(mu_s_l_volume_coarse, phi_total: Tensor) -> Tensor:
     """Scattering operator applied to the total collided fluence (source_iteration.py:109)."""
     return scatter_moments(mu_s_l_volume_coarse, phi_total)

@@ -300,7 +300,8 @@ def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: Sourc
 
 
 def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out: Optional[Tensor] = None,
-                direction_chunk: Optional[int] = None, source: Optional[Tensor] = None) -> Tensor:
+                direction_chunk: Optional[int] = None, source: Optional[Tensor] = None,
+                contraction: str = "fp32") -> Tensor:
     """Sweep V views that share the material fields, each from its own scattering-source moments, in one launch
     sequence: the batched form of one ``sweep`` per view of an outer iteration (citrus.py:1920-1935).
 
@@ -313,6 +314,8 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         source: optional explicit source added to the moments' expansion, every direction of every view:
             [V, G, R, C, S, M, 4] LD moments or [V, G, R, C, S, M] cell means (zero slopes), M the reordered
             directions (the summed source of the CITRUS step, citrus.py:232-236).
+        contraction: "fp32", or "bf16" for the source expansion as a bfloat16 matrix product with float32
+            accumulation, the moments scaled per (view, group) by a power of two (P3 and P5).
 
     Returns:
         [V, G, R, C, S, nsh, 4] fluence moments.
@@ -338,14 +341,22 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             raise ValueError(f"source must be {base + (4,)} or {base}, got {tuple(source.shape)}")
         if not source.is_contiguous() or source.device != moments.device or source.dtype != torch.float32:
             raise ValueError("source must be a contiguous float32 tensor on the moments' device")
+    if contraction not in ("fp32", "bf16"):
+        raise ValueError(f"contraction must be 'fp32' or 'bf16', got {contraction!r}")
     if out is None:
         out = torch.empty_like(moments)
     D8 = sweep_context.directions_per_octant
     Dc = _resolve_chunk(sweep_context, direction_chunk)
     hip = sweep_context.hip
     mu = mu_t.reshape(G, -1).contiguous()
+    T, T16, tinv = moments, None, None
+    if contraction == "bf16":
+        from .source_iteration import bf16_operand
+
+        T16, tinv = bf16_operand(moments)
+        T = None
     for d0 in range(0, D8, Dc):
         dc = min(d0 + Dc, D8) - d0
         hip.schedule.run(mu=mu, phi=out, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V,
-                         T=moments, accumulate=d0 > 0, source=source, source_layout=layout)
+                         T=T, accumulate=d0 > 0, source=source, source_layout=layout, T16=T16, T_inv_scale=tinv)
     return out

@@ -78,6 +78,9 @@ typedef struct CitrusSweepArgs {
                           * 1: group-major [V][G][N][8*D8][4] of every direction, indexed by the reordered direction
                           *    m = o*D8 + d (not with write_psi);
                           * 2: group-major cell means [V][G][N][8*D8], slopes zero (not with write_psi) */
+  const void* T_bf16;    /* in place of T: bfloat16 operand [V][G][N][4][KP] from citrus_bf16_operand (P3 and P5);
+                          * the source expansion then runs as a bfloat16 matrix product with float32 accumulation */
+  const float* T_inv_scale; /* [V][G] with T_bf16: the scale undoing the operand's per-group power of two */
 } CitrusSweepArgs;
 
 /* Sweep one direction chunk of every octant, all groups and views, through the grid with vacuum boundaries, and
@@ -266,6 +269,15 @@ int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, int t
 /* Footprint box-filter matrices out [k][N][N], B[c][j] = |[j, j+1] n W_c| / |W_c|, W_c the window of width
  * widths[m] (pixels) centred on pixel c clipped to [0, N] (box_matrix, scatter_image.py:952). */
 int citrus_box_matrix(const float* widths, int k, int N, float* out, void* stream);
+
+/* bfloat16 operand of the scattering-source contraction (configs with the mixed-precision contraction of
+ * scattering_kernels.py:162 / transport_sweep.py:524-528): T [V][G][N][nsh][4] float32 -> T16 [V][G][N][4][KP]
+ * (KP = nsh rounded up to 16, zero padded; bfloat16 bit patterns, round to nearest even) of T * 2^-e per (view, group)
+ * with max|T| in [2^e, 2^(e+1)), and inv_scale [V][G] = 2^e. Deterministic (fixed-order max). workspace holds
+ * citrus_bf16_operand_workspace_floats(G, V) floats. */
+int64_t citrus_bf16_operand_workspace_floats(int G, int V);
+int citrus_bf16_operand(const float* T, int G, int nsh, int64_t N, int V, uint16_t* T16, float* inv_scale,
+                        float* workspace, void* stream);
 
 /* ---- Phi_{>2} surrogate network ------------------------------------------------------------------------------
  * No counterpart in the reference: the learned part of this path, a 3D U-Net that predicts the collided moments of

@@ -149,3 +149,20 @@ def test_views_step_equals_view_steps(spacing, q):
         citrus_views_step(solver, batched, fields, angles[:2])
     with pytest.raises(ValueError):
         citrus_views_step(solver, batched, fields, [a + 0.1 for a in angles])
+
+
+def test_views_step_bf16_contraction_error():
+    """The view step with the bfloat16 source contraction: scatter image within 1e-3 and accumulator within the
+    bfloat16 rounding of the float32 step (two steps, so the contraction acts on a nonzero accumulator)."""
+    dev = gpu_device()
+    solver, grid, rho = _solver(dev, 1.0)
+    fields = solver.material_fields(rho, grid)
+    angles = [0.4, 2.0]
+    s32 = [CitrusState() for _ in angles]
+    s16 = [CitrusState() for _ in angles]
+    for _ in range(2):
+        r32, s32 = citrus_views_step(solver, s32, fields, angles)
+        r16, s16 = citrus_views_step(solver, s16, fields, angles, contraction="bf16")
+    for v in range(len(angles)):
+        assert rel_err(r16[v][0], r32[v][0]) < 1e-3
+        assert rel_err(s16[v].collided_moments, s32[v].collided_moments) < 4e-3

@@ -247,3 +247,62 @@ def test_group_major_explicit_source_with_moments(kernel, chunk, means, monkeypa
         extra_cm.double().cpu().permute(0, 4, 5, 1, 2, 3).numpy()
     psi = osw.sweep(shape, grid.deltas, om, mu.double().cpu().numpy(), src)
     assert rel_err(phi_gm, osw.moments(psi, w, Y)) < F32_REL
+
+
+def test_bf16_operand_is_the_scaled_rounding():
+    """The bfloat16 operand equals round-to-nearest-even of T * 2^-e per (view, group), e = floor(log2 max|T|),
+    zero padded to 48 moments; inv_scale = 2^e."""
+    from citrus_amd.scatter_solver.source_iteration import bf16_operand
+    from oracle import surrogate as osg
+
+    dev = gpu_device()
+    gen = torch.Generator().manual_seed(21)
+    V, G, shape, nsh = 2, 3, (3, 4, 5), 36
+    T = torch.randn(V, G, *shape, nsh, 4, generator=gen)
+    T *= torch.tensor([1e-3, 5.0, 3e4]).reshape(1, G, 1, 1, 1, 1, 1)  # groups of very different magnitudes
+    T16, inv = bf16_operand(T.to(dev))
+    Tn = T.double().numpy().reshape(V, G, -1, nsh, 4)
+    for v in range(V):
+        for g in range(G):
+            e = math.floor(math.log2(np.abs(Tn[v, g]).max()))
+            assert float(inv[v, g]) == 2.0 ** e
+            scaled = (Tn[v, g] * 2.0 ** -e).astype(np.float32)  # exact: a power of two
+            want = osg.bf16_round(scaled).transpose(0, 2, 1)  # [N, 4, nsh]
+            got = T16[v, g].double().cpu().numpy()
+            assert np.array_equal(got[..., :nsh], want) and not got[..., nsh:].any()
+
+
+@pytest.mark.parametrize("L", [3, 5])
+def test_bf16_contraction_sweep(L):
+    """Source expansion as a bfloat16 matrix product with float32 accumulation and per-group scales: within the
+    bfloat16 rounding of the source of the float32 sweep and of the FP64 oracle; unsupported orders refused."""
+    from citrus_amd.scatter_solver.transport_sweep import sweep_views
+
+    dev = gpu_device()
+    shape, G = (6, 5, 7), 3
+    grid = Grid(shape, 0.5)
+    q = level_symmetric_quadrature(8)
+    ctx = make_sweep_context(grid, q, L, device=dev)
+    gen = torch.Generator().manual_seed(23)
+    mu = (torch.rand(G, *shape, generator=gen) + 0.3).to(dev)
+    T = torch.randn(2, G, *shape, ctx.num_moments, 4, generator=gen) * 0.2
+    T[..., 0, 0] = T[..., 0, 0].abs() + 0.5
+    T *= torch.tensor([1.0, 1e-3, 40.0]).reshape(1, G, 1, 1, 1, 1, 1)
+    Td = T.to(dev).contiguous()
+    f32 = sweep_views(ctx, mu, Td)
+    b16 = sweep_views(ctx, mu, Td, contraction="bf16")
+    for g in range(G):  # each group against its own magnitude
+        assert rel_err(b16[:, g], f32[:, g]) < 4e-3
+    om = q.omega[ctx.permutation.cpu()].numpy()
+    w = q.weight[ctx.permutation.cpu()].numpy()
+    Y = oq.real_spherical_harmonics(L, om)
+    for v in range(2):
+        Tv = T[v].double().permute(0, 4, 5, 1, 2, 3).numpy()
+        psi = osw.sweep(shape, grid.deltas, om, mu.double().cpu().numpy(), osw.sources_from_moments(Tv, Y))
+        ref = osw.moments(psi, w, Y)
+        got = b16[v].permute(0, 4, 5, 1, 2, 3)
+        for g in range(G):
+            assert rel_err(got[g], ref[g]) < 4e-3
+    ctx2 = make_sweep_context(grid, q, 2, device=dev)
+    with pytest.raises(RuntimeError, match="P3 and P5"):
+        sweep_views(ctx2, mu, torch.zeros(1, G, *shape, 9, 4, device=dev), contraction="bf16")
