@@ -429,6 +429,19 @@ def main():
     sweep_total_ms = s_start.elapsed_time(s_stop)
     sweep_ms = sum(a.elapsed_time(b) for a, b in ev) / sweep_steps  # per step = V view-sweeps
     assert bool(torch.isfinite(phi_a).all()), "non-finite moments"
+    # the same batched sweep captured once in a HIP graph and replayed
+    from citrus_amd.scatter_solver.transport_sweep import SweepGraph
+
+    graph = SweepGraph(ctx, mu_t, T)
+    torch.cuda.synchronize()
+    g_start, g_stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g_start.record(stream)
+    for _ in range(sweep_steps):
+        graph.replay()
+    g_stop.record(stream)
+    torch.cuda.synchronize()
+    graph_ms = g_start.elapsed_time(g_stop) / sweep_steps
+    del graph
     G, nsh = 16, ctx.num_moments
     N = ctx.num_cells
     # compulsory bytes of one view-sweep in the sweep kernel: moments T in and phi out (float32) plus mu_t; the
@@ -464,6 +477,7 @@ def main():
                    "grid": f"{coarse.shape[0]}^3 cells at {coarse.delta_voxel:.2f} cm", "groups": G,
                    "directions": ctx.num_directions, "moments": nsh, "views_per_step": V,
                    "sweep_only_view_sweeps_per_s": sweep_rate, "sweep_ms_per_view": sweep_ms / V,
+                   "sweep_graph_ms_per_view": graph_ms / V,
                    "wall_s": wall,
                    **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),
                    **({"learned_vs_plain": view["learned"]} if view and "learned" in view else {}),
