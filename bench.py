@@ -345,6 +345,14 @@ def main():
     n = args.cells
     rho_fine = ellipsoid_phantom(Grid((4 * n, 4 * n, 4 * n), 0.08 * 64 / n), seed=1234)
 
+    # ---- launch shape: the autotuner's direction chunk for V views of this grid (measured, outside the timing) --
+    from citrus_amd.scatter_solver.autotune import default_tuner
+
+    ctx_t, mu_t_t = build_problem(argparse.Namespace(**{**vars(args), "views": 1}), device)[:2]
+    chunk = default_tuner().best_chunk(ctx_t, mu_t_t, V)
+    tuned = {str(k): round(v, 3) for k, v in default_tuner().timings[default_tuner().key(ctx_t, 16, V, "fp32")].items()}
+    del ctx_t, mu_t_t
+
     # ---- CITRUS view steps (the metric) --------------------------------------------------------------------
     view = None
     if not args.sweep_only:
@@ -352,7 +360,8 @@ def main():
         states = [CitrusState() for _ in range(V)]
 
         def view_step(contraction="fp32", st=states):
-            results, st[:] = citrus_views_step(solver, st, fields, angles, contraction=contraction)
+            results, st[:] = citrus_views_step(solver, st, fields, angles, contraction=contraction,
+                                               direction_chunk=chunk)
             return results
 
         for _ in range(max(1, args.warmup)):  # the first step of a view has no accumulator yet
@@ -409,7 +418,7 @@ def main():
     T = torch.empty_like(phi_a)
     for _ in range(1):
         scatter_moments_views(transfer, phi_a, out=T)
-        sweep_views(ctx, mu_t, T, out=phi_b)
+        sweep_views(ctx, mu_t, T, out=phi_b, direction_chunk=chunk)
         phi_a, phi_b = phi_b, phi_a
     torch.cuda.synchronize()
     sweep_steps = 2 if view is not None else args.steps
@@ -420,7 +429,7 @@ def main():
     for s in range(sweep_steps):
         scatter_moments_views(transfer, phi_a, out=T)
         ev[s][0].record(stream)
-        sweep_views(ctx, mu_t, T, out=phi_b)
+        sweep_views(ctx, mu_t, T, out=phi_b, direction_chunk=chunk)
         ev[s][1].record(stream)
         phi_a, phi_b = phi_b, phi_a
     s_stop.record(stream)
@@ -432,7 +441,7 @@ def main():
     # the same batched sweep captured once in a HIP graph and replayed
     from citrus_amd.scatter_solver.transport_sweep import SweepGraph
 
-    graph = SweepGraph(ctx, mu_t, T)
+    graph = SweepGraph(ctx, mu_t, T, direction_chunk=chunk)
     torch.cuda.synchronize()
     g_start, g_stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     g_start.record(stream)
@@ -447,9 +456,10 @@ def main():
     # compulsory bytes of one view-sweep in the sweep kernel: moments T in and phi out (float32) plus mu_t; the
     # launches of a sweep split it by block planes, so bytes per launch / launch time averages to the same ratio
     alg_bytes = 2 * G * N * nsh * 4 * 4 + G * N * 4
-    launches = sweep_launches(coarse.shape[0])
+    n_chunks = -(-ctx.directions_per_octant // chunk)
+    launches = sweep_launches(coarse.shape[0]) * n_chunks
     achieved = alg_bytes * V / (sweep_ms * 1e-3) / 1e9
-    kernel = f"ld_sweep_mfma_kernel<{nsh}, {(ctx.directions_per_octant + 3) // 4}>"
+    kernel = f"ld_sweep_mfma_kernel<{nsh}, {(chunk + 3) // 4}>"
     traffic = profiled_traffic(kernel, V)
     sweep_rate = V * sweep_steps / (sweep_total_ms * 1e-3)
     if view is not None:
@@ -478,6 +488,7 @@ def main():
                    "directions": ctx.num_directions, "moments": nsh, "views_per_step": V,
                    "sweep_only_view_sweeps_per_s": sweep_rate, "sweep_ms_per_view": sweep_ms / V,
                    "sweep_graph_ms_per_view": graph_ms / V,
+                   "direction_chunk": chunk, "tuner_ms_per_chunk": tuned,
                    "wall_s": wall,
                    **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),
                    **({"learned_vs_plain": view["learned"]} if view and "learned" in view else {}),

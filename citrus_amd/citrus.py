@@ -128,14 +128,16 @@ def citrus_view_step(solver: ScatterSolver, state: CitrusState, fields: Material
 
 
 def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fields: MaterialFields,
-                      angles: Sequence[float], *, primary: bool = True, contraction: str = "fp32"):
+                      angles: Sequence[float], *, primary: bool = True, contraction: str = "fp32",
+                      direction_chunk=None):
     """``citrus_view_step`` of V views that share the densities, batched for the GPU: per view the uncollided
     fluence and the corrected first-collision source (group-major, every sweep direction); the scattering moments of
     every accumulator; then the V accumulator updates ``Phi_new = G(S_FC + K Phi)`` in one sweep with the views
     batched in every launch (the reference loops over views one step at a time, citrus.py:1907-1970); then per view
     the signal-only image and the primary. Returns ([(scatter, primary or None)] per view, [new state] per view);
     each view's result equals its own ``citrus_view_step`` to float32 rounding. ``contraction="bf16"`` runs the
-    sweep's scattering-source expansion as a bfloat16 matrix product (float32 accumulation, per-group scales)."""
+    sweep's scattering-source expansion as a bfloat16 matrix product (float32 accumulation, per-group scales);
+    ``direction_chunk`` is passed to ``sweep_views`` ("auto": the autotuner's choice)."""
     V = len(states)
     if V < 1 or len(angles) != V:
         raise ValueError(f"need one angle per state, got {len(angles)} angles for {V} states")
@@ -166,7 +168,8 @@ def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fiel
                 _check_accumulator(solver, fields, st.collided_moments)
                 prev = st.collided_moments.to(device, torch.float32).permute(0, 3, 4, 5, 1, 2).contiguous()
                 scatter_moments_views(fields.transfer_coarse, prev[None], out=T[v:v + 1])
-    collided = sweep_views(ctx, fields.mu_t_coarse, T, source=source, contraction=contraction)
+    collided = sweep_views(ctx, fields.mu_t_coarse, T, source=source, contraction=contraction,
+                           direction_chunk=direction_chunk)
     del source
     # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same
     T = scatter_moments_views(fields.transfer_coarse, collided)

@@ -1,0 +1,84 @@
+"""Autotuning of the batched sweep's launch shape.
+
+The matrix-core sweep takes a direction chunk (directions per octant per launch sequence, at most 32): all D8
+directions in one sequence of block-diagonal launches, or fewer, larger-occupancy launches repeated per chunk. The
+best chunk depends on the grid, the groups, the views per launch and the contraction, so it is measured: each
+candidate sweeps synthetic moments of the real shape (one warm-up, then the fastest of ``repeats`` timed runs with HIP
+events on torch's stream), and the fastest is kept per configuration for the life of the process. The tile of the
+cell planes (4^3 blocks) and one group per wave are fixed by the kernel.
+"""
+
+from __future__ import annotations
+
+from typing import Callable, Optional, Sequence
+
+import torch
+from torch import Tensor
+
+
+def chunk_candidates(D8: int) -> list[int]:
+    """Direction chunks worth timing: all directions of an octant (when <= 32), about half and a third of them, and 8."""
+    out = []
+    for c in (D8, (D8 + 1) // 2, (D8 + 2) // 3, 8):
+        c = min(c, 32)
+        if 1 <= c <= D8 and c not in out:
+            out.append(c)
+    return out
+
+
+class SweepTuner:
+    """Fastest direction chunk of ``sweep_views`` per (grid shape, moments, directions, groups, views, contraction)."""
+
+    def __init__(self, repeats: int = 2):
+        self.repeats = int(repeats)
+        self.cache: dict = {}
+        self.timings: dict = {}
+
+    def key(self, context, G: int, V: int, contraction: str) -> tuple:
+        return (tuple(context.shape), context.num_moments, context.directions_per_octant, int(G), int(V), contraction)
+
+    def best_chunk(self, context, mu_t: Tensor, V: int, contraction: str = "fp32",
+                   candidates: Optional[Sequence[int]] = None,
+                   run: Optional[Callable[[int], None]] = None) -> int:
+        """The fastest chunk for this configuration (measured on first use, then cached)."""
+        G = int(mu_t.shape[0])
+        key = self.key(context, G, V, contraction)
+        if key in self.cache:
+            return self.cache[key]
+        from .transport_sweep import sweep_views
+
+        cands = list(candidates) if candidates is not None else chunk_candidates(context.directions_per_octant)
+        if run is None:
+            R, C, S = context.shape
+            T = torch.zeros(V, G, R, C, S, context.num_moments, 4, device=mu_t.device, dtype=torch.float32)
+            T[..., 0, 0] = 1.0
+            out = torch.empty_like(T)
+
+            def run(chunk: int) -> None:
+                sweep_views(context, mu_t, T, out=out, direction_chunk=chunk, contraction=contraction)
+
+        stream = torch.cuda.current_stream(mu_t.device)
+        times = {}
+        for c in cands:
+            run(c)  # warm-up
+            best = float("inf")
+            for _ in range(self.repeats):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                run(c)
+                b.record(stream)
+                b.synchronize()
+                best = min(best, a.elapsed_time(b))
+            times[c] = best
+        choice = min(times, key=times.get)
+        self.cache[key] = choice
+        self.timings[key] = times
+        return choice
+
+
+_TUNER = SweepTuner()
+
+
+def default_tuner() -> SweepTuner:
+    """The process-wide tuner ``sweep_views(direction_chunk="auto")`` consults."""
+    return _TUNER

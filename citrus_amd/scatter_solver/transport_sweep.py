@@ -310,7 +310,8 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         mu_t: [G, R, C, S] float32 total attenuation shared by the views.
         moments: [V, G, R, C, S, nsh, 4] float32 contiguous scattering-source moments (group-major layout).
         out: optional [V, G, R, C, S, nsh, 4] float32 output buffer.
-        direction_chunk: directions per octant per launch sequence. Defaults to all.
+        direction_chunk: directions per octant per launch sequence. Defaults to all; "auto" takes the fastest
+            measured by the autotuner for this configuration.
         source: optional explicit source added to the moments' expansion, every direction of every view:
             [V, G, R, C, S, M, 4] LD moments or [V, G, R, C, S, M] cell means (zero slopes), M the reordered
             directions (the summed source of the CITRUS step, citrus.py:232-236).
@@ -346,6 +347,10 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     if out is None:
         out = torch.empty_like(moments)
     D8 = sweep_context.directions_per_octant
+    if direction_chunk == "auto":  # measured once per configuration (autotune.SweepTuner)
+        from .autotune import default_tuner
+
+        direction_chunk = default_tuner().best_chunk(sweep_context, mu_t, V, contraction)
     Dc = _resolve_chunk(sweep_context, direction_chunk)
     hip = sweep_context.hip
     mu = mu_t.reshape(G, -1).contiguous()

@@ -85,3 +85,12 @@ def test_coarsen_density_block_means():
     assert torch.allclose(c.sum() * 8, rho.sum())
     with pytest.raises(ValueError):
         coarsen_density(rho, fine, Grid((3, 3, 1), 1.0))
+
+
+def test_sweep_tuner_candidates():
+    from citrus_amd.scatter_solver.autotune import chunk_candidates
+
+    assert chunk_candidates(21) == [21, 11, 7, 8]
+    assert chunk_candidates(10) == [10, 5, 4, 8]
+    assert chunk_candidates(3) == [3, 2, 1]
+    assert chunk_candidates(40) == [32, 20, 14, 8]

@@ -333,3 +333,24 @@ def test_sweep_graph_replays_equal_eager(contraction, with_source):
         torch.cuda.synchronize()
         assert torch.equal(got, want), step
         del noise
+
+
+def test_sweep_autotuner_picks_and_caches_a_chunk():
+    from citrus_amd.scatter_solver.autotune import SweepTuner, chunk_candidates, default_tuner
+    from citrus_amd.scatter_solver.transport_sweep import sweep_views
+
+    dev = gpu_device()
+    shape, G, V = (8, 8, 8), 2, 2
+    ctx = make_sweep_context(Grid(shape, 0.5), level_symmetric_quadrature(8), 2, device=dev)
+    gen = torch.Generator().manual_seed(31)
+    mu = (torch.rand(G, *shape, generator=gen) + 0.3).to(dev)
+    tuner = SweepTuner(repeats=1)
+    calls = []
+    chunk = tuner.best_chunk(ctx, mu, V)
+    assert chunk in chunk_candidates(ctx.directions_per_octant)
+    assert tuner.best_chunk(ctx, mu, V, run=lambda c: calls.append(c)) == chunk and not calls  # cached
+    T = torch.randn(V, G, *shape, ctx.num_moments, 4, generator=gen).to(dev)
+    auto = sweep_views(ctx, mu, T, direction_chunk="auto")
+    chosen = default_tuner().cache[default_tuner().key(ctx, G, V, "fp32")]
+    assert rel_err(auto, sweep_views(ctx, mu, T, direction_chunk=chosen)) < F32_REL
+    assert rel_err(auto, sweep_views(ctx, mu, T)) < F32_REL
