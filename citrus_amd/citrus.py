@@ -199,6 +199,34 @@ def _previous_scatter_moments(states: Sequence[CitrusState], fields: MaterialFie
     return batch[0]
 
 
+def citrus_cropped_views_step(solver: ScatterSolver, states: Sequence[CitrusState], rho_fine: Tensor,
+                              grid_fine: Grid, angles: Sequence[float], *, primary: bool = True,
+                              contraction: str = "fp32"):
+    """The step of V views of a helical (or any) scan, each on the transport grid cropped to its beam
+    (``cropping.beam_slice_range``, whole transport cells): views with the same slab share its material fields and
+    run as one ``citrus_views_step`` batch. Returns ([(scatter, primary or None)], [new state], [slab (k0, k1) of the
+    fine grid]) per view; a view's state belongs to its slab."""
+    from .scatter_solver.cropping import beam_slice_range, crop_slices
+
+    if len(states) != len(angles):
+        raise ValueError(f"need one angle per state, got {len(angles)} angles for {len(states)} states")
+    cell = solver.transport_grid(grid_fine)
+    factor = max(1, round(cell.deltas[2] / grid_fine.deltas[2]))
+    slabs = [beam_slice_range(solver.geometry, a, grid_fine, factor) for a in angles]
+    results: list = [None] * len(angles)
+    new_states: list = [None] * len(angles)
+    for slab in sorted(set(slabs)):
+        idx = [i for i, s in enumerate(slabs) if s == slab]
+        sub, rho_sub = crop_slices(grid_fine, rho_fine, *slab)
+        fields = solver.material_fields(rho_sub, sub)
+        r, st = citrus_views_step(solver, [states[i] for i in idx], fields, [angles[i] for i in idx],
+                                  primary=primary, contraction=contraction)
+        for j, i in enumerate(idx):
+            results[i], new_states[i] = r[j], st[j]
+        del fields
+    return results, new_states, slabs
+
+
 def _change(new: Tensor, state: CitrusState) -> float:
     """Relative change of the scalar-flux cell means over one step (1 from an empty state with nonzero fluence)."""
     if state.collided_moments is None:

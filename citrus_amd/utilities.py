@@ -3,13 +3,14 @@
 A water cylinder (1.0 g/cm^3, axis along z) holding 3-6 bone ellipsoids (density U[1.4, 1.9]) and 1-3 air
 ellipsoids (0.001 g/cm^3), returned as two material channels (water-like, bone-like) of densities in g/cm^3 — the
 ``rho [M, R, C, S]`` input of the solver (the reference's WLAB phantom, citrus/utilities.py:104, plays the same role
-for its clinical geometry).
+for its clinical geometry). Also the scan helpers ``view_angles`` and ``helical_scan``.
 """
 
 from __future__ import annotations
 
 import math
 
+import numpy as np
 import torch
 from torch import Tensor
 
@@ -56,3 +57,23 @@ def coarsen(rho: Tensor, factor: int) -> Tensor:
     M, R, C, S = rho.shape
     f = int(factor)
     return rho.reshape(M, R // f, f, C // f, f, S // f, f).mean(dim=(2, 4, 6))
+
+
+def view_angles(num_views: int, start_view_angle: float = 0.0, end_view_angle: float = 2.0 * math.pi):
+    """[V] float64 view angles ``start + (end - start) i / V`` (utilities.py:1119; the end is excluded)."""
+    n = int(num_views)
+    if n < 1:
+        raise ValueError("num_views must be >= 1")
+    if not float(end_view_angle) > float(start_view_angle):
+        raise ValueError("end_view_angle must be larger than start_view_angle")
+    return float(start_view_angle) + (float(end_view_angle) - float(start_view_angle)) * \
+        np.arange(n, dtype=np.float64) / n
+
+
+def helical_scan(geometry, angles, pitch_cm: float, center_cm: float = 0.0):
+    """Geometry of a helical scan with constant pitch (utilities.py:1153): the z shift of view angle a is
+    ``center_cm + pitch_cm (a - a_mid) / (2 pi)``, a_mid the middle of the angular range."""
+    a = np.asarray(angles.detach().cpu().numpy() if isinstance(angles, Tensor) else angles,
+                   dtype=np.float64).reshape(-1)
+    middle = 0.5 * (float(a.min()) + float(a.max())) if a.size else 0.0
+    return geometry.with_helical_z_shifts(a, float(center_cm) + float(pitch_cm) * (a - middle) / (2.0 * math.pi))

@@ -166,3 +166,33 @@ def test_views_step_bf16_contraction_error():
     for v in range(len(angles)):
         assert rel_err(r16[v][0], r32[v][0]) < 1e-3
         assert rel_err(s16[v].collided_moments, s32[v].collided_moments) < 4e-3
+
+
+def test_cropped_helical_step_equals_the_covering_grid_without_the_rest():
+    """Each helical view on its beam slab equals the covering-grid step of the same densities with everything outside
+    the slab removed (scatter image and the slab's collided moments)."""
+    from citrus_amd.citrus import citrus_cropped_views_step
+    from citrus_amd.scatter_solver.cropping import crop_slices
+
+    dev = gpu_device()
+    library = seeded_library(4, 3, 120.0, 20.0, seed=31, num_eta=129)
+    config = SolverConfig(legendre_order=3, sweep_quadrature="level_symmetric", sn_order=4, transport_cell_cm=1.0,
+                          quadrature_point_spacing_cm=1.0, cone_num_polar=3, cone_num_azimuthal=6)
+    angles = [0.0, 2.5, 5.0]
+    geometry = ConeGeometry(30.0, 50.0, 8, 16, 0.5, 0.5).with_helical_z_shifts(angles, [-3.0, 0.0, 3.0])
+    solver = ScatterSolver(library, config, geometry, device=dev)
+    grid = Grid((16, 16, 32), 0.5)
+    rho = ellipsoid_phantom(grid, seed=8).to(dev)
+    # one cone set per view for both (a slab's own covering cone set is narrower: another discretization)
+    cones = [solver.cone_quadrature(grid, a) for a in angles]
+    results, states, slabs = citrus_cropped_views_step(solver, [CitrusState(cone_quadrature=c) for c in cones], rho,
+                                                       grid, angles)
+    assert len(set(slabs)) == 3 and all(k1 - k0 < 32 for k0, k1 in slabs)
+    for v, (a, (k0, k1)) in enumerate(zip(angles, slabs)):
+        kept = torch.zeros_like(rho)
+        kept[..., k0:k1] = rho[..., k0:k1]
+        fields = solver.material_fields(kept, grid)
+        (full,), (fstate,) = citrus_views_step(solver, [CitrusState(cone_quadrature=cones[v])], fields, [a])
+        assert rel_err(results[v][0], full[0]) < F32_REL
+        sub = fstate.collided_moments[..., k0 // 2:k1 // 2]
+        assert rel_err(states[v].collided_moments, sub) < F32_REL  # a photon leaving the convex slab never returns

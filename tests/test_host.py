@@ -94,3 +94,47 @@ def test_sweep_tuner_candidates():
     assert chunk_candidates(10) == [10, 5, 4, 8]
     assert chunk_candidates(3) == [3, 2, 1]
     assert chunk_candidates(40) == [32, 20, 14, 8]
+
+
+def test_helical_scan_and_view_angles():
+    from citrus_amd.utilities import helical_scan, view_angles
+
+    a = view_angles(8, 0.0, 4 * math.pi)
+    assert np.allclose(a, np.arange(8) * 4 * math.pi / 8)
+    geo = helical_scan(ConeGeometry(40.0, 60.0, 8, 8, 1.0, 1.0), a, pitch_cm=2.0, center_cm=0.5)
+    mid = 0.5 * (a[0] + a[-1])
+    for ang in a:
+        assert abs(geo.z_shift(ang) - (0.5 + 2.0 * (ang - mid) / (2 * math.pi))) < 1e-12
+        assert geo.source_position(ang)[2] == geo.z_shift(ang) == geo.panel_center(ang)[2]
+    with pytest.raises(ValueError):
+        view_angles(0)
+
+
+def test_beam_slab_contains_the_beam_and_crops_align():
+    from citrus_amd.scatter_solver.cropping import beam_slice_range, crop_slices
+
+    grid = Grid((12, 14, 40), 0.5)
+    geo = ConeGeometry(30.0, 50.0, 10, 16, 0.6, 0.6, use_curved_detector=True).with_helical_z_shifts(
+        [0.0, 10.0], [-4.0, 4.0])
+    g = torch.Generator().manual_seed(4)
+    for angle in (0.0, 2.3, 7.9):
+        k0, k1 = beam_slice_range(geo, angle, grid, multiple=2)
+        assert k0 % 2 == 0 and (k1 % 2 == 0 or k1 == grid.num_slices) and 0 <= k0 < k1 <= grid.num_slices
+        zl = grid.lower_corner()[2] + k0 * 0.5
+        zh = grid.lower_corner()[2] + k1 * 0.5
+        # rays from the source to panel points: inside the volume's xy box they stay in [zl, zh]
+        u, v = geo.pixel_uv(dtype=torch.float64)
+        uu = u[torch.randint(0, u.numel(), (400,), generator=g)]
+        vv = v[torch.randint(0, v.numel(), (400,), generator=g)]
+        pts = geo.panel_points(uu, vv, angle)
+        src = torch.tensor(geo.source_position(angle), dtype=torch.float64)
+        t = torch.linspace(0, 1, 2001, dtype=torch.float64)[:, None, None]
+        ray = src + t * (pts - src)  # [2001, 400, 3]
+        inside = (ray[..., 0].abs() <= 3.5) & (ray[..., 1].abs() <= 3.0)
+        zs = ray[..., 2][inside]
+        assert float(zs.min()) >= zl - 1e-9 and float(zs.max()) <= zh + 1e-9
+        rho = torch.arange(2 * 12 * 14 * 40, dtype=torch.float64).reshape(2, 12, 14, 40)
+        sub, rs = crop_slices(grid, rho, k0, k1)
+        zc = sub.axis_coordinates(dtype=torch.float64)[2]
+        assert torch.allclose(zc, grid.axis_coordinates(dtype=torch.float64)[2][k0:k1])
+        assert torch.equal(rs, rho[..., k0:k1])
