@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--cells", type=int, default=64, help="transport cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
+    ap.add_argument("--no-helical", action="store_true", help="skip the helical cropped-grid comparison")
     ap.add_argument("--cpu-directions", type=int, default=8, help="directions in the CPU-baseline sample")
     return ap.parse_args()
 
@@ -262,6 +263,68 @@ def build_view_solver(args, device, rho_fine):
     return solver, fields, angles
 
 
+def helical_views(args, device, rho_fine, views: int = 8):
+    """A turn of a helical scan (pitch 1.0: table feed per turn = beam width at the isocentre) on a 64-row curved panel
+    with a bowtie filter (0-3 cm aluminium-like, quadratic in the channel) and 5% off-focal radiation: the views' CITRUS
+    steps on their beam-cropped transport grids against the same steps on the covering grid."""
+    import torch
+
+    from citrus_amd.citrus import CitrusState, citrus_cropped_views_step, citrus_views_step
+    from citrus_amd.scatter_solver.config import SolverConfig
+    from citrus_amd.scatter_solver.cross_sections import seeded_library
+    from citrus_amd.scatter_solver.detector import energy_weighted_response
+    from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
+    from citrus_amd.scatter_solver.scatter_solver import ScatterSolver
+    from citrus_amd.scatter_solver.source import SourceModel
+    from citrus_amd.utilities import helical_scan, view_angles
+
+    n = args.cells
+    fine = Grid((4 * n, 4 * n, 4 * n), 0.08 * 64 / n)
+    library = seeded_library(16, 5, 140.0, 20.0, seed=99)
+    base = ConeGeometry(100.0, 150.0, 64, 256, 0.12, 0.12, use_curved_detector=True)
+    beam_at_iso = base.num_det_rows * base.delta_det_row / base.magnification
+    angles = [float(a) for a in view_angles(views, 0.0, 2 * math.pi)]
+    geometry = helical_scan(base, angles, pitch_cm=1.0 * beam_at_iso)
+    u = (torch.arange(256, dtype=torch.float64) - 127.5) * 0.12
+    al_mu = 0.15 + 5.0 * (30.0 / library.fine_energy.double()) ** 2.5  # cm^2/g, aluminium-like
+    source = SourceModel(off_focal_distance_cm=(0.5,), off_focal_fraction=(0.05,),
+                         filter_thickness_cm=3.0 * (u / u.abs().max()) ** 2, filter_density=2.7,
+                         filter_attenuation_table=al_mu)
+    config = SolverConfig(legendre_order=5, sweep_quadrature="level_symmetric", sn_order=12,
+                          transport_cell_cm=0.32 * 64 / n, quadrature_point_spacing_cm=0.5, cone_num_polar=16,
+                          cone_num_azimuthal=32, direction_chunk=64)
+    solver = ScatterSolver(library, config, geometry, detector=energy_weighted_response(), source=source,
+                           device=device)
+    rho = rho_fine.to(device, torch.float32)
+    cones = [solver.cone_quadrature(fine, a) for a in angles]
+    fields = solver.material_fields(rho, fine)
+
+    def cropped(states):
+        return citrus_cropped_views_step(solver, states, rho, fine, angles)
+
+    def covering(states):
+        return citrus_views_step(solver, states, fields, angles)
+
+    out = {}
+    for name, step in (("cropped", cropped), ("covering", covering)):
+        states = [CitrusState(cone_quadrature=c) for c in cones]
+        res = step(states)
+        states = res[1]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = step(states)
+        torch.cuda.synchronize()
+        out[name] = {"views_per_s": views / (time.perf_counter() - t0)}
+        if name == "cropped":
+            cells = [(k1 - k0) // 4 for k0, k1 in res[2]]
+            out[name]["transport_slices_per_view"] = cells
+    out["cropped_speedup"] = out["cropped"]["views_per_s"] / out["covering"]["views_per_s"]
+    out["note"] = (f"{views} views of one helical turn (pitch 1.0 = {beam_at_iso:.2f} cm per turn), 64 x 256 curved "
+                   "panel, bowtie 0-3 cm at 2.7 g/cm^3, 5% off-focal at 0.5 cm; second step of each view (with its "
+                   "accumulator), wall clock; the covering grid is 64 slices")
+    return out
+
+
 SURROGATE_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "citrus_amd", "weights",
                                  "surrogate_64cube_s12_p5_g16.npz")
 
@@ -410,6 +473,12 @@ def main():
                 view["learned"] = {"error": f"{type(e).__name__}: {e}"}
         del solver, fields
         torch.cuda.empty_cache()
+        if not args.no_helical:
+            try:
+                view["helical"] = helical_views(args, device, rho_fine)
+            except Exception as e:  # reported, not fatal
+                view["helical"] = {"error": f"{type(e).__name__}: {e}"}
+            torch.cuda.empty_cache()
 
     # ---- sweep only: scattering moments + one fused sweep per view, V views per launch sequence -------------
     args_sweep = argparse.Namespace(**vars(args))
@@ -492,7 +561,8 @@ def main():
                    "wall_s": wall,
                    **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),
                    **({"learned_vs_plain": view["learned"]} if view and "learned" in view else {}),
-                   **({"bf16_contraction": view["bf16"]} if view and "bf16" in view else {})},
+                   **({"bf16_contraction": view["bf16"]} if view and "bf16" in view else {}),
+                   **({"helical_cropped": view["helical"]} if view and "helical" in view else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "alg_bytes_per_view_sweep": alg_bytes,
