@@ -439,8 +439,11 @@ __device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, unsigned of
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
 }
 
-template <int NSH, int DC4, bool HAS_SRC, bool BF16>
+// SRC: 0 no explicit source; 1 group-major LD moments (layout 1, read per plane); 2 group-major cell means (layout 2,
+// prefetched one plane ahead with T, the slopes zero)
+template <int NSH, int DC4, int SRC, bool BF16>
 __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
+  constexpr bool HAS_SRC = SRC != 0;
   constexpr int B = 4;
   constexpr PlaneTable<B> ptab{};
   constexpr int NPL = PlaneTable<B>::kPlanes;
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4, 0, a.N * (int)CB, kBufferWord3);
   // explicit source, group-major [G][N][M][4] (layout 1: the 16 directions of one cell in a D tile are 256 contiguous
   // bytes) or cell means [G][N][M] (layout 2, zero slopes: 64 contiguous bytes)
-  const int src_words = a.src_layout == 2 ? 1 : 4;
+  constexpr int src_words = SRC == 2 ? 1 : 4;
   const __amdgpu_buffer_rsrc_t src_r = __builtin_amdgcn_make_buffer_rsrc(
       HAS_SRC ? a.src + (int64_t)v * a.sview + (int64_t)g * a.N * a.M * src_words : a.phi, 0,
       HAS_SRC ? a.N * a.M * 4 * src_words : 0, kBufferWord3);
@@ -592,6 +595,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
     float t[3][KS];
     s4 tb[3][KB];
     float mu[3];
+    float sm[3][SRC == 2 ? NT : 1];  // cell-mean source of (cell 4 mt + l/16, direction 16 nt + l%16)
   };
   auto load_plane = [&](const int pl) __attribute__((always_inline)) -> Staged {
     Staged st;
@@ -617,6 +621,14 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       const int sd = 4 * mt + q;
       const int cd = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
       st.mu[mt] = buf_load(mu_r, cd >= 0 ? 4u * cd : kDrop);
+      if (SRC == 2) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int d = nt * 16 + r16;
+          const unsigned e = (unsigned)cd * a.M + (unsigned)(mbase + d);
+          st.sm[mt][nt] = buf_load(src_r, (cd >= 0 && d < dc) ? e * 4u : kDrop);
+        }
+      }
     }
     return st;
   };
@@ -641,7 +653,12 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
       }
     }
     f4 srcv[3][NT];  // explicit source of (cell 4 mt + l/16, direction 16 nt + l%16)
-    if (HAS_SRC) {
+    if (SRC == 2) {
+#pragma unroll
+      for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) srcv[mt][nt] = f4{cur.sm[mt][nt], 0.f, 0.f, 0.f};
+    } else if (SRC == 1) {
 #pragma unroll
       for (int mt = 0; mt < 3; ++mt) {
         if (mt * 4 >= P) break;
@@ -650,10 +667,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
           const int d = nt * 16 + r16;
           const bool use = cells[mt] >= 0 && d < dc;
           const unsigned e = (unsigned)cells[mt] * a.M + (unsigned)(mbase + d);
-          if (src_words == 1)
-            srcv[mt][nt] = f4{buf_load(src_r, use ? e * 4u : kDrop), 0.f, 0.f, 0.f};
-          else
-            srcv[mt][nt] = buf_load4(src_r, use ? e * 16u : kDrop);
+          srcv[mt][nt] = buf_load4(src_r, use ? e * 16u : kDrop);
         }
       }
     }
@@ -754,7 +768,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
 
 size_t mfma_lds_bytes(int dc) { return sizeof(float) * (size_t)(144 * dc + 4 * kMfmaPsiStride + 128 + 2 * dc + 64); }
 
-template <int NSH, bool HAS_SRC, bool BF16>
+template <int NSH, int HAS_SRC, bool BF16>
 hipError_t launch_mfma_src(const LaunchArgs& la, int nwg, hipStream_t stream) {
   const size_t lds = mfma_lds_bytes(la.dc);
   const dim3 grid(nwg), block(64);
@@ -772,17 +786,22 @@ hipError_t launch_mfma_src(const LaunchArgs& la, int nwg, hipStream_t stream) {
   return hipGetLastError();
 }
 
+template <int NSH, bool BF16>
+hipError_t launch_mfma_layout(const LaunchArgs& la, int nwg, hipStream_t stream) {
+  if (!la.src) return launch_mfma_src<NSH, 0, BF16>(la, nwg, stream);
+  return la.src_layout == 2 ? launch_mfma_src<NSH, 2, BF16>(la, nwg, stream)
+                            : launch_mfma_src<NSH, 1, BF16>(la, nwg, stream);
+}
+
 template <int NSH>
 hipError_t launch_mfma_nsh(const LaunchArgs& la, int nwg, hipStream_t stream) {
-  return la.src ? launch_mfma_src<NSH, true, false>(la, nwg, stream)
-                : launch_mfma_src<NSH, false, false>(la, nwg, stream);
+  return launch_mfma_layout<NSH, false>(la, nwg, stream);
 }
 
 // the bfloat16 expansion is built for P3 and P5 (16 and 36 moments)
 template <int NSH>
 hipError_t launch_mfma_bf16(const LaunchArgs& la, int nwg, hipStream_t stream) {
-  return la.src ? launch_mfma_src<NSH, true, true>(la, nwg, stream)
-                : launch_mfma_src<NSH, false, true>(la, nwg, stream);
+  return launch_mfma_layout<NSH, true>(la, nwg, stream);
 }
 
 hipError_t launch_mfma(int nsh, const LaunchArgs& la, int nwg, hipStream_t stream) {
