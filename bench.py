@@ -568,6 +568,8 @@ def main():
                      "kernel": kernel, "alg_bytes_per_view_sweep": alg_bytes,
                      "alg_bytes_per_launch": alg_bytes * V / launches,
                      "avg_launch_ms": sweep_ms / launches, "launches_per_view_sweep": launches,
+                     "traffic_rate_gbs": (traffic / (sweep_ms / launches * 1e-3) / 1e9) if traffic else None,
+                     "traffic_frac": (traffic / (sweep_ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                      "traffic_source": "profiles/ld_sweep_mfma_64cube_s12_p5_g16.txt" if traffic else None},
     }
     if not args.no_cpu_baseline:
