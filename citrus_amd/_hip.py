@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -37,6 +37,24 @@ class SweepArgs(ctypes.Structure):
     ]
 
 
+class SweepView(ctypes.Structure):
+    """Mirror of ``CitrusSweepView`` (include/citrus_hip.h)."""
+
+    _fields_ = [
+        ("mu_t", c_void), ("T", c_void), ("T_bf16", c_void), ("T_inv_scale", c_void), ("source", c_void),
+        ("phi", c_void), ("cdir", c_void),
+    ]
+
+
+class PersistentArgs(ctypes.Structure):
+    """Mirror of ``CitrusPersistentArgs`` (include/citrus_hip.h)."""
+
+    _fields_ = [
+        ("G", c_int), ("nsh", c_int), ("D8", c_int), ("d0", c_int), ("dc", c_int), ("accumulate", c_int),
+        ("source_layout", c_int), ("Y", c_void), ("Yw", c_void), ("views", ctypes.POINTER(SweepView)),
+    ]
+
+
 class FcArgs(ctypes.Structure):
     """Mirror of ``CitrusFcArgs`` (include/citrus_hip.h)."""
 
@@ -57,6 +75,13 @@ _SIGNATURES = {
     "citrus_sweep_plan_destroy": (None, [c_void]),
     "citrus_sweep_face_floats": (c_i64, [c_void, c_int, c_int, c_int]),
     "citrus_sweep_run": (c_int, [c_void, ctypes.POINTER(SweepArgs), c_void]),
+    "citrus_persistent_plan_create": (c_int, [c_int, ctypes.POINTER(c_int), c_int, ctypes.POINTER(c_void)]),
+    "citrus_persistent_plan_destroy": (None, [c_void]),
+    "citrus_persistent_workspace_bytes": (c_i64, [c_void, c_int]),
+    "citrus_persistent_num_tasks": (c_i64, [c_void]),
+    "citrus_persistent_schedule_table": (c_i64, [c_int, ctypes.POINTER(c_int), c_int, c_void, c_void, c_void]),
+    "citrus_persistent_sweep_run": (c_int, [c_void, ctypes.POINTER(PersistentArgs), c_void, c_i64, c_void]),
+    "citrus_persistent_check": (c_int, [c_void, c_void, ctypes.POINTER(c_int), ctypes.POINTER(c_i64), c_void]),
     "citrus_scatter_moments_dense": (c_int, [c_void, c_int, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),
     "citrus_scatter_moments_factored": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_int, c_int,
                                                 c_i64, c_int, c_void]),

@@ -100,6 +100,130 @@ class SweepSchedule:
                        "citrus_sweep_run")
 
 
+PERSISTENT_MAX_VIEWS = 16
+PERSISTENT_MOMENTS = (16, 36)  # moment counts (P3, P5) the persistent kernel is built for
+PERSISTENT_STATUS = {0: "ok", 1: "a dependency wait passed its spin limit", 2: "a wave ran on an XCD id outside 0..7"}
+
+
+class PersistentSchedule:
+    """Owner of a native ``CitrusPersistentPlan`` (per-XCD task queues of a list of view grids) and its workspace.
+
+    One ``run`` sweeps one direction chunk of every view in a single launch of resident waves; the views' grids may
+    differ in shape and cell size. The moments equal those of ``SweepSchedule.run`` on each view's grid bit for bit.
+    """
+
+    def __init__(self, shapes, G: int, device: torch.device):
+        shapes = [tuple(int(v) for v in s) for s in shapes]
+        if not 1 <= len(shapes) <= PERSISTENT_MAX_VIEWS:
+            raise ValueError(f"a persistent sweep takes 1..{PERSISTENT_MAX_VIEWS} views, got {len(shapes)}")
+        lib = _hip.library()
+        flat = (ctypes.c_int * (3 * len(shapes)))(*[v for s in shapes for v in s])
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            _hip.check(lib.citrus_persistent_plan_create(len(shapes), flat, int(G), ctypes.byref(handle)),
+                       "citrus_persistent_plan_create")
+        self.handle = handle
+        self.shapes = shapes
+        self.G = int(G)
+        self.device = device
+        self._workspace: Optional[Tensor] = None
+        self._finalizer = weakref.finalize(self, lib.citrus_persistent_plan_destroy, handle)
+
+    @property
+    def num_tasks(self) -> int:
+        """Block tasks of one chunk sweep."""
+        return int(_hip.library().citrus_persistent_num_tasks(self.handle))
+
+    def workspace(self, dc: int) -> Tensor:
+        """256-byte aligned workspace (flags, tickets, faces) for chunks of ``dc`` directions, grown on demand."""
+        n = int(_hip.library().citrus_persistent_workspace_bytes(self.handle, int(dc)))
+        if n < 0:
+            raise ValueError("invalid persistent workspace request")
+        if self._workspace is None or self._workspace.numel() < n:
+            raw = torch.empty(n + 256, device=self.device, dtype=torch.uint8)
+            off = (-raw.data_ptr()) % 256
+            self._raw = raw
+            self._workspace = raw[off:off + n]
+        return self._workspace
+
+    def run(self, *, views, Y: Tensor, Yw: Tensor, nsh: int, D8: int, d0: int, dc: int, accumulate: bool = False,
+            source_layout: int = -1) -> None:
+        """Enqueue one chunk sweep of every view on torch's current stream. ``views`` holds one dict per view of the
+        plan with the device tensors ``mu`` [G, N], ``phi`` [G, N, nsh, 4], ``cdir`` [8 D8, 3], and either ``T``
+        [G, N, nsh, 4] or ``T16`` [G, N, 4, KP] with ``T_inv_scale`` [G], and optionally ``source`` (group-major, the
+        same ``source_layout`` 1 or 2 for every view; -1 without)."""
+        if len(views) != len(self.shapes):
+            raise ValueError(f"the plan has {len(self.shapes)} views, got {len(views)}")
+        if nsh not in PERSISTENT_MOMENTS:
+            raise ValueError(f"the persistent sweep is built for {PERSISTENT_MOMENTS} moments, got {nsh}")
+        if source_layout not in (-1, 1, 2):
+            raise ValueError("source_layout must be -1, 1 or 2")
+        G, M, kp = self.G, 8 * D8, (nsh + 15) // 16 * 16
+        arr = (_hip.SweepView * len(views))()
+        for v, (spec, shape) in enumerate(zip(views, self.shapes)):
+            N = shape[0] * shape[1] * shape[2]
+            for name in ("mu", "phi", "cdir"):
+                _hip.require_gpu_tensor(spec[name], f"views[{v}].{name}")
+            if spec["mu"].numel() != G * N or spec["phi"].numel() != G * N * nsh * 4:
+                raise ValueError(f"views[{v}]: mu or phi size does not match the grid {shape}")
+            if tuple(spec["cdir"].shape) != (M, 3):
+                raise ValueError(f"views[{v}]: cdir must be [{M}, 3]")
+            T, T16, tinv, src = spec.get("T"), spec.get("T16"), spec.get("T_inv_scale"), spec.get("source")
+            if T16 is not None:
+                if T is not None or tinv is None or T16.dtype != torch.bfloat16 or not T16.is_contiguous() \
+                        or T16.numel() != G * N * 4 * kp or tinv.numel() != G:
+                    raise ValueError(f"views[{v}]: T16 must be contiguous bfloat16 [G, N, 4, {kp}] with G scales")
+                _hip.require_gpu_tensor(tinv, f"views[{v}].T_inv_scale")
+            else:
+                if T is None:
+                    raise ValueError(f"views[{v}]: need T or T16")
+                _hip.require_gpu_tensor(T, f"views[{v}].T")
+                if T.numel() != G * N * nsh * 4:
+                    raise ValueError(f"views[{v}]: T size does not match [G, N, nsh, 4]")
+            if (src is None) != (source_layout == -1):
+                raise ValueError(f"views[{v}]: source does not match source_layout {source_layout}")
+            if src is not None:
+                _hip.require_gpu_tensor(src, f"views[{v}].source")
+                if src.numel() != G * N * M * (4 if source_layout == 1 else 1):
+                    raise ValueError(f"views[{v}]: source size does not match layout {source_layout}")
+            arr[v] = _hip.SweepView(mu_t=spec["mu"].data_ptr(), T=_hip.ptr(T), T_bf16=_hip.ptr(T16),
+                                    T_inv_scale=_hip.ptr(tinv), source=_hip.ptr(src), phi=spec["phi"].data_ptr(),
+                                    cdir=spec["cdir"].data_ptr())
+        if Y.shape != (M, nsh) or Yw.shape != (M, nsh):
+            raise ValueError("direction tables do not match the quadrature")
+        ws = self.workspace(dc)
+        args = _hip.PersistentArgs(G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, accumulate=int(accumulate),
+                                   source_layout=int(source_layout), Y=Y.data_ptr(), Yw=Yw.data_ptr(), views=arr)
+        lib = _hip.library()
+        with torch.cuda.device(self.device):
+            _hip.check(lib.citrus_persistent_sweep_run(self.handle, ctypes.byref(args), ws.data_ptr(), ws.numel(),
+                                                       _hip.stream_of(self.device)), "citrus_persistent_sweep_run")
+
+    def check(self) -> tuple[int, int]:
+        """(status, unfinished tasks) of the last sweep; synchronizes torch's current stream. (0, 0) when it ran
+        through."""
+        if self._workspace is None:
+            raise RuntimeError("no sweep has run on this schedule")
+        status, unfinished = ctypes.c_int(), ctypes.c_int64()
+        with torch.cuda.device(self.device):
+            _hip.check(_hip.library().citrus_persistent_check(self.handle, self._workspace.data_ptr(),
+                                                              ctypes.byref(status), ctypes.byref(unfinished),
+                                                              _hip.stream_of(self.device)), "citrus_persistent_check")
+        return int(status.value), int(unfinished.value)
+
+
+_PERSISTENT: dict = {}
+
+
+def persistent_schedule(shapes, G: int, device: torch.device) -> PersistentSchedule:
+    """The cached ``PersistentSchedule`` of a list of view grid shapes."""
+    key = (tuple(tuple(int(v) for v in s) for s in shapes), int(G), str(device))
+    sched = _PERSISTENT.get(key)
+    if sched is None:
+        sched = _PERSISTENT[key] = PersistentSchedule(shapes, G, device)
+    return sched
+
+
 def hip_available() -> tuple[bool, str]:
     """Whether the HIP library loads (honouring the ``CITRUS_DISABLE_HIP`` kill switch)."""
     if os.environ.get(_hip.DISABLE_ENV_VAR, "") not in ("", "0"):

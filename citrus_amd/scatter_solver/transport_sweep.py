@@ -301,7 +301,7 @@ def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: Sourc
 
 def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out: Optional[Tensor] = None,
                 direction_chunk: Optional[int] = None, source: Optional[Tensor] = None,
-                contraction: str = "fp32") -> Tensor:
+                contraction: str = "fp32", schedule: str = "launches") -> Tensor:
     """Sweep V views that share the material fields, each from its own scattering-source moments, in one launch
     sequence: the batched form of one ``sweep`` per view of an outer iteration (citrus.py:1920-1935).
 
@@ -317,6 +317,8 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             directions (the summed source of the CITRUS step, citrus.py:232-236).
         contraction: "fp32", or "bf16" for the source expansion as a bfloat16 matrix product with float32
             accumulation, the moments scaled per (view, group) by a power of two (P3 and P5).
+        schedule: "launches" (one launch per block diagonal) or "persistent" (one launch of resident waves per
+            direction chunk and at most 16 views, P3 and P5); both give the same moments bit for bit.
 
     Returns:
         [V, G, R, C, S, nsh, 4] fluence moments.
@@ -344,8 +346,15 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             raise ValueError("source must be a contiguous float32 tensor on the moments' device")
     if contraction not in ("fp32", "bf16"):
         raise ValueError(f"contraction must be 'fp32' or 'bf16', got {contraction!r}")
+    if schedule not in ("launches", "persistent"):
+        raise ValueError(f"schedule must be 'launches' or 'persistent', got {schedule!r}")
     if out is None:
         out = torch.empty_like(moments)
+    elif tuple(out.shape) != tuple(moments.shape) or not out.is_contiguous() or out.dtype != torch.float32:
+        raise ValueError("out must be a contiguous float32 tensor shaped like moments")
+    if schedule == "persistent":
+        return _sweep_grids_persistent([(sweep_context, mu_t, moments, source, out)], direction_chunk,
+                                       contraction)[0]
     D8 = sweep_context.directions_per_octant
     if direction_chunk == "auto":  # measured once per configuration (autotune.SweepTuner)
         from .autotune import default_tuner
@@ -365,6 +374,92 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         hip.schedule.run(mu=mu, phi=out, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V,
                          T=T, accumulate=d0 > 0, source=source, source_layout=layout, T16=T16, T_inv_scale=tinv)
     return out
+
+
+def sweep_grids(items, direction_chunk: Optional[int] = None, contraction: str = "fp32") -> list:
+    """Sweep the views of several grids (views cropped to their beams, coarse and fine multigrid stages) together in
+    persistent launches of at most 16 views each: every view's block tasks in one launch per direction chunk.
+
+    Args:
+        items: sequence of ``(sweep_context, mu_t, moments, source)`` with the ``sweep_views`` shapes for that
+            context's grid (``source`` may be None; all items then have none, else all the same layout). The
+            contexts share the quadrature and the SH order; their grids may differ in shape and cell size.
+        direction_chunk: directions per octant per launch (default all).
+        contraction: "fp32" or "bf16" as in ``sweep_views``.
+
+    Returns:
+        One [V_i, G, R_i, C_i, S_i, nsh, 4] moment tensor per item, equal bit for bit to ``sweep_views`` of the item.
+    """
+    if not items:
+        return []
+    prepared = []
+    for ctx, mu_t, moments, source in items:
+        if mu_t.dim() != 4 or tuple(mu_t.shape[1:]) != tuple(ctx.shape):
+            raise ValueError(f"mu_t must be [G, {ctx.shape}], got {tuple(mu_t.shape)}")
+        _resolve_hip(ctx, None, mu_t)
+        prepared.append((ctx, mu_t, moments, source, torch.empty_like(moments)))
+    return _sweep_grids_persistent(prepared, direction_chunk, contraction)
+
+
+def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
+    """Persistent sweeps of (context, mu_t, moments, source, out) items, 16 views per launch."""
+    from .kernels.sweep_kernels import PERSISTENT_MAX_VIEWS, PERSISTENT_MOMENTS, persistent_schedule
+
+    if contraction not in ("fp32", "bf16"):
+        raise ValueError(f"contraction must be 'fp32' or 'bf16', got {contraction!r}")
+    ctx0 = items[0][0]
+    D8, nsh, M = ctx0.directions_per_octant, ctx0.num_moments, ctx0.num_directions
+    G = int(items[0][1].shape[0])
+    if nsh not in PERSISTENT_MOMENTS:
+        raise ValueError(f"the persistent sweep is built for {PERSISTENT_MOMENTS} moments (P3, P5), got {nsh}")
+    layouts = set()
+    views = []  # (shape, spec) per view
+    for ctx, mu_t, moments, source, out in items:
+        if ctx.directions_per_octant != D8 or ctx.num_moments != nsh or ctx.hip.Y.shape != ctx0.hip.Y.shape \
+                or not torch.equal(ctx.hip.Y, ctx0.hip.Y):
+            raise ValueError("the grids of one persistent sweep must share the quadrature and the SH order")
+        if int(mu_t.shape[0]) != G:
+            raise ValueError("the grids of one persistent sweep must have the same number of groups")
+        R, C, S = ctx.shape
+        if moments.dim() != 7 or tuple(moments.shape[1:]) != (G, R, C, S, nsh, 4) or not moments.is_contiguous() \
+                or moments.dtype != torch.float32:
+            raise ValueError(f"moments must be contiguous float32 [V, {G}, {R}, {C}, {S}, {nsh}, 4]")
+        V = int(moments.shape[0])
+        layout = -1
+        if source is not None:
+            base = (V, G, R, C, S, M)
+            layout = 1 if tuple(source.shape) == base + (4,) else 2 if tuple(source.shape) == base else 0
+            if layout == 0:
+                raise ValueError(f"source must be {base + (4,)} or {base}, got {tuple(source.shape)}")
+            if not source.is_contiguous() or source.dtype != torch.float32:
+                raise ValueError("source must be a contiguous float32 tensor")
+        layouts.add(layout)
+        mu = mu_t.reshape(G, -1).contiguous()
+        T16 = tinv = None
+        if contraction == "bf16":
+            from .source_iteration import bf16_operand
+
+            T16, tinv = bf16_operand(moments)
+        for v in range(V):
+            spec = dict(mu=mu, phi=out[v], cdir=ctx.hip.cdir, source=None if source is None else source[v])
+            if T16 is None:
+                spec["T"] = moments[v]
+            else:
+                spec["T16"], spec["T_inv_scale"] = T16[v], tinv[v]
+            views.append((tuple(ctx.shape), spec))
+    if len(layouts) != 1:
+        raise ValueError("every item of one persistent sweep needs a source of the same layout, or none")
+    layout = layouts.pop()
+    Dc = _resolve_chunk(ctx0, None if direction_chunk == "auto" else direction_chunk)
+    device = items[0][2].device
+    for b0 in range(0, len(views), PERSISTENT_MAX_VIEWS):
+        batch = views[b0:b0 + PERSISTENT_MAX_VIEWS]
+        sched = persistent_schedule([s for s, _ in batch], G, device)
+        for d0 in range(0, D8, Dc):
+            dc = min(d0 + Dc, D8) - d0
+            sched.run(views=[spec for _, spec in batch], Y=ctx0.hip.Y, Yw=ctx0.hip.Yw, nsh=nsh, D8=D8, d0=d0,
+                      dc=dc, accumulate=d0 > 0, source_layout=layout)
+    return [it[4] for it in items]
 
 
 class SweepGraph:

@@ -89,6 +89,70 @@ typedef struct CitrusSweepArgs {
 int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepArgs* args, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Persistent sweep: the same sweep (same tasks, same order of every floating-point reduction, bit-identical moments)
+ * in ONE launch of resident waves, for up to CITRUS_PERSISTENT_MAX_VIEWS views whose grids may differ in shape and
+ * cell size (views cropped to their beams, coarse and fine stages of a multigrid schedule). Each (view, group) is
+ * pinned to one XCD; waves take its block tasks from that XCD's queue and wait only on the tasks they depend on.
+ *
+ * Replaces the same reference functions as citrus_sweep_run (the per-view sweep of transport_sweep.py:457-537, called
+ * once per view by citrus.py:1920-1935); the reference has no multi-grid batching.
+ * --------------------------------------------------------------------------------------------------------------- */
+
+#define CITRUS_PERSISTENT_MAX_VIEWS 16
+#define CITRUS_PERSISTENT_TIMEOUT 1 /* status: a dependency wait passed its spin limit; the sweep was abandoned */
+#define CITRUS_PERSISTENT_BAD_XCC 2 /* status: a wave ran on an XCD id outside 0..7 */
+
+/* Opaque schedule of a list of view grids: per-XCD task queues on the device. */
+typedef struct CitrusPersistentPlan CitrusPersistentPlan;
+
+/* One view of a persistent sweep: its grid's buffers (N = R*C*S of the view's grid). */
+typedef struct CitrusSweepView {
+  const float* mu_t;         /* [G][N] */
+  const float* T;            /* [G][N][nsh][4] scattering-source moments, or NULL with T_bf16 */
+  const void* T_bf16;        /* bfloat16 operand [G][N][4][KP] of citrus_bf16_operand (P3, P5), or NULL */
+  const float* T_inv_scale;  /* [G] with T_bf16 */
+  float* source;             /* group-major source of every direction ([G][N][8*D8][4] or [G][N][8*D8]) or NULL */
+  float* phi;                /* [G][N][nsh][4] fluence moments (output) */
+  const float* cdir;         /* [8*D8][3] |Omega_a| / Delta_a on this view's grid */
+} CitrusSweepView;
+
+typedef struct CitrusPersistentArgs {
+  int G, nsh, D8, d0, dc;    /* as in CitrusSweepArgs */
+  int accumulate;            /* 0: phi is overwritten; 1: phi += this chunk's moments */
+  int source_layout;         /* -1: no explicit source; 1 or 2 as in CitrusSweepArgs (the same for every view) */
+  const float* Y;            /* [8*D8][nsh], shared by the views */
+  const float* Yw;           /* [8*D8][nsh] */
+  const CitrusSweepView* views; /* HOST array of the plan's nviews views, read before the launch */
+} CitrusPersistentArgs;
+
+/* Schedule of nviews view grids, shapes[v] = (R, C, S), with G groups. Uploads the task queues (hipMalloc). */
+int citrus_persistent_plan_create(int nviews, const int* shapes, int G, CitrusPersistentPlan** plan);
+void citrus_persistent_plan_destroy(CitrusPersistentPlan* plan);
+
+/* Bytes of the device workspace (done flags, tickets, status, inter-block faces) of a sweep with dc directions per
+ * octant; the workspace must be 256-byte aligned. */
+int64_t citrus_persistent_workspace_bytes(const CitrusPersistentPlan* plan, int dc);
+
+/* Block tasks of one chunk sweep (8 octants x groups x blocks of every view). */
+int64_t citrus_persistent_num_tasks(const CitrusPersistentPlan* plan);
+
+/* Host-only view of the schedule a plan of these shapes would run (no device needed): returns the task count n and,
+ * where the pointers are non-NULL, the tasks in queue order (tasks[n][4] = {view | group << 8 | octant << 16, block
+ * row, block col, block slice}), the queue offsets qoff[9] (queue q = XCD q holds tasks [qoff[q], qoff[q+1])) and,
+ * per task, the indices of the tasks it waits for (deps[n][7], -1 padded). Returns -1 on bad arguments. */
+int64_t citrus_persistent_schedule_table(int nviews, const int* shapes, int G, int* tasks, int* qoff, int* deps);
+
+/* Sweep one direction chunk of every view in one persistent launch (preceded by a small kernel that clears the
+ * workspace's flags; no memset, so the call can be captured in a graph). */
+int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, const CitrusPersistentArgs* args, void* workspace,
+                                int64_t workspace_bytes, void* stream);
+
+/* After a sweep (synchronizes the stream): *status is 0 or a CITRUS_PERSISTENT_* code, *unfinished the number of
+ * block tasks that did not complete (0 when the sweep ran through). */
+int citrus_persistent_check(const CitrusPersistentPlan* plan, const void* workspace, int* status,
+                            int64_t* unfinished, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Scattering-source moments T[go][n][k] = sum_{gi} mu_s_l[gi][go][l(n)] phi[gi][n][k] per cell.
  *
  * Replaces `_scattering_moments_kernel` / `scattering_moments` (citrus/scatter_solver/kernels/

@@ -1,0 +1,128 @@
+"""The persistent sweep on the GPU: one launch per direction chunk for all views, including views on grids of
+different shapes and cell sizes. Its moments equal the launch-sequence sweep's bit for bit (same tasks, same order of
+every addition), every task completes (workspace flags, status 0), and a mixed-grid batch matches the FP64 oracle."""
+
+from __future__ import annotations
+
+import pytest
+import torch
+
+from citrus_amd.scatter_solver.angular_quadrature import level_symmetric_quadrature
+from citrus_amd.scatter_solver.geometry import Grid
+from citrus_amd.scatter_solver.kernels.sweep_kernels import persistent_schedule
+from citrus_amd.scatter_solver.transport_sweep import make_sweep_context, sweep_grids, sweep_views
+from oracle import quadrature as oq, sweep as osw
+from tests.conftest import gpu_device
+from tests.helpers import F32_REL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(ctx, G, V, layout, seed):
+    gen = torch.Generator().manual_seed(seed)
+    R, C, S = ctx.shape
+    mu = torch.rand(G, R, C, S, generator=gen) + 0.3
+    T = torch.randn(V, G, R, C, S, ctx.num_moments, 4, generator=gen) * 0.2
+    T[..., 0, 0] = T[..., 0, 0].abs() + 0.3
+    src = None
+    if layout == 1:
+        src = torch.rand(V, G, R, C, S, ctx.num_directions, 4, generator=gen)
+    elif layout == 2:
+        src = torch.rand(V, G, R, C, S, ctx.num_directions, generator=gen)
+    dev = ctx.hip.Y.device
+    return mu.to(dev), T.to(dev).contiguous(), None if src is None else src.to(dev).contiguous()
+
+
+def _check_ran_through(shapes, G, dev):
+    status, unfinished = persistent_schedule(shapes, G, dev).check()
+    assert (status, unfinished) == (0, 0)
+
+
+@pytest.mark.parametrize("shape,order,L,G,V,layout,contraction,chunk", [
+    ((9, 6, 7), 8, 3, 3, 2, None, "fp32", None),    # ragged blocks, P3, no explicit source
+    ((6, 5, 7), 6, 5, 2, 3, 2, "fp32", 3),         # P5, cell-mean source, two direction chunks
+    ((8, 8, 8), 8, 3, 2, 2, 1, "fp32", None),       # LD-moment source
+    ((7, 9, 5), 8, 5, 2, 2, 2, "bf16", None),       # bfloat16 contraction
+    ((12, 4, 9), 4, 3, 1, 1, 2, "fp32", None),      # one view, one group: a single XCD queue has work
+])
+def test_persistent_equals_launch_sequence(shape, order, L, G, V, layout, contraction, chunk):
+    dev = gpu_device()
+    print(f"persistent vs launches: shape {shape} S{order} L{L} G{G} V{V} src {layout} {contraction} chunk {chunk}",
+          flush=True)
+    ctx = make_sweep_context(Grid(shape, 0.45), level_symmetric_quadrature(order), L, device=dev)
+    mu, T, src = _inputs(ctx, G, V, layout, seed=41)
+    want = sweep_views(ctx, mu, T, source=src, contraction=contraction, direction_chunk=chunk)
+    got = sweep_views(ctx, mu, T, source=src, contraction=contraction, direction_chunk=chunk, schedule="persistent")
+    torch.cuda.synchronize()
+    _check_ran_through([shape] * V, G, dev)
+    assert torch.equal(got, want)
+    again = sweep_views(ctx, mu, T, source=src, contraction=contraction, direction_chunk=chunk,
+                        schedule="persistent")
+    assert torch.equal(again, want)  # the workspace is cleared for each sweep
+
+
+def test_mixed_grids_in_one_launch_match_per_grid_sweeps_and_oracle():
+    """Three grids of different shapes and cell sizes (one with two views) swept together equal their own sweeps
+    bit for bit; one view against the FP64 oracle."""
+    dev = gpu_device()
+    q = level_symmetric_quadrature(8)
+    L, G = 3, 2
+    grids = [Grid((8, 8, 8), 0.5), Grid((6, 10, 5), 0.3), Grid((12, 7, 9), 0.45, 1.2, 0.8)]
+    items, refs = [], []
+    for i, (grid, V) in enumerate(zip(grids, (2, 1, 1))):
+        ctx = make_sweep_context(grid, q, L, device=dev)
+        mu, T, src = _inputs(ctx, G, V, 2, seed=50 + i)
+        items.append((ctx, mu, T, src))
+        refs.append(sweep_views(ctx, mu, T, source=src))
+    outs = sweep_grids(items)
+    torch.cuda.synchronize()
+    _check_ran_through([g.shape for g, V in zip(grids, (2, 1, 1)) for _ in range(V)], G, dev)
+    for got, want in zip(outs, refs):
+        assert torch.equal(got, want)
+    ctx, mu, T, src = items[2]
+    om = q.omega[ctx.permutation.cpu()].numpy()
+    w = q.weight[ctx.permutation.cpu()].numpy()
+    Y = oq.real_spherical_harmonics(L, om)
+    Tv = T[0].double().permute(0, 4, 5, 1, 2, 3).cpu().numpy()
+    s = osw.sources_from_moments(Tv, Y) + src[0].double().permute(4, 0, 1, 2, 3).cpu().numpy()[:, :, None] * \
+        torch.tensor([1.0, 0.0, 0.0, 0.0], dtype=torch.float64).reshape(1, 1, 4, 1, 1, 1).numpy()
+    psi = osw.sweep(ctx.shape, ctx.deltas, om, mu.double().cpu().numpy(), s)
+    ref = osw.moments(psi, w, Y)
+    assert rel_err(outs[2][0].permute(0, 4, 5, 1, 2, 3), ref) < F32_REL
+
+
+def test_more_views_than_one_launch_and_errors():
+    """17 views run as two persistent launches; unsupported moment counts and mismatched items are refused."""
+    dev = gpu_device()
+    ctx = make_sweep_context(Grid((5, 6, 4), 0.5), level_symmetric_quadrature(4), 3, device=dev)
+    mu, T, src = _inputs(ctx, 1, 17, 2, seed=61)
+    want = sweep_views(ctx, mu, T, source=src)
+    assert torch.equal(sweep_views(ctx, mu, T, source=src, schedule="persistent"), want)
+    ctx2 = make_sweep_context(Grid((5, 6, 4), 0.5), level_symmetric_quadrature(4), 2, device=dev)
+    mu2, T2, _ = _inputs(ctx2, 1, 1, None, seed=62)
+    with pytest.raises(ValueError, match="built for"):
+        sweep_views(ctx2, mu2, T2, schedule="persistent")
+    ctx3 = make_sweep_context(Grid((4, 4, 4), 0.5), level_symmetric_quadrature(6), 3, device=dev)
+    mu3, T3, _ = _inputs(ctx3, 1, 1, None, seed=63)
+    with pytest.raises(ValueError, match="quadrature"):
+        sweep_grids([(ctx, mu, T[:1].contiguous(), None), (ctx3, mu3, T3, None)])
+    with pytest.raises(ValueError, match="schedule"):
+        sweep_views(ctx, mu, T, schedule="graph")
+
+
+def test_view_step_shape_persistent_equals_launch_sequence():
+    """64^3 cells, 16 groups, S12 (168 directions), P5, 4 views with the cell-mean source: the persistent launch
+    equals the launch sequence bit for bit at the benchmark's shape."""
+    dev = gpu_device()
+    print("persistent vs launches at 64^3 x 16 groups x S12 x P5, 4 views", flush=True)
+    ctx = make_sweep_context(Grid((64, 64, 64), 0.32), level_symmetric_quadrature(12), 5, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(71)
+    G, V = 16, 4
+    mu = torch.rand(G, 64, 64, 64, generator=gen, device=dev) * 0.45 + 0.15
+    T = torch.randn(V, G, 64, 64, 64, 36, 4, generator=gen, device=dev) * 0.05
+    src = torch.rand(V, G, 64, 64, 64, 168, generator=gen, device=dev)
+    want = sweep_views(ctx, mu, T, source=src)
+    got = sweep_views(ctx, mu, T, source=src, schedule="persistent")
+    torch.cuda.synchronize()
+    _check_ran_through([(64, 64, 64)] * V, G, dev)
+    assert torch.equal(got, want)
