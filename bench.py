@@ -319,6 +319,32 @@ def helical_views(args, device, rho_fine, views: int = 8):
             cells = [(k1 - k0) // 4 for k0, k1 in res[2]]
             out[name]["transport_slices_per_view"] = cells
     out["cropped_speedup"] = out["cropped"]["views_per_s"] / out["covering"]["views_per_s"]
+    # the two-stage schedule on the cropped grids: views 0-3 take their coarse step first, then one step of all
+    # eight (0-3 entering the fine stage with prolonged accumulators, 4-7 at the coarse stage) with every view's
+    # accumulator update in one persistent launch over the uneven grids
+    from dataclasses import replace as dc_replace
+
+    from citrus_amd.multigrid import MultigridSchedule, citrus_multigrid_views_step
+
+    coarse_solver = ScatterSolver(library, dc_replace(config, transport_cell_cm=2 * config.transport_cell_cm),
+                                  geometry, detector=energy_weighted_response(), source=source, device=device)
+    sched = MultigridSchedule(coarse=coarse_solver, fine=solver, coarse_steps=1)
+    half = views // 2
+    fresh = [CitrusState(cone_quadrature=c) for c in cones]
+    _, early, _ = citrus_multigrid_views_step(sched, fresh[:half], rho, fine, angles[:half], crop=True)
+    mixed = early + fresh[half:]
+    res = citrus_multigrid_views_step(sched, mixed, rho, fine, angles, crop=True)  # warm-up of this configuration
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = citrus_multigrid_views_step(sched, mixed, rho, fine, angles, crop=True)
+    torch.cuda.synchronize()
+    out["multigrid_cropped"] = {
+        "views_per_s": views / (time.perf_counter() - t0),
+        "grids": [list(s.collided_moments.shape[-3:]) for s in res[1]],
+        "stages": [s for s, _ in res[2]],
+        "note": "one step of 8 helical views on beam-cropped grids, 4 entering the fine stage (64x64 cells at "
+                "0.32 cm, accumulators prolonged from the coarse stage) and 4 at the coarse stage (32x32 at 0.64 cm); "
+                "all accumulator updates in one persistent sweep launch; wall clock"}
     out["note"] = (f"{views} views of one helical turn (pitch 1.0 = {beam_at_iso:.2f} cm per turn), 64 x 256 curved "
                    "panel, bowtie 0-3 cm at 2.7 g/cm^3, 5% off-focal at 0.5 cm; second step of each view (with its "
                    "accumulator), wall clock; the covering grid is 64 slices")
@@ -412,8 +438,9 @@ def main():
     from citrus_amd.scatter_solver.autotune import default_tuner
 
     ctx_t, mu_t_t = build_problem(argparse.Namespace(**{**vars(args), "views": 1}), device)[:2]
-    chunk = default_tuner().best_chunk(ctx_t, mu_t_t, V)
-    tuned = {str(k): round(v, 3) for k, v in default_tuner().timings[default_tuner().key(ctx_t, 16, V, "fp32")].items()}
+    schedule, chunk = default_tuner().best_config(ctx_t, mu_t_t, V)
+    tuned = {f"{s}:{c}": round(v, 3)
+             for (s, c), v in default_tuner().timings[("config",) + default_tuner().key(ctx_t, 16, V, "fp32")].items()}
     del ctx_t, mu_t_t
 
     # ---- CITRUS view steps (the metric) --------------------------------------------------------------------
@@ -424,7 +451,7 @@ def main():
 
         def view_step(contraction="fp32", st=states):
             results, st[:] = citrus_views_step(solver, st, fields, angles, contraction=contraction,
-                                               direction_chunk=chunk)
+                                               direction_chunk=chunk, schedule=schedule)
             return results
 
         for _ in range(max(1, args.warmup)):  # the first step of a view has no accumulator yet
@@ -487,7 +514,7 @@ def main():
     T = torch.empty_like(phi_a)
     for _ in range(1):
         scatter_moments_views(transfer, phi_a, out=T)
-        sweep_views(ctx, mu_t, T, out=phi_b, direction_chunk=chunk)
+        sweep_views(ctx, mu_t, T, out=phi_b, direction_chunk=chunk, schedule=schedule)
         phi_a, phi_b = phi_b, phi_a
     torch.cuda.synchronize()
     sweep_steps = 2 if view is not None else args.steps
@@ -498,7 +525,7 @@ def main():
     for s in range(sweep_steps):
         scatter_moments_views(transfer, phi_a, out=T)
         ev[s][0].record(stream)
-        sweep_views(ctx, mu_t, T, out=phi_b, direction_chunk=chunk)
+        sweep_views(ctx, mu_t, T, out=phi_b, direction_chunk=chunk, schedule=schedule)
         ev[s][1].record(stream)
         phi_a, phi_b = phi_b, phi_a
     s_stop.record(stream)
@@ -510,7 +537,7 @@ def main():
     # the same batched sweep captured once in a HIP graph and replayed
     from citrus_amd.scatter_solver.transport_sweep import SweepGraph
 
-    graph = SweepGraph(ctx, mu_t, T, direction_chunk=chunk)
+    graph = SweepGraph(ctx, mu_t, T, direction_chunk=chunk, schedule=schedule)
     torch.cuda.synchronize()
     g_start, g_stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     g_start.record(stream)
@@ -522,13 +549,18 @@ def main():
     del graph
     G, nsh = 16, ctx.num_moments
     N = ctx.num_cells
-    # compulsory bytes of one view-sweep in the sweep kernel: moments T in and phi out (float32) plus mu_t; the
-    # launches of a sweep split it by block planes, so bytes per launch / launch time averages to the same ratio
+    # compulsory bytes of one view-sweep in the sweep kernel: moments T in and phi out (float32) plus mu_t. The
+    # launch sequence splits a batched sweep into block-plane launches, the persistent schedule runs one launch per
+    # direction chunk for all V views; either way bytes per launch / launch time averages to the same ratio
     alg_bytes = 2 * G * N * nsh * 4 * 4 + G * N * 4
     n_chunks = -(-ctx.directions_per_octant // chunk)
-    launches = sweep_launches(coarse.shape[0]) * n_chunks
+    if schedule == "persistent":
+        launches = n_chunks  # per batched sweep of V views
+        kernel = f"ld_sweep_persistent_kernel<{nsh}, {(chunk + 3) // 4}>"
+    else:
+        launches = sweep_launches(coarse.shape[0]) * n_chunks * 1  # per batched sweep (V views per launch)
+        kernel = f"ld_sweep_mfma_kernel<{nsh}, {(chunk + 3) // 4}>"
     achieved = alg_bytes * V / (sweep_ms * 1e-3) / 1e9
-    kernel = f"ld_sweep_mfma_kernel<{nsh}, {(chunk + 3) // 4}>"
     traffic = profiled_traffic(kernel, V)
     sweep_rate = V * sweep_steps / (sweep_total_ms * 1e-3)
     if view is not None:
@@ -557,7 +589,7 @@ def main():
                    "directions": ctx.num_directions, "moments": nsh, "views_per_step": V,
                    "sweep_only_view_sweeps_per_s": sweep_rate, "sweep_ms_per_view": sweep_ms / V,
                    "sweep_graph_ms_per_view": graph_ms / V,
-                   "direction_chunk": chunk, "tuner_ms_per_chunk": tuned,
+                   "schedule": schedule, "direction_chunk": chunk, "tuner_ms_per_schedule_chunk": tuned,
                    "wall_s": wall,
                    **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),
                    **({"learned_vs_plain": view["learned"]} if view and "learned" in view else {}),
@@ -567,7 +599,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "alg_bytes_per_view_sweep": alg_bytes,
                      "alg_bytes_per_launch": alg_bytes * V / launches,
-                     "avg_launch_ms": sweep_ms / launches, "launches_per_view_sweep": launches,
+                     "avg_launch_ms": sweep_ms / launches, "launches_per_batched_sweep": launches,
+                     "views_per_batched_sweep": V,
                      "traffic_rate_gbs": (traffic / (sweep_ms / launches * 1e-3) / 1e9) if traffic else None,
                      "traffic_frac": (traffic / (sweep_ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                      "traffic_source": "profiles/ld_sweep_mfma_64cube_s12_p5_g16.txt" if traffic else None},

@@ -129,7 +129,7 @@ def citrus_view_step(solver: ScatterSolver, state: CitrusState, fields: Material
 
 def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fields: MaterialFields,
                       angles: Sequence[float], *, primary: bool = True, contraction: str = "fp32",
-                      direction_chunk=None):
+                      direction_chunk=None, schedule: str = "auto"):
     """``citrus_view_step`` of V views that share the densities, batched for the GPU: per view the uncollided
     fluence and the corrected first-collision source (group-major, every sweep direction); the scattering moments of
     every accumulator; then the V accumulator updates ``Phi_new = G(S_FC + K Phi)`` in one sweep with the views
@@ -137,7 +137,31 @@ def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fiel
     the signal-only image and the primary. Returns ([(scatter, primary or None)] per view, [new state] per view);
     each view's result equals its own ``citrus_view_step`` to float32 rounding. ``contraction="bf16"`` runs the
     sweep's scattering-source expansion as a bfloat16 matrix product (float32 accumulation, per-group scales);
-    ``direction_chunk`` is passed to ``sweep_views`` ("auto": the autotuner's choice)."""
+    ``direction_chunk`` and ``schedule`` are passed to ``sweep_views`` ("auto": the autotuner's choice)."""
+    batch = _prepare_views(solver, states, fields, angles)
+    collided = sweep_views(batch.ctx, fields.mu_t_coarse, batch.T, source=batch.source, contraction=contraction,
+                           direction_chunk=direction_chunk, schedule=schedule)
+    batch.source = None
+    return _finish_views(solver, batch, collided, primary)
+
+
+@dataclass
+class _ViewBatch:
+    """Views of one step that share material fields, between the source preparation and the sweep."""
+
+    states: Sequence[CitrusState]
+    fields: MaterialFields
+    angles: Sequence[float]
+    ctx: object
+    cones: list
+    phi_uc: list
+    source: Optional[Tensor]
+    T: Tensor
+
+
+def _prepare_views(solver: ScatterSolver, states: Sequence[CitrusState], fields: MaterialFields,
+                   angles: Sequence[float]) -> _ViewBatch:
+    """Uncollided fluence, group-major first-collision source and scattering moments of the accumulators."""
     V = len(states)
     if V < 1 or len(angles) != V:
         raise ValueError(f"need one angle per state, got {len(angles)} angles for {V} states")
@@ -168,22 +192,61 @@ def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fiel
                 _check_accumulator(solver, fields, st.collided_moments)
                 prev = st.collided_moments.to(device, torch.float32).permute(0, 3, 4, 5, 1, 2).contiguous()
                 scatter_moments_views(fields.transfer_coarse, prev[None], out=T[v:v + 1])
-    collided = sweep_views(ctx, fields.mu_t_coarse, T, source=source, contraction=contraction,
-                           direction_chunk=direction_chunk)
-    del source
+    return _ViewBatch(states=states, fields=fields, angles=angles, ctx=ctx, cones=cones, phi_uc=phi_uc,
+                      source=source, T=T)
+
+
+def _finish_views(solver: ScatterSolver, batch: _ViewBatch, collided: Tensor, primary: bool):
+    """Scattering moments of the new accumulators (kept for the next step), images and new states."""
+    fields = batch.fields
     # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same
     T = scatter_moments_views(fields.transfer_coarse, collided)
     results, new_states = [], []
-    for v, (st, a) in enumerate(zip(states, angles)):
+    for v, (st, a) in enumerate(zip(batch.states, batch.angles)):
         new = collided[v].permute(0, 4, 5, 1, 2, 3)
-        image = solver.scatter_image(fields, a, phi_uc[v], T[v].permute(0, 4, 5, 1, 2, 3)[:, :, 0], cones[v],
-                                     signal_only=True)["signal"]
+        image = solver.scatter_image(fields, a, batch.phi_uc[v], T[v].permute(0, 4, 5, 1, 2, 3)[:, :, 0],
+                                     batch.cones[v], signal_only=True)["signal"]
         primary_signal = solver.primary(fields, a)["signal"] if primary else None
         results.append((image, primary_signal))
-        new_states.append(CitrusState(angle=float(a), collided_moments=new, cone_quadrature=cones[v],
+        new_states.append(CitrusState(angle=float(a), collided_moments=new, cone_quadrature=batch.cones[v],
                                       num_steps=st.num_steps + 1, history=st.history + [_change(new, st)],
                                       scatter_source=T[v], scatter_source_fields=fields, _batch=(T, v)))
     return results, new_states
+
+
+def citrus_grids_step(batches, *, primary: bool = True, contraction: str = "fp32", schedule: str = "auto"):
+    """The steps of view groups on different transport grids -- views cropped to their beams, coarse and fine
+    multigrid stages -- with every group's accumulator updates in ONE sweep: one persistent launch (per 16 views)
+    over all the grids when the schedule allows it (P3/P5, the persistent self-check passed), else one batched sweep
+    per group. ``batches`` is a sequence of ``(solver, states, fields, angles)``; the solvers must share the sweep
+    quadrature and groups (their transport cells may differ). Returns ([results], [new states]) per group, each as
+    ``citrus_views_step`` returns them (bit for bit the same moments)."""
+    from .scatter_solver.kernels.sweep_kernels import PERSISTENT_MOMENTS, persistent_usable
+    from .scatter_solver.transport_sweep import sweep_grids
+
+    prepared = [(solver, _prepare_views(solver, states, fields, angles))
+                for solver, states, fields, angles in batches]
+    if not prepared:
+        return [], []
+    nsh = prepared[0][1].ctx.num_moments
+    device = prepared[0][0].device
+    if schedule not in ("auto", "launches", "persistent"):
+        raise ValueError(f"schedule must be 'auto', 'launches' or 'persistent', got {schedule!r}")
+    one_launch = schedule == "persistent" or (schedule == "auto" and nsh in PERSISTENT_MOMENTS
+                                              and persistent_usable(device)[0])
+    if one_launch:
+        collided = sweep_grids([(b.ctx, b.fields.mu_t_coarse, b.T, b.source) for _, b in prepared],
+                               contraction=contraction)
+    else:
+        collided = [sweep_views(b.ctx, b.fields.mu_t_coarse, b.T, source=b.source, contraction=contraction,
+                                schedule="launches") for _, b in prepared]
+    out_results, out_states = [], []
+    for (solver, b), phi in zip(prepared, collided):
+        b.source = None
+        r, s = _finish_views(solver, b, phi, primary)
+        out_results.append(r)
+        out_states.append(s)
+    return out_results, out_states
 
 
 def _previous_scatter_moments(states: Sequence[CitrusState], fields: MaterialFields) -> Optional[Tensor]:
@@ -201,11 +264,12 @@ def _previous_scatter_moments(states: Sequence[CitrusState], fields: MaterialFie
 
 def citrus_cropped_views_step(solver: ScatterSolver, states: Sequence[CitrusState], rho_fine: Tensor,
                               grid_fine: Grid, angles: Sequence[float], *, primary: bool = True,
-                              contraction: str = "fp32"):
+                              contraction: str = "fp32", schedule: str = "auto"):
     """The step of V views of a helical (or any) scan, each on the transport grid cropped to its beam
-    (``cropping.beam_slice_range``, whole transport cells): views with the same slab share its material fields and
-    run as one ``citrus_views_step`` batch. Returns ([(scatter, primary or None)], [new state], [slab (k0, k1) of the
-    fine grid]) per view; a view's state belongs to its slab."""
+    (``cropping.beam_slice_range``, whole transport cells): views with the same slab share its material fields, and
+    the accumulator updates of every slab run in one sweep (``citrus_grids_step``: one persistent launch over the
+    uneven grids). Returns ([(scatter, primary or None)], [new state], [slab (k0, k1) of the fine grid]) per view; a
+    view's state belongs to its slab."""
     from .scatter_solver.cropping import beam_slice_range, crop_slices
 
     if len(states) != len(angles):
@@ -213,17 +277,19 @@ def citrus_cropped_views_step(solver: ScatterSolver, states: Sequence[CitrusStat
     cell = solver.transport_grid(grid_fine)
     factor = max(1, round(cell.deltas[2] / grid_fine.deltas[2]))
     slabs = [beam_slice_range(solver.geometry, a, grid_fine, factor) for a in angles]
-    results: list = [None] * len(angles)
-    new_states: list = [None] * len(angles)
+    groups = []
     for slab in sorted(set(slabs)):
         idx = [i for i, s in enumerate(slabs) if s == slab]
         sub, rho_sub = crop_slices(grid_fine, rho_fine, *slab)
         fields = solver.material_fields(rho_sub, sub)
-        r, st = citrus_views_step(solver, [states[i] for i in idx], fields, [angles[i] for i in idx],
-                                  primary=primary, contraction=contraction)
+        groups.append((idx, (solver, [states[i] for i in idx], fields, [angles[i] for i in idx])))
+    res, sts = citrus_grids_step([g for _, g in groups], primary=primary, contraction=contraction,
+                                 schedule=schedule)
+    results: list = [None] * len(angles)
+    new_states: list = [None] * len(angles)
+    for (idx, _), r, st in zip(groups, res, sts):
         for j, i in enumerate(idx):
             results[i], new_states[i] = r[j], st[j]
-        del fields
     return results, new_states, slabs
 
 

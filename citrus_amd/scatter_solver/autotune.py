@@ -1,11 +1,12 @@
 """Autotuning of the batched sweep's launch shape.
 
-The matrix-core sweep takes a direction chunk (directions per octant per launch sequence, at most 32): all D8
-directions in one sequence of block-diagonal launches, or fewer, larger-occupancy launches repeated per chunk. The
-best chunk depends on the grid, the groups, the views per launch and the contraction, so it is measured: each
-candidate sweeps synthetic moments of the real shape (one warm-up, then the fastest of ``repeats`` timed runs with HIP
-events on torch's stream), and the fastest is kept per configuration for the life of the process. The tile of the
-cell planes (4^3 blocks) and one group per wave are fixed by the kernel.
+The matrix-core sweep takes a direction chunk (directions per octant per pass, at most 32): all D8 directions at
+once, or fewer, larger-occupancy passes repeated per chunk. It runs under one of two schedules with identical
+results: a sequence of block-diagonal launches, or one persistent launch per chunk (P3 and P5). The best pair depends
+on the grid, the groups, the views per launch and the contraction, so it is measured: each candidate sweeps synthetic
+moments of the real shape (one warm-up, then the fastest of ``repeats`` timed runs with HIP events on torch's stream),
+and the fastest is kept per configuration for the life of the process. The tile of the cell planes (4^3 blocks) and
+one group per wave are fixed by the kernel.
 """
 
 from __future__ import annotations
@@ -27,7 +28,8 @@ def chunk_candidates(D8: int) -> list[int]:
 
 
 class SweepTuner:
-    """Fastest direction chunk of ``sweep_views`` per (grid shape, moments, directions, groups, views, contraction)."""
+    """Fastest direction chunk of the launch-sequence ``sweep_views`` (``best_chunk``) and fastest (schedule, chunk)
+    over both schedules (``best_config``) per (grid shape, moments, directions, groups, views, contraction)."""
 
     def __init__(self, repeats: int = 2):
         self.repeats = int(repeats)
@@ -55,7 +57,8 @@ class SweepTuner:
             out = torch.empty_like(T)
 
             def run(chunk: int) -> None:
-                sweep_views(context, mu_t, T, out=out, direction_chunk=chunk, contraction=contraction)
+                sweep_views(context, mu_t, T, out=out, direction_chunk=chunk, contraction=contraction,
+                            schedule="launches")
 
         stream = torch.cuda.current_stream(mu_t.device)
         times = {}
@@ -70,6 +73,49 @@ class SweepTuner:
                 b.synchronize()
                 best = min(best, a.elapsed_time(b))
             times[c] = best
+        choice = min(times, key=times.get)
+        self.cache[key] = choice
+        self.timings[key] = times
+        return choice
+
+    def best_config(self, context, mu_t: Tensor, V: int, contraction: str = "fp32",
+                    run: Optional[Callable[[str, int], None]] = None) -> tuple[str, int]:
+        """The fastest (schedule, chunk) for this configuration, over both schedules (measured once, then cached)."""
+        G = int(mu_t.shape[0])
+        key = ("config",) + self.key(context, G, V, contraction)
+        if key in self.cache:
+            return self.cache[key]
+        from .kernels.sweep_kernels import PERSISTENT_MAX_VIEWS, PERSISTENT_MOMENTS, persistent_usable
+        from .transport_sweep import sweep_views
+
+        chunks = chunk_candidates(context.directions_per_octant)
+        cands = [("launches", c) for c in chunks]
+        if context.num_moments in PERSISTENT_MOMENTS and V <= PERSISTENT_MAX_VIEWS and \
+                persistent_usable(mu_t.device)[0]:
+            cands += [("persistent", c) for c in chunks]
+        if run is None:
+            R, C, S = context.shape
+            T = torch.zeros(V, G, R, C, S, context.num_moments, 4, device=mu_t.device, dtype=torch.float32)
+            T[..., 0, 0] = 1.0
+            out = torch.empty_like(T)
+
+            def run(schedule: str, chunk: int) -> None:
+                sweep_views(context, mu_t, T, out=out, direction_chunk=chunk, contraction=contraction,
+                            schedule=schedule)
+
+        stream = torch.cuda.current_stream(mu_t.device)
+        times = {}
+        for sched, c in cands:
+            run(sched, c)  # warm-up
+            best = float("inf")
+            for _ in range(self.repeats):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                run(sched, c)
+                b.record(stream)
+                b.synchronize()
+                best = min(best, a.elapsed_time(b))
+            times[(sched, c)] = best
         choice = min(times, key=times.get)
         self.cache[key] = choice
         self.timings[key] = times

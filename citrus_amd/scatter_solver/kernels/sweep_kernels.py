@@ -213,6 +213,49 @@ class PersistentSchedule:
 
 
 _PERSISTENT: dict = {}
+_PERSISTENT_CHECK: dict[str, tuple[bool, str]] = {}
+
+
+def persistent_usable(device) -> tuple[bool, str]:
+    """Whether the persistent sweep runs through on ``device``: a small two-grid problem swept persistently must
+    complete every task (status 0) and equal the launch-sequence sweep bit for bit. Cached per device; never
+    raises."""
+    device = torch.device(device)
+    key = str(device)
+    if key not in _PERSISTENT_CHECK:
+        ok, why = kernels_usable(device)
+        if ok:
+            try:
+                ok, why = _persistent_self_check(device)
+            except Exception as e:  # noqa: BLE001
+                ok, why = False, f"persistent self-check failed ({type(e).__name__}: {str(e)[:300]})"
+        _PERSISTENT_CHECK[key] = (ok, why)
+    return _PERSISTENT_CHECK[key]
+
+
+def _persistent_self_check(device: torch.device) -> tuple[bool, str]:
+    from ..angular_quadrature import level_symmetric_quadrature
+    from ..geometry import Grid
+    from ..transport_sweep import make_sweep_context, sweep_grids, sweep_views
+
+    q = level_symmetric_quadrature(4)
+    gen = torch.Generator().manual_seed(13)
+    items = []
+    for shape in ((5, 6, 7), (8, 4, 4)):
+        ctx = make_sweep_context(Grid(shape, 0.5), q, 3, device=device)
+        mu = (torch.rand(2, *shape, generator=gen) + 0.3).to(device)
+        T = torch.randn(2, 2, *shape, 16, 4, generator=gen).to(device)
+        items.append((ctx, mu, T, None))
+    outs = sweep_grids(items)
+    shapes = [tuple(c.shape) for c, _, T, _ in items for _ in range(int(T.shape[0]))]
+    status, unfinished = persistent_schedule(shapes, 2, device).check()
+    if status or unfinished:
+        return False, f"persistent sweep did not run through: {PERSISTENT_STATUS.get(status, status)}, " \
+                      f"{unfinished} tasks unfinished"
+    for (ctx, mu, T, _), got in zip(items, outs):
+        if not torch.equal(got, sweep_views(ctx, mu, T, schedule="launches")):
+            return False, "persistent sweep differs from the launch-sequence sweep"
+    return True, "persistent sweep runs through and equals the launch-sequence sweep"
 
 
 def persistent_schedule(shapes, G: int, device: torch.device) -> PersistentSchedule:

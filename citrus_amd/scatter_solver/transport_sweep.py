@@ -301,7 +301,7 @@ def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: Sourc
 
 def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out: Optional[Tensor] = None,
                 direction_chunk: Optional[int] = None, source: Optional[Tensor] = None,
-                contraction: str = "fp32", schedule: str = "launches") -> Tensor:
+                contraction: str = "fp32", schedule: str = "auto") -> Tensor:
     """Sweep V views that share the material fields, each from its own scattering-source moments, in one launch
     sequence: the batched form of one ``sweep`` per view of an outer iteration (citrus.py:1920-1935).
 
@@ -310,15 +310,16 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         mu_t: [G, R, C, S] float32 total attenuation shared by the views.
         moments: [V, G, R, C, S, nsh, 4] float32 contiguous scattering-source moments (group-major layout).
         out: optional [V, G, R, C, S, nsh, 4] float32 output buffer.
-        direction_chunk: directions per octant per launch sequence. Defaults to all; "auto" takes the fastest
-            measured by the autotuner for this configuration.
+        direction_chunk: directions per octant per pass. Defaults to all; "auto" takes the fastest chunk (and, with
+            ``schedule="auto"``, schedule) measured by the autotuner for this configuration.
         source: optional explicit source added to the moments' expansion, every direction of every view:
             [V, G, R, C, S, M, 4] LD moments or [V, G, R, C, S, M] cell means (zero slopes), M the reordered
             directions (the summed source of the CITRUS step, citrus.py:232-236).
         contraction: "fp32", or "bf16" for the source expansion as a bfloat16 matrix product with float32
             accumulation, the moments scaled per (view, group) by a power of two (P3 and P5).
-        schedule: "launches" (one launch per block diagonal) or "persistent" (one launch of resident waves per
-            direction chunk and at most 16 views, P3 and P5); both give the same moments bit for bit.
+        schedule: "launches" (one launch per block diagonal), "persistent" (one launch of resident waves per
+            direction chunk and at most 16 views, P3 and P5) or "auto" (persistent where it is built and passed its
+            self-check on this device, else launches); all give the same moments bit for bit.
 
     Returns:
         [V, G, R, C, S, nsh, 4] fluence moments.
@@ -346,12 +347,23 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             raise ValueError("source must be a contiguous float32 tensor on the moments' device")
     if contraction not in ("fp32", "bf16"):
         raise ValueError(f"contraction must be 'fp32' or 'bf16', got {contraction!r}")
-    if schedule not in ("launches", "persistent"):
-        raise ValueError(f"schedule must be 'launches' or 'persistent', got {schedule!r}")
+    if schedule not in ("auto", "launches", "persistent"):
+        raise ValueError(f"schedule must be 'auto', 'launches' or 'persistent', got {schedule!r}")
     if out is None:
         out = torch.empty_like(moments)
     elif tuple(out.shape) != tuple(moments.shape) or not out.is_contiguous() or out.dtype != torch.float32:
         raise ValueError("out must be a contiguous float32 tensor shaped like moments")
+    if schedule == "auto":
+        from .kernels.sweep_kernels import PERSISTENT_MOMENTS, persistent_usable
+
+        if direction_chunk == "auto":  # both measured once per configuration (autotune.SweepTuner)
+            from .autotune import default_tuner
+
+            schedule, direction_chunk = default_tuner().best_config(sweep_context, mu_t, V, contraction)
+        elif nsh in PERSISTENT_MOMENTS and persistent_usable(moments.device)[0]:
+            schedule = "persistent"
+        else:
+            schedule = "launches"
     if schedule == "persistent":
         return _sweep_grids_persistent([(sweep_context, mu_t, moments, source, out)], direction_chunk,
                                        contraction)[0]
@@ -463,21 +475,23 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
 
 
 class SweepGraph:
-    """``sweep_views`` captured once in a HIP graph on fixed buffers and replayed: the 184-odd launches of a batched
-    view-sweep (and the bfloat16 operand's conversion) become one graph launch. The caller writes each step's
+    """``sweep_views`` captured once in a HIP graph on fixed buffers and replayed: the launches of a batched
+    view-sweep (184-odd block-plane launches, or the persistent schedule's flag clear and one launch per chunk, and
+    the bfloat16 operand's conversion) become one graph launch. The caller writes each step's
     moments (and explicit source) into ``moments`` (``source``) and calls ``replay()``; the result is in ``out``.
     The sweep records no memset and its kernels are deterministic, so every replay equals the eager sweep of the
     same inputs bit for bit.
     """
 
     def __init__(self, sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, source: Optional[Tensor] = None,
-                 contraction: str = "fp32", direction_chunk: Optional[int] = None):
+                 contraction: str = "fp32", direction_chunk: Optional[int] = None, schedule: str = "auto"):
         self.context = sweep_context
         self.mu_t = mu_t
         self.moments = moments
         self.source = source
         self.out = torch.empty_like(moments)
-        kwargs = dict(out=self.out, direction_chunk=direction_chunk, source=source, contraction=contraction)
+        kwargs = dict(out=self.out, direction_chunk=direction_chunk, source=source, contraction=contraction,
+                      schedule=schedule)
         device = moments.device
         side = torch.cuda.Stream(device=device)  # default priority
         side.wait_stream(torch.cuda.current_stream(device))

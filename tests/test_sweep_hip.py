@@ -351,6 +351,7 @@ def test_sweep_autotuner_picks_and_caches_a_chunk():
     assert tuner.best_chunk(ctx, mu, V, run=lambda c: calls.append(c)) == chunk and not calls  # cached
     T = torch.randn(V, G, *shape, ctx.num_moments, 4, generator=gen).to(dev)
     auto = sweep_views(ctx, mu, T, direction_chunk="auto")
-    chosen = default_tuner().cache[default_tuner().key(ctx, G, V, "fp32")]
-    assert rel_err(auto, sweep_views(ctx, mu, T, direction_chunk=chosen)) < F32_REL
+    schedule, chosen = default_tuner().cache[("config",) + default_tuner().key(ctx, G, V, "fp32")]
+    assert schedule == "launches"  # P2: the persistent kernel is built for P3 and P5 only
+    assert torch.equal(auto, sweep_views(ctx, mu, T, direction_chunk=chosen, schedule=schedule))
     assert rel_err(auto, sweep_views(ctx, mu, T)) < F32_REL
