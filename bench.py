@@ -226,19 +226,22 @@ def sweep_launches(grid_cells: int, block: int = 4) -> int:
 
 
 def profiled_traffic(kernel: str, views: int):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this workload, or None (also
-    when the summary was taken at another number of views per launch).
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summaries of this workload
+    (profiles/ld_sweep_*_64cube_s12_p5_g16.txt), or None (also when the summary was taken at another number of views
+    per launch).
 
-    The summary lists FETCH_SIZE and WRITE_SIZE per dispatch scaled by the same counters' reading of a plain copy of
+    A summary lists FETCH_SIZE and WRITE_SIZE per dispatch scaled by the same counters' reading of a plain copy of
     known size (its `traffic_bytes_per_launch` line)."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ld_sweep_mfma_64cube_s12_p5_g16.txt")
+    folder = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
     try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch") and kernel in line and f"{views} views" in line:
-                    return float(line.split("=")[1].split()[0])
+        names = sorted(f for f in os.listdir(folder) if f.startswith("ld_sweep_") and f.endswith(".txt"))
     except OSError:
         return None
+    for name in names:
+        with open(os.path.join(folder, name)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch") and kernel in line and f"{views} views" in line:
+                    return float(line.split("=")[1].split()[0]), f"profiles/{name}"
     return None
 
 
@@ -561,7 +564,7 @@ def main():
         launches = sweep_launches(coarse.shape[0]) * n_chunks * 1  # per batched sweep (V views per launch)
         kernel = f"ld_sweep_mfma_kernel<{nsh}, {(chunk + 3) // 4}>"
     achieved = alg_bytes * V / (sweep_ms * 1e-3) / 1e9
-    traffic = profiled_traffic(kernel, V)
+    traffic, traffic_source = profiled_traffic(kernel, V) or (None, None)
     sweep_rate = V * sweep_steps / (sweep_total_ms * 1e-3)
     if view is not None:
         value, ms_per_step, wall = V * args.steps / (view["ms"] * 1e-3), view["ms"] / args.steps, view["wall"]
@@ -603,7 +606,7 @@ def main():
                      "views_per_batched_sweep": V,
                      "traffic_rate_gbs": (traffic / (sweep_ms / launches * 1e-3) / 1e9) if traffic else None,
                      "traffic_frac": (traffic / (sweep_ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
-                     "traffic_source": "profiles/ld_sweep_mfma_64cube_s12_p5_g16.txt" if traffic else None},
+                     "traffic_source": traffic_source},
     }
     if not args.no_cpu_baseline:
         sweep_part = cpu_baseline(args, ctx, library, rho, coarse)
