@@ -126,6 +126,10 @@ __device__ __forceinline__ void eval_source(const FcDev& d, const PointData<GMAX
   const int64_t node = (int64_t)d.M * d.G * GMAX;
   for (int s = 0; s < d.n_slots; ++s) {
     const float rho = d.slot_density[(int64_t)s * d.P + pd.p];
+    // a material absent from the point adds exact zeros (the table is finite, and x + 0 == x for the sums, which
+    // never hold -0): skip its gathers, which a wave then issues only for the lanes that hold the material -- none
+    // outside the body, one slot in most cells
+    if (rho == 0.0f) continue;
     const float* t0 = d.table + (int64_t)i0 * node + (int64_t)d.slot_material[(int64_t)s * d.P + pd.p] * d.G * GMAX;
 #pragma unroll
     for (int gi = 0; gi < GMAX; ++gi) {
