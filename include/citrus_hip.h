@@ -153,6 +153,26 @@ int citrus_persistent_check(const CitrusPersistentPlan* plan, const void* worksp
                             int64_t* unfinished, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Compact per-view accumulators: the LD cell mean of every (view, group, moment) as float16 scaled by the
+ * (group, moment)'s maximum magnitude over 2^14 (scale 1 where that maximum is 0).
+ *
+ * Replaces StateStore.put / StateStore.get (citrus/citrus.py:686-737), batched over views and reading the sweep's
+ * group-major moments in place.
+ * --------------------------------------------------------------------------------------------------------------- */
+
+/* Floats of the workspace citrus_state_pack needs (per-block maxima). */
+int64_t citrus_state_pack_workspace_floats(int V, int G, int nsh, int64_t N);
+
+/* phi [V][G][N][nsh][4] (component 0 = cell mean) -> scale [V][G][nsh] float32 and q [V][G][nsh][N] float16. */
+int citrus_state_pack(const float* phi, int V, int G, int nsh, int64_t N, float* scale, void* q, float* workspace,
+                      int64_t workspace_floats, void* stream);
+
+/* q [V][G][nsh][N] float16 and scale [V][G][nsh] -> phi [V][G][N][nsh][4] with phi[..][0] = q * scale and zero
+ * LD slopes. */
+int citrus_state_unpack(const void* q, const float* scale, int V, int G, int nsh, int64_t N, float* phi,
+                        void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Scattering-source moments T[go][n][k] = sum_{gi} mu_s_l[gi][go][l(n)] phi[gi][n][k] per cell.
  *
  * Replaces `_scattering_moments_kernel` / `scattering_moments` (citrus/scatter_solver/kernels/

@@ -580,3 +580,30 @@ def test_surrogate_adam_and_bf16_rounding():
             assert gv == wv
     ref = torch.from_numpy(np.random.default_rng(7).standard_normal(1000).astype(np.float32))
     assert np.array_equal(osg.bf16_round(ref.numpy()), ref.to(torch.bfloat16).double().numpy())
+
+
+def test_state_store_oracle_format():
+    """The compact accumulator: the maximum maps to 2^14 exactly, zeros and float16-representable values round-trip
+    exactly, anything else within half a float16 step of its (group, moment) scale."""
+    from oracle import state_store as oss
+
+    rng = np.random.default_rng(17)
+    m = (rng.standard_normal((3, 4, 5, 6, 7)) * np.array([1e-3, 1.0, 1e4])[:, None, None, None, None]).astype(np.float32)
+    m[1, 2] = 0.0  # an all-zero moment: scale 1
+    q, scale = oss.pack(m)
+    assert q.dtype == np.float16 and scale.dtype == np.float32 and scale[1, 2] == 1.0
+    flat = np.abs(m.reshape(3, 4, -1))
+    for g in range(3):
+        for n in range(4):
+            if flat[g, n].max() > 0:
+                i = int(flat[g, n].argmax())
+                assert abs(float(q.reshape(3, 4, -1)[g, n, i])) == 2.0 ** 14
+    back = oss.unpack(q, scale)
+    assert np.array_equal(back[1, 2], m[1, 2])
+    err = np.abs(back - m).reshape(3, 4, -1).max(axis=2)
+    assert np.all(err <= scale * 2.0 ** 14 * 2.0 ** -11 + 1e-30)
+    # values already on the float16 grid times a power-of-two scale come back exactly
+    grid_vals = (np.arange(-2048, 2048, dtype=np.float32) * 8.0).reshape(1, 1, 64, 64)
+    grid_vals[0, 0, 0, 0] = 2.0 ** 14
+    q2, s2 = oss.pack(grid_vals)
+    assert s2[0, 0] == 1.0 and np.array_equal(oss.unpack(q2, s2), grid_vals)
