@@ -354,6 +354,35 @@ def helical_views(args, device, rho_fine, views: int = 8):
     return out
 
 
+def scan_stage(solver, fields, device, views: int, batch: int):
+    """One outer iteration's scatter stage over a scan of `views` views (16 of a 64-view circle) with the
+    accumulators in the compact state store (scaled float16 cell means, resident in HBM): per batch of `batch` views,
+    the accumulators from the store, one batched view step, the new accumulators back; scatter, primary and SPR of
+    every view. The second stage is timed (the first starts from empty accumulators)."""
+    import torch
+
+    from citrus_amd.citrus import CitrusState
+    from citrus_amd.state_store import StateStore, citrus_scan_step
+
+    angles = [2.0 * math.pi * k / 64 for k in range(views)]
+    ctx = solver.sweep_context(fields.grid_coarse)
+    store = StateStore(views, (solver.num_groups, ctx.num_moments, 4) + tuple(fields.grid_coarse.shape), device)
+    states = [CitrusState() for _ in angles]
+    _, _, _, states = citrus_scan_step(solver, store, states, fields, angles, views_per_batch=batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, ratio, states = citrus_scan_step(solver, store, states, fields, angles, views_per_batch=batch)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    out = {"views_per_s": views / seconds, "views": views, "views_per_batch": batch,
+           "store": store.describe(), "store_seconds_per_stage": store.transfer_seconds / 2,
+           "spr_mean": float(ratio.mean()),
+           "note": "second scatter stage of a scan: accumulators decoded from and packed into the state store "
+                   "(float16 cell means with per-(group, moment) scales) around each batched view step; wall clock"}
+    store.close()
+    return out
+
+
 SURROGATE_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "citrus_amd", "weights",
                                  "surrogate_64cube_s12_p5_g16.npz")
 
@@ -496,6 +525,8 @@ def main():
                     "the same empty states, which match the FP64 oracle to 1e-5 at the tested sizes"}
         del states, images, states16, images16
         torch.cuda.empty_cache()
+        view["scan_stage"] = scan_stage(solver, fields, device, views=16, batch=V)
+        torch.cuda.empty_cache()
         if not args.no_learned:
             try:
                 view["learned"] = learned_vs_plain(solver, fields, angles, device)
@@ -597,6 +628,7 @@ def main():
                    **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),
                    **({"learned_vs_plain": view["learned"]} if view and "learned" in view else {}),
                    **({"bf16_contraction": view["bf16"]} if view and "bf16" in view else {}),
+                   **({"scan_stage": view["scan_stage"]} if view and "scan_stage" in view else {}),
                    **({"helical_cropped": view["helical"]} if view and "helical" in view else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
