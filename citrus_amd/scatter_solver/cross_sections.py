@@ -36,11 +36,41 @@ class MultigroupLibrary:
     fine_group: Tensor         # [E] int64 group of each fine energy
     eta_map: Optional[Tensor] = None  # [K] node map of an adaptive angular table, None for a uniform one
     group_energy: Optional[Tensor] = None  # [G] keV, the group-edge midpoints when not given
+    mu_pe: Optional[Tensor] = None  # [M, G] cm^2/g photoelectric part of mu_t (zeros when not given)
+    nominal_density: Optional[Tensor] = None  # [M] g/cm^3 nominal densities of the materials (the density model's)
 
     def __post_init__(self) -> None:
         if self.group_energy is None:
             e = self.group_edges.to(torch.float64)
             object.__setattr__(self, "group_energy", 0.5 * (e[:-1] + e[1:]))
+        if self.mu_pe is None:
+            object.__setattr__(self, "mu_pe", torch.zeros_like(self.mu_t))
+
+    # fields of the reference's library file, in its order (cross_sections.py:519-534)
+    FILE_FIELDS = ("group_edges", "group_energy", "spectrum_fraction", "mu_t", "mu_pe", "mu_s_l", "eta_grid",
+                   "mu_s_eta", "fine_energy", "fine_mu_t", "fine_weight", "fine_group", "materials", "nominal_density",
+                   "eta_map")
+
+    def save(self, path: str) -> None:
+        """Write every field with ``torch.save`` as the dict of the reference's ``MultigroupLibrary.save``
+        (cross_sections.py:624-631), so either implementation reads the other's files."""
+        fields = {k: getattr(self, k) for k in self.FILE_FIELDS}
+        fields["materials"] = tuple(self.materials)
+        torch.save(fields, path)
+
+    @staticmethod
+    def load(path: str) -> "MultigroupLibrary":
+        """Read a library file written by ``save`` here or by the reference (cross_sections.py:633-650), with
+        ``torch.load(weights_only=True)`` (tensors, strings and None only; nothing in the file is executed). A file
+        without ``eta_map`` loads as a uniform table; a file without another field raises ``KeyError``."""
+        fields = torch.load(path, map_location="cpu", weights_only=True)
+        if not isinstance(fields, dict):
+            raise ValueError(f"library file {path!r} does not hold a field dict")
+        fields.setdefault("eta_map", None)
+        missing = [k for k in MultigroupLibrary.FILE_FIELDS if k not in fields]
+        if missing:
+            raise KeyError(f"library file {path!r} lacks fields {missing} (written by an older version)")
+        return MultigroupLibrary(**{k: fields[k] for k in MultigroupLibrary.FILE_FIELDS})
 
     @property
     def num_groups(self) -> int:
@@ -127,7 +157,8 @@ def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int
                          dtype=torch.float64)
     mu_t = mu_pe + mu_s_l[..., 0].sum(dim=2)
     return MultigroupLibrary(edges, mu_t, mu_s_l, eta, mu_s_eta, ("water", "bone_cortical_icrp"),
-                             *_fine_tables(edges, mu_t, num_fine_per_group, fine_spread), eta_map=eta_map)
+                             *_fine_tables(edges, mu_t, num_fine_per_group, fine_spread), eta_map=eta_map,
+                             mu_pe=mu_pe, nominal_density=torch.tensor([1.0, 1.92], dtype=torch.float64))
 
 
 def seeded_library(num_groups: int, legendre_order: int, e_max_kev: float, e_min_kev: float, seed: int,
@@ -162,7 +193,9 @@ def seeded_library(num_groups: int, legendre_order: int, e_max_kev: float, e_min
                 mu_s_l[m, gi, go] = s * g_hg ** torch.arange(L + 1, dtype=torch.float64)
                 mu_s_eta[m, gi, go] = s / (4 * math.pi) * (1 - g_hg ** 2) / (1 + g_hg ** 2 - 2 * g_hg * eta) ** 1.5
     return MultigroupLibrary(edges, totals, mu_s_l, eta, mu_s_eta, ("water", "bone"),
-                             *_fine_tables(edges, totals, num_fine_per_group, fine_spread))
+                             *_fine_tables(edges, totals, num_fine_per_group, fine_spread),
+                             mu_pe=totals - mu_s_l[..., 0].sum(dim=2),
+                             nominal_density=torch.tensor([1.0, 1.85], dtype=torch.float64))
 
 
 def mu_t_volume(library: MultigroupLibrary, rho: Tensor) -> Tensor:

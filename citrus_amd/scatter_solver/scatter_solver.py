@@ -178,6 +178,16 @@ class ScatterSolver:
                               traced_materials=tuple(traced) if traced else (0,),
                               fine_materials=tuple(present_materials(rho)))
 
+    def material_fields_from_attenuation(self, X: Tensor, grid_fine: Grid, density_model) -> MaterialFields:
+        """Fields of the densities ``density_model(X)`` of an attenuation volume X [R, C, S] (1/cm) on
+        ``grid_fine`` (scatter_solver.py:692): the material maps come from the density model's HIP kernel in one pass
+        and are kept whole on the fine grid (an MI355X holds them; the reference keeps them only under a budget),
+        then resampled as ``material_fields`` does."""
+        if tuple(X.shape) != tuple(grid_fine.shape):
+            raise ValueError(f"X shape {tuple(X.shape)} != grid shape {grid_fine.shape}")
+        X = X.to(self.device, torch.float32).contiguous()
+        return self.material_fields(density_model(X), grid_fine)
+
     # ---- operators -------------------------------------------------------------------------------------------
     def uncollided_fluence(self, fields: MaterialFields, angle: float) -> Tensor:
         """[G, Rq, Cq, Sq] uncollided fluence at the quadrature points (scatter_solver.py:745)."""

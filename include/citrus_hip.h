@@ -153,6 +153,20 @@ int citrus_persistent_check(const CitrusPersistentPlan* plan, const void* worksp
                             int64_t* unfinished, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Density model: the reference materials' densities that reproduce an attenuation volume X (1/cm at the model's
+ * energy), interpolating between the two materials whose nominal attenuations bracket max(X, 0); below the first or
+ * above the last, that material alone with density X / m. Replaces attenuation_to_densities
+ * (citrus/utilities.py:710-753) as DensityModel applies it (citrus/citrus.py:466-509).
+ * --------------------------------------------------------------------------------------------------------------- */
+
+/* X [n] -> rho [M][n] (library material order). material, mu_nominal, nominal_density and mass_attenuation are HOST
+ * arrays of M <= 8 slots in bracketing order (mu_nominal strictly increasing, mu_nominal = nominal_density *
+ * mass_attenuation as float32): each slot's library material index and its parameters. */
+int citrus_attenuation_to_densities(const float* X, int64_t n, int M, const int* material, const float* mu_nominal,
+                                    const float* nominal_density, const float* mass_attenuation, float* rho,
+                                    void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Compact per-view accumulators: the LD cell mean of every (view, group, moment) as float16 scaled by the
  * (group, moment)'s maximum magnitude over 2^14 (scale 1 where that maximum is 0).
  *

@@ -138,3 +138,60 @@ def test_beam_slab_contains_the_beam_and_crops_align():
         zc = sub.axis_coordinates(dtype=torch.float64)[2]
         assert torch.allclose(zc, grid.axis_coordinates(dtype=torch.float64)[2][k0:k1])
         assert torch.equal(rs, rho[..., k0:k1])
+
+
+def test_library_file_round_trip_and_reference_format(tmp_path):
+    """``MultigroupLibrary.save``/``load`` use the reference's field dict (cross_sections.py:624-650): a round trip
+    keeps every field, a dict written with the reference's field names loads, a file without ``eta_map`` loads as a
+    uniform table and a file missing another field raises KeyError."""
+    import torch
+
+    from citrus_amd.scatter_solver.cross_sections import MultigroupLibrary, seeded_library, synthetic_library
+
+    for lib in (seeded_library(4, 3, 120.0, 20.0, seed=5, num_eta=33), synthetic_library(3, 2, num_eta=17,
+                                                                                           angular_table_grid="adaptive")):
+        path = tmp_path / "lib.pt"
+        lib.save(str(path))
+        back = MultigroupLibrary.load(str(path))
+        for k in MultigroupLibrary.FILE_FIELDS:
+            a, b = getattr(lib, k), getattr(back, k)
+            if isinstance(a, torch.Tensor):
+                assert torch.equal(a, b), k
+            else:
+                assert a == b, k
+    fields = torch.load(str(path), weights_only=True)
+    assert set(fields) == set(MultigroupLibrary.FILE_FIELDS)
+    fields.pop("eta_map")
+    torch.save(fields, str(tmp_path / "no_map.pt"))
+    assert MultigroupLibrary.load(str(tmp_path / "no_map.pt")).eta_map is None
+    fields.pop("nominal_density")
+    torch.save(fields, str(tmp_path / "old.pt"))
+    import pytest
+
+    with pytest.raises(KeyError):
+        MultigroupLibrary.load(str(tmp_path / "old.pt"))
+
+
+def test_density_model_host_tables():
+    """Effective energy, log-log mass attenuation (tabulated energies exact, interpolation between), material order."""
+    import math
+
+    import torch
+
+    from citrus_amd.density import effective_energy, mass_attenuation_at, material_order, nominal_attenuation
+    from citrus_amd.scatter_solver.cross_sections import seeded_library
+
+    lib = seeded_library(4, 3, 120.0, 20.0, seed=5, num_eta=33)
+    w, E = lib.fine_weight.double(), lib.fine_energy.double()
+    assert abs(effective_energy(lib) - float((w * E).sum() / w.sum())) < 1e-12
+    order = torch.argsort(E)
+    e0, e1 = float(E[order[3]]), float(E[order[4]])
+    assert torch.equal(mass_attenuation_at(lib, e0), lib.fine_mu_t[:, order[3]].double())
+    mid = math.sqrt(e0 * e1)  # halfway in log E: the geometric mean of the coefficients
+    want = torch.sqrt(lib.fine_mu_t[:, order[3]].double() * lib.fine_mu_t[:, order[4]].double())
+    assert torch.allclose(mass_attenuation_at(lib, mid), want, rtol=1e-12)
+    assert material_order(lib) == torch.argsort(nominal_attenuation(lib)).tolist() == [0, 1]
+    import pytest
+
+    with pytest.raises(ValueError):
+        mass_attenuation_at(lib, 1000.0)

@@ -607,3 +607,31 @@ def test_state_store_oracle_format():
     grid_vals[0, 0, 0, 0] = 2.0 ** 14
     q2, s2 = oss.pack(grid_vals)
     assert s2[0, 0] == 1.0 and np.array_equal(oss.unpack(q2, s2), grid_vals)
+
+
+def test_density_model_oracle_identities():
+    """The interpolating model reproduces the attenuation, maps each nominal attenuation to its material at nominal
+    density, maps X <= 0 to nothing and never mixes more than two adjacent materials."""
+    from oracle import density as od
+
+    mass = np.array([0.2, 0.25, 0.5], dtype=np.float32)  # bracketing order: nominal attenuations 0.2, 0.4, 0.95
+    nominal = np.array([1.0, 1.6, 1.9], dtype=np.float32)
+    mu = (nominal * mass).astype(np.float32)
+    order = np.array([2, 0, 1])  # library index of each slot
+    rng = np.random.default_rng(3)
+    X = np.concatenate([rng.uniform(-0.2, 1.5, 997).astype(np.float32), mu, np.zeros(2, np.float32)])
+    rho = od.attenuation_to_densities(X, order, mu, nominal, mass)
+    m_lib = np.empty(3, np.float32)
+    m_lib[order] = mass
+    back = (rho.astype(np.float64) * m_lib[:, None]).sum(0)
+    assert np.allclose(back, np.maximum(X, 0), rtol=2e-6, atol=1e-7)
+    for s in range(3):  # X = mu_s -> slot s alone at its nominal density
+        r = rho[:, 997 + s]
+        assert r[order[s]] == nominal[s] and np.count_nonzero(r) == 1
+    assert not rho[:, X <= 0].any()
+    nz = rho != 0
+    assert nz.sum(0).max() <= 2
+    slot_of = np.argsort(order)  # library index -> slot
+    for i in np.nonzero(nz.sum(0) == 2)[0]:
+        s = sorted(slot_of[np.nonzero(nz[:, i])[0]])
+        assert s[1] - s[0] == 1
