@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -89,7 +89,7 @@ _SIGNATURES = {
     "citrus_state_unpack": (c_int, [c_void, c_void, c_int, c_int, c_int, c_i64, c_void, c_void]),
     "citrus_scatter_moments_dense": (c_int, [c_void, c_int, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),
     "citrus_scatter_moments_factored": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_int, c_int,
-                                                c_i64, c_int, c_void]),
+                                                c_i64, c_int, c_void, c_void]),
     "citrus_ray_segments": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_int, c_i64, c_void,
                                     c_void]),
     "citrus_ray_trace_indices": (c_int, [c_int, c_int, c_int, c_void, c_void, c_void, c_i64, c_int, c_void, c_void,

@@ -199,18 +199,20 @@ def _prepare_views(solver: ScatterSolver, states: Sequence[CitrusState], fields:
 def _finish_views(solver: ScatterSolver, batch: _ViewBatch, collided: Tensor, primary: bool):
     """Scattering moments of the new accumulators (kept for the next step), images and new states."""
     fields = batch.fields
-    # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same
-    T = scatter_moments_views(fields.transfer_coarse, collided)
+    # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same; its
+    # cell means also moment-major, the image's expansion operand
+    T, means = scatter_moments_views(fields.transfer_coarse, collided, cell_means=True)
     results, new_states = [], []
     for v, (st, a) in enumerate(zip(batch.states, batch.angles)):
         new = collided[v].permute(0, 4, 5, 1, 2, 3)
         image = solver.scatter_image(fields, a, batch.phi_uc[v], T[v].permute(0, 4, 5, 1, 2, 3)[:, :, 0],
-                                     batch.cones[v], signal_only=True)["signal"]
+                                     batch.cones[v], signal_only=True, source_means=means[v])["signal"]
         primary_signal = solver.primary(fields, a)["signal"] if primary else None
         results.append((image, primary_signal))
         new_states.append(CitrusState(angle=float(a), collided_moments=new, cone_quadrature=batch.cones[v],
                                       num_steps=st.num_steps + 1, history=st.history + [_change(new, st)],
                                       scatter_source=T[v], scatter_source_fields=fields, _batch=(T, v)))
+    del means
     return results, new_states
 
 

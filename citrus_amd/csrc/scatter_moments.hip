@@ -35,7 +35,7 @@ constexpr int degree_stride() { return GMAX * GMAX + 1; }
 template <int GMAX, bool FACTORED>
 __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
     const float* __restrict__ transfer, const float* __restrict__ rho, int M, int Lt, const float* __restrict__ phi,
-    float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only) {
+    float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only, float* __restrict__ means) {
   extern __shared__ float lds[];
   constexpr int DS = degree_stride<GMAX>();
   const int per_cell = (L + 1) * DS + 1;  // odd: the two cells' matrices also fall in different banks
@@ -95,6 +95,8 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
     for (int gi = 0; gi < GMAX; ++gi)
       if (gi < G && !(lower_only && gi > go)) acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
     T[off + go * group_stride] = acc;
+    // the cell means again, moment-major [V][nsh][G][N]: the operand of the image's multiple-collision expansion
+    if (means && (r & 3) == 0) means[(((int64_t)v * nsh + (r >> 2)) * G + go) * N + cell] = acc;
   }
 }
 
@@ -110,7 +112,7 @@ int check(int G, int nsh, int Lt, int64_t N, int V, int& L) {
 
 template <bool FACTORED>
 int launch(const float* transfer, const float* rho, int M, int Lt, const float* phi, float* T, int G, int nsh,
-           int64_t N, int V, int lower_only, void* stream) {
+           int64_t N, int V, int lower_only, float* means, void* stream) {
   int L = 0;
   if (int rc = check(G, nsh, Lt, N, V, L)) return rc;
   const int gmax = G <= 8 ? 8 : (G <= 16 ? 16 : 32);
@@ -124,13 +126,13 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
   const dim3 grid((unsigned)blocks), block((unsigned)threads);
   if (G <= 8)
     hipLaunchKernelGGL((scatter_moments_kernel<8, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
-                       nsh, L, (int)N, V, lower_only);
+                       nsh, L, (int)N, V, lower_only, means);
   else if (G <= 16)
     hipLaunchKernelGGL((scatter_moments_kernel<16, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
-                       nsh, L, (int)N, V, lower_only);
+                       nsh, L, (int)N, V, lower_only, means);
   else
     hipLaunchKernelGGL((scatter_moments_kernel<32, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
-                       nsh, L, (int)N, V, lower_only);
+                       nsh, L, (int)N, V, lower_only, means);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("scatter moments: ") + hipGetErrorString(e));
   return CITRUS_OK;
@@ -225,14 +227,15 @@ extern "C" int citrus_scatter_moments_dense(const float* transfer, int Lt, const
                                             int64_t N, int V, void* stream) {
   using namespace citrus;
   if (!transfer || !phi || !T) return fail(CITRUS_ERR_ARG, "NULL buffer");
-  return launch<false>(transfer, nullptr, 0, Lt, phi, T, G, nsh, N, V, 0, stream);
+  return launch<false>(transfer, nullptr, 0, Lt, phi, T, G, nsh, N, V, 0, nullptr, stream);
 }
 
 extern "C" int citrus_scatter_moments_factored(const float* rho, int num_materials, const float* sigma, int Lt,
                                                int downscatter_only, const float* phi, float* T, int G, int nsh,
-                                               int64_t N, int V, void* stream) {
+                                               int64_t N, int V, float* cell_means, void* stream) {
   using namespace citrus;
   if (!rho || !sigma || !phi || !T) return fail(CITRUS_ERR_ARG, "NULL buffer");
   if (num_materials < 1 || num_materials > kMaxMaterials) return fail(CITRUS_ERR_ARG, "1..8 materials supported");
-  return launch<true>(sigma, rho, num_materials, Lt, phi, T, G, nsh, N, V, downscatter_only ? 1 : 0, stream);
+  return launch<true>(sigma, rho, num_materials, Lt, phi, T, G, nsh, N, V, downscatter_only ? 1 : 0, cell_means,
+                      stream);
 }

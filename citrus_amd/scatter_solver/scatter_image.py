@@ -445,12 +445,15 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
                            grid_response: GridResponse, context: Optional[DirectionSetContext] = None,
                            chunk: int = 8, *, first_collision_chunk: int = 4, footprint: bool = True, pairs=None,
                            grid_coarse: Optional[Grid] = None, signal_only: bool = False,
-                           density_coarse: Optional[Tensor] = None) -> dict:
+                           density_coarse: Optional[Tensor] = None, source_means: Optional[Tensor] = None) -> dict:
     """Total scatter image (first order plus multiple collision) of one view in one collapsed-cone pass
     (scatter_image.py:1141). ``scatter_source_moments`` is [G, (L+1)^2, Rc, Cc, Sc] cell means of the multiple-
     collision source moments T, or [G, (L+1)^2, 4, Rc, Cc, Sc] whose means are used. With ``density_coarse``
     [M, Rc, Cc, Sc] (the densities whose ``library.mu_t`` mix is ``mu_t_coarse``) the downstream optical depths come
-    from the M density channels traced along one shared crossing sequence per direction."""
+    from the M density channels traced along one shared crossing sequence per direction. ``source_means``
+    [(L+1)^2, G, Rc, Cc, Sc] (contiguous) are the same cell means already moment-major, the operand of the
+    multiple-collision expansion (``scatter_moments_views(cell_means=True)`` writes them), which then needs no
+    transposition."""
     device = phi_uncollided.device
     rho = rho_quadrature.to(device=device, dtype=torch.float32)
     f = int(q)
@@ -491,7 +494,12 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
             context._cache[key] = projectors.crossing_sequences(grid_coarse, context.omega)
         sequences = context._cache[key]
     nsh = int(T.shape[1])
-    by_cell = T.to(device, torch.float32).transpose(0, 1).reshape(nsh, -1).contiguous()  # [N_sh, G cells]
+    if source_means is not None:
+        if tuple(source_means.shape) != (nsh, G) + tuple(grid_coarse.shape) or not source_means.is_contiguous():
+            raise ValueError(f"source_means must be contiguous [{nsh}, {G}, *{grid_coarse.shape}]")
+        by_cell = source_means.reshape(nsh, -1)
+    else:
+        by_cell = T.to(device, torch.float32).transpose(0, 1).reshape(nsh, -1).contiguous()  # [N_sh, G cells]
     Rc, Cc, Sc = grid_coarse.shape
     fp = _Footprint(grid_quadrature, geometry, angle, context) if footprint else None
     group_weight = context.signal_group_weight if signal_only else None

@@ -277,14 +277,15 @@ class ScatterSolver:
 
     def scatter_image(self, fields: MaterialFields, angle: float, phi_uncollided: Tensor,
                       scatter_source_moments: Tensor, quadrature: Optional[Quadrature] = None, context=None,
-                      signal_only: bool = False) -> dict:
+                      signal_only: bool = False, source_means: Optional[Tensor] = None) -> dict:
         quadrature = self.cone_quadrature(fields.grid_fine, angle) if quadrature is None else quadrature
         context = self.direction_context(fields, angle, quadrature) if context is None else context
         return combined_scatter_image(self.library, fields.rho_quadrature, phi_uncollided, fields.grid_quadrature,
                                       fields.q, scatter_source_moments, self.L, self.geometry, angle, quadrature,
                                       fields.mu_t_coarse, self.projectors, self.detector, self.grid_response,
                                       context, chunk=self.config.direction_chunk, grid_coarse=fields.grid_coarse,
-                                      signal_only=signal_only, density_coarse=fields.rho_coarse)
+                                      signal_only=signal_only, density_coarse=fields.rho_coarse,
+                                      source_means=source_means)
 
     def primary(self, fields: MaterialFields, angle: float) -> dict:
         return primary_image(self.library, fields.rho_fine, fields.grid_fine, self.geometry, angle, self.source,

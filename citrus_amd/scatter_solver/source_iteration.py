@@ -90,7 +90,7 @@ def scatter_moments(mu_s_l_volume, phi: Tensor) -> Tensor:
                 raise ValueError("MaterialTransfer shapes do not match phi")
             _hip.check(lib.citrus_scatter_moments_factored(
                 rho.data_ptr(), int(rho.shape[0]), sigma.data_ptr(), int(sigma.shape[3]) - 1,
-                int(mu_s_l_volume.downscatter_only), src.data_ptr(), out.data_ptr(), G, nsh, N, 1, stream),
+                int(mu_s_l_volume.downscatter_only), src.data_ptr(), out.data_ptr(), G, nsh, N, 1, None, stream),
                 "citrus_scatter_moments_factored")
         else:
             transfer = mu_s_l_volume.contiguous()
@@ -159,20 +159,24 @@ def source_is_cell_major_fn(fn) -> bool:
     return bool(getattr(fn, "cell_major", False))
 
 
-def scatter_moments_views(transfer: MaterialTransfer, phi_views: Tensor, out: Optional[Tensor] = None) -> Tensor:
-    """Batched ``scatter_moments`` of V views sharing one factored transfer: [V, G, R, C, S, nsh, 4] in and out."""
+def scatter_moments_views(transfer: MaterialTransfer, phi_views: Tensor, out: Optional[Tensor] = None,
+                          cell_means: bool = False):
+    """Batched ``scatter_moments`` of V views sharing one factored transfer: [V, G, R, C, S, nsh, 4] in and out.
+    With ``cell_means`` also returns the cell means of the result moment-major, [V, nsh, G, R, C, S] (the operand of
+    the image's multiple-collision expansion, written by the same kernel)."""
     V, G, R, C, S, nsh, four = (int(v) for v in phi_views.shape)
     _hip.require_gpu_tensor(phi_views, "phi_views")
     rho, sigma = transfer.rho.contiguous(), transfer.sigma.contiguous()
     if out is None:
         out = torch.empty_like(phi_views)
+    means = torch.empty(V, nsh, G, R, C, S, device=phi_views.device, dtype=torch.float32) if cell_means else None
     lib = _hip.library()
     with torch.cuda.device(phi_views.device):
         _hip.check(lib.citrus_scatter_moments_factored(
             rho.data_ptr(), int(rho.shape[0]), sigma.data_ptr(), int(sigma.shape[3]) - 1,
             int(transfer.downscatter_only), phi_views.data_ptr(), out.data_ptr(), G, nsh, R * C * S, V,
-            _hip.stream_of(phi_views.device)), "citrus_scatter_moments_factored")
-    return out
+            _hip.ptr(means), _hip.stream_of(phi_views.device)), "citrus_scatter_moments_factored")
+    return (out, means) if cell_means else out
 
 
 def bf16_operand(moments: Tensor) -> tuple[Tensor, Tensor]:

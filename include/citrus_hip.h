@@ -201,10 +201,13 @@ int citrus_scatter_moments_dense(const float* transfer, int Lt, const float* phi
 /* Factored transfer: mu_s_l(cell) = sum_m rho[m][cell] sigma[m][Gi][Go][Lt+1] (mass transfer moments, cm^2/g,
  * times densities, g/cm^3), formed on the fly; the counterpart of transfer_moments_volume
  * (citrus/scatter_solver/cross_sections.py:886-902) fused into the contraction. rho is [M][N] (shared by the V
- * views). downscatter_only = 1 asserts sigma[m][gi][go] == 0 for gi > go, and the upper triangle is skipped. */
+ * views). downscatter_only = 1 asserts sigma[m][gi][go] == 0 for gi > go, and the upper triangle is skipped.
+ * cell_means (or NULL) receives the cell means of T again, moment-major [V][nsh][G][N]: the operand of the
+ * collapsed-cone image's multiple-collision expansion (the moments_by_cell operand, scatter_image.py:1225,1246),
+ * with no separate transposition. */
 int citrus_scatter_moments_factored(const float* rho, int num_materials, const float* sigma, int Lt,
                                     int downscatter_only, const float* phi, float* T, int G, int nsh, int64_t N,
-                                    int V, void* stream);
+                                    int V, float* cell_means, void* stream);
 
 
 /* ---------------------------------------------------------------------------------------------------------------

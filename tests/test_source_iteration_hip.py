@@ -130,3 +130,21 @@ def test_source_iteration_cap_and_thick_cells():
     phi_ref, _, _, _ = osc.source_iteration(ctx.shape, grid.deltas, om, w, Y, mu_t64, mu_s64, fc_all.numpy(),
                                             1e-12, 3)
     assert rel_err(result.phi_total, phi_ref) < 1e-5
+
+
+def test_scatter_moments_also_write_moment_major_cell_means():
+    """``scatter_moments_views(cell_means=True)``: the second output is the first's cell means, moment-major, bit for
+    bit (the same accumulator written twice)."""
+    from citrus_amd.scatter_solver.source_iteration import MaterialTransfer, scatter_moments_views
+
+    dev = gpu_device()
+    gen = torch.Generator().manual_seed(12)
+    V, G, shape, nsh, M, L = 2, 5, (3, 4, 6), 16, 2, 3
+    rho = torch.rand(M, *shape, generator=gen).to(dev)
+    sigma = (torch.rand(M, G, G, L + 1, generator=gen) * torch.triu(torch.ones(G, G))[None, :, :, None]).to(dev)
+    phi = torch.randn(V, G, *shape, nsh, 4, generator=gen).to(dev)
+    transfer = MaterialTransfer(rho, sigma)
+    T = scatter_moments_views(transfer, phi)
+    T2, means = scatter_moments_views(transfer, phi, cell_means=True)
+    assert torch.equal(T, T2)
+    assert torch.equal(means, T[..., 0].permute(0, 5, 1, 2, 3, 4))  # [V, G, R, C, S, nsh] -> [V, nsh, G, R, C, S]
