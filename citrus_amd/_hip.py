@@ -82,6 +82,7 @@ _SIGNATURES = {
     "citrus_persistent_schedule_table": (c_i64, [c_int, ctypes.POINTER(c_int), c_int, c_void, c_void, c_void]),
     "citrus_persistent_sweep_run": (c_int, [c_void, ctypes.POINTER(PersistentArgs), c_void, c_i64, c_void]),
     "citrus_persistent_check": (c_int, [c_void, c_void, ctypes.POINTER(c_int), ctypes.POINTER(c_i64), c_void]),
+    "citrus_sh_expand": (c_int, [c_void, c_int, c_int, c_void, c_i64, c_void, c_void]),
     "citrus_attenuation_to_densities": (c_int, [c_void, c_i64, c_int, c_void, c_void, c_void, c_void, c_void,
                                                 c_void]),
     "citrus_state_pack_workspace_floats": (c_i64, [c_int, c_int, c_int, c_i64]),

@@ -438,6 +438,24 @@ def scatter_image_from_source(source_function: SourceFunction, grid: Grid, geome
     return context.finalize(acc)
 
 
+def sh_expand(Y: Tensor, B: Tensor, out: Optional[Tensor] = None) -> Tensor:
+    """``Y [K, nsh] @ B [nsh, ncol]`` on the HIP kernel ``citrus_sh_expand`` (float32, sums in moment order): the
+    multiple-collision source of a chunk of cone directions from moment-major cell means (scatter_image.py:1246)."""
+    _hip.require_gpu_tensor(Y, "Y")
+    _hip.require_gpu_tensor(B, "B")
+    K, nsh = (int(v) for v in Y.shape)
+    if B.dim() != 2 or int(B.shape[0]) != nsh:
+        raise ValueError(f"B must be [{nsh}, ncol], got {tuple(B.shape)}")
+    ncol = int(B.shape[1])
+    if out is None:
+        out = torch.empty(K, ncol, device=B.device, dtype=torch.float32)
+    lib = _hip.library()
+    with torch.cuda.device(B.device):
+        _hip.check(lib.citrus_sh_expand(Y.data_ptr(), K, nsh, B.data_ptr(), ncol, out.data_ptr(),
+                                        _hip.stream_of(B.device)), "citrus_sh_expand")
+    return out
+
+
 def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, phi_uncollided: Tensor,
                            grid_quadrature: Grid, q: int, scatter_source_moments: Tensor, L: int,
                            geometry: ConeGeometry, angle: float, quadrature: Quadrature,
@@ -514,7 +532,7 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
                                                          directions, sequences[start:stop])
         else:
             tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, directions)
-        multiple = torch.mm(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
+        multiple = sh_expand(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
         emission, sums = _attenuated_emission(
             library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse, multiple, f,
             None if fp is None else fp.heights, None if group_weight is None else group_weight[start:stop], mix)

@@ -153,6 +153,15 @@ int citrus_persistent_check(const CitrusPersistentPlan* plan, const void* worksp
                             int64_t* unfinished, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Spherical-harmonic expansion at a set of directions: C[k][j] = sum_n Y[k][n] B[n][j] (float32, n order), the
+ * multiple-collision source of the collapsed-cone image at a chunk of cone directions from the moment-major cell
+ * means (replaces `torch.mm(harmonics, moments_by_cell)`, citrus/scatter_solver/scatter_image.py:1225,1246).
+ * --------------------------------------------------------------------------------------------------------------- */
+
+/* Y [K][nsh], B [nsh][ncol], C [K][ncol]; K * nsh <= 16384. */
+int citrus_sh_expand(const float* Y, int K, int nsh, const float* B, int64_t ncol, float* C, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Density model: the reference materials' densities that reproduce an attenuation volume X (1/cm at the model's
  * energy), interpolating between the two materials whose nominal attenuations bracket max(X, 0); below the first or
  * above the last, that material alone with density X / m. Replaces attenuation_to_densities

@@ -152,3 +152,20 @@ def test_primary_image_and_spr():
     assert rel_err(out["signal"], sg.sum(0)) < F32_REL
     spr = scatter_fraction(0.3 * out["signal"], out["signal"])
     assert torch.allclose(spr, torch.full_like(spr, 0.3 / 1.3), rtol=1e-6)
+
+
+def test_sh_expand_matches_float64_product():
+    """The image's multiple-collision expansion on the HIP kernel equals the float64 product to float32 rounding
+    (K = 64 directions, P5, ragged column count) and refuses mismatched shapes."""
+    from citrus_amd.scatter_solver.scatter_image import sh_expand
+
+    dev = gpu_device()
+    gen = torch.Generator().manual_seed(44)
+    Y = torch.randn(64, 36, generator=gen, dtype=torch.float64)
+    B = torch.randn(36, 3 * 1000 + 17, generator=gen, dtype=torch.float64)
+    got = sh_expand(Y.to(dev, torch.float32), B.to(dev, torch.float32)).double().cpu()
+    want = Y.float().double() @ B.float().double()
+    scale = (Y.abs().float().double() @ B.abs().float().double())
+    assert float(((got - want).abs() / scale).max()) < 1e-6
+    with pytest.raises(ValueError):
+        sh_expand(Y.to(dev, torch.float32), B[:16].to(dev, torch.float32))
