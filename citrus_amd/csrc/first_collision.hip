@@ -354,9 +354,8 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
 // (point, direction), blockIdx.y = direction; each block also reduces sum_g e and h sum_g e over its points into
 // partial [K][blocks][2] (no atomics), the sums of the footprint's emission-weighted mean height.
 template <int GMAX>
-// 5 waves per SIMD (95 VGPRs, no spills): 2% faster than the unconstrained 4 (112 VGPRs) on the 64^3 view step;
-// 6 or 8 waves spill and are slower
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 5))) void emission_kernel(FcDev d, const float* __restrict__ tau, int tau_channels,
+// 4 waves per SIMD (121 VGPRs with the prefetched multiple-collision loads): forcing 5, 6 or 8 spills and is slower
+__global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* __restrict__ tau, int tau_channels,
                                                           const float* __restrict__ tau_mix,
                                                           const float* __restrict__ mu_c,
                                                           const float* __restrict__ multiple, int f,
@@ -375,6 +374,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 5))) 
     const float ox = d.omegas[3 * k], oy = d.omegas[3 * k + 1], oz = d.omegas[3 * k + 2];
     float ux, uy, uz;
     primary_direction(d, r, c, s, ux, uy, uz);
+    // the cell's multiple-collision emission first: its loads are in flight during the table gathers (the loads
+    // issued after the gathers cost 3.3 of the kernel's 11 ms per 64^3 view)
+    float mc[GMAX];
+#pragma unroll
+    for (int g = 0; g < GMAX; ++g) mc[g] = g < d.G ? multiple[((int64_t)k * d.G + g) * Nc + cell] : 0.0f;
     PointData<GMAX> pd;
     load_point(d, p, pd);
     float S[GMAX];
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 5))) 
         t = tau[kg * Nc + cell];
       }
       if (f > 1) t = fmaxf(t - mu_c[(int64_t)g * Nc + cell] * off, 0.0f);
-      const float e = (S[g] + multiple[kg * Nc + cell]) * expf(-t);
+      const float e = (S[g] + mc[g]) * expf(-t);
       esum += e;
       if (signal_weight) {
         sig = fmaf(signal_weight[kg], e, sig);
