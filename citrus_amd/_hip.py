@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -83,6 +83,8 @@ _SIGNATURES = {
     "citrus_persistent_sweep_run": (c_int, [c_void, ctypes.POINTER(PersistentArgs), c_void, c_i64, c_void]),
     "citrus_persistent_check": (c_int, [c_void, c_void, ctypes.POINTER(c_int), ctypes.POINTER(c_i64), c_void]),
     "citrus_sh_expand": (c_int, [c_void, c_int, c_int, c_void, c_i64, c_void, c_void]),
+    "citrus_multiple_emission": (c_int, [c_void, c_int, c_int, c_void, c_int, c_i64, c_void, c_int, c_void, c_void,
+                                         c_void, c_void]),
     "citrus_attenuation_to_densities": (c_int, [c_void, c_i64, c_int, c_void, c_void, c_void, c_void, c_void,
                                                 c_void]),
     "citrus_state_pack_workspace_floats": (c_i64, [c_int, c_int, c_int, c_i64]),
@@ -136,6 +138,8 @@ _SIGNATURES = {
     "citrus_fc_moments": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_int, c_void, c_void]),
     "citrus_attenuated_emission": (c_int, [ctypes.POINTER(FcArgs), c_void, c_int, c_void, c_void, c_void, c_int,
                                            c_void, c_void, c_void, c_void, c_void]),
+    "citrus_attenuated_emission_collapsed": (c_int, [ctypes.POINTER(FcArgs), c_void, c_int, c_void, c_void, c_void,
+                                                     c_void, c_void, c_void, c_void]),
     "citrus_box_matrix": (c_int, [c_void, c_int, c_int, c_void, c_void]),
 }
 

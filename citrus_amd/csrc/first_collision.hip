@@ -353,7 +353,9 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
 // optical depth tau_j = max(0, tau_cell - mu_cell (w . (r_j - r_cell))) (tau_cell exactly for f = 1). Thread per
 // (point, direction), blockIdx.y = direction; each block also reduces sum_g e and h sum_g e over its points into
 // partial [K][blocks][2] (no atomics), the sums of the footprint's emission-weighted mean height.
-template <int GMAX>
+// COLLAPSED (one point per cell, signal weights): `multiple` is [K][2][cells], the multiple-collision part already
+// attenuated and summed over the groups, weighted (0) and unweighted (1), by citrus_multiple_emission.
+template <int GMAX, bool COLLAPSED>
 // 4 waves per SIMD (121 VGPRs with the prefetched multiple-collision loads): forcing 5, 6 or 8 spills and is slower
 __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* __restrict__ tau, int tau_channels,
                                                           const float* __restrict__ tau_mix,
@@ -376,9 +378,14 @@ __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* 
     primary_direction(d, r, c, s, ux, uy, uz);
     // the cell's multiple-collision emission first: its loads are in flight during the table gathers (the loads
     // issued after the gathers cost 3.3 of the kernel's 11 ms per 64^3 view)
-    float mc[GMAX];
+    float mc[GMAX], mcw = 0.0f, mcs = 0.0f;
+    if constexpr (COLLAPSED) {
+      mcw = multiple[(int64_t)k * 2 * Nc + cell];
+      mcs = multiple[((int64_t)k * 2 + 1) * Nc + cell];
+    } else {
 #pragma unroll
-    for (int g = 0; g < GMAX; ++g) mc[g] = g < d.G ? multiple[((int64_t)k * d.G + g) * Nc + cell] : 0.0f;
+      for (int g = 0; g < GMAX; ++g) mc[g] = g < d.G ? multiple[((int64_t)k * d.G + g) * Nc + cell] : 0.0f;
+    }
     PointData<GMAX> pd;
     load_point(d, p, pd);
     float S[GMAX];
@@ -412,13 +419,17 @@ __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* 
         t = tau[kg * Nc + cell];
       }
       if (f > 1) t = fmaxf(t - mu_c[(int64_t)g * Nc + cell] * off, 0.0f);
-      const float e = (S[g] + mc[g]) * expf(-t);
+      const float e = COLLAPSED ? S[g] * expf(-t) : (S[g] + mc[g]) * expf(-t);
       esum += e;
       if (signal_weight) {
         sig = fmaf(signal_weight[kg], e, sig);
       } else {
         out[kg * d.P + p] = e;
       }
+    }
+    if constexpr (COLLAPSED) {
+      sig += mcw;
+      esum += mcs;
     }
     if (signal_weight) out[(int64_t)k * d.P + p] = sig;
     ehsum = esum * (heights ? heights[p] : 0.0f);
@@ -558,9 +569,32 @@ extern "C" int citrus_attenuated_emission(const CitrusFcArgs* args, const float*
   const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock), (unsigned)d.K), block(kBlock);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
-    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value>), grid, block, 0, s, d, tau, tau_channels, tau_mix,
-                       mu_t_coarse, multiple, factor, heights, signal_weight, out, partial);
+    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value, false>), grid, block, 0, s, d, tau, tau_channels,
+                       tau_mix, mu_t_coarse, multiple, factor, heights, signal_weight, out, partial);
     return 0;
   });
   return finish("attenuated emission");
+}
+
+extern "C" int citrus_attenuated_emission_collapsed(const CitrusFcArgs* args, const float* tau, int tau_channels,
+                                                    const float* tau_mix, const float* collapsed,
+                                                    const float* heights, const float* signal_weight, float* out,
+                                                    float* partial, void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!tau || !collapsed || !signal_weight || !out || !partial)
+    return fail(CITRUS_ERR_ARG, "NULL input or output (the collapsed form needs the signal weights)");
+  if (tau_mix && (tau_channels < 1 || tau_channels > kMaxTauChannels))
+    return fail(CITRUS_ERR_ARG, "1..8 optical-depth channels with a mix table");
+  if (args->K > 65535) return fail(CITRUS_ERR_ARG, "at most 65535 directions per launch");
+  const FcDev d = make_dev(*args);
+  if (d.K == 0) return CITRUS_OK;
+  const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock), (unsigned)d.K), block(kBlock);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dispatch_groups(d.G, [&](auto gm) {
+    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value, true>), grid, block, 0, s, d, tau, tau_channels,
+                       tau_mix, nullptr, collapsed, 1, heights, signal_weight, out, partial);
+    return 0;
+  });
+  return finish("attenuated emission (collapsed)");
 }

@@ -379,9 +379,12 @@ def supervoxel_subpoint_emission(source_quadrature: Tensor, tau_coarse: Tensor, 
 
 
 def _attenuated_emission(library, rho, phi_uncollided, grid_q, geometry, angle, directions, tau, mu_c, multiple, f,
-                         heights, signal_weight, tau_mix=None):
+                         heights, signal_weight, tau_mix=None, collapsed=False):
     """One ``citrus_attenuated_emission`` launch: (emission [k, G or 1, Rq, Cq, Sq], sums [k, 2]). ``tau`` is
-    [k, G, cells], or [k, M, cells] of traced densities whose group optical depths are ``tau_mix [G, M]`` mixes."""
+    [k, G, cells], or [k, M, cells] of traced densities whose group optical depths are ``tau_mix [G, M]`` mixes.
+    ``collapsed``: ``multiple`` is the [k, 2, cells] output of ``multiple_emission`` (f = 1 with signal weights)."""
+    if collapsed and (f != 1 or signal_weight is None):
+        raise ValueError("the collapsed multiple-collision emission needs one point per cell and signal weights")
     a, keep, _ = _fc_args(library, rho, phi_uncollided, grid_q, geometry, angle, directions)
     k, G, P = a.K, library.num_groups, grid_q.num_voxels
     device = phi_uncollided.device
@@ -402,10 +405,16 @@ def _attenuated_emission(library, rho, phi_uncollided, grid_q, geometry, angle, 
     if int(tau.shape[1]) != channels:
         raise ValueError(f"tau has {int(tau.shape[1])} channels, expected {channels}")
     with torch.cuda.device(device):
-        _hip.check(_hip.library().citrus_attenuated_emission(
-            ctypes.byref(a), tau.data_ptr(), channels, _hip.ptr(tau_mix), mu_c.data_ptr(), multiple.data_ptr(),
-            int(f), _hip.ptr(h), _hip.ptr(sw), out.data_ptr(), partial.data_ptr(), _hip.stream_of(device)),
-            "citrus_attenuated_emission")
+        if collapsed:
+            _hip.check(_hip.library().citrus_attenuated_emission_collapsed(
+                ctypes.byref(a), tau.data_ptr(), channels, _hip.ptr(tau_mix), multiple.data_ptr(), _hip.ptr(h),
+                sw.data_ptr(), out.data_ptr(), partial.data_ptr(), _hip.stream_of(device)),
+                "citrus_attenuated_emission_collapsed")
+        else:
+            _hip.check(_hip.library().citrus_attenuated_emission(
+                ctypes.byref(a), tau.data_ptr(), channels, _hip.ptr(tau_mix), mu_c.data_ptr(), multiple.data_ptr(),
+                int(f), _hip.ptr(h), _hip.ptr(sw), out.data_ptr(), partial.data_ptr(), _hip.stream_of(device)),
+                "citrus_attenuated_emission")
     del keep
     return out, partial.sum(1)
 
@@ -439,8 +448,9 @@ def scatter_image_from_source(source_function: SourceFunction, grid: Grid, geome
 
 
 def sh_expand(Y: Tensor, B: Tensor, out: Optional[Tensor] = None) -> Tensor:
-    """``Y [K, nsh] @ B [nsh, ncol]`` on the HIP kernel ``citrus_sh_expand`` (float32, sums in moment order): the
-    multiple-collision source of a chunk of cone directions from moment-major cell means (scatter_image.py:1246)."""
+    """``Y [K, nsh] @ B [nsh, ncol]`` on the HIP kernel ``citrus_sh_expand`` (float32 on the matrix cores, four
+    moments per step): the multiple-collision source of a chunk of cone directions from moment-major cell means
+    (scatter_image.py:1246)."""
     _hip.require_gpu_tensor(Y, "Y")
     _hip.require_gpu_tensor(B, "B")
     K, nsh = (int(v) for v in Y.shape)
@@ -453,6 +463,39 @@ def sh_expand(Y: Tensor, B: Tensor, out: Optional[Tensor] = None) -> Tensor:
     with torch.cuda.device(B.device):
         _hip.check(lib.citrus_sh_expand(Y.data_ptr(), K, nsh, B.data_ptr(), ncol, out.data_ptr(),
                                         _hip.stream_of(B.device)), "citrus_sh_expand")
+    return out
+
+
+def multiple_emission(Y: Tensor, means: Tensor, tau: Tensor, signal_weight: Tensor,
+                      tau_mix: Optional[Tensor] = None) -> Tensor:
+    """[K, 2, N] multiple-collision emission of a direction chunk at one point per cell, attenuated and summed over
+    the groups on the HIP kernel ``citrus_multiple_emission``: ``[k, 0]`` weighted by ``signal_weight [K, G]``,
+    ``[k, 1]`` unweighted. ``Y [K, nsh]`` harmonics, ``means [nsh, G, N]`` moment-major cell means, ``tau`` the
+    chunk's optical depths [K, G, N], or [K, M, N] traced channels (M <= 2) mixed by ``tau_mix [G, M]``
+    (scatter_image.py:1246-1262, uncollided_kernels.py:1138)."""
+    for name, x in (("Y", Y), ("means", means), ("tau", tau), ("signal_weight", signal_weight)):
+        _hip.require_gpu_tensor(x, name)
+    K, nsh = (int(v) for v in Y.shape)
+    if means.dim() != 3 or int(means.shape[0]) != nsh:
+        raise ValueError(f"means must be [{nsh}, G, N], got {tuple(means.shape)}")
+    G, N = int(means.shape[1]), int(means.shape[2])
+    channels = G
+    if tau_mix is not None:
+        _hip.require_gpu_tensor(tau_mix, "tau_mix")
+        channels = int(tau_mix.shape[1])
+        if tuple(tau_mix.shape) != (G, channels) or channels > 2:
+            raise ValueError(f"tau_mix must be [{G}, 1 or 2], got {tuple(tau_mix.shape)}")
+    if tuple(tau.shape) != (K, channels, N):
+        raise ValueError(f"tau must be [{K}, {channels}, {N}], got {tuple(tau.shape)}")
+    if tuple(signal_weight.shape) != (K, G):
+        raise ValueError(f"signal_weight must be [{K}, {G}], got {tuple(signal_weight.shape)}")
+    out = torch.empty(K, 2, N, device=means.device, dtype=torch.float32)
+    lib = _hip.library()
+    with torch.cuda.device(means.device):
+        _hip.check(lib.citrus_multiple_emission(Y.data_ptr(), K, nsh, means.data_ptr(), G, N, tau.data_ptr(),
+                                                channels, _hip.ptr(tau_mix), signal_weight.data_ptr(),
+                                                out.data_ptr(), _hip.stream_of(means.device)),
+                   "citrus_multiple_emission")
     return out
 
 
@@ -532,10 +575,16 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
                                                          directions, sequences[start:stop])
         else:
             tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, directions)
-        multiple = sh_expand(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
+        weight = None if group_weight is None else group_weight[start:stop].to(device, torch.float32).contiguous()
+        collapsed = f == 1 and weight is not None and (mix is None or int(mix.shape[1]) <= 2)
+        if collapsed:  # signal-only at one point per cell: expansion, attenuation and group sum in one pass
+            tau_c = tau.reshape(k, -1, Rc * Cc * Sc).to(device, torch.float32).contiguous()
+            multiple = multiple_emission(harmonics[i].contiguous(), by_cell.view(nsh, G, -1), tau_c, weight, mix)
+        else:
+            multiple = sh_expand(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
         emission, sums = _attenuated_emission(
             library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse, multiple, f,
-            None if fp is None else fp.heights, None if group_weight is None else group_weight[start:stop], mix)
+            None if fp is None else fp.heights, weight, mix, collapsed=collapsed)
         del tau, multiple
         image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, directions)
         del emission

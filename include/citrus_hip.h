@@ -153,13 +153,26 @@ int citrus_persistent_check(const CitrusPersistentPlan* plan, const void* worksp
                             int64_t* unfinished, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
- * Spherical-harmonic expansion at a set of directions: C[k][j] = sum_n Y[k][n] B[n][j] (float32, n order), the
- * multiple-collision source of the collapsed-cone image at a chunk of cone directions from the moment-major cell
- * means (replaces `torch.mm(harmonics, moments_by_cell)`, citrus/scatter_solver/scatter_image.py:1225,1246).
+ * Spherical-harmonic expansion at a set of directions: C[k][j] = sum_n Y[k][n] B[n][j] (float32 on the matrix
+ * cores, four moments per step), the multiple-collision source of the collapsed-cone image at a chunk of cone
+ * directions from the moment-major cell means (replaces `torch.mm(harmonics, moments_by_cell)`,
+ * citrus/scatter_solver/scatter_image.py:1225,1246).
  * --------------------------------------------------------------------------------------------------------------- */
 
-/* Y [K][nsh], B [nsh][ncol], C [K][ncol]; K * nsh <= 16384. */
+/* Y [K][nsh], B [nsh][ncol], C [K][ncol]; nsh = (L+1)^2 with L <= 7; the directions' harmonics, K x nsh padded to
+ * 16-direction tiles, must fit 64 KB. */
 int citrus_sh_expand(const float* Y, int K, int nsh, const float* B, int64_t ncol, float* C, void* stream);
+
+/* The multiple-collision part of the signal-only image's emission at one point per cell, expanded, attenuated and
+ * summed over the groups in one pass (the multiple-collision term of `attenuated_emission` weighted by the signal
+ * response, uncollided_kernels.py:1138, scatter_image.py:1246-1262): with S_g = sum_l Y[k][l] means[l][g][n] and
+ * tau_g the cell's group optical depth (as in citrus_attenuated_emission: tau [K][G][N] with tau_mix NULL, or
+ * [K][tau_channels][N] with tau_mix [G][tau_channels], tau_channels 1 or 2, +inf channel 0 = no path),
+ *   out[k][0][n] = sum_g signal_weight[k][g] exp(-tau_g) S_g,   out[k][1][n] = sum_g exp(-tau_g) S_g.
+ * Y [K][nsh], means [nsh][G][N], signal_weight [K][G], out [K][2][N]; G <= 32. */
+int citrus_multiple_emission(const float* Y, int K, int nsh, const float* means, int G, int64_t N, const float* tau,
+                             int tau_channels, const float* tau_mix, const float* signal_weight, float* out,
+                             void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Density model: the reference materials' densities that reproduce an attenuation volume X (1/cm at the model's
@@ -375,6 +388,13 @@ int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float*
 int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, int tau_channels, const float* tau_mix,
                                const float* mu_t_coarse, const float* multiple, int factor, const float* heights,
                                const float* signal_weight, float* out, float* partial, void* stream);
+
+/* citrus_attenuated_emission at one point per cell (factor 1) with signal weights, the multiple-collision part
+ * taken from citrus_multiple_emission: collapsed [K][2][cells]; out [K][P] = sum_g signal_weight e_g as above and the
+ * same partial sums. */
+int citrus_attenuated_emission_collapsed(const CitrusFcArgs* args, const float* tau, int tau_channels,
+                                         const float* tau_mix, const float* collapsed, const float* heights,
+                                         const float* signal_weight, float* out, float* partial, void* stream);
 
 /* Footprint box-filter matrices out [k][N][N], B[c][j] = |[j, j+1] n W_c| / |W_c|, W_c the window of width
  * widths[m] (pixels) centred on pixel c clipped to [0, N] (box_matrix, scatter_image.py:952). */

@@ -156,16 +156,59 @@ def test_primary_image_and_spr():
 
 def test_sh_expand_matches_float64_product():
     """The image's multiple-collision expansion on the HIP kernel equals the float64 product to float32 rounding
-    (K = 64 directions, P5, ragged column count) and refuses mismatched shapes."""
+    (sums of nsh products, four per matrix-core step) for P5 and P3, whole and partial direction tiles, column counts
+    with and without the 16-byte stores, and refuses mismatched shapes."""
     from citrus_amd.scatter_solver.scatter_image import sh_expand
 
     dev = gpu_device()
     gen = torch.Generator().manual_seed(44)
-    Y = torch.randn(64, 36, generator=gen, dtype=torch.float64)
-    B = torch.randn(36, 3 * 1000 + 17, generator=gen, dtype=torch.float64)
-    got = sh_expand(Y.to(dev, torch.float32), B.to(dev, torch.float32)).double().cpu()
-    want = Y.float().double() @ B.float().double()
-    scale = (Y.abs().float().double() @ B.abs().float().double())
-    assert float(((got - want).abs() / scale).max()) < 1e-6
+    for K, nsh, ncol in ((64, 36, 3 * 1000 + 17), (64, 36, 4096 + 256), (37, 16, 999), (5, 9, 64), (1, 1, 3)):
+        Y = torch.randn(K, nsh, generator=gen, dtype=torch.float64)
+        B = torch.randn(nsh, ncol, generator=gen, dtype=torch.float64)
+        got = sh_expand(Y.to(dev, torch.float32), B.to(dev, torch.float32)).double().cpu()
+        want = Y.float().double() @ B.float().double()
+        scale = (Y.abs().float().double() @ B.abs().float().double())
+        assert float(((got - want).abs() / scale).max()) < 1e-6, (K, nsh, ncol)
+    Y = torch.randn(64, 36, device=dev)
     with pytest.raises(ValueError):
-        sh_expand(Y.to(dev, torch.float32), B[:16].to(dev, torch.float32))
+        sh_expand(Y, torch.randn(16, 100, device=dev))
+    with pytest.raises(ValueError):
+        sh_expand(Y, torch.randn(100, 36, device=dev).t())
+
+
+def test_multiple_emission_matches_float64():
+    """The collapsed multiple-collision emission (expansion, attenuation, group sums in one pass) equals the float64
+    evaluation of sum_g w exp(-tau_g) Y.means_g and sum_g exp(-tau_g) Y.means_g: per-group optical depths and one or
+    two traced channels with a mix table (+inf channel 0 = no path), direction counts across a 64-direction
+    workgroup and cell counts without the 16-byte accesses."""
+    from citrus_amd.scatter_solver.scatter_image import multiple_emission
+
+    dev = gpu_device()
+    gen = torch.Generator().manual_seed(45)
+    for K, nsh, G, N, channels in ((64, 36, 16, 4096, 2), (70, 16, 5, 1001, 0), (3, 9, 3, 64, 1), (17, 36, 32, 130, 2)):
+        Y = torch.randn(K, nsh, generator=gen, dtype=torch.float64)
+        means = torch.randn(nsh, G, N, generator=gen, dtype=torch.float64)
+        w = torch.rand(K, G, generator=gen, dtype=torch.float64)
+        if channels:
+            tau = 3.0 * torch.rand(K, channels, N, generator=gen, dtype=torch.float64)
+            tau[:, 0, ::7] = math.inf
+            mix = torch.rand(G, channels, generator=gen, dtype=torch.float64)
+            mix[0, -1] = 0.0  # 0 x inf in the mix
+            tg = torch.einsum("gc,kcn->kgn", mix.float().double(), tau.float().double())
+            tg[:, :, ::7] = math.inf
+        else:
+            tau = 3.0 * torch.rand(K, G, N, generator=gen, dtype=torch.float64)
+            mix = None
+            tg = tau.float().double()
+        s = torch.einsum("kl,lgn->kgn", Y.float().double(), means.float().double())
+        e = torch.exp(-tg) * s
+        want = torch.stack([torch.einsum("kg,kgn->kn", w.float().double(), e), e.sum(1)], 1)
+        s_abs = torch.einsum("kl,lgn->kgn", Y.float().double().abs(), means.float().double().abs())
+        scale = torch.stack([torch.einsum("kg,kgn->kn", w.float().double(), torch.exp(-tg) * s_abs),
+                             (torch.exp(-tg) * s_abs).sum(1)], 1) + 1e-30
+        f32 = lambda t: t.to(dev, torch.float32).contiguous()
+        got = multiple_emission(f32(Y), f32(means), f32(tau), f32(w), None if mix is None else f32(mix))
+        err = float(((got.double().cpu() - want).abs() / scale).max())
+        assert err < 2e-6, (K, nsh, G, N, channels, err)
+    with pytest.raises(ValueError):
+        multiple_emission(f32(Y), f32(means), f32(tau[:, :1]), f32(w), f32(mix))
