@@ -109,7 +109,7 @@ constexpr int kDirTiles = 4;    // 64 directions per workgroup
 constexpr int kMaxGroups = 32;
 
 template <int NSH, int TC>
-__global__ __launch_bounds__(256) void multiple_emission_kernel(const float* __restrict__ Y, int K,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void multiple_emission_kernel(const float* __restrict__ Y, int K,
                                                                 const float* __restrict__ means, int G, int64_t N,
                                                                 const float* __restrict__ tau,
                                                                 const float* __restrict__ tau_mix,
@@ -163,23 +163,36 @@ __global__ __launch_bounds__(256) void multiple_emission_kernel(const float* __r
         tc[dt][c] = dir < K ? load4(tau + ((int64_t)dir * TC + c) * N) : f4{0.f, 0.f, 0.f, 0.f};
     }
   }
-  float a[KS], an[KS];
+  // A-operand loads: wave-uniform row base (moment 4 s, group g) plus the lane's 32-bit byte offset (moment 4 s + q,
+  // cell ca), clamped into the array. A clamped cell only feeds result row ca, which is not stored; a clamped moment
+  // (nsh not a multiple of 4) meets a zero harmonic but is zeroed as well, in case the data holds an infinity.
+  const int64_t cac = ca < N ? ca : N - 1;
+  uint32_t off[KS];
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
-    const int n = 4 * s + q;
-    a[s] = (n < NSH && ca < N) ? means[(int64_t)n * G * N + ca] : 0.0f;
+    const int n = 4 * s + q < NSH ? 4 * s + q : NSH - 1;
+    off[s] = (uint32_t)((((int64_t)(n - 4 * s) * G) * N + cac) * (int64_t)sizeof(float));
   }
+  auto load_a = [&](int g, float (&dst)[KS]) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const char* row = reinterpret_cast<const char*>(means + ((int64_t)4 * s * G + g) * N);
+      const float v = *reinterpret_cast<const float*>(row + off[s]);
+      if constexpr (NSH % 4 == 0)
+        dst[s] = v;
+      else
+        dst[s] = 4 * s + q < NSH ? v : 0.0f;
+    }
+  };
+  float a[KS], an[KS];
+  load_a(0, a);
   f4 sw[kDirTiles], ss[kDirTiles];
 #pragma unroll
   for (int dt = 0; dt < kDirTiles; ++dt) sw[dt] = ss[dt] = f4{0.f, 0.f, 0.f, 0.f};
   __syncthreads();
   for (int g = 0; g < G; ++g) {
     // the next group's moments are in flight during this group's products
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int n = 4 * s + q;
-      an[s] = (g + 1 < G && n < NSH && ca < N) ? means[((int64_t)n * G + g + 1) * N + ca] : 0.0f;
-    }
+    load_a(g + 1 < G ? g + 1 : g, an);
 #pragma unroll
     for (int dt = 0; dt < kDirTiles; ++dt) {
       const int dl = dt * 16 + j;
