@@ -112,6 +112,10 @@ __device__ __forceinline__ void load_point(const FcDev& d, int p, PointData<GMAX
   for (int g = 0; g < GMAX; ++g) pd.ph[g] = g < d.G ? d.phi_uc[(int64_t)g * d.P + p] : 0.0f;
 }
 
+// acc += cv (a + w1 (b - a)): one table term, linear in eta between two nodes (one expression, so that the per-channel
+// and the all-channel forms contract it alike)
+__device__ __forceinline__ f4 table_term(f4 acc, float cv, f4 a, f4 b, float w1) { return acc + cv * (a + w1 * (b - a)); }
+
 // S[go] at one eta; entries go < g' of the table are zero (downscatter only)
 template <int GMAX>
 __device__ __forceinline__ void eval_source(const FcDev& d, const PointData<GMAX>& pd, float eta, float S[GMAX]) {
@@ -138,11 +142,7 @@ __device__ __forceinline__ void eval_source(const FcDev& d, const PointData<GMAX
       const f4* r0 = reinterpret_cast<const f4*>(t0 + row * GMAX);
       const f4* r1 = reinterpret_cast<const f4*>(t0 + node + row * GMAX);
 #pragma unroll
-      for (int ch = gi / 4; ch < NCH; ++ch) {
-        const f4 a = r0[ch];
-        const f4 b = r1[ch];
-        acc[ch] += cv * (a + w1 * (b - a));
-      }
+      for (int ch = gi / 4; ch < NCH; ++ch) acc[ch] = table_term(acc[ch], cv, r0[ch], r1[ch], w1);
     }
   }
 #pragma unroll
@@ -152,6 +152,32 @@ __device__ __forceinline__ void eval_source(const FcDev& d, const PointData<GMAX
     S[4 * ch + 2] = acc[ch].z;
     S[4 * ch + 3] = acc[ch].w;
   }
+}
+
+// S of the four output groups 4 ch .. 4 ch + 3 at one eta: eval_source restricted to one channel (the same terms in
+// the same order, so the values are equal bit for bit)
+template <int GMAX>
+__device__ __forceinline__ f4 eval_source_channel(const FcDev& d, const PointData<GMAX>& pd, float eta, int ch) {
+  int i0;
+  float w1;
+  table_position(d, eta, i0, w1);
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int64_t node = (int64_t)d.M * d.G * GMAX;
+  for (int s = 0; s < d.n_slots; ++s) {
+    const float rho = d.slot_density[(int64_t)s * d.P + pd.p];
+    if (rho == 0.0f) continue;  // as in eval_source
+    const float* t0 = d.table + (int64_t)i0 * node + (int64_t)d.slot_material[(int64_t)s * d.P + pd.p] * d.G * GMAX;
+#pragma unroll
+    for (int gi = 0; gi < GMAX; ++gi) {
+      if (gi / 4 > ch) break;
+      const float cv = rho * pd.ph[gi];
+      const int row = gi < d.G ? gi : d.G - 1;
+      const f4* r0 = reinterpret_cast<const f4*>(t0 + row * GMAX);
+      const f4* r1 = reinterpret_cast<const f4*>(t0 + node + row * GMAX);
+      acc = table_term(acc, cv, r0[ch], r1[ch], w1);
+    }
+  }
+  return acc;
 }
 
 template <int GMAX>
@@ -173,77 +199,84 @@ __global__ __launch_bounds__(kBlock) void fc_source_kernel(FcDev d, float* __res
     if (g < d.G) out[((int64_t)k * d.G + g) * d.P + p] = S[g];
 }
 
+// One thread per (point, output channel of four groups): the channel is uniform per workgroup (the heaviest, which
+// gathers the most table rows, first) and each thread runs all K directions for its four groups. Splitting the
+// groups over threads keeps the accumulators of a thread at 4 groups (about 70 VGPRs instead of 225), so 4x as many
+// points' gathers are in flight. Per group the terms and their order are those of the all-group form.
 template <int GMAX>
 __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* __restrict__ weights,
                                                          const float* __restrict__ mu_s_l, int lp1,
-                                                         float* __restrict__ out, float* __restrict__ samples) {
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= d.P) return;
+                                                         float* __restrict__ out, float* __restrict__ samples,
+                                                         int blocks_per_channel) {
+  constexpr int NCH = GMAX / 4;
+  const int ch = NCH - 1 - (int)(blockIdx.x / blocks_per_channel);
+  const int p = (int)(blockIdx.x % blocks_per_channel) * kBlock + threadIdx.x;
+  if (p >= d.P || 4 * ch >= d.G) return;
   const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
   float ux, uy, uz;
   primary_direction(d, r, c, s, ux, uy, uz);
   PointData<GMAX> pd;
   load_point(d, p, pd);
-  float I0[GMAX], I1[GMAX], I2[GMAX];
-#pragma unroll
-  for (int g = 0; g < GMAX; ++g) I0[g] = I1[g] = I2[g] = 0.0f;
+  f4 I0 = {0.f, 0.f, 0.f, 0.f}, I1 = I0, I2 = I0;
   // directions in runs of four: with samples and K % 4 == 0 each group's four samples leave as one 16-byte store
   const bool vec = samples && (d.K % 4) == 0;
   for (int k0 = 0; k0 < d.K; k0 += 4) {
-    f4 run[GMAX];
+    f4 run[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = k0 + j;
       if (k >= d.K) break;
       const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
-      float S[GMAX];
-      eval_source(d, pd, eta, S);
+      const f4 S = eval_source_channel(d, pd, eta, ch);
       const float w = weights[k];
       const float we = w * eta;
       const float wee = we * eta;
 #pragma unroll
-      for (int g = 0; g < GMAX; ++g) {
-        I0[g] = fmaf(w, S[g], I0[g]);
-        I1[g] = fmaf(we, S[g], I1[g]);
-        I2[g] = fmaf(wee, S[g], I2[g]);
-        run[g][j] = S[g];
-        if (samples && !vec && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[g];
+      for (int gl = 0; gl < 4; ++gl) {
+        I0[gl] = fmaf(w, S[gl], I0[gl]);
+        I1[gl] = fmaf(we, S[gl], I1[gl]);
+        I2[gl] = fmaf(wee, S[gl], I2[gl]);
+        run[gl][j] = S[gl];
+        const int g = 4 * ch + gl;
+        if (samples && !vec && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[gl];
       }
     }
     if (vec) {
 #pragma unroll
-      for (int g = 0; g < GMAX; ++g)
-        if (g < d.G) *reinterpret_cast<f4*>(samples + ((int64_t)g * d.P + p) * d.K + k0) = run[g];
+      for (int gl = 0; gl < 4; ++gl)
+        if (4 * ch + gl < d.G) *reinterpret_cast<f4*>(samples + ((int64_t)(4 * ch + gl) * d.P + p) * d.K + k0) = run[gl];
     }
   }
   // exact Legendre moments l = 0, 1 of the same source (no quadrature)
-  float E0[GMAX], E1[GMAX];
-#pragma unroll
-  for (int g = 0; g < GMAX; ++g) E0[g] = E1[g] = 0.0f;
+  f4 E0 = {0.f, 0.f, 0.f, 0.f}, E1 = E0;
   for (int sl = 0; sl < d.n_slots; ++sl) {
     const float rho = d.slot_density[(int64_t)sl * d.P + p];
     const float* ms = mu_s_l + (int64_t)d.slot_material[(int64_t)sl * d.P + p] * d.G * d.G * lp1;
 #pragma unroll
     for (int gi = 0; gi < GMAX; ++gi) {
+      if (gi / 4 > ch) break;
       const float cv = rho * pd.ph[gi];  // 0 for g' >= G
 #pragma unroll
-      for (int go = gi; go < GMAX; ++go) {
+      for (int gl = 0; gl < 4; ++gl) {
+        const int go = 4 * ch + gl;
+        if (go < gi) continue;
         const int a = (gi < d.G ? gi : d.G - 1) * d.G + (go < d.G ? go : d.G - 1);
-        E0[go] = fmaf(cv, ms[a * lp1 + 0], E0[go]);
-        E1[go] = fmaf(cv, ms[a * lp1 + 1], E1[go]);
+        E0[gl] = fmaf(cv, ms[a * lp1 + 0], E0[gl]);
+        E1[gl] = fmaf(cv, ms[a * lp1 + 1], E1[gl]);
       }
     }
   }
   const int64_t plane = (int64_t)d.G * d.P;
 #pragma unroll
-  for (int g = 0; g < GMAX; ++g) {
+  for (int gl = 0; gl < 4; ++gl) {
+    const int g = 4 * ch + gl;
     if (g >= d.G) continue;
     const int64_t o = (int64_t)g * d.P + p;
-    out[o] = I0[g];
-    out[plane + o] = I1[g];
-    out[2 * plane + o] = I2[g];
-    out[3 * plane + o] = E0[g];
-    out[4 * plane + o] = E1[g];
+    out[o] = I0[gl];
+    out[plane + o] = I1[gl];
+    out[2 * plane + o] = I2[gl];
+    out[3 * plane + o] = E0[gl];
+    out[4 * plane + o] = E1[gl];
   }
 }
 
@@ -504,11 +537,12 @@ extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, co
   if (!weights || !mu_s_l || !out) return fail(CITRUS_ERR_ARG, "NULL weights, moments or output");
   if (lp1 < 2) return fail(CITRUS_ERR_ARG, "the exact moments need Legendre order >= 1");
   const FcDev d = make_dev(*args);
-  const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock)), block(kBlock);
+  const int blocks = (d.P + kBlock - 1) / kBlock;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
-    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), grid, block, 0, s, d, weights, mu_s_l, lp1, out,
-                       samples);
+    constexpr int NCH = decltype(gm)::value / 4;
+    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0, s, d,
+                       weights, mu_s_l, lp1, out, samples, blocks);
     return 0;
   });
   return finish("first-collision sums");
