@@ -5,7 +5,8 @@
 // per-ray geometry ever goes through HBM. The segment clipping, entry voxel and plane parameters are computed and
 // compared in float64 with one IEEE operation per step (contraction is off in this file), which makes the voxel
 // sequence identical to a float64 CPU traversal written with the same operations; the field values are summed in
-// float32 with Kahan compensation, so long rays keep float32 relative accuracy. Fields are channel-last
+// float32 with Kahan compensation (the shared-sequence downstream kernel adds its exact float32 products in float64),
+// so long rays keep float32 relative accuracy. Fields are channel-last
 // [rows][cols][slices][channels] so the channels of a voxel are one contiguous read.
 //
 // Traversal rules (citrus/scatter_solver/kernels/ray_tracing_kernels.py:178-327): slab clip to [a_lo, a_hi] within
@@ -493,20 +494,17 @@ __global__ __launch_bounds__(256) void downstream_shared_kernel(Box B, Panel P, 
   const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
   const double tp = fmax(panel_distance(P, p0, w), 0.0) * wn;  // cm to the panel
   const float* base = vol + i * nch;
-  float sum[CH], comp[CH];
+  // FP64 sums: a float32 value times a float32 length is exact in FP64, so each term is added with one rounding
+  // (2 vector operations per channel and crossing against 5 for a compensated float32 sum, at the same FP64 rate)
+  double sum[CH];
 #pragma unroll
-  for (int c = 0; c < CH; ++c) { sum[c] = 0.0f; comp[c] = 0.0f; }
+  for (int c = 0; c < CH; ++c) sum[c] = 0.0;
   auto add = [&](int s, float len) {
     const float* v = base + (int64_t)flat[s] * nch;
+    const double l = (double)len;
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-      if (c < nch) {  // Kahan-compensated sum
-        const float yv = v[c] * len - comp[c];
-        const float tv = sum[c] + yv;
-        comp[c] = (tv - sum[c]) - yv;
-        sum[c] = tv;
-      }
-    }
+    for (int c = 0; c < CH; ++c)
+      if (c < nch) sum[c] = fma((double)v[c], l, sum[c]);
   };
   if ((double)prefix[n_in] * (1.0 + 1e-5) + 1e-6 < tp) {
     // the panel lies beyond the grid exit: every in-grid sample counts in full
@@ -529,13 +527,13 @@ __global__ __launch_bounds__(256) void downstream_shared_kernel(Box B, Panel P, 
       float acc = 0.0f;
 #pragma unroll
       for (int c = 0; c < CH; ++c)
-        if (c < nch) acc = fmaf(mix[g * nch + c], sum[c], acc);
+        if (c < nch) acc = fmaf(mix[g * nch + c], (float)sum[c], acc);
       o[(int64_t)g * N] = acc;
     }
   } else {
 #pragma unroll
     for (int c = 0; c < CH; ++c)
-      if (c < nch) o[(int64_t)c * N] = sum[c];
+      if (c < nch) o[(int64_t)c * N] = (float)sum[c];
   }
 }
 
