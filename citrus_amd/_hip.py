@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 8
+ABI_VERSION = 9
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -141,6 +141,8 @@ _SIGNATURES = {
     "citrus_attenuated_emission_collapsed": (c_int, [ctypes.POINTER(FcArgs), c_void, c_int, c_void, c_void, c_void,
                                                      c_void, c_void, c_void, c_void]),
     "citrus_box_matrix": (c_int, [c_void, c_int, c_int, c_void, c_void]),
+    "citrus_footprint_filters": (c_int, [c_void, c_int, c_int, ctypes.c_float, c_void, ctypes.c_float, ctypes.c_float,
+                                         c_int, c_int, c_void, c_void, c_void, c_void]),
 }
 
 

@@ -324,6 +324,8 @@ class _Footprint:
             cached = (scale, heights, 0.5 * (min(hs) + max(hs)))
             context._cache[key] = cached
         self.scale, self.heights, self.center_height = cached
+        # the same float32 scales on the host, for the filter kernel's arguments (no device read)
+        self.scale_host = (1.0 / geometry.delta_det_channel, 1.0 / geometry.delta_det_row)
 
     def mean_height_from_sums(self, sums: Tensor) -> Tensor:
         """[k] mean heights from the [k, 2] sums (sum e, sum e h); the box-centre height where the emission vanishes."""
@@ -344,6 +346,34 @@ class _Footprint:
     def blur(self, image: Tensor, chunk_index: int, mean_heights: Tensor) -> Tensor:
         w = self.widths(chunk_index, mean_heights)
         return box_blur_rows_cols(image, w[:, 0], w[:, 1])
+
+    def filters(self, chunk_index: int, partial: Tensor) -> tuple:
+        """(transposed channel filters [k, Cd, Cd], row filters [k, Rd, Rd]) of a chunk straight from the emission
+        kernel's per-block sums ``partial [k, blocks, 2]``: ``mean_height_from_sums``, ``widths`` and ``box_matrix``
+        in one ``citrus_footprint_filters`` call."""
+        start, stop = self.context.chunk_bounds[chunk_index]
+        k, nb = int(partial.shape[0]), int(partial.shape[1])
+        cw = self.context.cell_width[start:stop].contiguous()
+        _hip.require_gpu_tensor(partial, "partial")
+        _hip.require_gpu_tensor(cw, "cell_width")
+        Rd, Cd = self.context.image_shape
+        dev = partial.device
+        heights = torch.empty(k, device=dev, dtype=torch.float32)
+        bct = torch.empty(k, Cd, Cd, device=dev, dtype=torch.float32)
+        br = torch.empty(k, Rd, Rd, device=dev, dtype=torch.float32)
+        with torch.cuda.device(dev):
+            _hip.check(_hip.library().citrus_footprint_filters(
+                partial.data_ptr(), k, nb, float(self.center_height), cw.data_ptr(), self.scale_host[0],
+                self.scale_host[1], Cd, Rd, heights.data_ptr(), bct.data_ptr(), br.data_ptr(),
+                _hip.stream_of(dev)), "citrus_footprint_filters")
+        return bct, br
+
+    @staticmethod
+    def blur_with(image: Tensor, channel_filters_t: Tensor, row_filters: Tensor) -> Tensor:
+        """``box_blur_rows_cols`` with the filters of ``filters``."""
+        k, G, Rd, Cd = image.shape
+        x = torch.bmm(image.reshape(k, G * Rd, Cd), channel_filters_t).view(k, G, Rd, Cd)
+        return torch.matmul(row_filters[:, None], x.contiguous())
 
 
 # ---- emission ---------------------------------------------------------------------------------------------------
@@ -380,7 +410,8 @@ def supervoxel_subpoint_emission(source_quadrature: Tensor, tau_coarse: Tensor, 
 
 def _attenuated_emission(library, rho, phi_uncollided, grid_q, geometry, angle, directions, tau, mu_c, multiple, f,
                          heights, signal_weight, tau_mix=None, collapsed=False):
-    """One ``citrus_attenuated_emission`` launch: (emission [k, G or 1, Rq, Cq, Sq], sums [k, 2]). ``tau`` is
+    """One ``citrus_attenuated_emission`` launch: (emission [k, G or 1, Rq, Cq, Sq], per-block sums
+    [k, blocks, 2] of (sum_g e, h sum_g e), to be summed over the blocks). ``tau`` is
     [k, G, cells], or [k, M, cells] of traced densities whose group optical depths are ``tau_mix [G, M]`` mixes.
     ``collapsed``: ``multiple`` is the [k, 2, cells] output of ``multiple_emission`` (f = 1 with signal weights)."""
     if collapsed and (f != 1 or signal_weight is None):
@@ -416,7 +447,7 @@ def _attenuated_emission(library, rho, phi_uncollided, grid_q, geometry, angle, 
                 int(f), _hip.ptr(h), _hip.ptr(sw), out.data_ptr(), partial.data_ptr(), _hip.stream_of(device)),
                 "citrus_attenuated_emission")
     del keep
-    return out, partial.sum(1)
+    return out, partial
 
 
 # ---- images -----------------------------------------------------------------------------------------------------
@@ -582,14 +613,14 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
             multiple = multiple_emission(harmonics[i].contiguous(), by_cell.view(nsh, G, -1), tau_c, weight, mix)
         else:
             multiple = sh_expand(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
-        emission, sums = _attenuated_emission(
+        emission, partial = _attenuated_emission(
             library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse, multiple, f,
             None if fp is None else fp.heights, weight, mix, collapsed=collapsed)
         del tau, multiple
         image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, directions)
         del emission
         if fp is not None:
-            image = fp.blur(image, i, fp.mean_height_from_sums(sums))
+            image = fp.blur_with(image, *fp.filters(i, partial))
         if group_weight is not None and context.planar_weight is None:
             acc.add_(image.sum(dim=(0, 1)))
         elif group_weight is not None:  # curved panel, angle-independent detector: w cos(theta_c) per channel

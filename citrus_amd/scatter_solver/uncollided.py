@@ -124,6 +124,9 @@ def _fc_tables(library: MultigroupLibrary, device: torch.device):
     return tables
 
 
+_SLOT_MATERIALS: dict = {}
+
+
 def _material_slots(rho: Tensor):
     """(slot material int32 [n][P], slot density float32 [n][P]) with n = the largest count of nonzero materials at
     one point (at least 1, at most 4): every material when M <= 4, else each point's nonzero ones (the reference's
@@ -131,7 +134,11 @@ def _material_slots(rho: Tensor):
     M = int(rho.shape[0])
     flat = rho.reshape(M, -1).to(torch.float32)
     if M <= 4:
-        mats = torch.arange(M, device=rho.device, dtype=torch.int32)[:, None].expand(M, flat.shape[1]).contiguous()
+        key = (M, int(flat.shape[1]), str(rho.device))
+        mats = _SLOT_MATERIALS.get(key)
+        if mats is None:  # every material in its own slot: the same table for every call on this grid size
+            mats = torch.arange(M, device=rho.device, dtype=torch.int32)[:, None].expand(M, flat.shape[1]).contiguous()
+            _SLOT_MATERIALS[key] = mats
         return mats, flat.contiguous()
     n = int((flat != 0).sum(0).max())
     if n > 4:

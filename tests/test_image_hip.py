@@ -212,3 +212,34 @@ def test_multiple_emission_matches_float64():
         assert err < 2e-6, (K, nsh, G, N, channels, err)
     with pytest.raises(ValueError):
         multiple_emission(f32(Y), f32(means), f32(tau[:, :1]), f32(w), f32(mix))
+
+
+def test_footprint_filters_match_the_torch_footprint():
+    """citrus_footprint_filters (mean heights from the emission kernel's per-block sums, widths, both box filters)
+    equals the torch form mean_height_from_sums -> widths -> box_matrix to float32 rounding, including a direction
+    whose emission vanishes (the box-centre height)."""
+    from citrus_amd.scatter_solver.scatter_image import _Footprint, box_matrix
+
+    dev = gpu_device()
+    library, coarse, gq, rho, phi, T, mu_c, geometry = _image_problem(True, 1)
+    angle = 0.35
+    quad = panel_direction_set(geometry, angle, (3, 6), max_polar_angle=gq)
+    ctx = make_direction_context(quad, geometry, angle, energy_weighted_response(), no_grid(), library, dev,
+                                 torch.float32, chunk=5)
+    fp = _Footprint(gq, geometry, angle, ctx)
+    gen = torch.Generator().manual_seed(46)
+    k = ctx.chunk_bounds[1][1] - ctx.chunk_bounds[1][0]
+    partial = torch.rand(k, 37, 2, generator=gen)
+    partial[:, :, 1] *= 10.0
+    partial[0] = 0.0  # no emission: the box-centre height
+    partial = partial.to(dev)
+    bct, br = fp.filters(1, partial)
+    w = fp.widths(1, fp.mean_height_from_sums(partial.sum(1)))
+    Rd, Cd = ctx.image_shape
+    want_c = box_matrix(Cd, w[:, 0]).transpose(1, 2)
+    want_r = box_matrix(Rd, w[:, 1])
+    assert float((bct - want_c).abs().max()) < 1e-5
+    assert float((br - want_r).abs().max()) < 1e-5
+    image = torch.rand(k, 1, Rd, Cd, generator=gen).to(dev)
+    from citrus_amd.scatter_solver.scatter_image import box_blur_rows_cols
+    assert rel_err(fp.blur_with(image, bct, br), box_blur_rows_cols(image, w[:, 0], w[:, 1])) < F32_REL
