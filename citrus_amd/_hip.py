@@ -142,7 +142,7 @@ _SIGNATURES = {
                                                      c_void, c_void, c_void, c_void]),
     "citrus_box_matrix": (c_int, [c_void, c_int, c_int, c_void, c_void]),
     "citrus_footprint_filters": (c_int, [c_void, c_int, c_int, ctypes.c_float, c_void, ctypes.c_float, ctypes.c_float,
-                                         c_int, c_int, c_void, c_void, c_void, c_void]),
+                                         c_int, c_int, c_void, c_void, c_void, c_void, c_void]),
 }
 
 

@@ -67,11 +67,12 @@ __global__ __launch_bounds__(256) void footprint_heights_kernel(const float* __r
 }
 
 // Both footprint filters of a chunk from the mean heights: widths (h cell_width[k][a]) scale_a in pixels; the
-// channel filter transposed, channel_t[k][j][c] = B_c[c][j] (the right operand of the channel blur), the row filter
-// rows[k][c][j] = B_r[c][j].
+// channel filter transposed, channel_t[k][j][c] = B_c[c][j] w[k][c] (the right operand of the channel blur, its
+// columns scaled by the optional channel weights w), the row filter rows[k][c][j] = B_r[c][j].
 __global__ __launch_bounds__(256) void footprint_filters_kernel(const float* __restrict__ heights,
                                                                 const float* __restrict__ cell_width, float sc,
                                                                 float sr, int K, int Nc, int Nr,
+                                                                const float* __restrict__ channel_weight,
                                                                 float* __restrict__ channel_t,
                                                                 float* __restrict__ rows) {
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(256) void footprint_filters_kernel(const float* __r
     const int c = (int)(idx % Nc);            // column of the transposed matrix: the window's pixel
     const int j = (int)((idx / Nc) % Nc);     // row: the pixel covered
     const int m = (int)(idx / ((int64_t)Nc * Nc));
-    channel_t[idx] = box_value(heights[m] * cell_width[2 * m] * sc, Nc, c, j);
+    const float b = box_value(heights[m] * cell_width[2 * m] * sc, Nc, c, j);
+    channel_t[idx] = channel_weight ? b * channel_weight[(int64_t)m * Nc + c] : b;
   } else if (idx < nc + nr) {
     const int64_t i = idx - nc;
     const int j = (int)(i % Nr);
@@ -108,7 +110,8 @@ extern "C" int citrus_box_matrix(const float* widths, int k, int N, float* out, 
 
 extern "C" int citrus_footprint_filters(const float* partial, int K, int nblocks, float center_height,
                                         const float* cell_width, float scale_channel, float scale_row, int Nc, int Nr,
-                                        float* heights, float* channel_filter_t, float* row_filter, void* stream) {
+                                        const float* channel_weight, float* heights,
+                                        float* channel_filter_t, float* row_filter, void* stream) {
   using namespace citrus;
   if (!partial || !cell_width || !heights || !channel_filter_t || !row_filter)
     return fail(CITRUS_ERR_ARG, "NULL input or output");
@@ -121,7 +124,8 @@ extern "C" int citrus_footprint_filters(const float* partial, int K, int nblocks
   hipLaunchKernelGGL(footprint_heights_kernel, dim3((unsigned)K), dim3(256), 0, s, partial, nblocks, center_height,
                      heights);
   hipLaunchKernelGGL(footprint_filters_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, heights,
-                     cell_width, scale_channel, scale_row, K, Nc, Nr, channel_filter_t, row_filter);
+                     cell_width, scale_channel, scale_row, K, Nc, Nr, channel_weight, channel_filter_t,
+                     row_filter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("footprint filters: ") + hipGetErrorString(e));
   return CITRUS_OK;

@@ -347,16 +347,21 @@ class _Footprint:
         w = self.widths(chunk_index, mean_heights)
         return box_blur_rows_cols(image, w[:, 0], w[:, 1])
 
-    def filters(self, chunk_index: int, partial: Tensor) -> tuple:
+    def filters(self, chunk_index: int, partial: Tensor, channel_weight: Optional[Tensor] = None) -> tuple:
         """(transposed channel filters [k, Cd, Cd], row filters [k, Rd, Rd]) of a chunk straight from the emission
         kernel's per-block sums ``partial [k, blocks, 2]``: ``mean_height_from_sums``, ``widths`` and ``box_matrix``
-        in one ``citrus_footprint_filters`` call."""
+        in one ``citrus_footprint_filters`` call. ``channel_weight [k, Cd]`` scales the blurred images' channels
+        (folded into the channel filters)."""
         start, stop = self.context.chunk_bounds[chunk_index]
         k, nb = int(partial.shape[0]), int(partial.shape[1])
         cw = self.context.cell_width[start:stop].contiguous()
         _hip.require_gpu_tensor(partial, "partial")
         _hip.require_gpu_tensor(cw, "cell_width")
         Rd, Cd = self.context.image_shape
+        if channel_weight is not None:
+            _hip.require_gpu_tensor(channel_weight, "channel_weight")
+            if tuple(channel_weight.shape) != (k, Cd):
+                raise ValueError(f"channel_weight must be [{k}, {Cd}], got {tuple(channel_weight.shape)}")
         dev = partial.device
         heights = torch.empty(k, device=dev, dtype=torch.float32)
         bct = torch.empty(k, Cd, Cd, device=dev, dtype=torch.float32)
@@ -364,9 +369,23 @@ class _Footprint:
         with torch.cuda.device(dev):
             _hip.check(_hip.library().citrus_footprint_filters(
                 partial.data_ptr(), k, nb, float(self.center_height), cw.data_ptr(), self.scale_host[0],
-                self.scale_host[1], Cd, Rd, heights.data_ptr(), bct.data_ptr(), br.data_ptr(),
-                _hip.stream_of(dev)), "citrus_footprint_filters")
+                self.scale_host[1], Cd, Rd, _hip.ptr(channel_weight), heights.data_ptr(), bct.data_ptr(),
+                br.data_ptr(), _hip.stream_of(dev)), "citrus_footprint_filters")
         return bct, br
+
+    def accumulate_blurred(self, acc: Tensor, image: Tensor, channel_filters_t: Tensor, row_filters: Tensor) -> None:
+        """acc [Rd, Cd] += sum_k (row blur of (channel blur of image_k)) for single-channel images [k, 1, Rd, Cd]:
+        two batched products and one accumulating product with a ones vector (the sum over k)."""
+        k, G, Rd, Cd = image.shape
+        if G != 1:
+            raise ValueError("accumulate_blurred takes single-channel images")
+        x = torch.bmm(image.reshape(k, Rd, Cd), channel_filters_t)
+        y = torch.bmm(row_filters, x)
+        key = ("ones", k, str(acc.device))
+        ones = self.context._cache.get(key)
+        if ones is None:
+            ones = self.context._cache[key] = torch.ones(1, k, device=acc.device, dtype=torch.float32)
+        acc.view(1, Rd * Cd).addmm_(ones, y.view(k, Rd * Cd))
 
     @staticmethod
     def blur_with(image: Tensor, channel_filters_t: Tensor, row_filters: Tensor) -> Tensor:
@@ -619,6 +638,11 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
         del tau, multiple
         image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, directions)
         del emission
+        if fp is not None and group_weight is not None:  # blur, channel weights and the sum over k in two products
+            pw = None if context.planar_weight is None else context.planar_weight[start:stop]
+            fp.accumulate_blurred(acc, image, *fp.filters(i, partial, pw))
+            del image
+            continue
         if fp is not None:
             image = fp.blur_with(image, *fp.filters(i, partial))
         if group_weight is not None and context.planar_weight is None:

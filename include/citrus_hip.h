@@ -403,10 +403,11 @@ int citrus_box_matrix(const float* widths, int k, int N, float* out, void* strea
 /* The finite-cone footprint filters of one direction chunk straight from citrus_attenuated_emission's partial sums
  * [K][nblocks][2] (_Footprint of scatter_image.py:1033-1060 with box_matrix, :952): heights[k] = sum e h / sum e
  * (center_height where sum e is 0), widths (heights[k] cell_width[k][a]) scale_a pixels (a = 0 channels, 1 rows),
- * channel_filter_t [K][Nc][Nc] the transposed channel matrices B^T, row_filter [K][Nr][Nr] the row matrices B. */
+ * channel_filter_t [K][Nc][Nc] the transposed channel matrices B^T, each column c scaled by channel_weight[k][c]
+ * (NULL: 1), row_filter [K][Nr][Nr] the row matrices B. heights [K] receives the mean heights. */
 int citrus_footprint_filters(const float* partial, int K, int nblocks, float center_height, const float* cell_width,
-                             float scale_channel, float scale_row, int Nc, int Nr, float* heights,
-                             float* channel_filter_t, float* row_filter, void* stream);
+                             float scale_channel, float scale_row, int Nc, int Nr, const float* channel_weight,
+                             float* heights, float* channel_filter_t, float* row_filter, void* stream);
 
 /* bfloat16 operand of the scattering-source contraction (configs with the mixed-precision contraction of
  * scattering_kernels.py:162 / transport_sweep.py:524-528): T [V][G][N][nsh][4] float32 -> T16 [V][G][N][4][KP]

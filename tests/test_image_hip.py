@@ -242,4 +242,12 @@ def test_footprint_filters_match_the_torch_footprint():
     assert float((br - want_r).abs().max()) < 1e-5
     image = torch.rand(k, 1, Rd, Cd, generator=gen).to(dev)
     from citrus_amd.scatter_solver.scatter_image import box_blur_rows_cols
-    assert rel_err(fp.blur_with(image, bct, br), box_blur_rows_cols(image, w[:, 0], w[:, 1])) < F32_REL
+    blurred = box_blur_rows_cols(image, w[:, 0], w[:, 1])
+    assert rel_err(fp.blur_with(image, bct, br), blurred) < F32_REL
+    # channel weights folded into the channel filters; the sum over the chunk's images accumulated in place
+    pw = torch.rand(k, Cd, generator=gen).to(dev)
+    acc = torch.ones(Rd, Cd, device=dev)
+    fp.accumulate_blurred(acc, image, *fp.filters(1, partial, pw))
+    assert rel_err(acc, 1.0 + torch.einsum("kgrc,kc->rc", blurred, pw)) < F32_REL
+    with pytest.raises(ValueError):
+        fp.filters(1, partial, pw[:, 1:].contiguous())
