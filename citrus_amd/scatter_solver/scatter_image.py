@@ -35,7 +35,7 @@ from .cross_sections import MultigroupLibrary, group_transmission, mu_t_volume
 from .detector import DetectorResponse, GridResponse, detector_angle
 from .geometry import ConeGeometry, Grid
 from .source import SourceModel
-from .uncollided import _fc_args, present_materials
+from .uncollided import _device_index, _fc_args, present_materials
 
 SourceFunction = Callable[[Tensor], Tensor]
 
@@ -712,16 +712,55 @@ def primary_from_path_lengths(library: MultigroupLibrary, path: Tensor, material
         raise ValueError(f"path must be [M', {Rd} n, {Cd} n] (n >= 1), got {tuple(path.shape)}")
     fpath = fmu = None
     if source.has_filter:
-        sub = primary_subpixel_geometry(geometry, n)
-        us, vs = sub.pixel_uv(device, torch.float64)
-        thick = source.device_tables(geometry, device)[1].double()
-        fpath = (_bilinear_at(thick, us[None, :].expand(Rd * n, Cd * n), vs[:, None].expand(Rd * n, Cd * n),
-                              geometry) * source.filter_density).float()
+        fpath = _cached(("filter_path", n, str(device)), (source, geometry),
+                        lambda: _filter_path(source, geometry, n, device))
         fmu = source.filter_attenuation_table
     T = group_transmission(library, path.to(torch.float32).contiguous(), materials=materials, filter_path=fpath,
                            filter_attenuation=fmu)  # [G, Rd n, Cd n]
     if n > 1:
         T = T.reshape(-1, Rd, n, Cd, n).mean(dim=(2, 4))
+    inv_r2, cos0, grid_t, resp = _cached(("primary_view", float(angle), str(device)),
+                                         (library, geometry, source, detector, grid_response),
+                                         lambda: _primary_view_factors(library, geometry, angle, source, detector,
+                                                                       grid_response, device))
+    fluence = inv_r2[None] * T
+    planar = fluence * cos0
+    signal = planar * grid_t * resp
+    return {"fluence": fluence, "planar_fluence": planar, "signal_per_group": signal, "signal": signal.sum(0)}
+
+
+# per-view factors that depend only on the geometry, the angle and the models: recomputing them each step costs
+# about a hundred small launches per view; a bounded cache keeps the last views' factors (8 MB per view at
+# 16 groups x 256 x 256 pixels)
+_VIEW_CACHE: "dict" = {}
+_VIEW_CACHE_LIMIT = 128
+
+
+def _cached(key, owners: tuple, make):
+    hit = _VIEW_CACHE.get(key)
+    if hit is not None and len(hit[0]) == len(owners) and all(a is b for a, b in zip(hit[0], owners)):
+        return hit[1]
+    value = make()
+    if len(_VIEW_CACHE) >= _VIEW_CACHE_LIMIT:
+        _VIEW_CACHE.pop(next(iter(_VIEW_CACHE)))
+    _VIEW_CACHE[key] = (owners, value)
+    return value
+
+
+def _filter_path(source: SourceModel, geometry: ConeGeometry, n: int, device) -> Tensor:
+    """[Rd n, Cd n] filter (bowtie) path rho_f t(u, v) at the sub-pixel centres, float32."""
+    Rd, Cd = geometry.num_det_rows, geometry.num_det_channels
+    sub = primary_subpixel_geometry(geometry, n)
+    us, vs = sub.pixel_uv(device, torch.float64)
+    thick = source.device_tables(geometry, device)[1].double()
+    return (_bilinear_at(thick, us[None, :].expand(Rd * n, Cd * n), vs[:, None].expand(Rd * n, Cd * n),
+                         geometry) * source.filter_density).float()
+
+
+def _primary_view_factors(library, geometry, angle, source, detector, grid_response, device) -> tuple:
+    """(b / r^2 [Rd, Cd], cos theta0 [Rd, Cd], grid transmission [G, Rd, Cd], detector response [G, Rd, Cd]) at the
+    pixel centres of one view, float32 from float64 forms."""
+    Rd, Cd = geometry.num_det_rows, geometry.num_det_channels
     pixels = geometry.pixel_centers(angle, device, torch.float64)
     r0 = torch.tensor(geometry.source_position(angle), device=device, dtype=torch.float64)
     d = pixels - r0
@@ -729,15 +768,12 @@ def primary_from_path_lengths(library: MultigroupLibrary, path: Tensor, material
     b = torch.ones_like(r2)
     if source.emission_profile is not None:
         b = source.emission_profile.to(device, torch.float64).reshape(Rd, Cd)
-    fluence = (b / r2).float()[None] * T
     omega_p = d / r2.sqrt()[..., None]
     theta0 = detector_angle(omega_p, geometry, angle, pixels)
-    planar = fluence * torch.cos(theta0).float()
     energy = library.group_energy.to(device, torch.float64)
-    resp = detector.at(energy, theta0).float()
-    grid_t = grid_response.transmission_at(energy, omega_p, pixels, geometry, angle).float()
-    signal = planar * grid_t * resp
-    return {"fluence": fluence, "planar_fluence": planar, "signal_per_group": signal, "signal": signal.sum(0)}
+    return ((b / r2).float(), torch.cos(theta0).float(),
+            grid_response.transmission_at(energy, omega_p, pixels, geometry, angle).float(),
+            detector.at(energy, theta0).float())
 
 
 def primary_image(library: MultigroupLibrary, rho: Tensor, grid: Grid, geometry: ConeGeometry, angle: float,
@@ -751,7 +787,7 @@ def primary_image(library: MultigroupLibrary, rho: Tensor, grid: Grid, geometry:
     if present == list(range(library.num_materials)) and rho.is_contiguous():
         sel = rho
     else:
-        sel = rho.index_select(0, torch.tensor(present, device=rho.device)).contiguous()
+        sel = rho.index_select(0, _device_index(present, rho.device)).contiguous()
     path = primary_path_lengths(projectors, sel, grid, geometry, angle, samples_per_pixel_axis)
     return primary_from_path_lengths(library, path, present, geometry, angle, source, detector, grid_response)
 

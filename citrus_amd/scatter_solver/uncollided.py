@@ -82,7 +82,7 @@ def uncollided_fluence(library: MultigroupLibrary, rho: Tensor, grid: Grid, geom
                                                                         else [int(m) for m in materials])
     if present is not None and not present:  # vacuum: trace one all-zero map (zero path, no attenuation)
         present = [0]
-    traced = rho if present is None else rho.index_select(0, torch.tensor(present, device=rho.device))
+    traced = rho if present is None else rho.index_select(0, _device_index(present, rho.device))
     paths = uncollided_path_lengths(projectors, traced.contiguous(), grid, geometry, angle, point_grid)
     return uncollided_from_path_lengths(library, paths, present, point_grid, geometry, angle, source)
 
@@ -125,6 +125,16 @@ def _fc_tables(library: MultigroupLibrary, device: torch.device):
 
 
 _SLOT_MATERIALS: dict = {}
+_INDEX: dict = {}
+
+
+def _device_index(values, device) -> Tensor:
+    """A small int64 index tensor on the device, kept per (values, device): no host-to-device copy per call."""
+    key = (tuple(int(v) for v in values), str(device))
+    hit = _INDEX.get(key)
+    if hit is None:
+        hit = _INDEX[key] = torch.tensor(key[0], device=device, dtype=torch.int64)
+    return hit
 
 
 def _material_slots(rho: Tensor):
