@@ -202,17 +202,22 @@ def _finish_views(solver: ScatterSolver, batch: _ViewBatch, collided: Tensor, pr
     # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same; its
     # cell means also moment-major, the image's expansion operand
     T, means = scatter_moments_views(fields.transfer_coarse, collided, cell_means=True)
-    results, new_states = [], []
+    results, news, norms = [], [], []
     for v, (st, a) in enumerate(zip(batch.states, batch.angles)):
         new = collided[v].permute(0, 4, 5, 1, 2, 3)
         image = solver.scatter_image(fields, a, batch.phi_uc[v], T[v].permute(0, 4, 5, 1, 2, 3)[:, :, 0],
                                      batch.cones[v], signal_only=True, source_means=means[v])["signal"]
         primary_signal = solver.primary(fields, a)["signal"] if primary else None
         results.append((image, primary_signal))
-        new_states.append(CitrusState(angle=float(a), collided_moments=new, cone_quadrature=batch.cones[v],
-                                      num_steps=st.num_steps + 1, history=st.history + [_change(new, st)],
-                                      scatter_source=T[v], scatter_source_fields=fields, _batch=(T, v)))
+        news.append(new)
+        norms.append(_change_norms(new, st))
     del means
+    # one host synchronization for the step's accumulator changes, after every view's work is queued
+    changes = [_change_from_norms(nd, st) for nd, st in zip(torch.stack(norms).tolist(), batch.states)]
+    new_states = [CitrusState(angle=float(a), collided_moments=new, cone_quadrature=batch.cones[v],
+                              num_steps=st.num_steps + 1, history=st.history + [c], scatter_source=T[v],
+                              scatter_source_fields=fields, _batch=(T, v))
+                  for v, (st, a, new, c) in enumerate(zip(batch.states, batch.angles, news, changes))]
     return results, new_states
 
 
@@ -295,15 +300,26 @@ def citrus_cropped_views_step(solver: ScatterSolver, states: Sequence[CitrusStat
     return results, new_states, slabs
 
 
+def _change_norms(new: Tensor, state: CitrusState) -> Tensor:
+    """[2] device tensor (|new|, |new - previous|) of the scalar-flux cell means, float64 (no synchronization)."""
+    x = new[:, 0, 0].double()
+    if state.collided_moments is None:
+        return torch.stack([torch.linalg.vector_norm(x), torch.zeros((), device=x.device, dtype=torch.float64)])
+    prev = state.collided_moments[:, 0, 0].to(x)
+    return torch.stack([torch.linalg.vector_norm(x), torch.linalg.vector_norm(x - prev)])
+
+
+def _change_from_norms(nd, state: CitrusState) -> float:
+    """Relative change of the scalar-flux cell means over one step (1 from an empty state with nonzero fluence)."""
+    n, d = nd
+    if state.collided_moments is None:
+        return 1.0 if n > 0 else 0.0
+    return d / n if n > 0 else 0.0
+
+
 def _change(new: Tensor, state: CitrusState) -> float:
     """Relative change of the scalar-flux cell means over one step (1 from an empty state with nonzero fluence)."""
-    if state.collided_moments is None:
-        n = float(torch.linalg.vector_norm(new[:, 0, 0].double()))
-        return 1.0 if n > 0 else 0.0
-    prev = state.collided_moments[:, 0, 0].to(new)
-    n, d = torch.stack([torch.linalg.vector_norm(new[:, 0, 0].double()),
-                        torch.linalg.vector_norm((new[:, 0, 0] - prev).double())]).tolist()
-    return d / n if n > 0 else 0.0
+    return _change_from_norms(_change_norms(new, state).tolist(), state)
 
 
 def citrus_step(solver: ScatterSolver, state: CitrusState, rho, grid: Grid, angle: float,
