@@ -208,6 +208,25 @@ def transfer_moments_volume(library: MultigroupLibrary, rho: Tensor) -> Tensor:
     return torch.einsum("mabl,mijk->ablijk", library.mu_s_l.to(rho.device, rho.dtype), rho)
 
 
+_FINE_TABLES: dict = {}
+
+
+def fine_device_tables(library: "MultigroupLibrary", device, materials=None) -> tuple:
+    """(fine mu_t [M', F] float32, fine group int32 [F], fine weight float32 [F]) of the library on the device for
+    the materials listed (all when None), cached per library, materials and device: no host-to-device copy per
+    call."""
+    mats = None if materials is None else tuple(int(m) for m in materials)
+    key = (id(library), mats, str(device))
+    hit = _FINE_TABLES.get(key)
+    if hit is not None and hit[0] is library:
+        return hit[1]
+    fine_mu = library.fine_mu_t if mats is None else library.fine_mu_t[list(mats)]
+    tables = (fine_mu.to(device, torch.float32).contiguous(), library.fine_group.to(device, torch.int32).contiguous(),
+              library.fine_weight.to(device, torch.float32).contiguous())
+    _FINE_TABLES[key] = (library, tables)
+    return tables
+
+
 def group_transmission(library: MultigroupLibrary, path: Tensor, materials=None, filter_path: Optional[Tensor] = None,
                        filter_attenuation: Optional[Tensor] = None) -> Tensor:
     """[G, ...] group transmission from material line integrals ``path [M', ...]`` in g/cm^2 on the HIP kernel
@@ -219,15 +238,12 @@ def group_transmission(library: MultigroupLibrary, path: Tensor, materials=None,
     device = path.device
     Mp = int(path.shape[0])
     spatial = tuple(path.shape[1:])
-    fine_mu = library.fine_mu_t if materials is None else library.fine_mu_t[[int(m) for m in materials]]
-    if int(fine_mu.shape[0]) != Mp:
-        raise ValueError(f"path has {Mp} materials, the library {int(fine_mu.shape[0])}")
+    mu, grp, w = fine_device_tables(library, device, materials)
+    if int(mu.shape[0]) != Mp:
+        raise ValueError(f"path has {Mp} materials, the library {int(mu.shape[0])}")
     paths = path.reshape(Mp, -1).contiguous()
     _hip.require_gpu_tensor(paths, "path")
     nrays = int(paths.shape[1])
-    mu = fine_mu.to(device, torch.float32).contiguous()
-    grp = library.fine_group.to(device, torch.int32).contiguous()
-    w = library.fine_weight.to(device, torch.float32).contiguous()
     fp = fmu = None
     if filter_path is not None:
         fp = filter_path.reshape(-1).to(device, torch.float32).contiguous()

@@ -597,7 +597,8 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
         if tuple(density_coarse.shape) != (library.num_materials,) + tuple(grid_coarse.shape):
             raise ValueError(f"density_coarse must be [{library.num_materials}, *{grid_coarse.shape}], got "
                              f"{tuple(density_coarse.shape)}")
-        mix = library.mu_t.to(device, torch.float32).t().contiguous()  # [G, M]
+        mix = _cached(("mu_t_mix", str(device)), (library,),
+                      lambda: library.mu_t.to(device, torch.float32).t().contiguous())  # [G, M]
     harmonics = context.spherical_harmonics(L)
     if mix is not None:
         key = ("crossing_sequences", tuple(grid_coarse.shape), tuple(grid_coarse.deltas))

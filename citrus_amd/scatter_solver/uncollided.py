@@ -18,7 +18,7 @@ import torch
 from torch import Tensor
 
 from .. import _hip
-from .cross_sections import MultigroupLibrary
+from .cross_sections import MultigroupLibrary, fine_device_tables
 from .geometry import ConeGeometry, Grid
 from .ray_tracing import HipProjectors
 from .source import SourceModel
@@ -45,14 +45,11 @@ def uncollided_from_path_lengths(library: MultigroupLibrary, path_lengths: Tenso
     """[G, *points.shape] uncollided fluence from the upstream line integrals (uncollided.py:160)."""
     device = path_lengths.device
     Mp = int(path_lengths.shape[0])
-    fine_mu = library.fine_mu_t if materials is None else library.fine_mu_t[[int(m) for m in materials]]
-    if int(fine_mu.shape[0]) != Mp:
-        raise ValueError(f"path_lengths has {Mp} materials, expected {int(fine_mu.shape[0])}")
+    mu, grp, w = fine_device_tables(library, device, materials)
+    if int(mu.shape[0]) != Mp:
+        raise ValueError(f"path_lengths has {Mp} materials, expected {int(mu.shape[0])}")
     paths = path_lengths.reshape(Mp, -1).contiguous()
     _hip.require_gpu_tensor(paths, "path_lengths")
-    mu = fine_mu.to(device, torch.float32).contiguous()
-    grp = library.fine_group.to(device, torch.int32).contiguous()
-    w = library.fine_weight.to(device, torch.float32).contiguous()
     profile, thickness, mu_f = source.device_tables(geometry, device)
     values, ints, table = source.host_block(geometry, angle)
     x0, y0, z0 = points.lower_corner()
@@ -96,7 +93,7 @@ def primary_directions_to_voxels(grid: Grid, geometry: ConeGeometry, angle: floa
     """[rows, cols, slices, 3] unit vectors ``(r - r0)/|r - r0|`` from the focal spot to the voxel centres
     (uncollided.py:74-96), formed in float64."""
     r = grid.center_coordinates(device, torch.float64)
-    r0 = torch.tensor(geometry.source_position(angle), device=device, dtype=torch.float64)
+    r0 = _device_vector(geometry.source_position(angle), device, torch.float64)
     d = r - r0
     return (d / torch.linalg.norm(d, dim=-1, keepdim=True)).to(dtype)
 
@@ -126,6 +123,20 @@ def _fc_tables(library: MultigroupLibrary, device: torch.device):
 
 _SLOT_MATERIALS: dict = {}
 _INDEX: dict = {}
+
+
+_VECTORS: dict = {}
+
+
+def _device_vector(values, device, dtype) -> Tensor:
+    """A small constant vector on the device, kept per (values, device, dtype) in a bounded table."""
+    key = (tuple(float(v) for v in values), str(device), dtype)
+    hit = _VECTORS.get(key)
+    if hit is None:
+        if len(_VECTORS) >= 4096:
+            _VECTORS.clear()
+        hit = _VECTORS[key] = torch.tensor(key[0], device=device, dtype=dtype)
+    return hit
 
 
 def _device_index(values, device) -> Tensor:
