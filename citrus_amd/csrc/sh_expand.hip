@@ -278,6 +278,9 @@ extern "C" int citrus_multiple_emission(const float* Y, int K, int nsh, const fl
   if (tau_mix && (tau_channels < 1 || tau_channels > 2))
     return fail(CITRUS_ERR_UNSUPPORTED, "1 or 2 traced channels with a mix table");
   if ((N + 63) / 64 > 0x7fffffffLL || (K + 63) / 64 > 65535) return fail(CITRUS_ERR_ARG, "too many cells or directions");
+  // the lanes' moment offsets within four moment rows are 32-bit byte offsets
+  if ((3 * (int64_t)G + 1) * N * (int64_t)sizeof(float) > 0xffffffffLL)
+    return fail(CITRUS_ERR_UNSUPPORTED, "(3 G + 1) N x 4 bytes must stay below 4 GiB");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipError_t e;
   if (!tau_mix)

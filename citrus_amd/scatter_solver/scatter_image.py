@@ -627,7 +627,8 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
         else:
             tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, directions)
         weight = None if group_weight is None else group_weight[start:stop].to(device, torch.float32).contiguous()
-        collapsed = f == 1 and weight is not None and (mix is None or int(mix.shape[1]) <= 2)
+        collapsed = (f == 1 and weight is not None and (mix is None or int(mix.shape[1]) <= 2)
+                     and (3 * G + 1) * Rc * Cc * Sc * 4 < 2 ** 32)
         if collapsed:  # signal-only at one point per cell: expansion, attenuation and group sum in one pass
             tau_c = tau.reshape(k, -1, Rc * Cc * Sc).to(device, torch.float32).contiguous()
             multiple = multiple_emission(harmonics[i].contiguous(), by_cell.view(nsh, G, -1), tau_c, weight, mix)

@@ -169,7 +169,7 @@ int citrus_sh_expand(const float* Y, int K, int nsh, const float* B, int64_t nco
  * tau_g the cell's group optical depth (as in citrus_attenuated_emission: tau [K][G][N] with tau_mix NULL, or
  * [K][tau_channels][N] with tau_mix [G][tau_channels], tau_channels 1 or 2, +inf channel 0 = no path),
  *   out[k][0][n] = sum_g signal_weight[k][g] exp(-tau_g) S_g,   out[k][1][n] = sum_g exp(-tau_g) S_g.
- * Y [K][nsh], means [nsh][G][N], signal_weight [K][G], out [K][2][N]; G <= 32. */
+ * Y [K][nsh], means [nsh][G][N], signal_weight [K][G], out [K][2][N]; G <= 32 and (3 G + 1) N x 4 bytes < 4 GiB. */
 int citrus_multiple_emission(const float* Y, int K, int nsh, const float* means, int G, int64_t N, const float* tau,
                              int tau_channels, const float* tau_mix, const float* signal_weight, float* out,
                              void* stream);
