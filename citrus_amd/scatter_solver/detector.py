@@ -4,8 +4,10 @@ Keeps the interface of ``citrus/scatter_solver/detector.py``: ``DetectorResponse
 keV deposited per incident photon, bilinear in energy and angle, clamped to the end nodes, |theta| used),
 ``ideal_counting_response`` (:294, D = 1), ``energy_weighted_response`` (:305, D = E), an absorber response with the
 oblique path ``t / cos(theta)`` like ``csi_response`` (:258; the absorber's mass attenuation is passed in, since the
-photon database is not available offline), ``GridResponse`` with ``no_grid`` (:657, :758) and ``detector_angle``
-(:1054). Septa models of real anti-scatter grids are outside the transport hot path.
+photon database is not available offline), ``GridResponse`` (:657: trilinear in energy and the grid angles, focused
+or unfocused, septa along the channels or the rows, crossed septa) with ``no_grid`` (:758), ``grid_angles`` (:1077),
+``default_focal_distance`` (:1039), the direct transmission of a 1-D grid and ``focused_1d_grid`` (:779, the direct
+part of grid_model.py) and ``detector_angle`` (:1054).
 """
 
 from __future__ import annotations
@@ -94,36 +96,82 @@ def absorber_response(thickness_cm: float, density: float, mass_attenuation: Cal
     return DetectorResponse(torch.as_tensor(en), torch.as_tensor(th), torch.as_tensor(resp), name=name)
 
 
+SEPTA_DIRECTIONS = ("channels", "rows")
+
+
 @dataclass(frozen=True, eq=False)
 class GridResponse:
-    """Anti-scatter-grid transmission table [E, T1, T2] on energy and grid-angle nodes (detector.py:657); only the
-    angle-independent table of ``no_grid`` is evaluated by this build."""
+    """Anti-scatter-grid transmission table G(E, theta1, theta2) (detector.py:657): trilinear in energy and the
+    grid angles of ``grid_angles``, clamped to the end nodes, |theta| used. ``focal_distance_cm`` sets the focusing
+    direction (None: ``default_focal_distance``; ``math.inf``: unfocused); ``septa_along`` is "channels" (lamellae
+    along u, repeating along the rows) or "rows"; ``crossed`` is a second set of septa in the other direction, whose
+    transmission multiplies this one."""
 
     energy_kev: Tensor
     theta1_rad: Tensor
     theta2_rad: Tensor
     transmission: Tensor
     name: str = "grid"
+    focal_distance_cm: Optional[float] = None
+    septa_along: str = "channels"
+    crossed: Optional["GridResponse"] = None
+
+    def __post_init__(self) -> None:
+        shape = (self.energy_kev.numel(), self.theta1_rad.numel(), self.theta2_rad.numel())
+        if tuple(self.transmission.shape) != shape:
+            raise ValueError(f"transmission must be [E, T1, T2] = {list(shape)}, got {tuple(self.transmission.shape)}")
+        if self.septa_along not in SEPTA_DIRECTIONS:
+            raise ValueError(f"septa_along must be one of {SEPTA_DIRECTIONS}, got {self.septa_along!r}")
+        if self.crossed is not None and (self.crossed.septa_along == self.septa_along
+                                         or self.crossed.crossed is not None):
+            raise ValueError("crossed septa must run in the other direction and have no crossed septa of their own")
 
     @property
     def depends_on_angles(self) -> bool:
         t = self.transmission
-        return not bool((t == t[:, :1, :1]).all())
+        return self.crossed is not None or not bool((t == t[:, :1, :1]).all())
 
     def at_energy(self, group_energy: Tensor, device, dtype) -> Tensor:
         """[G] transmission of an angle-independent grid at the group energies."""
         if self.depends_on_angles:
-            raise NotImplementedError("septa whose transmission depends on the angles are not supported")
+            raise NotImplementedError("this grid's transmission depends on the photon's angles: use transmission_at")
         en = self.energy_kev.to(device, dtype)
         table = self.transmission[:, 0, 0].to(device, dtype)
         i0, i1, w = _bracket(en, group_energy.to(device, dtype).reshape(-1))
         return table[i0] + (table[i1] - table[i0]) * w
 
+    def at(self, group_energy: Tensor, theta1: Tensor, theta2: Tensor) -> Tensor:
+        """[G, ...] table interpolated trilinearly at the group energies and the grid angles (|theta| used),
+        on the device and in the dtype of ``theta1`` (detector.py:729)."""
+        device, dtype = theta1.device, theta1.dtype
+        theta1, theta2 = torch.broadcast_tensors(theta1, theta2.to(device, dtype))
+        en, t1, t2 = (x.to(device, dtype) for x in (self.energy_kev, self.theta1_rad, self.theta2_rad))
+        table = self.transmission.to(device, dtype)
+        ie0, ie1, we = _bracket(en, group_energy.to(device, dtype).reshape(-1))
+        at_e = table[ie0] + (table[ie1] - table[ie0]) * we[:, None, None]  # [G, T1, T2]
+        i10, i11, w1 = _bracket(t1, theta1.abs())
+        i20, i21, w2 = _bracket(t2, theta2.abs())
+        lo = at_e[:, i10, i20] + (at_e[:, i10, i21] - at_e[:, i10, i20]) * w2
+        hi = at_e[:, i11, i20] + (at_e[:, i11, i21] - at_e[:, i11, i20]) * w2
+        return lo + (hi - lo) * w1
+
     def transmission_at(self, group_energy: Tensor, omega: Tensor, pixel_positions: Tensor, geometry,
                         angle: float) -> Tensor:
-        """[G, ...pixels] transmission of photons along ``omega`` at the pixels (angle-independent grids)."""
-        t = self.at_energy(group_energy, pixel_positions.device, pixel_positions.dtype)
-        return t.reshape(-1, *([1] * (pixel_positions.dim() - 1))).expand(-1, *pixel_positions.shape[:-1])
+        """[G, ...] transmission of photons along ``omega`` arriving at the pixels (detector.py:704): the table at
+        their grid angles, times the crossed septa's."""
+        if not self.depends_on_angles:
+            t = self.at_energy(group_energy, pixel_positions.device, pixel_positions.dtype)
+            if omega.dim() == 2 and pixel_positions.dim() == 3:  # the shapes of grid_angles
+                shape = (int(omega.shape[0]),) + tuple(pixel_positions.shape[:-1])
+            else:
+                shape = tuple(torch.broadcast_shapes(omega.shape[:-1], pixel_positions.shape[:-1]))
+            return t.reshape(-1, *([1] * len(shape))).expand(-1, *shape)
+        theta1, theta2 = grid_angles(omega, pixel_positions, geometry, angle, self.focal_distance_cm,
+                                     self.septa_along)
+        out = self.at(group_energy, theta1, theta2)
+        if self.crossed is not None:
+            out = out * self.crossed.transmission_at(group_energy, omega, pixel_positions, geometry, angle)
+        return out
 
 
 def no_grid() -> GridResponse:
@@ -131,6 +179,127 @@ def no_grid() -> GridResponse:
     e = torch.tensor([1.0, 1000.0], dtype=torch.float64)
     t = torch.tensor([0.0, math.pi / 2], dtype=torch.float64)
     return GridResponse(e, t, t.clone(), torch.ones(2, 2, 2, dtype=torch.float64), name="no grid")
+
+
+def default_focal_distance(geometry) -> float:
+    """Focal distance of a grid built for a geometry (detector.py:1039): the source-detector distance of a cone
+    beam, infinite (unfocused) for a parallel beam."""
+    return math.inf if geometry.is_parallel else float(geometry.source_detector_dist)
+
+
+def grid_angles(omega: Tensor, pixel_positions: Tensor, geometry, angle: float,
+                focal_distance: Optional[float] = None, septa_along: str = "channels") -> tuple[Tensor, Tensor]:
+    """(theta1, theta2) grid angles of photons along ``omega`` arriving at pixels (detector.py:1077).
+
+    With the panel axes (u, v, n) at the pixel (n toward the source; on a curved panel the surface's axes at each
+    pixel) and a = -omega: theta2 = atan2(a.u, a.n) and theta1 = atan2(a.v, a.n) - atan2(f.v, f.n), f the direction
+    from the pixel to the focal line (f = n for an unfocused grid, theta1 then measured from the normal). Septa along
+    the channels focus on the line parallel to u at ``focal_distance`` along the normal from the panel centre (on a
+    curved panel: from the central row of each column); septa along the rows swap u and v and focus along each
+    pixel's normal. ``omega [K, 3]`` with ``pixel_positions [rows, channels, 3]`` gives [K, rows, channels]."""
+    if septa_along not in SEPTA_DIRECTIONS:
+        raise ValueError(f"septa_along must be one of {SEPTA_DIRECTIONS}, got {septa_along!r}")
+    device, dtype = pixel_positions.device, pixel_positions.dtype
+    omega = omega.to(device, dtype)
+    fd = default_focal_distance(geometry) if focal_distance is None else float(focal_distance)
+    if omega.dim() == 2 and pixel_positions.dim() == 3:
+        a, pixels = -omega[:, None, None, :], pixel_positions[None]
+    else:
+        a, pixels = -omega, pixel_positions
+    pc = torch.tensor(geometry.panel_center(angle), device=device, dtype=dtype)
+    if geometry.use_curved_detector:
+        u_hat, v_hat, n_hat = geometry.local_panel_axes(pixels.expand(*torch.broadcast_shapes(a.shape, pixels.shape)),
+                                                        angle)
+    else:
+        u_hat, v_hat, n_hat = (torch.tensor(x, device=device, dtype=dtype) for x in geometry.panel_axes(angle))
+    if math.isinf(fd):
+        f = None
+    elif not geometry.use_curved_detector:
+        f = pc + fd * n_hat - pixels
+    elif septa_along == "channels":  # the focal line at the central row of each column
+        f = -((pixels - pc) * v_hat).sum(-1, keepdim=True) * v_hat + fd * n_hat
+    else:
+        f = fd * n_hat + torch.zeros_like(pixels)
+    if septa_along == "rows":
+        u_hat, v_hat = v_hat, u_hat
+    a_n = (a * n_hat).sum(-1)
+    theta1 = torch.atan2((a * v_hat).sum(-1), a_n)
+    if f is not None:
+        theta1 = theta1 - torch.atan2((f * v_hat).sum(-1), (f * n_hat).sum(-1))
+    theta2 = torch.atan2((a * u_hat).sum(-1), a_n)
+    return torch.broadcast_tensors(theta1, theta2)
+
+
+def _strip_measure(x: Tensor, pitch: float, thickness: float) -> Tensor:
+    """Length of the strips [k p, k p + t) between 0 and x across them (negative for x < 0)."""
+    k = torch.floor(x / pitch)
+    return k * thickness + torch.clamp(x - k * pitch, 0.0, thickness)
+
+
+def grid_direct_transmission(pitch: float, thickness: float, height: float, mu_strip: Tensor, mu_interspace: Tensor,
+                             theta1: Tensor, theta2: Tensor) -> Tensor:
+    """[E, T1, T2] float64 transmission of the photons that cross a 1-D grid without interacting, averaged over the
+    entry offset across one pitch (the direct part of grid_model.py:270).
+
+    A photon at grid angles (theta1, theta2) crosses the grid height h over the path L = h sqrt(1 + tan^2 theta1 +
+    tan^2 theta2) and moves D = h tan|theta1| across the strips; entering at offset x it spends the share
+    s(x) = (C(x + D) - C(x)) / D of the path in the strips (C the strip measure). s is linear between the offsets where
+    x or x + D meets a strip face, so the mean of exp(-(mu_s - mu_i) L s(x)) over x is a sum of closed-form integrals
+    of exponentials of linear functions; with D = 0 it is ((p - t) + t exp(-(mu_s - mu_i) L)) / p. Both parts carry
+    the interspace's exp(-mu_i L)."""
+    if not (pitch > 0 and 0 <= thickness <= pitch and height > 0):
+        raise ValueError("need pitch > 0, 0 <= thickness <= pitch and height > 0")
+    f64 = torch.float64
+    ms = torch.as_tensor(mu_strip, dtype=f64).reshape(-1)
+    mi = torch.as_tensor(mu_interspace, dtype=f64).reshape(-1)
+    t1 = torch.as_tensor(theta1, dtype=f64).reshape(-1).abs().clamp(max=math.pi / 2 - 1e-9)
+    t2 = torch.as_tensor(theta2, dtype=f64).reshape(-1).abs().clamp(max=math.pi / 2 - 1e-9)
+    p, t, h = float(pitch), float(thickness), float(height)
+    L = h * torch.sqrt(1.0 + torch.tan(t1)[:, None] ** 2 + torch.tan(t2)[None, :] ** 2)  # [T1, T2]
+    D = h * torch.tan(t1)  # [T1]
+    moving = D > 1e-12 * p
+    Ds = torch.where(moving, D, torch.ones_like(D))
+    # offsets in [0, p) where the share s is not linear: the entry or the exit meets a strip face
+    knots = torch.stack([torch.zeros_like(D), torch.full_like(D, t), torch.remainder(-D, p),
+                         torch.remainder(t - D, p), torch.full_like(D, p)], -1).sort(-1).values  # [T1, 5]
+    share = (_strip_measure(knots + D[:, None], p, t) - _strip_measure(knots, p, t)) / Ds[:, None]  # [T1, 5]
+    c = (ms - mi)[:, None, None, None] * L[None, :, :, None]  # [E, T1, T2, 1]
+    e = c * share[None, :, None, :]  # exponent at the knots [E, T1, T2, 5]
+    e0, e1 = e[..., :-1], e[..., 1:]
+    d = (e1 - e0).abs()
+    mean = torch.exp(-torch.minimum(e0, e1)) * torch.where(d > 1e-12, -torch.expm1(-d) / d.clamp(min=1e-300),
+                                                          torch.ones_like(d))
+    widths = (knots[:, 1:] - knots[:, :-1])[None, :, None, :]
+    tilted = (mean * widths).sum(-1) / p
+    normal = ((p - t) + t * torch.exp(-c[..., 0])) / p
+    across = torch.where(moving[None, :, None], tilted, normal)
+    return across * torch.exp(-mi[:, None, None] * L[None])
+
+
+def focused_1d_grid(ratio: float, lamella_thickness_cm: float, lamella_height_cm: float,
+                    strip_attenuation: Callable[[np.ndarray], np.ndarray],
+                    interspace_attenuation: Callable[[np.ndarray], np.ndarray], *,
+                    energy_kev: Optional[np.ndarray] = None, theta1_rad: Optional[np.ndarray] = None,
+                    theta2_rad: Optional[np.ndarray] = None, focal_distance_cm: Optional[float] = None,
+                    septa_along: str = "channels", name: str = "focused 1-D grid") -> GridResponse:
+    """A focused 1-D grid of lamellae of thickness t and height h with the interspace width h / ratio (pitch
+    t + h / ratio), tabulated from its direct transmission (``grid_direct_transmission``; detector.py:779). The
+    attenuations (1/cm, of energy in keV) are passed in, since the photon database is not available offline; the
+    septa's own scatter and fluorescence (the secondary part of grid_model.py) are not modelled."""
+    if ratio <= 0:
+        raise ValueError("ratio must be positive")
+    h, tl = float(lamella_height_cm), float(lamella_thickness_cm)
+    pitch = tl + h / float(ratio)
+    en = np.asarray(np.linspace(10.0, 150.0, 141) if energy_kev is None else energy_kev, dtype=np.float64).reshape(-1)
+    t1 = np.asarray(np.radians(np.linspace(0.0, 30.0, 241)) if theta1_rad is None else theta1_rad,
+                    dtype=np.float64).reshape(-1)
+    t2 = np.asarray(np.radians(np.linspace(0.0, 60.0, 61)) if theta2_rad is None else theta2_rad,
+                    dtype=np.float64).reshape(-1)
+    table = grid_direct_transmission(pitch, tl, h, torch.as_tensor(strip_attenuation(en)),
+                                     torch.as_tensor(interspace_attenuation(en)), torch.as_tensor(t1),
+                                     torch.as_tensor(t2))
+    return GridResponse(torch.as_tensor(en), torch.as_tensor(t1), torch.as_tensor(t2), table, name=name,
+                        focal_distance_cm=focal_distance_cm, septa_along=septa_along)
 
 
 def detector_angle(omega: Tensor, geometry, angle: float, pixel_positions: Optional[Tensor] = None) -> Tensor:

@@ -213,6 +213,9 @@ def make_direction_context(quadrature: Quadrature, geometry: ConeGeometry, angle
     K = int(omega.shape[0])
     pixels = geometry.pixel_centers(angle, device, f64)
     Rd, Cd = int(pixels.shape[0]), int(pixels.shape[1])
+    if grid_response.depends_on_angles:
+        raise NotImplementedError("the collapsed-cone image takes grids whose transmission does not depend on the "
+                                  "photon's angles; angle-dependent grids apply to the primary")
     grid_t = grid_response.at_energy(energy, device, f64)  # [G]
     theta0 = detector_angle(omega, geometry, angle)
     det = detector.at(energy, theta0) if K else torch.zeros(G, 0, device=device, dtype=f64)  # [G, K]

@@ -154,6 +154,38 @@ def test_primary_image_and_spr():
     assert torch.allclose(spr, torch.full_like(spr, 0.3 / 1.3), rtol=1e-6)
 
 
+def test_primary_image_through_a_focused_grid():
+    """The primary behind an angle-dependent grid (focused 1-D septa, direct transmission, pinned on the CPU) is the
+    primary without a grid times the grid's transmission at each pixel's incidence angles, per group, to float32
+    rounding; the collapsed-cone image refuses such a grid."""
+    from citrus_amd.scatter_solver.detector import focused_1d_grid
+
+    dev = gpu_device()
+    lib = synthetic_library(3, 1, num_fine_per_group=3, fine_spread=0.3)
+    grid = Grid((6, 8, 5), 0.5)
+    angle = 0.3
+    rho = torch.zeros(2, *grid.shape, dtype=torch.float32)
+    rho[0] = 1.0
+    rho[1, 2:4, 3:6, 1:4] = 1.5
+    septa = focused_1d_grid(8.0, 0.0036, 0.15, lambda e: 3000.0 / e, lambda e: 0.0 * e + 0.05,
+                            focal_distance_cm=60.0)  # focused off the source: every pixel has its own angles
+    for curved in (False, True):
+        geometry = ConeGeometry(40.0, 70.0, 10, 12, 0.8, 0.8, use_curved_detector=curved)
+        args = (lib, rho.to(dev), grid, geometry, angle, SourceModel(), HipProjectors(), ideal_counting_response())
+        plain = primary_image(*args, no_grid())["signal_per_group"]
+        gridded = primary_image(*args, septa)["signal_per_group"]
+        pixels = geometry.pixel_centers(angle, None, torch.float64)
+        omega = torch.nn.functional.normalize(pixels - torch.tensor(geometry.source_position(angle),
+                                                                    dtype=torch.float64), dim=-1)
+        trans = septa.transmission_at(lib.group_energy.double(), omega, pixels, geometry, angle)  # [G, rows, cols]
+        assert float(trans.max() - trans.min()) > 0.02  # the pixels see the septa at different angles
+        assert rel_err(gridded, plain.double().cpu() * trans) < F32_REL
+    quad = hemisphere_quadrature(torch.tensor([0.0, -1.0, 0.0], dtype=torch.float64), 2, 4, 0.3,
+                                 dtype=torch.float64)
+    with pytest.raises(NotImplementedError):
+        make_direction_context(quad, geometry, angle, ideal_counting_response(), septa, lib, dev, torch.float32)
+
+
 def test_sh_expand_matches_float64_product():
     """The image's multiple-collision expansion on the HIP kernel equals the float64 product to float32 rounding
     (sums of nsh products, four per matrix-core step) for P5 and P3, whole and partial direction tiles, column counts
