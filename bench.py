@@ -266,6 +266,9 @@ def build_view_solver(args, device, rho_fine):
     return solver, fields, angles
 
 
+REPEATS = 3  # timed repeats of each helical step
+
+
 def helical_views(args, device, rho_fine, views: int = 8):
     """A turn of a helical scan (pitch 1.0: table feed per turn = beam width at the isocentre) on a 64-row curved panel
     with a bowtie filter (0-3 cm aluminium-like, quadratic in the channel) and 5% off-focal radiation: the views' CITRUS
@@ -315,9 +318,10 @@ def helical_views(args, device, rho_fine, views: int = 8):
         states = res[1]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = step(states)
+        for _ in range(REPEATS):  # the same step from the same states: the mean of a few repeats
+            res = step(states)
         torch.cuda.synchronize()
-        out[name] = {"views_per_s": views / (time.perf_counter() - t0)}
+        out[name] = {"views_per_s": REPEATS * views / (time.perf_counter() - t0)}
         if name == "cropped":
             cells = [(k1 - k0) // 4 for k0, k1 in res[2]]
             out[name]["transport_slices_per_view"] = cells
@@ -339,18 +343,19 @@ def helical_views(args, device, rho_fine, views: int = 8):
     res = citrus_multigrid_views_step(sched, mixed, rho, fine, angles, crop=True)  # warm-up of this configuration
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = citrus_multigrid_views_step(sched, mixed, rho, fine, angles, crop=True)
+    for _ in range(REPEATS):
+        res = citrus_multigrid_views_step(sched, mixed, rho, fine, angles, crop=True)
     torch.cuda.synchronize()
     out["multigrid_cropped"] = {
-        "views_per_s": views / (time.perf_counter() - t0),
+        "views_per_s": REPEATS * views / (time.perf_counter() - t0),
         "grids": [list(s.collided_moments.shape[-3:]) for s in res[1]],
         "stages": [s for s, _ in res[2]],
         "note": "one step of 8 helical views on beam-cropped grids, 4 entering the fine stage (64x64 cells at "
                 "0.32 cm, accumulators prolonged from the coarse stage) and 4 at the coarse stage (32x32 at 0.64 cm); "
-                "all accumulator updates in one persistent sweep launch; wall clock"}
+                "all accumulator updates in one persistent sweep launch; wall clock, mean of 3 repeats of the step"}
     out["note"] = (f"{views} views of one helical turn (pitch 1.0 = {beam_at_iso:.2f} cm per turn), 64 x 256 curved "
                    "panel, bowtie 0-3 cm at 2.7 g/cm^3, 5% off-focal at 0.5 cm; second step of each view (with its "
-                   "accumulator), wall clock; the covering grid is 64 slices")
+                   "accumulator), wall clock, mean of 3 repeats; the covering grid is 64 slices")
     return out
 
 
