@@ -283,3 +283,29 @@ def test_footprint_filters_match_the_torch_footprint():
     assert rel_err(acc, 1.0 + torch.einsum("kgrc,kc->rc", blurred, pw)) < F32_REL
     with pytest.raises(ValueError):
         fp.filters(1, partial, pw[:, 1:].contiguous())
+
+
+def test_multiple_emission_at_the_view_step_size():
+    """At the 64^3 view step's size (K = 64 directions, P5, 16 groups, 262,144 cells, 2 traced channels) the
+    collapsed multiple-collision emission equals the expansion on sh_expand followed by the attenuation and group sums
+    in float64, relative to the sums of magnitudes (bar 2e-6): the 32-bit moment offsets and the 64-cell workgroups
+    hold at full size."""
+    from citrus_amd.scatter_solver.scatter_image import multiple_emission, sh_expand
+
+    dev = gpu_device()
+    gen = torch.Generator(device=dev).manual_seed(47)
+    K, nsh, G, N = 64, 36, 16, 64 ** 3
+    Y = torch.randn(K, nsh, device=dev, generator=gen)
+    means = torch.randn(nsh, G, N, device=dev, generator=gen)
+    tau = 3.0 * torch.rand(K, 2, N, device=dev, generator=gen)
+    mix = torch.rand(G, 2, device=dev, generator=gen)
+    w = torch.rand(K, G, device=dev, generator=gen)
+    got = multiple_emission(Y, means, tau, w, mix).double()
+    S = sh_expand(Y, means.view(nsh, G * N)).view(K, G, N)
+    S_abs = sh_expand(Y.abs(), means.abs().view(nsh, G * N)).view(K, G, N)
+    del means
+    e_t = torch.exp(-torch.einsum("gc,kcn->kgn", mix.double(), tau.double()))
+    for c, weight in ((0, w.double()), (1, torch.ones_like(w, dtype=torch.float64))):
+        want = torch.einsum("kg,kgn->kn", weight, e_t * S.double())
+        scale = torch.einsum("kg,kgn->kn", weight, e_t * S_abs.double()) + 1e-30
+        assert float(((got[:, c] - want).abs() / scale).max()) < 2e-6, c
