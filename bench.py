@@ -41,7 +41,25 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
     ap.add_argument("--no-helical", action="store_true", help="skip the helical cropped-grid comparison")
     ap.add_argument("--cpu-directions", type=int, default=8, help="directions in the CPU-baseline sample")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a float32 buffer of MIB MiB three times on cuda:0 (the known-size kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters), then exit")
     return ap.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    """Three plain device copies of ``mib`` MiB. Under ``rocprofv3 --pmc FETCH_SIZE`` (then ``WRITE_SIZE``) each copy
+    dispatch moves exactly the buffer size each way, so its per-dispatch readings give the factors that turn the
+    counters into bytes for the other kernels."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device="cuda:0", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copy_probe: 3 copies of {mib} MiB ({n * 4} bytes each way)", flush=True)
 
 
 def build_problem(args, device):
@@ -451,6 +469,9 @@ def learned_vs_plain(solver, fields, angles, device):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): the per-view "

@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 9
+ABI_VERSION = 10
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -87,6 +87,9 @@ _SIGNATURES = {
                                          c_void, c_void]),
     "citrus_attenuation_to_densities": (c_int, [c_void, c_i64, c_int, c_void, c_void, c_void, c_void, c_void,
                                                 c_void]),
+    "citrus_resample_conservative": (c_int, [c_void, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                             c_void, c_void, c_void, c_int, c_void, c_void, c_void, c_int,
+                                             c_void, c_void, c_void, c_int, c_void, c_void]),
     "citrus_state_pack_workspace_floats": (c_i64, [c_int, c_int, c_int, c_i64]),
     "citrus_state_pack": (c_int, [c_void, c_int, c_int, c_int, c_i64, c_void, c_void, c_void, c_i64, c_void]),
     "citrus_state_unpack": (c_int, [c_void, c_void, c_int, c_int, c_int, c_i64, c_void, c_void]),

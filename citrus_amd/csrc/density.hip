@@ -97,3 +97,83 @@ extern "C" int citrus_attenuation_to_densities(const float* X, int64_t n, int M,
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("density model launch: ") + hipGetErrorString(e));
   return CITRUS_OK;
 }
+
+// ---- conservative resampling between grids ------------------------------------------------------------------------
+// out[ch][j] = sum_i field[ch][i] |V_i n V_j| / |V_j| over source voxels i and target cells j (geometry.py:304). The
+// overlap share of a target cell factors over the axes; along each axis the source cells a target cell overlaps are
+// contiguous, given as (first index, count, shares padded to `width`). One thread per (channel, target cell) sums its
+// count_r x count_c x count_s source voxels in a fixed order (slices innermost: contiguous reads), in float32.
+
+namespace citrus {
+namespace {
+
+struct Band {
+  const int* first;
+  const int* count;
+  const float* share;
+  int width;
+};
+
+__global__ void resample_kernel(const float* __restrict__ src, int channels, int R, int C, int S, int Rt, int Ct,
+                                int St, Band br, Band bc, Band bs, float* __restrict__ out) {
+  const int64_t cells = (int64_t)Rt * Ct * St;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells * channels) return;
+  const int ch = (int)(t / cells);
+  const int64_t j = t - (int64_t)ch * cells;
+  const int r = (int)(j / ((int64_t)Ct * St));
+  const int c = (int)((j / St) % Ct);
+  const int s = (int)(j % St);
+  const float* f = src + (int64_t)ch * R * C * S;
+  const int r0 = br.first[r], nr = min(br.count[r], br.width);
+  const int c0 = bc.first[c], nc = min(bc.count[c], bc.width);
+  const int s0 = bs.first[s], ns = min(bs.count[s], bs.width);
+  float acc = 0.0f;
+  for (int a = 0; a < nr; ++a) {
+    const int ri = r0 + a;
+    if (ri < 0 || ri >= R) continue;
+    const float wr = br.share[(int64_t)r * br.width + a];
+    for (int b = 0; b < nc; ++b) {
+      const int ci = c0 + b;
+      if (ci < 0 || ci >= C) continue;
+      const float wrc = wr * bc.share[(int64_t)c * bc.width + b];
+      const float* row = f + ((int64_t)ri * C + ci) * S;
+      float line = 0.0f;
+      for (int d = 0; d < ns; ++d) {
+        const int si = s0 + d;
+        if (si < 0 || si >= S) continue;
+        line += row[si] * bs.share[(int64_t)s * bs.width + d];
+      }
+      acc += wrc * line;
+    }
+  }
+  out[t] = acc;
+}
+
+}  // namespace
+}  // namespace citrus
+
+extern "C" int citrus_resample_conservative(const float* field, int channels, int R, int C, int S, int Rt, int Ct,
+                                            int St, const int* row_first, const int* row_count,
+                                            const float* row_share, int row_width, const int* col_first,
+                                            const int* col_count, const float* col_share, int col_width,
+                                            const int* slice_first, const int* slice_count,
+                                            const float* slice_share, int slice_width, float* out, void* stream) {
+  using namespace citrus;
+  if (!field || !out || !row_first || !row_count || !row_share || !col_first || !col_count || !col_share ||
+      !slice_first || !slice_count || !slice_share)
+    return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (channels < 0 || R < 1 || C < 1 || S < 1 || Rt < 1 || Ct < 1 || St < 1)
+    return fail(CITRUS_ERR_ARG, "grid sizes must be >= 1 and channels >= 0");
+  if (row_width < 1 || col_width < 1 || slice_width < 1) return fail(CITRUS_ERR_ARG, "band widths must be >= 1");
+  const int64_t total = (int64_t)Rt * Ct * St * channels;
+  if (total == 0) return CITRUS_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const Band br{row_first, row_count, row_share, row_width}, bc{col_first, col_count, col_share, col_width},
+      bs{slice_first, slice_count, slice_share, slice_width};
+  hipLaunchKernelGGL(resample_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, field, channels, R, C,
+                     S, Rt, Ct, St, br, bc, bs, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("resample launch: ") + hipGetErrorString(e));
+  return CITRUS_OK;
+}

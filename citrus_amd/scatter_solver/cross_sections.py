@@ -130,16 +130,44 @@ def angular_table_nodes(num_nodes: int, eta_map: Optional[Tensor] = None) -> Ten
     return eta
 
 
+# (scale, bone share) of each default reference material's row in the seven-material synthetic library: the row is
+# scale x ((1 - share) x water + share x bone) of the two closed-form rows (cross_sections.py:1112-1116)
+_SEVEN_MATERIAL_BLENDS = {"air": (0.90, 0.0), "lung_icrp": (0.95, 0.02), "adipose_icrp": (0.97, 0.0),
+                          "water": (1.0, 0.0), "muscle_icru44": (1.0, 0.02), "soft_tissue_icru44": (1.0, 0.04),
+                          "bone_cortical_icrp": (1.0, 1.0)}
+# nominal densities (g/cm^3) of the default reference materials (cross_sections.py:83-89)
+_NOMINAL_DENSITIES = {"air": 0.0012, "lung_icrp": 0.26, "adipose_icrp": 0.95, "water": 1.0, "muscle_icru44": 1.05,
+                      "soft_tissue_icru44": 1.06, "bone_cortical_icrp": 1.85}
+
+
+def _blend_rows(rows: list, a: Tensor) -> Tensor:
+    """[M, ...] material rows mixed from the two base rows ``a [2, ...]``: a row with share 0 or 1 and scale 1 is
+    the base row itself (bit for bit), every other row ``scale ((1 - share) a0 + share a1)``."""
+    out = []
+    for scale, share in rows:
+        if scale == 1.0 and share in (0.0, 1.0):
+            out.append(a[int(share)].clone())
+        else:
+            out.append(scale * ((1.0 - share) * a[0] + share * a[1]))
+    return torch.stack(out)
+
+
 def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int = 401, num_fine_per_group: int = 4,
-                      fine_spread: float = 0.0, angular_table_grid: str = "uniform") -> MultigroupLibrary:
-    """The reference's two-material closed-form test library (cross_sections.py:1043-1100): kernel
-    ``a (1 + b eta) / 4 pi`` with ``a = a_m 0.6^(go-gi)`` (a_m = 0.15, 0.12), ``b = 0.5`` in-group else 0.2, photo-
-    electric ``0.05 / 0.2 * 1.5^g``; its Legendre moments are exactly ``a`` and ``a b / 3``. ``angular_table_grid``
-    "adaptive" places the table nodes by ``synthetic_angular_map``."""
+                      fine_spread: float = 0.0, num_materials: int = 2,
+                      angular_table_grid: str = "uniform") -> MultigroupLibrary:
+    """The reference's closed-form test library (cross_sections.py:1043-1131): kernel ``a (1 + b eta) / 4 pi`` with
+    ``a = a_m 0.6^(go-gi)`` (a_m = 0.15, 0.12), ``b = 0.5`` in-group else 0.2, photoelectric ``0.05 / 0.2 * 1.5^g``;
+    its Legendre moments are exactly ``a`` and ``a b / 3``. ``num_materials`` 2 gives (water, bone); 7 the seven
+    default reference materials, each row a fixed blend of the two; any other M > 2 the rows "m0", "m1", ... (water,
+    bone, then ``1 + 0.1 m`` times the blend at bone share ``(m - 1)/(M - 1)``). ``angular_table_grid`` "adaptive"
+    places the table nodes by ``synthetic_angular_map``."""
     G, L = num_groups, legendre_order
     edges = torch.linspace(125.0, 10.0, G + 1, dtype=torch.float64)
     if angular_table_grid not in ("uniform", "adaptive"):
         raise ValueError(f"angular_table_grid must be 'uniform' or 'adaptive', got {angular_table_grid!r}")
+    M = int(num_materials)
+    if M < 2:
+        raise ValueError("num_materials must be >= 2")
     eta_map = synthetic_angular_map() if angular_table_grid == "adaptive" else None
     eta = angular_table_nodes(num_eta, eta_map)
     mu_s_eta = torch.zeros(2, G, G, num_eta, dtype=torch.float64)
@@ -156,9 +184,23 @@ def synthetic_library(num_groups: int = 4, legendre_order: int = 5, num_eta: int
     mu_pe = torch.tensor([[0.05 * 1.5 ** g for g in range(G)], [0.2 * 1.5 ** g for g in range(G)]],
                          dtype=torch.float64)
     mu_t = mu_pe + mu_s_l[..., 0].sum(dim=2)
-    return MultigroupLibrary(edges, mu_t, mu_s_l, eta, mu_s_eta, ("water", "bone_cortical_icrp"),
-                             *_fine_tables(edges, mu_t, num_fine_per_group, fine_spread), eta_map=eta_map,
-                             mu_pe=mu_pe, nominal_density=torch.tensor([1.0, 1.92], dtype=torch.float64))
+    fraction, fine_energy, fine_mu_t, fine_weight, fine_group = _fine_tables(edges, mu_t, num_fine_per_group,
+                                                                             fine_spread)
+    if M == 2:
+        names, nominal = ("water", "bone_cortical_icrp"), torch.tensor([1.0, 1.85], dtype=torch.float64)
+    else:
+        if M == len(_SEVEN_MATERIAL_BLENDS):
+            names = tuple(_SEVEN_MATERIAL_BLENDS)
+            rows = [_SEVEN_MATERIAL_BLENDS[n] for n in names]
+            nominal = torch.tensor([_NOMINAL_DENSITIES[n] for n in names], dtype=torch.float64)
+        else:
+            names = tuple(f"m{m}" for m in range(M))
+            rows = [(1.0, 0.0), (1.0, 1.0)] + [(1.0 + 0.1 * m, (m - 1) / (M - 1)) for m in range(2, M)]
+            nominal = torch.arange(1, M + 1, dtype=torch.float64)
+        mu_t, mu_pe, mu_s_l, mu_s_eta, fine_mu_t = (_blend_rows(rows, a)
+                                                    for a in (mu_t, mu_pe, mu_s_l, mu_s_eta, fine_mu_t))
+    return MultigroupLibrary(edges, mu_t, mu_s_l, eta, mu_s_eta, names, fraction, fine_energy, fine_mu_t,
+                             fine_weight, fine_group, eta_map=eta_map, mu_pe=mu_pe, nominal_density=nominal)
 
 
 def seeded_library(num_groups: int, legendre_order: int, e_max_kev: float, e_min_kev: float, seed: int,

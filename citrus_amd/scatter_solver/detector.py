@@ -2,12 +2,14 @@
 
 Keeps the interface of ``citrus/scatter_solver/detector.py``: ``DetectorResponse`` (:197, a table D(E, theta0) in
 keV deposited per incident photon, bilinear in energy and angle, clamped to the end nodes, |theta| used),
-``ideal_counting_response`` (:294, D = 1), ``energy_weighted_response`` (:305, D = E), an absorber response with the
-oblique path ``t / cos(theta)`` like ``csi_response`` (:258; the absorber's mass attenuation is passed in, since the
-photon database is not available offline), ``GridResponse`` (:657: trilinear in energy and the grid angles, focused
+``ideal_counting_response`` (:294, D = 1), ``energy_weighted_response`` (:305, D = E), ``csi_response`` (:258, a CsI
+scintillator with the oblique path ``t / cos(theta)`` and the coarse XCOM mass-attenuation table of CsI, :58-76) and
+the general ``absorber_response`` behind it, ``csi_mass_attenuation`` / ``lead_mass_attenuation`` (:120-144, log-log
+interpolation of the two embedded tables), ``GridResponse`` (:657: trilinear in energy and the grid angles, focused
 or unfocused, septa along the channels or the rows, crossed septa) with ``no_grid`` (:758), ``grid_angles`` (:1077),
-``default_focal_distance`` (:1039), the direct transmission of a 1-D grid and ``focused_1d_grid`` (:779, the direct
-part of grid_model.py) and ``detector_angle`` (:1054).
+``default_focal_distance`` (:1039), ``focused_1d_grid`` (:779, the septum-shadow table of the reference's signature)
+and the closed-form direct transmission of a 1-D grid (``analytic_1d_grid``, the direct part of grid_model.py:270),
+and ``detector_angle`` (:1054).
 """
 
 from __future__ import annotations
@@ -19,6 +21,51 @@ from typing import Callable, Optional
 import numpy as np
 import torch
 from torch import Tensor
+
+
+RHO_CSI = 4.51  # g/cm^3 (detector.py:52)
+RHO_LEAD = 11.35  # g/cm^3 (detector.py:53)
+_EDGE_STEP_KEV = 1e-3  # the two energies of an absorption-edge pair are this far apart (detector.py:54)
+
+# Total mass attenuation with coherent scattering (cm^2/g) of the coarse NIST XCOM tables the reference embeds
+# (detector.py:58-76 and :78-93). Data, kept as published: CsI with the iodine (33.17 keV) and caesium (35.98 keV) K
+# edges, lead with its L3 (13.04 keV) and K (88.0 keV) edges.
+CSI_TABLE = np.array([(10.0, 168.0), (15.0, 57.5), (20.0, 26.6), (25.0, 14.6), (30.0, 8.9), (33.17, 6.6),
+                      (33.17 + _EDGE_STEP_KEV, 21.6), (35.98, 17.5), (35.98 + _EDGE_STEP_KEV, 31.3), (40.0, 23.5),
+                      (50.0, 12.9), (60.0, 7.95), (80.0, 3.75), (100.0, 2.07), (150.0, 0.72)])
+LEAD_TABLE = np.array([(10.0, 130.6), (13.04, 67.0), (13.04 + _EDGE_STEP_KEV, 157.0), (15.0, 111.6), (20.0, 86.4),
+                       (30.0, 30.3), (40.0, 14.4), (50.0, 8.04), (60.0, 5.02), (80.0, 2.42), (88.0, 1.91),
+                       (88.0 + _EDGE_STEP_KEV, 7.68), (100.0, 5.55), (150.0, 2.01)])
+
+
+def _as_numpy(x) -> np.ndarray:
+    return np.asarray(x.detach().cpu().numpy() if isinstance(x, Tensor) else x, dtype=np.float64)
+
+
+def log_log_table(table: np.ndarray, energy_kev) -> np.ndarray:
+    """Mass attenuation of an ascending ``[N, 2]`` (keV, cm^2/g) table at positive energies: a straight line in
+    (log E, log mu) between neighbouring nodes, and the end segments' lines continued beyond the table
+    (detector.py:96-118)."""
+    e = _as_numpy(energy_kev)
+    if np.any(e <= 0):
+        raise ValueError("energies must be positive")
+    le, lm = np.log(table[:, 0]), np.log(table[:, 1])
+    x = np.log(e)
+    seg = np.clip(np.searchsorted(le, x, side="right") - 1, 0, len(le) - 2)  # segment, end segments extended
+    inside = (x >= le[0]) & (x <= le[-1])
+    y_line = lm[seg] + (lm[seg + 1] - lm[seg]) / (le[seg + 1] - le[seg]) * (x - le[seg])
+    y = np.where(inside, np.interp(x, le, lm), y_line)
+    return np.exp(y)
+
+
+def csi_mass_attenuation(energy_kev) -> np.ndarray:
+    """Total mass attenuation of CsI (cm^2/g) from the embedded table (detector.py:121)."""
+    return log_log_table(CSI_TABLE, energy_kev)
+
+
+def lead_mass_attenuation(energy_kev) -> np.ndarray:
+    """Total mass attenuation of lead (cm^2/g) from the embedded table (detector.py:134)."""
+    return log_log_table(LEAD_TABLE, energy_kev)
 
 
 def _bracket(nodes: Tensor, x: Tensor) -> tuple[Tensor, Tensor, Tensor]:
@@ -80,20 +127,29 @@ def energy_weighted_response() -> DetectorResponse:
 
 
 def absorber_response(thickness_cm: float, density: float, mass_attenuation: Callable[[np.ndarray], np.ndarray],
-                      energies: Optional[np.ndarray] = None, thetas: Optional[np.ndarray] = None,
-                      name: str = "absorber") -> DetectorResponse:
-    """Energy-integrating absorber with the oblique path ``t / cos(theta)`` (the model of csi_response,
-    detector.py:258): ``D(E, theta) = E (1 - exp(-mu(E) rho t / cos(theta)))``, ``mass_attenuation`` in cm^2/g."""
+                      energies=None, thetas=None, name: str = "absorber") -> DetectorResponse:
+    """Energy-integrating absorber with the oblique path ``t / cos(theta)``: every interacting photon deposits its
+    energy, ``D(E, theta) = E (1 - exp(-mu(E) rho t / cos(theta)))`` keV per incident photon, ``mass_attenuation`` in
+    cm^2/g of keV (numpy in and out). Table nodes: 10-150 keV every 0.5 keV and 181 angles from 0 to 89.5 degrees
+    unless given (detector.py:258-291)."""
     if thickness_cm < 0:
         raise ValueError("thickness must be non-negative")
-    en = np.sort(np.asarray(np.linspace(10.0, 150.0, 281) if energies is None else energies,
-                            dtype=np.float64).reshape(-1))
-    th = np.sort(np.asarray(np.radians(np.linspace(0.0, 89.5, 181)) if thetas is None else thetas,
-                            dtype=np.float64).reshape(-1))
+    en = np.sort(_as_numpy(np.linspace(10.0, 150.0, 281) if energies is None else energies).reshape(-1))
+    th = np.sort(_as_numpy(np.radians(np.linspace(0.0, 89.5, 181)) if thetas is None else thetas).reshape(-1))
     mu = np.asarray(mass_attenuation(en), dtype=np.float64)
     path = thickness_cm / np.cos(np.clip(th, 0.0, math.pi / 2 - 1e-9))
     resp = en[:, None] * (-np.expm1(-mu[:, None] * density * path[None, :]))
     return DetectorResponse(torch.as_tensor(en), torch.as_tensor(th), torch.as_tensor(resp), name=name)
+
+
+def csi_response(thickness_cm: float = 0.06, density: float = RHO_CSI, energies=None,
+                 mass_attenuation: Optional[Callable[[np.ndarray], np.ndarray]] = None,
+                 thetas=None) -> DetectorResponse:
+    """CsI scintillator of thickness t (default 0.6 mm) with the oblique path model (detector.py:258), its mass
+    attenuation from the embedded table unless given."""
+    return absorber_response(thickness_cm, density, csi_mass_attenuation if mass_attenuation is None
+                             else mass_attenuation, energies, thetas,
+                             name=f"CsI {thickness_cm * 10:.2f} mm (1/cos model)")
 
 
 SEPTA_DIRECTIONS = ("channels", "rows")
@@ -276,30 +332,73 @@ def grid_direct_transmission(pitch: float, thickness: float, height: float, mu_s
     return across * torch.exp(-mi[:, None, None] * L[None])
 
 
-def focused_1d_grid(ratio: float, lamella_thickness_cm: float, lamella_height_cm: float,
-                    strip_attenuation: Callable[[np.ndarray], np.ndarray],
-                    interspace_attenuation: Callable[[np.ndarray], np.ndarray], *,
-                    energy_kev: Optional[np.ndarray] = None, theta1_rad: Optional[np.ndarray] = None,
-                    theta2_rad: Optional[np.ndarray] = None, focal_distance_cm: Optional[float] = None,
-                    septa_along: str = "channels", name: str = "focused 1-D grid") -> GridResponse:
+def analytic_1d_grid(ratio: float, lamella_thickness_cm: float, lamella_height_cm: float,
+                     strip_attenuation: Callable[[np.ndarray], np.ndarray],
+                     interspace_attenuation: Callable[[np.ndarray], np.ndarray], *,
+                     energy_kev=None, theta1_rad=None, theta2_rad=None, focal_distance_cm: Optional[float] = None,
+                     septa_along: str = "channels", name: str = "analytic 1-D grid") -> GridResponse:
     """A focused 1-D grid of lamellae of thickness t and height h with the interspace width h / ratio (pitch
-    t + h / ratio), tabulated from its direct transmission (``grid_direct_transmission``; detector.py:779). The
-    attenuations (1/cm, of energy in keV) are passed in, since the photon database is not available offline; the
-    septa's own scatter and fluorescence (the secondary part of grid_model.py) are not modelled."""
+    t + h / ratio), tabulated from its direct transmission through strips and interspace (``grid_direct_transmission``,
+    the direct part of the reference's analytic model, grid_model.py:270). The attenuations (1/cm, of energy in keV)
+    are passed in; the septa's own scatter and fluorescence (the secondary part of grid_model.py) are not modelled."""
     if ratio <= 0:
         raise ValueError("ratio must be positive")
     h, tl = float(lamella_height_cm), float(lamella_thickness_cm)
     pitch = tl + h / float(ratio)
-    en = np.asarray(np.linspace(10.0, 150.0, 141) if energy_kev is None else energy_kev, dtype=np.float64).reshape(-1)
-    t1 = np.asarray(np.radians(np.linspace(0.0, 30.0, 241)) if theta1_rad is None else theta1_rad,
-                    dtype=np.float64).reshape(-1)
-    t2 = np.asarray(np.radians(np.linspace(0.0, 60.0, 61)) if theta2_rad is None else theta2_rad,
-                    dtype=np.float64).reshape(-1)
+    en = _as_numpy(np.linspace(10.0, 150.0, 141) if energy_kev is None else energy_kev).reshape(-1)
+    t1 = _as_numpy(np.radians(np.linspace(0.0, 30.0, 241)) if theta1_rad is None else theta1_rad).reshape(-1)
+    t2 = _as_numpy(np.radians(np.linspace(0.0, 60.0, 61)) if theta2_rad is None else theta2_rad).reshape(-1)
     table = grid_direct_transmission(pitch, tl, h, torch.as_tensor(strip_attenuation(en)),
                                      torch.as_tensor(interspace_attenuation(en)), torch.as_tensor(t1),
                                      torch.as_tensor(t2))
     return GridResponse(torch.as_tensor(en), torch.as_tensor(t1), torch.as_tensor(t2), table, name=name,
                         focal_distance_cm=focal_distance_cm, septa_along=septa_along)
+
+
+def _grid_theta1_nodes() -> np.ndarray:
+    """theta1 nodes (rad): 401 from 0 to 10 degrees, where a lamella's shadow opens, then 160 up to 89.5 degrees
+    (detector.py:767-776)."""
+    return np.radians(np.unique(np.concatenate([np.linspace(0.0, 10.0, 401), np.linspace(10.0, 89.5, 160)])))
+
+
+def septum_shadow_transmission(pitch_cm: float, thickness_cm: float, height_cm: float, mu_lead_cm: np.ndarray,
+                               theta1: np.ndarray) -> np.ndarray:
+    """[E, T1] transmission of the geometric lamella-shadow model (detector.py:779-829): a photon tilted by theta1 from
+    the focusing direction passes the open part of a period, ``open = max(0, (p - t) - h |tan theta1|) / p``, or
+    crosses lead along the mean path ``L = (t / p) h / cos(theta1) / (1 - open)`` (the lead volume per period seen
+    along the tilted track, spread over the shadowed share), so ``G = open + (1 - open) exp(-mu L)``; ``mu_lead_cm``
+    [E] is the linear attenuation of the lamellae in 1/cm."""
+    p, t, h = float(pitch_cm), float(thickness_cm), float(height_cm)
+    th = np.clip(np.asarray(theta1, dtype=np.float64), -math.pi / 2 + 1e-9, math.pi / 2 - 1e-9)
+    open_share = np.maximum(0.0, (p - t) - h * np.abs(np.tan(th))) / p
+    shadow = 1.0 - open_share
+    safe = np.where(shadow > 0, shadow, 1.0)
+    lead_path = np.where(shadow > 0, (t / p) * h / np.cos(th) / safe, 0.0)
+    mu = np.asarray(mu_lead_cm, dtype=np.float64).reshape(-1, 1)
+    return open_share[None, :] + shadow[None, :] * np.exp(-mu * lead_path[None, :])
+
+
+def focused_1d_grid(ratio: float = 10.0, lamella_thickness_cm: float = 0.0036, lamella_height_cm: float = 0.15,
+                    pitch_cm: float = 1.0 / 60.0, focal_distance_cm: float = 150.0,
+                    mass_attenuation: Optional[Callable[[np.ndarray], np.ndarray]] = None,
+                    lead_density: float = RHO_LEAD, energies=None, theta1=None) -> GridResponse:
+    """A focused 1-D lead grid in the geometric lamella-shadow model, with the reference's signature and defaults
+    (detector.py:779): lamellae of thickness t and height h at pitch p, the transmission
+    ``septum_shadow_transmission`` of lead (``lead_mass_attenuation`` x ``lead_density`` unless ``mass_attenuation``
+    is given), independent of theta2. ``ratio`` only names the grid; t, h and p set its geometry."""
+    p, t, h = float(pitch_cm), float(lamella_thickness_cm), float(lamella_height_cm)
+    if not (0.0 <= t < p) or h < 0:
+        raise ValueError("need 0 <= lamella thickness < pitch and height >= 0")
+    en = np.sort(_as_numpy(np.linspace(10.0, 150.0, 281) if energies is None else energies).reshape(-1))
+    t1 = np.sort(_as_numpy(_grid_theta1_nodes() if theta1 is None else theta1).reshape(-1))
+    mu = np.asarray((lead_mass_attenuation if mass_attenuation is None else mass_attenuation)(en), dtype=np.float64)
+    table = septum_shadow_transmission(p, t, h, mu * float(lead_density), t1)
+    t2 = np.array([0.0, math.pi / 2])
+    return GridResponse(torch.as_tensor(en), torch.as_tensor(t1), torch.as_tensor(t2),
+                        torch.as_tensor(np.repeat(table[:, :, None], 2, axis=2)),
+                        focal_distance_cm=float(focal_distance_cm),
+                        name=f"focused 1D grid (nominal {ratio:g}:1, h/(p-t) = {h / (p - t):.1f}), {1 / p:.0f} lp/cm "
+                             "(geometric model)")
 
 
 def detector_angle(omega: Tensor, geometry, angle: float, pixel_positions: Optional[Tensor] = None) -> Tensor:

@@ -274,3 +274,8 @@ class HipProjectors:
 
 
 SiddonProjectors = HipProjectors
+
+
+# The reference's name of its default projector class (ray_tracing.py:291); callers that construct
+# ``SiddonProjectors()`` get the HIP traversal.
+SiddonProjectors = HipProjectors

@@ -4,14 +4,18 @@ Keeps the interface of ``ScatterSolver`` and ``MaterialFields`` (``citrus/scatte
 for the operators of the per-view transport path: the transport and quadrature grids, the sweep and cone direction
 sets, the density fields, the uncollided fluence, the first-collision moment source with its sampling correction, the
 scattering-moment source, the transport sweep, source iteration, the multiple-collision source moments, the scatter
-image and the primary. Densities are coarsened by integer block averages (the conservative resampling of
-geometry.py:304 when the grids nest); library building and the attenuation-to-density models are outside this build.
+image and the primary. Densities reach the transport and quadrature grids by conservative resampling (block means
+when the grids nest, the overlap kernel otherwise); library building from photon data is outside this build.
 """
 
 from __future__ import annotations
 
-from dataclasses import dataclass
-from typing import Optional
+import time
+from contextlib import contextmanager
+from dataclasses import dataclass, field, replace
+from typing import Optional, Sequence
+
+import numpy as np
 
 import torch
 from torch import Tensor
@@ -20,29 +24,42 @@ from .angular_quadrature import Quadrature, level_symmetric_quadrature, product_
 from .config import SolverConfig
 from .cross_sections import MultigroupLibrary, mu_t_volume
 from .detector import DetectorResponse, GridResponse, ideal_counting_response, no_grid
-from .geometry import ConeGeometry, Grid
+from .geometry import ConeGeometry, Grid, resample_conservative
 from .ray_tracing import HipProjectors
-from .scatter_image import combined_scatter_image, make_direction_context, panel_direction_set, primary_image
+from .scatter_image import (combined_scatter_image, make_direction_context, panel_direction_set, primary_image,
+                            scatter_fraction)
 from .source import SourceModel
-from .source_iteration import (MaterialTransfer, scatter_moments, scatter_source_function, source_iteration,
-                               summed_source_function)
+from .source_iteration import (MaterialTransfer, SourceIterationResult, scatter_moments, scatter_source_function,
+                               source_iteration, summed_source_function)
 from .transport_sweep import SweepContext, make_sweep_context, sweep
 from .uncollided import (first_collision_moments_at_directions, first_collision_sampling_correction,
                          present_materials, uncollided_fluence, first_collision_corrected_samples)
 
 
 def coarsen_density(rho_fine: Tensor, grid_fine: Grid, grid_coarse: Grid) -> Tensor:
-    """[M, Rc, Cc, Sc] cell-averaged densities of nested grids (scatter_solver.py:115): each coarse cell is the mean of
-    its f^3 fine voxels."""
+    """[M, Rc, Cc, Sc] cell-averaged densities of nested grids: each coarse cell is the mean of its f^3 fine voxels,
+    the conservative resampling (geometry.py:304) in the case where every coarse cell holds whole fine voxels.
+    Raises ``ValueError`` for grids that do not nest by one integer factor along every axis."""
     if grid_coarse == grid_fine:
         return rho_fine
     fs = [c // n if n and c % n == 0 else 0 for c, n in zip(grid_fine.shape, grid_coarse.shape)]
     f = fs[0]
-    if f < 1 or any(x != f for x in fs) or abs(grid_coarse.delta_voxel - f * grid_fine.delta_voxel) > 1e-9 * f:
+    nested = f >= 1 and all(x == f for x in fs) and all(
+        abs(dc - f * df) <= 1e-9 * f * df for dc, df in zip(grid_coarse.deltas, grid_fine.deltas))
+    if not nested or abs(grid_coarse.recon_slice_offset - grid_fine.recon_slice_offset) > 1e-9:
         raise ValueError(f"grid {grid_coarse.shape} does not nest the fine grid {grid_fine.shape} by an integer factor")
     M = int(rho_fine.shape[0])
     Rc, Cc, Sc = grid_coarse.shape
     return rho_fine.reshape(M, Rc, f, Cc, f, Sc, f).mean(dim=(2, 4, 6))
+
+
+def resample_density(rho_fine: Tensor, grid_fine: Grid, grid_to: Grid) -> Tensor:
+    """Densities of ``grid_fine`` on ``grid_to`` by conservative resampling: block means when the grids nest by an
+    integer factor, otherwise the overlap-weighted HIP kernel of ``resample_conservative``."""
+    try:
+        return coarsen_density(rho_fine, grid_fine, grid_to)
+    except ValueError:
+        return resample_conservative(rho_fine, grid_fine, grid_to)
 
 
 @dataclass
@@ -71,14 +88,65 @@ class MaterialFields:
         return self.transfer_coarse
 
 
+_IMAGE_KEYS = ("fluence", "planar_fluence", "signal_per_group", "signal")
+
+
+@dataclass
+class ViewResult:
+    """Primary and scatter images of one view from ``ScatterSolver.solve_view`` (scatter_solver.py:237-312): image
+    dicts (``fluence``, ``planar_fluence``, ``signal_per_group`` [G, Rd, Cd] and ``signal`` [Rd, Cd], or the signal
+    alone for a signal-only solve), the scatter signal and fraction, the source-iteration record (its ``phi_total``
+    empty unless kept), the collided moments when kept, and seconds and peak bytes per stage."""
+
+    angle: float
+    primary: dict
+    scatter: dict
+    scatter_signal: Tensor
+    scatter_fraction: Tensor
+    source_iteration_result: SourceIterationResult
+    phi_total: Optional[Tensor] = None
+    timings: dict = field(default_factory=dict)
+    peak_memory: dict = field(default_factory=dict)
+    peak_memory_bytes: Optional[int] = None
+
+    @property
+    def scatter_fluence(self) -> Tensor:
+        return self.scatter["fluence"]
+
+    @property
+    def scatter_planar_fluence(self) -> Tensor:
+        return self.scatter["planar_fluence"]
+
+    def to_numpy_dict(self) -> dict:
+        """Flat dict of numpy arrays for ``np.savez`` with the reference's keys (scatter_solver.py:282-312)."""
+        def host(t: Tensor) -> np.ndarray:
+            return t.detach().cpu().numpy()
+
+        out = {"angle": np.asarray(self.angle)}
+        for name, images in (("primary", self.primary), ("scatter", self.scatter)):
+            out.update({f"{name}_{k}": host(images[k]) for k in _IMAGE_KEYS if k in images})
+        out["scatter_signal"] = host(self.scatter_signal)
+        out["scatter_fraction"] = host(self.scatter_fraction)
+        r = self.source_iteration_result
+        out["source_iteration_history"] = np.asarray(r.history, dtype=np.float64)
+        out["source_iteration_phi_norms"] = np.asarray(r.phi_norms, dtype=np.float64)
+        out["source_iteration_num_iterations"] = np.asarray(r.num_iterations)
+        out["source_iteration_converged"] = np.asarray(r.converged)
+        out.update({f"timing_{k}": np.asarray(v) for k, v in self.timings.items()})
+        out["peak_memory_bytes"] = np.asarray(-1 if self.peak_memory_bytes is None else self.peak_memory_bytes)
+        if self.phi_total is not None:
+            out["phi_total"] = host(self.phi_total)
+        return out
+
+
 class ScatterSolver:
     """Per-view scatter solver (scatter_solver.py:314) on one GPU, float32."""
 
     def __init__(self, library: MultigroupLibrary, config: SolverConfig, geometry: ConeGeometry, projectors=None,
                  detector: Optional[DetectorResponse] = None, grid_response: Optional[GridResponse] = None,
                  source: Optional[SourceModel] = None, device=None, dtype: torch.dtype = torch.float32) -> None:
-        if dtype != torch.float32:
-            raise ValueError("the HIP path computes in float32")
+        if dtype != torch.float32 or getattr(config, "dtype", "float32") != "float32":
+            raise ValueError("the HIP path computes in float32 (dtype torch.float32, config.dtype 'float32')")
         self.library = library
         self.config = config.validate()
         self.geometry = geometry
@@ -168,8 +236,8 @@ class ScatterSolver:
         coarse = self.transport_grid(grid_fine)
         q = self.quadrature_points(grid_fine)
         gq = coarse.refine(q)
-        rho_c = coarsen_density(rho, grid_fine, coarse).contiguous()
-        rho_q = rho_c if gq == coarse else coarsen_density(rho, grid_fine, gq).contiguous()
+        rho_c = resample_density(rho, grid_fine, coarse).contiguous()
+        rho_q = rho_c if gq == coarse else resample_density(rho, grid_fine, gq).contiguous()
         traced = present_materials(rho_q)
         return MaterialFields(grid_fine=grid_fine, grid_coarse=coarse, grid_quadrature=gq, q=q, rho_fine=rho,
                               rho_coarse=rho_c, rho_quadrature=rho_q,
@@ -290,3 +358,70 @@ class ScatterSolver:
     def primary(self, fields: MaterialFields, angle: float) -> dict:
         return primary_image(self.library, fields.rho_fine, fields.grid_fine, self.geometry, angle, self.source,
                              self.projectors, self.detector, self.grid_response, materials=fields.fine_materials)
+
+    # ---- whole views ------------------------------------------------------------------------------------------
+    @contextmanager
+    def _stage(self, name: str, timings: dict, peaks: dict):
+        """Seconds (device work included) and peak allocated bytes of one stage (scatter_solver.py:1020-1036)."""
+        torch.cuda.synchronize(self.device)
+        torch.cuda.reset_peak_memory_stats(self.device)
+        t0 = time.perf_counter()
+        yield
+        torch.cuda.synchronize(self.device)
+        peaks[name] = int(torch.cuda.max_memory_allocated(self.device))
+        timings[name] = time.perf_counter() - t0
+
+    def solve_view(self, rho_fine: Tensor, grid_fine: Grid, angle: float, keep_phi_total: bool = False,
+                   signal_only: bool = False) -> ViewResult:
+        """One view: fields, uncollided fluence, source iteration, scatter image, primary and scatter fraction
+        (scatter_solver.py:1038-1062)."""
+        timings: dict = {}
+        peaks: dict = {}
+        start = time.perf_counter()
+        with self._stage("materials", timings, peaks):
+            fields = self.material_fields(rho_fine, grid_fine)
+        return self._solve_view(fields, angle, keep_phi_total, timings, peaks, start, signal_only)
+
+    def solve_view_from_fields(self, fields: MaterialFields, angle: float, keep_phi_total: bool = False,
+                               signal_only: bool = False) -> ViewResult:
+        """One view from prebuilt fields, which every view of a volume can share (scatter_solver.py:1063-1077)."""
+        return self._solve_view(fields, angle, keep_phi_total, {}, {}, time.perf_counter(), signal_only)
+
+    def solve_views(self, rho_fine: Tensor, grid_fine: Grid, angles: Sequence[float], keep_phi_total: bool = False,
+                    signal_only: bool = False) -> list:
+        """Every view independently with ``solve_view`` (scatter_solver.py:1125-1140)."""
+        return [self.solve_view(rho_fine, grid_fine, float(a), keep_phi_total=keep_phi_total, signal_only=signal_only)
+                for a in angles]
+
+    def _solve_view(self, fields: MaterialFields, angle: float, keep_phi_total: bool, timings: dict, peaks: dict,
+                    start_time: float, signal_only: bool = False) -> ViewResult:
+        """The stages of one view after its fields (scatter_solver.py:1079-1124): the source iteration from the
+        corrected first-collision source, the multiple-collision source moments of the summed collided fluence, the
+        collapsed-cone scatter image over the view's cone set and the primary."""
+        with self._stage("sweep_context", timings, peaks):
+            sweep_context = self.sweep_context(fields.grid_coarse)
+        with self._stage("uncollided", timings, peaks):
+            phi_uncollided = self.uncollided_fluence(fields, angle)
+        with self._stage("source_iteration", timings, peaks):
+            result = self.run_source_iteration(
+                fields, self.first_collision_moment_source(fields, angle, phi_uncollided, sweep_context))
+        with self._stage("multiple_collision_source", timings, peaks):
+            source_moments = self.multiple_collision_source_moments(fields, result.phi_total)
+            phi_total = result.phi_total if keep_phi_total else None
+            result = replace(result, phi_total=result.phi_total if keep_phi_total
+                             else torch.empty(0, device=self.device))
+        with self._stage("cone_directions", timings, peaks):
+            quadrature = self.cone_quadrature(fields.grid_fine, angle)
+            context = self.direction_context(fields, angle, quadrature)
+        with self._stage("scatter_image", timings, peaks):
+            scatter = self.scatter_image(fields, angle, phi_uncollided, source_moments, quadrature, context,
+                                         signal_only=signal_only)
+        del phi_uncollided, source_moments
+        with self._stage("primary", timings, peaks):
+            primary = self.primary(fields, angle)
+        signal = scatter["signal"]
+        timings["total"] = time.perf_counter() - start_time
+        return ViewResult(angle=float(angle), primary=primary, scatter=scatter, scatter_signal=signal,
+                          scatter_fraction=scatter_fraction(signal, primary["signal"]),
+                          source_iteration_result=result, phi_total=phi_total, timings=timings, peak_memory=peaks,
+                          peak_memory_bytes=max(peaks.values()) if peaks else None)

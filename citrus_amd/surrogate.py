@@ -281,14 +281,41 @@ class UNet3D:
             arrays[f"{name}.b"] = self.biases[name].cpu().numpy()
         np.savez(path, **arrays)
 
+    @staticmethod
+    def read_arrays(path: str) -> tuple[tuple[int, int, int], dict]:
+        """((cin, cout, width), {array name: numpy array}) of a weight file written by ``save``, every array checked
+        against ``layer_shapes``: weights ``[Co, Ci, k^3]`` and biases ``[Co]``, float32. The convolution kernels
+        index the weights by those shapes, so a truncated, reshaped or float64 file raises ``ValueError`` here instead
+        of being read out of bounds on the device."""
+        with np.load(path, allow_pickle=False) as f:
+            if "shape" not in f.files:
+                raise ValueError(f"{path}: no 'shape' entry (not a surrogate weight file)")
+            shape = f["shape"]
+            if shape.shape != (3,) or shape.dtype.kind not in "iu":
+                raise ValueError(f"{path}: 'shape' must be 3 integers, got {shape.dtype} {shape.shape}")
+            cin, cout, width = (int(v) for v in shape)
+            if min(cin, cout, width) < 1:
+                raise ValueError(f"{path}: channel counts must be positive, got {(cin, cout, width)}")
+            arrays = {}
+            for name, (co, ci, k) in layer_shapes(cin, cout, width).items():
+                for key, want in ((f"{name}.w", (co, ci, k ** 3)), (f"{name}.b", (co,))):
+                    if key not in f.files:
+                        raise ValueError(f"{path}: missing array {key!r}")
+                    a = f[key]
+                    if a.dtype != np.float32:
+                        raise ValueError(f"{path}: {key} is {a.dtype}, expected float32")
+                    if tuple(a.shape) != want:
+                        raise ValueError(f"{path}: {key} has shape {tuple(a.shape)}, expected {want}")
+                    arrays[key] = np.ascontiguousarray(a)
+        return (cin, cout, width), arrays
+
     @classmethod
     def load(cls, path: str, device="cuda") -> "UNet3D":
-        with np.load(path, allow_pickle=False) as f:
-            cin, cout, width = (int(v) for v in f["shape"])
-            net = cls.create(cin, cout, width, device)
-            for name in LAYERS:
-                net.weights[name] = torch.from_numpy(f[f"{name}.w"]).to(net.device).contiguous()
-                net.biases[name] = torch.from_numpy(f[f"{name}.b"]).to(net.device).contiguous()
+        (cin, cout, width), arrays = cls.read_arrays(path)
+        net = cls.create(cin, cout, width, device)
+        for name in LAYERS:
+            net.weights[name] = torch.from_numpy(arrays[f"{name}.w"]).to(net.device).contiguous()
+            net.biases[name] = torch.from_numpy(arrays[f"{name}.b"]).to(net.device).contiguous()
         net._round_all()
         return net
 

@@ -188,6 +188,18 @@ int citrus_attenuation_to_densities(const float* X, int64_t n, int M, const int*
                                     const float* nominal_density, const float* mass_attenuation, float* rho,
                                     void* stream);
 
+/* Conservative resampling of a voxel-wise constant field between two grids: each target cell gets the exact average
+ * of the field over its volume (zero outside the source box), so every channel's integral is conserved. Replaces
+ * resample_conservative (citrus/scatter_solver/geometry.py:304-327) with its per-axis overlap matrices. field
+ * [channels][R][C][S] -> out [channels][Rt][Ct][St]; per axis (rows, cols, slices) the DEVICE arrays first [n_t],
+ * count [n_t] (int32) and share [n_t][width] (float32) give, for target cell j, the source cells first[j] ..
+ * first[j] + count[j] - 1 it overlaps and the share of j each one covers (the nonzero band of the overlap matrix). */
+int citrus_resample_conservative(const float* field, int channels, int R, int C, int S, int Rt, int Ct, int St,
+                                 const int* row_first, const int* row_count, const float* row_share, int row_width,
+                                 const int* col_first, const int* col_count, const float* col_share, int col_width,
+                                 const int* slice_first, const int* slice_count, const float* slice_share,
+                                 int slice_width, float* out, void* stream);
+
 /* ---------------------------------------------------------------------------------------------------------------
  * Compact per-view accumulators: the LD cell mean of every (view, group, moment) as float16 scaled by the
  * (group, moment)'s maximum magnitude over 2^14 (scale 1 where that maximum is 0).

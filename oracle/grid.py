@@ -33,3 +33,17 @@ def direct_transmission(pitch: float, thickness: float, height: float, mu_strip,
             for e in range(ms.size):
                 out[e, a, b] = np.exp(-(ms[e] * share + mi[e] * (1.0 - share)) * L).mean()
     return out
+
+
+def lamella_paths(pitch: float, thickness: float, height: float, theta1: float,
+                  n_offsets: int = 20000, n_track: int = 4000) -> np.ndarray:
+    """[n_offsets] lead path (cm) of a track tilted by theta1 from the focusing direction through the grid height,
+    for entry offsets x across one pitch (midpoint rule), by sampling the track (midpoint rule) against the lamellae
+    [k p, k p + t). The septum-shadow model of the reference's focused_1d_grid (detector.py:779-829) reduces the
+    distribution to two numbers: the share of offsets whose path is zero (the open share) and the mean path over
+    the others; both follow from these samples without its formulas."""
+    x = (np.arange(n_offsets) + 0.5) / n_offsets * pitch
+    z = (np.arange(n_track) + 0.5) / n_track
+    pos = x[:, None] + height * math.tan(abs(theta1)) * z[None, :]
+    share = (np.mod(pos, pitch) < thickness).mean(axis=1)
+    return share * height / math.cos(theta1)

@@ -10,8 +10,9 @@ import numpy as np
 import pytest
 import torch
 
-from citrus_amd.scatter_solver.detector import (GridResponse, focused_1d_grid, grid_angles,
-                                                grid_direct_transmission, no_grid)
+from citrus_amd.scatter_solver.detector import (GridResponse, analytic_1d_grid, focused_1d_grid, grid_angles,
+                                                grid_direct_transmission, lead_mass_attenuation, no_grid,
+                                                septum_shadow_transmission)
 from citrus_amd.scatter_solver.geometry import ConeGeometry
 from oracle import grid as ogrid
 
@@ -116,10 +117,10 @@ def test_grid_angles_of_focal_spot_photons(curved):
     assert tuple(k1.shape) == (2, 5, 7)
 
 
-def test_focused_grid_table_and_no_grid():
-    """focused_1d_grid tabulates the direct transmission on its nodes (pitch t + h / ratio); no_grid stays
+def test_analytic_grid_table_and_no_grid():
+    """analytic_1d_grid tabulates the direct transmission on its nodes (pitch t + h / ratio); no_grid stays
     angle-independent and transmits 1."""
-    g = focused_1d_grid(10.0, THICK, HEIGHT, lambda e: 2000.0 / e, lambda e: 0.0 * e,
+    g = analytic_1d_grid(10.0, THICK, HEIGHT, lambda e: 2000.0 / e, lambda e: 0.0 * e,
                         energy_kev=np.array([30.0, 60.0]), theta1_rad=np.array([0.0, 0.05]),
                         theta2_rad=np.array([0.0]))
     pitch = THICK + HEIGHT / 10.0
@@ -132,3 +133,29 @@ def test_focused_grid_table_and_no_grid():
     pix = geo.pixel_centers(0.0, None, torch.float64)
     t = no_grid().transmission_at(torch.tensor([40.0, 80.0]), torch.zeros(4, 6, 3, dtype=torch.float64), pix, geo, 0.0)
     assert tuple(t.shape) == (2, 4, 6) and bool((t == 1).all())
+
+
+@pytest.mark.parametrize("theta_deg", [0.0, 1.0, 3.0, 8.0, 25.0])
+def test_septum_shadow_model_against_track_sampling(theta_deg):
+    """The reference-signature focused_1d_grid (the lamella-shadow model, detector.py:779): its open share and mean
+    lead path equal the brute-force sampled tracks' share of zero paths and mean non-zero path (oracle.grid), and the
+    tabulated transmission is open + shadow exp(-mu rho L) of those two numbers at the lead table's attenuation."""
+    p, t, h = 1.0 / 60.0, 0.0036, 0.15
+    th = math.radians(theta_deg)
+    paths = ogrid.lamella_paths(p, t, h, th)
+    hit = paths > 1e-12
+    open_share = 1.0 - hit.mean()
+    mean_path = paths[hit].mean()
+    mu = np.array([0.5, 30.0, 300.0])  # 1/cm
+    got = septum_shadow_transmission(p, t, h, mu, np.array([th]))[:, 0]
+    want = open_share + (1 - open_share) * np.exp(-mu * mean_path)
+    assert np.allclose(got, want, rtol=2e-3, atol=2e-4)
+    g = focused_1d_grid(energies=np.array([40.0, 90.0]), theta1=np.array([0.0, th]) if th > 0 else None)
+    assert g.focal_distance_cm == 150.0 and "10:1" in g.name and not torch.equal(g.transmission[0], g.transmission[1])
+    i = int(np.argmin(np.abs(g.theta1_rad.numpy() - th)))
+    mu_pb = lead_mass_attenuation(np.array([40.0, 90.0])) * 11.35
+    assert np.allclose(g.transmission[:, i, 0].numpy(),
+                       septum_shadow_transmission(p, t, h, mu_pb, g.theta1_rad.numpy()[i:i + 1])[:, 0], rtol=1e-12)
+    assert torch.equal(g.transmission[..., 0], g.transmission[..., 1])  # no theta2 dependence
+    with pytest.raises(ValueError):
+        focused_1d_grid(lamella_thickness_cm=0.02)

@@ -27,6 +27,29 @@ def test_config_validation_and_quadrature_points():
             SolverConfig(**bad).validate()
 
 
+def test_reference_solver_config_constructs_unchanged():
+    """The configuration of the reference's quadrature-grid tests (test_quadrature_grid.py:38-40) constructs and
+    validates as written; the reference's validation rules reject what they reject there (config.py:177-222)."""
+    from dataclasses import replace
+
+    c = SolverConfig(num_groups=4, legendre_order=3, num_polar=4, num_azimuthal=8, cone_num_polar=8,
+                     cone_num_azimuthal=16, transport_cell_cm=4.0, quadrature_point_spacing_cm=2.0,
+                     source_iteration_tolerance=1e-4, max_source_iterations=40, direction_chunk=8)
+    assert c.validate() is c and c.quadrature_points(2.0) == 2 and c.materials[3] == "water"
+    assert replace(c, uncollided_trace_grid="fine").validate().uncollided_trace_grid == "fine"
+    assert SolverConfig(group_edges_kev=[120, 80, 60, 40, 30, 25, 20]).group_edges_kev == (120, 80, 60, 40, 30, 25, 20)
+    assert SolverConfig(grouping_medium={"water": 1.0, "air": 0.0}).validate().grouping_medium == (
+        ("air", 0.0), ("water", 1.0))
+    assert c.to_dict()["num_groups"] == 4
+    for bad in (dict(num_polar=3), dict(num_azimuthal=6), dict(num_groups=0), dict(group_edges_kev=(120.0, 20.0)),
+                dict(group_edges_method="linear"), dict(grouping_object_size_cm=0.0),
+                dict(grouping_medium=(("water", -1.0),)), dict(grouping_medium="lead"), dict(num_eta_table=1),
+                dict(angular_table_grid="log"), dict(materials=("water",)), dict(dtype="float16"),
+                dict(group_averaged_responses=1)):
+        with pytest.raises(ValueError):
+            replace(c, **bad).validate()
+
+
 def test_cone_set_weights_and_axis():
     axis = torch.tensor([0.3, -0.9, 0.2], dtype=torch.float64)
     q = hemisphere_quadrature(axis, 5, 10, 0.7, tangent_axes=([1.0, 0.0, 0.0], [0.0, 0.0, 1.0]))
@@ -195,3 +218,56 @@ def test_density_model_host_tables():
 
     with pytest.raises(ValueError):
         mass_attenuation_at(lib, 1000.0)
+
+
+def test_shipped_surrogate_weights_match_the_layer_shapes(tmp_path):
+    """The shipped 64^3 weights load on the CPU with every array of the shape and dtype the convolution kernels
+    index; a truncated array, a float64 array or a missing entry raises ValueError before anything reaches a GPU."""
+    from pathlib import Path
+
+    from citrus_amd.surrogate import LAYERS, UNet3D, layer_shapes
+
+    path = Path(__file__).resolve().parents[1] / "citrus_amd" / "weights" / "surrogate_64cube_s12_p5_g16.npz"
+    (cin, cout, width), arrays = UNet3D.read_arrays(str(path))
+    shapes = layer_shapes(cin, cout, width)
+    assert (cin, cout, width) == (2 + 4 * 16, 4 * 16, 16)  # two densities + P1 moments of 16 groups -> P1 of 16
+    assert set(arrays) == {f"{n}.{k}" for n in LAYERS for k in "wb"}
+    for name, (co, ci, k) in shapes.items():
+        assert arrays[f"{name}.w"].shape == (co, ci, k ** 3) and arrays[f"{name}.b"].shape == (co,)
+    assert sum(a.size for a in arrays.values()) > 100_000
+    base = {"shape": np.array([cin, cout, width], dtype=np.int64), **arrays}
+    for key, value in (("e0a.w", arrays["e0a.w"][:, :, :26]), ("head.b", arrays["head.b"].astype(np.float64))):
+        np.savez(tmp_path / "bad.npz", **{**base, key: value})
+        with pytest.raises(ValueError):
+            UNet3D.read_arrays(str(tmp_path / "bad.npz"))
+    np.savez(tmp_path / "bad.npz", **{k: v for k, v in base.items() if k != "b1.w"})
+    with pytest.raises(ValueError, match="missing"):
+        UNet3D.read_arrays(str(tmp_path / "bad.npz"))
+
+
+def test_csi_and_lead_tables_and_csi_response():
+    """The embedded CsI and lead tables (detector.py:58-93) interpolate linearly in (log E, log mu): exact at the
+    nodes, the geometric mean at the geometric-mean energy of two nodes, the end segments' power laws beyond the
+    table; csi_response is E (1 - exp(-mu rho t / cos theta)) on its nodes (detector.py:258-291)."""
+    from citrus_amd.scatter_solver.detector import (CSI_TABLE, LEAD_TABLE, RHO_CSI, csi_mass_attenuation,
+                                                    csi_response, lead_mass_attenuation)
+
+    for table, f in ((CSI_TABLE, csi_mass_attenuation), (LEAD_TABLE, lead_mass_attenuation)):
+        assert np.allclose(f(table[:, 0]), table[:, 1], rtol=1e-13)
+        e_mid = np.sqrt(table[:-1, 0] * table[1:, 0])
+        assert np.allclose(f(e_mid), np.sqrt(table[:-1, 1] * table[1:, 1]), rtol=1e-12)
+        lo = table[0, 1] * (0.5 ** (math.log(table[1, 1] / table[0, 1]) / math.log(table[1, 0] / table[0, 0])))
+        assert math.isclose(float(f([table[0, 0] * 0.5])[0]), lo, rel_tol=1e-12)
+        hi = table[-1, 1] * (2.0 ** (math.log(table[-1, 1] / table[-2, 1]) / math.log(table[-1, 0] / table[-2, 0])))
+        assert math.isclose(float(f([table[-1, 0] * 2.0])[0]), hi, rel_tol=1e-12)
+    assert float(csi_mass_attenuation([33.17 + 1e-3])[0]) == 21.6  # above the iodine K edge
+    with pytest.raises(ValueError):
+        csi_mass_attenuation([0.0])
+    r = csi_response()
+    assert tuple(r.response.shape) == (281, 181) and not r.angle_independent
+    e, th = r.energy_kev.numpy(), r.theta_rad.numpy()
+    want = e[:, None] * (1 - np.exp(-csi_mass_attenuation(e)[:, None] * RHO_CSI * 0.06 / np.cos(th)[None, :]))
+    assert np.allclose(r.response.numpy(), want, rtol=1e-12)
+    g = torch.tensor([30.0, 35.98 + 1e-3, 70.0], dtype=torch.float64)
+    d0 = r.at(g, torch.zeros(1, dtype=torch.float64))[:, 0]
+    assert float(d0[1]) > float(d0[0])  # the Cs K edge raises the absorbed share

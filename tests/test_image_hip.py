@@ -158,7 +158,7 @@ def test_primary_image_through_a_focused_grid():
     """The primary behind an angle-dependent grid (focused 1-D septa, direct transmission, pinned on the CPU) is the
     primary without a grid times the grid's transmission at each pixel's incidence angles, per group, to float32
     rounding; the collapsed-cone image refuses such a grid."""
-    from citrus_amd.scatter_solver.detector import focused_1d_grid
+    from citrus_amd.scatter_solver.detector import analytic_1d_grid
 
     dev = gpu_device()
     lib = synthetic_library(3, 1, num_fine_per_group=3, fine_spread=0.3)
@@ -167,7 +167,7 @@ def test_primary_image_through_a_focused_grid():
     rho = torch.zeros(2, *grid.shape, dtype=torch.float32)
     rho[0] = 1.0
     rho[1, 2:4, 3:6, 1:4] = 1.5
-    septa = focused_1d_grid(8.0, 0.0036, 0.15, lambda e: 3000.0 / e, lambda e: 0.0 * e + 0.05,
+    septa = analytic_1d_grid(8.0, 0.0036, 0.15, lambda e: 3000.0 / e, lambda e: 0.0 * e + 0.05,
                             focal_distance_cm=60.0)  # focused off the source: every pixel has its own angles
     for curved in (False, True):
         geometry = ConeGeometry(40.0, 70.0, 10, 12, 0.8, 0.8, use_curved_detector=curved)
