@@ -92,6 +92,7 @@ _SIGNATURES = {
                                              c_void, c_void, c_void, c_int, c_void, c_void]),
     "citrus_state_pack_workspace_floats": (c_i64, [c_int, c_int, c_int, c_i64]),
     "citrus_state_pack": (c_int, [c_void, c_int, c_int, c_int, c_i64, c_void, c_void, c_void, c_i64, c_void]),
+    "citrus_state_pack_static": (c_int, [c_void, c_int, c_int, c_int, c_i64, c_void, c_void, c_void]),
     "citrus_state_unpack": (c_int, [c_void, c_void, c_int, c_int, c_int, c_i64, c_void, c_void]),
     "citrus_scatter_moments_dense": (c_int, [c_void, c_int, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),
     "citrus_scatter_moments_factored": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_int, c_int,

@@ -13,10 +13,17 @@
 #include <hip/hip_fp16.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "citrus_hip.h"
 #include "common.h"
+
+// Scratch of the allocation-free pack (citrus_state_pack_static): the per-block maxima live in this file-scope device
+// array instead of a caller workspace. Every GPU the library runs on holds its own copy until the process ends; host
+// code never passes the array's name to a kernel (on the host it is a stand-in, not a device address) but the address
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+static __device__ __align__(256) float scratch[CITRUS_STATE_PACK_SCRATCH_FLOATS];
 
 namespace citrus {
 namespace {
@@ -120,6 +127,27 @@ struct PackK { static constexpr auto fn = state_pack_kernel<NSH>; };
 template <int NSH>
 struct UnpackK { static constexpr auto fn = state_unpack_kernel<NSH>; };
 
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+float* scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CITRUS_ERR_UNSUPPORTED, "device index outside 0..63");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CITRUS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return CITRUS_OK;
+}
+
 int check(int V, int G, int nsh, int64_t N) {
   if (V < 1 || G < 1 || N < 1) return fail(CITRUS_ERR_ARG, "V, G and N must be >= 1");
   if (nsh < 1 || nsh > kMaxMoments) return fail(CITRUS_ERR_ARG, "nsh must be (L+1)^2 <= 64");
@@ -136,13 +164,13 @@ extern "C" int64_t citrus_state_pack_workspace_floats(int V, int G, int nsh, int
   return (int64_t)V * G * citrus::blocks_per_slab(N) * nsh;
 }
 
-extern "C" int citrus_state_pack(const float* phi, int V, int G, int nsh, int64_t N, float* scale, void* q,
-                                 float* workspace, int64_t workspace_floats, void* stream) {
-  using namespace citrus;
-  if (int rc = check(V, G, nsh, N)) return rc;
-  if (!phi || !scale || !q || !workspace) return fail(CITRUS_ERR_ARG, "NULL buffer");
-  if (workspace_floats < citrus_state_pack_workspace_floats(V, G, nsh, N))
-    return fail(CITRUS_ERR_ARG, "workspace smaller than citrus_state_pack_workspace_floats");
+namespace citrus {
+namespace {
+
+// The three passes of the pack on `workspace` (>= citrus_state_pack_workspace_floats floats, fully written by the
+// first pass before the second reads it, so it needs no clearing).
+int pack(const float* phi, int V, int G, int nsh, int64_t N, float* scale, void* q, float* workspace,
+         void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const int nblk = blocks_per_slab(N);
   const int64_t slabs = (int64_t)V * G;
@@ -158,6 +186,32 @@ extern "C" int citrus_state_pack(const float* phi, int V, int G, int nsh, int64_
   if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("state pack: ") + hipGetErrorString(e));
   return CITRUS_OK;
+}
+
+}  // namespace
+}  // namespace citrus
+
+extern "C" int citrus_state_pack(const float* phi, int V, int G, int nsh, int64_t N, float* scale, void* q,
+                                 float* workspace, int64_t workspace_floats, void* stream) {
+  using namespace citrus;
+  if (int rc = check(V, G, nsh, N)) return rc;
+  if (!phi || !scale || !q || !workspace) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (workspace_floats < citrus_state_pack_workspace_floats(V, G, nsh, N))
+    return fail(CITRUS_ERR_ARG, "workspace smaller than citrus_state_pack_workspace_floats");
+  return pack(phi, V, G, nsh, N, scale, q, workspace, stream);
+}
+
+extern "C" int citrus_state_pack_static(const float* phi, int V, int G, int nsh, int64_t N, float* scale, void* q,
+                                        void* stream) {
+  using namespace citrus;
+  if (int rc = check(V, G, nsh, N)) return rc;
+  if (!phi || !scale || !q) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (citrus_state_pack_workspace_floats(V, G, nsh, N) > CITRUS_STATE_PACK_SCRATCH_FLOATS)
+    return fail(CITRUS_ERR_ARG, "V * G * ceil(N / 256) * nsh exceeds CITRUS_STATE_PACK_SCRATCH_FLOATS (" +
+                                    std::to_string((long long)CITRUS_STATE_PACK_SCRATCH_FLOATS) + " floats)");
+  float* ws = nullptr;
+  if (int rc = scratch_for_current_device(&ws)) return rc;
+  return pack(phi, V, G, nsh, N, scale, q, ws, stream);
 }
 
 extern "C" int citrus_state_unpack(const void* q, const float* scale, int V, int G, int nsh, int64_t N, float* phi,

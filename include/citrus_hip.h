@@ -215,6 +215,20 @@ int64_t citrus_state_pack_workspace_floats(int V, int G, int nsh, int64_t N);
 int citrus_state_pack(const float* phi, int V, int G, int nsh, int64_t N, float* scale, void* q, float* workspace,
                       int64_t workspace_floats, void* stream);
 
+/* Size of the device scratch array of citrus_state_pack_static, in floats (32 MiB per GPU, held until the process
+ * ends). */
+#define CITRUS_STATE_PACK_SCRATCH_FLOATS (1 << 23)
+
+/* citrus_state_pack without a workspace argument and without allocating: the per-block maxima go to a file-scope
+ * device array of CITRUS_STATE_PACK_SCRATCH_FLOATS floats, one per GPU, whose address is looked up on the first call
+ * on each device. No counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same inputs, outputs and results (bit for
+ * bit) as citrus_state_pack; returns CITRUS_ERR_ARG when citrus_state_pack_workspace_floats(V, G, nsh, N) exceeds
+ * the array. Every call on one GPU shares the array: calls that overlap on one GPU (two streams, or two threads)
+ * race and must be serialized by the caller. */
+int citrus_state_pack_static(const float* phi, int V, int G, int nsh, int64_t N, float* scale, void* q,
+                             void* stream);
+
 /* q [V][G][nsh][N] float16 and scale [V][G][nsh] -> phi [V][G][N][nsh][4] with phi[..][0] = q * scale and zero
  * LD slopes. */
 int citrus_state_unpack(const void* q, const float* scale, int V, int G, int nsh, int64_t N, float* phi,
