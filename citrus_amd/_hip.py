@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 10
+ABI_VERSION = 11
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -52,6 +52,7 @@ class PersistentArgs(ctypes.Structure):
     _fields_ = [
         ("G", c_int), ("nsh", c_int), ("D8", c_int), ("d0", c_int), ("dc", c_int), ("accumulate", c_int),
         ("source_layout", c_int), ("Y", c_void), ("Yw", c_void), ("views", ctypes.POINTER(SweepView)),
+        ("status", c_void), ("spin_limit", c_int),
     ]
 
 

@@ -212,8 +212,12 @@ def _finish_views(solver: ScatterSolver, batch: _ViewBatch, collided: Tensor, pr
         news.append(new)
         norms.append(_change_norms(new, st))
     del means
-    # one host synchronization for the step's accumulator changes, after every view's work is queued
-    changes = [_change_from_norms(nd, st) for nd, st in zip(torch.stack(norms).tolist(), batch.states)]
+    # one host synchronization for the step's accumulator changes and the persistent sweep's status word, after
+    # every view's work is queued; an aborted sweep raises here instead of handing on partial moments
+    from .scatter_solver.kernels.sweep_kernels import decode_status, fold_status
+
+    flat = decode_status(fold_status(torch.stack(norms)).tolist(), collided.device)
+    changes = [_change_from_norms(flat[2 * v:2 * v + 2], st) for v, st in enumerate(batch.states)]
     new_states = [CitrusState(angle=float(a), collided_moments=new, cone_quadrature=batch.cones[v],
                               num_steps=st.num_steps + 1, history=st.history + [c], scatter_source=T[v],
                               scatter_source_fields=fields, _batch=(T, v))

@@ -249,7 +249,10 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
     return fail(CITRUS_ERR_UNSUPPORTED, "the bfloat16 contraction is built for P3 and P5 (nsh 16 and 36)");
   PersistentArgs pa{};
   pa.G = A.G; pa.D8 = A.D8; pa.d0 = A.d0; pa.dc = A.dc; pa.M = 8 * A.D8;
-  pa.nviews = plan->nviews; pa.accumulate = A.accumulate ? 1 : 0; pa.spin_limit = kSpinLimit;
+  if (A.spin_limit < 0) return fail(CITRUS_ERR_ARG, "spin_limit must be >= 0 (0: the default)");
+  pa.nviews = plan->nviews; pa.accumulate = A.accumulate ? 1 : 0;
+  pa.spin_limit = A.spin_limit > 0 ? A.spin_limit : kSpinLimit;
+  pa.sink = A.status;
   pa.Y = A.Y; pa.Yw = A.Yw;
   pa.tasks = plan->d_tasks; pa.qoff = plan->d_qoff;
   pa.ctrl = static_cast<int*>(workspace);

@@ -63,6 +63,7 @@ struct PersistentArgs {
   float* faces;
   int* flags;
   int* ctrl;
+  int* sink;          // caller's sticky status word (OR of every abort code), or NULL
   PersistentView views[kMaxViews];
 };
 
@@ -122,14 +123,20 @@ __device__ __forceinline__ int load_flag(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Abandon the sweep: the workspace's status word (read by every wave's next check) and the caller's sticky word.
+__device__ __forceinline__ void abort_sweep(int* status, int* sink, int code) {
+  __hip_atomic_store(status, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (sink) __hip_atomic_fetch_or(sink, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Wait until the flag at `p` is set; false when the sweep is (or gets) aborted. Every lane of the wave runs it on the
 // same address and every branch is on a wave-uniform (scalar) value: control flow inside the task loop must never
 // depend on the lane, or the compiler may split the wave across the loop's exits.
-__device__ __forceinline__ bool wait_flag(const int* p, int* status, int limit) {
+__device__ __forceinline__ bool wait_flag(const int* p, int* status, int* sink, int limit) {
   int spins = 0;
   while (__builtin_amdgcn_readfirstlane(load_flag(p)) == 0) {
     if (++spins > limit) {
-      __hip_atomic_store(status, CITRUS_PERSISTENT_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      abort_sweep(status, sink, CITRUS_PERSISTENT_TIMEOUT);
       return false;
     }
     if ((spins & 31) == 0 && __builtin_amdgcn_readfirstlane(load_flag(status)) != 0) return false;
@@ -147,7 +154,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   int* status = a.ctrl + kStatusWord;
   const int xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15;  // HW_REG_XCC_ID, bits [3:0]
   if (xcc >= kQueues) {
-    __hip_atomic_store(status, CITRUS_PERSISTENT_BAD_XCC, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    abort_sweep(status, a.sink, CITRUS_PERSISTENT_BAD_XCC);
     return;
   }
   const int qbeg = a.qoff[xcc];
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
     bool ok = true;
     for_each_dependency(pv.nbr, pv.nbc, pv.nbs, pv.K, o, bi, bj, bk, [&](int oo, int i, int j, int k) {
-      if (ok) ok = wait_flag(fl + oo * nb + (i * pv.nbc + j) * pv.nbs + k, status, a.spin_limit);
+      if (ok) ok = wait_flag(fl + oo * nb + (i * pv.nbc + j) * pv.nbs + k, status, a.sink, a.spin_limit);
     });
     if (!ok) break;
 

@@ -24,6 +24,25 @@ MAX_CHUNK = 32
 _SELF_CHECK: dict[str, tuple[bool, str]] = {}
 
 
+class _HeldBuffers:
+    """Scratch buffers a captured HIP graph still points to. A schedule's scratch grows on demand; a graph captured
+    while an older buffer was current replays into that buffer, so once any capture has used the schedule its
+    replaced buffers are kept here until the schedule itself goes, instead of returning to torch's allocator."""
+
+    def __init__(self) -> None:
+        self.captured = False
+        self.buffers: list = []
+
+    def note_use(self) -> None:
+        if torch.cuda.is_available() and torch.cuda.is_current_stream_capturing():
+            self.captured = True
+
+    def retire(self, old: Optional[Tensor]) -> None:
+        if old is not None and (self.captured or (torch.cuda.is_available()
+                                                   and torch.cuda.is_current_stream_capturing())):
+            self.buffers.append(old)
+
+
 class SweepSchedule:
     """Owner of a native ``CitrusSweepPlan`` (block-diagonal task tables on the device) and its face scratch."""
 
@@ -38,15 +57,19 @@ class SweepSchedule:
         self.device = device
         self.block = block
         self._faces: Optional[Tensor] = None
+        self._held = _HeldBuffers()
         self._finalizer = weakref.finalize(self, lib.citrus_sweep_plan_destroy, handle)
 
     def faces(self, G: int, dc: int, V: int) -> Tensor:
-        """Face scratch for (G, dc, V), grown on demand and reused across sweeps."""
+        """Face scratch for (G, dc, V), grown on demand and reused across sweeps. A buffer a captured graph has
+        recorded is kept alive when a larger one replaces it (``_HeldBuffers``)."""
         n = int(_hip.library().citrus_sweep_face_floats(self.handle, G, dc, V))
         if n < 0:
             raise ValueError("invalid face-buffer request")
         if self._faces is None or self._faces.numel() < n:
+            self._held.retire(self._faces)
             self._faces = torch.empty(n, device=self.device, dtype=torch.float32)
+        self._held.note_use()
         return self._faces
 
     def run(self, *, mu: Tensor, phi: Tensor, Y: Tensor, Yw: Tensor, cdir: Tensor, G: int, nsh: int, D8: int,
@@ -127,6 +150,7 @@ class PersistentSchedule:
         self.G = int(G)
         self.device = device
         self._workspace: Optional[Tensor] = None
+        self._held = _HeldBuffers()
         self._finalizer = weakref.finalize(self, lib.citrus_persistent_plan_destroy, handle)
 
     @property
@@ -140,18 +164,22 @@ class PersistentSchedule:
         if n < 0:
             raise ValueError("invalid persistent workspace request")
         if self._workspace is None or self._workspace.numel() < n:
+            self._held.retire(getattr(self, "_raw", None))
             raw = torch.empty(n + 256, device=self.device, dtype=torch.uint8)
             off = (-raw.data_ptr()) % 256
             self._raw = raw
             self._workspace = raw[off:off + n]
+        self._held.note_use()
         return self._workspace
 
     def run(self, *, views, Y: Tensor, Yw: Tensor, nsh: int, D8: int, d0: int, dc: int, accumulate: bool = False,
-            source_layout: int = -1) -> None:
+            source_layout: int = -1, spin_limit: int = 0) -> None:
         """Enqueue one chunk sweep of every view on torch's current stream. ``views`` holds one dict per view of the
         plan with the device tensors ``mu`` [G, N], ``phi`` [G, N, nsh, 4], ``cdir`` [8 D8, 3], and either ``T``
         [G, N, nsh, 4] or ``T16`` [G, N, 4, KP] with ``T_inv_scale`` [G], and optionally ``source`` (group-major, the
-        same ``source_layout`` 1 or 2 for every view; -1 without)."""
+        same ``source_layout`` 1 or 2 for every view; -1 without). A sweep that aborts ORs its status code into the
+        device's status word (``persistent_status_sink``), which the callers' next host read reports
+        (``decode_status``); ``spin_limit`` > 0 replaces the default bound of each dependency wait."""
         if len(views) != len(self.shapes):
             raise ValueError(f"the plan has {len(self.shapes)} views, got {len(views)}")
         if nsh not in PERSISTENT_MOMENTS:
@@ -192,8 +220,11 @@ class PersistentSchedule:
         if Y.shape != (M, nsh) or Yw.shape != (M, nsh):
             raise ValueError("direction tables do not match the quadrature")
         ws = self.workspace(dc)
+        if spin_limit < 0:
+            raise ValueError("spin_limit must be >= 0")
         args = _hip.PersistentArgs(G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, accumulate=int(accumulate),
-                                   source_layout=int(source_layout), Y=Y.data_ptr(), Yw=Yw.data_ptr(), views=arr)
+                                   source_layout=int(source_layout), Y=Y.data_ptr(), Yw=Yw.data_ptr(), views=arr,
+                                   status=persistent_status_sink(self.device).data_ptr(), spin_limit=int(spin_limit))
         lib = _hip.library()
         with torch.cuda.device(self.device):
             _hip.check(lib.citrus_persistent_sweep_run(self.handle, ctypes.byref(args), ws.data_ptr(), ws.numel(),
@@ -214,6 +245,50 @@ class PersistentSchedule:
 
 _PERSISTENT: dict = {}
 _PERSISTENT_CHECK: dict[str, tuple[bool, str]] = {}
+_STATUS_SINKS: dict[str, Tensor] = {}
+
+
+def persistent_status_sink(device) -> Tensor:
+    """[1] int32 device word per device into which every persistent sweep that aborts ORs its status code. The
+    library never clears it: a step's single host read (``fold_status`` / ``decode_status``) sees the aborts of every
+    chunk and sweep since the last read."""
+    device = torch.device(device)
+    key = str(device)
+    sink = _STATUS_SINKS.get(key)
+    if sink is None:
+        sink = _STATUS_SINKS[key] = torch.zeros(1, device=device, dtype=torch.int32)
+    return sink
+
+
+def fold_status(values: Tensor) -> Tensor:
+    """``values`` flattened to float64 with the device's persistent-sweep status word appended (0 when no persistent
+    sweep ran there), for one host read of both (no synchronization here)."""
+    flat = values.reshape(-1).to(torch.float64)
+    sink = _STATUS_SINKS.get(str(values.device))
+    tail = sink.to(torch.float64) if sink is not None else torch.zeros(1, device=values.device, dtype=torch.float64)
+    return torch.cat([flat, tail])
+
+
+def decode_status(host_values, device) -> list:
+    """The payload of a ``fold_status`` read; raises ``RuntimeError`` naming the abort when the status is nonzero
+    (after clearing the device word, so the next step starts clean). A persistent sweep that aborted left partial
+    moments, so nothing computed from it may be used."""
+    values = list(host_values)
+    status = int(values[-1]) if values else 0
+    if status:
+        sink = _STATUS_SINKS.get(str(torch.device(device)))
+        if sink is not None:
+            sink.zero_()
+        reasons = [why for code, why in PERSISTENT_STATUS.items() if code and status & code] or [f"code {status}"]
+        raise RuntimeError("persistent sweep aborted (" + "; ".join(reasons) + "): its moments are incomplete")
+    return values[:-1]
+
+
+def check_persistent_status(device) -> None:
+    """Host read of the device's status word (synchronizes); raises ``RuntimeError`` after an abort."""
+    sink = _STATUS_SINKS.get(str(torch.device(device)))
+    if sink is not None:
+        decode_status([float(sink.item())], device)
 
 
 def persistent_usable(device) -> tuple[bool, str]:

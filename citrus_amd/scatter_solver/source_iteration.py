@@ -206,8 +206,12 @@ def multiple_collision_source_moments(mu_s_l_volume_coarse, phi_total: Tensor) -
 
 
 def _scalar_norms(*phis: Tensor) -> list[float]:
-    """float64 norms of the scalar-flux cell means ``phi[:, 0, 0]``, one host synchronization (:126-135)."""
-    return torch.stack([torch.linalg.vector_norm(p[:, 0, 0].to(torch.float64)) for p in phis]).tolist()
+    """float64 norms of the scalar-flux cell means ``phi[:, 0, 0]``, one host synchronization (:126-135) that also
+    reads the persistent sweep's status word (an aborted sweep raises ``RuntimeError``)."""
+    from .kernels.sweep_kernels import decode_status, fold_status
+
+    norms = torch.stack([torch.linalg.vector_norm(p[:, 0, 0].to(torch.float64)) for p in phis])
+    return decode_status(fold_status(norms).tolist(), norms.device)
 
 
 def source_iteration(sweep_context: SweepContext, mu_t_coarse: Tensor, mu_s_l_volume_coarse,

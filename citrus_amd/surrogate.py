@@ -518,8 +518,13 @@ def _api(phi_v: Tensor) -> Tensor:
 
 def _scalar_norms_views(*phis: Tensor) -> list:
     """Per view, float64 norms of the scalar-flux cell means of [V, G, R, C, S, nsh, 4] moments (one sync)."""
-    rows = [torch.linalg.vector_norm(p[..., 0, 0].reshape(p.shape[0], -1).double(), dim=1) for p in phis]
-    return torch.stack(rows).tolist()
+    from .scatter_solver.kernels.sweep_kernels import decode_status, fold_status
+
+    rows = torch.stack([torch.linalg.vector_norm(p[..., 0, 0].reshape(p.shape[0], -1).double(), dim=1)
+                        for p in phis])
+    flat = decode_status(fold_status(rows).tolist(), rows.device)  # raises if a persistent sweep aborted
+    V = int(rows.shape[1])
+    return [flat[i * V:(i + 1) * V] for i in range(len(phis))]
 
 
 def collision_orders_views(solver, fields, angles: Sequence[float], tolerance: float = 1e-4,

@@ -123,6 +123,12 @@ typedef struct CitrusPersistentArgs {
   const float* Y;            /* [8*D8][nsh], shared by the views */
   const float* Yw;           /* [8*D8][nsh] */
   const CitrusSweepView* views; /* HOST array of the plan's nviews views, read before the launch */
+  int* status;               /* DEVICE int or NULL: a sweep that aborts ORs its CITRUS_PERSISTENT_* code into it. The
+                              * word is never cleared by the library, so one word can collect the status of every
+                              * chunk and sweep until the caller reads and clears it (the workspace's own status word
+                              * is cleared by each chunk's launch) */
+  int spin_limit;            /* sleeps of one dependency wait before the sweep aborts; 0: the default (about a
+                              * second). A small value forces the abort path from the host, for tests */
 } CitrusPersistentArgs;
 
 /* Schedule of nviews view grids, shapes[v] = (R, C, S), with G groups. Uploads the task queues (hipMalloc). */

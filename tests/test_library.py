@@ -93,3 +93,52 @@ def test_product_sh_matches_oracle():
     om, w = oq.level_symmetric_quadrature(12)
     q = level_symmetric_quadrature(12)
     assert np.abs(q.omega.numpy() - om).max() < 1e-15 and np.abs(q.weight.numpy() - w).max() < 1e-15
+
+
+def test_status_fold_and_decode_on_the_cpu():
+    """The plumbing of the persistent sweep's status word without a GPU: folded reads carry the word as their last
+    value, a zero word returns the payload, a nonzero word raises RuntimeError naming the abort and clears the word."""
+    from citrus_amd.scatter_solver.kernels import sweep_kernels as sk
+
+    values = torch.tensor([[1.0, 2.0], [3.0, 4.0]], dtype=torch.float64)
+    assert sk.fold_status(values).tolist() == [1.0, 2.0, 3.0, 4.0, 0.0]  # no sink on this device: status 0
+    sink = torch.tensor([1], dtype=torch.int32)
+    sk._STATUS_SINKS["cpu"] = sink
+    try:
+        folded = sk.fold_status(values).tolist()
+        assert folded[-1] == 1.0
+        with pytest.raises(RuntimeError, match="spin limit"):
+            sk.decode_status(folded, "cpu")
+        assert int(sink.item()) == 0
+        assert sk.decode_status(sk.fold_status(values).tolist(), "cpu") == [1.0, 2.0, 3.0, 4.0]
+        sink.fill_(3)
+        with pytest.raises(RuntimeError, match="XCD"):
+            sk.check_persistent_status("cpu")
+    finally:
+        sk._STATUS_SINKS.pop("cpu", None)
+
+
+def _struct_fields(header_text: str, name: str) -> list:
+    body = re.search(r"typedef struct " + name + r" \{(.*?)\} " + name + ";", header_text, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if decl:
+            fields += [re.findall(r"(\w+)\s*$", part.strip())[0] for part in decl.split(",")]
+    return fields
+
+
+def test_bindings_and_documented_stub_match_the_header_structs():
+    """The ctypes mirrors of the argument structs, and the reference-side stub documented in INTEGRATION.md, list the
+    header's fields in the header's order (a short struct would make the library read past its end)."""
+    header = (ROOT / "include" / "citrus_hip.h").read_text()
+    assert [f for f, _ in _hip.SweepArgs._fields_] == _struct_fields(header, "CitrusSweepArgs")
+    assert [f for f, _ in _hip.SweepView._fields_] == _struct_fields(header, "CitrusSweepView")
+    assert [f for f, _ in _hip.PersistentArgs._fields_] == _struct_fields(header, "CitrusPersistentArgs")
+    doc = (ROOT / "INTEGRATION.md").read_text()
+    block = re.search(r"```python\n(.*?)```", doc, flags=re.S).group(1)
+    stub = block[block.index("class CitrusSweepArgs"):block.index("lib.citrus_sweep_plan_create")]
+    namespace = {"ctypes": ctypes}
+    exec(stub, namespace)  # the documented class definition only
+    assert [f for f, _ in namespace["CitrusSweepArgs"]._fields_] == _struct_fields(header, "CitrusSweepArgs")

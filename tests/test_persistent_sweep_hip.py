@@ -126,3 +126,36 @@ def test_view_step_shape_persistent_equals_launch_sequence():
     torch.cuda.synchronize()
     _check_ran_through([(64, 64, 64)] * V, G, dev)
     assert torch.equal(got, want)
+
+
+def test_forced_abort_is_reported_by_the_next_host_read():
+    """A persistent sweep whose dependency waits are bounded by a tiny spin limit (set from the host: every wave still
+    leaves normally) aborts; the device's sticky status word then makes the next folded host read raise, is cleared
+    by it, and the following full sweep runs through and equals the launch sequence bit for bit."""
+    from citrus_amd.scatter_solver.kernels.sweep_kernels import (decode_status, fold_status,
+                                                                 persistent_status_sink)
+
+    dev = gpu_device()
+    shape, G, V = (24, 24, 24), 2, 2
+    print(f"forced abort: shape {shape} G{G} V{V} spin limit 1", flush=True)
+    ctx = make_sweep_context(Grid(shape, 0.45), level_symmetric_quadrature(8), 3, device=dev)
+    mu, T, _ = _inputs(ctx, G, V, None, seed=5)
+    sched = persistent_schedule([shape] * V, G, dev)
+    out = torch.empty_like(T)
+    mu_flat = mu.reshape(G, -1).contiguous()
+    views = [dict(mu=mu_flat, phi=out[v], cdir=ctx.hip.cdir, T=T[v]) for v in range(V)]
+    D8 = ctx.directions_per_octant
+    sched.run(views=views, Y=ctx.hip.Y, Yw=ctx.hip.Yw, nsh=ctx.num_moments, D8=D8, d0=0, dc=D8, spin_limit=1)
+    torch.cuda.synchronize()
+    status, unfinished = sched.check()
+    assert status == 1 and unfinished > 0
+    assert int(persistent_status_sink(dev).item()) & 1
+    with pytest.raises(RuntimeError, match="spin limit"):
+        decode_status(fold_status(torch.ones(3, device=dev)).tolist(), dev)
+    assert int(persistent_status_sink(dev).item()) == 0
+    assert decode_status(fold_status(torch.ones(3, device=dev)).tolist(), dev) == [1.0, 1.0, 1.0]
+    got = sweep_views(ctx, mu, T, schedule="persistent")
+    want = sweep_views(ctx, mu, T, schedule="launches")
+    torch.cuda.synchronize()
+    _check_ran_through([shape] * V, G, dev)
+    assert torch.equal(got, want)

@@ -335,6 +335,34 @@ def test_sweep_graph_replays_equal_eager(contraction, with_source):
         del noise
 
 
+@pytest.mark.parametrize("schedule", ["persistent", "launches"])
+def test_sweep_graph_survives_scratch_growth(schedule):
+    """A graph captured on a schedule whose scratch later grows (an eager sweep of the same grid with more directions
+    per chunk, or more views) keeps replaying into the buffer it recorded: the replay still equals the eager sweep of
+    its inputs bit for bit, and tensors allocated after the growth are not written by it."""
+    from citrus_amd.scatter_solver.transport_sweep import SweepGraph, sweep_views
+
+    dev = gpu_device()
+    shape, G, V = (7, 6, 5), 2, 2
+    ctx = make_sweep_context(Grid(shape, 0.5), level_symmetric_quadrature(6), 3, device=dev)
+    gen = torch.Generator().manual_seed(37)
+    mu = (torch.rand(G, *shape, generator=gen) + 0.3).to(dev)
+    T = torch.randn(V, G, *shape, ctx.num_moments, 4, generator=gen).to(dev)
+    graph = SweepGraph(ctx, mu, T, direction_chunk=2, schedule=schedule)
+    want = sweep_views(ctx, mu, T, direction_chunk=2, schedule=schedule)
+    # eager sweeps that need larger scratch on the same schedule objects: all directions per chunk, more views
+    big = torch.randn(2 * V, G, *shape, ctx.num_moments, 4, generator=gen).to(dev)
+    sweep_views(ctx, mu, T, schedule=schedule)
+    sweep_views(ctx, mu, big, schedule="launches")
+    torch.cuda.synchronize()
+    junk = [torch.full((1 << 20,), 7.0, device=dev) for _ in range(8)]  # reuse of any freed block shows here
+    for _ in range(2):
+        got = graph.replay().clone()
+        torch.cuda.synchronize()
+        assert torch.equal(got, want)
+    assert all(bool((j == 7.0).all()) for j in junk)
+
+
 def test_sweep_autotuner_picks_and_caches_a_chunk():
     from citrus_amd.scatter_solver.autotune import SweepTuner, chunk_candidates, default_tuner
     from citrus_amd.scatter_solver.transport_sweep import sweep_views
