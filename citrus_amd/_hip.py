@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 11
+ABI_VERSION = 12
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -33,7 +33,7 @@ class SweepArgs(ctypes.Structure):
         ("accumulate", c_int), ("write_psi", c_int),
         ("mu_t", c_void), ("T", c_void), ("source", c_void), ("phi", c_void),
         ("Y", c_void), ("Yw", c_void), ("cdir", c_void), ("faces", c_void), ("source_layout", c_int),
-        ("T_bf16", c_void), ("T_inv_scale", c_void),
+        ("T_bf16", c_void), ("T_low", c_void),
     ]
 
 
@@ -41,7 +41,7 @@ class SweepView(ctypes.Structure):
     """Mirror of ``CitrusSweepView`` (include/citrus_hip.h)."""
 
     _fields_ = [
-        ("mu_t", c_void), ("T", c_void), ("T_bf16", c_void), ("T_inv_scale", c_void), ("source", c_void),
+        ("mu_t", c_void), ("T", c_void), ("T_bf16", c_void), ("T_low", c_void), ("source", c_void),
         ("phi", c_void), ("cdir", c_void),
     ]
 
@@ -118,8 +118,9 @@ _SIGNATURES = {
                                           c_void, c_void]),
     "citrus_uncollided_fluence": (c_int, [c_void, c_int, c_void, c_void, c_void, c_void, c_void, c_int, c_int,
                                           c_void, c_void, c_void, c_void, c_void, c_void, c_void, c_void]),
-    "citrus_bf16_operand_workspace_floats": (c_i64, [c_int, c_int]),
-    "citrus_bf16_operand": (c_int, [c_void, c_int, c_int, c_i64, c_int, c_void, c_void, c_void, c_void]),
+    "citrus_bf16_operand": (c_int, [c_void, c_int, c_int, c_i64, c_int, c_void, c_void, c_void]),
+    "citrus_scatter_moments_mixed": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_void, c_int,
+                                             c_int, c_i64, c_int, c_void, c_void]),
     "citrus_nn_conv3d_forward": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_void, c_int,
                                          c_int, c_int, c_void, c_i64, c_void]),
     "citrus_nn_conv3d_backward_data": (c_int, [c_void, c_i64, c_int, c_int, c_int, c_int, c_int, c_void, c_int, c_int,

@@ -20,7 +20,8 @@ from torch import Tensor
 from .scatter_solver.angular_quadrature import Quadrature
 from .scatter_solver.geometry import Grid
 from .scatter_solver.scatter_solver import MaterialFields, ScatterSolver
-from .scatter_solver.source_iteration import scatter_moments_views, summed_source_function
+from .scatter_solver.source_iteration import (MixedOperand, scatter_moments_mixed, scatter_moments_views,
+                                              summed_source_function)
 from .scatter_solver.transport_sweep import sweep_views
 
 
@@ -136,13 +137,15 @@ def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fiel
     batched in every launch (the reference loops over views one step at a time, citrus.py:1907-1970); then per view
     the signal-only image and the primary. Returns ([(scatter, primary or None)] per view, [new state] per view);
     each view's result equals its own ``citrus_view_step`` to float32 rounding. ``contraction="bf16"`` runs the
-    sweep's scattering-source expansion as a bfloat16 matrix product (float32 accumulation, per-group scales);
-    ``direction_chunk`` and ``schedule`` are passed to ``sweep_views`` ("auto": the autotuner's choice)."""
-    batch = _prepare_views(solver, states, fields, angles)
+    sweep's scattering-source expansion in mixed precision (degrees l <= 1 in float32, l >= 2 as a bfloat16 matrix
+    product, float32 accumulation): the scattering-moment kernel then writes that operand directly, and the next
+    step's sweep reads it; ``direction_chunk`` and ``schedule`` are passed to ``sweep_views`` ("auto": the
+    autotuner's choice)."""
+    batch = _prepare_views(solver, states, fields, angles, contraction)
     collided = sweep_views(batch.ctx, fields.mu_t_coarse, batch.T, source=batch.source, contraction=contraction,
                            direction_chunk=direction_chunk, schedule=schedule)
     batch.source = None
-    return _finish_views(solver, batch, collided, primary)
+    return _finish_views(solver, batch, collided, primary, contraction)
 
 
 @dataclass
@@ -160,7 +163,7 @@ class _ViewBatch:
 
 
 def _prepare_views(solver: ScatterSolver, states: Sequence[CitrusState], fields: MaterialFields,
-                   angles: Sequence[float]) -> _ViewBatch:
+                   angles: Sequence[float], contraction: str = "fp32") -> _ViewBatch:
     """Uncollided fluence, group-major first-collision source and scattering moments of the accumulators."""
     V = len(states)
     if V < 1 or len(angles) != V:
@@ -181,12 +184,14 @@ def _prepare_views(solver: ScatterSolver, states: Sequence[CitrusState], fields:
     for v, a in enumerate(angles):
         solver.first_collision_group_major(fields, a, phi_uc[v], ctx, out=source[v])
     T = _previous_scatter_moments(states, fields)
+    if isinstance(T, MixedOperand) and contraction != "bf16":
+        T = None  # the float32 contraction needs the float32 moments: formed again below
     if T is None:
         T = torch.empty(V, G, R, C, S, nsh, 4, device=device, dtype=torch.float32)
         for v, st in enumerate(states):
             if st.collided_moments is None:
                 T[v].zero_()
-            elif st.scatter_source is not None and st.scatter_source_fields is fields:
+            elif isinstance(st.scatter_source, Tensor) and st.scatter_source_fields is fields:
                 T[v].copy_(st.scatter_source)
             else:
                 _check_accumulator(solver, fields, st.collided_moments)
@@ -196,17 +201,22 @@ def _prepare_views(solver: ScatterSolver, states: Sequence[CitrusState], fields:
                       source=source, T=T)
 
 
-def _finish_views(solver: ScatterSolver, batch: _ViewBatch, collided: Tensor, primary: bool):
+def _finish_views(solver: ScatterSolver, batch: _ViewBatch, collided: Tensor, primary: bool,
+                  contraction: str = "fp32"):
     """Scattering moments of the new accumulators (kept for the next step), images and new states."""
     fields = batch.fields
-    # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same; its
-    # cell means also moment-major, the image's expansion operand
-    T, means = scatter_moments_views(fields.transfer_coarse, collided, cell_means=True)
+    # multiple-collision source of the images, kept for the next step's sweep while the fields stay the same (as
+    # the mixed-precision operand with the bfloat16 contraction); its cell means also moment-major, the image's
+    # expansion operand
+    if contraction == "bf16" and batch.ctx.num_moments in (16, 36):
+        T, means = scatter_moments_mixed(fields.transfer_coarse, collided, cell_means=True)
+    else:
+        T, means = scatter_moments_views(fields.transfer_coarse, collided, cell_means=True)
     results, news, norms = [], [], []
     for v, (st, a) in enumerate(zip(batch.states, batch.angles)):
         new = collided[v].permute(0, 4, 5, 1, 2, 3)
-        image = solver.scatter_image(fields, a, batch.phi_uc[v], T[v].permute(0, 4, 5, 1, 2, 3)[:, :, 0],
-                                     batch.cones[v], signal_only=True, source_means=means[v])["signal"]
+        image = solver.scatter_image(fields, a, batch.phi_uc[v], means[v].transpose(0, 1), batch.cones[v],
+                                     signal_only=True, source_means=means[v])["signal"]
         primary_signal = solver.primary(fields, a)["signal"] if primary else None
         results.append((image, primary_signal))
         news.append(new)
@@ -219,7 +229,8 @@ def _finish_views(solver: ScatterSolver, batch: _ViewBatch, collided: Tensor, pr
     flat = decode_status(fold_status(torch.stack(norms)).tolist(), collided.device)
     changes = [_change_from_norms(flat[2 * v:2 * v + 2], st) for v, st in enumerate(batch.states)]
     new_states = [CitrusState(angle=float(a), collided_moments=new, cone_quadrature=batch.cones[v],
-                              num_steps=st.num_steps + 1, history=st.history + [c], scatter_source=T[v],
+                              num_steps=st.num_steps + 1, history=st.history + [c],
+                              scatter_source=T.view(v) if isinstance(T, MixedOperand) else T[v],
                               scatter_source_fields=fields, _batch=(T, v))
                   for v, (st, a, new, c) in enumerate(zip(batch.states, batch.angles, news, changes))]
     return results, new_states
@@ -235,7 +246,7 @@ def citrus_grids_step(batches, *, primary: bool = True, contraction: str = "fp32
     from .scatter_solver.kernels.sweep_kernels import PERSISTENT_MOMENTS, persistent_usable
     from .scatter_solver.transport_sweep import sweep_grids
 
-    prepared = [(solver, _prepare_views(solver, states, fields, angles))
+    prepared = [(solver, _prepare_views(solver, states, fields, angles, contraction))
                 for solver, states, fields, angles in batches]
     if not prepared:
         return [], []
@@ -254,7 +265,7 @@ def citrus_grids_step(batches, *, primary: bool = True, contraction: str = "fp32
     out_results, out_states = [], []
     for (solver, b), phi in zip(prepared, collided):
         b.source = None
-        r, s = _finish_views(solver, b, phi, primary)
+        r, s = _finish_views(solver, b, phi, primary, contraction)
         out_results.append(r)
         out_states.append(s)
     return out_results, out_states
@@ -264,7 +275,10 @@ def _previous_scatter_moments(states: Sequence[CitrusState], fields: MaterialFie
     """The [V, ...] scattering moments of the previous batched step when every state still holds its slot of it, in
     order, for these same fields (no copy); else None."""
     batch = states[0]._batch
-    if batch is None or int(batch[0].shape[0]) != len(states):
+    if batch is None:
+        return None
+    views = batch[0].num_views if isinstance(batch[0], MixedOperand) else int(batch[0].shape[0])
+    if views != len(states):
         return None
     for v, st in enumerate(states):
         same = st._batch is not None and st._batch[0] is batch[0] and st._batch[1] == v

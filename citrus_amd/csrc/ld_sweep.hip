@@ -55,8 +55,8 @@ struct LaunchArgs {
   const float* cdir;
   float* faces;
   int64_t tview, sview, fview;
-  const uint16_t* T16;     // bfloat16 operand of the source expansion [V][G][N][4][KP] (or NULL)
-  const float* tinv;       // [V][G] scale undoing the operand's per-group power of two
+  const uint16_t* T16;     // mixed operand: bfloat16 degrees l >= 2 [V][G][N][4][KH] (or NULL)
+  const float* Tlo;        // mixed operand: float32 degrees l <= 1 [V][G][N][4][4]
   int64_t t16view;         // elements per view of T16
 };
 
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
 template <int NSH, int DC4, int SRC, bool BF16>
 __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   extern __shared__ float lds[];
-  constexpr int KP = 16 * ((NSH + 15) / 16);
+  constexpr int KH = 16 * ((NSH - 4 + 15) / 16);
   const int task = blockIdx.x % a.ntask;
   const int rest = blockIdx.x / a.ntask;
   const int g = rest % a.G;
@@ -361,11 +361,11 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   t.gface = a.faces + (int64_t)v * a.fview + ((int64_t)slot * a.G + g) * a.nblocks * 3 * (48 * a.dc);
   t.mu = a.mu + (int64_t)g * a.N;
   if (BF16) {
-    t.T = a.T16 + (int64_t)v * a.t16view + (int64_t)g * a.N * 4 * KP;
-    t.tinv = a.tinv[v * a.G + g];
+    t.T = a.T16 + (int64_t)v * a.t16view + (int64_t)g * a.N * 4 * KH;
+    t.Tlo = a.Tlo + ((int64_t)v * a.G + g) * a.N * 16;
   } else {
     t.T = a.T + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
-    t.tinv = 1.0f;
+    t.Tlo = nullptr;
   }
   t.phi = a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
   t.src = SRC != 0 ? a.src + (int64_t)v * a.sview + (int64_t)g * a.N * a.M * (SRC == 2 ? 1 : 4) : nullptr;
@@ -586,7 +586,7 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
   if (!A.T && !A.T_bf16 && !A.source) return fail(CITRUS_ERR_ARG, "need T moments and/or an explicit source");
   if (A.write_psi && !A.source) return fail(CITRUS_ERR_ARG, "write_psi needs the explicit source buffer");
   if (A.source_layout < 0 || A.source_layout > 2) return fail(CITRUS_ERR_ARG, "source_layout must be 0, 1 or 2");
-  if (A.T_bf16 && (!A.T_inv_scale || A.T)) return fail(CITRUS_ERR_ARG, "T_bf16 needs T_inv_scale and no float32 T");
+  if (A.T_bf16 && (!A.T_low || A.T)) return fail(CITRUS_ERR_ARG, "T_bf16 needs T_low and no float32 T");
   if (A.T_bf16 && A.nsh != 16 && A.nsh != 36)
     return fail(CITRUS_ERR_UNSUPPORTED, "the bfloat16 contraction is built for P3 and P5 (nsh 16 and 36)");
   if (A.source_layout != 0 && A.write_psi) return fail(CITRUS_ERR_ARG, "write_psi needs the cell-major layout");
@@ -605,8 +605,8 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
                                    : (int64_t)A.G * N * la.M * (A.source_layout == 1 ? 4 : 1);
   la.fview = (int64_t)2 * A.G * plan->nblocks * 3 * (int64_t)A.dc * plan->B * plan->B * 3;
   la.T16 = reinterpret_cast<const uint16_t*>(A.T_bf16);
-  la.tinv = A.T_inv_scale;
-  la.t16view = (int64_t)A.G * N * 4 * ((A.nsh + 15) / 16 * 16);
+  la.Tlo = A.T_low;
+  la.t16view = (int64_t)A.G * N * 4 * ((A.nsh - 4 + 15) / 16 * 16);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   la.ntask = plan->launches.empty() ? 0 : plan->launches[0].second;
   const bool mfma = use_mfma(la, A.nsh);

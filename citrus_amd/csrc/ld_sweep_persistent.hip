@@ -263,8 +263,8 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
     const CitrusSweepView& w = A.views[v];
     PersistentView pv = plan->shapes[v];
     if (!w.mu_t || !w.phi || !w.cdir) return fail(CITRUS_ERR_ARG, "view " + std::to_string(v) + ": NULL buffer");
-    if ((w.T_bf16 != nullptr) != bf16 || (bf16 ? (!w.T_inv_scale || w.T) : !w.T))
-      return fail(CITRUS_ERR_ARG, "every view needs float32 T, or every view the bfloat16 operand and its scales");
+    if ((w.T_bf16 != nullptr) != bf16 || (bf16 ? (!w.T_low || w.T) : !w.T))
+      return fail(CITRUS_ERR_ARG, "every view needs float32 T, or every view the mixed operand (T_bf16 and T_low)");
     if ((A.source_layout < 0) != (w.source == nullptr))
       return fail(CITRUS_ERR_ARG, "view " + std::to_string(v) + ": source does not match source_layout");
     if ((int64_t)pv.N * A.nsh * 16 >= (int64_t)kDrop || (int64_t)pv.N * pa.M * 4 * words >= (int64_t)kDrop)
@@ -272,7 +272,7 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
     pv.face_off *= A.dc;
     pv.mu = w.mu_t;
     pv.T = bf16 ? w.T_bf16 : static_cast<const void*>(w.T);
-    pv.tinv = w.T_inv_scale;
+    pv.Tlo = w.T_low;
     pv.src = w.source;
     pv.phi = w.phi;
     pv.cdir = w.cdir;

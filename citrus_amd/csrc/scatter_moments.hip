@@ -31,11 +31,19 @@ __device__ __forceinline__ int degree_of(int n) {
 template <int GMAX>
 constexpr int degree_stride() { return GMAX * GMAX + 1; }
 
-// LDS: mu [kCells][L+1][GMAX][GMAX] (degree stride padded), then the material table [M][G][G][Lt+1] (factored form)
-template <int GMAX, bool FACTORED>
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+// LDS: mu [kCells][L+1][GMAX][GMAX] (degree stride padded), then the material table [M][G][G][Lt+1] (factored form).
+// MIXED: the result goes out as the mixed-precision operand of the sweep instead of float32 T: degrees l <= 1 to
+// T (float32 [V][G][N][4 comp][4]), degrees l >= 2 to T16 (bfloat16 [V][G][N][4 comp][kh], zero padded).
+template <int GMAX, bool FACTORED, bool MIXED>
 __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
     const float* __restrict__ transfer, const float* __restrict__ rho, int M, int Lt, const float* __restrict__ phi,
-    float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only, float* __restrict__ means) {
+    float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only, float* __restrict__ means,
+    uint16_t* __restrict__ T16, int kh) {
   extern __shared__ float lds[];
   constexpr int DS = degree_stride<GMAX>();
   const int per_cell = (L + 1) * DS + 1;  // odd: the two cells' matrices also fall in different banks
@@ -84,6 +92,7 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
   const int cell = c0 + cc;
   if (cc >= kCells || cell >= N) return;
   const int l = degree_of(r / 4);
+  const int n = r >> 2, k = r & 3;
   const int64_t view_stride = (int64_t)G * N * elems;
   const int64_t group_stride = (int64_t)N * elems;
   const int64_t off = (int64_t)v * view_stride + (int64_t)cell * elems + r;
@@ -98,7 +107,17 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
 #pragma unroll
     for (int gi = 0; gi < GMAX; ++gi)
       if (gi < G && !(lower_only && gi > go)) acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
-    T[off + go * group_stride] = acc;
+    if (MIXED) {
+      const int64_t row = ((((int64_t)v * G + go) * N + cell) * 4 + k);  // (view, group, cell, component)
+      if (n < 4) {
+        T[row * 4 + n] = acc;
+      } else {
+        T16[row * kh + (n - 4)] = bf16_rne(acc);
+        if (n - 4 < kh - (nsh - 4)) T16[row * kh + (nsh - 4) + (n - 4)] = 0;  // padding of the row
+      }
+    } else {
+      T[off + go * group_stride] = acc;
+    }
     // the cell means again, moment-major [V][nsh][G][N]: the operand of the image's multiple-collision expansion
     if (means && (r & 3) == 0) means[(((int64_t)v * nsh + (r >> 2)) * G + go) * N + cell] = acc;
   }
@@ -114,9 +133,9 @@ int check(int G, int nsh, int Lt, int64_t N, int V, int& L) {
   return CITRUS_OK;
 }
 
-template <bool FACTORED>
+template <bool FACTORED, bool MIXED = false>
 int launch(const float* transfer, const float* rho, int M, int Lt, const float* phi, float* T, int G, int nsh,
-           int64_t N, int V, int lower_only, float* means, void* stream) {
+           int64_t N, int V, int lower_only, float* means, void* stream, uint16_t* T16 = nullptr) {
   int L = 0;
   if (int rc = check(G, nsh, Lt, N, V, L)) return rc;
   const int gmax = G <= 8 ? 8 : (G <= 16 ? 16 : 32);
@@ -126,105 +145,77 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
   const int threads = kCells * nsh * 4;
   const int64_t blocks = (int64_t)V * ((N + kCells - 1) / kCells);
   if (blocks > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many cells");
+  const int kh = MIXED ? 16 * ((nsh - 4 + 15) / 16) : 0;
+  if (MIXED && (nsh < 16 || kh - (nsh - 4) > nsh - 4))
+    return fail(CITRUS_ERR_UNSUPPORTED, "the mixed-precision operand is built for P3 and above");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)blocks), block((unsigned)threads);
   if (G <= 8)
-    hipLaunchKernelGGL((scatter_moments_kernel<8, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
-                       nsh, L, (int)N, V, lower_only, means);
+    hipLaunchKernelGGL((scatter_moments_kernel<8, FACTORED, MIXED>), grid, block, lds, s, transfer, rho, M, Lt, phi,
+                       T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
   else if (G <= 16)
-    hipLaunchKernelGGL((scatter_moments_kernel<16, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
-                       nsh, L, (int)N, V, lower_only, means);
+    hipLaunchKernelGGL((scatter_moments_kernel<16, FACTORED, MIXED>), grid, block, lds, s, transfer, rho, M, Lt, phi,
+                       T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
   else
-    hipLaunchKernelGGL((scatter_moments_kernel<32, FACTORED>), grid, block, lds, s, transfer, rho, M, Lt, phi, T, G,
-                       nsh, L, (int)N, V, lower_only, means);
+    hipLaunchKernelGGL((scatter_moments_kernel<32, FACTORED, MIXED>), grid, block, lds, s, transfer, rho, M, Lt, phi,
+                       T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("scatter moments: ") + hipGetErrorString(e));
   return CITRUS_OK;
 }
 
-// ---- bfloat16 operand of the source contraction -------------------------------------------------------------------
-// T [V][G][N][nsh][4] float32 -> T16 [V][G][N][4][KP] bfloat16 (KP = nsh rounded up to 16, zero padded), each
-// (view, group) scaled by a power of two 2^-e with max|T| in [2^e, 2^(e+1)), so every group's largest value is in
-// [1, 2); inv_scale[v][g] = 2^e undoes it after the contraction.
+// ---- mixed-precision operand of the source contraction ----------------------------------------------------------
+// T [V][G][N][nsh][4] float32 -> T_low [V][G][N][4][4] float32 (degrees l <= 1, component-major) and
+// T16 [V][G][N][4][kh] bfloat16 (degrees l >= 2, round to nearest even, zero padded to kh = nsh - 4 rounded up to 16).
+// One thread per (view, group, cell, component, slot), slot < 4 the float32 part.
 
-constexpr int kAmaxBlock = 256;
-
-__global__ __launch_bounds__(kAmaxBlock) void group_amax_kernel(const float* __restrict__ T, int64_t per_group,
-                                                                int blocks_per_group, float* __restrict__ partial) {
-  __shared__ float red[kAmaxBlock];
-  const int vg = blockIdx.y, b = blockIdx.x;
-  const int64_t chunk = (per_group + blocks_per_group - 1) / blocks_per_group;
-  const int64_t lo = (int64_t)b * chunk, hi = lo + chunk < per_group ? lo + chunk : per_group;
-  const float* src = T + (int64_t)vg * per_group;
-  float m = 0.0f;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kAmaxBlock) m = fmaxf(m, fabsf(src[i]));
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = kAmaxBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
+__global__ void mixed_operand_kernel(const float* __restrict__ T, int nsh, int kh, int64_t rows,
+                                     float* __restrict__ T_low, uint16_t* __restrict__ T16) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int width = 4 + kh;
+  if (i >= rows * width) return;
+  const int64_t row = i / width;  // (view, group, cell) * 4 + component
+  const int slot = (int)(i - row * width);
+  const int comp = (int)(row & 3);
+  const int64_t cell = row >> 2;
+  if (slot < 4) {
+    T_low[row * 4 + slot] = T[(cell * nsh + slot) * 4 + comp];
+  } else {
+    const int n = slot;
+    T16[row * kh + (slot - 4)] = n < nsh ? bf16_rne(T[(cell * nsh + n) * 4 + comp]) : (uint16_t)0;
   }
-  if (threadIdx.x == 0) partial[(int64_t)vg * blocks_per_group + b] = red[0];
-}
-
-__global__ void group_scale_kernel(const float* __restrict__ partial, int blocks_per_group, int n,
-                                   float* __restrict__ scale, float* __restrict__ inv_scale) {
-  const int vg = blockIdx.x * blockDim.x + threadIdx.x;
-  if (vg >= n) return;
-  float m = 0.0f;
-  for (int b = 0; b < blocks_per_group; ++b) m = fmaxf(m, partial[(int64_t)vg * blocks_per_group + b]);
-  const int e = m > 0.0f ? ilogbf(m) : 0;
-  scale[vg] = ldexpf(1.0f, -e);
-  inv_scale[vg] = ldexpf(1.0f, e);
-}
-
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-  const uint32_t b = __float_as_uint(f);
-  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
-}
-
-__global__ void to_bf16_operand_kernel(const float* __restrict__ T, int nsh, int kp, int64_t N, int64_t n_vg,
-                                       const float* __restrict__ scale, uint16_t* __restrict__ T16) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (vg, cell, comp, n), n fastest
-  const int64_t total = n_vg * N * 4 * kp;
-  if (i >= total) return;
-  const int n = (int)(i % kp);
-  const int comp = (int)((i / kp) % 4);
-  const int64_t cell_vg = i / (4 * (int64_t)kp);  // vg * N + cell
-  const int64_t vg = cell_vg / N;
-  T16[i] = n < nsh ? bf16_rne(T[(cell_vg * nsh + n) * 4 + comp] * scale[vg]) : (uint16_t)0;
 }
 
 }  // namespace
 }  // namespace citrus
 
-extern "C" int64_t citrus_bf16_operand_workspace_floats(int G, int V) {
-  if (G < 1 || V < 1) return -1;
-  return (int64_t)G * V * (64 + 1);
+extern "C" int citrus_bf16_operand(const float* T, int G, int nsh, int64_t N, int V, float* T_low, uint16_t* T16,
+                                   void* stream) {
+  using namespace citrus;
+  if (!T || !T16 || !T_low) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (G < 1 || V < 1 || N < 1) return fail(CITRUS_ERR_ARG, "bad sizes");
+  if (nsh != 16 && nsh != 36) return fail(CITRUS_ERR_UNSUPPORTED, "the mixed operand is built for P3 and P5");
+  const int kh = 16 * ((nsh - 4 + 15) / 16);
+  const int64_t rows = (int64_t)V * G * N * 4;
+  const int64_t total = rows * (4 + kh);
+  if ((total + 255) / 256 > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "operand too large");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(mixed_operand_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, T, nsh, kh, rows,
+                     T_low, T16);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("mixed operand: ") + hipGetErrorString(e));
+  return CITRUS_OK;
 }
 
-extern "C" int citrus_bf16_operand(const float* T, int G, int nsh, int64_t N, int V, uint16_t* T16, float* inv_scale,
-                                   float* workspace, void* stream) {
+extern "C" int citrus_scatter_moments_mixed(const float* rho, int num_materials, const float* sigma, int Lt,
+                                            int downscatter_only, const float* phi, float* T_low, void* T_bf16, int G,
+                                            int nsh, int64_t N, int V, float* cell_means, void* stream) {
   using namespace citrus;
-  if (!T || !T16 || !inv_scale || !workspace) return fail(CITRUS_ERR_ARG, "NULL buffer");
-  if (G < 1 || V < 1 || N < 1 || nsh < 1 || nsh > 64) return fail(CITRUS_ERR_ARG, "bad sizes");
-  const int kp = (nsh + 15) / 16 * 16;
-  const int blocks_per_group = 64;
-  const int n_vg = G * V;
-  float* partial = workspace;
-  float* scale = workspace + (int64_t)n_vg * blocks_per_group;
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(group_amax_kernel, dim3(blocks_per_group, n_vg), dim3(kAmaxBlock), 0, s, T, N * nsh * 4,
-                     blocks_per_group, partial);
-  hipLaunchKernelGGL(group_scale_kernel, dim3((n_vg + 63) / 64), dim3(64), 0, s, partial, blocks_per_group, n_vg,
-                     scale, inv_scale);
-  const int64_t total = (int64_t)n_vg * N * 4 * kp;
-  if ((total + 255) / 256 > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "operand too large");
-  hipLaunchKernelGGL(to_bf16_operand_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, T, nsh, kp, N,
-                     (int64_t)n_vg, scale, T16);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("bf16 operand: ") + hipGetErrorString(e));
-  return CITRUS_OK;
+  if (!rho || !sigma || !phi || !T_low || !T_bf16) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (num_materials < 1 || num_materials > kMaxMaterials) return fail(CITRUS_ERR_ARG, "1..8 materials supported");
+  if (nsh != 16 && nsh != 36) return fail(CITRUS_ERR_UNSUPPORTED, "the mixed operand is built for P3 and P5");
+  return launch<true, true>(sigma, rho, num_materials, Lt, phi, T_low, G, nsh, N, V, downscatter_only ? 1 : 0,
+                            cell_means, stream, static_cast<uint16_t*>(T_bf16));
 }
 
 extern "C" int citrus_scatter_moments_dense(const float* transfer, int Lt, const float* phi, float* T, int G, int nsh,

@@ -134,8 +134,9 @@ struct MfmaTask {
   int first_visit;        // the block's first visit of the sweep: phi is stored instead of added to
   float* gface;           // inter-block faces of the (view, slot, group): [blocks][3 axes][16 positions][3][dc]
   const float* mu;        // total attenuation of the group [N]
-  const void* T;          // moments of the (view, group): float32 [N][NSH][4] or the bfloat16 operand [N][4][KP]
-  float tinv;             // scale undoing the bfloat16 operand's power of two (1 for float32 moments)
+  const void* T;          // moments of the (view, group): float32 [N][NSH][4], or (mixed operand) the bfloat16
+                          // degrees l >= 2 [N][4][KH]
+  const float* Tlo;       // mixed operand: the float32 degrees l <= 1, [N][4][4] (component-major), else unused
   float* phi;             // fluence moments of the (view, group) [N][NSH][4]
   const float* src;       // explicit source of the (view, group): [N][M][4] (SRC 1) or [N][M] (SRC 2), or NULL
   const float* cdir;      // [8 D8][3] c_a = |Omega_a| / Delta_a on the view's grid
@@ -160,9 +161,10 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
   constexpr PlaneTable<B> ptab{};
   constexpr int NPL = PlaneTable<B>::kPlanes;
   constexpr int NT = (DC4 + 3) / 4;     // direction tiles of the expansion
-  constexpr int KS = BF16 ? 1 : (NSH + 3) / 4;  // k-steps of the float32 expansion
-  constexpr int KB = BF16 ? (NSH + 15) / 16 : 1;  // k-steps of the bfloat16 expansion (16 moments each)
-  constexpr int KP = 16 * ((NSH + 15) / 16);      // moments per row of the bfloat16 operand
+  // float32 k-steps (4 moments each): every moment, or with the mixed operand the degrees l <= 1 (moments 0..3)
+  constexpr int KS = BF16 ? 1 : (NSH + 3) / 4;
+  constexpr int KB = BF16 ? (NSH - 4 + 15) / 16 : 1;  // bfloat16 k-steps (16 moments each) of the degrees l >= 2
+  constexpr int KH = 16 * KB;                         // moments per row of the bfloat16 part (n = 4 .. 4 + KH)
   constexpr int NT2 = (NSH + 15) / 16;  // moment tiles of the reduction
   constexpr int PS = kMfmaPsiStride;
   constexpr unsigned CB = NSH * 16;     // bytes of one cell's moments
@@ -193,8 +195,9 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
   const __amdgpu_buffer_rsrc_t mu_r =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.mu), 0, a.N * 4, kBufferWord3);
   const __amdgpu_buffer_rsrc_t T_r =
-      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.T), 0, BF16 ? a.N * 8 * KP : a.N * (int)CB, kBufferWord3);
-  const float tinv = a.tinv;
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.T), 0, BF16 ? a.N * 8 * KH : a.N * (int)CB, kBufferWord3);
+  const __amdgpu_buffer_rsrc_t Tlo_r = __builtin_amdgcn_make_buffer_rsrc(
+      BF16 ? const_cast<float*>(a.Tlo) : const_cast<void*>(a.T), 0, BF16 ? a.N * 64 : 0, kBufferWord3);
   const __amdgpu_buffer_rsrc_t phi_r = __builtin_amdgcn_make_buffer_rsrc(a.phi, 0, a.N * (int)CB, kBufferWord3);
   // explicit source, group-major [N][M][4] (layout 1: the 16 directions of one cell in a D tile are 256 contiguous
   // bytes) or cell means [N][M] (layout 2, zero slopes: 64 contiguous bytes)
@@ -247,25 +250,24 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
     }
   }
   const int mbase = o * dt.D8 + dt.d0;
-  float yf[NT][KS];  // expansion B fragments: Y[d = 16 nt + l%16][n = 4 s + l/16]
-  s4 yb[NT][KB];     // bfloat16 expansion B fragments: Y[d = 16 nt + l%16][n = 16 s + 4 (l/16) + 0..3]
+  float yf[NT][KS];  // float32 expansion B fragments: Y[d = 16 nt + l%16][n = 4 s + l/16]
+  s4 yb[NT][KB];     // bfloat16 expansion B fragments: Y[d = 16 nt + l%16][n = 4 + 16 s + 4 (l/16) + 0..3]
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
     const int d = nt * 16 + r16;
     const int dd = d < dc ? d : dc - 1;
-    if (!BF16) {
 #pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        const int n = 4 * s + q;
-        const float val = dt.Y[(int64_t)(mbase + dd) * NSH + (n < NSH ? n : NSH - 1)];
-        yf[nt][s] = (d < dc && n < NSH) ? val : 0.0f;
-      }
-    } else {
+    for (int s = 0; s < KS; ++s) {
+      const int n = 4 * s + q;
+      const float val = dt.Y[(int64_t)(mbase + dd) * NSH + (n < NSH ? n : NSH - 1)];
+      yf[nt][s] = (d < dc && n < NSH) ? val : 0.0f;
+    }
+    if (BF16) {
 #pragma unroll
       for (int s = 0; s < KB; ++s) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int n = 16 * s + 4 * q + j;
+          const int n = 4 + 16 * s + 4 * q + j;
           const float val = dt.Y[(int64_t)(mbase + dd) * NSH + (n < NSH ? n : NSH - 1)];
           yb[nt][s][j] = bf16_bits((d < dc && n < NSH) ? val : 0.0f);
         }
@@ -317,8 +319,12 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
           const int n = 4 * s + q;
           st.t[mt][s] = buf_load(T_r, n < NSH ? ta + 16u * n : kDrop);
         }
-      } else {  // row (cell, component): KP contiguous bfloat16 moments, four per lane and k-step
-        const unsigned tb = ca >= 0 ? ((unsigned)ca * 4u + (unsigned)acomp) * (2u * KP) + 8u * q : kDrop;
+      } else {
+        // row (cell, component): its 4 float32 moments of degree <= 1 (one per lane) and its KH bfloat16 moments of
+        // degree >= 2 (four per lane and k-step)
+        const unsigned row = (unsigned)ca * 4u + (unsigned)acomp;
+        st.t[mt][0] = buf_load(Tlo_r, ca >= 0 ? row * 16u + 4u * q : kDrop);
+        const unsigned tb = ca >= 0 ? row * (2u * KH) + 8u * q : kDrop;
 #pragma unroll
         for (int s = 0; s < KB; ++s) st.tb[mt][s] = buf_load_s4(T_r, ca >= 0 ? tb + 32u * s : kDrop);
       }
@@ -383,13 +389,11 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
         sacc[nt] = f4{0.f, 0.f, 0.f, 0.f};
-        if (!BF16) {
 #pragma unroll
-          for (int s = 0; s < KS; ++s) sacc[nt] = mfma4(cur.t[mt][s], yf[nt][s], sacc[nt]);
-        } else {
+        for (int s = 0; s < KS; ++s) sacc[nt] = mfma4(cur.t[mt][s], yf[nt][s], sacc[nt]);
+        if (BF16) {
 #pragma unroll
           for (int s = 0; s < KB; ++s) sacc[nt] = mfma_bf16(cur.tb[mt][s], yb[nt][s], sacc[nt]);
-          sacc[nt] *= tinv;
         }
       }
       // (2) closed-form LD solve of (cell 4 mt + l/16, direction 16 nt + l%16)

@@ -47,8 +47,8 @@ struct PersistentView {
   int64_t face_off;   // floats into the face workspace: [2 slots][G][blocks][3][48 dc]
   int64_t flag_off;   // ints into the flag array: [G][8 octants][blocks]
   const float* mu;    // [G][N]
-  const void* T;      // [G][N][NSH][4] float32, or the bfloat16 operand [G][N][4][KP]
-  const float* tinv;  // [G] with the bfloat16 operand
+  const void* T;      // [G][N][NSH][4] float32, or the mixed operand's bfloat16 degrees l >= 2 [G][N][4][KH]
+  const float* Tlo;   // the mixed operand's float32 degrees l <= 1 [G][N][4][4]
   const float* src;   // [G][N][M] or [G][N][M][4], or NULL
   float* phi;         // [G][N][NSH][4]
   const float* cdir;  // [8 D8][3]
@@ -149,7 +149,7 @@ template <int NSH, int DC4, int SRC, bool BF16>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void ld_sweep_persistent_kernel(
     PersistentArgs a) {
   extern __shared__ float lds[];
-  constexpr int KP = 16 * ((NSH + 15) / 16);
+  constexpr int KH = 16 * ((NSH - 4 + 15) / 16);
   const int lane = threadIdx.x;
   int* status = a.ctrl + kStatusWord;
   const int xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15;  // HW_REG_XCC_ID, bits [3:0]
@@ -191,11 +191,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     t.gface = a.faces + pv.face_off + ((int64_t)which * a.G + g) * nb * 3 * face_stride;
     t.mu = pv.mu + (int64_t)g * pv.N;
     if (BF16) {
-      t.T = static_cast<const uint16_t*>(pv.T) + (int64_t)g * pv.N * 4 * KP;
-      t.tinv = pv.tinv[g];
+      t.T = static_cast<const uint16_t*>(pv.T) + (int64_t)g * pv.N * 4 * KH;
+      t.Tlo = pv.Tlo + (int64_t)g * pv.N * 16;
     } else {
       t.T = static_cast<const float*>(pv.T) + (int64_t)g * pv.N * NSH * 4;
-      t.tinv = 1.0f;
+      t.Tlo = nullptr;
     }
     t.phi = pv.phi + (int64_t)g * pv.N * NSH * 4;
     t.src = SRC != 0 ? pv.src + (int64_t)g * pv.N * a.M * (SRC == 2 ? 1 : 4) : nullptr;

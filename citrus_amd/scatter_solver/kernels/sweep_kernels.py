@@ -75,9 +75,10 @@ class SweepSchedule:
     def run(self, *, mu: Tensor, phi: Tensor, Y: Tensor, Yw: Tensor, cdir: Tensor, G: int, nsh: int, D8: int,
             d0: int, dc: int, V: int = 1, T: Optional[Tensor] = None, source: Optional[Tensor] = None,
             accumulate: bool = False, write_psi: bool = False, source_layout: int = 0,
-            T16: Optional[Tensor] = None, T_inv_scale: Optional[Tensor] = None) -> None:
-        """Enqueue one chunk sweep on torch's current stream (no host synchronization). ``T16``/``T_inv_scale`` (the
-        bfloat16 operand of ``bf16_operand``) replace ``T`` for the bfloat16 source contraction."""
+            T16: Optional[Tensor] = None, T_low: Optional[Tensor] = None) -> None:
+        """Enqueue one chunk sweep on torch's current stream (no host synchronization). ``T16``/``T_low`` (the two
+        parts of the mixed-precision operand, ``MixedOperand.high`` / ``.low``) replace ``T`` for the
+        mixed-precision source contraction."""
         N = self.shape[0] * self.shape[1] * self.shape[2]
         for name, t in (("mu", mu), ("phi", phi), ("Y", Y), ("Yw", Yw), ("cdir", cdir)):
             _hip.require_gpu_tensor(t, name)
@@ -88,15 +89,15 @@ class SweepSchedule:
             if T.numel() != V * G * N * nsh * 4:
                 raise ValueError("T size does not match [V, G, N, nsh, 4]")
         if T16 is not None:
-            kp = (nsh + 15) // 16 * 16
-            if T is not None or T_inv_scale is None:
-                raise ValueError("T16 replaces T and needs T_inv_scale")
+            kh = 16 * ((nsh - 4 + 15) // 16)
+            if T is not None or T_low is None:
+                raise ValueError("T16 replaces T and needs T_low")
             if T16.dtype != torch.bfloat16 or T16.device.type != "cuda" or not T16.is_contiguous() \
-                    or T16.numel() != V * G * N * 4 * kp:
-                raise ValueError(f"T16 must be contiguous bfloat16 [V, G, N, 4, {kp}] on the GPU")
-            _hip.require_gpu_tensor(T_inv_scale, "T_inv_scale")
-            if T_inv_scale.numel() != V * G:
-                raise ValueError("T_inv_scale must hold V * G scales")
+                    or T16.numel() != V * G * N * 4 * kh:
+                raise ValueError(f"T16 must be contiguous bfloat16 [V, G, N, 4, {kh}] on the GPU")
+            _hip.require_gpu_tensor(T_low, "T_low")
+            if T_low.numel() != V * G * N * 16:
+                raise ValueError("T_low must hold [V, G, N, 4, 4] floats")
         if source is not None:
             _hip.require_gpu_tensor(source, "source")
             if source_layout == 0 and source.numel() != V * 8 * dc * N * G * 4:
@@ -116,7 +117,7 @@ class SweepSchedule:
                               write_psi=int(write_psi), mu_t=mu.data_ptr(), T=_hip.ptr(T), source=_hip.ptr(source),
                               phi=phi.data_ptr(), Y=Y.data_ptr(), Yw=Yw.data_ptr(), cdir=cdir.data_ptr(),
                               faces=faces.data_ptr(), source_layout=int(source_layout), T_bf16=_hip.ptr(T16),
-                              T_inv_scale=_hip.ptr(T_inv_scale))
+                              T_low=_hip.ptr(T_low))
         lib = _hip.library()
         with torch.cuda.device(self.device):
             _hip.check(lib.citrus_sweep_run(self.handle, ctypes.byref(args), _hip.stream_of(self.device)),
@@ -176,7 +177,8 @@ class PersistentSchedule:
             source_layout: int = -1, spin_limit: int = 0) -> None:
         """Enqueue one chunk sweep of every view on torch's current stream. ``views`` holds one dict per view of the
         plan with the device tensors ``mu`` [G, N], ``phi`` [G, N, nsh, 4], ``cdir`` [8 D8, 3], and either ``T``
-        [G, N, nsh, 4] or ``T16`` [G, N, 4, KP] with ``T_inv_scale`` [G], and optionally ``source`` (group-major, the
+        [G, N, nsh, 4] or the mixed operand ``T16`` [G, N, 4, KH] with ``T_low`` [G, N, 4, 4], and optionally
+        ``source`` (group-major, the
         same ``source_layout`` 1 or 2 for every view; -1 without). A sweep that aborts ORs its status code into the
         device's status word (``persistent_status_sink``), which the callers' next host read reports
         (``decode_status``); ``spin_limit`` > 0 replaces the default bound of each dependency wait."""
@@ -186,7 +188,7 @@ class PersistentSchedule:
             raise ValueError(f"the persistent sweep is built for {PERSISTENT_MOMENTS} moments, got {nsh}")
         if source_layout not in (-1, 1, 2):
             raise ValueError("source_layout must be -1, 1 or 2")
-        G, M, kp = self.G, 8 * D8, (nsh + 15) // 16 * 16
+        G, M, kh = self.G, 8 * D8, 16 * ((nsh - 4 + 15) // 16)
         arr = (_hip.SweepView * len(views))()
         for v, (spec, shape) in enumerate(zip(views, self.shapes)):
             N = shape[0] * shape[1] * shape[2]
@@ -196,12 +198,13 @@ class PersistentSchedule:
                 raise ValueError(f"views[{v}]: mu or phi size does not match the grid {shape}")
             if tuple(spec["cdir"].shape) != (M, 3):
                 raise ValueError(f"views[{v}]: cdir must be [{M}, 3]")
-            T, T16, tinv, src = spec.get("T"), spec.get("T16"), spec.get("T_inv_scale"), spec.get("source")
+            T, T16, tlo, src = spec.get("T"), spec.get("T16"), spec.get("T_low"), spec.get("source")
             if T16 is not None:
-                if T is not None or tinv is None or T16.dtype != torch.bfloat16 or not T16.is_contiguous() \
-                        or T16.numel() != G * N * 4 * kp or tinv.numel() != G:
-                    raise ValueError(f"views[{v}]: T16 must be contiguous bfloat16 [G, N, 4, {kp}] with G scales")
-                _hip.require_gpu_tensor(tinv, f"views[{v}].T_inv_scale")
+                if T is not None or tlo is None or T16.dtype != torch.bfloat16 or not T16.is_contiguous() \
+                        or T16.numel() != G * N * 4 * kh or tlo.numel() != G * N * 16:
+                    raise ValueError(f"views[{v}]: T16 must be contiguous bfloat16 [G, N, 4, {kh}] with T_low "
+                                     "[G, N, 4, 4]")
+                _hip.require_gpu_tensor(tlo, f"views[{v}].T_low")
             else:
                 if T is None:
                     raise ValueError(f"views[{v}]: need T or T16")
@@ -215,7 +218,7 @@ class PersistentSchedule:
                 if src.numel() != G * N * M * (4 if source_layout == 1 else 1):
                     raise ValueError(f"views[{v}]: source size does not match layout {source_layout}")
             arr[v] = _hip.SweepView(mu_t=spec["mu"].data_ptr(), T=_hip.ptr(T), T_bf16=_hip.ptr(T16),
-                                    T_inv_scale=_hip.ptr(tinv), source=_hip.ptr(src), phi=spec["phi"].data_ptr(),
+                                    T_low=_hip.ptr(tlo), source=_hip.ptr(src), phi=spec["phi"].data_ptr(),
                                     cdir=spec["cdir"].data_ptr())
         if Y.shape != (M, nsh) or Yw.shape != (M, nsh):
             raise ValueError("direction tables do not match the quadrature")

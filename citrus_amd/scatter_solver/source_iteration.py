@@ -19,8 +19,9 @@ from torch import Tensor
 from .. import _hip
 from .transport_sweep import SourceFunction, SweepContext, sweep
 
-__all__ = ["SourceIterationResult", "MaterialTransfer", "source_iteration", "scatter_moments",
-           "scatter_source_function", "summed_source_function", "multiple_collision_source_moments"]
+__all__ = ["SourceIterationResult", "MaterialTransfer", "MixedOperand", "source_iteration", "scatter_moments",
+           "scatter_source_function", "summed_source_function", "multiple_collision_source_moments", "bf16_operand",
+           "scatter_moments_mixed"]
 
 
 @dataclass
@@ -179,25 +180,84 @@ def scatter_moments_views(transfer: MaterialTransfer, phi_views: Tensor, out: Op
     return (out, means) if cell_means else out
 
 
-def bf16_operand(moments: Tensor) -> tuple[Tensor, Tensor]:
-    """(T16 [V, G, N, 4, KP] bfloat16, inv_scale [V, G] float32): the scattering-source moments [V, G, R, C, S, nsh, 4]
-    as the bfloat16 operand of the source contraction (``citrus_bf16_operand``), each (view, group) scaled by the
-    power of two that puts its largest magnitude in [1, 2)."""
+@dataclass(frozen=True, eq=False)
+class MixedOperand:
+    """The mixed-precision operand of the sweep's source expansion for V views: the scattering-source moments of
+    degree l <= 1 in float32, ``low [V, G, R, C, S, 4 components, 4 moments]``, and those of degree l >= 2 in
+    bfloat16, ``high [V, G, R, C, S, 4 components, KH]`` (KH = nsh - 4 rounded up to 16, zero padded). The sweep
+    expands it as ``low Y + high Y`` (a float32 and a bfloat16 matrix product, both accumulated in float32)."""
+
+    low: Tensor
+    high: Tensor
+    nsh: int
+
+    @property
+    def num_views(self) -> int:
+        return int(self.low.shape[0])
+
+    @property
+    def spatial(self) -> tuple:
+        return tuple(int(v) for v in self.low.shape[2:5])
+
+    @property
+    def device(self) -> torch.device:
+        return self.low.device
+
+    def view(self, v: int) -> "MixedOperand":
+        """The operand of view v alone (views of the same storage)."""
+        return MixedOperand(self.low[v:v + 1], self.high[v:v + 1], self.nsh)
+
+    def check(self, V: int, G: int, spatial: tuple, nsh: int) -> None:
+        kh = 16 * ((nsh - 4 + 15) // 16)
+        if self.nsh != nsh or tuple(self.low.shape) != (V, G) + tuple(spatial) + (4, 4) \
+                or tuple(self.high.shape) != (V, G) + tuple(spatial) + (4, kh):
+            raise ValueError(f"mixed operand must be low [{V}, {G}, *{spatial}, 4, 4] and high [{V}, {G}, *{spatial}, "
+                             f"4, {kh}] for {nsh} moments")
+        if self.low.dtype != torch.float32 or self.high.dtype != torch.bfloat16 or not self.low.is_contiguous() \
+                or not self.high.is_contiguous() or self.low.device.type != "cuda" or self.high.device != self.low.device:
+            raise ValueError("mixed operand: contiguous float32 low and bfloat16 high on one GPU")
+
+
+def empty_mixed_operand(V: int, G: int, spatial: tuple, nsh: int, device) -> MixedOperand:
+    if nsh not in (16, 36):
+        raise ValueError("the mixed-precision operand is built for P3 and P5 (16 and 36 moments)")
+    kh = 16 * ((nsh - 4 + 15) // 16)
+    return MixedOperand(torch.empty((V, G) + tuple(spatial) + (4, 4), device=device, dtype=torch.float32),
+                        torch.empty((V, G) + tuple(spatial) + (4, kh), device=device, dtype=torch.bfloat16), nsh)
+
+
+def bf16_operand(moments: Tensor) -> MixedOperand:
+    """The mixed-precision operand (``citrus_bf16_operand``) of float32 scattering-source moments
+    [V, G, R, C, S, nsh, 4]: degrees l <= 1 copied, degrees l >= 2 rounded to bfloat16 (nearest, ties to even)."""
     V, G, R, C, S, nsh, four = (int(v) for v in moments.shape)
     _hip.require_gpu_tensor(moments, "moments")
     if not moments.is_contiguous() or four != 4:
         raise ValueError("moments must be contiguous [V, G, R, C, S, nsh, 4]")
-    N = R * C * S
-    kp = (nsh + 15) // 16 * 16
-    T16 = torch.empty(V, G, N, 4, kp, device=moments.device, dtype=torch.bfloat16)
-    inv = torch.empty(V, G, device=moments.device, dtype=torch.float32)
-    lib = _hip.library()
-    ws = torch.empty(int(lib.citrus_bf16_operand_workspace_floats(G, V)), device=moments.device,
-                     dtype=torch.float32)
+    op = empty_mixed_operand(V, G, (R, C, S), nsh, moments.device)
     with torch.cuda.device(moments.device):
-        _hip.check(lib.citrus_bf16_operand(moments.data_ptr(), G, nsh, N, V, T16.data_ptr(), inv.data_ptr(),
-                                           ws.data_ptr(), _hip.stream_of(moments.device)), "citrus_bf16_operand")
-    return T16, inv
+        _hip.check(_hip.library().citrus_bf16_operand(moments.data_ptr(), G, nsh, R * C * S, V, op.low.data_ptr(),
+                                                      op.high.data_ptr(), _hip.stream_of(moments.device)),
+                   "citrus_bf16_operand")
+    return op
+
+
+def scatter_moments_mixed(transfer: MaterialTransfer, phi_views: Tensor, out: Optional[MixedOperand] = None,
+                          cell_means: bool = False):
+    """``scatter_moments_views`` whose result is written directly as the mixed-precision operand of the next sweep
+    (``citrus_scatter_moments_mixed``): returns the operand, and with ``cell_means`` also the float32 cell means
+    [V, nsh, G, R, C, S] of the image's expansion."""
+    V, G, R, C, S, nsh, four = (int(v) for v in phi_views.shape)
+    _hip.require_gpu_tensor(phi_views, "phi_views")
+    rho, sigma = transfer.rho.contiguous(), transfer.sigma.contiguous()
+    out = empty_mixed_operand(V, G, (R, C, S), nsh, phi_views.device) if out is None else out
+    out.check(V, G, (R, C, S), nsh)
+    means = torch.empty(V, nsh, G, R, C, S, device=phi_views.device, dtype=torch.float32) if cell_means else None
+    with torch.cuda.device(phi_views.device):
+        _hip.check(_hip.library().citrus_scatter_moments_mixed(
+            rho.data_ptr(), int(rho.shape[0]), sigma.data_ptr(), int(sigma.shape[3]) - 1,
+            int(transfer.downscatter_only), phi_views.data_ptr(), out.low.data_ptr(), out.high.data_ptr(), G, nsh,
+            R * C * S, V, _hip.ptr(means), _hip.stream_of(phi_views.device)), "citrus_scatter_moments_mixed")
+    return (out, means) if cell_means else out
 
 
 def multiple_collision_source_moments(mu_s_l_volume_coarse, phi_total: Tensor) -> Tensor:

@@ -299,6 +299,28 @@ def _sweep_unchecked(context: SweepContext, mu_t: Tensor, source_function: Sourc
     return phi
 
 
+def _check_moments(moments, G: int, spatial: tuple, nsh: int) -> tuple:
+    """(V, mixed) of float32 moments [V, G, R, C, S, nsh, 4] or a mixed-precision operand; raises ValueError on a
+    shape, dtype or layout mismatch."""
+    from .source_iteration import MixedOperand
+
+    if isinstance(moments, MixedOperand):
+        V = moments.num_views
+        moments.check(V, G, spatial, nsh)
+        return V, True
+    if moments.dim() != 7 or tuple(moments.shape[1:]) != (G,) + tuple(spatial) + (nsh, 4) \
+            or not moments.is_contiguous() or moments.dtype != torch.float32:
+        raise ValueError(f"moments must be contiguous float32 [V, {G}, *{tuple(spatial)}, {nsh}, 4]")
+    return int(moments.shape[0]), False
+
+
+def _mixed(moments):
+    """The mixed-precision operand of float32 moments (converted once), or the operand itself."""
+    from .source_iteration import MixedOperand, bf16_operand
+
+    return moments if isinstance(moments, MixedOperand) else bf16_operand(moments)
+
+
 def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out: Optional[Tensor] = None,
                 direction_chunk: Optional[int] = None, source: Optional[Tensor] = None,
                 contraction: str = "fp32", schedule: str = "auto") -> Tensor:
@@ -308,15 +330,16 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     Args:
         sweep_context: context of the grid and the quadrature (float32, GPU).
         mu_t: [G, R, C, S] float32 total attenuation shared by the views.
-        moments: [V, G, R, C, S, nsh, 4] float32 contiguous scattering-source moments (group-major layout).
+        moments: [V, G, R, C, S, nsh, 4] float32 contiguous scattering-source moments (group-major layout), or their
+            mixed-precision operand (``source_iteration.MixedOperand``, which implies ``contraction="bf16"``).
         out: optional [V, G, R, C, S, nsh, 4] float32 output buffer.
         direction_chunk: directions per octant per pass. Defaults to all; "auto" takes the fastest chunk (and, with
             ``schedule="auto"``, schedule) measured by the autotuner for this configuration.
         source: optional explicit source added to the moments' expansion, every direction of every view:
             [V, G, R, C, S, M, 4] LD moments or [V, G, R, C, S, M] cell means (zero slopes), M the reordered
             directions (the summed source of the CITRUS step, citrus.py:232-236).
-        contraction: "fp32", or "bf16" for the source expansion as a bfloat16 matrix product with float32
-            accumulation, the moments scaled per (view, group) by a power of two (P3 and P5).
+        contraction: "fp32", or "bf16" for the mixed-precision source expansion (P3 and P5): the moments of degree
+            l <= 1 in float32 and those of degree l >= 2 as a bfloat16 matrix product, all accumulated in float32.
         schedule: "launches" (one launch per block diagonal), "persistent" (one launch of resident waves per
             direction chunk and at most 16 views, P3 and P5) or "auto" (persistent where it is built and passed its
             self-check on this device, else launches); all give the same moments bit for bit.
@@ -330,9 +353,9 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     G = int(mu_t.shape[0])
     R, C, S = sweep_context.shape
     nsh = sweep_context.num_moments
-    if moments.dim() != 7 or tuple(moments.shape[1:]) != (G, R, C, S, nsh, 4) or not moments.is_contiguous():
-        raise ValueError(f"moments must be contiguous [V, {G}, {R}, {C}, {S}, {nsh}, 4]")
-    V = int(moments.shape[0])
+    V, mixed = _check_moments(moments, G, (R, C, S), nsh)
+    if mixed:
+        contraction = "bf16"
     layout = 0
     if source is not None:
         M = sweep_context.num_directions
@@ -343,16 +366,17 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             layout = 2
         else:
             raise ValueError(f"source must be {base + (4,)} or {base}, got {tuple(source.shape)}")
-        if not source.is_contiguous() or source.device != moments.device or source.dtype != torch.float32:
+        if not source.is_contiguous() or source.device != mu_t.device or source.dtype != torch.float32:
             raise ValueError("source must be a contiguous float32 tensor on the moments' device")
     if contraction not in ("fp32", "bf16"):
         raise ValueError(f"contraction must be 'fp32' or 'bf16', got {contraction!r}")
     if schedule not in ("auto", "launches", "persistent"):
         raise ValueError(f"schedule must be 'auto', 'launches' or 'persistent', got {schedule!r}")
+    shape = (V, G, R, C, S, nsh, 4)
     if out is None:
-        out = torch.empty_like(moments)
-    elif tuple(out.shape) != tuple(moments.shape) or not out.is_contiguous() or out.dtype != torch.float32:
-        raise ValueError("out must be a contiguous float32 tensor shaped like moments")
+        out = torch.empty(shape, device=mu_t.device, dtype=torch.float32)
+    elif tuple(out.shape) != shape or not out.is_contiguous() or out.dtype != torch.float32:
+        raise ValueError(f"out must be a contiguous float32 {list(shape)} tensor")
     if schedule == "auto":
         from .kernels.sweep_kernels import PERSISTENT_MOMENTS, persistent_usable
 
@@ -360,7 +384,7 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             from .autotune import default_tuner
 
             schedule, direction_chunk = default_tuner().best_config(sweep_context, mu_t, V, contraction)
-        elif nsh in PERSISTENT_MOMENTS and persistent_usable(moments.device)[0]:
+        elif nsh in PERSISTENT_MOMENTS and persistent_usable(mu_t.device)[0]:
             schedule = "persistent"
         else:
             schedule = "launches"
@@ -375,16 +399,14 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     Dc = _resolve_chunk(sweep_context, direction_chunk)
     hip = sweep_context.hip
     mu = mu_t.reshape(G, -1).contiguous()
-    T, T16, tinv = moments, None, None
+    T, T16, tlo = moments, None, None
     if contraction == "bf16":
-        from .source_iteration import bf16_operand
-
-        T16, tinv = bf16_operand(moments)
-        T = None
+        op = _mixed(moments)
+        T, T16, tlo = None, op.high, op.low
     for d0 in range(0, D8, Dc):
         dc = min(d0 + Dc, D8) - d0
         hip.schedule.run(mu=mu, phi=out, Y=hip.Y, Yw=hip.Yw, cdir=hip.cdir, G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, V=V,
-                         T=T, accumulate=d0 > 0, source=source, source_layout=layout, T16=T16, T_inv_scale=tinv)
+                         T=T, accumulate=d0 > 0, source=source, source_layout=layout, T16=T16, T_low=tlo)
     return out
 
 
@@ -409,7 +431,10 @@ def sweep_grids(items, direction_chunk: Optional[int] = None, contraction: str =
         if mu_t.dim() != 4 or tuple(mu_t.shape[1:]) != tuple(ctx.shape):
             raise ValueError(f"mu_t must be [G, {ctx.shape}], got {tuple(mu_t.shape)}")
         _resolve_hip(ctx, None, mu_t)
-        prepared.append((ctx, mu_t, moments, source, torch.empty_like(moments)))
+        V, _ = _check_moments(moments, int(mu_t.shape[0]), tuple(ctx.shape), ctx.num_moments)
+        out = torch.empty((V, int(mu_t.shape[0])) + tuple(ctx.shape) + (ctx.num_moments, 4), device=mu_t.device,
+                          dtype=torch.float32)
+        prepared.append((ctx, mu_t, moments, source, out))
     return _sweep_grids_persistent(prepared, direction_chunk, contraction)
 
 
@@ -433,10 +458,9 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
         if int(mu_t.shape[0]) != G:
             raise ValueError("the grids of one persistent sweep must have the same number of groups")
         R, C, S = ctx.shape
-        if moments.dim() != 7 or tuple(moments.shape[1:]) != (G, R, C, S, nsh, 4) or not moments.is_contiguous() \
-                or moments.dtype != torch.float32:
-            raise ValueError(f"moments must be contiguous float32 [V, {G}, {R}, {C}, {S}, {nsh}, 4]")
-        V = int(moments.shape[0])
+        V, mixed = _check_moments(moments, G, (R, C, S), nsh)
+        if mixed and contraction != "bf16":
+            raise ValueError("a mixed-precision operand needs contraction='bf16'")
         layout = -1
         if source is not None:
             base = (V, G, R, C, S, M)
@@ -447,23 +471,19 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
                 raise ValueError("source must be a contiguous float32 tensor")
         layouts.add(layout)
         mu = mu_t.reshape(G, -1).contiguous()
-        T16 = tinv = None
-        if contraction == "bf16":
-            from .source_iteration import bf16_operand
-
-            T16, tinv = bf16_operand(moments)
+        op = _mixed(moments) if contraction == "bf16" else None
         for v in range(V):
             spec = dict(mu=mu, phi=out[v], cdir=ctx.hip.cdir, source=None if source is None else source[v])
-            if T16 is None:
+            if op is None:
                 spec["T"] = moments[v]
             else:
-                spec["T16"], spec["T_inv_scale"] = T16[v], tinv[v]
+                spec["T16"], spec["T_low"] = op.high[v], op.low[v]
             views.append((tuple(ctx.shape), spec))
     if len(layouts) != 1:
         raise ValueError("every item of one persistent sweep needs a source of the same layout, or none")
     layout = layouts.pop()
     Dc = _resolve_chunk(ctx0, None if direction_chunk == "auto" else direction_chunk)
-    device = items[0][2].device
+    device = items[0][1].device
     for b0 in range(0, len(views), PERSISTENT_MAX_VIEWS):
         batch = views[b0:b0 + PERSISTENT_MAX_VIEWS]
         sched = persistent_schedule([s for s, _ in batch], G, device)

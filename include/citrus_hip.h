@@ -78,9 +78,12 @@ typedef struct CitrusSweepArgs {
                           * 1: group-major [V][G][N][8*D8][4] of every direction, indexed by the reordered direction
                           *    m = o*D8 + d (not with write_psi);
                           * 2: group-major cell means [V][G][N][8*D8], slopes zero (not with write_psi) */
-  const void* T_bf16;    /* in place of T: bfloat16 operand [V][G][N][4][KP] from citrus_bf16_operand (P3 and P5);
-                          * the source expansion then runs as a bfloat16 matrix product with float32 accumulation */
-  const float* T_inv_scale; /* [V][G] with T_bf16: the scale undoing the operand's per-group power of two */
+  const void* T_bf16;    /* in place of T, the mixed-precision operand (P3 and P5) of citrus_bf16_operand or
+                          * citrus_scatter_moments_mixed: the moments of degree l >= 2 (n = 4 .. nsh-1) as bfloat16
+                          * [V][G][N][4 components][KH], KH = nsh - 4 rounded up to 16, zero padded. The source
+                          * expansion then runs as T_low Y (float32 matrix product) + T_bf16 Y (bfloat16 matrix
+                          * product, float32 accumulation) */
+  const float* T_low;    /* with T_bf16: the moments of degree l <= 1 (n = 0..3) as float32 [V][G][N][4 comp][4] */
 } CitrusSweepArgs;
 
 /* Sweep one direction chunk of every octant, all groups and views, through the grid with vacuum boundaries, and
@@ -109,8 +112,8 @@ typedef struct CitrusPersistentPlan CitrusPersistentPlan;
 typedef struct CitrusSweepView {
   const float* mu_t;         /* [G][N] */
   const float* T;            /* [G][N][nsh][4] scattering-source moments, or NULL with T_bf16 */
-  const void* T_bf16;        /* bfloat16 operand [G][N][4][KP] of citrus_bf16_operand (P3, P5), or NULL */
-  const float* T_inv_scale;  /* [G] with T_bf16 */
+  const void* T_bf16;        /* mixed operand: bfloat16 degrees l >= 2 [G][N][4][KH] (P3, P5), or NULL */
+  const float* T_low;        /* with T_bf16: float32 degrees l <= 1 [G][N][4][4] */
   float* source;             /* group-major source of every direction ([G][N][8*D8][4] or [G][N][8*D8]) or NULL */
   float* phi;                /* [G][N][nsh][4] fluence moments (output) */
   const float* cdir;         /* [8*D8][3] |Omega_a| / Delta_a on this view's grid */
@@ -441,14 +444,17 @@ int citrus_footprint_filters(const float* partial, int K, int nblocks, float cen
                              float scale_channel, float scale_row, int Nc, int Nr, const float* channel_weight,
                              float* heights, float* channel_filter_t, float* row_filter, void* stream);
 
-/* bfloat16 operand of the scattering-source contraction (configs with the mixed-precision contraction of
- * scattering_kernels.py:162 / transport_sweep.py:524-528): T [V][G][N][nsh][4] float32 -> T16 [V][G][N][4][KP]
- * (KP = nsh rounded up to 16, zero padded; bfloat16 bit patterns, round to nearest even) of T * 2^-e per (view, group)
- * with max|T| in [2^e, 2^(e+1)), and inv_scale [V][G] = 2^e. Deterministic (fixed-order max). workspace holds
- * citrus_bf16_operand_workspace_floats(G, V) floats. */
-int64_t citrus_bf16_operand_workspace_floats(int G, int V);
-int citrus_bf16_operand(const float* T, int G, int nsh, int64_t N, int V, uint16_t* T16, float* inv_scale,
-                        float* workspace, void* stream);
+/* Mixed-precision operand of the scattering-source contraction (the mixed-precision form of the source expansion of
+ * transport_sweep.py:524-528 / source_iteration.py:92-102): T [V][G][N][nsh][4] float32 -> T_low [V][G][N][4][4]
+ * float32 (moments n = 0..3, degrees l <= 1, component-major) and T16 [V][G][N][4][KH] bfloat16 (moments n >= 4,
+ * round to nearest even, KH = nsh - 4 rounded up to 16, zero padded). nsh 16 or 36. */
+int citrus_bf16_operand(const float* T, int G, int nsh, int64_t N, int V, float* T_low, uint16_t* T16, void* stream);
+
+/* citrus_scatter_moments_factored that writes its result directly as the mixed-precision operand (T_low, T_bf16 as
+ * above) instead of float32 T, with the same optional cell_means: the sweep of the next step reads it in place. */
+int citrus_scatter_moments_mixed(const float* rho, int num_materials, const float* sigma, int Lt, int downscatter_only,
+                                 const float* phi, float* T_low, void* T_bf16, int G, int nsh, int64_t N, int V,
+                                 float* cell_means, void* stream);
 
 /* ---- Phi_{>2} surrogate network ------------------------------------------------------------------------------
  * No counterpart in the reference: the learned part of this path, a 3D U-Net that predicts the collided moments of
