@@ -12,6 +12,7 @@ runs on the GPU in float32 with the reference's arithmetic.
 
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 from typing import Optional
 
@@ -26,29 +27,38 @@ from .scatter_solver.cross_sections import MultigroupLibrary
 MATERIAL_ORDER_GAP = 1e-6
 
 
+def _fine_tables(library: MultigroupLibrary) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(fine energies [E] ascending, their spectrum weights [E], mass attenuations [M, E]) as float64 numpy arrays
+    (a stable sort keeps equal energies in library order)."""
+    E = library.fine_energy.detach().cpu().double().numpy()
+    order = np.argsort(E, kind="stable")
+    return (E[order], library.fine_weight.detach().cpu().double().numpy()[order],
+            library.fine_mu_t.detach().cpu().double().numpy()[:, order])
+
+
 def effective_energy(library: MultigroupLibrary) -> float:
-    """Spectrum-mean photon energy ``sum_e w[e] E[e] / sum_e w[e]`` over the fine energies, keV
+    """Spectrum-mean photon energy (keV): the fine energies averaged with their spectrum weights
     (utilities.py:577)."""
-    w = library.fine_weight.to(torch.float64)
-    return float((w * library.fine_energy.to(torch.float64)).sum() / w.sum())
+    E, w, _ = _fine_tables(library)
+    return float(np.average(E, weights=w))
 
 
 def mass_attenuation_at(library: MultigroupLibrary, energy_kev: float) -> Tensor:
-    """[M] total mass attenuation at one energy, linear in (log E, log mu) on the sorted fine energies; a tabulated
-    energy gives its row (utilities.py:590). ValueError outside the fine-energy range."""
-    E = library.fine_energy.detach().to("cpu", torch.float64)
-    mu = library.fine_mu_t.detach().to("cpu", torch.float64)
-    order = torch.argsort(E)
-    E, mu = E[order], mu[:, order]
+    """[M] total mass attenuation (cm^2/g, float64 on the CPU) at one energy: the fine table's straight line in
+    (log E, log mu) between the neighbouring fine energies; an energy equal to a fine energy gives that column (the
+    last of equal ones), so a monoenergetic library works (utilities.py:590). ValueError outside the fine-energy
+    range."""
+    E, _, mu = _fine_tables(library)
     e = float(energy_kev)
-    if not float(E[0]) <= e <= float(E[-1]):
-        raise ValueError(f"energy {e} keV outside the library's fine-energy range [{float(E[0])}, {float(E[-1])}]")
-    i = int(torch.searchsorted(E, torch.tensor([e], dtype=torch.float64), right=True)[0]) - 1
-    if float(E[max(i, 0)]) == e:
-        return mu[:, max(i, 0)].clone()
-    i = min(max(i, 0), E.numel() - 2)
-    t = (torch.log(torch.tensor(e, dtype=torch.float64)) - torch.log(E[i])) / (torch.log(E[i + 1]) - torch.log(E[i]))
-    return torch.exp(torch.log(mu[:, i]) + t * (torch.log(mu[:, i + 1]) - torch.log(mu[:, i])))
+    if not E[0] <= e <= E[-1]:
+        raise ValueError(f"energy {e} keV outside the library's fine-energy range [{E[0]}, {E[-1]}]")
+    same = np.flatnonzero(E == e)
+    if same.size:
+        return torch.from_numpy(mu[:, same[-1]].copy())
+    j = int(np.searchsorted(E, e)) - 1  # E[j] < e < E[j + 1]
+    frac = (math.log(e) - math.log(E[j])) / (math.log(E[j + 1]) - math.log(E[j]))
+    log_mu = (1.0 - frac) * np.log(mu[:, j]) + frac * np.log(mu[:, j + 1])
+    return torch.from_numpy(np.exp(log_mu))
 
 
 def nominal_densities(library: MultigroupLibrary) -> Tensor:
@@ -59,9 +69,9 @@ def nominal_densities(library: MultigroupLibrary) -> Tensor:
 
 
 def nominal_attenuation(library: MultigroupLibrary, energy_kev: Optional[float] = None) -> Tensor:
-    """[M] attenuation of each material at its nominal density, 1/cm (utilities.py:634)."""
-    e = effective_energy(library) if energy_kev is None else float(energy_kev)
-    return nominal_densities(library) * mass_attenuation_at(library, e)
+    """[M] attenuation (1/cm) of each material at its nominal density (utilities.py:634)."""
+    return nominal_densities(library) * mass_attenuation_at(
+        library, effective_energy(library) if energy_kev is None else float(energy_kev))
 
 
 def material_order(library: MultigroupLibrary, energy_kev: Optional[float] = None) -> list[int]:
