@@ -160,3 +160,52 @@ def primary_flat(path, fine_mu, fine_group, fine_weight, G, pixels, src, nh, ene
 def scatter_fraction(S, P):
     den = P + S
     return np.where(den > 0, S / np.where(den > 0, den, 1.0), 0.0)
+
+
+def detector_response(energy_nodes, theta_nodes, table, energy, theta) -> np.ndarray:
+    """[G, ...] detector response D(E, theta) of an [E, T] table (detector.py:226-256): linear in energy, then linear
+    in |theta|, each clamped to its end nodes (a bilinear lookup is separable, so two numpy.interp passes give it)."""
+    en = np.asarray(energy_nodes, dtype=np.float64)
+    tn = np.asarray(theta_nodes, dtype=np.float64)
+    tab = np.asarray(table, dtype=np.float64)
+    e = np.asarray(energy, dtype=np.float64).reshape(-1)
+    th = np.abs(np.asarray(theta, dtype=np.float64))
+    at_energy = np.stack([np.interp(e, en, tab[:, t]) for t in range(tn.size)], axis=1)  # [G, T]
+    return np.stack([np.interp(th, tn, at_energy[g]) for g in range(e.size)])
+
+
+def channel_normals(sid, sdd, cols, dch, angle, off_ch=0.0) -> np.ndarray:
+    """[cols, 3] panel normal at each channel of a curved panel: horizontal, from the pixel toward the vertical axis
+    through the source (conventions.rst, "Curved panel"); its negative is the inward normal of the incidence angle."""
+    pix = rays.pixel_centers(sid, sdd, 1, cols, 1.0, dch, angle, True, off_ch=off_ch)[0, :, 0, 0]  # [cols, 3]
+    to_axis = rays.source_position(sid, angle) - pix
+    to_axis[:, 2] = 0.0
+    return to_axis / np.linalg.norm(to_axis, axis=1, keepdims=True)
+
+
+def signal_weights(omega, weight, sid, sdd, angle, curved, energy, detector, cols=None, dch=None):
+    """The emission-independent weights of the collapsed-cone image of one view (make_direction_context,
+    scatter_image.py:542-690, without a grid): the directions facing the panel (``omega . -n_hat > 0``), their
+    ``cos(theta0)`` to the panel normal at its centre, and
+
+    - flat panel: signal weights [K', G] = ``w cos(theta0) D_g(theta0)``, theta0 the angle of the normalized
+      direction;
+    - curved panel: planar weights [K', Cd] = ``w clamp(omega . -n_c, 0, 1)`` with the channel's own normal n_c, and
+      signal weights [K', G, Cd] = planar ``D_g(theta_c)``.
+
+    ``detector`` is (energy nodes, theta nodes, table). Returns (keep [K], cos0 [K'], signal, planar or None)."""
+    om = np.asarray(omega, dtype=np.float64)
+    w = np.asarray(weight, dtype=np.float64)
+    _, _, _, nh = rays.panel_frame(sid, sdd, angle)
+    cos_all = om @ (-nh)
+    keep = cos_all > 0
+    om, w, cos0 = om[keep], w[keep], cos_all[keep]
+    if not curved:  # theta0 from the normalized direction (detector_angle, detector.py:1054-1075)
+        theta0 = np.arccos(np.clip(cos0 / np.linalg.norm(om, axis=1), -1.0, 1.0))
+        d = detector_response(*detector, energy, theta0)  # [G, K']
+        return keep, cos0, (w * cos0)[:, None] * d.T, None
+    normals = channel_normals(sid, sdd, cols, dch, angle)
+    cos_c = np.clip(om @ (-normals).T, 0.0, 1.0)  # [K', Cd]
+    planar = w[:, None] * cos_c
+    d = detector_response(*detector, energy, np.arccos(cos_c))  # [G, K', Cd]
+    return keep, cos0, planar[:, None, :] * d.transpose(1, 0, 2), planar

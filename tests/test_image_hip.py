@@ -88,9 +88,16 @@ def test_combined_scatter_image_matches_oracle(curved, q_factor):
     sig_only = combined_scatter_image(library, f32(rho), f32(phi), gq, q_factor, f32(T), 2, geometry, angle, quad,
                                       f32(mu_c), HipProjectors(), det, grid_response, context=ctx,
                                       grid_coarse=coarse, signal_only=True)
-    # oracle on the same (float32-rounded) inputs and direction tables
+    # oracle on the same (float32-rounded) inputs; the direction tables (the panel-facing directions, cos(theta0),
+    # signal and planar weights) from the oracle's own restatement of the direction context, from the cone set alone
     d64 = lambda t: t.float().double().cpu().numpy()
-    om = ctx.omega.double().cpu().numpy()
+    table = (det.energy_kev.numpy(), det.theta_rad.numpy(), det.response.numpy())
+    keep, cos0, signal, planar = oim.signal_weights(d64(quad.omega), d64(quad.weight), geometry.source_iso_dist,
+                                                   geometry.source_detector_dist, angle, curved,
+                                                   library.group_energy.numpy(), table,
+                                                   cols=geometry.num_det_channels, dch=geometry.delta_det_channel)
+    om = d64(quad.omega)[keep]
+    assert np.array_equal(om, ctx.omega.double().cpu().numpy())
     pts = gq.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
     ou = ofc.primary_directions(pts, np.array(geometry.source_position(angle)))
     S = ofc.source_at_directions(library.mu_s_eta.numpy(), None, d64(rho).reshape(2, -1), d64(phi).reshape(3, -1),
@@ -105,11 +112,10 @@ def test_combined_scatter_image_matches_oracle(curved, q_factor):
           for c in [(i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]]
     fl, pl, sg = oim.collapsed_cone_image(
         lambda k: S[k], d64(T), Y, d64(mu_c), np.array(gq.lower_corner()), gq.deltas, gq.shape, q_factor, om,
-        ctx.weight.double().cpu().numpy(), ctx.cell_width.double().cpu().numpy(), pixels,
+        d64(quad.weight)[keep], d64(quad.cell_width)[keep], pixels,
         (geometry.delta_det_channel, geometry.delta_det_row), heights, 0.5 * (min(hs) + max(hs)),
-        ctx.signal_weight.double().cpu().numpy(), ctx.cos_theta0.double().cpu().numpy(), geometry.source_iso_dist,
-        geometry.source_detector_dist, angle, curved, footprint=True,
-        planar_weight=None if ctx.planar_weight is None else ctx.planar_weight.double().cpu().numpy())
+        signal, cos0, geometry.source_iso_dist, geometry.source_detector_dist, angle, curved, footprint=True,
+        planar_weight=planar)
     assert rel_err(out["fluence"], fl) < F32_REL
     assert rel_err(out["planar_fluence"], pl) < F32_REL
     assert rel_err(out["signal_per_group"], sg) < F32_REL
