@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 METRIC = "scatter view-sweeps/s (64³×16g×S12), learned vs plain at ≤1% SPR error"
 HBM_PEAK_GBS = 8000.0
 
@@ -89,151 +91,193 @@ def build_problem(args, device):
     return ctx, mu_t, transfer, phi, library, rho, coarse
 
 
-def cpu_baseline(args, ctx, library, rho, coarse):
-    """Oracle algorithm in float32 on the host: source expansion, sweep and moment reduction of a direction sample."""
-    import numpy as np
-    import torch
+# ---- CPU baseline: the oracle's algorithm on the host's cores ------------------------------------------------------
+# Runs before anything touches the GPU: each stage's bounded sample is split over a pool of forked worker processes
+# (numpy, one thread each), timed by wall clock and scaled to one whole view step. Worker inputs live in _CPU, set
+# before the pool forks.
 
-    from oracle import quadrature as oq
-    from oracle import sweep as osw
-    from oracle.scattering import scatter_moments
-
-    G = library.num_groups
-    nsh = ctx.num_moments
-    perm = ctx.permutation.cpu()
-    from citrus_amd.scatter_solver.angular_quadrature import level_symmetric_quadrature
-
-    q = level_symmetric_quadrature(12)
-    om = q.omega[perm].numpy()
-    w = q.weight[perm].numpy()
-    D8 = ctx.directions_per_octant
-    k = max(1, args.cpu_directions // 8)
-    sel = np.concatenate([np.arange(o * D8, o * D8 + k) for o in range(8)])
-    Y = oq.real_spherical_harmonics(5, om)
-    rho_np = rho.numpy().astype(np.float32)
-    mu_t = np.einsum("mg,mijk->gijk", library.mu_t.numpy(), rho_np).astype(np.float32)
-    R = coarse.shape[0]
-    rows = max(1, R // 8)  # the scattering-moment pass is timed on the first R/8 cell rows and scaled by R/rows
-    mu_s = np.einsum("mabl,mijk->ablijk", library.mu_s_l.numpy(), rho_np[:, :rows]).astype(np.float32)
-    rng = np.random.default_rng(7)
-    phi = (rng.standard_normal((G, nsh, 4) + coarse.shape) * 0.05).astype(np.float32)
-    phi[:, 0, 0] = np.abs(phi[:, 0, 0]) + 1.0
-    from threadpoolctl import threadpool_limits
-
-    limiter = threadpool_limits(limits=1)  # one host thread for every numpy kernel: cores = 1
-    t0 = time.perf_counter()
-    T_rows = scatter_moments(mu_s, phi[:, :, :, :rows]).astype(np.float32)
-    t_moments = (time.perf_counter() - t0) * R / rows
-    T = np.concatenate([T_rows] * (R // rows) + [T_rows[:, :, :, : R % rows]], axis=3)  # sample: cell rows repeated
-    t0 = time.perf_counter()
-    src = np.tensordot(Y[sel].astype(np.float32), T, axes=([1], [1])).astype(np.float32)  # [Msel, G, 4, R, C, S]
-    psi = osw.sweep(coarse.shape, coarse.deltas, om[sel], mu_t, src, dtype=np.float32)
-    out = np.tensordot((w[sel, None] * Y[sel]).astype(np.float32), psi, axes=([0], [0]))
-    t_dirs = time.perf_counter() - t0
-    limiter.unregister()
-    M = ctx.num_directions
-    per_view = t_moments + t_dirs * M / sel.size
-    del out
-    return {"value": 1.0 / per_view, "unit": "view-sweeps/s", "cores": 1, "kind": "port",
-            "moments": t_moments, "sweep": t_dirs * M / sel.size,
-            "sample": f"{sel.size} of {M} directions x {G} groups x {R}^3 cells (source expansion + LD sweep + "
-                      f"moment reduction, numpy float32, one thread) plus the scattering-moment pass on {rows} of {R} "
-                      f"cell rows; per-view time = moments {t_moments:.1f} s (scaled) + {t_dirs:.1f} s x "
-                      f"{M}/{sel.size}"}
+_CPU: dict = {}
+CPU_WORKERS_MAX = 16  # the host's share of cores per GPU
 
 
-def cpu_view_step_baseline(args, ctx, library, rho, coarse, sweep_part: dict):
-    """The oracle's algorithm for one CITRUS view step, each stage timed on a bounded sample of the 64^3 view and
-    scaled to the whole view (one host thread): uncollided trace and fluence, first-collision source with its
-    sampling correction, scattering moments and the sweep (from ``cpu_baseline``), and the collapsed-cone image
-    (downstream optical depth, multiple-collision emission, oblique projection) and the primary."""
-    import numpy as np
-    import torch
-    from threadpoolctl import threadpool_limits
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
 
-    from citrus_amd.scatter_solver.angular_quadrature import hemisphere_quadrature
-    from citrus_amd.scatter_solver.geometry import Grid
-    from oracle import first_collision as ofc
-    from oracle import image as oim
-    from oracle import quadrature as oq
+    return platform.processor() or "unknown"
+
+
+def _w_uncollided(chunk):
     from oracle import rays as orays
     from oracle import uncollided as ounc
 
-    rng = np.random.default_rng(11)
-    sid, sdd, rows, cols, pitch, angle = 100.0, 150.0, 256, 256, 0.12, 0.0
-    n = coarse.shape[0]
-    lo = np.array(coarse.lower_corner())
-    delta = np.array(coarse.deltas)
-    nn = (coarse.num_cols, coarse.num_rows, coarse.num_slices)
-    rho_np = rho.numpy().astype(np.float64)  # [2, R, C, S]
-    pts_all = coarse.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
-    P = pts_all.shape[0]
-    sample = rng.choice(P, P // 64, replace=False)
-    pts = pts_all[sample]
-    src = orays.source_position(sid, angle)
-    fm, fg, fw = (library.fine_mu_t.numpy(), library.fine_group.numpy(), library.fine_weight.numpy())
-    G = library.num_groups
-    times = {}
-    limiter = threadpool_limits(limits=1)
-    t0 = time.perf_counter()
-    paths = orays.line_integrals(rho_np, np.broadcast_to(src, pts.shape), pts, lo, delta, nn)
-    phi_uc = ounc.uncollided_fluence(paths, pts, fm, fg, fw, G, sid, sdd, rows, cols, pitch, pitch, angle, True)
-    times["uncollided"] = (time.perf_counter() - t0) * 64
-    om, w = ctx.quadrature.omega.double().cpu().numpy(), ctx.quadrature.weight.double().cpu().numpy()
-    rho_s = rho_np.reshape(2, -1)[:, sample]
-    t0 = time.perf_counter()
-    ou = ofc.primary_directions(pts, src)
-    S = ofc.source_at_directions(library.mu_s_eta.numpy(), None, rho_s, phi_uc, ou, om)
-    I = ofc.quadrature_sums(S, ou, om, w)
-    E = ofc.exact_moments(library.mu_s_l.numpy(), rho_s, phi_uc, 1)
-    scale, beta = ofc.correction_factors(I, E)
-    ofc.apply_correction(S, scale, beta, ou, om)
-    times["first_collision"] = (time.perf_counter() - t0) * 64
-    del S
-    times["scatter_moments"] = sweep_part["moments"]
-    times["sweep"] = sweep_part["sweep"]
-    # image: one of the 512 cone directions, its downstream depths and oblique rays on 1/64 of the cells and pixels
-    cone = hemisphere_quadrature(torch.tensor([0.0, -1.0, 0.0], dtype=torch.float64), 16, 32, 0.3,
-                                 tangent_axes=([1.0, 0.0, 0.0], [0.0, 0.0, 1.0]))
-    wk = cone.omega[0].numpy()
-    mu_c = np.einsum("mg,mrcs->grcs", library.mu_t.numpy(), rho_np)
-    T = rng.standard_normal((G, 36) + coarse.shape, dtype=np.float32) * 1e-3
-    Y = oq.real_spherical_harmonics(5, wk[None])[0]
-    pixels = orays.pixel_centers(sid, sdd, rows, cols, pitch, pitch, angle, True)[:, :, 0, 0].reshape(-1, 3)
-    pix = pixels[rng.choice(pixels.shape[0], pixels.shape[0] // 64, replace=False)]
-    t0 = time.perf_counter()
-    tau = oim.downstream_tau(mu_c, lo, delta, nn, pts, wk, sid, sdd, angle, True)
-    t_tau = (time.perf_counter() - t0) * 64
-    t0 = time.perf_counter()
-    S_mc = np.einsum("n,gnrcs->grcs", Y, T)
-    e = S_mc * np.exp(-np.abs(S_mc))  # the attenuated emission's arithmetic on a full [G, 64^3] field
-    t_emit = time.perf_counter() - t0
-    hi = lo + delta * np.array(nn)
-    t0 = time.perf_counter()
-    a, b = orays.oblique_segments(pix, wk, lo, hi)
-    orays.line_integrals(e[:1], a, b, lo, delta, nn)
-    t_obl = (time.perf_counter() - t0) * 64
-    times["image"] = (t_tau + t_emit + t_obl) * int(cone.omega.shape[0])
-    del tau, e, S_mc, T
-    # primary: 1/64 of the pixel rays through the fine 256^3 densities
-    fine = Grid((4 * n, 4 * n, 4 * n), 0.08 * 64 / n)
-    from citrus_amd.utilities import ellipsoid_phantom
+    c = _CPU
+    pts = c["pts"][chunk]
+    paths = orays.line_integrals(c["rho_c"], np.broadcast_to(c["src"], pts.shape), pts, c["lo"], c["delta"], c["nn"])
+    return chunk, ounc.uncollided_fluence(paths, pts, c["fm"], c["fg"], c["fw"], c["G"], 100.0, 150.0, 256, 256,
+                                          0.12, 0.12, 0.0, True)
 
-    rho_f = ellipsoid_phantom(fine, seed=1234).numpy()
-    lof, df = np.array(fine.lower_corner()), np.array(fine.deltas)
-    nf = (fine.num_cols, fine.num_rows, fine.num_slices)
+
+def _w_first_collision(item):
+    from oracle import first_collision as ofc
+
+    chunk, phi_uc = item
+    c = _CPU
+    pts = c["pts"][chunk]
+    rho_s = c["rho_c"].reshape(2, -1)[:, c["sample"][chunk]]
+    ou = ofc.primary_directions(pts, c["src"])
+    S = ofc.source_at_directions(c["mu_s_eta"], None, rho_s, phi_uc, ou, c["om"])
+    I = ofc.quadrature_sums(S, ou, c["om"], c["w"])
+    E = ofc.exact_moments(c["mu_s_l"], rho_s, phi_uc, 1)
+    scale, beta = ofc.correction_factors(I, E)
+    ofc.apply_correction(S, scale, beta, ou, c["om"])
+    return S.shape
+
+
+def _w_moments(rows):
+    from oracle.scattering import scatter_moments
+
+    c = _CPU
+    mu_s = np.einsum("mabl,mijk->ablijk", c["mu_s_l"], c["rho32"][:, rows]).astype(np.float32)
+    return scatter_moments(mu_s, c["phi"][:, :, :, rows]).shape
+
+
+def _w_sweep(sel):
+    from oracle import sweep as osw
+
+    c = _CPU
+    src = np.tensordot(c["Y"][sel].astype(np.float32), c["T"], axes=([1], [1])).astype(np.float32)
+    psi = osw.sweep(c["shape"], c["deltas"], c["om"][sel], c["mu_t"], src, dtype=np.float32)
+    out = np.tensordot((c["w"][sel, None] * c["Y"][sel]).astype(np.float32), psi, axes=([0], [0]))
+    return out.shape
+
+
+def _w_image(k):
+    from oracle import image as oim
+    from oracle import quadrature as oq
+    from oracle import rays as orays
+
+    c = _CPU
+    wk = c["cone"][k]
     t0 = time.perf_counter()
-    orays.line_integrals(rho_f, np.broadcast_to(src, pix.shape), pix, lof, df, nf)
-    times["primary"] = (time.perf_counter() - t0) * 64
-    limiter.unregister()
+    oim.downstream_tau(c["mu_coarse"], c["lo"], c["delta"], c["nn"], c["pts"], wk, 100.0, 150.0, 0.0, True)
+    t1 = time.perf_counter()
+    Y = oq.real_spherical_harmonics(5, wk[None])[0]
+    S_mc = np.einsum("n,gnrcs->grcs", Y, c["Tm"])
+    e = S_mc * np.exp(-np.abs(S_mc))  # the attenuated emission's arithmetic on the whole [G, 64^3] field
+    t2 = time.perf_counter()
+    a, b = orays.oblique_segments(c["pix"], wk, c["lo"], c["hi"])
+    orays.line_integrals(e[:1], a, b, c["lo"], c["delta"], c["nn"])
+    t3 = time.perf_counter()
+    # one direction's image: depths on 1/64 of the cells and rays through 1/64 of the pixels, emission on every cell
+    return (t1 - t0) * 64 + (t2 - t1) + (t3 - t2) * 64
+
+
+def _w_primary(chunk):
+    from oracle import rays as orays
+
+    c = _CPU
+    pix = c["pix"][chunk]
+    return orays.line_integrals(c["rho_f"], np.broadcast_to(c["src"], pix.shape), pix, c["lof"], c["df"],
+                                c["nf"]).shape
+
+
+def cpu_baseline_parallel(args) -> dict:
+    """The oracle's algorithm for one CITRUS view step of the benchmark (64^3 x 16 groups x S12 x P5, the 256^3
+    phantom, the 16 x 32 cone set, the 256^2 curved panel) on P = min(16, host cores) worker processes: each stage
+    on a bounded sample split over the workers, scaled to the whole view by its sample fraction."""
+    import multiprocessing as mp
+
+    import torch
+
+    from citrus_amd.scatter_solver.angular_quadrature import hemisphere_quadrature, level_symmetric_quadrature
+    from citrus_amd.scatter_solver.cross_sections import seeded_library
+    from citrus_amd.scatter_solver.geometry import Grid
+    from citrus_amd.utilities import coarsen, ellipsoid_phantom
+    from oracle import quadrature as oq
+    from oracle import rays as orays
+
+    P = max(1, min(CPU_WORKERS_MAX, os.cpu_count() or 1))
+    n = args.cells
+    fine = Grid((4 * n, 4 * n, 4 * n), 0.08 * 64 / n)
+    coarse = Grid((n, n, n), 0.32 * 64 / n)
+    rho_f = ellipsoid_phantom(fine, seed=1234)
+    rho_c = coarsen(rho_f, 4).numpy()
+    library = seeded_library(16, 5, 140.0, 20.0, seed=99)
+    q = level_symmetric_quadrature(12)
+    om, w = q.omega.numpy(), q.weight.numpy()
+    G, M = 16, int(om.shape[0])
+    rng = np.random.default_rng(11)
+    pts_all = coarse.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    sample = rng.choice(pts_all.shape[0], pts_all.shape[0] // 64, replace=False)
+    pixels = orays.pixel_centers(100.0, 150.0, 256, 256, 0.12, 0.12, 0.0, True)[:, :, 0, 0].reshape(-1, 3)
+    pix = pixels[rng.choice(pixels.shape[0], pixels.shape[0] // 64, replace=False)]
+    cone = hemisphere_quadrature(torch.tensor([0.0, -1.0, 0.0], dtype=torch.float64), 16, 32, 0.3,
+                                 tangent_axes=([1.0, 0.0, 0.0], [0.0, 0.0, 1.0])).omega.numpy()
+    phi = (rng.standard_normal((G, 36, 4) + coarse.shape) * 0.05).astype(np.float32)
+    phi[:, 0, 0] = np.abs(phi[:, 0, 0]) + 1.0
+    lo, delta = np.array(coarse.lower_corner()), np.array(coarse.deltas)
+    nn = (coarse.num_cols, coarse.num_rows, coarse.num_slices)
+    _CPU.clear()
+    _CPU.update(
+        pts=pts_all[sample], sample=sample, rho_c=rho_c, rho32=rho_c.astype(np.float32), src=orays.source_position(100.0, 0.0),
+        lo=lo, hi=lo + delta * np.array(nn), delta=delta, nn=nn, fm=library.fine_mu_t.numpy(),
+        fg=library.fine_group.numpy(), fw=library.fine_weight.numpy(), G=G, mu_s_eta=library.mu_s_eta.numpy(),
+        mu_s_l=library.mu_s_l.numpy(), om=om, w=w, Y=oq.real_spherical_harmonics(5, om), phi=phi,
+        T=(rng.standard_normal((G, 36, 4) + coarse.shape) * 0.05).astype(np.float32),
+        Tm=rng.standard_normal((G, 36) + coarse.shape).astype(np.float32) * 1e-3,
+        mu_t=np.einsum("mg,mijk->gijk", library.mu_t.numpy(), rho_c).astype(np.float32),
+        mu_coarse=np.einsum("mg,mrcs->grcs", library.mu_t.numpy(), rho_c), shape=coarse.shape, deltas=coarse.deltas,
+        cone=cone, pix=pix, rho_f=rho_f.numpy(), lof=np.array(fine.lower_corner()), df=np.array(fine.deltas),
+        nf=(fine.num_cols, fine.num_rows, fine.num_slices))
+    octant = oq.octant_of(om)
+    per_octant = [np.nonzero(octant == o)[0] for o in range(8)]
+    sweep_sets = [np.concatenate([d[i % d.size:i % d.size + 1] for d in per_octant]) for i in range(P)]
+    chunks = np.array_split(np.arange(sample.size), P)
+    rows = np.array_split(np.arange(n), P)
+    pix_chunks = np.array_split(np.arange(pix.shape[0]), P)
+    times = {}
+    ctx = mp.get_context("fork")
+    with ctx.Pool(P) as pool:
+        t0 = time.perf_counter()
+        unc = pool.map(_w_uncollided, chunks)
+        times["uncollided"] = (time.perf_counter() - t0) * 64
+        t0 = time.perf_counter()
+        pool.map(_w_first_collision, [(ch, phi_uc) for ch, phi_uc in unc])
+        times["first_collision"] = (time.perf_counter() - t0) * 64
+        t0 = time.perf_counter()
+        pool.map(_w_moments, rows)
+        times["scatter_moments"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pool.map(_w_sweep, sweep_sets)
+        t_sweep = time.perf_counter() - t0
+        times["sweep"] = t_sweep * M / (8 * P)
+        per_direction = pool.map(_w_image, list(range(P)))  # each timed inside its worker, all P running at once
+        times["image"] = float(np.mean(per_direction)) * cone.shape[0] / P
+        t0 = time.perf_counter()
+        pool.map(_w_primary, pix_chunks)
+        times["primary"] = (time.perf_counter() - t0) * 64
+    _CPU.clear()
     per_view = sum(times.values())
-    return {"value": 1.0 / per_view, "unit": "view-sweeps/s", "cores": 1, "kind": "port",
-            "sample": "one view step of the oracle's algorithm (numpy, one thread), each stage on a sample scaled to "
-                      "the view: " + ", ".join(f"{k} {v:.1f} s" for k, v in times.items())
-                      + f" = {per_view:.1f} s per view step (uncollided, first collision and downstream/oblique "
-                      "rays on 1/64 of the points and pixels, 1 of 512 cone directions, the primary on 1/64 of the "
-                      "pixel rays; the sweep and scattering moments as in the sweep-only sample: " + sweep_part["sample"]
-                      + ")"}
+    sweep_only = times["scatter_moments"] + times["sweep"]
+    return {"value": 1.0 / per_view, "unit": "view-sweeps/s", "cores": P, "kind": "port",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
+            "sweep_only_value": 1.0 / sweep_only,
+            "sample": f"one view step of the oracle's algorithm (numpy float32/float64) on {P} worker processes of "
+                      "one thread each, every stage's sample split over them and timed by wall clock: "
+                      + ", ".join(f"{k} {v:.1f} s" for k, v in times.items())
+                      + f" = {per_view:.1f} s per view step (uncollided, first collision and primary on 1/64 of the "
+                        f"points or pixel rays; scattering moments on every cell; the sweep (source expansion, LD "
+                        f"sweep, moment reduction) on {8 * P} of {M} directions x 16 groups x {n}^3 cells; the image "
+                        f"(downstream depths and oblique rays on 1/64 of the cells and pixels, the emission on every "
+                        f"cell) on {P} of {cone.shape[0]} cone directions, one per worker, each timed in its worker "
+                        f"while all run at once; each stage scaled to the whole view)"}
 
 
 def sweep_launches(grid_cells: int, block: int = 4) -> int:
@@ -478,6 +522,9 @@ def main():
               "transport step is sharded by views across GPUs by design but this build measures one GPU.",
               file=sys.stderr)
         sys.exit(2)
+    cpu = None
+    if not args.no_cpu_baseline:  # on the host's cores, before anything touches the GPU (workers are forked)
+        cpu = cpu_baseline_parallel(args)
     import torch
 
     from citrus_amd.citrus import CitrusState, citrus_views_step
@@ -666,13 +713,10 @@ def main():
                      "traffic_frac": (traffic / (sweep_ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                      "traffic_source": traffic_source},
     }
-    if not args.no_cpu_baseline:
-        sweep_part = cpu_baseline(args, ctx, library, rho, coarse)
-        if view is not None:  # the metric's unit is the view step: time the oracle's whole step on its samples
-            result["cpu_baseline"] = cpu_view_step_baseline(args, ctx, library, rho, coarse, sweep_part)
-            result["cpu_baseline"]["sweep_only_value"] = sweep_part["value"]
-        else:
-            result["cpu_baseline"] = {k: v for k, v in sweep_part.items() if k not in ("moments", "sweep")}
+    if cpu is not None:
+        if view is None:  # sweep-only unit: the scattering moments and the sweep of the same run
+            cpu = {**cpu, "value": cpu["sweep_only_value"]}
+        result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
 
 
