@@ -43,6 +43,12 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
     ap.add_argument("--no-helical", action="store_true", help="skip the helical cropped-grid comparison")
     ap.add_argument("--cpu-directions", type=int, default=8, help="directions in the CPU-baseline sample")
+    ap.add_argument("--schedule", choices=("auto", "persistent", "launches"), default="auto",
+                    help="sweep schedule; 'auto' lets the autotuner pick it with the direction chunk")
+    ap.add_argument("--chunk", type=int, default=0, help="directions per octant per sweep launch (0: the tuner's)")
+    ap.add_argument("--profile", action="store_true",
+                    help="the timed view steps and sweeps only (no bf16 form, scan stage, learned form, helical "
+                         "configuration, graph or CPU baseline): the run to trace or count under rocprofv3")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a float32 buffer of MIB MiB three times on cuda:0 (the known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -511,6 +517,38 @@ def learned_vs_plain(solver, fields, angles, device):
                     "numerator) and of the scalar-flux cell means, against a source iteration converged to 1e-5"}
 
 
+def mixed_precision_steps(view_step, states, images, V: int, args, stream) -> dict:
+    """The same view steps with the mixed-precision source contraction (degrees l <= 1 in float32, l >= 2 as a
+    bfloat16 matrix product, float32 accumulation) from empty states: rate, and errors at the last step against the
+    float32 steps (which the oracle tests pin to the FP64 restatement at 1e-5)."""
+    import torch
+
+    from citrus_amd.citrus import CitrusState
+
+    states16 = [CitrusState() for _ in range(V)]
+    for _ in range(max(1, args.warmup)):
+        view_step("bf16", states16)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        images16 = view_step("bf16", states16)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms16 = start.elapsed_time(stop)
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    return {"bf16": {
+        "view_steps_per_s": V * args.steps / (ms16 * 1e-3), "ms_per_step": ms16 / args.steps,
+        "image_error_vs_fp32": max(rel(a[0], b[0]) for a, b in zip(images16, images)),
+        "spr_error_vs_fp32": max(rel(a[0] / a[1], b[0] / b[1]) for a, b in zip(images16, images)),
+        "accumulator_error_vs_fp32": max(rel(a.collided_moments, b.collided_moments)
+                                         for a, b in zip(states16, states)),
+        "note": "the step's scattering-source expansion in mixed precision: degrees l <= 1 in float32, l >= 2 as a "
+                "bfloat16 matrix product with float32 accumulation, the operand written by the scattering-moment "
+                "kernel; errors are relative L2 against the float32 steps from the same empty states after "
+                f"{max(1, args.warmup) + args.steps} steps"}}
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -522,6 +560,8 @@ def main():
               "transport step is sharded by views across GPUs by design but this build measures one GPU.",
               file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        args.no_cpu_baseline = args.no_learned = args.no_helical = True
     cpu = None
     if not args.no_cpu_baseline:  # on the host's cores, before anything touches the GPU (workers are forked)
         cpu = cpu_baseline_parallel(args)
@@ -543,9 +583,12 @@ def main():
     from citrus_amd.scatter_solver.autotune import default_tuner
 
     ctx_t, mu_t_t = build_problem(argparse.Namespace(**{**vars(args), "views": 1}), device)[:2]
-    schedule, chunk = default_tuner().best_config(ctx_t, mu_t_t, V)
-    tuned = {f"{s}:{c}": round(v, 3)
-             for (s, c), v in default_tuner().timings[("config",) + default_tuner().key(ctx_t, 16, V, "fp32")].items()}
+    if args.schedule != "auto" and args.chunk > 0:
+        schedule, chunk, tuned = args.schedule, min(args.chunk, ctx_t.directions_per_octant), {}
+    else:
+        schedule, chunk = default_tuner().best_config(ctx_t, mu_t_t, V)
+        tuned = {f"{s}:{c}": round(v, 3) for (s, c), v in
+                 default_tuner().timings[("config",) + default_tuner().key(ctx_t, 16, V, "fp32")].items()}
     del ctx_t, mu_t_t
 
     # ---- CITRUS view steps (the metric) --------------------------------------------------------------------
@@ -575,31 +618,14 @@ def main():
         assert all(bool(torch.isfinite(s.collided_moments).all()) for s in states), "non-finite accumulator"
         view["spr"] = spr
         view["history"] = [round(h, 6) for h in states[0].history]
-        # the same steps with the bfloat16 source contraction: rate and error against the float32 steps
-        states16 = [CitrusState() for _ in range(V)]
-        for _ in range(max(1, args.warmup)):
-            view_step("bf16", states16)
-        torch.cuda.synchronize()
-        start.record(stream)
-        for _ in range(args.steps):
-            images16 = view_step("bf16", states16)
-        stop.record(stream)
-        torch.cuda.synchronize()
-        ms16 = start.elapsed_time(stop)
-        rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
-        view["bf16"] = {
-            "view_steps_per_s": V * args.steps / (ms16 * 1e-3), "ms_per_step": ms16 / args.steps,
-            "image_error_vs_fp32": max(rel(a[0], b[0]) for a, b in zip(images16, images)),
-            "spr_error_vs_fp32": max(rel(a[0] / b[1], c[0] / c[1]) for a, b, c in zip(images16, images16, images)),
-            "accumulator_error_vs_fp32": max(rel(a.collided_moments, b.collided_moments)
-                                             for a, b in zip(states16, states)),
-            "note": "the step's scattering-source expansion as a bfloat16 matrix product (float32 accumulation, "
-                    "per-(view, group) power-of-two scales); errors are relative L2 against the float32 steps from "
-                    "the same empty states, which match the FP64 oracle to 1e-5 at the tested sizes"}
-        del states, images, states16, images16
+        # the same steps with the mixed-precision source contraction: rate and error against the float32 steps
+        if not args.profile:
+            view.update(mixed_precision_steps(view_step, states, images, V, args, stream))
+        del states, images
         torch.cuda.empty_cache()
-        view["scan_stage"] = scan_stage(solver, fields, device, views=16, batch=V)
-        torch.cuda.empty_cache()
+        if not args.profile:
+            view["scan_stage"] = scan_stage(solver, fields, device, views=16, batch=V)
+            torch.cuda.empty_cache()
         if not args.no_learned:
             try:
                 view["learned"] = learned_vs_plain(solver, fields, angles, device)
@@ -644,16 +670,18 @@ def main():
     # the same batched sweep captured once in a HIP graph and replayed
     from citrus_amd.scatter_solver.transport_sweep import SweepGraph
 
-    graph = SweepGraph(ctx, mu_t, T, direction_chunk=chunk, schedule=schedule)
-    torch.cuda.synchronize()
-    g_start, g_stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    g_start.record(stream)
-    for _ in range(sweep_steps):
-        graph.replay()
-    g_stop.record(stream)
-    torch.cuda.synchronize()
-    graph_ms = g_start.elapsed_time(g_stop) / sweep_steps
-    del graph
+    graph_ms = None
+    if not args.profile:
+        graph = SweepGraph(ctx, mu_t, T, direction_chunk=chunk, schedule=schedule)
+        torch.cuda.synchronize()
+        g_start, g_stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        g_start.record(stream)
+        for _ in range(sweep_steps):
+            graph.replay()
+        g_stop.record(stream)
+        torch.cuda.synchronize()
+        graph_ms = g_start.elapsed_time(g_stop) / sweep_steps
+        del graph
     G, nsh = 16, ctx.num_moments
     N = ctx.num_cells
     # compulsory bytes of one view-sweep in the sweep kernel: moments T in and phi out (float32) plus mu_t. The
@@ -695,7 +723,7 @@ def main():
                    "grid": f"{coarse.shape[0]}^3 cells at {coarse.delta_voxel:.2f} cm", "groups": G,
                    "directions": ctx.num_directions, "moments": nsh, "views_per_step": V,
                    "sweep_only_view_sweeps_per_s": sweep_rate, "sweep_ms_per_view": sweep_ms / V,
-                   "sweep_graph_ms_per_view": graph_ms / V,
+                   "sweep_graph_ms_per_view": None if graph_ms is None else graph_ms / V,
                    "schedule": schedule, "direction_chunk": chunk, "tuner_ms_per_schedule_chunk": tuned,
                    "wall_s": wall,
                    **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),

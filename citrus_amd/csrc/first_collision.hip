@@ -218,12 +218,14 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
   PointData<GMAX> pd;
   load_point(d, p, pd);
   f4 I0 = {0.f, 0.f, 0.f, 0.f}, I1 = I0, I2 = I0;
-  // directions in runs of four: with samples and K % 4 == 0 each group's four samples leave as one 16-byte store
-  const bool vec = samples && (d.K % 4) == 0;
-  for (int k0 = 0; k0 < d.K; k0 += 4) {
-    f4 run[4];
+  // directions in runs of eight: with samples and K % 8 == 0 each group's eight samples leave as two adjacent 16-byte
+  // stores, one whole 32-byte sector (runs of four leave half sectors, which reach memory as whole ones)
+  constexpr int RUN = 8;
+  const bool vec = samples && (d.K % RUN) == 0;
+  for (int k0 = 0; k0 < d.K; k0 += RUN) {
+    f4 run[4][RUN / 4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < RUN; ++j) {
       const int k = k0 + j;
       if (k >= d.K) break;
       const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
@@ -236,15 +238,19 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
         I0[gl] = fmaf(w, S[gl], I0[gl]);
         I1[gl] = fmaf(we, S[gl], I1[gl]);
         I2[gl] = fmaf(wee, S[gl], I2[gl]);
-        run[gl][j] = S[gl];
+        run[gl][j / 4][j % 4] = S[gl];
         const int g = 4 * ch + gl;
         if (samples && !vec && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[gl];
       }
     }
     if (vec) {
 #pragma unroll
-      for (int gl = 0; gl < 4; ++gl)
-        if (4 * ch + gl < d.G) *reinterpret_cast<f4*>(samples + ((int64_t)(4 * ch + gl) * d.P + p) * d.K + k0) = run[gl];
+      for (int gl = 0; gl < 4; ++gl) {
+        if (4 * ch + gl >= d.G) continue;
+        f4* dst = reinterpret_cast<f4*>(samples + ((int64_t)(4 * ch + gl) * d.P + p) * d.K + k0);
+#pragma unroll
+        for (int h = 0; h < RUN / 4; ++h) dst[h] = run[gl][h];
+      }
     }
   }
   // exact Legendre moments l = 0, 1 of the same source (no quadrature)

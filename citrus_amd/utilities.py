@@ -163,3 +163,60 @@ def wlab_geometry() -> ConeGeometry:
     """The WLAB cone-beam setup (utilities.py:514): SID 100 cm, SDD 150 cm, a flat 100 x 75 panel of 4 mm pixels."""
     return ConeGeometry(source_iso_dist=100.0, source_detector_dist=150.0, num_det_rows=75, num_det_channels=100,
                         delta_det_row=0.4, delta_det_channel=0.4)
+
+
+# ---- data sets (utilities.py:1480-1527) ------------------------------------------------------------------------------
+# A data set is a dict of numpy arrays (sinograms, angles, images, ...) plus "meta", a JSON-serialisable dict. On disk
+# it is <base>.npz holding the arrays and the meta as the JSON string "meta_json", and <base>.json with the meta alone
+# (the reference's layout, so either implementation reads the other's files). Format 3 and later only.
+DATASET_FORMAT = 3
+
+
+def _dataset_base(path) -> str:
+    base = str(path)
+    for ext in (".npz", ".json"):
+        if base.endswith(ext):
+            return base[: -len(ext)]
+    return base
+
+
+def save_dataset(data: dict, out_path) -> list:
+    """Write ``data`` (arrays plus "meta") as ``<base>.npz`` and ``<base>.json`` (a trailing .npz or .json of
+    ``out_path`` is dropped); sets ``meta["files"]`` to the two paths and returns them."""
+    import json
+    import os
+
+    if "meta" not in data or not isinstance(data["meta"], dict):
+        raise ValueError("a data set needs a 'meta' dict")
+    base = _dataset_base(out_path)
+    folder = os.path.dirname(os.path.abspath(base))
+    os.makedirs(folder, exist_ok=True)
+    meta = data["meta"]
+    meta.setdefault("data_format", DATASET_FORMAT)
+    text = json.dumps(meta)
+    arrays = {name: np.asarray(value) for name, value in data.items() if name != "meta"}
+    if "meta_json" in arrays:
+        raise ValueError("'meta_json' is reserved for the data set's meta")
+    np.savez(base + ".npz", meta_json=np.asarray(text), **arrays)
+    with open(base + ".json", "w") as f:
+        f.write(json.dumps(meta, indent=2))
+    meta["files"] = [base + ".npz", base + ".json"]
+    return meta["files"]
+
+
+def load_dataset(path) -> dict:
+    """Read a data set written by ``save_dataset`` (here or by the reference): the arrays and "meta". Nothing in the
+    file is unpickled (``numpy.load`` without pickles); a data set older than format 3 raises ``ValueError``."""
+    import json
+
+    base = _dataset_base(path)
+    with np.load(base + ".npz", allow_pickle=False) as z:
+        if "meta_json" not in z.files:
+            raise ValueError(f"{base}.npz holds no 'meta_json': not a data set")
+        out = {name: z[name] for name in z.files if name != "meta_json"}
+        meta = json.loads(str(z["meta_json"][()]))
+    if int(meta.get("data_format", 0)) < DATASET_FORMAT:
+        raise ValueError(f"{base}.npz has data format {meta.get('data_format', 0)} (< {DATASET_FORMAT}): written "
+                         "before the polychromatic primary; synthesize it again")
+    out["meta"] = meta
+    return out

@@ -271,3 +271,29 @@ def test_csi_and_lead_tables_and_csi_response():
     g = torch.tensor([30.0, 35.98 + 1e-3, 70.0], dtype=torch.float64)
     d0 = r.at(g, torch.zeros(1, dtype=torch.float64))[:, 0]
     assert float(d0[1]) > float(d0[0])  # the Cs K edge raises the absorbed share
+
+
+def test_dataset_round_trip_and_the_reference_layout(tmp_path):
+    """save_dataset / load_dataset (utilities.py:1480-1527): arrays and meta round-trip; a file in the reference's
+    layout (np.savez with the meta as the JSON string meta_json) reads; format < 3 and a missing meta are refused."""
+    import json
+
+    from citrus_amd.utilities import load_dataset, save_dataset
+
+    data = {"sinogram": np.arange(24, dtype=np.float32).reshape(2, 3, 4), "angles": np.array([0.0, 0.5]),
+            "meta": {"description": "test", "data_format": 3, "num_views": 2}}
+    files = save_dataset(data, tmp_path / "sub" / "scan.npz")
+    assert files == [str(tmp_path / "sub" / "scan.npz"), str(tmp_path / "sub" / "scan.json")]
+    back = load_dataset(tmp_path / "sub" / "scan")
+    assert np.array_equal(back["sinogram"], data["sinogram"]) and back["sinogram"].dtype == np.float32
+    assert back["meta"]["num_views"] == 2 and json.loads((tmp_path / "sub" / "scan.json").read_text())["num_views"] == 2
+    # written the way the reference writes it
+    meta = {"data_format": 3, "spr": 0.2}
+    np.savez(tmp_path / "ref.npz", meta_json=np.asarray(json.dumps(meta)), counts=np.ones(5))
+    ref = load_dataset(tmp_path / "ref.npz")
+    assert ref["meta"]["spr"] == 0.2 and ref["counts"].sum() == 5
+    np.savez(tmp_path / "old.npz", meta_json=np.asarray(json.dumps({"data_format": 2})), counts=np.ones(5))
+    with pytest.raises(ValueError, match="format"):
+        load_dataset(tmp_path / "old.npz")
+    with pytest.raises(ValueError):
+        save_dataset({"x": np.ones(2)}, tmp_path / "bad")
