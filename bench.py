@@ -612,7 +612,11 @@ def main():
             images = view_step()
         stop.record(stream)
         torch.cuda.synchronize()
-        view = {"wall": time.perf_counter() - t0, "ms": start.elapsed_time(stop)}
+        view = {"wall": time.perf_counter() - t0, "ms": start.elapsed_time(stop),
+                # HBM of the view steps: the peak within a step, and what stays allocated between steps (states,
+                # material fields, solver tables and the last step's images)
+                "memory_gib": {"step_peak": torch.cuda.max_memory_allocated(device) / 2**30,
+                               "between_steps": torch.cuda.memory_allocated(device) / 2**30}}
         scatter, primary = images[0]
         spr = float((scatter.sum() / primary.sum()).item())
         assert all(bool(torch.isfinite(s.collided_moments).all()) for s in states), "non-finite accumulator"
@@ -725,8 +729,9 @@ def main():
                    "sweep_only_view_sweeps_per_s": sweep_rate, "sweep_ms_per_view": sweep_ms / V,
                    "sweep_graph_ms_per_view": None if graph_ms is None else graph_ms / V,
                    "schedule": schedule, "direction_chunk": chunk, "tuner_ms_per_schedule_chunk": tuned,
-                   "wall_s": wall,
-                   **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"]} if view else {}),
+                   "wall_s": wall, "peak_allocated_gib": torch.cuda.max_memory_allocated(device) / 2**30,
+                   **({"spr_view0": view["spr"], "accumulator_change_view0": view["history"],
+                       "view_step_memory_gib": view["memory_gib"]} if view else {}),
                    **({"learned_vs_plain": view["learned"]} if view and "learned" in view else {}),
                    **({"bf16_contraction": view["bf16"]} if view and "bf16" in view else {}),
                    **({"scan_stage": view["scan_stage"]} if view and "scan_stage" in view else {}),

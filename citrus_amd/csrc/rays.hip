@@ -455,7 +455,26 @@ __global__ void crossing_sequence_kernel(Box B, const double* __restrict__ omega
   rec[1] = rec[2] = rec[3] = 0;
 }
 
+// The channels of one voxel at byte offset `off` of the volume: one vector load for 2 or 4 channels.
 template <int CH>
+__device__ __forceinline__ void load_channels(const char* __restrict__ vb, unsigned off, float v[CH]) {
+  if constexpr (CH == 2) {
+    const float2 w = *reinterpret_cast<const float2*>(vb + off);
+    v[0] = w.x; v[1] = w.y;
+  } else if constexpr (CH == 4) {
+    const float4 w = *reinterpret_cast<const float4*>(vb + off);
+    v[0] = w.x; v[1] = w.y; v[2] = w.z; v[3] = w.w;
+  } else {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) v[c] = reinterpret_cast<const float*>(vb + off)[c];
+  }
+}
+
+// EXACT: nch == CH and the volume is below 2^31 bytes. The samples of a ray that leaves the grid before the panel are
+// then taken eight at a time: their offsets and chords from LDS and their channels (32-bit byte offsets, one vector
+// load per voxel) are all requested before the first is added, and the additions run in sample order, so the sums are
+// those of the one-sample loop bit for bit.
+template <int CH, bool EXACT>
 __global__ __launch_bounds__(256) void downstream_shared_kernel(Box B, Panel P, const float* __restrict__ vol,
                                                                 int nch, const double* __restrict__ omega,
                                                                 const int* __restrict__ records, int64_t words,
@@ -493,23 +512,43 @@ __global__ __launch_bounds__(256) void downstream_shared_kernel(Box B, Panel P, 
                   B.lo[2] + ((double)kk + 0.5) * B.delta[2]};
   const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
   const double tp = fmax(panel_distance(P, p0, w), 0.0) * wn;  // cm to the panel
-  const float* base = vol + i * nch;
+  const int nc = EXACT ? CH : nch;
+  const float* base = vol + i * nc;
   // FP64 sums: a float32 value times a float32 length is exact in FP64, so each term is added with one rounding
   // (2 vector operations per channel and crossing against 5 for a compensated float32 sum, at the same FP64 rate)
   double sum[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) sum[c] = 0.0;
   auto add = [&](int s, float len) {
-    const float* v = base + (int64_t)flat[s] * nch;
+    const float* v = base + (int64_t)flat[s] * nc;
     const double l = (double)len;
 #pragma unroll
     for (int c = 0; c < CH; ++c)
-      if (c < nch) sum[c] = fma((double)v[c], l, sum[c]);
+      if (c < nc) sum[c] = fma((double)v[c], l, sum[c]);
   };
   if ((double)prefix[n_in] * (1.0 + 1e-5) + 1e-6 < tp) {
     // the panel lies beyond the grid exit: every in-grid sample counts in full
-#pragma unroll 4
-    for (int s = 0; s < n_in; ++s) add(s, chord[s]);
+    int s = 0;
+    if constexpr (EXACT) {
+      constexpr int U = 8;
+      const char* vb = reinterpret_cast<const char*>(vol);
+      const unsigned ib = (unsigned)i * (unsigned)(4 * CH);
+      for (; s + U <= n_in; s += U) {
+        float v[U][CH], len[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          len[u] = chord[s + u];
+          load_channels<CH>(vb, ib + (unsigned)(flat[s + u] * (4 * CH)), v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const double l = (double)len[u];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) sum[c] = fma((double)v[u][c], l, sum[c]);
+        }
+      }
+    }
+    for (; s < n_in; ++s) add(s, chord[s]);
   } else {
     double cum = 0.0;
     for (int s = 0; s < n_in; ++s) {
@@ -527,13 +566,13 @@ __global__ __launch_bounds__(256) void downstream_shared_kernel(Box B, Panel P, 
       float acc = 0.0f;
 #pragma unroll
       for (int c = 0; c < CH; ++c)
-        if (c < nch) acc = fmaf(mix[g * nch + c], (float)sum[c], acc);
+        if (c < nc) acc = fmaf(mix[g * nc + c], (float)sum[c], acc);
       o[(int64_t)g * N] = acc;
     }
   } else {
 #pragma unroll
     for (int c = 0; c < CH; ++c)
-      if (c < nch) o[(int64_t)c * N] = (float)sum[c];
+      if (c < nc) o[(int64_t)c * N] = (float)sum[c];
   }
 }
 
@@ -688,20 +727,16 @@ extern "C" int citrus_ray_downstream_shared(const float* volume, int R, int C, i
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)((N + 255) / 256), (unsigned)K), block(256);
   const size_t lds = (size_t)words * 4;
-  if (nch <= 1)
-    hipLaunchKernelGGL((downstream_shared_kernel<1>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
-                       words, mix, nout, out);
-  else if (nch <= 2)
-    hipLaunchKernelGGL((downstream_shared_kernel<2>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
-                       words, mix, nout, out);
-  else if (nch <= 4)
-    hipLaunchKernelGGL((downstream_shared_kernel<4>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
-                       words, mix, nout, out);
-  else if (nch <= 8)
-    hipLaunchKernelGGL((downstream_shared_kernel<8>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
-                       words, mix, nout, out);
-  else
-    hipLaunchKernelGGL((downstream_shared_kernel<16>), grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec,
-                       words, mix, nout, out);
+  auto launch = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, block, lds, s, J.box, J.panel, volume, nch, omega, rec, words, mix, nout, out);
+  };
+  const bool small = N * nch * 4 < ((int64_t)1 << 31);  // 32-bit byte offsets of the batched samples
+  if (small && nch == 1) launch(downstream_shared_kernel<1, true>);
+  else if (small && nch == 2) launch(downstream_shared_kernel<2, true>);
+  else if (small && nch == 3) launch(downstream_shared_kernel<3, true>);
+  else if (small && nch == 4) launch(downstream_shared_kernel<4, true>);
+  else if (nch <= 4) launch(downstream_shared_kernel<4, false>);
+  else if (nch <= 8) launch(downstream_shared_kernel<8, false>);
+  else launch(downstream_shared_kernel<16, false>);
   return finish(hipGetLastError(), "citrus_ray_downstream_shared");
 }

@@ -193,6 +193,12 @@ def test_downstream_shared_sequence_matches_oracle(curved, angle):
     assert rel_err(got_mix[:5], per_group) < F32_REL
     assert proj.downstream_cumulative_mixed(rho_t, None, grid, geo, angle, torch.zeros(0, 3)).shape == (0, 2, 12, 14,
                                                                                                        10)
+    # 1 to 4 channels take their samples eight at a time, more than 4 one at a time: the same sums bit for bit
+    extra = torch.from_numpy(rng.random((3, *grid.shape)).astype(np.float32)).to(dev)
+    five = proj.downstream_cumulative_mixed(torch.cat([rho_t, extra]), None, grid, geo, angle, torch.from_numpy(w))
+    assert torch.equal(five[:, :2].cpu(), torch.from_numpy(got).float())
+    one = proj.downstream_cumulative_mixed(rho_t[1:].contiguous(), None, grid, geo, angle, torch.from_numpy(w))
+    assert torch.equal(one[:, 0], five[:, 1])
 
 
 def test_downstream_shared_sequence_limits():
