@@ -112,9 +112,17 @@ __device__ __forceinline__ void load_point(const FcDev& d, int p, PointData<GMAX
   for (int g = 0; g < GMAX; ++g) pd.ph[g] = g < d.G ? d.phi_uc[(int64_t)g * d.P + p] : 0.0f;
 }
 
-// acc += cv (a + w1 (b - a)): one table term, linear in eta between two nodes (one expression, so that the per-channel
-// and the all-channel forms contract it alike)
-__device__ __forceinline__ f4 table_term(f4 acc, float cv, f4 a, f4 b, float w1) { return acc + cv * (a + w1 * (b - a)); }
+// The source is linear in the table, so the two nodes are contracted separately, acc0 += cv a and acc1 += cv b, and
+// interpolated once at the end, acc0 + w1 (acc1 - acc0): two operations per table entry instead of three. Explicit
+// fused multiply-adds, so that the per-channel and the all-channel forms round alike.
+__device__ __forceinline__ f4 fma4(float c, f4 a, f4 acc) {
+  return f4{fmaf(c, a.x, acc.x), fmaf(c, a.y, acc.y), fmaf(c, a.z, acc.z), fmaf(c, a.w, acc.w)};
+}
+
+__device__ __forceinline__ f4 interpolate4(f4 acc0, f4 acc1, float w1) {
+  return f4{fmaf(w1, acc1.x - acc0.x, acc0.x), fmaf(w1, acc1.y - acc0.y, acc0.y), fmaf(w1, acc1.z - acc0.z, acc0.z),
+            fmaf(w1, acc1.w - acc0.w, acc0.w)};
+}
 
 // S[go] at one eta; entries go < g' of the table are zero (downscatter only)
 template <int GMAX>
@@ -123,9 +131,9 @@ __device__ __forceinline__ void eval_source(const FcDev& d, const PointData<GMAX
   float w1;
   table_position(d, eta, i0, w1);
   constexpr int NCH = GMAX / 4;
-  f4 acc[NCH];
+  f4 acc0[NCH], acc1[NCH];
 #pragma unroll
-  for (int ch = 0; ch < NCH; ++ch) acc[ch] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int ch = 0; ch < NCH; ++ch) acc0[ch] = acc1[ch] = f4{0.f, 0.f, 0.f, 0.f};
   // rows are GMAX wide (the host pads g_out to the kernel's group count); rows g' >= G are clamped and carry phi 0
   const int64_t node = (int64_t)d.M * d.G * GMAX;
   for (int s = 0; s < d.n_slots; ++s) {
@@ -142,42 +150,20 @@ __device__ __forceinline__ void eval_source(const FcDev& d, const PointData<GMAX
       const f4* r0 = reinterpret_cast<const f4*>(t0 + row * GMAX);
       const f4* r1 = reinterpret_cast<const f4*>(t0 + node + row * GMAX);
 #pragma unroll
-      for (int ch = gi / 4; ch < NCH; ++ch) acc[ch] = table_term(acc[ch], cv, r0[ch], r1[ch], w1);
+      for (int ch = gi / 4; ch < NCH; ++ch) {
+        acc0[ch] = fma4(cv, r0[ch], acc0[ch]);
+        acc1[ch] = fma4(cv, r1[ch], acc1[ch]);
+      }
     }
   }
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
-    S[4 * ch + 0] = acc[ch].x;
-    S[4 * ch + 1] = acc[ch].y;
-    S[4 * ch + 2] = acc[ch].z;
-    S[4 * ch + 3] = acc[ch].w;
+    const f4 v = interpolate4(acc0[ch], acc1[ch], w1);
+    S[4 * ch + 0] = v.x;
+    S[4 * ch + 1] = v.y;
+    S[4 * ch + 2] = v.z;
+    S[4 * ch + 3] = v.w;
   }
-}
-
-// S of the four output groups 4 ch .. 4 ch + 3 at one eta: eval_source restricted to one channel (the same terms in
-// the same order, so the values are equal bit for bit)
-template <int GMAX>
-__device__ __forceinline__ f4 eval_source_channel(const FcDev& d, const PointData<GMAX>& pd, float eta, int ch) {
-  int i0;
-  float w1;
-  table_position(d, eta, i0, w1);
-  f4 acc = {0.f, 0.f, 0.f, 0.f};
-  const int64_t node = (int64_t)d.M * d.G * GMAX;
-  for (int s = 0; s < d.n_slots; ++s) {
-    const float rho = d.slot_density[(int64_t)s * d.P + pd.p];
-    if (rho == 0.0f) continue;  // as in eval_source
-    const float* t0 = d.table + (int64_t)i0 * node + (int64_t)d.slot_material[(int64_t)s * d.P + pd.p] * d.G * GMAX;
-#pragma unroll
-    for (int gi = 0; gi < GMAX; ++gi) {
-      if (gi / 4 > ch) break;
-      const float cv = rho * pd.ph[gi];
-      const int row = gi < d.G ? gi : d.G - 1;
-      const f4* r0 = reinterpret_cast<const f4*>(t0 + row * GMAX);
-      const f4* r1 = reinterpret_cast<const f4*>(t0 + node + row * GMAX);
-      acc = table_term(acc, cv, r0[ch], r1[ch], w1);
-    }
-  }
-  return acc;
 }
 
 template <int GMAX>
@@ -199,60 +185,132 @@ __global__ __launch_bounds__(kBlock) void fc_source_kernel(FcDev d, float* __res
     if (g < d.G) out[((int64_t)k * d.G + g) * d.P + p] = S[g];
 }
 
+// The material slots of one point, held in registers for the directions of a point-major loop.
+struct PointSlots {
+  float rho[kMaxSlots];
+  int mat[kMaxSlots];
+};
+
+__device__ __forceinline__ void load_slots(const FcDev& d, int p, PointSlots& ps) {
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    ps.rho[s] = s < d.n_slots ? d.slot_density[(int64_t)s * d.P + p] : 0.0f;
+    ps.mat[s] = s < d.n_slots ? d.slot_material[(int64_t)s * d.P + p] : 0;
+  }
+}
+
+// S of the four output groups 4 ch .. 4 ch + 3 at one eta, for a wave-uniform channel with the slots in registers:
+// eval_source restricted to one channel (the same terms in the same order, so the values are equal bit for bit), the
+// table rows of four source groups requested together before they are added.
+template <int GMAX>
+__device__ __forceinline__ f4 eval_source_channel_batched(const FcDev& d, const PointData<GMAX>& pd,
+                                                          const PointSlots& ps, float eta, int ch) {
+  int i0;
+  float w1;
+  table_position(d, eta, i0, w1);
+  f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+  const int64_t node = (int64_t)d.M * d.G * GMAX;
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    if (s >= d.n_slots) break;
+    const float rho = ps.rho[s];
+    if (rho == 0.0f) continue;  // as in eval_source
+    const float* t0 = d.table + (int64_t)i0 * node + (int64_t)ps.mat[s] * d.G * GMAX;
+#pragma unroll
+    for (int gb = 0; gb < GMAX / 4; ++gb) {
+      if (gb > ch) break;
+      f4 x0[4], x1[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int gi = 4 * gb + j;
+        const int row = gi < d.G ? gi : d.G - 1;
+        x0[j] = reinterpret_cast<const f4*>(t0 + row * GMAX)[ch];
+        x1[j] = reinterpret_cast<const f4*>(t0 + node + row * GMAX)[ch];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float cv = rho * pd.ph[4 * gb + j];
+        acc0 = fma4(cv, x0[j], acc0);
+        acc1 = fma4(cv, x1[j], acc1);
+      }
+    }
+  }
+  return interpolate4(acc0, acc1, w1);
+}
+
 // One thread per (point, output channel of four groups): the channel is uniform per workgroup (the heaviest, which
 // gathers the most table rows, first) and each thread runs all K directions for its four groups. Splitting the
-// groups over threads keeps the accumulators of a thread at 4 groups (about 70 VGPRs instead of 225), so 4x as many
-// points' gathers are in flight. Per group the terms and their order are those of the all-group form.
+// groups over threads keeps the accumulators of a thread at 4 groups, so 4x as many points' gathers are in flight.
+// The directions (unit vector and weight) pass through LDS in tiles of one per thread; the point's material slots stay
+// in registers. Per group the terms and their order are those of the all-group form.
 template <int GMAX>
 __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* __restrict__ weights,
                                                          const float* __restrict__ mu_s_l, int lp1,
                                                          float* __restrict__ out, float* __restrict__ samples,
                                                          int blocks_per_channel) {
+  __shared__ f4 sdir[kBlock];  // (omega x, y, z, weight) of the tile's directions
   constexpr int NCH = GMAX / 4;
   const int ch = NCH - 1 - (int)(blockIdx.x / blocks_per_channel);
   const int p = (int)(blockIdx.x % blocks_per_channel) * kBlock + threadIdx.x;
-  if (p >= d.P || 4 * ch >= d.G) return;
-  const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
+  if (4 * ch >= d.G) return;  // uniform over the workgroup
+  const bool active = p < d.P;
+  const int pp = active ? p : d.P - 1;  // inactive threads only help stage the directions
+  const int r = pp / (d.C * d.S), c = (pp / d.S) % d.C, s = pp % d.S;
   float ux, uy, uz;
   primary_direction(d, r, c, s, ux, uy, uz);
   PointData<GMAX> pd;
-  load_point(d, p, pd);
+  load_point(d, pp, pd);
+  PointSlots ps;
+  load_slots(d, pp, ps);
   f4 I0 = {0.f, 0.f, 0.f, 0.f}, I1 = I0, I2 = I0;
   // directions in runs of eight: with samples and K % 8 == 0 each group's eight samples leave as two adjacent 16-byte
   // stores, one whole 32-byte sector (runs of four leave half sectors, which reach memory as whole ones)
   constexpr int RUN = 8;
   const bool vec = samples && (d.K % RUN) == 0;
-  for (int k0 = 0; k0 < d.K; k0 += RUN) {
-    f4 run[4][RUN / 4];
-#pragma unroll
-    for (int j = 0; j < RUN; ++j) {
-      const int k = k0 + j;
-      if (k >= d.K) break;
-      const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
-      const f4 S = eval_source_channel(d, pd, eta, ch);
-      const float w = weights[k];
-      const float we = w * eta;
-      const float wee = we * eta;
-#pragma unroll
-      for (int gl = 0; gl < 4; ++gl) {
-        I0[gl] = fmaf(w, S[gl], I0[gl]);
-        I1[gl] = fmaf(we, S[gl], I1[gl]);
-        I2[gl] = fmaf(wee, S[gl], I2[gl]);
-        run[gl][j / 4][j % 4] = S[gl];
-        const int g = 4 * ch + gl;
-        if (samples && !vec && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[gl];
-      }
+  for (int t0 = 0; t0 < d.K; t0 += kBlock) {
+    const int kt = min(kBlock, d.K - t0);
+    __syncthreads();  // the previous tile is no longer read
+    if ((int)threadIdx.x < kt) {
+      const int k = t0 + (int)threadIdx.x;
+      sdir[threadIdx.x] = f4{d.omegas[3 * k], d.omegas[3 * k + 1], d.omegas[3 * k + 2], weights[k]};
     }
-    if (vec) {
+    __syncthreads();
+    if (!active) continue;
+    for (int j0 = 0; j0 < kt; j0 += RUN) {
+      const int k0 = t0 + j0;
+      f4 run[4][RUN / 4];
 #pragma unroll
-      for (int gl = 0; gl < 4; ++gl) {
-        if (4 * ch + gl >= d.G) continue;
-        f4* dst = reinterpret_cast<f4*>(samples + ((int64_t)(4 * ch + gl) * d.P + p) * d.K + k0);
+      for (int j = 0; j < RUN; ++j) {
+        const int k = k0 + j;
+        if (k >= d.K) break;
+        const f4 dir = sdir[j0 + j];
+        const float eta = ux * dir.x + uy * dir.y + uz * dir.z;
+        const f4 S = eval_source_channel_batched(d, pd, ps, eta, ch);
+        const float w = dir.w;
+        const float we = w * eta;
+        const float wee = we * eta;
 #pragma unroll
-        for (int h = 0; h < RUN / 4; ++h) dst[h] = run[gl][h];
+        for (int gl = 0; gl < 4; ++gl) {
+          I0[gl] = fmaf(w, S[gl], I0[gl]);
+          I1[gl] = fmaf(we, S[gl], I1[gl]);
+          I2[gl] = fmaf(wee, S[gl], I2[gl]);
+          run[gl][j / 4][j % 4] = S[gl];
+          const int g = 4 * ch + gl;
+          if (samples && !vec && g < d.G) samples[((int64_t)g * d.P + p) * d.K + k] = S[gl];
+        }
+      }
+      if (vec) {
+#pragma unroll
+        for (int gl = 0; gl < 4; ++gl) {
+          if (4 * ch + gl >= d.G) continue;
+          f4* dst = reinterpret_cast<f4*>(samples + ((int64_t)(4 * ch + gl) * d.P + p) * d.K + k0);
+#pragma unroll
+          for (int h = 0; h < RUN / 4; ++h) dst[h] = run[gl][h];
+        }
       }
     }
   }
+  if (!active) return;
   // exact Legendre moments l = 0, 1 of the same source (no quadrature)
   f4 E0 = {0.f, 0.f, 0.f, 0.f}, E1 = E0;
   for (int sl = 0; sl < d.n_slots; ++sl) {
@@ -404,10 +462,16 @@ __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* 
                                                           const float* __restrict__ signal_weight,
                                                           float* __restrict__ out, float* __restrict__ partial) {
   __shared__ float red[2][kBlock];
+  // the block's direction: its signal weights per group and the mix table, read from LDS in the group loop
+  __shared__ float s_weight[32];
+  __shared__ float s_mix[32 * kMaxTauChannels];
   const int k = blockIdx.y;
   const int p = blockIdx.x * kBlock + threadIdx.x;
   const int Rc = d.R / f, Cc = d.C / f, Sc = d.S / f;
   const int Nc = Rc * Cc * Sc;
+  if (signal_weight && threadIdx.x < d.G) s_weight[threadIdx.x] = signal_weight[(int64_t)k * d.G + threadIdx.x];
+  if (tau_mix && threadIdx.x < d.G * tau_channels) s_mix[threadIdx.x] = tau_mix[threadIdx.x];
+  __syncthreads();
   float esum = 0.0f, ehsum = 0.0f;
   if (p < d.P) {
     const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
@@ -452,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* 
         t = 0.0f;
 #pragma unroll
         for (int c = 0; c < kMaxTauChannels; ++c)
-          if (c < tau_channels) t = fmaf(tau_mix[g * tau_channels + c], tc[c], t);
+          if (c < tau_channels) t = fmaf(s_mix[g * tau_channels + c], tc[c], t);
         if (unreachable) t = INFINITY;
       } else {
         t = tau[kg * Nc + cell];
@@ -461,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* 
       const float e = COLLAPSED ? S[g] * expf(-t) : (S[g] + mc[g]) * expf(-t);
       esum += e;
       if (signal_weight) {
-        sig = fmaf(signal_weight[kg], e, sig);
+        sig = fmaf(s_weight[g], e, sig);
       } else {
         out[kg * d.P + p] = e;
       }
