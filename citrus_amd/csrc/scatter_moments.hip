@@ -19,6 +19,7 @@ namespace {
 
 constexpr int kCells = 2;
 constexpr int kMaxMaterials = 8;
+constexpr int kTableShare = 8;  // table words per thread held in registers while phi loads (4096 at 512 threads)
 
 __device__ __forceinline__ int degree_of(int n) {
   int l = 0;
@@ -58,47 +59,75 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
   const unsigned b = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
   const int v = (int)(b / cells_per_view);
   const int c0 = (int)(b % cells_per_view) * kCells;
-  if (FACTORED) {
-    const int nt = M * G * G * (Lt + 1);
-    for (int i = tid; i < nt; i += nthreads) table[i] = transfer[i];
-    __syncthreads();
-  }
-  // per-cell transfer matrices, entries (l, gi, go) with compile-time strides
-  const int entries = (L + 1) * GMAX * GMAX;
-  for (int cc = 0; cc < kCells; ++cc) {
-    const int cell = c0 + cc;
-    for (int r = tid; r < entries; r += nthreads) {
-      const int l = r / (GMAX * GMAX);
-      const int gi = (r / GMAX) % GMAX;
-      const int go = r % GMAX;
-      float m_val = 0.0f;
-      if (cell < N && gi < G && go < G && !(lower_only && gi > go)) {
-        if (FACTORED) {
-#pragma unroll
-          for (int m = 0; m < kMaxMaterials; ++m)
-            if (m < M)
-              m_val = fmaf(rho[(int64_t)m * N + cell], table[((m * G + gi) * G + go) * (Lt + 1) + l], m_val);
-        } else {
-          m_val = transfer[(((int64_t)gi * G + go) * (Lt + 1) + l) * N + cell];
-        }
-      }
-      mu[cc * per_cell + l * DS + gi * GMAX + go] = m_val;
-    }
-  }
-  __syncthreads();
+  // this thread's element (cell, n, k) and its phi of every input group, requested first so that the loads are in
+  // flight while the transfer matrices are formed
   const int elems = nsh * 4;
   const int cc = tid / elems;
   const int r = tid - cc * elems;
   const int cell = c0 + cc;
-  if (cc >= kCells || cell >= N) return;
-  const int l = degree_of(r / 4);
-  const int n = r >> 2, k = r & 3;
+  const bool active = cc < kCells && cell < N;
   const int64_t view_stride = (int64_t)G * N * elems;
   const int64_t group_stride = (int64_t)N * elems;
   const int64_t off = (int64_t)v * view_stride + (int64_t)cell * elems + r;
+  // the factored table's share of this thread (at most kTableShare words, else a loop after the phi loads), loaded
+  // before phi so that storing it to LDS waits for its own loads only
+  const int nt = FACTORED ? M * G * G * (Lt + 1) : 0;
+  const bool share_fits = nt <= kTableShare * nthreads;
+  float tv[kTableShare];
+  if (FACTORED && share_fits) {
+#pragma unroll
+    for (int j = 0; j < kTableShare; ++j) {
+      const int i = tid + j * nthreads;
+      tv[j] = transfer[i < nt ? i : 0];
+    }
+  }
+  // unconditional loads (inactive threads and groups past G read a valid element and discard it): no branches
+  // between the loads, so all of them are in flight together
+  const int64_t safe = active ? off : (int64_t)v * view_stride + (int64_t)c0 * elems;
   float p[GMAX];
 #pragma unroll
-  for (int gi = 0; gi < GMAX; ++gi) p[gi] = gi < G ? phi[off + gi * group_stride] : 0.0f;
+  for (int gi = 0; gi < GMAX; ++gi) {
+    const float x = phi[safe + (int64_t)(gi < G ? gi : G - 1) * group_stride];
+    p[gi] = gi < G ? x : 0.0f;  // inactive threads never use theirs
+  }
+  if (FACTORED) {
+    if (share_fits) {
+#pragma unroll
+      for (int j = 0; j < kTableShare; ++j) {
+        const int i = tid + j * nthreads;
+        if (i < nt) table[i] = tv[j];
+      }
+    } else {
+      for (int i = tid; i < nt; i += nthreads) table[i] = transfer[i];
+    }
+    __syncthreads();
+  }
+  // per-cell transfer matrices, entries (l, gi, go) with compile-time strides
+  const int entries = (L + 1) * GMAX * GMAX;
+  for (int mc = 0; mc < kCells; ++mc) {
+    const int mcell_index = c0 + mc;
+    for (int e = tid; e < entries; e += nthreads) {
+      const int l = e / (GMAX * GMAX);
+      const int gi = (e / GMAX) % GMAX;
+      const int go = e % GMAX;
+      float m_val = 0.0f;
+      if (mcell_index < N && gi < G && go < G && !(lower_only && gi > go)) {
+        if (FACTORED) {
+#pragma unroll
+          for (int m = 0; m < kMaxMaterials; ++m)
+            if (m < M)
+              m_val = fmaf(rho[(int64_t)m * N + mcell_index], table[((m * G + gi) * G + go) * (Lt + 1) + l], m_val);
+        } else {
+          m_val = transfer[(((int64_t)gi * G + go) * (Lt + 1) + l) * N + mcell_index];
+        }
+      }
+      mu[mc * per_cell + l * DS + gi * GMAX + go] = m_val;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int l = degree_of(r / 4);
+  const int n = r >> 2, k = r & 3;
   const float* mcell = mu + cc * per_cell + l * DS;
 #pragma unroll
   for (int go = 0; go < GMAX; ++go) {

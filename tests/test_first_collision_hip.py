@@ -85,10 +85,11 @@ def test_fc_source_matches_oracle(kind):
     assert rel_err(S.reshape(ref.shape), ref) < F32_REL
 
 
-def test_fc_sums_and_factors_match_oracle():
+@pytest.mark.parametrize("shape", [(8, 6, 7), (2, 8, 32)])
+def test_fc_sums_and_factors_match_oracle(shape):
     dev = gpu_device()
     library = seeded_library(8, 3, 120.0, 20.0, seed=12, num_eta=129)
-    grid = Grid((8, 6, 7), 0.5)
+    grid = Grid(shape, 0.5)
     rho, phi = _problem(library, grid.shape, 2)
     q = level_symmetric_quadrature(12)
     rho_d, phi_d = rho.float().to(dev), phi.float().to(dev)
@@ -111,11 +112,11 @@ def test_fc_sums_and_factors_match_oracle():
     assert np.allclose(s2.numpy(), s_ref, rtol=1e-14) and np.allclose(b2.numpy(), b_ref, atol=1e-14)
 
 
-@pytest.mark.parametrize("factor", [1, 2])
-def test_fc_moments_match_oracle_in_every_layout(factor):
+@pytest.mark.parametrize("factor,shape", [(1, (6, 4, 8)), (2, (6, 4, 8)), (1, (2, 8, 16))])
+def test_fc_moments_match_oracle_in_every_layout(factor, shape):
     dev = gpu_device()
     library = seeded_library(5, 3, 120.0, 20.0, seed=13, num_eta=97)
-    grid = Grid((6, 4, 8), 0.4)
+    grid = Grid(shape, 0.4)
     rho, phi = _problem(library, grid.shape, 3)
     q = level_symmetric_quadrature(6)
     rho_d, phi_d = rho.float().to(dev), phi.float().to(dev)

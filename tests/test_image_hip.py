@@ -60,10 +60,10 @@ def test_single_voxel_bookkeeping_on_gpu(angle):
     assert sig == pytest.approx(float((planar * lib.group_energy).sum()), rel=1e-5)
 
 
-def _image_problem(curved, q_factor, seed=5):
+def _image_problem(curved, q_factor, seed=5, shape=(4, 5, 3)):
     gen = torch.Generator().manual_seed(seed)
     library = seeded_library(3, 2, 120.0, 20.0, seed=21, num_eta=65)
-    coarse = Grid((4, 5, 3), 0.8)
+    coarse = Grid(shape, 0.8)
     gq = Grid(tuple(n * q_factor for n in coarse.shape), 0.8 / q_factor)
     rho = torch.rand(2, *gq.shape, generator=gen, dtype=torch.float64) * 1.2
     phi = torch.rand(3, *gq.shape, generator=gen, dtype=torch.float64) * 0.01 + 0.001
@@ -74,10 +74,11 @@ def _image_problem(curved, q_factor, seed=5):
     return library, coarse, gq, rho, phi, T, mu_c, geometry
 
 
-@pytest.mark.parametrize("curved,q_factor", [(False, 1), (False, 2), (True, 1)])
-def test_combined_scatter_image_matches_oracle(curved, q_factor):
+@pytest.mark.parametrize("curved,q_factor,shape", [(False, 1, (4, 5, 3)), (False, 2, (4, 5, 3)), (True, 1, (4, 5, 3)),
+                                                   (True, 1, (2, 8, 16))])
+def test_combined_scatter_image_matches_oracle(curved, q_factor, shape):
     dev = gpu_device()
-    library, coarse, gq, rho, phi, T, mu_c, geometry = _image_problem(curved, q_factor)
+    library, coarse, gq, rho, phi, T, mu_c, geometry = _image_problem(curved, q_factor, shape=shape)
     angle = 0.35
     quad = panel_direction_set(geometry, angle, (3, 6), max_polar_angle=gq)
     det, grid_response = energy_weighted_response(), no_grid()
