@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 12
+ABI_VERSION = 13
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None
@@ -42,7 +42,7 @@ class SweepView(ctypes.Structure):
 
     _fields_ = [
         ("mu_t", c_void), ("T", c_void), ("T_bf16", c_void), ("T_low", c_void), ("source", c_void),
-        ("phi", c_void), ("cdir", c_void),
+        ("phi", c_void), ("cdir", c_void), ("fc_scale", c_void), ("fc_beta", c_void), ("fc_u", c_void),
     ]
 
 
@@ -52,7 +52,7 @@ class PersistentArgs(ctypes.Structure):
     _fields_ = [
         ("G", c_int), ("nsh", c_int), ("D8", c_int), ("d0", c_int), ("dc", c_int), ("accumulate", c_int),
         ("source_layout", c_int), ("Y", c_void), ("Yw", c_void), ("views", ctypes.POINTER(SweepView)),
-        ("status", c_void), ("spin_limit", c_int),
+        ("status", c_void), ("spin_limit", c_int), ("fc_omega", c_void),
     ]
 
 
@@ -141,6 +141,8 @@ _SIGNATURES = {
     "citrus_fc_source": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
     "citrus_fc_sums": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void, c_void]),
     "citrus_fc_correct": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_void, c_void]),
+    "citrus_fc_primary_directions": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
+    "citrus_fc_correct_means": (c_int, [c_void, c_void, c_void, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),
     "citrus_fc_moments": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_int, c_void, c_void]),
     "citrus_attenuated_emission": (c_int, [ctypes.POINTER(FcArgs), c_void, c_int, c_void, c_void, c_void, c_int,
                                            c_void, c_void, c_void, c_void, c_void]),

@@ -23,6 +23,7 @@ from .scatter_solver.scatter_solver import MaterialFields, ScatterSolver
 from .scatter_solver.source_iteration import (MixedOperand, scatter_moments_mixed, scatter_moments_views,
                                               summed_source_function)
 from .scatter_solver.transport_sweep import sweep_views
+from .scatter_solver.uncollided import DeferredFirstCollision, first_collision_deferred_samples
 
 
 @dataclass
@@ -179,10 +180,19 @@ def _prepare_views(solver: ScatterSolver, states: Sequence[CitrusState], fields:
     cones = [st.cone_quadrature if st.cone_quadrature is not None else solver.cone_quadrature(grid, a)
              for st, a in zip(states, angles)]
     phi_uc = [uncollided_operator(solver, None, grid, a, fields) for a in angles]
-    src_shape = (V, G, R, C, S, M) if fields.q == 1 else (V, G, R, C, S, M, 4)
-    source = torch.empty(src_shape, device=device, dtype=torch.float32)
-    for v, a in enumerate(angles):
-        solver.first_collision_group_major(fields, a, phi_uc[v], ctx, out=source[v])
+    if fields.q == 1:
+        # one point per cell: the first-collision cell means stay uncorrected; the sweep applies the sampling
+        # correction as it reads them (bit for bit the corrected means)
+        source = DeferredFirstCollision.empty(V, G, (R, C, S), M, device)
+        omega = ctx.quadrature.omega.to(device, torch.float32)
+        for v, a in enumerate(angles):
+            first_collision_deferred_samples(solver.library, fields.rho_quadrature, phi_uc[v],
+                                             fields.grid_quadrature, solver.geometry, a, omega,
+                                             ctx.quadrature.weight, source, v)
+    else:
+        source = torch.empty((V, G, R, C, S, M, 4), device=device, dtype=torch.float32)
+        for v, a in enumerate(angles):
+            solver.first_collision_group_major(fields, a, phi_uc[v], ctx, out=source[v])
     T = _previous_scatter_moments(states, fields)
     if isinstance(T, MixedOperand) and contraction != "bf16":
         T = None  # the float32 contraction needs the float32 moments: formed again below

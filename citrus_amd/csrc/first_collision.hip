@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void fc_source_kernel(FcDev d, float* __res
   primary_direction(d, r, c, s, ux, uy, uz);
   PointData<GMAX> pd;
   load_point(d, p, pd);
-  const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
+  const float eta = direction_cosine(ux, uy, uz, d.omegas[3 * k], d.omegas[3 * k + 1], d.omegas[3 * k + 2]);
   float S[GMAX];
   eval_source(d, pd, eta, S);
 #pragma unroll
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
         const int k = k0 + j;
         if (k >= d.K) break;
         const f4 dir = sdir[j0 + j];
-        const float eta = ux * dir.x + uy * dir.y + uz * dir.z;
+        const float eta = direction_cosine(ux, uy, uz, dir.x, dir.y, dir.z);
         const f4 S = eval_source_channel_batched(d, pd, ps, eta, ch);
         const float w = dir.w;
         const float we = w * eta;
@@ -357,11 +357,46 @@ __global__ __launch_bounds__(kBlock) void fc_correct_kernel(FcDev d, const float
   const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
   float ux, uy, uz;
   primary_direction(d, r, c, s, ux, uy, uz);
-  const float eta = ux * d.omegas[3 * k] + uy * d.omegas[3 * k + 1] + uz * d.omegas[3 * k + 2];
+  const float eta = direction_cosine(ux, uy, uz, d.omegas[3 * k], d.omegas[3 * k + 1], d.omegas[3 * k + 2]);
   for (int g = 0; g < d.G; ++g) {
     const int64_t gp = (int64_t)g * d.P + p;
     const float v = samples[g * PK + idx];
     samples[g * PK + idx] = v * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+  }
+}
+
+// Omega_UC of every point as the kernels above form it (FP64, rounded to float32), [P][4] with w = 0: the table the
+// sweep reads when it applies the sampling correction to stored samples.
+__global__ __launch_bounds__(kBlock) void fc_directions_kernel(FcDev d, float* __restrict__ out) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= d.P) return;
+  const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
+  float ux, uy, uz;
+  primary_direction(d, r, c, s, ux, uy, uz);
+  reinterpret_cast<f4*>(out)[p] = f4{ux, uy, uz, 0.0f};
+}
+
+// fc_correct_kernel on cell means stored [V][G][N][M] with the directions given as tables: u [V][N][4] (from
+// fc_directions_kernel) and omega [M][4]; the same operations, so the values are equal bit for bit. Thread per
+// (view, cell, direction), the groups in a loop.
+__global__ __launch_bounds__(kBlock) void fc_correct_means_kernel(float* __restrict__ samples,
+                                                                 const float* __restrict__ scale,
+                                                                 const float* __restrict__ beta,
+                                                                 const float* __restrict__ u4,
+                                                                 const float* __restrict__ omega4, int G, int64_t N,
+                                                                 int M, int64_t total) {
+  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (v, n, m), m fastest
+  if (idx >= total) return;
+  const int m = (int)(idx % M);
+  const int64_t vn = idx / M;
+  const int64_t n = vn % N, v = vn / N;
+  const f4 u = reinterpret_cast<const f4*>(u4)[vn];
+  const f4 o = reinterpret_cast<const f4*>(omega4)[m];
+  const float eta = direction_cosine(u.x, u.y, u.z, o.x, o.y, o.z);
+  for (int g = 0; g < G; ++g) {
+    const int64_t gp = (v * G + g) * N + n;
+    const float x = samples[gp * M + m];
+    samples[gp * M + m] = x * scale[gp] * fmaf(beta[gp], eta, 1.0f);
   }
 }
 
@@ -391,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
         primary_direction(d, r, c, s, ux, uy, uz);
         PointData<GMAX> pd;
         load_point(d, p, pd);
-        const float eta = ux * ox + uy * oy + uz * oz;
+        const float eta = direction_cosine(ux, uy, uz, ox, oy, oz);
         float S[GMAX];
         eval_source(d, pd, eta, S);
         // local coordinates u_j = (2j + 1)/f - 1 (x along cols, y along rows, z along slices)
@@ -492,7 +527,7 @@ __global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* 
     PointData<GMAX> pd;
     load_point(d, p, pd);
     float S[GMAX];
-    eval_source(d, pd, ux * ox + uy * oy + uz * oz, S);
+    eval_source(d, pd, direction_cosine(ux, uy, uz, ox, oy, oz), S);
     // offset of the point from its cell centre along w (centred sub-point index times the pitch)
     const float half = 0.5f * (float)(f - 1);
     const float off = f > 1 ? ox * (((float)(c % f) - half) * (float)d.dx) + oy * (((float)(r % f) - half) * (float)d.dy)
@@ -631,6 +666,28 @@ extern "C" int citrus_fc_correct(const CitrusFcArgs* args, const float* scale, c
   hipLaunchKernelGGL(fc_correct_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, scale,
                      beta, samples);
   return finish("first-collision correction");
+}
+
+extern "C" int citrus_fc_primary_directions(const CitrusFcArgs* args, float* out, void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!out) return fail(CITRUS_ERR_ARG, "NULL output");
+  const FcDev d = make_dev(*args);
+  hipLaunchKernelGGL(fc_directions_kernel, dim3((unsigned)((d.P + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), d, out);
+  return finish("first-collision primary directions");
+}
+
+extern "C" int citrus_fc_correct_means(float* samples, const float* scale, const float* beta, const float* u4,
+                                       const float* omega4, int V, int G, int64_t N, int M, void* stream) {
+  using namespace citrus;
+  if (!samples || !scale || !beta || !u4 || !omega4) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (V < 1 || G < 1 || N < 1 || M < 1) return fail(CITRUS_ERR_ARG, "V, G, N and M must be >= 1");
+  const int64_t total = (int64_t)V * N * M;
+  if ((total + kBlock - 1) / kBlock > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many samples for one launch");
+  hipLaunchKernelGGL(fc_correct_means_kernel, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     reinterpret_cast<hipStream_t>(stream), samples, scale, beta, u4, omega4, G, N, M, total);
+  return finish("first-collision correction of cell means");
 }
 
 extern "C" int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float* beta, int factor,

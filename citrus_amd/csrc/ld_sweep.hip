@@ -370,6 +370,7 @@ __global__ __launch_bounds__(64) void ld_sweep_mfma_kernel(LaunchArgs a) {
   t.phi = a.phi + (int64_t)v * a.tview + (int64_t)g * a.N * NSH * 4;
   t.src = SRC != 0 ? a.src + (int64_t)v * a.sview + (int64_t)g * a.N * a.M * (SRC == 2 ? 1 : 4) : nullptr;
   t.cdir = a.cdir;
+  t.fc_scale = t.fc_beta = t.fc_u = nullptr;  // source layout 3 runs on the persistent sweep only
   mfma_task<NSH, DC4, SRC, BF16>(t, DirTables{a.D8, a.d0, a.dc, a.M, a.Y, a.Yw}, lds, threadIdx.x);
 }
 

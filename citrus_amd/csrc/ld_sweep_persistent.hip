@@ -239,8 +239,9 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
   if (A.nsh != 16 && A.nsh != 36)
     return fail(CITRUS_ERR_UNSUPPORTED, "the persistent sweep is built for nsh 16 and 36 (P3, P5)");
   if (!A.Y || !A.Yw || !A.views) return fail(CITRUS_ERR_ARG, "NULL direction table or view array");
-  if (A.source_layout != -1 && A.source_layout != 1 && A.source_layout != 2)
-    return fail(CITRUS_ERR_ARG, "source_layout must be -1 (no source), 1 or 2");
+  if (A.source_layout != -1 && (A.source_layout < 1 || A.source_layout > 3))
+    return fail(CITRUS_ERR_ARG, "source_layout must be -1 (no source), 1, 2 or 3");
+  if (A.source_layout == 3 && !A.fc_omega) return fail(CITRUS_ERR_ARG, "source_layout 3 needs fc_omega");
   if (workspace_bytes < citrus_persistent_workspace_bytes(plan, A.dc))
     return fail(CITRUS_ERR_ARG, "workspace smaller than citrus_persistent_workspace_bytes(plan, dc)");
   if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0) return fail(CITRUS_ERR_ARG, "workspace must be 256-B aligned");
@@ -254,6 +255,7 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
   pa.spin_limit = A.spin_limit > 0 ? A.spin_limit : kSpinLimit;
   pa.sink = A.status;
   pa.Y = A.Y; pa.Yw = A.Yw;
+  pa.fc_omega = A.source_layout == 3 ? A.fc_omega : nullptr;
   pa.tasks = plan->d_tasks; pa.qoff = plan->d_qoff;
   pa.ctrl = static_cast<int*>(workspace);
   pa.flags = pa.ctrl + kCtrlInts;
@@ -267,6 +269,8 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
       return fail(CITRUS_ERR_ARG, "every view needs float32 T, or every view the mixed operand (T_bf16 and T_low)");
     if ((A.source_layout < 0) != (w.source == nullptr))
       return fail(CITRUS_ERR_ARG, "view " + std::to_string(v) + ": source does not match source_layout");
+    if (A.source_layout == 3 && (!w.fc_scale || !w.fc_beta || !w.fc_u))
+      return fail(CITRUS_ERR_ARG, "view " + std::to_string(v) + ": source_layout 3 needs fc_scale, fc_beta and fc_u");
     if ((int64_t)pv.N * A.nsh * 16 >= (int64_t)kDrop || (int64_t)pv.N * pa.M * 4 * words >= (int64_t)kDrop)
       return fail(CITRUS_ERR_UNSUPPORTED, "view " + std::to_string(v) + ": grid too large for 32-bit buffer offsets");
     pv.face_off *= A.dc;
@@ -276,6 +280,9 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
     pv.src = w.source;
     pv.phi = w.phi;
     pv.cdir = w.cdir;
+    pv.fc_scale = A.source_layout == 3 ? w.fc_scale : nullptr;
+    pv.fc_beta = A.source_layout == 3 ? w.fc_beta : nullptr;
+    pv.fc_u = A.source_layout == 3 ? w.fc_u : nullptr;
     pa.views[v] = pv;
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

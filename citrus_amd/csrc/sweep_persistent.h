@@ -52,6 +52,9 @@ struct PersistentView {
   const float* src;   // [G][N][M] or [G][N][M][4], or NULL
   float* phi;         // [G][N][NSH][4]
   const float* cdir;  // [8 D8][3]
+  const float* fc_scale;  // source layout 3: [G][N] scale and beta of the first-collision sampling correction
+  const float* fc_beta;
+  const float* fc_u;      // source layout 3: [N][4] Omega_UC of each cell
 };
 
 struct PersistentArgs {
@@ -64,12 +67,13 @@ struct PersistentArgs {
   int* flags;
   int* ctrl;
   int* sink;          // caller's sticky status word (OR of every abort code), or NULL
+  const float* fc_omega;  // source layout 3: [M][4] the reordered directions
   PersistentView views[kMaxViews];
 };
 
 
 // Launch the persistent kernel of one moment count (P3: 16, P5: 36) on a grid of resident waves. layout: -1 no explicit
-// source, 1 or 2 the group-major source layouts. Defined in ld_sweep_persistent_p3.hip / _p5.hip.
+// source, 1 or 2 the group-major source layouts, 3 layout 2 with the first-collision correction applied in the sweep. Defined in ld_sweep_persistent_p3.hip / _p5.hip.
 hipError_t persistent_launch_p3(const PersistentArgs& a, bool bf16, int layout, size_t lds, hipStream_t s);
 hipError_t persistent_launch_p5(const PersistentArgs& a, bool bf16, int layout, size_t lds, hipStream_t s);
 
@@ -159,7 +163,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   }
   const int qbeg = a.qoff[xcc];
   const int qlen = a.qoff[xcc + 1] - qbeg;
-  const DirTables dt{a.D8, a.d0, a.dc, a.M, a.Y, a.Yw};
+  const DirTables dt{a.D8, a.d0, a.dc, a.M, a.Y, a.Yw, a.fc_omega};
   const int face_stride = 48 * a.dc;
   for (;;) {
     // lane 0 adds 1 and the others 0, so the wave takes one ticket whether or not the adds are combined
@@ -198,8 +202,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       t.Tlo = nullptr;
     }
     t.phi = pv.phi + (int64_t)g * pv.N * NSH * 4;
-    t.src = SRC != 0 ? pv.src + (int64_t)g * pv.N * a.M * (SRC == 2 ? 1 : 4) : nullptr;
+    t.src = SRC != 0 ? pv.src + (int64_t)g * pv.N * a.M * (SRC == 1 ? 4 : 1) : nullptr;
     t.cdir = pv.cdir;
+    t.fc_scale = SRC == 3 ? pv.fc_scale + (int64_t)g * pv.N : nullptr;
+    t.fc_beta = SRC == 3 ? pv.fc_beta + (int64_t)g * pv.N : nullptr;
+    t.fc_u = SRC == 3 ? pv.fc_u : nullptr;
     // the lane index is re-read per task so that nothing derived from it is hoisted out of the task loop and kept
     // live across tasks (the task body needs every register it has)
     int task_lane;
@@ -247,6 +254,7 @@ hipError_t launch_persistent_dc(const PersistentArgs& a, size_t lds, hipStream_t
 template <int NSH, bool BF16>
 hipError_t launch_persistent_src(const PersistentArgs& a, int layout, size_t lds, hipStream_t s) {
   if (layout < 0) return launch_persistent_dc<NSH, 0, BF16>(a, lds, s);
+  if (layout == 3) return launch_persistent_dc<NSH, 3, BF16>(a, lds, s);
   return layout == 2 ? launch_persistent_dc<NSH, 2, BF16>(a, lds, s) : launch_persistent_dc<NSH, 1, BF16>(a, lds, s);
 }
 

@@ -174,20 +174,28 @@ class PersistentSchedule:
         return self._workspace
 
     def run(self, *, views, Y: Tensor, Yw: Tensor, nsh: int, D8: int, d0: int, dc: int, accumulate: bool = False,
-            source_layout: int = -1, spin_limit: int = 0) -> None:
+            source_layout: int = -1, spin_limit: int = 0, fc_omega: Optional[Tensor] = None) -> None:
         """Enqueue one chunk sweep of every view on torch's current stream. ``views`` holds one dict per view of the
         plan with the device tensors ``mu`` [G, N], ``phi`` [G, N, nsh, 4], ``cdir`` [8 D8, 3], and either ``T``
         [G, N, nsh, 4] or the mixed operand ``T16`` [G, N, 4, KH] with ``T_low`` [G, N, 4, 4], and optionally
-        ``source`` (group-major, the
-        same ``source_layout`` 1 or 2 for every view; -1 without). A sweep that aborts ORs its status code into the
-        device's status word (``persistent_status_sink``), which the callers' next host read reports
-        (``decode_status``); ``spin_limit`` > 0 replaces the default bound of each dependency wait."""
+        ``source`` (group-major, the same ``source_layout`` 1, 2 or 3 for every view; -1 without). Layout 3 is layout
+        2's cell means before the first-collision sampling correction, which the sweep applies: each view then also
+        holds ``fc_scale`` and ``fc_beta`` [G, N] and ``fc_u`` [N, 4], and ``fc_omega`` [8 D8, 4] gives the
+        directions. A sweep that aborts ORs its status code into the device's status word
+        (``persistent_status_sink``), which the callers' next host read reports (``decode_status``); ``spin_limit``
+        > 0 replaces the default bound of each dependency wait."""
         if len(views) != len(self.shapes):
             raise ValueError(f"the plan has {len(self.shapes)} views, got {len(views)}")
         if nsh not in PERSISTENT_MOMENTS:
             raise ValueError(f"the persistent sweep is built for {PERSISTENT_MOMENTS} moments, got {nsh}")
-        if source_layout not in (-1, 1, 2):
-            raise ValueError("source_layout must be -1, 1 or 2")
+        if source_layout not in (-1, 1, 2, 3):
+            raise ValueError("source_layout must be -1, 1, 2 or 3")
+        if (source_layout == 3) != (fc_omega is not None):
+            raise ValueError("fc_omega goes with source_layout 3, and only with it")
+        if fc_omega is not None:
+            _hip.require_gpu_tensor(fc_omega, "fc_omega")
+            if tuple(fc_omega.shape) != (8 * D8, 4) or not fc_omega.is_contiguous():
+                raise ValueError(f"fc_omega must be a contiguous float32 [{8 * D8}, 4] tensor")
         G, M, kh = self.G, 8 * D8, 16 * ((nsh - 4 + 15) // 16)
         arr = (_hip.SweepView * len(views))()
         for v, (spec, shape) in enumerate(zip(views, self.shapes)):
@@ -217,9 +225,20 @@ class PersistentSchedule:
                 _hip.require_gpu_tensor(src, f"views[{v}].source")
                 if src.numel() != G * N * M * (4 if source_layout == 1 else 1):
                     raise ValueError(f"views[{v}]: source size does not match layout {source_layout}")
+            fc = [spec.get(k) for k in ("fc_scale", "fc_beta", "fc_u")]
+            if source_layout == 3:
+                for name, t, n in zip(("fc_scale", "fc_beta", "fc_u"), fc, (G * N, G * N, 4 * N)):
+                    if t is None:
+                        raise ValueError(f"views[{v}]: source_layout 3 needs {name}")
+                    _hip.require_gpu_tensor(t, f"views[{v}].{name}")
+                    if t.numel() != n or not t.is_contiguous():
+                        raise ValueError(f"views[{v}]: {name} must be a contiguous float32 tensor of {n} values")
+            else:
+                fc = [None, None, None]
             arr[v] = _hip.SweepView(mu_t=spec["mu"].data_ptr(), T=_hip.ptr(T), T_bf16=_hip.ptr(T16),
                                     T_low=_hip.ptr(tlo), source=_hip.ptr(src), phi=spec["phi"].data_ptr(),
-                                    cdir=spec["cdir"].data_ptr())
+                                    cdir=spec["cdir"].data_ptr(), fc_scale=_hip.ptr(fc[0]), fc_beta=_hip.ptr(fc[1]),
+                                    fc_u=_hip.ptr(fc[2]))
         if Y.shape != (M, nsh) or Yw.shape != (M, nsh):
             raise ValueError("direction tables do not match the quadrature")
         ws = self.workspace(dc)
@@ -227,7 +246,8 @@ class PersistentSchedule:
             raise ValueError("spin_limit must be >= 0")
         args = _hip.PersistentArgs(G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, accumulate=int(accumulate),
                                    source_layout=int(source_layout), Y=Y.data_ptr(), Yw=Yw.data_ptr(), views=arr,
-                                   status=persistent_status_sink(self.device).data_ptr(), spin_limit=int(spin_limit))
+                                   status=persistent_status_sink(self.device).data_ptr(), spin_limit=int(spin_limit),
+                                   fc_omega=_hip.ptr(fc_omega))
         lib = _hip.library()
         with torch.cuda.device(self.device):
             _hip.check(lib.citrus_persistent_sweep_run(self.handle, ctypes.byref(args), ws.data_ptr(), ws.numel(),

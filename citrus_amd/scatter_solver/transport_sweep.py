@@ -337,7 +337,9 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             ``schedule="auto"``, schedule) measured by the autotuner for this configuration.
         source: optional explicit source added to the moments' expansion, every direction of every view:
             [V, G, R, C, S, M, 4] LD moments or [V, G, R, C, S, M] cell means (zero slopes), M the reordered
-            directions (the summed source of the CITRUS step, citrus.py:232-236).
+            directions (the summed source of the CITRUS step, citrus.py:232-236), or the first-collision cell means
+            before their sampling correction (``uncollided.DeferredFirstCollision``): the persistent schedule applies
+            the correction as it reads them, the launch sequence after ``apply_()``; the moments are the same.
         contraction: "fp32", or "bf16" for the mixed-precision source expansion (P3 and P5): the moments of degree
             l <= 1 in float32 and those of degree l >= 2 as a bfloat16 matrix product, all accumulated in float32.
         schedule: "launches" (one launch per block diagonal), "persistent" (one launch of resident waves per
@@ -357,6 +359,10 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     if mixed:
         contraction = "bf16"
     layout = 0
+    deferred = source if _is_deferred(source) else None
+    if deferred is not None:
+        _check_deferred(deferred, V, G, (R, C, S), sweep_context.num_directions)
+        source = deferred.samples
     if source is not None:
         M = sweep_context.num_directions
         base = (V, G, R, C, S, M)
@@ -389,8 +395,10 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         else:
             schedule = "launches"
     if schedule == "persistent":
-        return _sweep_grids_persistent([(sweep_context, mu_t, moments, source, out)], direction_chunk,
-                                       contraction)[0]
+        return _sweep_grids_persistent([(sweep_context, mu_t, moments, deferred if deferred is not None else source,
+                                         out)], direction_chunk, contraction)[0]
+    if deferred is not None:
+        source = deferred.apply_()
     D8 = sweep_context.directions_per_octant
     if direction_chunk == "auto":  # measured once per configuration (autotune.SweepTuner)
         from .autotune import default_tuner
@@ -438,8 +446,26 @@ def sweep_grids(items, direction_chunk: Optional[int] = None, contraction: str =
     return _sweep_grids_persistent(prepared, direction_chunk, contraction)
 
 
+def _is_deferred(source) -> bool:
+    from .uncollided import DeferredFirstCollision
+
+    return isinstance(source, DeferredFirstCollision)
+
+
+def _check_deferred(src, V: int, G: int, spatial: tuple, M: int) -> None:
+    N = spatial[0] * spatial[1] * spatial[2]
+    want = {"samples": (V, G) + tuple(spatial) + (M,), "scale": (V, G, N), "beta": (V, G, N), "u": (V, N, 4),
+            "omega": (M, 4)}
+    for name, shape in want.items():
+        t = getattr(src, name)
+        if tuple(t.shape) != shape or not t.is_contiguous() or t.dtype != torch.float32:
+            raise ValueError(f"deferred first-collision {name} must be contiguous float32 {list(shape)}, got "
+                             f"{tuple(t.shape)} {t.dtype}")
+
+
 def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
-    """Persistent sweeps of (context, mu_t, moments, source, out) items, 16 views per launch."""
+    """Persistent sweeps of (context, mu_t, moments, source, out) items, 16 views per launch. A
+    ``DeferredFirstCollision`` source not yet applied runs as source layout 3 (the sweep applies the correction)."""
     from .kernels.sweep_kernels import PERSISTENT_MAX_VIEWS, PERSISTENT_MOMENTS, persistent_schedule
 
     if contraction not in ("fp32", "bf16"):
@@ -450,6 +476,7 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
     if nsh not in PERSISTENT_MOMENTS:
         raise ValueError(f"the persistent sweep is built for {PERSISTENT_MOMENTS} moments (P3, P5), got {nsh}")
     layouts = set()
+    omega = None  # the directions of the deferred first-collision sources (layout 3)
     views = []  # (shape, spec) per view
     for ctx, mu_t, moments, source, out in items:
         if ctx.directions_per_octant != D8 or ctx.num_moments != nsh or ctx.hip.Y.shape != ctx0.hip.Y.shape \
@@ -462,6 +489,13 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
         if mixed and contraction != "bf16":
             raise ValueError("a mixed-precision operand needs contraction='bf16'")
         layout = -1
+        fc = None
+        if _is_deferred(source):
+            _check_deferred(source, V, G, (R, C, S), M)
+            if source.applied:
+                source = source.samples
+            else:
+                fc, source = source, source.samples
         if source is not None:
             base = (V, G, R, C, S, M)
             layout = 1 if tuple(source.shape) == base + (4,) else 2 if tuple(source.shape) == base else 0
@@ -469,11 +503,19 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
                 raise ValueError(f"source must be {base + (4,)} or {base}, got {tuple(source.shape)}")
             if not source.is_contiguous() or source.dtype != torch.float32:
                 raise ValueError("source must be a contiguous float32 tensor")
+        if fc is not None:
+            layout = 3
+            if omega is None:
+                omega = fc.omega
+            elif not torch.equal(omega, fc.omega):
+                raise ValueError("the deferred first-collision sources of one sweep must share the directions")
         layouts.add(layout)
         mu = mu_t.reshape(G, -1).contiguous()
         op = _mixed(moments) if contraction == "bf16" else None
         for v in range(V):
             spec = dict(mu=mu, phi=out[v], cdir=ctx.hip.cdir, source=None if source is None else source[v])
+            if fc is not None:
+                spec.update(fc_scale=fc.scale[v], fc_beta=fc.beta[v], fc_u=fc.u[v])
             if op is None:
                 spec["T"] = moments[v]
             else:
@@ -490,7 +532,7 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
         for d0 in range(0, D8, Dc):
             dc = min(d0 + Dc, D8) - d0
             sched.run(views=[spec for _, spec in batch], Y=ctx0.hip.Y, Yw=ctx0.hip.Yw, nsh=nsh, D8=D8, d0=d0,
-                      dc=dc, accumulate=d0 > 0, source_layout=layout)
+                      dc=dc, accumulate=d0 > 0, source_layout=layout, fc_omega=omega if layout == 3 else None)
     return [it[4] for it in items]
 
 
@@ -505,6 +547,8 @@ class SweepGraph:
 
     def __init__(self, sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, source: Optional[Tensor] = None,
                  contraction: str = "fp32", direction_chunk: Optional[int] = None, schedule: str = "auto"):
+        if _is_deferred(source):
+            raise ValueError("a graph replays fixed buffers: pass the corrected source tensor, not a deferred one")
         self.context = sweep_context
         self.mu_t = mu_t
         self.moments = moments

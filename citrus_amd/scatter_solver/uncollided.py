@@ -338,6 +338,77 @@ def first_collision_corrected_samples(library: MultigroupLibrary, rho: Tensor, p
     return out, correction
 
 
+@dataclass(eq=False)
+class DeferredFirstCollision:
+    """The first-collision cell means of V views at every sweep direction (one quadrature point per cell), stored
+    before their sampling correction, with what the correction needs: the persistent sweep applies it as it reads
+    them (source layout 3), so the corrected source is never written; ``apply_`` applies it in place for any other
+    consumer. Either way each value is ``mean scale (1 + beta eta)`` with the operations of ``citrus_fc_correct``
+    (uncollided.py:420-433), so the two are equal bit for bit.
+
+    Attributes: samples [V, G, R, C, S, M] (reordered sweep directions), scale and beta [V, G, N] float32, u [V, N, 4]
+    the primary direction of each cell (``citrus_fc_primary_directions``), omega [M, 4] the directions; ``applied``
+    once the samples hold the corrected values."""
+
+    samples: Tensor
+    scale: Tensor
+    beta: Tensor
+    u: Tensor
+    omega: Tensor
+    applied: bool = False
+
+    @staticmethod
+    def empty(V: int, G: int, shape, M: int, device) -> "DeferredFirstCollision":
+        N = int(np.prod(shape))
+        f = dict(device=device, dtype=torch.float32)
+        return DeferredFirstCollision(samples=torch.empty((V, G) + tuple(shape) + (M,), **f),
+                                      scale=torch.empty(V, G, N, **f), beta=torch.empty(V, G, N, **f),
+                                      u=torch.empty(V, N, 4, **f), omega=torch.empty(M, 4, **f))
+
+    def apply_(self) -> Tensor:
+        """Apply the correction to ``samples`` in place (once) and return them: the group-major cell means
+        [V, G, R, C, S, M] of the corrected source."""
+        if not self.applied:
+            V, G = int(self.samples.shape[0]), int(self.samples.shape[1])
+            M = int(self.samples.shape[-1])
+            N = int(self.samples[0, 0].numel()) // M
+            with torch.cuda.device(self.samples.device):
+                _hip.check(_hip.library().citrus_fc_correct_means(
+                    self.samples.data_ptr(), self.scale.data_ptr(), self.beta.data_ptr(), self.u.data_ptr(),
+                    self.omega.data_ptr(), V, G, N, M, _hip.stream_of(self.samples.device)),
+                    "citrus_fc_correct_means")
+            self.applied = True
+        return self.samples
+
+
+def first_collision_deferred_samples(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
+                                     geometry: ConeGeometry, angle: float, omegas: Tensor, weights: Tensor,
+                                     out: DeferredFirstCollision, view: int) -> FirstCollisionSamplingCorrection:
+    """``first_collision_corrected_samples`` of view ``view`` of ``out`` without applying the correction: the
+    uncorrected samples, the factors and the cells' primary directions go to ``out`` (one point per cell: ``grid``
+    is the transport grid). Returns the correction."""
+    K = int(omegas.reshape(-1, 3).shape[0])
+    G = library.num_groups
+    samples = out.samples[view].reshape((G,) + tuple(grid.shape) + (K,))
+    sums = _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights,
+                        samples=samples).double()
+    scale, beta = first_collision_correction_factors(sums[:3], sums[3:])
+    out.scale[view].copy_(scale.reshape(G, -1))
+    out.beta[view].copy_(beta.reshape(G, -1))
+    a, keep, _ = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
+    with torch.cuda.device(phi_uncollided.device):
+        _hip.check(_hip.library().citrus_fc_primary_directions(ctypes.byref(a), out.u[view].data_ptr(),
+                                                               _hip.stream_of(phi_uncollided.device)),
+                   "citrus_fc_primary_directions")
+    del keep
+    out.omega[:, :3].copy_(omegas.reshape(-1, 3))
+    out.omega[:, 3].zero_()
+    out.applied = False
+    ou = out.u[view, :, :3].reshape(tuple(grid.shape) + (3,))
+    return FirstCollisionSamplingCorrection(scale=out.scale[view].reshape(scale.shape),
+                                            beta=out.beta[view].reshape(beta.shape), omega_uncollided=ou)
+
+
 _FC_LAYOUTS = {"moment_major": 0, "cell_major": 1, "group_major": 2, "group_major_means": 3}
 
 

@@ -117,12 +117,19 @@ typedef struct CitrusSweepView {
   float* source;             /* group-major source of every direction ([G][N][8*D8][4] or [G][N][8*D8]) or NULL */
   float* phi;                /* [G][N][nsh][4] fluence moments (output) */
   const float* cdir;         /* [8*D8][3] |Omega_a| / Delta_a on this view's grid */
+  const float* fc_scale;     /* source_layout 3: [G][N] scale of the first-collision sampling correction */
+  const float* fc_beta;      /* source_layout 3: [G][N] beta of the correction */
+  const float* fc_u;         /* source_layout 3: [N][4] Omega_UC of each cell (citrus_fc_primary_directions) */
 } CitrusSweepView;
 
 typedef struct CitrusPersistentArgs {
   int G, nsh, D8, d0, dc;    /* as in CitrusSweepArgs */
   int accumulate;            /* 0: phi is overwritten; 1: phi += this chunk's moments */
-  int source_layout;         /* -1: no explicit source; 1 or 2 as in CitrusSweepArgs (the same for every view) */
+  int source_layout;         /* -1: no explicit source; 1 or 2 as in CitrusSweepArgs (the same for every view); 3: the
+                              * cell means of layout 2 before the first-collision sampling correction, which the sweep
+                              * applies as it reads them: mean scale (1 + beta eta), eta = u[cell] . omega[m], the
+                              * operations of citrus_fc_correct, so the moments equal those of layout 2 after
+                              * citrus_fc_correct_means bit for bit (needs fc_scale, fc_beta, fc_u and fc_omega) */
   const float* Y;            /* [8*D8][nsh], shared by the views */
   const float* Yw;           /* [8*D8][nsh] */
   const CitrusSweepView* views; /* HOST array of the plan's nviews views, read before the launch */
@@ -132,6 +139,7 @@ typedef struct CitrusPersistentArgs {
                               * is cleared by each chunk's launch) */
   int spin_limit;            /* sleeps of one dependency wait before the sweep aborts; 0: the default (about a
                               * second). A small value forces the abort path from the host, for tests */
+  const float* fc_omega;     /* source_layout 3: [8*D8][4] the sweep directions (x, y, z, 0) in float32 */
 } CitrusPersistentArgs;
 
 /* Schedule of nviews view grids, shapes[v] = (R, C, S), with G groups. Uploads the task queues (hipMalloc). */
@@ -399,6 +407,18 @@ int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* 
  * cosine between Omega_UC of the point and omegas[k]. At one point per cell (factor 1) the result equals
  * citrus_fc_moments layout 3 bit for bit, without a second evaluation of the source (uncollided.py:420-433). */
 int citrus_fc_correct(const CitrusFcArgs* args, const float* scale, const float* beta, float* samples, void* stream);
+
+/* out [P][4]: Omega_UC of every point as the first-collision kernels form it (FP64, rounded to float32), w = 0. With
+ * citrus_fc_correct_means or the persistent sweep's source layout 3 it applies the sampling correction away from the
+ * kernel that stored the samples (the split of uncollided.py:420-433 between the sums and the sweep). */
+int citrus_fc_primary_directions(const CitrusFcArgs* args, float* out, void* stream);
+
+/* citrus_fc_correct at one point per cell for V views at once, the directions given as tables: samples [V][G][N][M]
+ * (cell means, in place) *= scale (1 + beta eta), scale and beta [V][G][N], eta from u4 [V][N][4]
+ * (citrus_fc_primary_directions) and omega4 [M][4]. The same operations as citrus_fc_correct, so the values are equal
+ * bit for bit (uncollided.py:420-433). */
+int citrus_fc_correct_means(float* samples, const float* scale, const float* beta, const float* u4,
+                            const float* omega4, int V, int G, int64_t N, int M, void* stream);
 
 /* LD moments of the corrected first-collision source on the cells of the factor-coarsened point grid
  * (first_collision_moments_at_directions, uncollided.py:464-535): samples S scale (1 + beta eta) (scale, beta

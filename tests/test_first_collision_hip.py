@@ -180,3 +180,38 @@ def test_fc_correction_conserves_the_exact_moments():
     free = corr.beta.abs() < 1
     assert float(free.double().mean()) > 0.5
     assert float((J - E[1])[free].abs().max()) < 1e-5 * float(E[1].abs().max())
+
+
+def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike():
+    """The sampling correction applied away from the sums kernel: by ``citrus_fc_correct_means`` or by the
+    persistent sweep as it reads the stored samples (source layout 3). Both give the corrected samples of
+    ``first_collision_corrected_samples`` (and their sweep) bit for bit; the launch sequence applies it first."""
+    from citrus_amd.scatter_solver.transport_sweep import make_sweep_context, sweep_views
+    from citrus_amd.scatter_solver.uncollided import DeferredFirstCollision, first_collision_deferred_samples
+
+    dev = gpu_device()
+    library = seeded_library(6, 3, 120.0, 20.0, seed=15, num_eta=97)
+    grid = Grid((8, 8, 8), 0.5)
+    ctx = make_sweep_context(grid, level_symmetric_quadrature(6), 3, device=dev)
+    om, w = ctx.quadrature.omega.to(dev, torch.float32), ctx.quadrature.weight
+    V, G, M = 2, library.num_groups, ctx.num_directions
+    deferred = DeferredFirstCollision.empty(V, G, grid.shape, M, dev)
+    corrected = []
+    for v in range(V):
+        rho, phi = _problem(library, grid.shape, 20 + v)
+        rho_d, phi_d = rho.float().to(dev), phi.float().to(dev)
+        angle = ANGLE + 0.9 * v
+        corr = first_collision_deferred_samples(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w, deferred, v)
+        ref, ref_corr = first_collision_corrected_samples(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w)
+        assert torch.equal(corr.scale, ref_corr.scale) and torch.equal(corr.beta, ref_corr.beta)
+        assert rel_err(corr.omega_uncollided, ref_corr.omega_uncollided) < 1e-6
+        corrected.append(ref)
+    corrected = torch.stack(corrected).contiguous()
+    gen = torch.Generator(device=dev).manual_seed(3)
+    mu = torch.rand(G, *grid.shape, device=dev, generator=gen) * 0.5 + 0.2
+    T = torch.randn(V, G, *grid.shape, ctx.num_moments, 4, device=dev, generator=gen) * 0.01
+    want = sweep_views(ctx, mu, T, source=corrected, schedule="persistent")
+    got = sweep_views(ctx, mu, T, source=deferred, schedule="persistent")
+    assert not deferred.applied and torch.equal(got, want)
+    got = sweep_views(ctx, mu, T, source=deferred, schedule="launches")
+    assert deferred.applied and torch.equal(deferred.samples, corrected) and torch.equal(got, want)

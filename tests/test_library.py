@@ -142,3 +142,18 @@ def test_bindings_and_documented_stub_match_the_header_structs():
     namespace = {"ctypes": ctypes}
     exec(stub, namespace)  # the documented class definition only
     assert [f for f, _ in namespace["CitrusSweepArgs"]._fields_] == _struct_fields(header, "CitrusSweepArgs")
+
+
+def test_deferred_first_collision_shapes_are_checked():
+    """A deferred first-collision source whose buffers disagree with the sweep's shape is refused before any launch."""
+    from citrus_amd.scatter_solver.transport_sweep import _check_deferred
+    from citrus_amd.scatter_solver.uncollided import DeferredFirstCollision
+
+    d = DeferredFirstCollision.empty(2, 3, (2, 3, 4), 8, "cpu")
+    _check_deferred(d, 2, 3, (2, 3, 4), 8)
+    for args in ((1, 3, (2, 3, 4), 8), (2, 4, (2, 3, 4), 8), (2, 3, (2, 3, 5), 8), (2, 3, (2, 3, 4), 16)):
+        with pytest.raises(ValueError):
+            _check_deferred(d, *args)
+    d.u = d.u[..., :3]
+    with pytest.raises(ValueError, match="u"):
+        _check_deferred(d, 2, 3, (2, 3, 4), 8)
