@@ -11,6 +11,19 @@ namespace citrus {
 // Records `msg` as the calling thread's last error and returns `code`.
 int fail(int code, const std::string& msg);
 
+// A dispatch counts the work-items of each grid dimension in 32 bits: a 1-D grid of 2^32 or more work-items is cut
+// short without an error. Launchers check their grids with grid_fits, and elementwise kernels over sizes that can
+// pass the limit loop with a stride over at most kStrideBlocks workgroups.
+constexpr int64_t kMaxGridItems = 0xffffffffLL;
+constexpr int64_t kStrideBlocks = 1 << 20;
+
+inline bool grid_fits(int64_t blocks, int64_t threads) { return blocks >= 0 && blocks * threads <= kMaxGridItems; }
+
+inline unsigned stride_grid(int64_t total, int threads) {
+  const int64_t blocks = (total + threads - 1) / threads;
+  return (unsigned)(blocks < kStrideBlocks ? (blocks > 0 ? blocks : 1) : kStrideBlocks);
+}
+
 // eta = Omega_UC . w in float32, with the fused multiply-adds written out so that every kernel that forms it (the
 // first-collision source, sums, moments and correction, and the sweep applying the correction to stored samples)
 // gets the same value bit for bit

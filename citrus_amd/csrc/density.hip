@@ -91,6 +91,7 @@ extern "C" int citrus_attenuation_to_densities(const float* X, int64_t n, int M,
     p.mass[s] = mass_attenuation[s];
   }
   if (n == 0) return CITRUS_OK;
+  if (!grid_fits((n + 255) / 256, 256)) return fail(CITRUS_ERR_ARG, "too many voxels for one launch (2^32)");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(densities_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, n, p, rho);
   const hipError_t e = hipGetLastError();
@@ -168,6 +169,7 @@ extern "C" int citrus_resample_conservative(const float* field, int channels, in
   if (row_width < 1 || col_width < 1 || slice_width < 1) return fail(CITRUS_ERR_ARG, "band widths must be >= 1");
   const int64_t total = (int64_t)Rt * Ct * St * channels;
   if (total == 0) return CITRUS_OK;
+  if (!grid_fits((total + 255) / 256, 256)) return fail(CITRUS_ERR_ARG, "too many output values for one launch (2^32)");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const Band br{row_first, row_count, row_share, row_width}, bc{col_first, col_count, col_share, col_width},
       bs{slice_first, slice_count, slice_share, slice_width};

@@ -349,19 +349,19 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
 __global__ __launch_bounds__(kBlock) void fc_correct_kernel(FcDev d, const float* __restrict__ scale,
                                                            const float* __restrict__ beta,
                                                            float* __restrict__ samples) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (p, k), k fastest; the groups in a loop
-  const int64_t PK = (int64_t)d.P * d.K;
-  if (idx >= PK) return;
-  const int k = (int)(idx % d.K);
-  const int p = (int)(idx / d.K);
-  const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
-  float ux, uy, uz;
-  primary_direction(d, r, c, s, ux, uy, uz);
-  const float eta = direction_cosine(ux, uy, uz, d.omegas[3 * k], d.omegas[3 * k + 1], d.omegas[3 * k + 2]);
-  for (int g = 0; g < d.G; ++g) {
-    const int64_t gp = (int64_t)g * d.P + p;
-    const float v = samples[g * PK + idx];
-    samples[g * PK + idx] = v * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+  const int64_t PK = (int64_t)d.P * d.K;  // (p, k), k fastest, over a stride loop; the groups in a loop
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < PK; idx += (int64_t)gridDim.x * kBlock) {
+    const int k = (int)(idx % d.K);
+    const int p = (int)(idx / d.K);
+    const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
+    float ux, uy, uz;
+    primary_direction(d, r, c, s, ux, uy, uz);
+    const float eta = direction_cosine(ux, uy, uz, d.omegas[3 * k], d.omegas[3 * k + 1], d.omegas[3 * k + 2]);
+    for (int g = 0; g < d.G; ++g) {
+      const int64_t gp = (int64_t)g * d.P + p;
+      const float v = samples[g * PK + idx];
+      samples[g * PK + idx] = v * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+    }
   }
 }
 
@@ -385,18 +385,19 @@ __global__ __launch_bounds__(kBlock) void fc_correct_means_kernel(float* __restr
                                                                  const float* __restrict__ u4,
                                                                  const float* __restrict__ omega4, int G, int64_t N,
                                                                  int M, int64_t total) {
-  const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (v, n, m), m fastest
-  if (idx >= total) return;
-  const int m = (int)(idx % M);
-  const int64_t vn = idx / M;
-  const int64_t n = vn % N, v = vn / N;
-  const f4 u = reinterpret_cast<const f4*>(u4)[vn];
-  const f4 o = reinterpret_cast<const f4*>(omega4)[m];
-  const float eta = direction_cosine(u.x, u.y, u.z, o.x, o.y, o.z);
-  for (int g = 0; g < G; ++g) {
-    const int64_t gp = (v * G + g) * N + n;
-    const float x = samples[gp * M + m];
-    samples[gp * M + m] = x * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+  // (v, n, m), m fastest, over a stride loop
+  for (int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * kBlock) {
+    const int m = (int)(idx % M);
+    const int64_t vn = idx / M;
+    const int64_t n = vn % N, v = vn / N;
+    const f4 u = reinterpret_cast<const f4*>(u4)[vn];
+    const f4 o = reinterpret_cast<const f4*>(omega4)[m];
+    const float eta = direction_cosine(u.x, u.y, u.z, o.x, o.y, o.z);
+    for (int g = 0; g < G; ++g) {
+      const int64_t gp = (v * G + g) * N + n;
+      const float x = samples[gp * M + m];
+      samples[gp * M + m] = x * scale[gp] * fmaf(beta[gp], eta, 1.0f);
+    }
   }
 }
 
@@ -626,6 +627,8 @@ extern "C" int citrus_fc_source(const CitrusFcArgs* args, float* out, void* stre
   const FcDev d = make_dev(*args);
   const int64_t n = (int64_t)d.P * d.K;
   if (n == 0) return CITRUS_OK;
+  if (!grid_fits((n + kBlock - 1) / kBlock, kBlock))
+    return fail(CITRUS_ERR_ARG, "points x directions exceed 2^32 work-items per launch: pass fewer directions");
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
@@ -643,6 +646,8 @@ extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, co
   if (lp1 < 2) return fail(CITRUS_ERR_ARG, "the exact moments need Legendre order >= 1");
   const FcDev d = make_dev(*args);
   const int blocks = (d.P + kBlock - 1) / kBlock;
+  if (!grid_fits((int64_t)blocks * (group_pad(d.G) / 4), kBlock))
+    return fail(CITRUS_ERR_ARG, "points x group channels exceed 2^32 work-items per launch");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
     constexpr int NCH = decltype(gm)::value / 4;
@@ -661,10 +666,8 @@ extern "C" int citrus_fc_correct(const CitrusFcArgs* args, const float* scale, c
   const FcDev d = make_dev(*args);
   const int64_t n = (int64_t)d.P * d.K;
   if (n == 0) return CITRUS_OK;
-  if ((n + kBlock - 1) / kBlock > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many samples for one launch");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(fc_correct_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d, scale,
-                     beta, samples);
+  hipLaunchKernelGGL(fc_correct_kernel, dim3(stride_grid(n, kBlock)), dim3(kBlock), 0, s, d, scale, beta, samples);
   return finish("first-collision correction");
 }
 
@@ -684,8 +687,7 @@ extern "C" int citrus_fc_correct_means(float* samples, const float* scale, const
   if (!samples || !scale || !beta || !u4 || !omega4) return fail(CITRUS_ERR_ARG, "NULL buffer");
   if (V < 1 || G < 1 || N < 1 || M < 1) return fail(CITRUS_ERR_ARG, "V, G, N and M must be >= 1");
   const int64_t total = (int64_t)V * N * M;
-  if ((total + kBlock - 1) / kBlock > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many samples for one launch");
-  hipLaunchKernelGGL(fc_correct_means_kernel, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+  hipLaunchKernelGGL(fc_correct_means_kernel, dim3(stride_grid(total, kBlock)), dim3(kBlock), 0,
                      reinterpret_cast<hipStream_t>(stream), samples, scale, beta, u4, omega4, G, N, M, total);
   return finish("first-collision correction of cell means");
 }
@@ -703,6 +705,8 @@ extern "C" int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, c
   const FcDev d = make_dev(*args);
   const int64_t n = (int64_t)(d.P / (factor * factor * factor)) * d.K;
   if (n == 0) return CITRUS_OK;
+  if (!grid_fits((n + kBlock - 1) / kBlock, kBlock))
+    return fail(CITRUS_ERR_ARG, "cells x directions exceed 2^32 work-items per launch: pass fewer directions");
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock)), block(kBlock);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {

@@ -119,7 +119,7 @@ extern "C" int citrus_footprint_filters(const float* partial, int K, int nblocks
     return fail(CITRUS_ERR_ARG, "0 <= K <= 65535, nblocks >= 1, Nc >= 1 and Nr >= 1 required");
   if (K == 0) return CITRUS_OK;
   const int64_t total = (int64_t)K * Nc * Nc + (int64_t)K * Nr * Nr;
-  if ((total + 255) / 256 > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "filters too large for one launch");
+  if (!grid_fits((total + 255) / 256, 256)) return fail(CITRUS_ERR_ARG, "filters too large for one launch");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(footprint_heights_kernel, dim3((unsigned)K), dim3(256), 0, s, partial, nblocks, center_height,
                      heights);

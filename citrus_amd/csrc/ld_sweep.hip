@@ -617,7 +617,7 @@ extern "C" int citrus_sweep_run(const CitrusSweepPlan* plan, const CitrusSweepAr
     la.ntask = plan->launches[l].second;
     la.first = (!A.accumulate && plan->launch_pass[l] == 0) ? 1 : 0;
     const int64_t nwg = (int64_t)la.ntask * A.G * A.V;
-    if (nwg > 0x7fffffff) return fail(CITRUS_ERR_ARG, "too many workgroups in one launch");
+    if (!grid_fits(nwg, kThreads)) return fail(CITRUS_ERR_ARG, "too many workgroups in one launch");
     hipError_t e = mfma ? launch_mfma(A.nsh, la, (int)nwg, s) : launch_nsh<4>(A.nsh, la, (int)nwg, s);
     if (e == hipErrorInvalidValue) return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
     if (e == hipErrorInvalidConfiguration) return fail(CITRUS_ERR_UNSUPPORTED, "chunk needs more than 64 KiB of LDS");

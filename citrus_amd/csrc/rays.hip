@@ -318,7 +318,7 @@ hipError_t launch_mode(const RayJob& J, hipStream_t s) {
   const int64_t total = J.nrays * J.K;
   if (total == 0) return hipSuccess;
   const int64_t blocks = (total + kRayThreads - 1) / kRayThreads;
-  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (!grid_fits(blocks, kRayThreads)) return hipErrorInvalidValue;
   const dim3 grid((unsigned)blocks);
   if (J.nch <= 1) hipLaunchKernelGGL((ray_kernel<MODE, 1>), grid, dim3(kRayThreads), 0, s, J);
   else if (J.nch <= 2) hipLaunchKernelGGL((ray_kernel<MODE, 2>), grid, dim3(kRayThreads), 0, s, J);

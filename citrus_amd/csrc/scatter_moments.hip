@@ -173,7 +173,7 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
   if (lds > 64 * 1024) return fail(CITRUS_ERR_UNSUPPORTED, "transfer tables exceed 64 KiB of LDS");
   const int threads = kCells * nsh * 4;
   const int64_t blocks = (int64_t)V * ((N + kCells - 1) / kCells);
-  if (blocks > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many cells");
+  if (!grid_fits(blocks, threads)) return fail(CITRUS_ERR_ARG, "too many cells for one launch (2^32 work-items)");
   const int kh = MIXED ? 16 * ((nsh - 4 + 15) / 16) : 0;
   if (MIXED && (nsh < 16 || kh - (nsh - 4) > nsh - 4))
     return fail(CITRUS_ERR_UNSUPPORTED, "the mixed-precision operand is built for P3 and above");
@@ -200,18 +200,19 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
 
 __global__ void mixed_operand_kernel(const float* __restrict__ T, int nsh, int kh, int64_t rows,
                                      float* __restrict__ T_low, uint16_t* __restrict__ T16) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int width = 4 + kh;
-  if (i >= rows * width) return;
-  const int64_t row = i / width;  // (view, group, cell) * 4 + component
-  const int slot = (int)(i - row * width);
-  const int comp = (int)(row & 3);
-  const int64_t cell = row >> 2;
-  if (slot < 4) {
-    T_low[row * 4 + slot] = T[(cell * nsh + slot) * 4 + comp];
-  } else {
-    const int n = slot;
-    T16[row * kh + (slot - 4)] = n < nsh ? bf16_rne(T[(cell * nsh + n) * 4 + comp]) : (uint16_t)0;
+  const int64_t total = rows * width;  // 4.8e9 at 8 views of 64^3 x 16 groups x P5: a stride loop
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / width;  // (view, group, cell) * 4 + component
+    const int slot = (int)(i - row * width);
+    const int comp = (int)(row & 3);
+    const int64_t cell = row >> 2;
+    if (slot < 4) {
+      T_low[row * 4 + slot] = T[(cell * nsh + slot) * 4 + comp];
+    } else {
+      const int n = slot;
+      T16[row * kh + (slot - 4)] = n < nsh ? bf16_rne(T[(cell * nsh + n) * 4 + comp]) : (uint16_t)0;
+    }
   }
 }
 
@@ -227,10 +228,9 @@ extern "C" int citrus_bf16_operand(const float* T, int G, int nsh, int64_t N, in
   const int kh = 16 * ((nsh - 4 + 15) / 16);
   const int64_t rows = (int64_t)V * G * N * 4;
   const int64_t total = rows * (4 + kh);
-  if ((total + 255) / 256 > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "operand too large");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(mixed_operand_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, T, nsh, kh, rows,
-                     T_low, T16);
+  hipLaunchKernelGGL(mixed_operand_kernel, dim3(stride_grid(total, 256)), dim3(256), 0, s, T, nsh, kh, rows, T_low,
+                     T16);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("mixed operand: ") + hipGetErrorString(e));
   return CITRUS_OK;

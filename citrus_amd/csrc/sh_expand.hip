@@ -277,7 +277,7 @@ extern "C" int citrus_multiple_emission(const float* Y, int K, int nsh, const fl
     return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
   if (tau_mix && (tau_channels < 1 || tau_channels > 2))
     return fail(CITRUS_ERR_UNSUPPORTED, "1 or 2 traced channels with a mix table");
-  if ((N + 63) / 64 > 0x7fffffffLL || (K + 63) / 64 > 65535) return fail(CITRUS_ERR_ARG, "too many cells or directions");
+  if (!grid_fits((N + 63) / 64, 256) || (K + 63) / 64 > 65535) return fail(CITRUS_ERR_ARG, "too many cells or directions");
   // the lanes' moment offsets within four moment rows are 32-bit byte offsets
   if ((3 * (int64_t)G + 1) * N * (int64_t)sizeof(float) > 0xffffffffLL)
     return fail(CITRUS_ERR_UNSUPPORTED, "(3 G + 1) N x 4 bytes must stay below 4 GiB");
@@ -301,7 +301,7 @@ extern "C" int citrus_sh_expand(const float* Y, int K, int nsh, const float* B, 
   if (nsh < 1 || nsh > 64) return fail(CITRUS_ERR_UNSUPPORTED, "nsh must be (L+1)^2 with L <= 7");
   if (sizeof(float) * (size_t)((K + 15) / 16) * 16 * (4 * ((nsh + 3) / 4) + 1) > (size_t)kMaxLdsBytes)
     return fail(CITRUS_ERR_ARG, "too many directions: the directions' harmonics (K x nsh, padded) must fit 64 KB of LDS");
-  if ((ncol + kBlockCols - 1) / kBlockCols > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many columns");
+  if (!grid_fits((ncol + kBlockCols - 1) / kBlockCols, 64 * kWaves)) return fail(CITRUS_ERR_ARG, "too many columns");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipError_t e;
   switch (nsh) {

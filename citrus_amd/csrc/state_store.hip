@@ -152,7 +152,7 @@ int check(int V, int G, int nsh, int64_t N) {
   if (V < 1 || G < 1 || N < 1) return fail(CITRUS_ERR_ARG, "V, G and N must be >= 1");
   if (nsh < 1 || nsh > kMaxMoments) return fail(CITRUS_ERR_ARG, "nsh must be (L+1)^2 <= 64");
   if ((int64_t)V * G > 65535) return fail(CITRUS_ERR_ARG, "at most 65535 (view, group) slabs per call");
-  if (N > (int64_t)0x7fffffff * kThreads) return fail(CITRUS_ERR_ARG, "too many cells");
+  if (!grid_fits(blocks_per_slab(N), kThreads)) return fail(CITRUS_ERR_ARG, "too many cells");
   return CITRUS_OK;
 }
 

@@ -315,7 +315,7 @@ template <bool TRANSPOSED, bool ACT>
 int launch_conv(int k, const float* in, int64_t in_bs, int Cin, Vol v, const uint16_t* w, int w_ci,
                 const float* bias, float* out, int64_t out_bs, int Cout, int B, hipStream_t s) {
   const int64_t tiles = (int64_t)((v.W + kTx - 1) / kTx) * ((v.H + kTy - 1) / kTy) * ((v.D + kTz - 1) / kTz);
-  if (tiles > 0x7fffffffLL || (Cout + kCob - 1) / kCob > 65535) return fail(CITRUS_ERR_ARG, "grid too large");
+  if (!grid_fits(tiles, kThreads) || (Cout + kCob - 1) / kCob > 65535) return fail(CITRUS_ERR_ARG, "grid too large");
   const dim3 grid((unsigned)tiles, (unsigned)((Cout + kCob - 1) / kCob), (unsigned)B);
   if (k == 3)
     hipLaunchKernelGGL((conv_kernel<3, TRANSPOSED, ACT>), grid, dim3(kThreads), 0, s, in, in_bs, Cin, v, w, w_ci,
@@ -375,7 +375,7 @@ extern "C" int citrus_nn_conv3d_backward_weight(const float* x, int64_t x_bstrid
   const int tiles_per_chunk = 16;
   const int64_t tiles = (int64_t)((W + kTx - 1) / kTx) * ((H + kTy - 1) / kTy) * ((D + kTz - 1) / kTz) * B;
   const int64_t chunks = (tiles + tiles_per_chunk - 1) / tiles_per_chunk;
-  if (chunks > 0x7fffffffLL) return fail(CITRUS_ERR_ARG, "too many tiles");
+  if (!grid_fits(chunks, kThreads)) return fail(CITRUS_ERR_ARG, "too many tiles");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)chunks, (unsigned)Ci, (unsigned)((Co + kCob - 1) / kCob));
   if (k == 3)

@@ -219,6 +219,7 @@ extern "C" int citrus_group_transmission(const float* paths, int M, int64_t nray
   if (nrays == 0) return CITRUS_OK;
   const size_t lds = sizeof(float) * (size_t)(M + 3) * E;
   const int64_t blocks = (nrays + kThreads - 1) / kThreads;
+  if (!grid_fits(blocks, kThreads)) return fail(CITRUS_ERR_ARG, "too many rays for one launch (2^32)");
   hipLaunchKernelGGL(group_transmission_kernel, dim3((unsigned)blocks), dim3(kThreads), lds,
                      reinterpret_cast<hipStream_t>(stream), F, paths, nrays, filter_path, out);
   hipError_t e = hipGetLastError();
@@ -267,6 +268,7 @@ extern "C" int citrus_uncollided_fluence(const float* paths, int M, const double
   if (npts == 0) return CITRUS_OK;
   const size_t lds = sizeof(float) * (size_t)(M + 3) * E;
   const int64_t blocks = (npts + kThreads - 1) / kThreads;
+  if (!grid_fits(blocks, kThreads)) return fail(CITRUS_ERR_ARG, "too many points for one launch (2^32)");
   hipLaunchKernelGGL(uncollided_kernel, dim3((unsigned)blocks), dim3(kThreads), lds,
                      reinterpret_cast<hipStream_t>(stream), F, P, paths, npts, lattice[0], lattice[1], lattice[2],
                      lattice[3], lattice[4], lattice[5], lattice_shape[1], lattice_shape[2], out);

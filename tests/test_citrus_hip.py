@@ -229,3 +229,31 @@ def test_cropped_helical_step_equals_the_covering_grid_without_the_rest():
         assert rel_err(results[v][0], full[0]) < F32_REL
         sub = fstate.collided_moments[..., k0 // 2:k1 // 2]
         assert rel_err(states[v].collided_moments, sub) < F32_REL  # a photon leaving the convex slab never returns
+
+
+def test_bf16_view_step_at_p5_sixteen_groups_and_s12():
+    """The mixed-precision view step at the benchmark's angular and energy shape (16 groups, P5, S12: 21 directions per
+    chunk, two direction tiles) on a small grid (16^3 transport cells), two views, four steps: every state stays finite
+    and the scatter image is within 1e-3 of the float32 steps."""
+    dev = gpu_device()
+    library = seeded_library(16, 5, 140.0, 20.0, seed=99, num_eta=257)
+    config = SolverConfig(num_groups=16, legendre_order=5, sweep_quadrature="level_symmetric", sn_order=12,
+                          transport_cell_cm=1.28, quadrature_point_spacing_cm=2.0, cone_num_polar=4,
+                          cone_num_azimuthal=8)
+    geometry = ConeGeometry(100.0, 150.0, 64, 64, 0.48, 0.48, use_curved_detector=True)
+    solver = ScatterSolver(library, config, geometry, device=dev)
+    fine = Grid((64, 64, 64), 0.32)
+    fields = solver.material_fields(ellipsoid_phantom(fine, seed=1234).to(dev), fine)
+    assert fields.grid_coarse.shape == (16, 16, 16) and fields.q == 1
+    angles = [0.0, 1.3]
+    s32 = [CitrusState() for _ in angles]
+    s16 = [CitrusState() for _ in angles]
+    for step in range(4):
+        r32, s32 = citrus_views_step(solver, s32, fields, angles)
+        r16, s16 = citrus_views_step(solver, s16, fields, angles, contraction="bf16")
+        for v in range(len(angles)):
+            assert bool(torch.isfinite(s32[v].collided_moments).all()), (step, v, "fp32")
+            assert bool(torch.isfinite(s16[v].collided_moments).all()), (step, v, "bf16")
+        image = max(rel_err(r16[v][0], r32[v][0]) for v in range(len(angles)))
+        print(f"P5 bf16 step {step}: image error {image:.2e}", flush=True)
+        assert image < 1e-3

@@ -182,17 +182,20 @@ def test_fc_correction_conserves_the_exact_moments():
     assert float((J - E[1])[free].abs().max()) < 1e-5 * float(E[1].abs().max())
 
 
-def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike():
+@pytest.mark.parametrize("sn,L,contraction", [(6, 3, "fp32"), (12, 5, "fp32"), (12, 5, "bf16"), (8, 3, "bf16")])
+def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L, contraction):
     """The sampling correction applied away from the sums kernel: by ``citrus_fc_correct_means`` or by the
     persistent sweep as it reads the stored samples (source layout 3). Both give the corrected samples of
-    ``first_collision_corrected_samples`` (and their sweep) bit for bit; the launch sequence applies it first."""
+    ``first_collision_corrected_samples`` (and their sweep) bit for bit; the launch sequence applies it first. S12
+    puts 21 directions in a chunk (two direction tiles); bf16 runs the mixed-precision expansion."""
+    from citrus_amd.scatter_solver.source_iteration import bf16_operand
     from citrus_amd.scatter_solver.transport_sweep import make_sweep_context, sweep_views
     from citrus_amd.scatter_solver.uncollided import DeferredFirstCollision, first_collision_deferred_samples
 
     dev = gpu_device()
     library = seeded_library(6, 3, 120.0, 20.0, seed=15, num_eta=97)
     grid = Grid((8, 8, 8), 0.5)
-    ctx = make_sweep_context(grid, level_symmetric_quadrature(6), 3, device=dev)
+    ctx = make_sweep_context(grid, level_symmetric_quadrature(sn), L, device=dev)
     om, w = ctx.quadrature.omega.to(dev, torch.float32), ctx.quadrature.weight
     V, G, M = 2, library.num_groups, ctx.num_directions
     deferred = DeferredFirstCollision.empty(V, G, grid.shape, M, dev)
@@ -210,8 +213,12 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike():
     gen = torch.Generator(device=dev).manual_seed(3)
     mu = torch.rand(G, *grid.shape, device=dev, generator=gen) * 0.5 + 0.2
     T = torch.randn(V, G, *grid.shape, ctx.num_moments, 4, device=dev, generator=gen) * 0.01
-    want = sweep_views(ctx, mu, T, source=corrected, schedule="persistent")
-    got = sweep_views(ctx, mu, T, source=deferred, schedule="persistent")
+    if contraction == "bf16":
+        T = bf16_operand(T)
+    print(f"deferred correction: S{sn} P{L} {contraction}", flush=True)
+    want = sweep_views(ctx, mu, T, source=corrected, schedule="persistent", contraction=contraction)
+    got = sweep_views(ctx, mu, T, source=deferred, schedule="persistent", contraction=contraction)
+    assert bool(torch.isfinite(want).all()) and bool(torch.isfinite(got).all())
     assert not deferred.applied and torch.equal(got, want)
-    got = sweep_views(ctx, mu, T, source=deferred, schedule="launches")
+    got = sweep_views(ctx, mu, T, source=deferred, schedule="launches", contraction=contraction)
     assert deferred.applied and torch.equal(deferred.samples, corrected) and torch.equal(got, want)

@@ -316,3 +316,51 @@ def test_multiple_emission_at_the_view_step_size():
         want = torch.einsum("kg,kgn->kn", weight, e_t * S.double())
         scale = torch.einsum("kg,kgn->kn", weight, e_t * S_abs.double()) + 1e-30
         assert float(((got[:, c] - want).abs() / scale).max()) < 2e-6, c
+
+
+@pytest.mark.parametrize("G", [8, 16])
+def test_collapsed_emission_with_split_groups_matches_oracle(G):
+    """The view step's collapsed emission at 8 and 16 groups (each (point, direction) split over two threads by group
+    channels) against the FP64 restatement: sum_g w_kg S_g(p, w_k) exp(-sum_c mix_gc tau_kc) plus the collapsed
+    multiple-collision part, unreachable cells (tau = inf) contributing nothing, and the per-block sums of the
+    footprint's emission-weighted heights."""
+    from citrus_amd.scatter_solver.scatter_image import _attenuated_emission
+
+    dev = gpu_device()
+    library = seeded_library(G, 3, 120.0, 20.0, seed=23, num_eta=129)
+    grid = Grid((6, 8, 9), 0.5)
+    geometry = ConeGeometry(40.0, 60.0, 12, 14, 0.6, 0.6)
+    angle = 0.4
+    gen = torch.Generator().manual_seed(8)
+    P, K = grid.num_voxels, 7
+    rho = (torch.rand(2, *grid.shape, generator=gen, dtype=torch.float64) * 1.2).float()
+    phi = (torch.rand(G, *grid.shape, generator=gen, dtype=torch.float64) * 0.01 + 0.001).float()
+    om = torch.randn(K, 3, generator=gen, dtype=torch.float64)
+    om = (om / om.norm(dim=1, keepdim=True)).float()
+    tau = torch.rand(K, 2, P, generator=gen) * 2.0
+    tau[0, :, :5] = float("inf")
+    mix = torch.rand(G, 2, generator=gen)
+    multiple = torch.randn(K, 2, P, generator=gen) * 1e-3
+    w = torch.rand(K, G, generator=gen)
+    heights = torch.randn(P, generator=gen)
+    d = lambda t: t.to(dev)
+    out, partial = _attenuated_emission(library, d(rho), d(phi), grid, geometry, angle, d(om), d(tau),
+                                        d(torch.zeros(G, P)), d(multiple), 1, d(heights), d(w), tau_mix=d(mix),
+                                        collapsed=True)
+    pts = grid.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    ou = ofc.primary_directions(pts, np.array(geometry.source_position(angle)))
+    S = ofc.source_at_directions(library.mu_s_eta.numpy(), None, rho.double().reshape(2, -1).numpy(),
+                                 phi.double().reshape(G, -1).numpy(), ou, om.double().numpy())  # [K, G, P]
+    t = np.einsum("gc,kcp->kgp", mix.double().numpy(), tau.double().numpy())
+    e = S * np.exp(-t)
+    want = np.einsum("kg,kgp->kp", w.double().numpy(), e) + multiple[:, 0].double().numpy()
+    esum = e.sum(1) + multiple[:, 1].double().numpy()
+    assert rel_err(out.reshape(K, P), want) < F32_REL
+    assert bool(torch.isfinite(out).all()) and float(np.abs(e[0, :, :5]).max()) == 0.0
+    nb = int(partial.shape[1])
+    sums = np.zeros((K, nb, 2))
+    for b in range(nb):
+        sl = slice(256 * b, min(P, 256 * (b + 1)))
+        sums[:, b, 0] = esum[:, sl].sum(1)
+        sums[:, b, 1] = (esum[:, sl] * heights[sl].double().numpy()).sum(1)
+    assert rel_err(partial, sums) < F32_REL

@@ -399,3 +399,21 @@ def test_sweep_autotuner_picks_and_caches_a_chunk():
     assert schedule == "launches"  # P2: the persistent kernel is built for P3 and P5 only
     assert torch.equal(auto, sweep_views(ctx, mu, T, direction_chunk=chosen, schedule=schedule))
     assert rel_err(auto, sweep_views(ctx, mu, T)) < F32_REL
+
+
+def test_bf16_operand_past_two_to_the_32_work_items():
+    """The operand conversion of 8 views x 16 groups x 64^3 cells x P5 spans 4.8e9 (cell, component, moment) slots,
+    more than one dispatch's 2^32 work-items: the kernel loops with a stride, so the last view, which lies past the
+    first 2^32 slots, is converted exactly (low: copied; high: bfloat16, round to nearest even)."""
+    from citrus_amd.scatter_solver.source_iteration import bf16_operand
+
+    dev = gpu_device()
+    V, G, n, nsh = 8, 16, 64, 36
+    T = torch.zeros(V, G, n, n, n, nsh, 4, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    T[V - 1].normal_(generator=gen)
+    op = bf16_operand(T)
+    last = T[V - 1]  # [G, R, C, S, nsh, 4]
+    assert torch.equal(op.low[V - 1], last[..., :4, :].transpose(-1, -2).contiguous())
+    assert torch.equal(op.high[V - 1, ..., :nsh - 4], last[..., 4:, :].transpose(-1, -2).to(torch.bfloat16))
+    assert not bool(op.low[:V - 1].any()) and not bool(op.high[:V - 1].float().any())
