@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--views", type=int, default=4, help="views per step (each with its own CITRUS state)")
+    ap.add_argument("--views", type=int, default=8, help="views per step (each with its own CITRUS state)")
     ap.add_argument("--sweep-only", action="store_true", help="time the sweep-only step alone")
     ap.add_argument("--cells", type=int, default=64, help="transport cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
