@@ -335,6 +335,7 @@ def build_view_solver(args, device, rho_fine):
 
 
 REPEATS = 3  # timed repeats of each helical step
+WARM = 2  # untimed repeats before them
 
 
 def helical_views(args, device, rho_fine, views: int = 8):
@@ -381,9 +382,13 @@ def helical_views(args, device, rho_fine, views: int = 8):
 
     out = {}
     for name, step in (("cropped", cropped), ("covering", covering)):
+        states = res = None
+        torch.cuda.empty_cache()
         states = [CitrusState(cone_quadrature=c) for c in cones]
         res = step(states)
         states = res[1]
+        for _ in range(WARM):  # the caching allocator reaches its steady state (fresh device memory is cleared by
+            res = step(states)  # the driver, about 20 ms per GB, so a step that allocates new segments is slower)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(REPEATS):  # the same step from the same states: the mean of a few repeats
@@ -408,7 +413,8 @@ def helical_views(args, device, rho_fine, views: int = 8):
     fresh = [CitrusState(cone_quadrature=c) for c in cones]
     _, early, _ = citrus_multigrid_views_step(sched, fresh[:half], rho, fine, angles[:half], crop=True)
     mixed = early + fresh[half:]
-    res = citrus_multigrid_views_step(sched, mixed, rho, fine, angles, crop=True)  # warm-up of this configuration
+    for _ in range(WARM):  # warm-up of this configuration
+        res = citrus_multigrid_views_step(sched, mixed, rho, fine, angles, crop=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(REPEATS):
@@ -625,7 +631,7 @@ def main():
         # the same steps with the mixed-precision source contraction: rate and error against the float32 steps
         if not args.profile:
             view.update(mixed_precision_steps(view_step, states, images, V, args, stream))
-        del states, images
+        del states, images, view_step  # the closure holds the states: without it their 6 GB per view stay allocated
         torch.cuda.empty_cache()
         if not args.profile:
             view["scan_stage"] = scan_stage(solver, fields, device, views=16, batch=V)

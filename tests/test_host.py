@@ -297,3 +297,15 @@ def test_dataset_round_trip_and_the_reference_layout(tmp_path):
         load_dataset(tmp_path / "old.npz")
     with pytest.raises(ValueError):
         save_dataset({"x": np.ones(2)}, tmp_path / "bad")
+
+
+def test_tuner_prefers_the_persistent_schedule_within_its_margin():
+    from citrus_amd.scatter_solver.autotune import PERSISTENT_MARGIN, pick_config
+
+    assert pick_config({("launches", 21): 118.0, ("persistent", 21): 119.3, ("persistent", 11): 214.6}) == \
+        ("persistent", 21)
+    far = 100.0 * PERSISTENT_MARGIN + 1.0
+    assert pick_config({("launches", 11): 100.0, ("persistent", 21): far}) == ("launches", 11)
+    assert pick_config({("launches", 7): 50.0, ("launches", 21): 40.0}) == ("launches", 21)
+    with pytest.raises(ValueError):
+        pick_config({})
