@@ -13,7 +13,9 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 13
+ABI_VERSION = 14
+# points per block of the emission kernels' partial sums (CITRUS_EMISSION_BLOCK of include/citrus_hip.h)
+EMISSION_BLOCK = 32
 DISABLE_ENV_VAR = "CITRUS_DISABLE_HIP"
 
 _lib = None

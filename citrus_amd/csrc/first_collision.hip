@@ -481,112 +481,274 @@ __global__ __launch_bounds__(kBlock) void fc_moments_kernel(FcDev d, const float
   }
 }
 
+// The source of the output channel ch (groups 4 ch .. 4 ch + 3) at eta node j, contracted once per point: node j of
+// eval_source's loop on its own, restricted to one channel (per accumulator the slots and source groups in the same
+// order, the same fused multiply-adds), so that interpolating two contracted nodes gives eval_source's value bit for
+// bit. ph: the point's phi_UC of every group (0 past G), a row of LDS; ps: its material slots. The table rows of four
+// source groups are requested together before they are added.
+template <int GMAX>
+__device__ __forceinline__ f4 contract_node_channel(const FcDev& d, const PointSlots& ps, const float* ph, int j,
+                                                    int ch) {
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int64_t node = (int64_t)d.M * d.G * GMAX;
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    if (s >= d.n_slots) break;
+    const float rho = ps.rho[s];
+    if (rho == 0.0f) continue;  // as in eval_source
+    const float* t0 = d.table + (int64_t)j * node + (int64_t)ps.mat[s] * d.G * GMAX;
+#pragma unroll
+    for (int gb = 0; gb < GMAX / 4; ++gb) {
+      if (gb > ch) break;
+      f4 x[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gi = 4 * gb + q;
+        const int row = gi < d.G ? gi : d.G - 1;
+        x[q] = reinterpret_cast<const f4*>(t0 + row * GMAX)[ch];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc = fma4(rho * ph[4 * gb + q], x[q], acc);
+    }
+  }
+  return acc;
+}
+
+// exp(-t) for t >= 0 (+inf gives 0) as the hardware base-2 exponential of -t log2(e): relative error about
+// |t| 6e-8 + 1 ulp, against the several instructions of expf's range handling per call.
+__device__ __forceinline__ float attenuation(float t) { return __builtin_amdgcn_exp2f(-t * 1.44269504088896341f); }
+
+// Workgroup shape of the emission kernel: a run of kEmPoints consecutive points (coalesced outputs) times the
+// directions of a tile of kEmTile, each point's directions spread over kEmLanes threads (kEmPerLane each).
+constexpr int kEmThreads = 512;
+constexpr int kEmPoints = CITRUS_EMISSION_BLOCK;
+constexpr int kEmLanes = kEmThreads / kEmPoints;
+constexpr int kEmTile = 64;
+constexpr int kEmPerLane = kEmTile / kEmLanes;
+
+template <int GMAX>
+struct EmissionShape {
+  static constexpr int kNodes = (GMAX <= 16 ? 384 : 352) / GMAX;  // eta nodes of a slice in LDS (24 at 16 groups)
+  static constexpr int kStride = kNodes * GMAX + 4;  // floats per point: 16-byte rows of neighbouring points 4 apart
+};
+constexpr int kEmSumStride = kEmTile + 1;  // floats per point of the per-pair sums (rows in different banks)
+
 // Attenuated total emission of the collapsed-cone image at the quadrature points (uncollided_kernels.py:713-830,
 // scatter_image.py:829-875): e_g = (S_FC,g(point, w) + S_MC,g(cell, w)) exp(-tau_j,g) with the first-order in-cell
-// optical depth tau_j = max(0, tau_cell - mu_cell (w . (r_j - r_cell))) (tau_cell exactly for f = 1). Thread per
-// (point, direction), blockIdx.y = direction; each block also reduces sum_g e and h sum_g e over its points into
-// partial [K][blocks][2] (no atomics), the sums of the footprint's emission-weighted mean height.
+// optical depth tau_j = max(0, tau_cell - mu_cell (w . (r_j - r_cell))) (tau_cell exactly for f = 1).
+//
+// A workgroup takes kEmPoints consecutive points and every direction, in tiles of kEmTile. The directions of a tile
+// leave a point within a narrow cone, so their eta = Omega_UC . w fall in a short run of table nodes: the workgroup
+// finds that window, contracts each of its points' sources at every node of the window once into LDS
+// (contract_node_channel, one (node, channel) per thread and step) and interpolates per (point, direction) from LDS --
+// eval_source's value bit for bit, with about 7x fewer table gathers than two nodes per (point, direction) at 64
+// panel-facing directions and 257 nodes. A window wider than LDS holds is taken in slices of kNodes nodes overlapping
+// by one; each (point, direction) is evaluated in the slice that holds both of its nodes.
+// Each (direction, workgroup) also sums sum_g e and h sum_g e over the workgroup's points into
+// partial [K][blocks][2] (through LDS, in point order; no atomics): the footprint's emission-weighted mean height.
 // COLLAPSED (one point per cell, signal weights): `multiple` is [K][2][cells], the multiple-collision part already
 // attenuated and summed over the groups, weighted (0) and unweighted (1), by citrus_multiple_emission.
-template <int GMAX, bool COLLAPSED>
-// 4 waves per SIMD (121 VGPRs with the prefetched multiple-collision loads): forcing 5, 6 or 8 spills and is slower
-__global__ __launch_bounds__(kBlock) void emission_kernel(FcDev d, const float* __restrict__ tau, int tau_channels,
-                                                          const float* __restrict__ tau_mix,
-                                                          const float* __restrict__ mu_c,
-                                                          const float* __restrict__ multiple, int f,
-                                                          const float* __restrict__ heights,
-                                                          const float* __restrict__ signal_weight,
-                                                          float* __restrict__ out, float* __restrict__ partial) {
-  __shared__ float red[2][kBlock];
-  // the block's direction: its signal weights per group and the mix table, read from LDS in the group loop
-  __shared__ float s_weight[32];
+template <int GMAX, bool COLLAPSED, bool FAST>
+__global__ __launch_bounds__(kEmThreads) __attribute__((amdgpu_waves_per_eu(4))) void emission_kernel(
+    FcDev d, const float* __restrict__ tau, int tau_channels, const float* __restrict__ tau_mix,
+    const float* __restrict__ mu_c, const float* __restrict__ multiple, int f, const float* __restrict__ heights,
+    const float* __restrict__ signal_weight, float* __restrict__ out, float* __restrict__ partial) {
+  using Shape = EmissionShape<GMAX>;
+  constexpr int NCH = GMAX / 4;
+  __shared__ f4 s_nodes[kEmPoints * Shape::kStride / 4];  // [point][node of the slice][GMAX] contracted sources
+  __shared__ f4 s_dir[kEmTile];                            // the tile's directions
+  __shared__ float s_weight[kEmTile * GMAX];               // their signal weights per group
   __shared__ float s_mix[32 * kMaxTauChannels];
-  const int k = blockIdx.y;
-  const int p = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ float s_sum[2][kEmPoints * kEmSumStride];    // per (point, direction): sum_g e and h sum_g e
+  __shared__ float s_ph[kEmPoints][GMAX + 1];              // phi_UC of the workgroup's points
+  __shared__ int s_window[2];
+  float* const s_node = reinterpret_cast<float*>(s_nodes);
+
+  const int tid = threadIdx.x;
+  const int pl = tid % kEmPoints, dl = tid / kEmPoints;
+  const int p = blockIdx.x * kEmPoints + pl;
+  const bool active = p < d.P;
+  const int pp = active ? p : d.P - 1;  // inactive threads evaluate nothing but take part in the shuffles
+  const int r = pp / (d.C * d.S), c = (pp / d.S) % d.C, s = pp % d.S;
   const int Rc = d.R / f, Cc = d.C / f, Sc = d.S / f;
   const int Nc = Rc * Cc * Sc;
-  if (signal_weight && threadIdx.x < d.G) s_weight[threadIdx.x] = signal_weight[(int64_t)k * d.G + threadIdx.x];
-  if (tau_mix && threadIdx.x < d.G * tau_channels) s_mix[threadIdx.x] = tau_mix[threadIdx.x];
-  __syncthreads();
-  float esum = 0.0f, ehsum = 0.0f;
-  if (p < d.P) {
-    const int r = p / (d.C * d.S), c = (p / d.S) % d.C, s = p % d.S;
-    const int cell = ((r / f) * Cc + (c / f)) * Sc + (s / f);
-    const float ox = d.omegas[3 * k], oy = d.omegas[3 * k + 1], oz = d.omegas[3 * k + 2];
-    float ux, uy, uz;
-    primary_direction(d, r, c, s, ux, uy, uz);
-    // the cell's multiple-collision emission first: its loads are in flight during the table gathers (the loads
-    // issued after the gathers cost 3.3 of the kernel's 11 ms per 64^3 view)
-    float mc[GMAX], mcw = 0.0f, mcs = 0.0f;
+  const int cell = ((r / f) * Cc + (c / f)) * Sc + (s / f);
+  float ux, uy, uz;
+  primary_direction(d, r, c, s, ux, uy, uz);
+  for (int g = dl; g < GMAX; g += kEmLanes) s_ph[pl][g] = g < d.G ? d.phi_uc[(int64_t)g * d.P + pp] : 0.0f;
+  PointSlots ps;
+  load_slots(d, pp, ps);
+  const float h = heights ? heights[pp] : 0.0f;
+  // offset of the point from its cell centre (centred sub-point index times the pitch), dotted with w per direction
+  const float half = 0.5f * (float)(f - 1);
+  const float offx = ((float)(c % f) - half) * (float)d.dx, offy = ((float)(r % f) - half) * (float)d.dy,
+              offz = ((float)(s % f) - half) * (float)d.dz;
+  if (tau_mix && tid < d.G * tau_channels) s_mix[tid] = tau_mix[tid];
+  const bool two = tau_mix && tau_channels <= 2;  // the depths of a direction in two registers
+
+  // e of (point, direction j of the tile, cell means interpolated from the slice's LDS row a0) into out, returning
+  // sum_g e (with the collapsed unweighted part)
+  auto emit = [&](const int k, const int j, const f4 w, const f4* a0, const float w1) __attribute__((always_inline)) {
+    if constexpr (FAST) {
+      // the view step's form: collapsed, two traced depth channels, signal weights, one point per cell, G == GMAX
+      const float mcw = multiple[(int64_t)k * 2 * Nc + cell];
+      const float mcs = multiple[((int64_t)k * 2 + 1) * Nc + cell];
+      const float tc0 = tau[(int64_t)k * 2 * Nc + cell];
+      const float tc1 = tau[((int64_t)k * 2 + 1) * Nc + cell];
+      const bool unreachable = tc0 == INFINITY;
+      float sig = 0.0f, es = 0.0f;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const f4 v = interpolate4(a0[ch], a0[NCH + ch], w1);
+        const f4 wv = reinterpret_cast<const f4*>(s_weight + j * GMAX)[ch];
+#pragma unroll
+        for (int gl = 0; gl < 4; ++gl) {
+          const int g = 4 * ch + gl;
+          float t = fmaf(tau_mix[2 * g + 1], tc1, tau_mix[2 * g] * tc0);
+          t = unreachable ? INFINITY : t;
+          const float e = v[gl] * attenuation(t);
+          es += e;
+          sig = fmaf(wv[gl], e, sig);
+        }
+      }
+      out[(int64_t)k * d.P + p] = sig + mcw;
+      return es + mcs;
+    }
+    float mcw = 0.0f, mcs = 0.0f;
     if constexpr (COLLAPSED) {
       mcw = multiple[(int64_t)k * 2 * Nc + cell];
       mcs = multiple[((int64_t)k * 2 + 1) * Nc + cell];
-    } else {
-#pragma unroll
-      for (int g = 0; g < GMAX; ++g) mc[g] = g < d.G ? multiple[((int64_t)k * d.G + g) * Nc + cell] : 0.0f;
     }
-    PointData<GMAX> pd;
-    load_point(d, p, pd);
-    float S[GMAX];
-    eval_source(d, pd, direction_cosine(ux, uy, uz, ox, oy, oz), S);
-    // offset of the point from its cell centre along w (centred sub-point index times the pitch)
-    const float half = 0.5f * (float)(f - 1);
-    const float off = f > 1 ? ox * (((float)(c % f) - half) * (float)d.dx) + oy * (((float)(r % f) - half) * (float)d.dy)
-                                  + oz * (((float)(s % f) - half) * (float)d.dz)
-                            : 0.0f;
-    float tc[kMaxTauChannels];  // optical depths of the traced channels (with a mix table)
-    bool unreachable = false;
-    if (tau_mix) {
-#pragma unroll
-      for (int c = 0; c < kMaxTauChannels; ++c)
-        tc[c] = c < tau_channels ? tau[((int64_t)k * tau_channels + c) * Nc + cell] : 0.0f;
-      unreachable = tc[0] == INFINITY;
+    float tc0 = 0.0f, tc1 = 0.0f;
+    if (two) {
+      tc0 = tau[(int64_t)k * tau_channels * Nc + cell];
+      tc1 = tau_channels > 1 ? tau[((int64_t)k * tau_channels + 1) * Nc + cell] : 0.0f;
     }
-    float sig = 0.0f;
+    const bool unreachable =
+        tau_mix && (two ? tc0 : tau[(int64_t)k * tau_channels * Nc + cell]) == INFINITY;
+    const float off = f > 1 ? w.x * offx + w.y * offy + w.z * offz : 0.0f;
+    float sig = 0.0f, es = 0.0f;
 #pragma unroll
-    for (int g = 0; g < GMAX; ++g) {
-      if (g >= d.G) continue;
-      const int64_t kg = (int64_t)k * d.G + g;
-      float t;
-      if (tau_mix) {
-        t = 0.0f;
+    for (int ch = 0; ch < NCH; ++ch) {
+      const f4 v = interpolate4(a0[ch], a0[NCH + ch], w1);
+      const f4 wv = signal_weight ? reinterpret_cast<const f4*>(s_weight + j * GMAX)[ch] : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int c = 0; c < kMaxTauChannels; ++c)
-          if (c < tau_channels) t = fmaf(s_mix[g * tau_channels + c], tc[c], t);
-        if (unreachable) t = INFINITY;
-      } else {
-        t = tau[kg * Nc + cell];
-      }
-      if (f > 1) t = fmaxf(t - mu_c[(int64_t)g * Nc + cell] * off, 0.0f);
-      const float e = COLLAPSED ? S[g] * expf(-t) : (S[g] + mc[g]) * expf(-t);
-      esum += e;
-      if (signal_weight) {
-        sig = fmaf(s_weight[g], e, sig);
-      } else {
-        out[kg * d.P + p] = e;
+      for (int gl = 0; gl < 4; ++gl) {
+        const int g = 4 * ch + gl;
+        if (g >= d.G) continue;
+        const int64_t kg = (int64_t)k * d.G + g;
+        float t;
+        if (two) {  // the mix coefficients are uniform: scalar loads
+          t = fmaf(tau_mix[g * tau_channels], tc0, 0.0f);
+          if (tau_channels > 1) t = fmaf(tau_mix[g * tau_channels + 1], tc1, t);
+          if (unreachable) t = INFINITY;
+        } else if (tau_mix) {
+          t = 0.0f;
+          for (int ci = 0; ci < tau_channels; ++ci)
+            t = fmaf(s_mix[g * tau_channels + ci], tau[((int64_t)k * tau_channels + ci) * Nc + cell], t);
+          if (unreachable) t = INFINITY;
+        } else {
+          t = tau[kg * Nc + cell];
+        }
+        if (f > 1) t = fmaxf(t - mu_c[(int64_t)g * Nc + cell] * off, 0.0f);
+        const float src = COLLAPSED ? v[gl] : v[gl] + multiple[kg * Nc + cell];
+        const float e = src * attenuation(t);
+        es += e;
+        if (signal_weight) {
+          sig = fmaf(wv[gl], e, sig);
+        } else {
+          out[kg * d.P + p] = e;
+        }
       }
     }
     if constexpr (COLLAPSED) {
       sig += mcw;
-      esum += mcs;
+      es += mcs;
     }
     if (signal_weight) out[(int64_t)k * d.P + p] = sig;
-    ehsum = esum * (heights ? heights[p] : 0.0f);
-  }
-  red[0][threadIdx.x] = esum;
-  red[1][threadIdx.x] = ehsum;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + w];
-      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    return es;
+  };
+
+  for (int k0 = 0; k0 < d.K; k0 += kEmTile) {
+    const int kt = min(kEmTile, d.K - k0);
+    __syncthreads();  // the previous tile's directions, weights, nodes and sums are no longer read
+    if (tid < kt)
+      s_dir[tid] = f4{d.omegas[3 * (k0 + tid)], d.omegas[3 * (k0 + tid) + 1], d.omegas[3 * (k0 + tid) + 2], 0.f};
+    for (int i = tid; i < kEmPoints * kEmSumStride; i += kEmThreads) s_sum[0][i] = s_sum[1][i] = 0.0f;
+    if (signal_weight)
+      for (int i = tid; i < kt * d.G; i += kEmThreads)
+        s_weight[(i / d.G) * GMAX + i % d.G] = signal_weight[(int64_t)k0 * d.G + i];
+    if (tid == 0) {
+      s_window[0] = 0x7fffffff;
+      s_window[1] = -1;
     }
     __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    float* o = partial + ((int64_t)k * gridDim.x + blockIdx.x) * 2;
-    o[0] = red[0][0];
-    o[1] = red[1][0];
+
+    // ---- the window of eta nodes the tile's directions reach from the workgroup's points -------------------------
+    int lo = 0x7fffffff, hi = -1;
+    if (active) {
+      for (int j = dl; j < kt; j += kEmLanes) {
+        const f4 w = s_dir[j];
+        int i0;
+        float w1;
+        table_position(d, direction_cosine(ux, uy, uz, w.x, w.y, w.z), i0, w1);
+        lo = min(lo, i0);
+        hi = max(hi, i0);
+      }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      lo = min(lo, __shfl_xor(lo, m));
+      hi = max(hi, __shfl_xor(hi, m));
+    }
+    if ((tid & 63) == 0) {
+      atomicMin(&s_window[0], lo);
+      atomicMax(&s_window[1], hi);
+    }
+    __syncthreads();
+    const int jlo = s_window[0], jhi = s_window[1];  // lower nodes of the tile (jhi < jlo: no active point)
+
+    // slices of lower nodes [a, a + ns - 2] with their upper nodes: nodes a .. a + ns - 1 in LDS
+    for (int a = jlo; a <= jhi; a += Shape::kNodes - 1) {
+      const int ns = min(Shape::kNodes, jhi + 2 - a);
+      if (a > jlo) __syncthreads();  // the previous slice is no longer read
+      if (active) {
+        for (int q = dl; q < ns * NCH; q += kEmLanes) {
+          const int jj = q / NCH, ch = q % NCH;
+          reinterpret_cast<f4*>(s_node + pl * Shape::kStride + jj * GMAX)[ch] =
+              contract_node_channel<GMAX>(d, ps, s_ph[pl], a + jj, ch);
+        }
+      }
+      __syncthreads();
+
+      // ---- emission per (point, direction) whose nodes the slice holds ----------------------------------------------
+#pragma unroll 2
+      for (int i = 0; i < kEmPerLane; ++i) {
+        const int j = dl + kEmLanes * i;
+        if (!active || j >= kt) continue;
+        const f4 w = s_dir[j];
+        int i0;
+        float w1;
+        table_position(d, direction_cosine(ux, uy, uz, w.x, w.y, w.z), i0, w1);
+        if (i0 < a || i0 > a + ns - 2) continue;
+        const float es =
+            emit(k0 + j, j, w, reinterpret_cast<const f4*>(s_node + pl * Shape::kStride + (i0 - a) * GMAX), w1);
+        s_sum[0][pl * kEmSumStride + j] = es;
+        s_sum[1][pl * kEmSumStride + j] = es * h;
+      }
+    }
+    __syncthreads();
+    // the direction's sums over the workgroup's points, in point order (zeros where a point has no pair)
+    if (tid < kt) {
+      float se = 0.0f, sh = 0.0f;
+      for (int q = 0; q < kEmPoints; ++q) {
+        se += s_sum[0][q * kEmSumStride + tid];
+        sh += s_sum[1][q * kEmSumStride + tid];
+      }
+      float* o = partial + ((int64_t)(k0 + tid) * gridDim.x + blockIdx.x) * 2;
+      o[0] = se;
+      o[1] = sh;
+    }
   }
 }
 
@@ -731,10 +893,12 @@ extern "C" int citrus_attenuated_emission(const CitrusFcArgs* args, const float*
   if (args->K > 65535) return fail(CITRUS_ERR_ARG, "at most 65535 directions per launch");
   const FcDev d = make_dev(*args);
   if (d.K == 0) return CITRUS_OK;
-  const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock), (unsigned)d.K), block(kBlock);
+  const int64_t blocks = ((int64_t)d.P + kEmPoints - 1) / kEmPoints;
+  if (!grid_fits(blocks, kEmThreads)) return fail(CITRUS_ERR_ARG, "points exceed 2^32 work-items per launch");
+  const dim3 grid((unsigned)blocks), block(kEmThreads);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
-    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value, false>), grid, block, 0, s, d, tau, tau_channels,
+    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value, false, false>), grid, block, 0, s, d, tau, tau_channels,
                        tau_mix, mu_t_coarse, multiple, factor, heights, signal_weight, out, partial);
     return 0;
   });
@@ -754,11 +918,21 @@ extern "C" int citrus_attenuated_emission_collapsed(const CitrusFcArgs* args, co
   if (args->K > 65535) return fail(CITRUS_ERR_ARG, "at most 65535 directions per launch");
   const FcDev d = make_dev(*args);
   if (d.K == 0) return CITRUS_OK;
-  const dim3 grid((unsigned)((d.P + kBlock - 1) / kBlock), (unsigned)d.K), block(kBlock);
+  const int64_t blocks = ((int64_t)d.P + kEmPoints - 1) / kEmPoints;
+  if (!grid_fits(blocks, kEmThreads)) return fail(CITRUS_ERR_ARG, "points exceed 2^32 work-items per launch");
+  const dim3 grid((unsigned)blocks), block(kEmThreads);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // the view step's form (two traced depth channels, G filling the kernel's group width) runs without per-group
+  // tests
+  const bool fast = tau_mix && tau_channels == 2;
   dispatch_groups(d.G, [&](auto gm) {
-    hipLaunchKernelGGL((emission_kernel<decltype(gm)::value, true>), grid, block, 0, s, d, tau, tau_channels,
-                       tau_mix, nullptr, collapsed, 1, heights, signal_weight, out, partial);
+    constexpr int GM = decltype(gm)::value;
+    if (fast && d.G == GM)
+      hipLaunchKernelGGL((emission_kernel<GM, true, true>), grid, block, 0, s, d, tau, tau_channels, tau_mix, nullptr,
+                         collapsed, 1, heights, signal_weight, out, partial);
+    else
+      hipLaunchKernelGGL((emission_kernel<GM, true, false>), grid, block, 0, s, d, tau, tau_channels, tau_mix,
+                         nullptr, collapsed, 1, heights, signal_weight, out, partial);
     return 0;
   });
   return finish("attenuated emission (collapsed)");

@@ -446,7 +446,7 @@ def _attenuated_emission(library, rho, phi_uncollided, grid_q, geometry, angle, 
     multiple = multiple.to(device, torch.float32).contiguous()
     sw = None if signal_weight is None else signal_weight.to(device, torch.float32).contiguous()
     out = torch.empty((k, 1 if sw is not None else G) + tuple(grid_q.shape), device=device, dtype=torch.float32)
-    nb = -(-P // 256)
+    nb = -(-P // _hip.EMISSION_BLOCK)
     partial = torch.empty(k, nb, 2, device=device, dtype=torch.float32)
     h = None if heights is None else heights.to(device, torch.float32).contiguous()
     channels = G

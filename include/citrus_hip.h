@@ -438,8 +438,10 @@ int citrus_fc_moments(const CitrusFcArgs* args, const float* scale, const float*
  * tau_mix NULL, or [K][tau_channels][cells] (tau_channels <= 8) with tau_mix [G][tau_channels], the cell's optical
  * depth of group g then being sum_c tau_mix[g][c] tau[k][c][cell] (+inf where tau[k][0][cell] is +inf); mu_t_coarse
  * [G][cells]; heights [P] panel heights of the points (or NULL: 0); signal_weight [K][G] (or NULL). out [K][G][P], or
- * [K][P] = sum_g signal_weight e_g with signal_weight. partial [K][ceil(P / 256)][2]: per block of 256 points the sums
- * of sum_g e_g and h sum_g e_g (unweighted), which the caller reduces. */
+ * [K][P] = sum_g signal_weight e_g with signal_weight. partial [K][ceil(P / CITRUS_EMISSION_BLOCK)][2]: per block of
+ * CITRUS_EMISSION_BLOCK consecutive points the sums of sum_g e_g and h sum_g e_g (unweighted), which the caller
+ * reduces. */
+#define CITRUS_EMISSION_BLOCK 32
 int citrus_attenuated_emission(const CitrusFcArgs* args, const float* tau, int tau_channels, const float* tau_mix,
                                const float* mu_t_coarse, const float* multiple, int factor, const float* heights,
                                const float* signal_weight, float* out, float* partial, void* stream);

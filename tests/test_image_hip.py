@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 import torch
 
+from citrus_amd import _hip
 from citrus_amd.scatter_solver.angular_quadrature import Quadrature, hemisphere_quadrature, real_spherical_harmonics
 from citrus_amd.scatter_solver.cross_sections import seeded_library, synthetic_library
 from citrus_amd.scatter_solver.detector import energy_weighted_response, ideal_counting_response, no_grid
@@ -318,12 +319,86 @@ def test_multiple_emission_at_the_view_step_size():
         assert float(((got[:, c] - want).abs() / scale).max()) < 2e-6, c
 
 
-@pytest.mark.parametrize("G", [8, 16])
-def test_collapsed_emission_with_split_groups_matches_oracle(G):
-    """The view step's collapsed emission at 8 and 16 groups (each (point, direction) split over two threads by group
-    channels) against the FP64 restatement: sum_g w_kg S_g(p, w_k) exp(-sum_c mix_gc tau_kc) plus the collapsed
-    multiple-collision part, unreachable cells (tau = inf) contributing nothing, and the per-block sums of the
-    footprint's emission-weighted heights."""
+def _cone_directions(grid, geometry, angle, K, half_angle_deg, gen):
+    """K unit vectors within ``half_angle_deg`` of the mean primary direction of the grid's voxel centres: a narrow
+    window of eta nodes, as the panel-facing directions of one image chunk give."""
+    pts = grid.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    axis = ofc.primary_directions(pts, np.array(geometry.source_position(angle))).mean(0)
+    axis /= np.linalg.norm(axis)
+    a = np.cross(axis, [0.0, 0.0, 1.0])
+    a /= np.linalg.norm(a)
+    b = np.cross(axis, a)
+    th = np.radians(half_angle_deg) * np.sqrt(torch.rand(K, generator=gen, dtype=torch.float64).numpy())
+    ph = 2 * np.pi * torch.rand(K, generator=gen, dtype=torch.float64).numpy()
+    om = (np.cos(th)[:, None] * axis + np.sin(th)[:, None] * (np.cos(ph)[:, None] * a + np.sin(ph)[:, None] * b))
+    return torch.from_numpy(om).float()
+
+
+def _node_windows(library, grid, geometry, angle, om, K):
+    """Largest eta-node window (lower nodes spanned + 2) over the emission kernel's point blocks and direction tiles,
+    from the FP64 restatement of the table position (uniform nodes)."""
+    pts = grid.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    u = ofc.primary_directions(pts, np.array(geometry.source_position(angle)))
+    n = int(library.mu_s_eta.shape[-1])
+    eta = np.clip(u @ om.double().numpy().T, -1.0, 1.0)
+    i0 = np.clip(np.floor((eta + 1.0) / 2.0 * (n - 1)), 0, n - 2)
+    blk, widest = _hip.EMISSION_BLOCK, 0
+    for b in range(0, len(pts), blk):
+        for t in range(0, K, 64):
+            w = i0[b:b + blk, t:t + 64]
+            widest = max(widest, int(w.max() - w.min()) + 2)
+    return widest
+
+
+@pytest.mark.parametrize("G", [5, 16, 32])
+@pytest.mark.parametrize("spread", ["cone", "wide", "sphere"])
+def test_emission_source_equals_the_first_collision_source(G, spread):
+    """The emission kernel's first-collision source, read off with zero depths and no multiple-collision part, equals
+    citrus_fc_source's bit for bit: narrow direction cones take the node-contracted path (each point's source at the
+    nodes of the window formed once in LDS and interpolated per direction), directions over the sphere the per-pair
+    evaluation; 70 directions make a whole and a partial tile of 64, the 990 points a partial last block."""
+    from citrus_amd.scatter_solver.scatter_image import _attenuated_emission
+    from citrus_amd.scatter_solver.uncollided import first_collision_source_at_directions
+
+    dev = gpu_device()
+    library = seeded_library(G, 3, 120.0, 20.0, seed=29, num_eta=129)
+    grid = Grid((9, 10, 11), 0.5)
+    geometry = ConeGeometry(40.0, 60.0, 12, 14, 0.6, 0.6)
+    angle = 0.7
+    gen = torch.Generator().manual_seed(11)
+    P, K = grid.num_voxels, 70
+    rho = (torch.rand(2, *grid.shape, generator=gen, dtype=torch.float64) * 1.2).float()
+    rho[0, :2] = 0.0  # points without the first material
+    phi = (torch.rand(G, *grid.shape, generator=gen, dtype=torch.float64) * 0.01 + 0.001).float()
+    if spread in ("cone", "wide"):  # wide: windows of 24-27 nodes (12 at 32 groups), near the LDS limit of 30 (15)
+        om = _cone_directions(grid, geometry, angle, K, 6.0 if spread == "cone" else (28.0 if G > 16 else 45.0), gen)
+        widest = _node_windows(library, grid, geometry, angle, om, K)
+        assert widest <= (13 if G > 16 else 27) and (spread == "cone" or widest >= 10)
+    else:
+        om = torch.randn(K, 3, generator=gen, dtype=torch.float64)
+        om = (om / om.norm(dim=1, keepdim=True)).float()
+        assert _node_windows(library, grid, geometry, angle, om, K) > 40
+    d = lambda t: t.to(dev)
+    want = first_collision_source_at_directions(library, d(rho), d(phi), grid, geometry, angle, d(om))
+    out, partial = _attenuated_emission(library, d(rho), d(phi), grid, geometry, angle, d(om),
+                                        d(torch.zeros(K, G, P)), d(torch.zeros(G, P)), d(torch.zeros(K, G, P)), 1,
+                                        None, None)
+    assert torch.equal(out, want)
+    sums = want.reshape(K, G, P).sum(1).double().cpu()
+    nb = -(-P // _hip.EMISSION_BLOCK)
+    ref = torch.stack([sums[:, b * _hip.EMISSION_BLOCK:(b + 1) * _hip.EMISSION_BLOCK].sum(1) for b in range(nb)], 1)
+    assert rel_err(partial[..., 0], ref) < F32_REL
+    assert float(partial[..., 1].abs().max()) == 0.0  # no heights: h = 0
+
+
+@pytest.mark.parametrize("spread", ["sphere", "cone"])
+@pytest.mark.parametrize("G", [5, 8, 16])
+def test_collapsed_emission_with_split_groups_matches_oracle(G, spread):
+    """The view step's collapsed emission against the FP64 restatement: sum_g w_kg S_g(p, w_k) exp(-sum_c mix_gc
+    tau_kc) plus the collapsed multiple-collision part, unreachable cells (tau = inf) contributing nothing, and the
+    per-block sums of the footprint's emission-weighted heights. 8 and 16 groups fill the kernel's group width (the
+    view step's straight-line form), 5 do not (the general form); directions over the sphere and in a cone (the
+    node-contracted path)."""
     from citrus_amd.scatter_solver.scatter_image import _attenuated_emission
 
     dev = gpu_device()
@@ -335,8 +410,11 @@ def test_collapsed_emission_with_split_groups_matches_oracle(G):
     P, K = grid.num_voxels, 7
     rho = (torch.rand(2, *grid.shape, generator=gen, dtype=torch.float64) * 1.2).float()
     phi = (torch.rand(G, *grid.shape, generator=gen, dtype=torch.float64) * 0.01 + 0.001).float()
-    om = torch.randn(K, 3, generator=gen, dtype=torch.float64)
-    om = (om / om.norm(dim=1, keepdim=True)).float()
+    if spread == "cone":  # the node-contracted path
+        om = _cone_directions(grid, geometry, angle, K, 8.0, gen)
+    else:
+        om = torch.randn(K, 3, generator=gen, dtype=torch.float64)
+        om = (om / om.norm(dim=1, keepdim=True)).float()
     tau = torch.rand(K, 2, P, generator=gen) * 2.0
     tau[0, :, :5] = float("inf")
     mix = torch.rand(G, 2, generator=gen)
@@ -359,8 +437,10 @@ def test_collapsed_emission_with_split_groups_matches_oracle(G):
     assert bool(torch.isfinite(out).all()) and float(np.abs(e[0, :, :5]).max()) == 0.0
     nb = int(partial.shape[1])
     sums = np.zeros((K, nb, 2))
+    blk = _hip.EMISSION_BLOCK
+    assert nb == -(-P // blk)
     for b in range(nb):
-        sl = slice(256 * b, min(P, 256 * (b + 1)))
+        sl = slice(blk * b, min(P, blk * (b + 1)))
         sums[:, b, 0] = esum[:, sl].sum(1)
         sums[:, b, 1] = (esum[:, sl] * heights[sl].double().numpy()).sum(1)
     assert rel_err(partial, sums) < F32_REL
