@@ -1,6 +1,6 @@
 // Scattering-source moments T[go][n][k] = sum_gi mu_s_l[gi][go][l(n)] phi[gi][n][k] for every cell (gfx950).
 //
-// A workgroup owns kCells cells of one view. It first forms each cell's transfer matrices mu[l][gi][go] for the
+// A workgroup owns CELLS cells of one view (4 up to 16 groups, else 2). It first forms each cell's transfer matrices mu[l][gi][go] for the
 // degrees l <= L in LDS (from the dense volume, or from the densities and the per-material table), once per cell
 // instead of once per moment; then one thread per (cell, n, k) element keeps phi[.][n][k] of every input group in
 // registers and accumulates all output groups from the matrix of its degree. phi and T are streamed once, coalesced
@@ -10,6 +10,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "citrus_hip.h"
 #include "common.h"
@@ -17,7 +18,6 @@
 namespace citrus {
 namespace {
 
-constexpr int kCells = 2;
 constexpr int kMaxMaterials = 8;
 constexpr int kTableShare = 8;  // table words per thread held in registers while phi loads (4096 at 512 threads)
 
@@ -37,11 +37,13 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
   return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
 }
 
-// LDS: mu [kCells][L+1][GMAX][GMAX] (degree stride padded), then the material table [M][G][G][Lt+1] (factored form).
+// LDS: mu [CELLS][L+1][GMAX][GMAX] (degree stride padded), then the material table [M][G][G][Lt+1] (factored form).
 // MIXED: the result goes out as the mixed-precision operand of the sweep instead of float32 T: degrees l <= 1 to
 // T (float32 [V][G][N][4 comp][4]), degrees l >= 2 to T16 (bfloat16 [V][G][N][4 comp][kh], zero padded).
-template <int GMAX, bool FACTORED, bool MIXED>
-__global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
+// EXACT: G == GMAX and downscatter only (lower_only), the view step's form: every group and triangle bound is a
+// compile-time constant, so the matrix formation and the contraction carry no per-entry tests.
+template <int GMAX, int CELLS, bool FACTORED, bool MIXED, bool EXACT>
+__global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
     const float* __restrict__ transfer, const float* __restrict__ rho, int M, int Lt, const float* __restrict__ phi,
     float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only, float* __restrict__ means,
     uint16_t* __restrict__ T16, int kh) {
@@ -49,23 +51,23 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
   constexpr int DS = degree_stride<GMAX>();
   const int per_cell = (L + 1) * DS + 1;  // odd: the two cells' matrices also fall in different banks
   float* mu = lds;
-  float* table = lds + kCells * per_cell;
+  float* table = lds + CELLS * per_cell;
   const int nthreads = blockDim.x;
   const int tid = threadIdx.x;
-  const int cells_per_view = (N + kCells - 1) / kCells;
+  const int cells_per_view = (N + CELLS - 1) / CELLS;
   // workgroups go to the 8 XCDs round robin; give each XCD a contiguous run of cells instead, so that the 4-byte
   // cell-mean stores of neighbouring cells (4 bytes apart, 16 MB apart across moments) meet in one L2
   const unsigned nb = gridDim.x;
   const unsigned b = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
   const int v = (int)(b / cells_per_view);
-  const int c0 = (int)(b % cells_per_view) * kCells;
+  const int c0 = (int)(b % cells_per_view) * CELLS;
   // this thread's element (cell, n, k) and its phi of every input group, requested first so that the loads are in
   // flight while the transfer matrices are formed
   const int elems = nsh * 4;
   const int cc = tid / elems;
   const int r = tid - cc * elems;
   const int cell = c0 + cc;
-  const bool active = cc < kCells && cell < N;
+  const bool active = cc < CELLS && cell < N;
   const int64_t view_stride = (int64_t)G * N * elems;
   const int64_t group_stride = (int64_t)N * elems;
   const int64_t off = (int64_t)v * view_stride + (int64_t)cell * elems + r;
@@ -104,7 +106,40 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
   }
   // per-cell transfer matrices, entries (l, gi, go) with compile-time strides
   const int entries = (L + 1) * GMAX * GMAX;
-  for (int mc = 0; mc < kCells; ++mc) {
+  if constexpr (EXACT && FACTORED) {
+    // densities of the workgroup's cells once per thread (uniform loads), then every entry as a material sum
+    float rc[CELLS][kMaxMaterials];
+#pragma unroll
+    for (int mc = 0; mc < CELLS; ++mc)
+#pragma unroll
+      for (int m = 0; m < kMaxMaterials; ++m)
+        rc[mc][m] = (m < M && c0 + mc < N) ? rho[(int64_t)m * N + c0 + mc] : 0.0f;
+    for (int e = tid; e < entries; e += nthreads) {
+      const int l = e / (GMAX * GMAX);
+      const int gi = (e / GMAX) % GMAX;
+      const int go = e % GMAX;
+      const float* tm = table + (gi * GMAX + go) * (Lt + 1) + l;
+      float m_val[CELLS];
+#pragma unroll
+      for (int mc = 0; mc < CELLS; ++mc) m_val[mc] = 0.0f;
+      if (gi <= go) {
+        for (int m = 0; m < M; ++m) {
+          const float t = tm[m * GMAX * GMAX * (Lt + 1)];
+#pragma unroll
+          for (int mc = 0; mc < CELLS; ++mc) {
+            // the densities of material m (a register array indexed by a loop variable: select it)
+            float r = rc[mc][0];
+#pragma unroll
+            for (int q = 1; q < kMaxMaterials; ++q) r = m == q ? rc[mc][q] : r;
+            m_val[mc] = fmaf(r, t, m_val[mc]);
+          }
+        }
+      }
+#pragma unroll
+      for (int mc = 0; mc < CELLS; ++mc) mu[mc * per_cell + l * DS + gi * GMAX + go] = m_val[mc];
+    }
+  } else
+  for (int mc = 0; mc < CELLS; ++mc) {
     const int mcell_index = c0 + mc;
     for (int e = tid; e < entries; e += nthreads) {
       const int l = e / (GMAX * GMAX);
@@ -131,11 +166,16 @@ __global__ __launch_bounds__(kCells * 256) void scatter_moments_kernel(
   const float* mcell = mu + cc * per_cell + l * DS;
 #pragma unroll
   for (int go = 0; go < GMAX; ++go) {
-    if (go >= G) break;
+    if (!EXACT && go >= G) break;
     float acc = 0.0f;
 #pragma unroll
-    for (int gi = 0; gi < GMAX; ++gi)
-      if (gi < G && !(lower_only && gi > go)) acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
+    for (int gi = 0; gi < GMAX; ++gi) {
+      if (EXACT) {
+        if (gi <= go) acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
+      } else if (gi < G && !(lower_only && gi > go)) {
+        acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
+      }
+    }
     if (MIXED) {
       const int64_t row = ((((int64_t)v * G + go) * N + cell) * 4 + k);  // (view, group, cell, component)
       if (n < 4) {
@@ -168,26 +208,38 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
   int L = 0;
   if (int rc = check(G, nsh, Lt, N, V, L)) return rc;
   const int gmax = G <= 8 ? 8 : (G <= 16 ? 16 : 32);
+  // 4 cells per workgroup (their transfer matrices and the material table in LDS, one table staging per 4 cells) up
+  // to 16 groups; 2 at 32 groups, whose matrices are 4x larger
+  const int cells = gmax <= 16 ? 4 : 2;
   const size_t per_cell = (size_t)(L + 1) * (gmax * gmax + 1) + 1;
-  const size_t lds = sizeof(float) * ((size_t)kCells * per_cell + (FACTORED ? (size_t)M * G * G * (Lt + 1) : 0));
+  const size_t lds = sizeof(float) * ((size_t)cells * per_cell + (FACTORED ? (size_t)M * G * G * (Lt + 1) : 0));
   if (lds > 64 * 1024) return fail(CITRUS_ERR_UNSUPPORTED, "transfer tables exceed 64 KiB of LDS");
-  const int threads = kCells * nsh * 4;
-  const int64_t blocks = (int64_t)V * ((N + kCells - 1) / kCells);
+  const int threads = cells * nsh * 4;
+  const int64_t blocks = (int64_t)V * ((N + cells - 1) / cells);
   if (!grid_fits(blocks, threads)) return fail(CITRUS_ERR_ARG, "too many cells for one launch (2^32 work-items)");
   const int kh = MIXED ? 16 * ((nsh - 4 + 15) / 16) : 0;
   if (MIXED && (nsh < 16 || kh - (nsh - 4) > nsh - 4))
     return fail(CITRUS_ERR_UNSUPPORTED, "the mixed-precision operand is built for P3 and above");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)blocks), block((unsigned)threads);
+  // the exact form: G fills the kernel's group width and only downscatter enters (the view step's transfer)
+  const bool exact = G == gmax && lower_only != 0;
+  auto go = [&](auto gm) {
+    constexpr int GM = decltype(gm)::value;
+    constexpr int CELLS = GM <= 16 ? 4 : 2;
+    if (exact)
+      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, true>), grid, block, lds, s, transfer,
+                         rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+    else
+      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, false>), grid, block, lds, s, transfer,
+                         rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+  };
   if (G <= 8)
-    hipLaunchKernelGGL((scatter_moments_kernel<8, FACTORED, MIXED>), grid, block, lds, s, transfer, rho, M, Lt, phi,
-                       T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+    go(std::integral_constant<int, 8>{});
   else if (G <= 16)
-    hipLaunchKernelGGL((scatter_moments_kernel<16, FACTORED, MIXED>), grid, block, lds, s, transfer, rho, M, Lt, phi,
-                       T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+    go(std::integral_constant<int, 16>{});
   else
-    hipLaunchKernelGGL((scatter_moments_kernel<32, FACTORED, MIXED>), grid, block, lds, s, transfer, rho, M, Lt, phi,
-                       T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+    go(std::integral_constant<int, 32>{});
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CITRUS_ERR_HIP, std::string("scatter moments: ") + hipGetErrorString(e));
   return CITRUS_OK;
