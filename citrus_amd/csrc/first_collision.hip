@@ -199,6 +199,21 @@ __device__ __forceinline__ void load_slots(const FcDev& d, int p, PointSlots& ps
   }
 }
 
+// The sweep's block order of first-collision cell means (source layout 3 of the persistent sweep): per group
+// [blocks of 4^3 cells][8 octants][64 cells of the block][dp], blocks (row block, column block, slice block) in
+// row-major order, a block's cells (row, column, slice) in row-major order, dp = directions per octant rounded up to
+// 4 (the padding zero), then CITRUS_FC_GROUP_PAD unused floats. One sweep task (block, octant, group) reads one
+// contiguous run of 64 dp floats; the pad skews the groups' and views' regions across HBM channels (unpadded they
+// all start on the same channel, and the persistent sweep runs the same block of 16 domains of one XCD together).
+__host__ __device__ __forceinline__ int64_t fc_sample_group_floats(int R, int C, int S, int dp) {
+  return (int64_t)((R + 3) / 4) * ((C + 3) / 4) * ((S + 3) / 4) * 8 * 64 * dp + CITRUS_FC_GROUP_PAD;
+}
+
+__host__ __device__ __forceinline__ int64_t fc_sample_offset(int R, int C, int S, int dp, int r, int c, int s) {
+  const int64_t blk = ((int64_t)(r >> 2) * ((C + 3) / 4) + (c >> 2)) * ((S + 3) / 4) + (s >> 2);
+  return (blk * 8 * 64 + (((r & 3) * 4 + (c & 3)) * 4 + (s & 3))) * dp;
+}
+
 // S of the four output groups 4 ch .. 4 ch + 3 at one eta, for a wave-uniform channel with the slots in registers:
 // eval_source restricted to one channel (the same terms in the same order, so the values are equal bit for bit), the
 // table rows of four source groups requested together before they are added.
@@ -243,11 +258,13 @@ __device__ __forceinline__ f4 eval_source_channel_batched(const FcDev& d, const 
 // groups over threads keeps the accumulators of a thread at 4 groups, so 4x as many points' gathers are in flight.
 // The directions (unit vector and weight) pass through LDS in tiles of one per thread; the point's material slots stay
 // in registers. Per group the terms and their order are those of the all-group form.
+// d8 > 0: the samples go out in the sweep's block order instead (fc_sample_offset), the K = 8 d8 directions octant
+// by octant in runs of four.
 template <int GMAX>
 __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* __restrict__ weights,
                                                          const float* __restrict__ mu_s_l, int lp1,
                                                          float* __restrict__ out, float* __restrict__ samples,
-                                                         int blocks_per_channel) {
+                                                         int blocks_per_channel, int d8) {
   __shared__ f4 sdir[kBlock];  // (omega x, y, z, weight) of the tile's directions
   constexpr int NCH = GMAX / 4;
   const int ch = NCH - 1 - (int)(blockIdx.x / blocks_per_channel);
@@ -263,11 +280,55 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
   PointSlots ps;
   load_slots(d, pp, ps);
   f4 I0 = {0.f, 0.f, 0.f, 0.f}, I1 = I0, I2 = I0;
+  if (d8 > 0) {
+    // block order: per octant the d8 samples in runs of four (padding of the last run zero), each run one 16-byte
+    // store per group; the sums in direction order as below
+    const int dp = (d8 + 3) & ~3;
+    if ((int)threadIdx.x < d.K)
+      sdir[threadIdx.x] = f4{d.omegas[3 * threadIdx.x], d.omegas[3 * threadIdx.x + 1], d.omegas[3 * threadIdx.x + 2],
+                             weights[threadIdx.x]};
+    __syncthreads();
+    if (active) {
+      const int64_t base = fc_sample_offset(d.R, d.C, d.S, dp, r, c, s);
+      const int64_t gstride = fc_sample_group_floats(d.R, d.C, d.S, dp);
+      for (int o = 0; o < 8; ++o) {
+        for (int d0 = 0; d0 < dp; d0 += 4) {
+          f4 run[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int dd = d0 + j;
+            if (dd >= d8) {
+#pragma unroll
+              for (int gl = 0; gl < 4; ++gl) run[gl][j] = 0.0f;
+              continue;
+            }
+            const f4 dir = sdir[o * d8 + dd];
+            const float eta = direction_cosine(ux, uy, uz, dir.x, dir.y, dir.z);
+            const f4 S = eval_source_channel_batched(d, pd, ps, eta, ch);
+            const float w = dir.w;
+            const float we = w * eta;
+            const float wee = we * eta;
+#pragma unroll
+            for (int gl = 0; gl < 4; ++gl) {
+              I0[gl] = fmaf(w, S[gl], I0[gl]);
+              I1[gl] = fmaf(we, S[gl], I1[gl]);
+              I2[gl] = fmaf(wee, S[gl], I2[gl]);
+              run[gl][j] = S[gl];
+            }
+          }
+#pragma unroll
+          for (int gl = 0; gl < 4; ++gl)
+            if (4 * ch + gl < d.G)
+              *reinterpret_cast<f4*>(samples + (4 * ch + gl) * gstride + base + (int64_t)o * 64 * dp + d0) = run[gl];
+        }
+      }
+    }
+  }
   // directions in runs of eight: with samples and K % 8 == 0 each group's eight samples leave as two adjacent 16-byte
   // stores, one whole 32-byte sector (runs of four leave half sectors, which reach memory as whole ones)
   constexpr int RUN = 8;
   const bool vec = samples && (d.K % RUN) == 0;
-  for (int t0 = 0; t0 < d.K; t0 += kBlock) {
+  for (int t0 = 0; t0 < (d8 > 0 ? 0 : d.K); t0 += kBlock) {
     const int kt = min(kBlock, d.K - t0);
     __syncthreads();  // the previous tile is no longer read
     if ((int)threadIdx.x < kt) {
@@ -814,10 +875,32 @@ extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, co
   dispatch_groups(d.G, [&](auto gm) {
     constexpr int NCH = decltype(gm)::value / 4;
     hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0, s, d,
-                       weights, mu_s_l, lp1, out, samples, blocks);
+                       weights, mu_s_l, lp1, out, samples, blocks, 0);
     return 0;
   });
   return finish("first-collision sums");
+}
+
+extern "C" int citrus_fc_sums_blocked(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1,
+                                      float* out, float* samples, int D8, void* stream) {
+  using namespace citrus;
+  if (int rc = check_args(args)) return rc;
+  if (!weights || !mu_s_l || !out || !samples) return fail(CITRUS_ERR_ARG, "NULL weights, moments, output or samples");
+  if (lp1 < 2) return fail(CITRUS_ERR_ARG, "the exact moments need Legendre order >= 1");
+  if (D8 < 1 || args->K != 8 * D8 || args->K > kBlock)
+    return fail(CITRUS_ERR_ARG, "the block order needs K = 8 D8 sweep directions (octant-major), at most 256");
+  const FcDev d = make_dev(*args);
+  const int blocks = (d.P + kBlock - 1) / kBlock;
+  if (!grid_fits((int64_t)blocks * (group_pad(d.G) / 4), kBlock))
+    return fail(CITRUS_ERR_ARG, "points x group channels exceed 2^32 work-items per launch");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dispatch_groups(d.G, [&](auto gm) {
+    constexpr int NCH = decltype(gm)::value / 4;
+    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0, s, d,
+                       weights, mu_s_l, lp1, out, samples, blocks, D8);
+    return 0;
+  });
+  return finish("first-collision sums (block order)");
 }
 
 extern "C" int citrus_fc_correct(const CitrusFcArgs* args, const float* scale, const float* beta, float* samples,

@@ -140,11 +140,12 @@ struct MfmaTask {
                           // degrees l >= 2 [N][4][KH]
   const float* Tlo;       // mixed operand: the float32 degrees l <= 1, [N][4][4] (component-major), else unused
   float* phi;             // fluence moments of the (view, group) [N][NSH][4]
-  const float* src;       // explicit source of the (view, group): [N][M][4] (SRC 1) or [N][M] (SRC 2, 3), or NULL
+  const float* src;       // explicit source of the (view, group): [N][M][4] (SRC 1), [N][M] (SRC 2), the block-order
+                          // cell means [blocks][8][64][dp] (SRC 3), or NULL
   const float* cdir;      // [8 D8][3] c_a = |Omega_a| / Delta_a on the view's grid
-  const float* fc_scale;  // SRC 3: [N] scale of the first-collision sampling correction of the (view, group)
-  const float* fc_beta;   // SRC 3: [N] its beta
-  const float* fc_u;      // SRC 3: [N][4] Omega_UC of each cell of the view
+  const float* fc_scale;  // SRC 3: [blocks][64] scale of the first-collision sampling correction of the (view, group)
+  const float* fc_beta;   // SRC 3: [blocks][64] its beta
+  const float* fc_u;      // SRC 3: [blocks][64][4] Omega_UC of each cell of the view
 };
 
 // Direction tables and chunk shared by every task of a sweep.
@@ -157,9 +158,11 @@ struct DirTables {
 
 
 // SRC: 0 no explicit source; 1 group-major LD moments (layout 1, read per plane); 2 group-major cell means (layout 2,
-// prefetched one plane ahead with T, the slopes zero); 3 the cell means of layout 2 before the first-collision
-// sampling correction, applied here as they are read: mean scale (1 + beta eta) with eta = u[cell] . omega[m]
-// (direction_cosine), the operations of the correction kernels. COH: the faces and moments other tasks of the same launch
+// prefetched one plane ahead with T, the slopes zero); 3 the cell means before the first-collision sampling
+// correction, stored in block order (per block and octant the 64 cells' dp = D8 rounded up to 4 directions, one
+// contiguous run per task; the correction factors and Omega_UC per block of 64 cells), the correction applied here as
+// they are read: mean scale (1 + beta eta) with eta = u[cell] . omega[m] (direction_cosine), the operations of the
+// correction kernels. COH: the faces and moments other tasks of the same launch
 // produce or consume are accessed at device scope (the persistent sweep); the launch-sequence sweep exchanges them
 // across kernel boundaries only.
 template <int NSH, int DC4, int SRC, bool BF16, bool COH = false>
@@ -209,16 +212,19 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
   const __amdgpu_buffer_rsrc_t phi_r = __builtin_amdgcn_make_buffer_rsrc(a.phi, 0, a.N * (int)CB, kBufferWord3);
   // explicit source, group-major [N][M][4] (layout 1: the 16 directions of one cell in a D tile are 256 contiguous
   // bytes) or cell means [N][M] (layout 2, zero slopes: 64 contiguous bytes)
-  constexpr int src_words = (SRC == 2 || SRC == 3) ? 1 : 4;
+  constexpr int src_words = SRC == 2 ? 1 : 4;
+  const int nblocks = a.nbr * a.nbc * a.nbs;
+  const int dp = (dt.D8 + 3) & ~3;  // SRC 3: stored directions per octant
   const __amdgpu_buffer_rsrc_t src_r = __builtin_amdgcn_make_buffer_rsrc(
-      HAS_SRC ? const_cast<float*>(a.src) : a.phi, 0, HAS_SRC ? a.N * dt.M * 4 * src_words : 0, kBufferWord3);
-  // SRC 3: the per-cell factors of the first-collision sampling correction
+      HAS_SRC ? const_cast<float*>(a.src) : a.phi, 0,
+      SRC == 3 ? nblocks * 8 * 64 * dp * 4 : (HAS_SRC ? a.N * dt.M * 4 * src_words : 0), kBufferWord3);
+  // SRC 3: the per-cell factors of the first-collision sampling correction, in block order
   const __amdgpu_buffer_rsrc_t fs_r = __builtin_amdgcn_make_buffer_rsrc(
-      SRC == 3 ? const_cast<float*>(a.fc_scale) : a.phi, 0, SRC == 3 ? a.N * 4 : 0, kBufferWord3);
+      SRC == 3 ? const_cast<float*>(a.fc_scale) : a.phi, 0, SRC == 3 ? nblocks * 64 * 4 : 0, kBufferWord3);
   const __amdgpu_buffer_rsrc_t fb_r = __builtin_amdgcn_make_buffer_rsrc(
-      SRC == 3 ? const_cast<float*>(a.fc_beta) : a.phi, 0, SRC == 3 ? a.N * 4 : 0, kBufferWord3);
+      SRC == 3 ? const_cast<float*>(a.fc_beta) : a.phi, 0, SRC == 3 ? nblocks * 64 * 4 : 0, kBufferWord3);
   const __amdgpu_buffer_rsrc_t fu_r = __builtin_amdgcn_make_buffer_rsrc(
-      SRC == 3 ? const_cast<float*>(a.fc_u) : a.phi, 0, SRC == 3 ? a.N * 16 : 0, kBufferWord3);
+      SRC == 3 ? const_cast<float*>(a.fc_u) : a.phi, 0, SRC == 3 ? nblocks * 64 * 16 : 0, kBufferWord3);
 
   // ---- slot tables, incoming faces (all requested before the first is used), direction fragments ------------------
   {
@@ -320,8 +326,11 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
     s4 tb[3][KB];
     float mu[3];
     float sm[3][MEANS ? NT : 1];  // cell-mean source of (cell 4 mt + l/16, direction 16 nt + l%16)
-    float fs[3], fb[3];  // SRC 3: scale and beta of the cell of (4 mt + l/16)
-    f4 fu[3];            // SRC 3: Omega_UC of that cell
+  };
+  // SRC 3: the cell of (4 mt + l/16) in block order, (row, column, slice) row-major within the block
+  auto block_cell = [&](const int pb, const int P, const int sd) __attribute__((always_inline)) -> unsigned {
+    const int pk = slot_pos[pb + (sd < P ? sd : P - 1)];
+    return (unsigned)blk * 64u + (unsigned)(((((pk >> 8) & 0xff) * 4) + (pk & 0xff)) * 4 + ((pk >> 16) & 0xff));
   };
   auto load_plane = [&](const int pl) __attribute__((always_inline)) -> Staged {
     Staged st;
@@ -351,18 +360,16 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
       const int sd = 4 * mt + q;
       const int cd = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
       st.mu[mt] = buf_load(mu_r, cd >= 0 ? 4u * cd : kDrop);
+      const unsigned bcell = SRC == 3 ? block_cell(pb, P, sd) : 0u;
       if (MEANS) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const int d = nt * 16 + r16;
-          const unsigned e = (unsigned)cd * dt.M + (unsigned)(mbase + d);
+          const unsigned e = SRC == 3 ? ((unsigned)blk * 8u * 64u + (unsigned)o * 64u + (bcell - (unsigned)blk * 64u)) *
+                                            (unsigned)dp + (unsigned)(dt.d0 + d)
+                                      : (unsigned)cd * dt.M + (unsigned)(mbase + d);
           st.sm[mt][nt] = buf_load(src_r, (cd >= 0 && d < dc) ? e * 4u : kDrop);
         }
-      }
-      if (SRC == 3) {
-        st.fs[mt] = buf_load(fs_r, cd >= 0 ? 4u * cd : kDrop);
-        st.fb[mt] = buf_load(fb_r, cd >= 0 ? 4u * cd : kDrop);
-        st.fu[mt] = buf_load4(fu_r, cd >= 0 ? 16u * cd : kDrop);
       }
     }
     return st;
@@ -387,25 +394,28 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
         pold[mt][t] = buf_load4_c<COH>(phi_r, use ? (unsigned)cells[mt] * CB + 16u * n : kDrop);
       }
     }
+    // SRC 3: this plane's correction factors (scale, beta, Omega_UC) of the cell of (4 mt + l/16), requested here and
+    // used after the plane's expansion (held for one plane only: prefetching them a plane ahead with the means doubles
+    // their registers)
+    float fs[3], fb[3];
+    f4 fu[3];
+    if (SRC == 3) {
+#pragma unroll
+      for (int mt = 0; mt < 3; ++mt) {
+        if (mt * 4 >= P) break;
+        const unsigned bcell = block_cell(pb, P, 4 * mt + q);
+        const bool on = cells[mt] >= 0;
+        fs[mt] = buf_load(fs_r, on ? 4u * bcell : kDrop);
+        fb[mt] = buf_load(fb_r, on ? 4u * bcell : kDrop);
+        fu[mt] = buf_load4(fu_r, on ? 16u * bcell : kDrop);
+      }
+    }
     f4 srcv[3][NT];  // explicit source of (cell 4 mt + l/16, direction 16 nt + l%16)
     if (SRC == 2) {
 #pragma unroll
       for (int mt = 0; mt < 3; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) srcv[mt][nt] = f4{cur.sm[mt][nt], 0.f, 0.f, 0.f};
-    } else if (SRC == 3) {
-      // the products stay rounded as the correction kernels round them: not fused into the source's later addition
-#pragma clang fp contract(off)
-#pragma unroll
-      for (int mt = 0; mt < 3; ++mt) {
-        const f4 u = cur.fu[mt];
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const f4 om = fomg[nt];
-          const float eta = direction_cosine(u.x, u.y, u.z, om.x, om.y, om.z);
-          srcv[mt][nt] = f4{cur.sm[mt][nt] * cur.fs[mt] * fmaf(cur.fb[mt], eta, 1.0f), 0.f, 0.f, 0.f};
-        }
-      }
     } else if (SRC == 1) {
 #pragma unroll
       for (int mt = 0; mt < 3; ++mt) {
@@ -432,6 +442,18 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
         if (BF16) {
 #pragma unroll
           for (int s = 0; s < KB; ++s) sacc[nt] = mfma_bf16(cur.tb[mt][s], yb[nt][s], sacc[nt]);
+        }
+      }
+      if (SRC == 3) {
+        // the corrected cell means; the products stay rounded as the correction kernels round them, not fused into
+        // the source's later addition
+#pragma clang fp contract(off)
+        const f4 u = fu[mt];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const f4 om = fomg[nt];
+          const float eta = direction_cosine(u.x, u.y, u.z, om.x, om.y, om.z);
+          srcv[mt][nt] = f4{cur.sm[mt][nt] * fs[mt] * fmaf(fb[mt], eta, 1.0f), 0.f, 0.f, 0.f};
         }
       }
       // (2) closed-form LD solve of (cell 4 mt + l/16, direction 16 nt + l%16)

@@ -49,12 +49,12 @@ struct PersistentView {
   const float* mu;    // [G][N]
   const void* T;      // [G][N][NSH][4] float32, or the mixed operand's bfloat16 degrees l >= 2 [G][N][4][KH]
   const float* Tlo;   // the mixed operand's float32 degrees l <= 1 [G][N][4][4]
-  const float* src;   // [G][N][M] or [G][N][M][4], or NULL
+  const float* src;   // [G][N][M] or [G][N][M][4], source layout 3 [G][blocks * 8 * 64 * dp + pad] (block order), or NULL
   float* phi;         // [G][N][NSH][4]
   const float* cdir;  // [8 D8][3]
-  const float* fc_scale;  // source layout 3: [G][N] scale and beta of the first-collision sampling correction
+  const float* fc_scale;  // source layout 3: [G][blocks * 64 + pad] scale and beta of the first-collision correction
   const float* fc_beta;
-  const float* fc_u;      // source layout 3: [N][4] Omega_UC of each cell
+  const float* fc_u;      // source layout 3: [blocks][64][4] Omega_UC of each cell
 };
 
 struct PersistentArgs {
@@ -202,10 +202,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       t.Tlo = nullptr;
     }
     t.phi = pv.phi + (int64_t)g * pv.N * NSH * 4;
-    t.src = SRC != 0 ? pv.src + (int64_t)g * pv.N * a.M * (SRC == 1 ? 4 : 1) : nullptr;
+    // SRC 3: per group [blocks][8][64][dp] cell means and [blocks][64] factors (block order), each group's region
+    // followed by CITRUS_FC_GROUP_PAD floats
+    const int64_t fc_cells = (int64_t)nb * 64;
+    t.src = SRC == 0   ? nullptr
+            : SRC == 3 ? pv.src + (int64_t)g * (fc_cells * 8 * ((a.D8 + 3) & ~3) + CITRUS_FC_GROUP_PAD)
+                       : pv.src + (int64_t)g * pv.N * a.M * (SRC == 1 ? 4 : 1);
     t.cdir = pv.cdir;
-    t.fc_scale = SRC == 3 ? pv.fc_scale + (int64_t)g * pv.N : nullptr;
-    t.fc_beta = SRC == 3 ? pv.fc_beta + (int64_t)g * pv.N : nullptr;
+    t.fc_scale = SRC == 3 ? pv.fc_scale + (int64_t)g * (fc_cells + CITRUS_FC_GROUP_PAD) : nullptr;
+    t.fc_beta = SRC == 3 ? pv.fc_beta + (int64_t)g * (fc_cells + CITRUS_FC_GROUP_PAD) : nullptr;
     t.fc_u = SRC == 3 ? pv.fc_u : nullptr;
     // the lane index is re-read per task so that nothing derived from it is hoisted out of the task loop and kept
     // live across tasks (the task body needs every register it has)

@@ -13,6 +13,7 @@ import os
 import weakref
 from typing import Optional
 
+import numpy as np
 import torch
 from torch import Tensor
 
@@ -178,10 +179,11 @@ class PersistentSchedule:
         """Enqueue one chunk sweep of every view on torch's current stream. ``views`` holds one dict per view of the
         plan with the device tensors ``mu`` [G, N], ``phi`` [G, N, nsh, 4], ``cdir`` [8 D8, 3], and either ``T``
         [G, N, nsh, 4] or the mixed operand ``T16`` [G, N, 4, KH] with ``T_low`` [G, N, 4, 4], and optionally
-        ``source`` (group-major, the same ``source_layout`` 1, 2 or 3 for every view; -1 without). Layout 3 is layout
-        2's cell means before the first-collision sampling correction, which the sweep applies: each view then also
-        holds ``fc_scale`` and ``fc_beta`` [G, N] and ``fc_u`` [N, 4], and ``fc_omega`` [8 D8, 4] gives the
-        directions. A sweep that aborts ORs its status code into the device's status word
+        ``source`` (group-major, the same ``source_layout`` 1, 2 or 3 for every view; -1 without). Layout 3 holds the
+        cell means before the first-collision sampling correction, which the sweep applies, in the sweep's block
+        order (``uncollided.to_block_order``): ``source`` [G, blocks * 8 * 64 * dp + pad] (dp = D8 rounded up to 4,
+        pad = ``FC_GROUP_PAD``), and ``fc_scale`` and ``fc_beta`` [G, blocks * 64 + pad] and ``fc_u`` [blocks * 64, 4]
+        in each view, ``fc_omega`` [8 D8, 4] the directions. A sweep that aborts ORs its status code into the device's status word
         (``persistent_status_sink``), which the callers' next host read reports (``decode_status``); ``spin_limit``
         > 0 replaces the default bound of each dependency wait."""
         if len(views) != len(self.shapes):
@@ -221,13 +223,17 @@ class PersistentSchedule:
                     raise ValueError(f"views[{v}]: T size does not match [G, N, nsh, 4]")
             if (src is None) != (source_layout == -1):
                 raise ValueError(f"views[{v}]: source does not match source_layout {source_layout}")
+            nb = int(np.prod([-(-int(n) // 4) for n in shape]))  # blocks of the block order (layout 3)
             if src is not None:
                 _hip.require_gpu_tensor(src, f"views[{v}].source")
-                if src.numel() != G * N * M * (4 if source_layout == 1 else 1):
+                want = G * (nb * 8 * 64 * (-(-D8 // 4) * 4) + _hip.FC_GROUP_PAD) if source_layout == 3 else \
+                    G * N * M * (4 if source_layout == 1 else 1)
+                if src.numel() != want or not src.is_contiguous():
                     raise ValueError(f"views[{v}]: source size does not match layout {source_layout}")
             fc = [spec.get(k) for k in ("fc_scale", "fc_beta", "fc_u")]
             if source_layout == 3:
-                for name, t, n in zip(("fc_scale", "fc_beta", "fc_u"), fc, (G * N, G * N, 4 * N)):
+                cells = G * (nb * 64 + _hip.FC_GROUP_PAD)
+                for name, t, n in zip(("fc_scale", "fc_beta", "fc_u"), fc, (cells, cells, 4 * nb * 64)):
                     if t is None:
                         raise ValueError(f"views[{v}]: source_layout 3 needs {name}")
                     _hip.require_gpu_tensor(t, f"views[{v}].{name}")

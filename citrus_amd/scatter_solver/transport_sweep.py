@@ -22,6 +22,7 @@ from typing import Callable, Literal, Optional, Union
 import torch
 from torch import Tensor
 
+from .. import _hip
 from .angular_quadrature import Quadrature, SphericalHarmonicBasis, spherical_harmonic_basis
 from .geometry import Grid
 from .kernels import sweep_kernels
@@ -362,7 +363,7 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     deferred = source if _is_deferred(source) else None
     if deferred is not None:
         _check_deferred(deferred, V, G, (R, C, S), sweep_context.num_directions)
-        source = deferred.samples
+        source = None  # block order: the persistent sweep reads it as it is, the launch sequence after apply_()
     if source is not None:
         M = sweep_context.num_directions
         base = (V, G, R, C, S, M)
@@ -398,7 +399,7 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         return _sweep_grids_persistent([(sweep_context, mu_t, moments, deferred if deferred is not None else source,
                                          out)], direction_chunk, contraction)[0]
     if deferred is not None:
-        source = deferred.apply_()
+        source, layout = deferred.apply_(), 2
     D8 = sweep_context.directions_per_octant
     if direction_chunk == "auto":  # measured once per configuration (autotune.SweepTuner)
         from .autotune import default_tuner
@@ -453,9 +454,15 @@ def _is_deferred(source) -> bool:
 
 
 def _check_deferred(src, V: int, G: int, spatial: tuple, M: int) -> None:
-    N = spatial[0] * spatial[1] * spatial[2]
-    want = {"samples": (V, G) + tuple(spatial) + (M,), "scale": (V, G, N), "beta": (V, G, N), "u": (V, N, 4),
-            "omega": (M, 4)}
+    from .uncollided import fc_blocks
+
+    if tuple(src.shape) != tuple(spatial):
+        raise ValueError(f"deferred first-collision source of grid {tuple(src.shape)}, expected {tuple(spatial)}")
+    if M % 8:
+        raise ValueError(f"the sweep's directions come in 8 octants, got {M}")
+    nb, dp, pad = fc_blocks(spatial), -(-(M // 8) // 4) * 4, _hip.FC_GROUP_PAD
+    want = {"samples": (V, G, nb * 8 * 64 * dp + pad), "scale": (V, G, nb * 64 + pad),
+            "beta": (V, G, nb * 64 + pad), "u": (V, nb * 64, 4), "omega": (M, 4)}
     for name, shape in want.items():
         t = getattr(src, name)
         if tuple(t.shape) != shape or not t.is_contiguous() or t.dtype != torch.float32:
@@ -493,9 +500,9 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
         if _is_deferred(source):
             _check_deferred(source, V, G, (R, C, S), M)
             if source.applied:
-                source = source.samples
+                source = source.corrected
             else:
-                fc, source = source, source.samples
+                fc, source = source, None
         if source is not None:
             base = (V, G, R, C, S, M)
             layout = 1 if tuple(source.shape) == base + (4,) else 2 if tuple(source.shape) == base else 0
@@ -503,8 +510,9 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
                 raise ValueError(f"source must be {base + (4,)} or {base}, got {tuple(source.shape)}")
             if not source.is_contiguous() or source.dtype != torch.float32:
                 raise ValueError("source must be a contiguous float32 tensor")
-        if fc is not None:
+        if fc is not None:  # block order, checked above
             layout = 3
+            source = fc.samples
             if omega is None:
                 omega = fc.omega
             elif not torch.equal(omega, fc.omega):

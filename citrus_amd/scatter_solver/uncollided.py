@@ -246,9 +246,11 @@ class FirstCollisionSamplingCorrection:
 
 
 def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights,
-                 samples: Optional[Tensor] = None) -> Tensor:
+                 samples: Optional[Tensor] = None, blocked_d8: int = 0) -> Tensor:
     """[5, G, R, C, S] float32 I0, I1, I2, E0, E1 from one launch of ``citrus_fc_sums`` (which also stores the
-    uncorrected samples [G, R, C, S, K] into ``samples`` when given)."""
+    uncorrected samples [G, R, C, S, K] into ``samples`` when given), or with ``blocked_d8`` (K = 8 blocked_d8
+    octant-major sweep directions) of ``citrus_fc_sums_blocked``, the samples [G, blocks 8 64 dp + pad] in the
+    sweep's block order (each group's region followed by ``FC_GROUP_PAD`` unused floats)."""
     if library.legendre_order < 1:
         raise ValueError("the sampling correction needs a library of Legendre order >= 1")
     a, keep, mu_s_l = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
@@ -257,14 +259,24 @@ def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, we
         raise ValueError("weights and omegas disagree in length")
     out = torch.empty((5, library.num_groups) + tuple(grid.shape), device=phi_uncollided.device, dtype=torch.float32)
     if samples is not None:
-        want = (library.num_groups,) + tuple(grid.shape) + (a.K,)
+        if blocked_d8:
+            want = (library.num_groups, fc_blocks(grid.shape) * 8 * 64 * (-(-blocked_d8 // 4) * 4) + _hip.FC_GROUP_PAD)
+        else:
+            want = (library.num_groups,) + tuple(grid.shape) + (a.K,)
         if (tuple(samples.shape) != want or samples.dtype != torch.float32 or not samples.is_contiguous()
                 or samples.device != phi_uncollided.device):
             raise ValueError(f"samples must be a contiguous float32 {want} tensor on {phi_uncollided.device}")
     with torch.cuda.device(phi_uncollided.device):
-        _hip.check(_hip.library().citrus_fc_sums(ctypes.byref(a), w.data_ptr(), mu_s_l.data_ptr(),
-                                                 int(mu_s_l.shape[-1]), out.data_ptr(), _hip.ptr(samples),
-                                                 _hip.stream_of(phi_uncollided.device)), "citrus_fc_sums")
+        if blocked_d8:
+            if samples is None:
+                raise ValueError("the block order needs a samples tensor")
+            _hip.check(_hip.library().citrus_fc_sums_blocked(
+                ctypes.byref(a), w.data_ptr(), mu_s_l.data_ptr(), int(mu_s_l.shape[-1]), out.data_ptr(),
+                samples.data_ptr(), int(blocked_d8), _hip.stream_of(phi_uncollided.device)), "citrus_fc_sums_blocked")
+        else:
+            _hip.check(_hip.library().citrus_fc_sums(ctypes.byref(a), w.data_ptr(), mu_s_l.data_ptr(),
+                                                     int(mu_s_l.shape[-1]), out.data_ptr(), _hip.ptr(samples),
+                                                     _hip.stream_of(phi_uncollided.device)), "citrus_fc_sums")
     del keep
     return out
 
@@ -338,75 +350,145 @@ def first_collision_corrected_samples(library: MultigroupLibrary, rho: Tensor, p
     return out, correction
 
 
+def fc_blocks(shape) -> int:
+    """Blocks of 4^3 cells of the sweep's block order over a (rows, cols, slices) grid (partial blocks at the edges)."""
+    return int(np.prod([-(-int(n) // 4) for n in shape]))
+
+
+def to_block_order(x: Tensor, shape) -> Tensor:
+    """[*lead, R, C, S, *tail] -> [*lead, blocks, 64, *tail] in the sweep's block order: blocks of 4^3 cells (row
+    block, column block, slice block) row-major, the cells of a block (row, column, slice) row-major; cells past the
+    grid (partial blocks) zero. The spatial dimensions are the first run of ``shape`` in ``x.shape``."""
+    R, C, S = (int(n) for n in shape)
+    dims = list(x.shape)
+    s0 = next((i for i in range(len(dims) - 2) if dims[i:i + 3] == [R, C, S]), None)
+    if s0 is None:
+        raise ValueError(f"no spatial dimensions {(R, C, S)} in {tuple(x.shape)}")
+    lead, tail = tuple(dims[:s0]), tuple(dims[s0 + 3:])
+    Rp, Cp, Sp = -(-R // 4) * 4, -(-C // 4) * 4, -(-S // 4) * 4
+    if (Rp, Cp, Sp) != (R, C, S):
+        pad = torch.zeros(lead + (Rp, Cp, Sp) + tail, device=x.device, dtype=x.dtype)
+        pad[(slice(None),) * s0 + (slice(0, R), slice(0, C), slice(0, S))] = x
+        x = pad
+    y = x.reshape(lead + (Rp // 4, 4, Cp // 4, 4, Sp // 4, 4) + tail)
+    nl = len(lead)
+    perm = list(range(nl)) + [nl + 0, nl + 2, nl + 4, nl + 1, nl + 3, nl + 5] + [nl + 6 + i for i in range(len(tail))]
+    return y.permute(perm).reshape(lead + ((Rp // 4) * (Cp // 4) * (Sp // 4), 64) + tail)
+
+
+def from_block_order(x: Tensor, shape, lead_dims: int) -> Tensor:
+    """Inverse of ``to_block_order``: [*lead, blocks, 64, *tail] -> [*lead, R, C, S, *tail]."""
+    R, C, S = (int(n) for n in shape)
+    nbr, nbc, nbs = -(-R // 4), -(-C // 4), -(-S // 4)
+    lead, tail = tuple(x.shape[:lead_dims]), tuple(x.shape[lead_dims + 2:])
+    nl = lead_dims
+    y = x.reshape(lead + (nbr, nbc, nbs, 4, 4, 4) + tail)
+    perm = list(range(nl)) + [nl + 0, nl + 3, nl + 1, nl + 4, nl + 2, nl + 5] + [nl + 6 + i for i in range(len(tail))]
+    y = y.permute(perm).reshape(lead + (nbr * 4, nbc * 4, nbs * 4) + tail)
+    return y[(slice(None),) * nl + (slice(0, R), slice(0, C), slice(0, S))]
+
+
 @dataclass(eq=False)
 class DeferredFirstCollision:
     """The first-collision cell means of V views at every sweep direction (one quadrature point per cell), stored
-    before their sampling correction, with what the correction needs: the persistent sweep applies it as it reads
-    them (source layout 3), so the corrected source is never written; ``apply_`` applies it in place for any other
-    consumer. Either way each value is ``mean scale (1 + beta eta)`` with the operations of ``citrus_fc_correct``
-    (uncollided.py:420-433), so the two are equal bit for bit.
+    before their sampling correction in the sweep's block order, with what the correction needs: the persistent sweep
+    applies it as it reads them (source layout 3), so the corrected source is never written; ``apply_`` forms the
+    corrected group-major means for any other consumer. Either way each value is ``mean scale (1 + beta eta)`` with
+    the operations of ``citrus_fc_correct`` (uncollided.py:420-433), so the two are equal bit for bit.
 
-    Attributes: samples [V, G, R, C, S, M] (reordered sweep directions), scale and beta [V, G, N] float32, u [V, N, 4]
-    the primary direction of each cell (``citrus_fc_primary_directions``), omega [M, 4] the directions; ``applied``
-    once the samples hold the corrected values."""
+    Block order (``to_block_order``): blocks of 4^3 cells, per block and octant the 64 cells' D8 directions padded to
+    dp = D8 rounded up to 4, so that one sweep task (block, octant, group) reads one contiguous run; each group's
+    region is followed by ``FC_GROUP_PAD`` unused floats, which skew the groups' and views' regions across HBM
+    channels.
 
+    Attributes: shape (R, C, S); samples [V, G, blocks * 8 * 64 * dp + pad] (``block_samples`` views the blocks);
+    scale and beta [V, G, blocks * 64 + pad] float32; u [V, blocks * 64, 4] the primary direction of each cell
+    (``citrus_fc_primary_directions``); omega [M, 4] the reordered sweep directions; ``applied`` once ``apply_`` has
+    formed the corrected means (kept in ``corrected``)."""
+
+    shape: tuple
     samples: Tensor
     scale: Tensor
     beta: Tensor
     u: Tensor
     omega: Tensor
     applied: bool = False
+    corrected: Optional[Tensor] = None
 
     @staticmethod
     def empty(V: int, G: int, shape, M: int, device) -> "DeferredFirstCollision":
-        N = int(np.prod(shape))
+        if M % 8:
+            raise ValueError(f"the sweep's directions come in 8 octants, got {M}")
+        nb, dp, pad = fc_blocks(shape), -(-(M // 8) // 4) * 4, _hip.FC_GROUP_PAD
         f = dict(device=device, dtype=torch.float32)
-        return DeferredFirstCollision(samples=torch.empty((V, G) + tuple(shape) + (M,), **f),
-                                      scale=torch.empty(V, G, N, **f), beta=torch.empty(V, G, N, **f),
-                                      u=torch.empty(V, N, 4, **f), omega=torch.empty(M, 4, **f))
+        return DeferredFirstCollision(shape=tuple(int(n) for n in shape),
+                                      samples=torch.empty((V, G, nb * 8 * 64 * dp + pad), **f),
+                                      scale=torch.zeros(V, G, nb * 64 + pad, **f),
+                                      beta=torch.zeros(V, G, nb * 64 + pad, **f),
+                                      u=torch.zeros(V, nb * 64, 4, **f), omega=torch.empty(M, 4, **f))
+
+    def block_samples(self) -> Tensor:
+        """The samples as [V, G, blocks, 8, 64, dp] (a view without the pads)."""
+        V, G = int(self.samples.shape[0]), int(self.samples.shape[1])
+        nb, dp = fc_blocks(self.shape), -(-(int(self.omega.shape[0]) // 8) // 4) * 4
+        return self.samples[..., :nb * 8 * 64 * dp].reshape(V, G, nb, 8, 64, dp)
 
     def apply_(self) -> Tensor:
-        """Apply the correction to ``samples`` in place (once) and return them: the group-major cell means
-        [V, G, R, C, S, M] of the corrected source."""
+        """The group-major cell means [V, G, R, C, S, M] of the corrected source (formed once): the block order
+        undone, then the correction of ``citrus_fc_correct_means``."""
         if not self.applied:
             V, G = int(self.samples.shape[0]), int(self.samples.shape[1])
-            M = int(self.samples.shape[-1])
-            N = int(self.samples[0, 0].numel()) // M
-            with torch.cuda.device(self.samples.device):
+            M = int(self.omega.shape[0])
+            D8 = M // 8
+            R, C, S = self.shape
+            N = R * C * S
+            nb = fc_blocks(self.shape)
+            means = from_block_order(self.block_samples()[..., :D8].permute(0, 1, 2, 4, 3, 5), self.shape, 2)
+            means = means.reshape(V, G, R, C, S, M).contiguous()
+            scale = from_block_order(self.scale[..., :nb * 64].reshape(V, G, nb, 64), self.shape, 2).contiguous()
+            beta = from_block_order(self.beta[..., :nb * 64].reshape(V, G, nb, 64), self.shape, 2).contiguous()
+            u = from_block_order(self.u.reshape(V, -1, 64, 4), self.shape, 1).contiguous()
+            with torch.cuda.device(means.device):
                 _hip.check(_hip.library().citrus_fc_correct_means(
-                    self.samples.data_ptr(), self.scale.data_ptr(), self.beta.data_ptr(), self.u.data_ptr(),
-                    self.omega.data_ptr(), V, G, N, M, _hip.stream_of(self.samples.device)),
-                    "citrus_fc_correct_means")
+                    means.data_ptr(), scale.data_ptr(), beta.data_ptr(), u.data_ptr(), self.omega.data_ptr(), V, G,
+                    N, M, _hip.stream_of(means.device)), "citrus_fc_correct_means")
+            self.corrected = means
             self.applied = True
-        return self.samples
+        return self.corrected
 
 
 def first_collision_deferred_samples(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
                                      geometry: ConeGeometry, angle: float, omegas: Tensor, weights: Tensor,
                                      out: DeferredFirstCollision, view: int) -> FirstCollisionSamplingCorrection:
     """``first_collision_corrected_samples`` of view ``view`` of ``out`` without applying the correction: the
-    uncorrected samples, the factors and the cells' primary directions go to ``out`` (one point per cell: ``grid``
-    is the transport grid). Returns the correction."""
+    uncorrected samples (block order), the factors and the cells' primary directions go to ``out`` (one point per
+    cell: ``grid`` is the transport grid). Returns the correction in the grid's layout."""
     K = int(omegas.reshape(-1, 3).shape[0])
     G = library.num_groups
-    samples = out.samples[view].reshape((G,) + tuple(grid.shape) + (K,))
+    if tuple(grid.shape) != tuple(out.shape) or K != int(out.omega.shape[0]):
+        raise ValueError(f"grid {tuple(grid.shape)} / {K} directions do not match the deferred source "
+                         f"{tuple(out.shape)} / {int(out.omega.shape[0])}")
     sums = _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights,
-                        samples=samples).double()
+                        samples=out.samples[view], blocked_d8=K // 8).double()
     scale, beta = first_collision_correction_factors(sums[:3], sums[3:])
-    out.scale[view].copy_(scale.reshape(G, -1))
-    out.beta[view].copy_(beta.reshape(G, -1))
+    scale, beta = scale.float(), beta.float()
+    cells = fc_blocks(grid.shape) * 64
+    out.scale[view, :, :cells].copy_(to_block_order(scale, grid.shape).reshape(G, -1))
+    out.beta[view, :, :cells].copy_(to_block_order(beta, grid.shape).reshape(G, -1))
     a, keep, _ = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
+    u = torch.empty(tuple(grid.shape) + (4,), device=phi_uncollided.device, dtype=torch.float32)
     with torch.cuda.device(phi_uncollided.device):
-        _hip.check(_hip.library().citrus_fc_primary_directions(ctypes.byref(a), out.u[view].data_ptr(),
+        _hip.check(_hip.library().citrus_fc_primary_directions(ctypes.byref(a), u.data_ptr(),
                                                                _hip.stream_of(phi_uncollided.device)),
                    "citrus_fc_primary_directions")
     del keep
+    out.u[view].copy_(to_block_order(u, grid.shape).reshape(-1, 4))
     out.omega[:, :3].copy_(omegas.reshape(-1, 3))
     out.omega[:, 3].zero_()
     out.applied = False
-    ou = out.u[view, :, :3].reshape(tuple(grid.shape) + (3,))
-    return FirstCollisionSamplingCorrection(scale=out.scale[view].reshape(scale.shape),
-                                            beta=out.beta[view].reshape(beta.shape), omega_uncollided=ou)
+    out.corrected = None
+    return FirstCollisionSamplingCorrection(scale=scale.contiguous(), beta=beta.contiguous(),
+                                            omega_uncollided=u[..., :3])
 
 
 _FC_LAYOUTS = {"moment_major": 0, "cell_major": 1, "group_major": 2, "group_major_means": 3}

@@ -114,21 +114,25 @@ typedef struct CitrusSweepView {
   const float* T;            /* [G][N][nsh][4] scattering-source moments, or NULL with T_bf16 */
   const void* T_bf16;        /* mixed operand: bfloat16 degrees l >= 2 [G][N][4][KH] (P3, P5), or NULL */
   const float* T_low;        /* with T_bf16: float32 degrees l <= 1 [G][N][4][4] */
-  float* source;             /* group-major source of every direction ([G][N][8*D8][4] or [G][N][8*D8]) or NULL */
+  float* source;             /* group-major source of every direction ([G][N][8*D8][4] or [G][N][8*D8]), source_layout
+                              * 3: the block-order cell means [G][blocks*8*64*dp + pad] (CITRUS_FC_BLOCK_ORDER), or
+                              * NULL */
   float* phi;                /* [G][N][nsh][4] fluence moments (output) */
   const float* cdir;         /* [8*D8][3] |Omega_a| / Delta_a on this view's grid */
-  const float* fc_scale;     /* source_layout 3: [G][N] scale of the first-collision sampling correction */
-  const float* fc_beta;      /* source_layout 3: [G][N] beta of the correction */
-  const float* fc_u;         /* source_layout 3: [N][4] Omega_UC of each cell (citrus_fc_primary_directions) */
+  const float* fc_scale;     /* source_layout 3: [G][blocks*64 + pad] scale of the first-collision sampling correction */
+  const float* fc_beta;      /* source_layout 3: [G][blocks*64 + pad] beta of the correction */
+  const float* fc_u;         /* source_layout 3: [blocks][64][4] Omega_UC of each cell (citrus_fc_primary_directions
+                              * in block order) */
 } CitrusSweepView;
 
 typedef struct CitrusPersistentArgs {
   int G, nsh, D8, d0, dc;    /* as in CitrusSweepArgs */
   int accumulate;            /* 0: phi is overwritten; 1: phi += this chunk's moments */
   int source_layout;         /* -1: no explicit source; 1 or 2 as in CitrusSweepArgs (the same for every view); 3: the
-                              * cell means of layout 2 before the first-collision sampling correction, which the sweep
-                              * applies as it reads them: mean scale (1 + beta eta), eta = u[cell] . omega[m], the
-                              * operations of citrus_fc_correct, so the moments equal those of layout 2 after
+                              * cell means of layout 2 before the first-collision sampling correction, in the block
+                              * order of CITRUS_FC_BLOCK_ORDER (citrus_fc_sums_blocked), which the sweep applies as it
+                              * reads them: mean scale (1 + beta eta), eta = u[cell] . omega[m], the operations of
+                              * citrus_fc_correct, so the moments equal those of layout 2 after
                               * citrus_fc_correct_means bit for bit (needs fc_scale, fc_beta, fc_u and fc_omega) */
   const float* Y;            /* [8*D8][nsh], shared by the views */
   const float* Yw;           /* [8*D8][nsh] */
@@ -402,6 +406,20 @@ int citrus_fc_source(const CitrusFcArgs* args, float* out, void* stream);
  * NULL): also stores the uncorrected S_FC of every point and direction, group-major [G][P][K]. */
 int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1, float* out,
                    float* samples, void* stream);
+
+/* CITRUS_FC_BLOCK_ORDER: the first-collision cell means as one persistent sweep task reads them. Per group
+ * [blocks][8 octants][64 cells][dp], dp = D8 rounded up to 4 (the padding zero), then CITRUS_FC_GROUP_PAD unused
+ * floats; blocks of 4^3 cells over (ceil(R/4), ceil(C/4), ceil(S/4)) row-major, a block's cells (row, column, slice)
+ * row-major (cells past the grid unused); per-cell quantities per group (scale, beta) [blocks][64] then the pad, per
+ * view (Omega_UC) [blocks][64][4]. The pad skews the regions of consecutive groups and views across HBM channels. */
+#define CITRUS_FC_BLOCK_ORDER 1
+#define CITRUS_FC_GROUP_PAD 1088
+
+/* citrus_fc_sums with the samples in block order: samples [G][blocks*8*64*dp + pad] of the K = 8 D8 octant-major sweep
+ * directions (K <= 256). The sums are those of citrus_fc_sums bit for bit. No counterpart in the reference: the
+ * layout in which the persistent sweep reads the deferred first-collision source. */
+int citrus_fc_sums_blocked(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1, float* out,
+                           float* samples, int D8, void* stream);
 
 /* In place, samples [G][P][K] (from citrus_fc_sums) *= scale (1 + beta eta) with scale, beta [G][P] and eta the
  * cosine between Omega_UC of the point and omegas[k]. At one point per cell (factor 1) the result equals

@@ -182,19 +182,24 @@ def test_fc_correction_conserves_the_exact_moments():
     assert float((J - E[1])[free].abs().max()) < 1e-5 * float(E[1].abs().max())
 
 
-@pytest.mark.parametrize("sn,L,contraction", [(6, 3, "fp32"), (12, 5, "fp32"), (12, 5, "bf16"), (8, 3, "bf16")])
-def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L, contraction):
+@pytest.mark.parametrize("sn,L,contraction,shape", [(6, 3, "fp32", (8, 8, 8)), (12, 5, "fp32", (8, 8, 8)),
+                                                     (12, 5, "bf16", (8, 8, 8)), (8, 3, "bf16", (8, 8, 8)),
+                                                     (12, 5, "fp32", (6, 5, 7)), (6, 3, "bf16", (9, 4, 6))])
+def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L, contraction, shape):
     """The sampling correction applied away from the sums kernel: by ``citrus_fc_correct_means`` or by the
-    persistent sweep as it reads the stored samples (source layout 3). Both give the corrected samples of
-    ``first_collision_corrected_samples`` (and their sweep) bit for bit; the launch sequence applies it first. S12
-    puts 21 directions in a chunk (two direction tiles); bf16 runs the mixed-precision expansion."""
+    persistent sweep as it reads the stored samples (source layout 3, block order). Both give the corrected samples
+    of ``first_collision_corrected_samples`` (and their sweep) bit for bit; the launch sequence applies it first. The
+    block-order samples equal the grid-order ones of ``citrus_fc_sums`` bit for bit (their padding zero). S12 puts 21
+    directions in a chunk (two direction tiles); bf16 runs the mixed-precision expansion; 6x5x7 and 9x4x6 leave
+    partial blocks."""
     from citrus_amd.scatter_solver.source_iteration import bf16_operand
     from citrus_amd.scatter_solver.transport_sweep import make_sweep_context, sweep_views
-    from citrus_amd.scatter_solver.uncollided import DeferredFirstCollision, first_collision_deferred_samples
+    from citrus_amd.scatter_solver.uncollided import (DeferredFirstCollision, _fc_sums_raw,
+                                                      first_collision_deferred_samples, from_block_order)
 
     dev = gpu_device()
     library = seeded_library(6, 3, 120.0, 20.0, seed=15, num_eta=97)
-    grid = Grid((8, 8, 8), 0.5)
+    grid = Grid(shape, 0.5)
     ctx = make_sweep_context(grid, level_symmetric_quadrature(sn), L, device=dev)
     om, w = ctx.quadrature.omega.to(dev, torch.float32), ctx.quadrature.weight
     V, G, M = 2, library.num_groups, ctx.num_directions
@@ -205,6 +210,15 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L
         rho_d, phi_d = rho.float().to(dev), phi.float().to(dev)
         angle = ANGLE + 0.9 * v
         corr = first_collision_deferred_samples(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w, deferred, v)
+        raw = torch.empty((G,) + tuple(shape) + (M,), device=dev)
+        _fc_sums_raw(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w, samples=raw)
+        D8 = M // 8
+        blocked = deferred.block_samples()[v].permute(0, 1, 3, 2, 4)  # [G, blocks, 64, 8, dp] (cells past the
+        # grid unused)
+        unblocked = from_block_order(blocked, shape, 1)  # [G, R, C, S, 8, dp]
+        assert torch.equal(unblocked[..., :D8].reshape(raw.shape), raw)
+        if unblocked.shape[-1] > D8:
+            assert float(unblocked[..., D8:].abs().max()) == 0.0  # the padding of the runs
         ref, ref_corr = first_collision_corrected_samples(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w)
         assert torch.equal(corr.scale, ref_corr.scale) and torch.equal(corr.beta, ref_corr.beta)
         assert rel_err(corr.omega_uncollided, ref_corr.omega_uncollided) < 1e-6
@@ -221,4 +235,4 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L
     assert bool(torch.isfinite(want).all()) and bool(torch.isfinite(got).all())
     assert not deferred.applied and torch.equal(got, want)
     got = sweep_views(ctx, mu, T, source=deferred, schedule="launches", contraction=contraction)
-    assert deferred.applied and torch.equal(deferred.samples, corrected) and torch.equal(got, want)
+    assert deferred.applied and torch.equal(deferred.apply_(), corrected) and torch.equal(got, want)
