@@ -575,10 +575,6 @@ __device__ __forceinline__ f4 contract_node_channel(const FcDev& d, const PointS
   return acc;
 }
 
-// exp(-t) for t >= 0 (+inf gives 0) as the hardware base-2 exponential of -t log2(e): relative error about
-// |t| 6e-8 + 1 ulp, against the several instructions of expf's range handling per call.
-__device__ __forceinline__ float attenuation(float t) { return __builtin_amdgcn_exp2f(-t * 1.44269504088896341f); }
-
 // Workgroup shape of the emission kernel: a run of kEmPoints consecutive points (coalesced outputs) times the
 // directions of a tile of kEmTile, each point's directions spread over kEmLanes threads (kEmPerLane each).
 constexpr int kEmThreads = 512;

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       const float w = ws[dl * kMaxGroups + g];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        float e = expf(-t[r]);
+        float e = attenuation(t[r]);
         if (TC > 0 && tc[dt][0][r] == INFINITY) e = 0.0f;  // no path to the panel
         const float v = acc[r] * e;
         ss[dt][r] += v;
