@@ -514,7 +514,9 @@ def learned_vs_plain(solver, fields, angles, device):
     plain_img, plain_flux = errors(plain, plain_images)
     learned_img, learned_flux = errors(lv.collided_moments, learned_images)
     V = len(angles)
-    return {"plain_views_per_s": V / t_plain, "learned_views_per_s": V / t_learned,
+    first_step = citrus_first_step(solver, fields, angles, net, ref, ref_images)
+    return {"citrus_first_step": first_step,
+            "plain_views_per_s": V / t_plain, "learned_views_per_s": V / t_learned,
             "learned_speedup": t_plain / t_learned, "plain_sweeps": plain_sweeps, "learned_sweeps": lv.sweeps,
             "learned_accepted": sum(lv.accepted), "judging_residuals": [round(r, 5) for r in lv.residuals],
             "plain_tolerance": tol, "plain_image_error": plain_img, "learned_image_error": learned_img,
@@ -522,6 +524,41 @@ def learned_vs_plain(solver, fields, angles, device):
             "surrogate_parameters": net.num_parameters(),
             "note": "errors: max over views of the relative L2 error of the signal-only scatter image (the SPR "
                     "numerator) and of the scalar-flux cell means, against a source iteration converged to 1e-5"}
+
+
+def citrus_first_step(solver, fields, angles, net, ref, ref_images) -> dict:
+    """The first CITRUS step of the views from empty states, plain (its accumulator is the first collision order)
+    against the learned start (``citrus_views_step(learned=net)``: two more batched sweeps and one inference, judged
+    at 1%): wall-clock rate and the errors of the step's scatter images and accumulators against the converged
+    source iteration, then the relative change the second (plain) step makes."""
+    import torch
+
+    from citrus_amd.citrus import CitrusState, citrus_views_step
+
+    out = {}
+    for name, kw in (("plain", {}), ("learned", {"learned": net, "learned_threshold": 0.01})):
+        citrus_views_step(solver, [CitrusState() for _ in angles], fields, angles, **kw)  # warm the allocator
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        results, states = citrus_views_step(solver, [CitrusState() for _ in angles], fields, angles, **kw)
+        torch.cuda.synchronize()
+        seconds = time.perf_counter() - t0
+        img = max(float(torch.linalg.vector_norm(r[0] - x) / torch.linalg.vector_norm(x))
+                  for r, x in zip(results, ref_images))
+        flux = max(float(torch.linalg.vector_norm((st.collided_moments - x)[:, 0, 0]) /
+                         torch.linalg.vector_norm(x[:, 0, 0])) for st, x in zip(states, ref))
+        residuals = [st.learned_residual for st in states]
+        _, states = citrus_views_step(solver, states, fields, angles)
+        out[name] = {"views_per_s": len(angles) / seconds, "image_error": img, "flux_error": flux,
+                     "second_step_change": max(st.history[1] for st in states)}
+        if name == "learned":
+            out[name]["judging_residuals"] = [round(r, 5) for r in residuals]
+        del results, states
+        torch.cuda.empty_cache()
+    out["note"] = ("first CITRUS step from empty states; errors are max over views against a source iteration "
+                   "converged to 1e-5; second_step_change is the relative change of the scalar-flux cell means the "
+                   "next plain step makes")
+    return out
 
 
 def mixed_precision_steps(view_step, states, images, V: int, args, stream) -> dict:

@@ -40,6 +40,7 @@ class CitrusState:
     history: list = field(default_factory=list)
     scatter_source: Optional[Tensor] = field(default=None, repr=False)
     scatter_source_fields: Optional[object] = field(default=None, repr=False)
+    learned_residual: Optional[float] = None  # the judging sweep's relative change when the learned form started it
     _batch: Optional[tuple] = field(default=None, repr=False)
 
 
@@ -131,7 +132,7 @@ def citrus_view_step(solver: ScatterSolver, state: CitrusState, fields: Material
 
 def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fields: MaterialFields,
                       angles: Sequence[float], *, primary: bool = True, contraction: str = "fp32",
-                      direction_chunk=None, schedule: str = "auto"):
+                      direction_chunk=None, schedule: str = "auto", learned=None, learned_threshold: float = 0.01):
     """``citrus_view_step`` of V views that share the densities, batched for the GPU: per view the uncollided
     fluence and the corrected first-collision source (group-major, every sweep direction); the scattering moments of
     every accumulator; then the V accumulator updates ``Phi_new = G(S_FC + K Phi)`` in one sweep with the views
@@ -141,12 +142,32 @@ def citrus_views_step(solver: ScatterSolver, states: Sequence[CitrusState], fiel
     sweep's scattering-source expansion in mixed precision (degrees l <= 1 in float32, l >= 2 as a bfloat16 matrix
     product, float32 accumulation): the scattering-moment kernel then writes that operand directly, and the next
     step's sweep reads it; ``direction_chunk`` and ``schedule`` are passed to ``sweep_views`` ("auto": the
-    autotuner's choice)."""
+    autotuner's choice).
+
+    ``learned`` (a trained ``surrogate.UNet3D``) starts views from the learned form when every state is empty: the
+    step's sweep from an empty accumulator is the first collision order phi^(1), and two more batched sweeps with one
+    batched inference (``surrogate.learned_judged``) turn it into the judged estimate of the converged collided
+    moments. A view whose judging sweep changes its scalar-flux cell means by more than ``learned_threshold`` keeps
+    the plain first step; the residual goes to the new state's ``learned_residual``. Later steps continue the CITRUS
+    iteration from whichever accumulator the view holds."""
     batch = _prepare_views(solver, states, fields, angles, contraction)
     collided = sweep_views(batch.ctx, fields.mu_t_coarse, batch.T, source=batch.source, contraction=contraction,
                            direction_chunk=direction_chunk, schedule=schedule)
+    residuals = None
+    if learned is not None and all(st.collided_moments is None for st in states):
+        from .surrogate import learned_judged
+
+        judged, residuals = learned_judged(solver, fields, batch.ctx, batch.source, collided, batch.T, learned)
+        for v, r in enumerate(residuals):
+            if r <= learned_threshold:
+                collided[v].copy_(judged[v])
+        del judged
     batch.source = None
-    return _finish_views(solver, batch, collided, primary, contraction)
+    results, new_states = _finish_views(solver, batch, collided, primary, contraction)
+    if residuals is not None:
+        for st, r in zip(new_states, residuals):
+            st.learned_residual = float(r)
+    return results, new_states
 
 
 @dataclass

@@ -571,17 +571,18 @@ class LearnedViews:
     sweeps: int
 
 
-def learned_views(solver, fields, angles: Sequence[float], net: UNet3D, threshold: float = 0.01,
-                  fallback_tolerance: float = 1e-3) -> LearnedViews:
-    """``learned_collided_moments`` of several views of one volume with batched sweeps and one batched inference:
-    phi^(1) and phi^(2) of every view (two batched sweeps), the network on all views at once, one batched judging
-    sweep; views whose relative change exceeds ``threshold`` are recomputed by the plain source iteration."""
+def learned_judged(solver, fields, ctx, source, phi1: Tensor, T: Tensor, net: UNet3D):
+    """The learned form's estimate of V views from their first collision order, judged: phi^(2) (one batched sweep
+    of K phi^(1)), the network on all views at once, then one batched judging sweep ``Phi~ = G(S_FC + K Phi^)`` of the
+    estimate ``Phi^ = phi^(1) + phi^(2) + Phi^_{>2}``. phi1 [V, G, R, C, S, nsh, 4] is left as it was; T is float32
+    scratch of the same shape (or None: allocated). Returns (judged [V, ...], residuals: per view the relative change
+    of the scalar-flux cell means from the estimate to the judged moments)."""
     from .scatter_solver.source_iteration import scatter_moments_views
     from .scatter_solver.transport_sweep import sweep_views
 
-    V = len(angles)
-    ctx, source, T = _views_setup(solver, fields, angles)
-    phi1 = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
+    V = int(phi1.shape[0])
+    if not (isinstance(T, Tensor) and T.shape == phi1.shape and T.dtype == torch.float32 and T.is_contiguous()):
+        T = torch.empty_like(phi1)
     T = scatter_moments_views(fields.transfer_coarse, phi1, out=T)
     phi2 = sweep_views(ctx, fields.mu_t_coarse, T)
     scales = [sample_scale(_api(phi2[v])) for v in range(V)]
@@ -590,8 +591,7 @@ def learned_views(solver, fields, angles: Sequence[float], net: UNet3D, threshol
                      for v in range(V)])
     pred = net.forward(x)
     G = solver.num_groups
-    estimate = phi1.add_(phi2)  # phi1 + phi2, in place
-    del phi2
+    estimate = phi2.add_(phi1)  # phi1 + phi2, in place of phi2
     for v in range(V):  # predicted P1 cell means of Phi_{>2}: moments 0..3, LD component 0
         est = estimate[v]
         est[..., :4, 0] += (pred[v].reshape(G, 4, *est.shape[1:4]) * scales[v]).permute(0, 2, 3, 4, 1)
@@ -599,13 +599,28 @@ def learned_views(solver, fields, angles: Sequence[float], net: UNet3D, threshol
     T = scatter_moments_views(fields.transfer_coarse, estimate, out=T)
     judged = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
     n, d = _scalar_norms_views(judged, judged - estimate)
-    residuals = [di / ni if ni > 0 else 0.0 for ni, di in zip(n, d)]
+    return judged, [di / ni if ni > 0 else 0.0 for ni, di in zip(n, d)]
+
+
+def learned_views(solver, fields, angles: Sequence[float], net: UNet3D, threshold: float = 0.01,
+                  fallback_tolerance: float = 1e-3) -> LearnedViews:
+    """``learned_collided_moments`` of several views of one volume with batched sweeps and one batched inference:
+    phi^(1) and phi^(2) of every view (two batched sweeps), the network on all views at once, one batched judging
+    sweep (``learned_judged``); views whose relative change exceeds ``threshold`` are recomputed by the plain source
+    iteration."""
+    from .scatter_solver.transport_sweep import sweep_views
+
+    V = len(angles)
+    ctx, source, T = _views_setup(solver, fields, angles)
+    phi1 = sweep_views(ctx, fields.mu_t_coarse, T, source=source)
+    judged, residuals = learned_judged(solver, fields, ctx, source, phi1, T, net)
+    del phi1
     accepted = [r <= threshold for r in residuals]
     result = [_api(judged[v]) for v in range(V)]
     sweeps = 3
     rejected = [v for v in range(V) if not accepted[v]]
     if rejected:
-        del estimate, source, T
+        del source, T
         totals, extra = plain_views(solver, fields, [angles[v] for v in rejected], fallback_tolerance)
         for v, tot in zip(rejected, totals):
             result[v] = tot

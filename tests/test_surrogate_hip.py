@@ -183,10 +183,10 @@ def _small_solver(dev):
     return ScatterSolver(library, config, ConeGeometry(30.0, 50.0, 16, 16, 0.5, 0.5), device=dev)
 
 
-def test_trained_surrogate_gains_over_two_orders_and_falls_back():
-    """Trained on the solver's own collision orders of seeded phantoms, the network's Phi_{>2} brings phi1 + phi2
-    much closer to the converged moments on held-out phantoms; the judging sweep accepts or, at threshold 0, falls
-    back to the plain source iteration."""
+@pytest.fixture(scope="module")
+def trained_small():
+    """(solver, fields_of(seed), net, loss history): the small solver's network trained on the collision orders of 12
+    seeded 16^3 phantoms (held-out seeds 500+ are not among them)."""
     from citrus_amd.scatter_solver.geometry import Grid
     from citrus_amd.utilities import ellipsoid_phantom
 
@@ -205,6 +205,14 @@ def test_trained_surrogate_gains_over_two_orders_and_falls_back():
     G = solver.num_groups
     net = UNet3D.create(2 + 4 * G, 4 * G, width=8, device=dev, seed=3)
     hist = train(net, xs, ys, epochs=60, lr=2e-3)
+    return solver, fields_of, net, hist
+
+
+def test_trained_surrogate_gains_over_two_orders_and_falls_back(trained_small):
+    """Trained on the solver's own collision orders of seeded phantoms, the network's Phi_{>2} brings phi1 + phi2
+    much closer to the converged moments on held-out phantoms; the judging sweep accepts or, at threshold 0, falls
+    back to the plain source iteration."""
+    solver, fields_of, net, hist = trained_small
     assert hist[-1] < 0.2 * hist[0], hist
     gains = []
     for seed in (500, 501, 502):
@@ -257,3 +265,35 @@ def test_batched_views_equal_single_views():
     forced = learned_views(solver, fields, angles, net, threshold=0.0, fallback_tolerance=1e-4)
     assert forced.accepted == [False, False] and forced.sweeps == 3 + batched[0].sweeps
     assert all(torch.equal(m, o.total) for m, o in zip(forced.collided_moments, batched))
+
+
+def test_learned_start_of_the_citrus_step(trained_small):
+    """The learned form as the first CITRUS step of views with empty states: the step's sweep is phi^(1), two more
+    batched sweeps and one inference give the judged estimate, so the first accumulator of a held-out phantom is
+    within a few percent of the converged collided moments where the plain first step (phi^(1) alone) is far off; the
+    next step changes it little. At threshold 0 the judge rejects and the views keep the plain first step bit for
+    bit; the residual is recorded either way."""
+    from citrus_amd.citrus import CitrusState, citrus_views_step
+
+    solver, fields_of, net, _ = trained_small
+    fields = fields_of(501)
+    angles = [0.7, 2.1]
+    print("learned start: 16^3, 4 groups, S4, P3, two views", flush=True)
+    _, plain = citrus_views_step(solver, [CitrusState() for _ in angles], fields, angles)
+    _, learned = citrus_views_step(solver, [CitrusState() for _ in angles], fields, angles, learned=net,
+                                   learned_threshold=0.05)
+    _, forced = citrus_views_step(solver, [CitrusState() for _ in angles], fields, angles, learned=net,
+                                  learned_threshold=0.0)
+    for v, a in enumerate(angles):
+        total = collision_orders(solver, fields, a).total
+        err_plain = rel_err(plain[v].collided_moments[:, 0, 0], total[:, 0, 0])
+        err_learned = rel_err(learned[v].collided_moments[:, 0, 0], total[:, 0, 0])
+        assert learned[v].learned_residual < 0.05 and err_learned < 0.05, (learned[v].learned_residual, err_learned)
+        assert err_plain > 5.0 * err_learned, (err_plain, err_learned)
+        assert plain[v].learned_residual is None and forced[v].learned_residual > 0.0
+        assert torch.equal(forced[v].collided_moments, plain[v].collided_moments)
+    _, after_learned = citrus_views_step(solver, learned, fields, angles)
+    _, after_plain = citrus_views_step(solver, plain, fields, angles)
+    for v in range(len(angles)):
+        assert after_learned[v].history[1] < 0.05 < after_plain[v].history[1], \
+            (after_learned[v].history, after_plain[v].history)
