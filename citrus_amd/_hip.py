@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 15
+ABI_VERSION = 16
 # points per block of the emission kernels' partial sums (CITRUS_EMISSION_BLOCK of include/citrus_hip.h)
 EMISSION_BLOCK = 32
 # unused floats after each group's region of the block-order first-collision source (CITRUS_FC_GROUP_PAD)
@@ -156,6 +156,8 @@ _SIGNATURES = {
     "citrus_box_matrix": (c_int, [c_void, c_int, c_int, c_void, c_void]),
     "citrus_footprint_filters": (c_int, [c_void, c_int, c_int, ctypes.c_float, c_void, ctypes.c_float, ctypes.c_float,
                                          c_int, c_int, c_void, c_void, c_void, c_void, c_void]),
+    "citrus_footprint_blur": (c_int, [c_void, c_int, c_int, ctypes.c_float, c_void, ctypes.c_float, ctypes.c_float,
+                                      c_int, c_int, c_void, c_void, c_void, c_void, c_void, c_void]),
 }
 
 

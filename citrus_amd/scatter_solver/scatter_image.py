@@ -376,6 +376,38 @@ class _Footprint:
                 br.data_ptr(), _hip.stream_of(dev)), "citrus_footprint_filters")
         return bct, br
 
+    def blur_accumulate(self, acc: Tensor, image: Tensor, chunk_index: int, partial: Tensor,
+                        channel_weight: Optional[Tensor] = None) -> None:
+        """acc [Rd, Cd] += sum_k (row blur of (channel blur of image_k)) for the single-channel images [k, 1, Rd, Cd]
+        of chunk ``chunk_index``, the footprint widths from the emission kernel's per-block sums ``partial``: the
+        filters of ``filters`` applied window by window (``citrus_footprint_blur``: two banded kernels, no dense
+        filter matrices, each pixel summed over the chunk's directions by one thread)."""
+        start, stop = self.context.chunk_bounds[chunk_index]
+        k, G, Rd, Cd = image.shape
+        if G != 1:
+            raise ValueError("blur_accumulate takes single-channel images")
+        if tuple(acc.shape) != (Rd, Cd) or not acc.is_contiguous() or acc.dtype != torch.float32:
+            raise ValueError(f"acc must be a contiguous float32 [{Rd}, {Cd}] tensor")
+        nb = int(partial.shape[1])
+        if int(partial.shape[0]) != k:
+            raise ValueError(f"partial has {int(partial.shape[0])} directions, the images {k}")
+        cw = self.context.cell_width[start:stop].contiguous()
+        for t, name in ((partial, "partial"), (cw, "cell_width"), (image, "image"), (acc, "acc")):
+            _hip.require_gpu_tensor(t, name)
+        if channel_weight is not None:
+            _hip.require_gpu_tensor(channel_weight, "channel_weight")
+            if tuple(channel_weight.shape) != (k, Cd) or not channel_weight.is_contiguous():
+                raise ValueError(f"channel_weight must be contiguous [{k}, {Cd}], got {tuple(channel_weight.shape)}")
+        dev = partial.device
+        image = image.contiguous()
+        heights = torch.empty(k, device=dev, dtype=torch.float32)
+        work = torch.empty(k, Rd, Cd, device=dev, dtype=torch.float32)
+        with torch.cuda.device(dev):
+            _hip.check(_hip.library().citrus_footprint_blur(
+                partial.data_ptr(), k, nb, float(self.center_height), cw.data_ptr(), self.scale_host[0],
+                self.scale_host[1], Cd, Rd, _hip.ptr(channel_weight), image.data_ptr(), heights.data_ptr(),
+                work.data_ptr(), acc.data_ptr(), _hip.stream_of(dev)), "citrus_footprint_blur")
+
     def accumulate_blurred(self, acc: Tensor, image: Tensor, channel_filters_t: Tensor, row_filters: Tensor) -> None:
         """acc [Rd, Cd] += sum_k (row blur of (channel blur of image_k)) for single-channel images [k, 1, Rd, Cd]:
         two batched products and one accumulating product with a ones vector (the sum over k)."""
@@ -643,9 +675,9 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
         del tau, multiple
         image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, directions)
         del emission
-        if fp is not None and group_weight is not None:  # blur, channel weights and the sum over k in two products
-            pw = None if context.planar_weight is None else context.planar_weight[start:stop]
-            fp.accumulate_blurred(acc, image, *fp.filters(i, partial, pw))
+        if fp is not None and group_weight is not None:  # blur, channel weights and the sum over k, banded
+            pw = None if context.planar_weight is None else context.planar_weight[start:stop].contiguous()
+            fp.blur_accumulate(acc, image, i, partial, pw)
             del image
             continue
         if fp is not None:

@@ -484,6 +484,15 @@ int citrus_footprint_filters(const float* partial, int K, int nblocks, float cen
                              float scale_channel, float scale_row, int Nc, int Nr, const float* channel_weight,
                              float* heights, float* channel_filter_t, float* row_filter, void* stream);
 
+/* The signal-only image's footprint blur of one direction chunk and its sum over the chunk's directions, banded
+ * (the box filters of citrus_footprint_filters applied window by window instead of as dense matrices:
+ * box_blur_rows_cols and the direction sum of scatter_image.py:897,1033-1060,1200-1230): heights[k] from the
+ * emission kernel's partial sums as in citrus_footprint_filters; work [K][Nr][Nc] = the channel blur of image
+ * [K][Nr][Nc] (times channel_weight [K][Nc] when given); acc [Nr][Nc] += sum_k the row blur of work[k]. */
+int citrus_footprint_blur(const float* partial, int K, int nblocks, float center_height, const float* cell_width,
+                          float scale_channel, float scale_row, int Nc, int Nr, const float* channel_weight,
+                          const float* image, float* heights, float* work, float* acc, void* stream);
+
 /* Mixed-precision operand of the scattering-source contraction (the mixed-precision form of the source expansion of
  * transport_sweep.py:524-528 / source_iteration.py:92-102): T [V][G][N][nsh][4] float32 -> T_low [V][G][N][4][4]
  * float32 (moments n = 0..3, degrees l <= 1, component-major) and T16 [V][G][N][4][KH] bfloat16 (moments n >= 4,

@@ -293,6 +293,40 @@ def test_footprint_filters_match_the_torch_footprint():
         fp.filters(1, partial, pw[:, 1:].contiguous())
 
 
+@pytest.mark.parametrize("height_scale", [1.0, 0.01, 40.0])
+def test_banded_footprint_blur_equals_the_dense_filters(height_scale):
+    """citrus_footprint_blur (the box filters summed window by window, the directions summed per pixel) equals the
+    dense filter matrices' two batched products and direction sum (accumulate_blurred), with and without channel
+    weights; the height scale makes windows of a few pixels, below one pixel (clamped) and wider than the panel
+    (clipped and renormalised); a direction without emission takes the box-centre height."""
+    from citrus_amd.scatter_solver.scatter_image import _Footprint
+
+    dev = gpu_device()
+    library, coarse, gq, rho, phi, T, mu_c, geometry = _image_problem(True, 1)
+    angle = 0.35
+    quad = panel_direction_set(geometry, angle, (3, 6), max_polar_angle=gq)
+    ctx = make_direction_context(quad, geometry, angle, energy_weighted_response(), no_grid(), library, dev,
+                                 torch.float32, chunk=5)
+    fp = _Footprint(gq, geometry, angle, ctx)
+    gen = torch.Generator().manual_seed(47)
+    k = ctx.chunk_bounds[1][1] - ctx.chunk_bounds[1][0]
+    partial = torch.rand(k, 37, 2, generator=gen)
+    partial[:, :, 1] *= 10.0 * height_scale
+    partial[0] = 0.0
+    partial = partial.to(dev)
+    Rd, Cd = ctx.image_shape
+    image = torch.rand(k, 1, Rd, Cd, generator=gen).to(dev)
+    pw = torch.rand(k, Cd, generator=gen).to(dev)
+    for weight in (None, pw):
+        want = torch.full((Rd, Cd), 0.5, device=dev)
+        fp.accumulate_blurred(want, image, *fp.filters(1, partial, weight))
+        got = torch.full((Rd, Cd), 0.5, device=dev)
+        fp.blur_accumulate(got, image, 1, partial, weight)
+        assert rel_err(got, want) < F32_REL, (height_scale, weight is None)
+    with pytest.raises(ValueError):
+        fp.blur_accumulate(got, image, 1, partial, pw[:, 1:].contiguous())
+
+
 def test_multiple_emission_at_the_view_step_size():
     """At the 64^3 view step's size (K = 64 directions, P5, 16 groups, 262,144 cells, 2 traced channels) the
     collapsed multiple-collision emission equals the expansion on sh_expand followed by the attenuation and group sums
