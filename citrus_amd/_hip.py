@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 16
+ABI_VERSION = 17
 # points per block of the emission kernels' partial sums (CITRUS_EMISSION_BLOCK of include/citrus_hip.h)
 EMISSION_BLOCK = 32
 # unused floats after each group's region of the block-order first-collision source (CITRUS_FC_GROUP_PAD)

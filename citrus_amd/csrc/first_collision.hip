@@ -24,6 +24,7 @@
 #include <type_traits>
 
 #include "citrus_hip.h"
+#include "block_planes.h"
 #include "common.h"
 
 namespace citrus {
@@ -201,17 +202,19 @@ __device__ __forceinline__ void load_slots(const FcDev& d, int p, PointSlots& ps
 
 // The sweep's block order of first-collision cell means (source layout 3 of the persistent sweep): per group
 // [blocks of 4^3 cells][8 octants][64 cells of the block][dp], blocks (row block, column block, slice block) in
-// row-major order, a block's cells (row, column, slice) in row-major order, dp = directions per octant rounded up to
-// 4 (the padding zero), then CITRUS_FC_GROUP_PAD unused floats. One sweep task (block, octant, group) reads one
+// row-major order, the cells of octant o's run in o's task order (block_slot4: by diagonal plane from the octant's
+// upstream corner, so the cells one plane of the task reads are adjacent), dp = directions per octant rounded up to 4
+// (the padding zero), then CITRUS_FC_GROUP_PAD unused floats. One sweep task (block, octant, group) reads one
 // contiguous run of 64 dp floats; the pad skews the groups' and views' regions across HBM channels (unpadded they
 // all start on the same channel, and the persistent sweep runs the same block of 16 domains of one XCD together).
 __host__ __device__ __forceinline__ int64_t fc_sample_group_floats(int R, int C, int S, int dp) {
   return (int64_t)((R + 3) / 4) * ((C + 3) / 4) * ((S + 3) / 4) * 8 * 64 * dp + CITRUS_FC_GROUP_PAD;
 }
 
-__host__ __device__ __forceinline__ int64_t fc_sample_offset(int R, int C, int S, int dp, int r, int c, int s) {
+// offset of octant o's run of block (r/4, c/4, s/4), and the position of cell (r, c, s) in it
+__device__ __forceinline__ int64_t fc_sample_run(int C, int S, int dp, int r, int c, int s, int o) {
   const int64_t blk = ((int64_t)(r >> 2) * ((C + 3) / 4) + (c >> 2)) * ((S + 3) / 4) + (s >> 2);
-  return (blk * 8 * 64 + (((r & 3) * 4 + (c & 3)) * 4 + (s & 3))) * dp;
+  return ((blk * 8 + o) * 64 + block_slot4(o, c & 3, r & 3, s & 3)) * dp;
 }
 
 // S of the four output groups 4 ch .. 4 ch + 3 at one eta, for a wave-uniform channel with the slots in registers:
@@ -258,9 +261,11 @@ __device__ __forceinline__ f4 eval_source_channel_batched(const FcDev& d, const 
 // groups over threads keeps the accumulators of a thread at 4 groups, so 4x as many points' gathers are in flight.
 // The directions (unit vector and weight) pass through LDS in tiles of one per thread; the point's material slots stay
 // in registers. Per group the terms and their order are those of the all-group form.
-// d8 > 0: the samples go out in the sweep's block order instead (fc_sample_offset), the K = 8 d8 directions octant
-// by octant in runs of four.
-template <int GMAX>
+// d8 > 0: the samples go out in the sweep's block order instead (fc_sample_run), the K = 8 d8 directions octant
+// by octant in runs of four, and the threads take the points in that order too (one wave per 4^3 block), so that a
+// wave's consecutive run stores fill the block's contiguous 64 x dp floats of one octant while it is in L2; in grid
+// order a wave's stores fall into 16 blocks at the stride of a cell, and lines left L2 partly written.
+template <int GMAX, bool BLOCKED>
 __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* __restrict__ weights,
                                                          const float* __restrict__ mu_s_l, int lp1,
                                                          float* __restrict__ out, float* __restrict__ samples,
@@ -268,8 +273,15 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
   __shared__ f4 sdir[kBlock];  // (omega x, y, z, weight) of the tile's directions
   constexpr int NCH = GMAX / 4;
   const int ch = NCH - 1 - (int)(blockIdx.x / blocks_per_channel);
-  const int p = (int)(blockIdx.x % blocks_per_channel) * kBlock + threadIdx.x;
+  const int t = (int)(blockIdx.x % blocks_per_channel) * kBlock + threadIdx.x;
   if (4 * ch >= d.G) return;  // uniform over the workgroup
+  int p = t;
+  if (BLOCKED) {  // t = 64 block + cell of the block, both row-major
+    const int bc = (d.C + 3) / 4, bs = (d.S + 3) / 4, blk = t / 64, lc = t % 64;
+    const int r = (blk / (bc * bs)) * 4 + lc / 16, c = ((blk / bs) % bc) * 4 + (lc / 4) % 4,
+              s = (blk % bs) * 4 + lc % 4;
+    p = r < d.R && c < d.C && s < d.S ? (r * d.C + c) * d.S + s : d.P;
+  }
   const bool active = p < d.P;
   const int pp = active ? p : d.P - 1;  // inactive threads only help stage the directions
   const int r = pp / (d.C * d.S), c = (pp / d.S) % d.C, s = pp % d.S;
@@ -280,26 +292,26 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
   PointSlots ps;
   load_slots(d, pp, ps);
   f4 I0 = {0.f, 0.f, 0.f, 0.f}, I1 = I0, I2 = I0;
-  if (d8 > 0) {
-    // block order: per octant the d8 samples in runs of four (padding of the last run zero), each run one 16-byte
-    // store per group; the sums in direction order as below
+  if (BLOCKED) {
+    // block order: per octant the d8 samples in runs of eight (32 bytes, whole sectors when dp % 8 == 0; padding
+    // zero), each run two 16-byte stores per group; the sums in direction order as below
     const int dp = (d8 + 3) & ~3;
     if ((int)threadIdx.x < d.K)
       sdir[threadIdx.x] = f4{d.omegas[3 * threadIdx.x], d.omegas[3 * threadIdx.x + 1], d.omegas[3 * threadIdx.x + 2],
                              weights[threadIdx.x]};
     __syncthreads();
     if (active) {
-      const int64_t base = fc_sample_offset(d.R, d.C, d.S, dp, r, c, s);
       const int64_t gstride = fc_sample_group_floats(d.R, d.C, d.S, dp);
       for (int o = 0; o < 8; ++o) {
-        for (int d0 = 0; d0 < dp; d0 += 4) {
-          f4 run[4];
+        const int64_t base = fc_sample_run(d.C, d.S, dp, r, c, s, o);
+        for (int d0 = 0; d0 < dp; d0 += 8) {
+          f4 run[4][2];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
+          for (int j = 0; j < 8; ++j) {
             const int dd = d0 + j;
             if (dd >= d8) {
 #pragma unroll
-              for (int gl = 0; gl < 4; ++gl) run[gl][j] = 0.0f;
+              for (int gl = 0; gl < 4; ++gl) run[gl][j / 4][j % 4] = 0.0f;
               continue;
             }
             const f4 dir = sdir[o * d8 + dd];
@@ -313,13 +325,16 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
               I0[gl] = fmaf(w, S[gl], I0[gl]);
               I1[gl] = fmaf(we, S[gl], I1[gl]);
               I2[gl] = fmaf(wee, S[gl], I2[gl]);
-              run[gl][j] = S[gl];
+              run[gl][j / 4][j % 4] = S[gl];
             }
           }
 #pragma unroll
-          for (int gl = 0; gl < 4; ++gl)
-            if (4 * ch + gl < d.G)
-              *reinterpret_cast<f4*>(samples + (4 * ch + gl) * gstride + base + (int64_t)o * 64 * dp + d0) = run[gl];
+          for (int gl = 0; gl < 4; ++gl) {
+            if (4 * ch + gl >= d.G) continue;
+            f4* dst = reinterpret_cast<f4*>(samples + (4 * ch + gl) * gstride + base + d0);
+            dst[0] = run[gl][0];
+            if (d0 + 4 < dp) dst[1] = run[gl][1];
+          }
         }
       }
     }
@@ -328,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
   // stores, one whole 32-byte sector (runs of four leave half sectors, which reach memory as whole ones)
   constexpr int RUN = 8;
   const bool vec = samples && (d.K % RUN) == 0;
-  for (int t0 = 0; t0 < (d8 > 0 ? 0 : d.K); t0 += kBlock) {
+  for (int t0 = 0; t0 < (BLOCKED ? 0 : d.K); t0 += kBlock) {
     const int kt = min(kBlock, d.K - t0);
     __syncthreads();  // the previous tile is no longer read
     if ((int)threadIdx.x < kt) {
@@ -870,8 +885,8 @@ extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, co
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
     constexpr int NCH = decltype(gm)::value / 4;
-    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0, s, d,
-                       weights, mu_s_l, lp1, out, samples, blocks, 0);
+    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value, false>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0,
+                       s, d, weights, mu_s_l, lp1, out, samples, blocks, 0);
     return 0;
   });
   return finish("first-collision sums");
@@ -886,14 +901,16 @@ extern "C" int citrus_fc_sums_blocked(const CitrusFcArgs* args, const float* wei
   if (D8 < 1 || args->K != 8 * D8 || args->K > kBlock)
     return fail(CITRUS_ERR_ARG, "the block order needs K = 8 D8 sweep directions (octant-major), at most 256");
   const FcDev d = make_dev(*args);
-  const int blocks = (d.P + kBlock - 1) / kBlock;
+  // threads over the padded block order: 64 per 4^3 block
+  const int64_t cells = (int64_t)((d.R + 3) / 4) * ((d.C + 3) / 4) * ((d.S + 3) / 4) * 64;
+  const int blocks = (int)((cells + kBlock - 1) / kBlock);
   if (!grid_fits((int64_t)blocks * (group_pad(d.G) / 4), kBlock))
     return fail(CITRUS_ERR_ARG, "points x group channels exceed 2^32 work-items per launch");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dispatch_groups(d.G, [&](auto gm) {
     constexpr int NCH = decltype(gm)::value / 4;
-    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0, s, d,
-                       weights, mu_s_l, lp1, out, samples, blocks, D8);
+    hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value, true>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0,
+                       s, d, weights, mu_s_l, lp1, out, samples, blocks, D8);
     return 0;
   });
   return finish("first-collision sums (block order)");

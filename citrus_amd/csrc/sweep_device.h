@@ -7,6 +7,7 @@
 
 #include <cstdint>
 
+#include "block_planes.h"
 #include "common.h"
 
 namespace citrus {
@@ -20,38 +21,6 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // __syncthreads also waits for them). Every cross-thread exchange in the sweep kernel goes through LDS.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-
-// Cells of a B^3 block grouped by diagonal plane a + b + c = p, in sweep-local coordinates.
-template <int B>
-struct PlaneTable {
-  static constexpr int kPlanes = 3 * B - 2;
-  int offset[3 * B - 1];
-  int cell[B * B * B];  // a | b << 8 | c << 16  (a along x, b along y, c along z)
-  constexpr PlaneTable() : offset(), cell() {
-    int n = 0;
-    for (int p = 0; p < kPlanes; ++p) {
-      offset[p] = n;
-      for (int a = 0; a < B; ++a)
-        for (int b = 0; b < B; ++b) {
-          int c = p - a - b;
-          if (c >= 0 && c < B) cell[n++] = a | (b << 8) | (c << 16);
-        }
-    }
-    offset[kPlanes] = n;
-  }
-  constexpr int max_plane() const {
-    int m = 0;
-    for (int p = 0; p < kPlanes; ++p) m = (offset[p + 1] - offset[p]) > m ? (offset[p + 1] - offset[p]) : m;
-    return m;
-  }
-};
-
-__constant__ PlaneTable<4> kPlanes4 = PlaneTable<4>();
-
-template <int B>
-__device__ __forceinline__ const PlaneTable<B>& planes();
-template <>
-__device__ __forceinline__ const PlaneTable<4>& planes<4>() { return kPlanes4; }
 
 // ---------------------------------------------------------------------------------------------------------------------
 // Matrix-core variant for the moment-source sweep (T given, plus an optional group-major explicit source), B = 4,
@@ -160,7 +129,8 @@ struct DirTables {
 // SRC: 0 no explicit source; 1 group-major LD moments (layout 1, read per plane); 2 group-major cell means (layout 2,
 // prefetched one plane ahead with T, the slopes zero); 3 the cell means before the first-collision sampling
 // correction, stored in block order (per block and octant the 64 cells' dp = D8 rounded up to 4 directions, one
-// contiguous run per task; the correction factors and Omega_UC per block of 64 cells), the correction applied here as
+// contiguous run per task with the cells in the task's plane order; the correction factors and Omega_UC per block of
+// 64 cells, row-major), the correction applied here as
 // they are read: mean scale (1 + beta eta) with eta = u[cell] . omega[m] (direction_cosine), the operations of the
 // correction kernels. COH: the faces and moments other tasks of the same launch
 // produce or consume are accessed at device scope (the persistent sweep); the launch-sequence sweep exchanges them
@@ -215,6 +185,7 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
   constexpr int src_words = SRC == 2 ? 1 : 4;
   const int nblocks = a.nbr * a.nbc * a.nbs;
   const int dp = (dt.D8 + 3) & ~3;  // SRC 3: stored directions per octant
+  const unsigned run64 = ((unsigned)blk * 8u + (unsigned)o) * 64u;  // SRC 3: the task's run, in cells
   const __amdgpu_buffer_rsrc_t src_r = __builtin_amdgcn_make_buffer_rsrc(
       HAS_SRC ? const_cast<float*>(a.src) : a.phi, 0,
       SRC == 3 ? nblocks * 8 * 64 * dp * 4 : (HAS_SRC ? a.N * dt.M * 4 * src_words : 0), kBufferWord3);
@@ -360,13 +331,14 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
       const int sd = 4 * mt + q;
       const int cd = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
       st.mu[mt] = buf_load(mu_r, cd >= 0 ? 4u * cd : kDrop);
-      const unsigned bcell = SRC == 3 ? block_cell(pb, P, sd) : 0u;
+      // SRC 3: the task's run holds the cells in this task order (slot pb + sd), so a plane's cells are adjacent
+      unsigned slot = (unsigned)(pb + (sd < P ? sd : P - 1));
+      if (SRC == 3) asm volatile("" : "+v"(slot));  // formed here, not hoisted for every plane up front
       if (MEANS) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const int d = nt * 16 + r16;
-          const unsigned e = SRC == 3 ? ((unsigned)blk * 8u * 64u + (unsigned)o * 64u + (bcell - (unsigned)blk * 64u)) *
-                                            (unsigned)dp + (unsigned)(dt.d0 + d)
+          const unsigned e = SRC == 3 ? (run64 + slot) * (unsigned)dp + (unsigned)(dt.d0 + d)
                                       : (unsigned)cd * dt.M + (unsigned)(mbase + d);
           st.sm[mt][nt] = buf_load(src_r, (cd >= 0 && d < dc) ? e * 4u : kDrop);
         }

@@ -388,6 +388,24 @@ def from_block_order(x: Tensor, shape, lead_dims: int) -> Tensor:
     return y[(slice(None),) * nl + (slice(0, R), slice(0, C), slice(0, S))]
 
 
+def octant_task_slots() -> np.ndarray:
+    """[8, 64]: where the sample of a block's cell lc (row, column, slice row-major within the 4^3 block) sits in
+    octant o's run of the block order, i.e. the cell's place in that octant's sweep task: by diagonal plane
+    a + b + c, then a, then b, where (a, b, c) are the cell's (column, row, slice) counted from the octant's upstream
+    corner (o & 4 reverses the columns, o & 2 the rows, o & 1 the slices)."""
+    order = sorted((a + b + c, a, b, c) for a in range(4) for b in range(4) for c in range(4))
+    slot_of = {abc[1:]: n for n, abc in enumerate(order)}
+    out = np.empty((8, 64), dtype=np.int64)
+    for o in range(8):
+        for lc in range(64):
+            ly, lx, lz = lc // 16, (lc // 4) % 4, lc % 4
+            a = 3 - lx if o & 4 else lx
+            b = 3 - ly if o & 2 else ly
+            c = 3 - lz if o & 1 else lz
+            out[o, lc] = slot_of[(a, b, c)]
+    return out
+
+
 @dataclass(eq=False)
 class DeferredFirstCollision:
     """The first-collision cell means of V views at every sweep direction (one quadrature point per cell), stored
@@ -396,10 +414,11 @@ class DeferredFirstCollision:
     corrected group-major means for any other consumer. Either way each value is ``mean scale (1 + beta eta)`` with
     the operations of ``citrus_fc_correct`` (uncollided.py:420-433), so the two are equal bit for bit.
 
-    Block order (``to_block_order``): blocks of 4^3 cells, per block and octant the 64 cells' D8 directions padded to
-    dp = D8 rounded up to 4, so that one sweep task (block, octant, group) reads one contiguous run; each group's
-    region is followed by ``FC_GROUP_PAD`` unused floats, which skew the groups' and views' regions across HBM
-    channels.
+    Block order (``to_block_order``'s blocks): blocks of 4^3 cells, per block and octant the 64 cells' D8 directions
+    padded to dp = D8 rounded up to 4, so that one sweep task (block, octant, group) reads one contiguous run, the
+    cells of octant o's run in o's task order (``octant_task_slots``: the cells of one plane of the task adjacent);
+    each group's region is followed by ``FC_GROUP_PAD`` unused floats, which skew the groups' and views' regions
+    across HBM channels. The factors and directions per cell are in ``to_block_order``'s order.
 
     Attributes: shape (R, C, S); samples [V, G, blocks * 8 * 64 * dp + pad] (``block_samples`` views the blocks);
     scale and beta [V, G, blocks * 64 + pad] float32; u [V, blocks * 64, 4] the primary direction of each cell
@@ -433,6 +452,13 @@ class DeferredFirstCollision:
         nb, dp = fc_blocks(self.shape), -(-(int(self.omega.shape[0]) // 8) // 4) * 4
         return self.samples[..., :nb * 8 * 64 * dp].reshape(V, G, nb, 8, 64, dp)
 
+    def cell_samples(self) -> Tensor:
+        """The samples as [V, G, blocks, 8, 64, dp] with the cells of every run in ``to_block_order``'s order (row,
+        column, slice row-major within the block)."""
+        x = self.block_samples()
+        slots = torch.as_tensor(octant_task_slots(), device=x.device)
+        return x[:, :, :, torch.arange(8, device=x.device)[:, None], slots]
+
     def apply_(self) -> Tensor:
         """The group-major cell means [V, G, R, C, S, M] of the corrected source (formed once): the block order
         undone, then the correction of ``citrus_fc_correct_means``."""
@@ -443,7 +469,7 @@ class DeferredFirstCollision:
             R, C, S = self.shape
             N = R * C * S
             nb = fc_blocks(self.shape)
-            means = from_block_order(self.block_samples()[..., :D8].permute(0, 1, 2, 4, 3, 5), self.shape, 2)
+            means = from_block_order(self.cell_samples()[..., :D8].permute(0, 1, 2, 4, 3, 5), self.shape, 2)
             means = means.reshape(V, G, R, C, S, M).contiguous()
             scale = from_block_order(self.scale[..., :nb * 64].reshape(V, G, nb, 64), self.shape, 2).contiguous()
             beta = from_block_order(self.beta[..., :nb * 64].reshape(V, G, nb, 64), self.shape, 2).contiguous()

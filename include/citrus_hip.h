@@ -409,10 +409,13 @@ int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* 
 
 /* CITRUS_FC_BLOCK_ORDER: the first-collision cell means as one persistent sweep task reads them. Per group
  * [blocks][8 octants][64 cells][dp], dp = D8 rounded up to 4 (the padding zero), then CITRUS_FC_GROUP_PAD unused
- * floats; blocks of 4^3 cells over (ceil(R/4), ceil(C/4), ceil(S/4)) row-major, a block's cells (row, column, slice)
- * row-major (cells past the grid unused); per-cell quantities per group (scale, beta) [blocks][64] then the pad, per
- * view (Omega_UC) [blocks][64][4]. The pad skews the regions of consecutive groups and views across HBM channels. */
-#define CITRUS_FC_BLOCK_ORDER 1
+ * floats; blocks of 4^3 cells over (ceil(R/4), ceil(C/4), ceil(S/4)) row-major; in octant o's run the cells come in
+ * the task order of that octant: sorted by diagonal plane a + b + c, then a, then b, where (a, b, c) are the cell's
+ * (column, row, slice) within the block counted from the octant's upstream corner (o & 4: columns reversed, o & 2:
+ * rows, o & 1: slices), cells past the grid unused. Per-cell quantities per group (scale, beta) [blocks][64] then the
+ * pad, per view (Omega_UC) [blocks][64][4], a block's cells (row, column, slice) row-major. The pad skews the regions
+ * of consecutive groups and views across HBM channels. */
+#define CITRUS_FC_BLOCK_ORDER 2
 #define CITRUS_FC_GROUP_PAD 1088
 
 /* citrus_fc_sums with the samples in block order: samples [G][blocks*8*64*dp + pad] of the K = 8 D8 octant-major sweep

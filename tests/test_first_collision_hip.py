@@ -213,7 +213,7 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L
         raw = torch.empty((G,) + tuple(shape) + (M,), device=dev)
         _fc_sums_raw(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w, samples=raw)
         D8 = M // 8
-        blocked = deferred.block_samples()[v].permute(0, 1, 3, 2, 4)  # [G, blocks, 64, 8, dp] (cells past the
+        blocked = deferred.cell_samples()[v].permute(0, 1, 3, 2, 4)  # [G, blocks, 64, 8, dp] (cells past the
         # grid unused)
         unblocked = from_block_order(blocked, shape, 1)  # [G, R, C, S, 8, dp]
         assert torch.equal(unblocked[..., :D8].reshape(raw.shape), raw)

@@ -7,6 +7,7 @@ import re
 from pathlib import Path
 
 import pytest
+import numpy as np
 import torch
 
 from citrus_amd import _hip
@@ -183,3 +184,35 @@ def test_block_order_round_trip_and_cell_placement():
             blk = ((r // 4) * nbc + c // 4) * nbs + s // 4
             assert torch.equal(b[:, blk, ((r % 4) * 4 + c % 4) * 4 + s % 4], x[:, r, c, s])
         assert int((b != 0).sum()) == x.numel()  # every value once, the padding cells zero
+
+
+def test_octant_task_slots_follow_each_octants_sweep():
+    """The cells of octant o's run of block-order samples come in o's sweep order: each run is a permutation of the
+    block's 64 cells; the octant's upstream corner is first and the opposite corner last; a cell comes after its
+    upstream neighbour along every axis (the direction of travel given by the octant's signs); the planes of equal
+    depth are contiguous, 1, 3, 6, 10, 12, 12, 10, 6, 3, 1 cells long."""
+    from citrus_amd.scatter_solver.uncollided import octant_task_slots
+
+    slots = octant_task_slots()
+    assert slots.shape == (8, 64)
+    sizes = [1, 3, 6, 10, 12, 12, 10, 6, 3, 1]
+    starts = np.cumsum([0] + sizes)
+    for o in range(8):
+        assert sorted(slots[o].tolist()) == list(range(64))
+        step = {"c": -1 if o & 4 else 1, "r": -1 if o & 2 else 1, "s": -1 if o & 1 else 1}
+        corner = lambda lo: tuple(0 if (step[a] > 0) == lo else 3 for a in "rcs")  # noqa: E731
+        lc = lambda r, c, s: (r * 4 + c) * 4 + s  # noqa: E731
+        assert slots[o, lc(*corner(True))] == 0 and slots[o, lc(*corner(False))] == 63
+        for r in range(4):
+            for c in range(4):
+                for s in range(4):
+                    here = slots[o, lc(r, c, s)]
+                    for axis, (ur, uc, us) in (("r", (r - step["r"], c, s)), ("c", (r, c - step["c"], s)),
+                                               ("s", (r, c, s - step["s"]))):
+                        if 0 <= ur < 4 and 0 <= uc < 4 and 0 <= us < 4:
+                            assert slots[o, lc(ur, uc, us)] < here, (o, axis)
+                    dr = r if step["r"] > 0 else 3 - r
+                    dc = c if step["c"] > 0 else 3 - c
+                    ds = s if step["s"] > 0 else 3 - s
+                    depth = dr + dc + ds
+                    assert starts[depth] <= here < starts[depth + 1]
