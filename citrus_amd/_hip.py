@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 17
+ABI_VERSION = 18
 # points per block of the emission kernels' partial sums (CITRUS_EMISSION_BLOCK of include/citrus_hip.h)
 EMISSION_BLOCK = 32
 # unused floats after each group's region of the block-order first-collision source (CITRUS_FC_GROUP_PAD)
@@ -144,7 +144,8 @@ _SIGNATURES = {
     "citrus_nn_to_bf16": (c_int, [c_void, c_void, c_i64, c_void]),
     "citrus_fc_source": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
     "citrus_fc_sums": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void, c_void]),
-    "citrus_fc_sums_blocked": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void, c_int, c_void]),
+    "citrus_fc_sums_blocked": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_int, c_void, c_void, c_int, c_void,
+                                       c_void, c_void, c_void]),
     "citrus_fc_correct": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void, c_void, c_void]),
     "citrus_fc_primary_directions": (c_int, [ctypes.POINTER(FcArgs), c_void, c_void]),
     "citrus_fc_correct_means": (c_int, [c_void, c_void, c_void, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),

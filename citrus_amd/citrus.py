@@ -209,7 +209,7 @@ def _prepare_views(solver: ScatterSolver, states: Sequence[CitrusState], fields:
         for v, a in enumerate(angles):
             first_collision_deferred_samples(solver.library, fields.rho_quadrature, phi_uc[v],
                                              fields.grid_quadrature, solver.geometry, a, omega,
-                                             ctx.quadrature.weight, source, v)
+                                             ctx.quadrature.weight, source, v, return_correction=False)
     else:
         source = torch.empty((V, G, R, C, S, M, 4), device=device, dtype=torch.float32)
         for v, a in enumerate(angles):

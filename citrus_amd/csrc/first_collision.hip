@@ -256,6 +256,24 @@ __device__ __forceinline__ f4 eval_source_channel_batched(const FcDev& d, const 
   return interpolate4(acc0, acc1, w1);
 }
 
+// Sampling-correction factors of one (point, group) from its sums (first_collision_correction_factors,
+// uncollided.py:611-640): in float64 with the host formula's operations in its order and none contracted, so that
+// the factors equal the host's float64 result rounded to float32 bit for bit.
+__device__ __forceinline__ void correction_factors(float i0, float i1, float i2, float e0, float e1, float& scale,
+                                                   float& beta) {
+#pragma clang fp contract(off)
+  const double I0 = i0, I1 = i1, I2 = i2, E0 = e0, E1 = e1;
+  const bool pos = E0 > 0.0;
+  const double c = pos ? E1 / E0 : 0.0;
+  const double den = I2 - c * I1;
+  const bool ok = pos && fabs(den) > 1e-12 * fabs(I0);
+  double b = ok ? (c * I0 - I1) / den : 0.0;
+  b = b < -1.0 ? -1.0 : (b > 1.0 ? 1.0 : b);  // a clamp that keeps a NaN, as torch's
+  const double d0 = I0 + b * I1;
+  scale = (float)(d0 > 0.0 ? E0 / d0 : 1.0);
+  beta = (float)b;
+}
+
 // One thread per (point, output channel of four groups): the channel is uniform per workgroup (the heaviest, which
 // gathers the most table rows, first) and each thread runs all K directions for its four groups. Splitting the
 // groups over threads keeps the accumulators of a thread at 4 groups, so 4x as many points' gathers are in flight.
@@ -269,7 +287,8 @@ template <int GMAX, bool BLOCKED>
 __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* __restrict__ weights,
                                                          const float* __restrict__ mu_s_l, int lp1,
                                                          float* __restrict__ out, float* __restrict__ samples,
-                                                         int blocks_per_channel, int d8) {
+                                                         int blocks_per_channel, int d8, float* __restrict__ fscale,
+                                                         float* __restrict__ fbeta, float* __restrict__ fu) {
   __shared__ f4 sdir[kBlock];  // (omega x, y, z, weight) of the tile's directions
   constexpr int NCH = GMAX / 4;
   const int ch = NCH - 1 - (int)(blockIdx.x / blocks_per_channel);
@@ -386,7 +405,22 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
       }
     }
   }
-  if (!active) return;
+  // BLOCKED with factor outputs: per group [cells + CITRUS_FC_GROUP_PAD] and per cell u, in the block order of t
+  // (cells past the grid zero); u from the lightest channel's threads
+  const int64_t fcells = (int64_t)((d.R + 3) / 4) * ((d.C + 3) / 4) * ((d.S + 3) / 4) * 64;
+  const bool factors = BLOCKED && fscale != nullptr && t < fcells;
+  if (!active) {
+    if (factors) {
+#pragma unroll
+      for (int gl = 0; gl < 4; ++gl)
+        if (4 * ch + gl < d.G) {
+          fscale[(4 * ch + gl) * (fcells + CITRUS_FC_GROUP_PAD) + t] = 0.0f;
+          fbeta[(4 * ch + gl) * (fcells + CITRUS_FC_GROUP_PAD) + t] = 0.0f;
+        }
+      if (ch == 0) reinterpret_cast<f4*>(fu)[t] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    return;
+  }
   // exact Legendre moments l = 0, 1 of the same source (no quadrature)
   f4 E0 = {0.f, 0.f, 0.f, 0.f}, E1 = E0;
   for (int sl = 0; sl < d.n_slots; ++sl) {
@@ -417,7 +451,14 @@ __global__ __launch_bounds__(kBlock) void fc_sums_kernel(FcDev d, const float* _
     out[2 * plane + o] = I2[gl];
     out[3 * plane + o] = E0[gl];
     out[4 * plane + o] = E1[gl];
+    if (factors) {
+      float sc, be;
+      correction_factors(I0[gl], I1[gl], I2[gl], E0[gl], E1[gl], sc, be);
+      fscale[g * (fcells + CITRUS_FC_GROUP_PAD) + t] = sc;
+      fbeta[g * (fcells + CITRUS_FC_GROUP_PAD) + t] = be;
+    }
   }
+  if (factors && ch == 0) reinterpret_cast<f4*>(fu)[t] = f4{ux, uy, uz, 0.0f};
 }
 
 // In-place sampling correction of captured samples [G][P][K]: S scale (1 + beta eta), the operations and order of
@@ -886,17 +927,20 @@ extern "C" int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, co
   dispatch_groups(d.G, [&](auto gm) {
     constexpr int NCH = decltype(gm)::value / 4;
     hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value, false>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0,
-                       s, d, weights, mu_s_l, lp1, out, samples, blocks, 0);
+                       s, d, weights, mu_s_l, lp1, out, samples, blocks, 0, nullptr, nullptr, nullptr);
     return 0;
   });
   return finish("first-collision sums");
 }
 
 extern "C" int citrus_fc_sums_blocked(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1,
-                                      float* out, float* samples, int D8, void* stream) {
+                                      float* out, float* samples, int D8, float* scale, float* beta, float* u,
+                                      void* stream) {
   using namespace citrus;
   if (int rc = check_args(args)) return rc;
   if (!weights || !mu_s_l || !out || !samples) return fail(CITRUS_ERR_ARG, "NULL weights, moments, output or samples");
+  if ((scale == nullptr) != (beta == nullptr) || (scale == nullptr) != (u == nullptr))
+    return fail(CITRUS_ERR_ARG, "scale, beta and u are given together or not at all");
   if (lp1 < 2) return fail(CITRUS_ERR_ARG, "the exact moments need Legendre order >= 1");
   if (D8 < 1 || args->K != 8 * D8 || args->K > kBlock)
     return fail(CITRUS_ERR_ARG, "the block order needs K = 8 D8 sweep directions (octant-major), at most 256");
@@ -910,7 +954,7 @@ extern "C" int citrus_fc_sums_blocked(const CitrusFcArgs* args, const float* wei
   dispatch_groups(d.G, [&](auto gm) {
     constexpr int NCH = decltype(gm)::value / 4;
     hipLaunchKernelGGL((fc_sums_kernel<decltype(gm)::value, true>), dim3((unsigned)(blocks * NCH)), dim3(kBlock), 0,
-                       s, d, weights, mu_s_l, lp1, out, samples, blocks, D8);
+                       s, d, weights, mu_s_l, lp1, out, samples, blocks, D8, scale, beta, u);
     return 0;
   });
   return finish("first-collision sums (block order)");

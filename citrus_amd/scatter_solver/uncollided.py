@@ -246,11 +246,13 @@ class FirstCollisionSamplingCorrection:
 
 
 def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights,
-                 samples: Optional[Tensor] = None, blocked_d8: int = 0) -> Tensor:
+                 samples: Optional[Tensor] = None, blocked_d8: int = 0, factors=None) -> Tensor:
     """[5, G, R, C, S] float32 I0, I1, I2, E0, E1 from one launch of ``citrus_fc_sums`` (which also stores the
     uncorrected samples [G, R, C, S, K] into ``samples`` when given), or with ``blocked_d8`` (K = 8 blocked_d8
     octant-major sweep directions) of ``citrus_fc_sums_blocked``, the samples [G, blocks 8 64 dp + pad] in the
-    sweep's block order (each group's region followed by ``FC_GROUP_PAD`` unused floats)."""
+    sweep's block order (each group's region followed by ``FC_GROUP_PAD`` unused floats); ``factors`` = (scale, beta
+    [G, blocks 64 + pad], u [blocks 64, 4]) then receive the correction factors and primary directions per cell in
+    block order."""
     if library.legendre_order < 1:
         raise ValueError("the sampling correction needs a library of Legendre order >= 1")
     a, keep, mu_s_l = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
@@ -270,9 +272,19 @@ def _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, we
         if blocked_d8:
             if samples is None:
                 raise ValueError("the block order needs a samples tensor")
+            fs, fb, fu = (None, None, None) if factors is None else factors
+            if factors is not None:
+                cells = fc_blocks(grid.shape) * 64
+                for t, shp, name in ((fs, (library.num_groups, cells + _hip.FC_GROUP_PAD), "scale"),
+                                     (fb, (library.num_groups, cells + _hip.FC_GROUP_PAD), "beta"),
+                                     (fu, (cells, 4), "u")):
+                    if (tuple(t.shape) != shp or t.dtype != torch.float32 or not t.is_contiguous()
+                            or t.device != phi_uncollided.device):
+                        raise ValueError(f"{name} must be a contiguous float32 {shp} tensor on {phi_uncollided.device}")
             _hip.check(_hip.library().citrus_fc_sums_blocked(
                 ctypes.byref(a), w.data_ptr(), mu_s_l.data_ptr(), int(mu_s_l.shape[-1]), out.data_ptr(),
-                samples.data_ptr(), int(blocked_d8), _hip.stream_of(phi_uncollided.device)), "citrus_fc_sums_blocked")
+                samples.data_ptr(), int(blocked_d8), _hip.ptr(fs), _hip.ptr(fb), _hip.ptr(fu),
+                _hip.stream_of(phi_uncollided.device)), "citrus_fc_sums_blocked")
         else:
             _hip.check(_hip.library().citrus_fc_sums(ctypes.byref(a), w.data_ptr(), mu_s_l.data_ptr(),
                                                      int(mu_s_l.shape[-1]), out.data_ptr(), _hip.ptr(samples),
@@ -442,9 +454,9 @@ class DeferredFirstCollision:
         f = dict(device=device, dtype=torch.float32)
         return DeferredFirstCollision(shape=tuple(int(n) for n in shape),
                                       samples=torch.empty((V, G, nb * 8 * 64 * dp + pad), **f),
-                                      scale=torch.zeros(V, G, nb * 64 + pad, **f),
-                                      beta=torch.zeros(V, G, nb * 64 + pad, **f),
-                                      u=torch.zeros(V, nb * 64, 4, **f), omega=torch.empty(M, 4, **f))
+                                      scale=torch.empty(V, G, nb * 64 + pad, **f),
+                                      beta=torch.empty(V, G, nb * 64 + pad, **f),
+                                      u=torch.empty(V, nb * 64, 4, **f), omega=torch.empty(M, 4, **f))
 
     def block_samples(self) -> Tensor:
         """The samples as [V, G, blocks, 8, 64, dp] (a view without the pads)."""
@@ -485,36 +497,29 @@ class DeferredFirstCollision:
 
 def first_collision_deferred_samples(library: MultigroupLibrary, rho: Tensor, phi_uncollided: Tensor, grid: Grid,
                                      geometry: ConeGeometry, angle: float, omegas: Tensor, weights: Tensor,
-                                     out: DeferredFirstCollision, view: int) -> FirstCollisionSamplingCorrection:
+                                     out: DeferredFirstCollision, view: int, return_correction: bool = True
+                                     ) -> Optional[FirstCollisionSamplingCorrection]:
     """``first_collision_corrected_samples`` of view ``view`` of ``out`` without applying the correction: the
-    uncorrected samples (block order), the factors and the cells' primary directions go to ``out`` (one point per
-    cell: ``grid`` is the transport grid). Returns the correction in the grid's layout."""
+    uncorrected samples, the correction factors and the cells' primary directions go to ``out`` in block order, all
+    from the one launch of ``citrus_fc_sums_blocked`` (one point per cell: ``grid`` is the transport grid). Returns
+    the correction in the grid's layout (or None with ``return_correction=False``)."""
     K = int(omegas.reshape(-1, 3).shape[0])
-    G = library.num_groups
     if tuple(grid.shape) != tuple(out.shape) or K != int(out.omega.shape[0]):
         raise ValueError(f"grid {tuple(grid.shape)} / {K} directions do not match the deferred source "
                          f"{tuple(out.shape)} / {int(out.omega.shape[0])}")
-    sums = _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights,
-                        samples=out.samples[view], blocked_d8=K // 8).double()
-    scale, beta = first_collision_correction_factors(sums[:3], sums[3:])
-    scale, beta = scale.float(), beta.float()
-    cells = fc_blocks(grid.shape) * 64
-    out.scale[view, :, :cells].copy_(to_block_order(scale, grid.shape).reshape(G, -1))
-    out.beta[view, :, :cells].copy_(to_block_order(beta, grid.shape).reshape(G, -1))
-    a, keep, _ = _fc_args(library, rho, phi_uncollided, grid, geometry, angle, omegas)
-    u = torch.empty(tuple(grid.shape) + (4,), device=phi_uncollided.device, dtype=torch.float32)
-    with torch.cuda.device(phi_uncollided.device):
-        _hip.check(_hip.library().citrus_fc_primary_directions(ctypes.byref(a), u.data_ptr(),
-                                                               _hip.stream_of(phi_uncollided.device)),
-                   "citrus_fc_primary_directions")
-    del keep
-    out.u[view].copy_(to_block_order(u, grid.shape).reshape(-1, 4))
+    _fc_sums_raw(library, rho, phi_uncollided, grid, geometry, angle, omegas, weights, samples=out.samples[view],
+                 blocked_d8=K // 8, factors=(out.scale[view], out.beta[view], out.u[view]))
     out.omega[:, :3].copy_(omegas.reshape(-1, 3))
     out.omega[:, 3].zero_()
     out.applied = False
     out.corrected = None
-    return FirstCollisionSamplingCorrection(scale=scale.contiguous(), beta=beta.contiguous(),
-                                            omega_uncollided=u[..., :3])
+    if not return_correction:
+        return None
+    nb = fc_blocks(grid.shape)
+    scale = from_block_order(out.scale[view, :, :nb * 64].reshape(-1, nb, 64), grid.shape, 1).contiguous()
+    beta = from_block_order(out.beta[view, :, :nb * 64].reshape(-1, nb, 64), grid.shape, 1).contiguous()
+    u = from_block_order(out.u[view].reshape(nb, 64, 4), grid.shape, 0)
+    return FirstCollisionSamplingCorrection(scale=scale, beta=beta, omega_uncollided=u[..., :3].contiguous())
 
 
 _FC_LAYOUTS = {"moment_major": 0, "cell_major": 1, "group_major": 2, "group_major_means": 3}

@@ -419,10 +419,14 @@ int citrus_fc_sums(const CitrusFcArgs* args, const float* weights, const float* 
 #define CITRUS_FC_GROUP_PAD 1088
 
 /* citrus_fc_sums with the samples in block order: samples [G][blocks*8*64*dp + pad] of the K = 8 D8 octant-major sweep
- * directions (K <= 256). The sums are those of citrus_fc_sums bit for bit. No counterpart in the reference: the
- * layout in which the persistent sweep reads the deferred first-collision source. */
+ * directions (K <= 256). The sums are those of citrus_fc_sums bit for bit. scale, beta [G][blocks*64 + pad] and
+ * u [blocks*64][4] (all three, or all NULL): the sampling-correction factors of the sums (the float64 formula of
+ * first_collision_correction_factors, uncollided.py:611-640, rounded to float32) and each cell's primary direction
+ * (as citrus_fc_primary_directions), in the per-cell block order of CITRUS_FC_BLOCK_ORDER, cells past the grid zero.
+ * No counterpart in the reference: the layout in which the persistent sweep reads the deferred first-collision
+ * source. */
 int citrus_fc_sums_blocked(const CitrusFcArgs* args, const float* weights, const float* mu_s_l, int lp1, float* out,
-                           float* samples, int D8, void* stream);
+                           float* samples, int D8, float* scale, float* beta, float* u, void* stream);
 
 /* In place, samples [G][P][K] (from citrus_fc_sums) *= scale (1 + beta eta) with scale, beta [G][P] and eta the
  * cosine between Omega_UC of the point and omegas[k]. At one point per cell (factor 1) the result equals

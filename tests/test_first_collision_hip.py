@@ -6,12 +6,14 @@ Reference tests mirrored: hand evaluation of the source (tests/test_uncollided.p
 
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
 import pytest
 import torch
 
+from citrus_amd import _hip
 from citrus_amd.scatter_solver.angular_quadrature import level_symmetric_quadrature
 from citrus_amd.scatter_solver.cross_sections import MultigroupLibrary, seeded_library, synthetic_library
 from citrus_amd.scatter_solver.geometry import ConeGeometry, Grid
@@ -194,8 +196,9 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L
     partial blocks."""
     from citrus_amd.scatter_solver.source_iteration import bf16_operand
     from citrus_amd.scatter_solver.transport_sweep import make_sweep_context, sweep_views
-    from citrus_amd.scatter_solver.uncollided import (DeferredFirstCollision, _fc_sums_raw,
-                                                      first_collision_deferred_samples, from_block_order)
+    from citrus_amd.scatter_solver.uncollided import (DeferredFirstCollision, _fc_args, _fc_sums_raw, fc_blocks,
+                                                      first_collision_deferred_samples, from_block_order,
+                                                      to_block_order)
 
     dev = gpu_device()
     library = seeded_library(6, 3, 120.0, 20.0, seed=15, num_eta=97)
@@ -207,6 +210,8 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L
     corrected = []
     for v in range(V):
         rho, phi = _problem(library, grid.shape, 20 + v)
+        if v == 1:
+            rho[:, 0] = 0.0  # a slab of void cells: E0 = 0, so scale 1 and beta 0
         rho_d, phi_d = rho.float().to(dev), phi.float().to(dev)
         angle = ANGLE + 0.9 * v
         corr = first_collision_deferred_samples(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w, deferred, v)
@@ -220,8 +225,21 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L
         if unblocked.shape[-1] > D8:
             assert float(unblocked[..., D8:].abs().max()) == 0.0  # the padding of the runs
         ref, ref_corr = first_collision_corrected_samples(library, rho_d, phi_d, grid, GEOMETRY, angle, om, w)
+        # the factors the sums kernel forms in float64 equal the host formula's, and land in block order with the
+        # cells past the grid zero
         assert torch.equal(corr.scale, ref_corr.scale) and torch.equal(corr.beta, ref_corr.beta)
+        cells = fc_blocks(shape) * 64
+        assert torch.equal(deferred.scale[v, :, :cells], to_block_order(ref_corr.scale, shape).reshape(G, -1))
+        assert torch.equal(deferred.beta[v, :, :cells], to_block_order(ref_corr.beta, shape).reshape(G, -1))
+        if v == 1:
+            assert bool((ref_corr.scale[:, 0] == 1.0).all()) and bool((ref_corr.beta[:, 0] == 0.0).all())
         assert rel_err(corr.omega_uncollided, ref_corr.omega_uncollided) < 1e-6
+        u_want = torch.zeros(tuple(shape) + (4,), device=dev)
+        with torch.cuda.device(dev):
+            a, keep, _ = _fc_args(library, rho_d, phi_d, grid, GEOMETRY, angle, om)
+            _hip.check(_hip.library().citrus_fc_primary_directions(ctypes.byref(a), u_want.data_ptr(),
+                                                                   _hip.stream_of(dev)), "primary directions")
+        assert torch.equal(deferred.u[v], to_block_order(u_want, shape).reshape(-1, 4))
         corrected.append(ref)
     corrected = torch.stack(corrected).contiguous()
     gen = torch.Generator(device=dev).manual_seed(3)
