@@ -20,6 +20,7 @@ namespace {
 
 constexpr int kMaxMaterials = 8;
 constexpr int kTableShare = 8;  // table words per thread held in registers while phi loads (4096 at 512 threads)
+constexpr int kExactRun = 4;    // groups of cells per workgroup in the exact (view step) form
 
 __device__ __forceinline__ int degree_of(int n) {
   int l = 0;
@@ -42,7 +43,11 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
 // T (float32 [V][G][N][4 comp][4]), degrees l >= 2 to T16 (bfloat16 [V][G][N][4 comp][kh], zero padded).
 // EXACT: G == GMAX and downscatter only (lower_only), the view step's form: every group and triangle bound is a
 // compile-time constant, so the matrix formation and the contraction carry no per-entry tests.
-template <int GMAX, int CELLS, bool FACTORED, bool MIXED, bool EXACT>
+// RUN: a workgroup takes RUN consecutive groups of CELLS cells in turn: the material table is staged in LDS once per
+// workgroup, and the next group's phi is in flight while the current group is contracted. With RUN > 1 the cell
+// means of the run are gathered in LDS ([nsh][GMAX][CELLS RUN] after the table) and leave as one run of CELLS RUN
+// cells per (moment, group): as 16-byte pieces per group of cells they reached memory as partly written sectors.
+template <int GMAX, int CELLS, bool FACTORED, bool MIXED, bool EXACT, int RUN>
 __global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
     const float* __restrict__ transfer, const float* __restrict__ rho, int M, int Lt, const float* __restrict__ phi,
     float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only, float* __restrict__ means,
@@ -52,25 +57,37 @@ __global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
   const int per_cell = (L + 1) * DS + 1;  // odd: the two cells' matrices also fall in different banks
   float* mu = lds;
   float* table = lds + CELLS * per_cell;
+  constexpr int RC = CELLS * RUN;  // cells of the run
+  float* mstage = table + (FACTORED ? M * G * G * (Lt + 1) : 0);  // RUN > 1: [nsh][GMAX][RC]
   const int nthreads = blockDim.x;
   const int tid = threadIdx.x;
-  const int cells_per_view = (N + CELLS - 1) / CELLS;
+  const int runs_per_view = (N + CELLS * RUN - 1) / (CELLS * RUN);
   // workgroups go to the 8 XCDs round robin; give each XCD a contiguous run of cells instead, so that the 4-byte
   // cell-mean stores of neighbouring cells (4 bytes apart, 16 MB apart across moments) meet in one L2
   const unsigned nb = gridDim.x;
   const unsigned b = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
-  const int v = (int)(b / cells_per_view);
-  const int c0 = (int)(b % cells_per_view) * CELLS;
-  // this thread's element (cell, n, k) and its phi of every input group, requested first so that the loads are in
-  // flight while the transfer matrices are formed
+  const int v = (int)(b / runs_per_view);
+  const int run0 = (int)(b % runs_per_view) * CELLS * RUN;
+  // this thread's element (cell of the group, n, k); its phi of every input group is requested before the transfer
+  // matrices are formed
   const int elems = nsh * 4;
   const int cc = tid / elems;
   const int r = tid - cc * elems;
-  const int cell = c0 + cc;
-  const bool active = cc < CELLS && cell < N;
   const int64_t view_stride = (int64_t)G * N * elems;
   const int64_t group_stride = (int64_t)N * elems;
-  const int64_t off = (int64_t)v * view_stride + (int64_t)cell * elems + r;
+  // unconditional loads (inactive threads and groups past G read a valid element and discard it): no branches
+  // between the loads, so all of them are in flight together
+  auto load_phi = [&](const int c0, float (&p)[GMAX]) __attribute__((always_inline)) {
+    const int cell = c0 + cc;
+    const bool on = cc < CELLS && cell < N;
+    const int64_t safe = on ? (int64_t)v * view_stride + (int64_t)cell * elems + r
+                            : (int64_t)v * view_stride + (int64_t)min(c0, N - 1) * elems;
+#pragma unroll
+    for (int gi = 0; gi < GMAX; ++gi) {
+      const float x = phi[safe + (int64_t)(gi < G ? gi : G - 1) * group_stride];
+      p[gi] = gi < G ? x : 0.0f;  // inactive threads never use theirs
+    }
+  };
   // the factored table's share of this thread (at most kTableShare words, else a loop after the phi loads), loaded
   // before phi so that storing it to LDS waits for its own loads only
   const int nt = FACTORED ? M * G * G * (Lt + 1) : 0;
@@ -83,15 +100,8 @@ __global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
       tv[j] = transfer[i < nt ? i : 0];
     }
   }
-  // unconditional loads (inactive threads and groups past G read a valid element and discard it): no branches
-  // between the loads, so all of them are in flight together
-  const int64_t safe = active ? off : (int64_t)v * view_stride + (int64_t)c0 * elems;
   float p[GMAX];
-#pragma unroll
-  for (int gi = 0; gi < GMAX; ++gi) {
-    const float x = phi[safe + (int64_t)(gi < G ? gi : G - 1) * group_stride];
-    p[gi] = gi < G ? x : 0.0f;  // inactive threads never use theirs
-  }
+  load_phi(run0, p);
   if (FACTORED) {
     if (share_fits) {
 #pragma unroll
@@ -104,91 +114,128 @@ __global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
     }
     __syncthreads();
   }
-  // per-cell transfer matrices, entries (l, gi, go) with compile-time strides
   const int entries = (L + 1) * GMAX * GMAX;
-  if constexpr (EXACT && FACTORED) {
-    // densities of the workgroup's cells once per thread (uniform loads), then every entry as a material sum
-    float rc[CELLS][kMaxMaterials];
-#pragma unroll
-    for (int mc = 0; mc < CELLS; ++mc)
-#pragma unroll
-      for (int m = 0; m < kMaxMaterials; ++m)
-        rc[mc][m] = (m < M && c0 + mc < N) ? rho[(int64_t)m * N + c0 + mc] : 0.0f;
-    for (int e = tid; e < entries; e += nthreads) {
-      const int l = e / (GMAX * GMAX);
-      const int gi = (e / GMAX) % GMAX;
-      const int go = e % GMAX;
-      const float* tm = table + (gi * GMAX + go) * (Lt + 1) + l;
-      float m_val[CELLS];
-#pragma unroll
-      for (int mc = 0; mc < CELLS; ++mc) m_val[mc] = 0.0f;
-      if (gi <= go) {
-        for (int m = 0; m < M; ++m) {
-          const float t = tm[m * GMAX * GMAX * (Lt + 1)];
-#pragma unroll
-          for (int mc = 0; mc < CELLS; ++mc) {
-            // the densities of material m (a register array indexed by a loop variable: select it)
-            float r = rc[mc][0];
-#pragma unroll
-            for (int q = 1; q < kMaxMaterials; ++q) r = m == q ? rc[mc][q] : r;
-            m_val[mc] = fmaf(r, t, m_val[mc]);
-          }
-        }
-      }
-#pragma unroll
-      for (int mc = 0; mc < CELLS; ++mc) mu[mc * per_cell + l * DS + gi * GMAX + go] = m_val[mc];
-    }
-  } else
-  for (int mc = 0; mc < CELLS; ++mc) {
-    const int mcell_index = c0 + mc;
-    for (int e = tid; e < entries; e += nthreads) {
-      const int l = e / (GMAX * GMAX);
-      const int gi = (e / GMAX) % GMAX;
-      const int go = e % GMAX;
-      float m_val = 0.0f;
-      if (mcell_index < N && gi < G && go < G && !(lower_only && gi > go)) {
-        if (FACTORED) {
-#pragma unroll
-          for (int m = 0; m < kMaxMaterials; ++m)
-            if (m < M)
-              m_val = fmaf(rho[(int64_t)m * N + mcell_index], table[((m * G + gi) * G + go) * (Lt + 1) + l], m_val);
-        } else {
-          m_val = transfer[(((int64_t)gi * G + go) * (Lt + 1) + l) * N + mcell_index];
-        }
-      }
-      mu[mc * per_cell + l * DS + gi * GMAX + go] = m_val;
-    }
-  }
-  __syncthreads();
-  if (!active) return;
   const int l = degree_of(r / 4);
   const int n = r >> 2, k = r & 3;
-  const float* mcell = mu + cc * per_cell + l * DS;
+  for (int it = 0; it < RUN; ++it) {
+    const int c0 = run0 + it * CELLS;
+    if (c0 >= N) break;  // uniform over the workgroup
+    float pn[GMAX];
+    if (RUN > 1 && it + 1 < RUN) load_phi(min(c0 + CELLS, N - 1), pn);
+    if (it > 0) __syncthreads();  // the previous group's contraction has read its matrices
+    // per-cell transfer matrices, entries (l, gi, go) with compile-time strides
+    if constexpr (EXACT && FACTORED) {
+      // densities of the group's cells once per thread (uniform loads), then every entry as a material sum
+      float rc[CELLS][kMaxMaterials];
 #pragma unroll
-  for (int go = 0; go < GMAX; ++go) {
-    if (!EXACT && go >= G) break;
-    float acc = 0.0f;
+      for (int mc = 0; mc < CELLS; ++mc)
 #pragma unroll
-    for (int gi = 0; gi < GMAX; ++gi) {
-      if (EXACT) {
-        if (gi <= go) acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
-      } else if (gi < G && !(lower_only && gi > go)) {
-        acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
-      }
-    }
-    if (MIXED) {
-      const int64_t row = ((((int64_t)v * G + go) * N + cell) * 4 + k);  // (view, group, cell, component)
-      if (n < 4) {
-        T[row * 4 + n] = acc;
-      } else {
-        T16[row * kh + (n - 4)] = bf16_rne(acc);
-        if (n - 4 < kh - (nsh - 4)) T16[row * kh + (nsh - 4) + (n - 4)] = 0;  // padding of the row
+        for (int m = 0; m < kMaxMaterials; ++m)
+          rc[mc][m] = (m < M && c0 + mc < N) ? rho[(int64_t)m * N + c0 + mc] : 0.0f;
+      for (int e = tid; e < entries; e += nthreads) {
+        const int le = e / (GMAX * GMAX);
+        const int gi = (e / GMAX) % GMAX;
+        const int go = e % GMAX;
+        const float* tm = table + (gi * GMAX + go) * (Lt + 1) + le;
+        float m_val[CELLS];
+#pragma unroll
+        for (int mc = 0; mc < CELLS; ++mc) m_val[mc] = 0.0f;
+        if (gi <= go) {
+          for (int m = 0; m < M; ++m) {
+            const float t = tm[m * GMAX * GMAX * (Lt + 1)];
+#pragma unroll
+            for (int mc = 0; mc < CELLS; ++mc) {
+              // the densities of material m (a register array indexed by a loop variable: select it)
+              float rr = rc[mc][0];
+#pragma unroll
+              for (int q = 1; q < kMaxMaterials; ++q) rr = m == q ? rc[mc][q] : rr;
+              m_val[mc] = fmaf(rr, t, m_val[mc]);
+            }
+          }
+        }
+#pragma unroll
+        for (int mc = 0; mc < CELLS; ++mc) mu[mc * per_cell + le * DS + gi * GMAX + go] = m_val[mc];
       }
     } else {
-      T[off + go * group_stride] = acc;
+      for (int mc = 0; mc < CELLS; ++mc) {
+        const int mcell_index = c0 + mc;
+        for (int e = tid; e < entries; e += nthreads) {
+          const int le = e / (GMAX * GMAX);
+          const int gi = (e / GMAX) % GMAX;
+          const int go = e % GMAX;
+          float m_val = 0.0f;
+          if (mcell_index < N && gi < G && go < G && !(lower_only && gi > go)) {
+            if (FACTORED) {
+#pragma unroll
+              for (int m = 0; m < kMaxMaterials; ++m)
+                if (m < M)
+                  m_val = fmaf(rho[(int64_t)m * N + mcell_index], table[((m * G + gi) * G + go) * (Lt + 1) + le],
+                               m_val);
+            } else {
+              m_val = transfer[(((int64_t)gi * G + go) * (Lt + 1) + le) * N + mcell_index];
+            }
+          }
+          mu[mc * per_cell + le * DS + gi * GMAX + go] = m_val;
+        }
+      }
     }
-    // the cell means again, moment-major [V][nsh][G][N]: the operand of the image's multiple-collision expansion
-    if (means && (r & 3) == 0) means[(((int64_t)v * nsh + (r >> 2)) * G + go) * N + cell] = acc;
+    __syncthreads();
+    const int cell = c0 + cc;
+    if (cc < CELLS && cell < N) {
+      const int64_t off = (int64_t)v * view_stride + (int64_t)cell * elems + r;
+      const float* mcell = mu + cc * per_cell + l * DS;
+#pragma unroll
+      for (int go = 0; go < GMAX; ++go) {
+        if (!EXACT && go >= G) break;
+        float acc = 0.0f;
+#pragma unroll
+        for (int gi = 0; gi < GMAX; ++gi) {
+          if (EXACT) {
+            if (gi <= go) acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
+          } else if (gi < G && !(lower_only && gi > go)) {
+            acc = fmaf(mcell[gi * GMAX + go], p[gi], acc);
+          }
+        }
+        if (MIXED) {
+          const int64_t row = ((((int64_t)v * G + go) * N + cell) * 4 + k);  // (view, group, cell, component)
+          if (n < 4) {
+            T[row * 4 + n] = acc;
+          } else {
+            T16[row * kh + (n - 4)] = bf16_rne(acc);
+            if (n - 4 < kh - (nsh - 4)) T16[row * kh + (nsh - 4) + (n - 4)] = 0;  // padding of the row
+          }
+        } else {
+          T[off + go * group_stride] = acc;
+        }
+        // the cell means again, moment-major [V][nsh][G][N]: the operand of the image's multiple-collision expansion
+        if (means && k == 0) {
+          if (RUN > 1)
+            mstage[(n * GMAX + go) * RC + it * CELLS + cc] = acc;
+          else
+            means[(((int64_t)v * nsh + n) * G + go) * N + cell] = acc;
+        }
+      }
+    }
+    if (RUN > 1) {
+#pragma unroll
+      for (int gi = 0; gi < GMAX; ++gi) p[gi] = pn[gi];
+    }
+  }
+  if (RUN > 1 && means) {
+    __syncthreads();
+    const int ncells = min(RC, N - run0);  // cells of the run inside the grid
+    for (int row = tid; row < nsh * G; row += nthreads) {
+      const int n = row / G, go = row - n * G;
+      const float* src = mstage + (n * GMAX + go) * RC;
+      float* dst = means + (((int64_t)v * nsh + n) * G + go) * N + run0;
+      if (ncells == RC && (N & 3) == 0) {
+#pragma unroll
+        for (int j = 0; j < RC; j += 4)
+          *reinterpret_cast<float4*>(dst + j) = float4{src[j], src[j + 1], src[j + 2], src[j + 3]};
+      } else {
+        for (int j = 0; j < ncells; ++j) dst[j] = src[j];
+      }
+    }
   }
 }
 
@@ -212,27 +259,30 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
   // to 16 groups; 2 at 32 groups, whose matrices are 4x larger
   const int cells = gmax <= 16 ? 4 : 2;
   const size_t per_cell = (size_t)(L + 1) * (gmax * gmax + 1) + 1;
-  const size_t lds = sizeof(float) * ((size_t)cells * per_cell + (FACTORED ? (size_t)M * G * G * (Lt + 1) : 0));
-  if (lds > 64 * 1024) return fail(CITRUS_ERR_UNSUPPORTED, "transfer tables exceed 64 KiB of LDS");
+  // the exact form: G fills the kernel's group width and only downscatter enters (the view step's transfer); it
+  // takes kExactRun groups of cells per workgroup and stages their cell means
+  const bool exact = G == gmax && lower_only != 0;
+  const int run = exact ? kExactRun : 1;
+  const size_t lds = sizeof(float) * ((size_t)cells * per_cell + (FACTORED ? (size_t)M * G * G * (Lt + 1) : 0) +
+                                      (run > 1 ? (size_t)nsh * gmax * cells * run : 0));
+  if (lds > 80 * 1024) return fail(CITRUS_ERR_UNSUPPORTED, "transfer tables exceed 80 KiB of LDS");
   const int threads = cells * nsh * 4;
-  const int64_t blocks = (int64_t)V * ((N + cells - 1) / cells);
+  const int64_t blocks = (int64_t)V * ((N + cells * run - 1) / (cells * run));
   if (!grid_fits(blocks, threads)) return fail(CITRUS_ERR_ARG, "too many cells for one launch (2^32 work-items)");
   const int kh = MIXED ? 16 * ((nsh - 4 + 15) / 16) : 0;
   if (MIXED && (nsh < 16 || kh - (nsh - 4) > nsh - 4))
     return fail(CITRUS_ERR_UNSUPPORTED, "the mixed-precision operand is built for P3 and above");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)blocks), block((unsigned)threads);
-  // the exact form: G fills the kernel's group width and only downscatter enters (the view step's transfer)
-  const bool exact = G == gmax && lower_only != 0;
   auto go = [&](auto gm) {
     constexpr int GM = decltype(gm)::value;
     constexpr int CELLS = GM <= 16 ? 4 : 2;
     if (exact)
-      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, true>), grid, block, lds, s, transfer,
-                         rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, true, kExactRun>), grid, block, lds, s,
+                         transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
     else
-      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, false>), grid, block, lds, s, transfer,
-                         rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, false, 1>), grid, block, lds, s,
+                         transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
   };
   if (G <= 8)
     go(std::integral_constant<int, 8>{});
