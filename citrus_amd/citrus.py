@@ -350,10 +350,11 @@ def citrus_cropped_views_step(solver: ScatterSolver, states: Sequence[CitrusStat
 
 
 def _change_norms(new: Tensor, state: CitrusState) -> Tensor:
-    """[2] device tensor (|new|, |new - previous|) of the scalar-flux cell means, float64 (no synchronization)."""
-    x = new[:, 0, 0].double()
+    """[2] device tensor (|new|, |new - previous|) of the scalar-flux cell means (no synchronization): float32 norms
+    of the strided views, as citrus.py:355-361 forms them, with no float64 copy of either accumulator."""
+    x = new[:, 0, 0]
     if state.collided_moments is None:
-        return torch.stack([torch.linalg.vector_norm(x), torch.zeros((), device=x.device, dtype=torch.float64)])
+        return torch.stack([torch.linalg.vector_norm(x), torch.zeros((), device=x.device, dtype=x.dtype)])
     prev = state.collided_moments[:, 0, 0].to(x)
     return torch.stack([torch.linalg.vector_norm(x), torch.linalg.vector_norm(x - prev)])
 

@@ -10,6 +10,7 @@ intersection setup in float64 and compensated float32 sums. There is no CPU path
 from __future__ import annotations
 
 import math
+import weakref
 from typing import Optional, Protocol
 
 import numpy as np
@@ -127,6 +128,17 @@ class HipProjectors:
             raise ValueError("backend must be 'auto' or 'hip'")
         self.samples_per_pixel_axis = int(samples_per_pixel_axis)
         self.backend = backend
+        self._channel_last = None  # (weak reference to a field, its version, its channel-last copy)
+
+    def _channel_last_of(self, field: Tensor, grid: Grid) -> Tensor:
+        """``_channel_last(field, grid)``, reused while ``field`` is the same tensor at the same version (a view's
+        image chunks trace one density field several times)."""
+        held = self._channel_last
+        if held is not None and held[0]() is field and held[1] == field._version:
+            return held[2]
+        vol = _channel_last(field, grid)
+        self._channel_last = (weakref.ref(field), field._version, vol)
+        return vol
 
     def line_integrals(self, volume: Tensor, starts: Tensor, ends: Tensor, grid: Grid) -> Tensor:
         return line_integrals(volume, starts, ends, grid)
@@ -214,7 +226,7 @@ class HipProjectors:
         (``citrus_ray_downstream_shared``): only the M density channels are traced. ``sequences`` are the
         ``crossing_sequences`` records of these directions on this grid (built here when None)."""
         M = int(density.shape[0])
-        vol = _channel_last(density, grid)
+        vol = self._channel_last_of(density, grid)
         w = omegas.to(device=density.device, dtype=torch.float64).reshape(-1, 3).contiguous()
         K = int(w.shape[0])
         R, C, S = grid.shape

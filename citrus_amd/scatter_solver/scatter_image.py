@@ -120,6 +120,13 @@ class DirectionSetContext:
     def num_directions(self) -> int:
         return int(self.omega.shape[0])
 
+    def omega_chunks_f64(self) -> tuple:
+        """The direction chunks in float64 (the ray kernels' directions), formed once per context."""
+        key = ("omega_chunks_f64",)
+        if key not in self._cache:
+            self._cache[key] = tuple(c.to(torch.float64).contiguous() for c in self.omega_chunks)
+        return self._cache[key]
+
     @property
     def num_groups(self) -> int:
         return int(self.group_energy.numel())
@@ -656,11 +663,12 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
     for i, directions in enumerate(context.omega_chunks):
         k = int(directions.shape[0])
         start, stop = context.chunk_bounds[i]
+        rays = context.omega_chunks_f64()[i]  # the same directions as the ray kernels take them
         if mix is not None:  # density line integrals; the emission kernel mixes them into group optical depths
             tau = projectors.downstream_cumulative_mixed(density_coarse, None, grid_coarse, geometry, angle,
-                                                         directions, sequences[start:stop])
+                                                         rays, sequences[start:stop])
         else:
-            tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, directions)
+            tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, rays)
         weight = None if group_weight is None else group_weight[start:stop].to(device, torch.float32).contiguous()
         collapsed = (f == 1 and weight is not None and (mix is None or int(mix.shape[1]) <= 2)
                      and (3 * G + 1) * Rc * Cc * Sc * 4 < 2 ** 32)
@@ -673,7 +681,7 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
             library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse, multiple, f,
             None if fp is None else fp.heights, weight, mix, collapsed=collapsed)
         del tau, multiple
-        image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, directions)
+        image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, rays)
         del emission
         if fp is not None and group_weight is not None:  # blur, channel weights and the sum over k, banded
             pw = None if context.planar_weight is None else context.planar_weight[start:stop].contiguous()
