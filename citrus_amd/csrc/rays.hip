@@ -97,7 +97,12 @@ __device__ __forceinline__ double plane_param(double lo, double delta, int idx, 
 }
 
 // Integrate the channel-last field along p0 -> p1. Returns |p1 - p0| and accumulates chord * value per channel.
-template <int CH, bool CAPTURE>
+// INCR: the next plane's parameter along the crossed axis is the current one plus that axis's plane spacing
+// (delta |1/d|) instead of being formed again from the plane's index: one addition instead of a conversion, a fused
+// multiply-add and a product per step. The parameters then differ from the exact ones by rounding only, so the
+// voxel sequence is the same except where two planes are crossed within that rounding of each other (a segment of
+// that length); for the images, not for the captured sequences.
+template <int CH, bool CAPTURE, bool INCR = false>
 __device__ void trace(const RayJob& J, const double* p0, const double* p1, const float* vol, float* sum,
                       int32_t* sidx, float* slen, int32_t* scount) {
   const Box& B = J.box;
@@ -120,12 +125,13 @@ __device__ void trace(const RayJob& J, const double* p0, const double* p1, const
   int count = 0;
   if (a_hi > a_lo) {
     int idx[3];
-    double an[3];
+    double an[3], spacing[3];
     int step[3];
     for (int a = 0; a < 3; ++a) {
       idx[a] = (int)entry_index(p0[a], d[a], a_lo, B.lo[a], B.delta[a], B.n[a]);
       step[a] = pos[a] ? 1 : (d[a] < 0.0 ? -1 : 0);
       an[a] = plane_param(B.lo[a], B.delta[a], idx[a], pos[a], p0[a], inv[a], nz[a]);
+      spacing[a] = nz[a] ? B.delta[a] * fabs(inv[a]) : 0.0;
     }
     float comp[CH];
 #pragma unroll
@@ -162,9 +168,15 @@ __device__ void trace(const RayJob& J, const double* p0, const double* p1, const
       const bool sx = (an[0] <= an[1]) && (an[0] <= an[2]);
       const bool sy = (an[1] < an[0]) && (an[1] <= an[2]);
       const bool sz = (an[2] < an[0]) && (an[2] < an[1]);
-      if (sx) { idx[0] += step[0]; an[0] = plane_param(B.lo[0], B.delta[0], idx[0], pos[0], p0[0], inv[0], nz[0]); }
-      if (sy) { idx[1] += step[1]; an[1] = plane_param(B.lo[1], B.delta[1], idx[1], pos[1], p0[1], inv[1], nz[1]); }
-      if (sz) { idx[2] += step[2]; an[2] = plane_param(B.lo[2], B.delta[2], idx[2], pos[2], p0[2], inv[2], nz[2]); }
+      if (INCR) {
+        if (sx) { idx[0] += step[0]; an[0] += spacing[0]; }
+        if (sy) { idx[1] += step[1]; an[1] += spacing[1]; }
+        if (sz) { idx[2] += step[2]; an[2] += spacing[2]; }
+      } else {
+        if (sx) { idx[0] += step[0]; an[0] = plane_param(B.lo[0], B.delta[0], idx[0], pos[0], p0[0], inv[0], nz[0]); }
+        if (sy) { idx[1] += step[1]; an[1] = plane_param(B.lo[1], B.delta[1], idx[1], pos[1], p0[1], inv[1], nz[1]); }
+        if (sz) { idx[2] += step[2]; an[2] = plane_param(B.lo[2], B.delta[2], idx[2], pos[2], p0[2], inv[2], nz[2]); }
+      }
       if (!(nxt < a_hi)) break;
       a = nxt;
     }
@@ -287,7 +299,7 @@ __global__ __launch_bounds__(kRayThreads) void ray_kernel(RayJob J) {
       p0[a] = p[a] + (cp - h) * w[a];
       p1[a] = p[a] + (cp + h) * w[a];
     }
-    trace<CH, false>(J, p0, p1, vol, sum, nullptr, nullptr, nullptr);
+    trace<CH, false, true>(J, p0, p1, vol, sum, nullptr, nullptr, nullptr);
   } else {  // kDownstream: voxel centre i of the traced grid to the panel along omega[k]
     const Box& B = J.box;
     const int S = B.n[2], C = B.n[0];
