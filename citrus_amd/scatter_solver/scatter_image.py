@@ -91,7 +91,12 @@ class DirectionSetContext:
 
     K panel-facing directions, G groups, Rd x Cd pixels. ``signal_weight`` is [K, G] (flat panel: the same at every
     pixel) or [K, G, Cd] (curved panel: the incidence angle varies along the channels); ``planar_weight`` [K, Cd] on a
-    curved panel. ``grid_weight`` [G] is the transmission of an angle-independent grid."""
+    curved panel. ``grid_weight`` [G] is the transmission of an angle-independent grid (ones with ``pixel_grid``).
+    ``pixel_grid`` is a grid whose transmission depends on the photon's angles: the signal weights then exclude it,
+    and each chunk's images are weighted by its transmission at every (direction, group, pixel) after the footprint
+    blur (``pixel_signal_weight``; the reference's per-row, per-channel and per-pixel septa factors,
+    scatter_image.py:373-390 and :606-650, are this product, applied there between or after the blurs, with which a
+    per-row or per-channel factor commutes)."""
 
     quadrature: Quadrature
     geometry: ConeGeometry
@@ -114,11 +119,29 @@ class DirectionSetContext:
     cell_width: Optional[Tensor] = None
     planar_weight: Optional[Tensor] = None
     signal_group_weight: Optional[Tensor] = None
+    pixel_grid: Optional[GridResponse] = None
     _cache: dict = field(default_factory=dict, repr=False)
 
     @property
     def num_directions(self) -> int:
         return int(self.omega.shape[0])
+
+    def pixel_signal_weight(self, chunk_index: int) -> Tensor:
+        """Signal weights of the chunk's directions at the pixels: [k, G, Rd, Cd] with ``pixel_grid`` (the signal
+        weights times the grid's transmission at each direction's grid angles at each pixel, ``transmission_at`` in
+        float64), else ``signal_weight`` shaped to broadcast against [k, G, Rd, Cd] images."""
+        start, stop = self.chunk_bounds[chunk_index]
+        sw = self.signal_weight[start:stop]
+        sw = sw[:, :, None, None] if sw.dim() == 2 else sw[:, :, None, :]
+        if self.pixel_grid is None:
+            return sw
+        key = ("pixels_f64",)
+        if key not in self._cache:
+            self._cache[key] = self.geometry.pixel_centers(self.angle, self.device, torch.float64)
+        omega = self.omega_chunks_f64()[chunk_index]
+        t = self.pixel_grid.transmission_at(self._cache[("group_energy_f64",)], omega, self._cache[key],
+                                            self.geometry, self.angle)  # [G, k, Rd, Cd]
+        return sw * t.permute(1, 0, 2, 3).to(sw.dtype)
 
     def omega_chunks_f64(self) -> tuple:
         """The direction chunks in float64 (the ray kernels' directions), formed once per context."""
@@ -180,9 +203,13 @@ class DirectionSetContext:
         accumulator[0].add_(torch.einsum("kgrc,k->grc", image, w))
         if self.planar_weight is not None:
             accumulator[1].add_(torch.einsum("kgrc,kc->grc", image, self.planar_weight[start:stop]))
-            accumulator[2].add_(torch.einsum("kgrc,kgc->grc", image, self.signal_weight[start:stop]))
         else:
             accumulator[1].add_(torch.einsum("kgrc,k->grc", image, w * self.cos_theta0[start:stop]))
+        if self.pixel_grid is not None:
+            accumulator[2].add_((image * self.pixel_signal_weight(chunk_index)).sum(0))
+        elif self.planar_weight is not None:
+            accumulator[2].add_(torch.einsum("kgrc,kgc->grc", image, self.signal_weight[start:stop]))
+        else:
             accumulator[2].add_(torch.einsum("kgrc,kg->grc", image, self.signal_weight[start:stop]))
 
     def finalize(self, accumulator: Tensor) -> dict:
@@ -220,10 +247,11 @@ def make_direction_context(quadrature: Quadrature, geometry: ConeGeometry, angle
     K = int(omega.shape[0])
     pixels = geometry.pixel_centers(angle, device, f64)
     Rd, Cd = int(pixels.shape[0]), int(pixels.shape[1])
-    if grid_response.depends_on_angles:
-        raise NotImplementedError("the collapsed-cone image takes grids whose transmission does not depend on the "
-                                  "photon's angles; angle-dependent grids apply to the primary")
-    grid_t = grid_response.at_energy(energy, device, f64)  # [G]
+    # a grid whose transmission depends on the photon's angles weights every pixel of the images (pixel_grid);
+    # otherwise its transmission at the group energies is part of the signal weights
+    pixel_grid = grid_response if grid_response.depends_on_angles else None
+    grid_t = (torch.ones(G, device=device, dtype=f64) if pixel_grid is not None
+              else grid_response.at_energy(energy, device, f64))  # [G]
     theta0 = detector_angle(omega, geometry, angle)
     det = detector.at(energy, theta0) if K else torch.zeros(G, 0, device=device, dtype=f64)  # [G, K]
     planar = None
@@ -234,25 +262,31 @@ def make_direction_context(quadrature: Quadrature, geometry: ConeGeometry, angle
         planar = weight[:, None] * cos_c
         signal = (planar[:, None, :] * detector.at(energy, torch.arccos(cos_c)).permute(1, 0, 2)
                   * grid_t[None, :, None])  # [K, G, Cd]
-        if detector.angle_independent:  # signal = planar[k, c] * D_g G_g: the groups collapse before the image
-            group_weight = (det.t() * grid_t[None, :]).to(dtype).contiguous()  # [K, G]
+        if detector.angle_independent and pixel_grid is None:  # signal = planar[k, c] D_g G_g: the groups
+            group_weight = (det.t() * grid_t[None, :]).to(dtype).contiguous()  # collapse before the image [K, G]
     else:
         signal = (weight * cos0)[:, None] * det.t() * grid_t[None, :]  # [K, G]
-        group_weight = signal.to(dtype).contiguous()
+        if pixel_grid is None:
+            group_weight = signal.to(dtype).contiguous()
     chunk = max(1, int(chunk))
     bounds = tuple((s, min(s + chunk, K)) for s in range(0, K, chunk))
     om = omega.to(dtype).contiguous()
     w = weight.to(dtype).contiguous()
     cw = getattr(quadrature, "cell_width", None)
     cw = None if cw is None else cw.detach().to(device, dtype)[keep].contiguous()
-    return DirectionSetContext(
+    context = DirectionSetContext(
         quadrature=quadrature, geometry=geometry, angle=float(angle), detector=detector, grid_response=grid_response,
         group_energy=energy.to(dtype), omega=om, weight=w, cos_theta0=cos0.to(dtype),
         detector_weight=det.t().to(dtype).contiguous(), grid_weight=grid_t.to(dtype),
         signal_weight=signal.to(dtype).contiguous(), separable=not geometry.use_curved_detector, chunk=chunk,
         chunk_bounds=bounds, omega_chunks=tuple(om[s:e] for s, e in bounds),
         weight_chunks=tuple(w[s:e] for s, e in bounds), image_shape=(Rd, Cd), cell_width=cw,
-        planar_weight=None if planar is None else planar.to(dtype).contiguous(), signal_group_weight=group_weight)
+        planar_weight=None if planar is None else planar.to(dtype).contiguous(), signal_group_weight=group_weight,
+        pixel_grid=pixel_grid)
+    if pixel_grid is not None:
+        context._cache[("group_energy_f64",)] = energy
+        context._cache[("omega_chunks_f64",)] = tuple(omega[s:e].contiguous() for s, e in bounds)
+    return context
 
 
 def _resolve_context(context, quadrature, geometry, angle, detector, grid_response, library, mu_t, chunk,
@@ -694,6 +728,8 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
             acc.add_(image.sum(dim=(0, 1)))
         elif group_weight is not None:  # curved panel, angle-independent detector: w cos(theta_c) per channel
             acc.add_(torch.einsum("kgrc,kc->rc", image, context.planar_weight[start:stop]))
+        elif signal_only and context.pixel_grid is not None:  # an angle-dependent grid: weights per pixel
+            acc.add_((image * context.pixel_signal_weight(i)).sum(dim=(0, 1)))
         elif signal_only:  # curved panel: the signal weights vary along the channels
             acc.add_(torch.einsum("kgrc,kgc->rc", image, context.signal_weight[start:stop]))
         else:

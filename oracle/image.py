@@ -97,13 +97,15 @@ def attenuated_emission(S_fc, S_mc, tau, mu_c, omega, deltas_q, f, coarse_shape)
 
 def collapsed_cone_image(S_fc_fn, T, Y, mu_c, lo, deltas_q, shape_q, f, omegas, weights, cell_width, pixels,
                          pitch, heights, center_height, signal_weight, cos0, sid, sdd, angle, curved,
-                         footprint=True, planar_weight=None):
+                         footprint=True, planar_weight=None, signal_pixel_weight=None):
     """(fluence, planar, signal_per_group) [G, Rd, Cd] of the total scatter of one view.
 
     S_fc_fn(k) -> [G, Rq, Cq, Sq] first-collision source at direction k; T [G, N_sh, Rc, Cc, Sc] cell-mean moments;
     Y [K, N_sh]; mu_c [G, Rc, Cc, Sc]; omegas [K, 3], weights [K], cell_width [K, 2]; pixels [Rd, Cd, 3]; pitch
     (channel, row) in cm; heights [Rq, Cq, Sq]; signal_weight [K, G] (flat) or [K, G, Cd] (curved, with
-    planar_weight [K, Cd])."""
+    planar_weight [K, Cd]); signal_pixel_weight(k) -> [G, Rd, Cd] an extra factor of the signal of direction k at
+    each pixel (the transmission of an angle-dependent grid, scatter_image.py:373-390), applied to the blurred
+    image."""
     G = T.shape[0]
     Rq, Cq, Sq = shape_q
     coarse = (Rq // f, Cq // f, Sq // f)
@@ -132,12 +134,13 @@ def collapsed_cone_image(S_fc_fn, T, Y, mu_c, lo, deltas_q, shape_q, f, omegas, 
             img = blur(img, [wc], [wr])
         img = img[0]
         fl += weights[k] * img
+        extra = 1.0 if signal_pixel_weight is None else signal_pixel_weight(k)
         if planar_weight is None:
             pl += weights[k] * cos0[k] * img
-            sg += signal_weight[k][:, None, None] * img
+            sg += signal_weight[k][:, None, None] * img * extra
         else:
             pl += planar_weight[k][None, None, :] * img
-            sg += signal_weight[k][:, None, :] * img
+            sg += signal_weight[k][:, None, :] * img * extra
     return fl, pl, sg
 
 

@@ -165,7 +165,7 @@ def test_primary_image_and_spr():
 def test_primary_image_through_a_focused_grid():
     """The primary behind an angle-dependent grid (focused 1-D septa, direct transmission, pinned on the CPU) is the
     primary without a grid times the grid's transmission at each pixel's incidence angles, per group, to float32
-    rounding; the collapsed-cone image refuses such a grid."""
+    rounding."""
     from citrus_amd.scatter_solver.detector import analytic_1d_grid
 
     dev = gpu_device()
@@ -188,10 +188,69 @@ def test_primary_image_through_a_focused_grid():
         trans = septa.transmission_at(lib.group_energy.double(), omega, pixels, geometry, angle)  # [G, rows, cols]
         assert float(trans.max() - trans.min()) > 0.02  # the pixels see the septa at different angles
         assert rel_err(gridded, plain.double().cpu() * trans) < F32_REL
-    quad = hemisphere_quadrature(torch.tensor([0.0, -1.0, 0.0], dtype=torch.float64), 2, 4, 0.3,
-                                 dtype=torch.float64)
-    with pytest.raises(NotImplementedError):
-        make_direction_context(quad, geometry, angle, ideal_counting_response(), septa, lib, dev, torch.float32)
+
+
+@pytest.mark.parametrize("curved,crossed", [(False, False), (True, False), (False, True)])
+def test_scatter_image_through_an_angle_dependent_grid(curved, crossed):
+    """The collapsed-cone image behind septa whose transmission depends on the photon's angles (focused off the
+    source, so every (direction, pixel) has its own grid angles; crossed: a second set along the rows): every
+    direction's blurred image weighted at each pixel by the signal weight times the grid's transmission there, against
+    the oracle's image with that factor (the transmission from transmission_at in float64, pinned on the CPU), in
+    the full image and the signal-only form; without the grid the same problem gives a different signal."""
+    from dataclasses import replace as dc_replace
+
+    from citrus_amd.scatter_solver.detector import analytic_1d_grid
+
+    dev = gpu_device()
+    library, coarse, gq, rho, phi, T, mu_c, geometry = _image_problem(curved, 1)
+    angle = 0.35
+    quad = panel_direction_set(geometry, angle, (3, 6), max_polar_angle=gq)
+    det = energy_weighted_response()
+    septa = analytic_1d_grid(8.0, 0.0036, 0.15, lambda e: 3000.0 / e, lambda e: 0.0 * e + 0.05,
+                            focal_distance_cm=45.0)
+    if crossed:
+        septa = dc_replace(septa, crossed=dc_replace(septa, septa_along="rows", name="rows"))
+    assert septa.depends_on_angles
+    ctx = make_direction_context(quad, geometry, angle, det, septa, library, dev, torch.float32, chunk=5)
+    assert ctx.pixel_grid is septa and ctx.signal_group_weight is None
+    f32 = lambda t: t.float().to(dev)
+    args = (library, f32(rho), f32(phi), gq, 1, f32(T), 2, geometry, angle, quad, f32(mu_c), HipProjectors(), det)
+    out = combined_scatter_image(*args, septa, context=ctx, grid_coarse=coarse)
+    sig_only = combined_scatter_image(*args, septa, context=ctx, grid_coarse=coarse, signal_only=True)
+    plain = combined_scatter_image(*args, no_grid(), grid_coarse=coarse, signal_only=True)
+    d64 = lambda t: t.float().double().cpu().numpy()
+    table = (det.energy_kev.numpy(), det.theta_rad.numpy(), det.response.numpy())
+    keep, cos0, signal, planar = oim.signal_weights(d64(quad.omega), d64(quad.weight), geometry.source_iso_dist,
+                                                   geometry.source_detector_dist, angle, curved,
+                                                   library.group_energy.numpy(), table,
+                                                   cols=geometry.num_det_channels, dch=geometry.delta_det_channel)
+    om = d64(quad.omega)[keep]
+    pts = gq.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
+    ou = ofc.primary_directions(pts, np.array(geometry.source_position(angle)))
+    S = ofc.source_at_directions(library.mu_s_eta.numpy(), None, d64(rho).reshape(2, -1), d64(phi).reshape(3, -1),
+                                 ou, om).reshape(om.shape[0], 3, *gq.shape)
+    Y = real_spherical_harmonics(2, torch.from_numpy(om)).numpy()
+    pixels = geometry.pixel_centers(angle, dtype=torch.float64)
+    trans = septa.transmission_at(library.group_energy.double(), torch.from_numpy(om), pixels, geometry,
+                                  angle).numpy()  # [G, K, Rd, Cd]
+    assert float(trans.max() - trans.min()) > 0.05
+    heights = panel_height_field(gq, geometry, angle, dtype=torch.float64).numpy()
+    _, _, n_hat = geometry.panel_axes(angle)
+    pc = geometry.panel_center(angle)
+    lo_, ext = gq.lower_corner(), gq.extent
+    hs = [sum((lo_[a] + c[a] * ext[a] - pc[a]) * n_hat[a] for a in range(3))
+          for c in [(i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]]
+    fl, pl, sg = oim.collapsed_cone_image(
+        lambda k: S[k], d64(T), Y, d64(mu_c), np.array(gq.lower_corner()), gq.deltas, gq.shape, 1, om,
+        d64(quad.weight)[keep], d64(quad.cell_width)[keep], pixels.numpy(),
+        (geometry.delta_det_channel, geometry.delta_det_row), heights, 0.5 * (min(hs) + max(hs)),
+        signal, cos0, geometry.source_iso_dist, geometry.source_detector_dist, angle, curved, footprint=True,
+        planar_weight=planar, signal_pixel_weight=lambda k: trans[:, k])
+    assert rel_err(out["fluence"], fl) < F32_REL
+    assert rel_err(out["planar_fluence"], pl) < F32_REL
+    assert rel_err(out["signal_per_group"], sg) < F32_REL
+    assert rel_err(sig_only["signal"], sg.sum(0)) < F32_REL
+    assert rel_err(plain["signal"], sg.sum(0)) > 0.02
 
 
 def test_sh_expand_matches_float64_product():
