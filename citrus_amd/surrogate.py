@@ -417,13 +417,14 @@ def training_pair(solver, fields, angle: float, tolerance: float = 1e-4, with_ph
 
 
 def train(net: UNet3D, inputs: Sequence[Tensor], targets: Sequence[Tensor], epochs: int, lr: float = 1e-3,
-          batch: int = 1, seed: int = 0, channel_weight: Optional[Tensor] = None) -> list[float]:
+          batch: int = 1, seed: int = 0, channel_weight: Optional[Tensor] = None, log=None,
+          log_every: int = 10) -> list[float]:
     """Adam on the (channel-weighted) mean squared error over the pairs, shuffled per epoch (seeded); returns the
-    mean loss per epoch."""
+    mean loss per epoch. ``log(message)``, when given, receives the epoch and its loss every ``log_every`` epochs."""
     gen = torch.Generator().manual_seed(seed)
     n = len(inputs)
     history = []
-    for _ in range(epochs):
+    for epoch in range(epochs):
         order = torch.randperm(n, generator=gen).tolist()
         total = 0.0
         for i in range(0, n, batch):
@@ -435,6 +436,8 @@ def train(net: UNet3D, inputs: Sequence[Tensor], targets: Sequence[Tensor], epoc
             net.adam_step(net.backward(cache, g), lr)
             total += loss * len(idx)
         history.append(total / n)
+        if log is not None and (epoch + 1) % max(1, int(log_every)) == 0:
+            log(f"  epoch {epoch + 1}/{epochs}: loss {history[-1]:.5f}")
     return history
 
 

@@ -92,7 +92,8 @@ def main():
     stages = [(args.epochs // 2, args.lr), (args.epochs // 4, args.lr / 4),
               (args.epochs - args.epochs // 2 - args.epochs // 4, args.lr / 16)]
     for k, (epochs, lr) in enumerate(stages):
-        hist = train(net, xs, ys, epochs=epochs, lr=lr, batch=args.batch, seed=k, channel_weight=weight)
+        hist = train(net, xs, ys, epochs=epochs, lr=lr, batch=args.batch, seed=k, channel_weight=weight,
+                     log=lambda m: print(m, flush=True), log_every=10)
         print(f"stage {k}: lr {lr:g}, {epochs} epochs, loss {hist[0]:.5f} -> {hist[-1]:.5f} "
               f"({time.perf_counter() - t0:.1f} s)", flush=True)
     del xs, ys
