@@ -633,6 +633,11 @@ def main():
         schedule, chunk = default_tuner().best_config(ctx_t, mu_t_t, V)
         tuned = {f"{s}:{c}": round(v, 3) for (s, c), v in
                  default_tuner().timings[("config",) + default_tuner().key(ctx_t, 16, V, "fp32")].items()}
+        if schedule == "persistent":  # resident waves per CU at the chosen chunk (0: the kernel's occupancy)
+            default_tuner().best_waves(ctx_t, mu_t_t, V, chunk)
+            tuned.update({f"persistent:{chunk}:waves{w}": round(v, 3) for w, v in
+                          default_tuner().timings[("waves", chunk) + default_tuner().key(ctx_t, 16, V, "fp32")]
+                          .items()})
     del ctx_t, mu_t_t
 
     # ---- CITRUS view steps (the metric) --------------------------------------------------------------------

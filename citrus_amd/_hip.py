@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 18
+ABI_VERSION = 19
 # points per block of the emission kernels' partial sums (CITRUS_EMISSION_BLOCK of include/citrus_hip.h)
 EMISSION_BLOCK = 32
 # unused floats after each group's region of the block-order first-collision source (CITRUS_FC_GROUP_PAD)
@@ -56,7 +56,7 @@ class PersistentArgs(ctypes.Structure):
     _fields_ = [
         ("G", c_int), ("nsh", c_int), ("D8", c_int), ("d0", c_int), ("dc", c_int), ("accumulate", c_int),
         ("source_layout", c_int), ("Y", c_void), ("Yw", c_void), ("views", ctypes.POINTER(SweepView)),
-        ("status", c_void), ("spin_limit", c_int), ("fc_omega", c_void),
+        ("status", c_void), ("spin_limit", c_int), ("fc_omega", c_void), ("waves_per_cu", c_int),
     ]
 
 

@@ -253,6 +253,8 @@ extern "C" int citrus_persistent_sweep_run(const CitrusPersistentPlan* plan, con
   if (A.spin_limit < 0) return fail(CITRUS_ERR_ARG, "spin_limit must be >= 0 (0: the default)");
   pa.nviews = plan->nviews; pa.accumulate = A.accumulate ? 1 : 0;
   pa.spin_limit = A.spin_limit > 0 ? A.spin_limit : kSpinLimit;
+  if (A.waves_per_cu < 0) return fail(CITRUS_ERR_ARG, "waves_per_cu must be >= 0");
+  pa.waves_per_cu = A.waves_per_cu;
   pa.sink = A.status;
   pa.Y = A.Y; pa.Yw = A.Yw;
   pa.fc_omega = A.source_layout == 3 ? A.fc_omega : nullptr;

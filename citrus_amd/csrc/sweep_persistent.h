@@ -59,6 +59,7 @@ struct PersistentView {
 
 struct PersistentArgs {
   int G, D8, d0, dc, M, nviews, accumulate, spin_limit;
+  int waves_per_cu;  // host side only: resident workgroups per CU of the launch (0: the occupancy)
   const float* Y;
   const float* Yw;
   const int4* tasks;  // (view | group << 8 | octant << 16, block row, block col, block slice), queue by queue
@@ -234,8 +235,10 @@ hipError_t launch_persistent_t(const PersistentArgs& a, size_t lds, hipStream_t 
   e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  // resident waves only, a multiple of the XCD count so that every XCD (dispatched round robin) gets the same number
-  int nwg = std::max(1, occ) * cus;
+  // resident waves only (at most the occupancy, fewer when asked), a multiple of the XCD count so that every XCD
+  // (dispatched round robin) gets the same number
+  const int per_cu = a.waves_per_cu > 0 ? std::min(a.waves_per_cu, std::max(1, occ)) : std::max(1, occ);
+  int nwg = per_cu * cus;
   nwg = std::max(kQueues, nwg / kQueues * kQueues);
   hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), lds, s, a);
   return hipGetLastError();

@@ -5,8 +5,9 @@ once, or fewer, larger-occupancy passes repeated per chunk. It runs under one of
 results: a sequence of block-diagonal launches, or one persistent launch per chunk (P3 and P5). The best pair depends
 on the grid, the groups, the views per launch and the contraction, so it is measured: each candidate sweeps synthetic
 moments of the real shape (one warm-up, then the fastest of ``repeats`` timed runs with HIP events on torch's stream),
-and the fastest is kept per configuration for the life of the process. The tile of the cell planes (4^3 blocks) and
-one group per wave are fixed by the kernel.
+and the fastest is kept per configuration for the life of the process. For the persistent schedule the resident waves
+per CU are measured as well (``best_waves``: the kernel's occupancy or a cap below it; the moments do not depend on
+them). The tile of the cell planes (4^3 blocks) and one group per wave are fixed by the kernel.
 """
 
 from __future__ import annotations
@@ -28,6 +29,7 @@ def chunk_candidates(D8: int) -> list[int]:
 
 
 PERSISTENT_MARGIN = 1.03  # the persistent schedule is chosen when within 3% of the fastest configuration
+WAVE_CANDIDATES = (0, 6, 4)  # resident waves per CU of the persistent schedule: 0 is the kernel's occupancy
 
 
 def pick_config(times: dict) -> tuple[str, int]:
@@ -136,6 +138,46 @@ class SweepTuner:
                 best = min(best, a.elapsed_time(b))
             times[(sched, c)] = best
         choice = pick_config(times)
+        self.cache[key] = choice
+        self.timings[key] = times
+        return choice
+
+    def best_waves(self, context, mu_t: Tensor, V: int, chunk: int, contraction: str = "fp32",
+                   candidates: Sequence[int] = WAVE_CANDIDATES,
+                   run: Optional[Callable[[int], None]] = None) -> int:
+        """The fastest resident-wave cap of the persistent schedule at this chunk (measured once, then cached); ties
+        go to the occupancy (0)."""
+        G = int(mu_t.shape[0])
+        key = ("waves", int(chunk)) + self.key(context, G, V, contraction)
+        if key in self.cache:
+            return self.cache[key]
+        from .transport_sweep import sweep_views
+
+        if run is None:
+            R, C, S = context.shape
+            T = torch.zeros(V, G, R, C, S, context.num_moments, 4, device=mu_t.device, dtype=torch.float32)
+            T[..., 0, 0] = 1.0
+            out = torch.empty_like(T)
+
+            def run(waves: int) -> None:
+                sweep_views(context, mu_t, T, out=out, direction_chunk=chunk, contraction=contraction,
+                            schedule="persistent", waves_per_cu=waves)
+
+        stream = torch.cuda.current_stream(mu_t.device)
+        times = {}
+        for w in candidates:
+            run(w)  # warm-up
+            best = float("inf")
+            for _ in range(self.repeats):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                run(w)
+                b.record(stream)
+                b.synchronize()
+                best = min(best, a.elapsed_time(b))
+            times[int(w)] = best
+        fastest = min(times, key=times.get)
+        choice = 0 if 0 in times and times[0] <= PERSISTENT_MARGIN * times[fastest] else fastest
         self.cache[key] = choice
         self.timings[key] = times
         return choice

@@ -175,7 +175,8 @@ class PersistentSchedule:
         return self._workspace
 
     def run(self, *, views, Y: Tensor, Yw: Tensor, nsh: int, D8: int, d0: int, dc: int, accumulate: bool = False,
-            source_layout: int = -1, spin_limit: int = 0, fc_omega: Optional[Tensor] = None) -> None:
+            source_layout: int = -1, spin_limit: int = 0, fc_omega: Optional[Tensor] = None,
+            waves_per_cu: int = 0) -> None:
         """Enqueue one chunk sweep of every view on torch's current stream. ``views`` holds one dict per view of the
         plan with the device tensors ``mu`` [G, N], ``phi`` [G, N, nsh, 4], ``cdir`` [8 D8, 3], and either ``T``
         [G, N, nsh, 4] or the mixed operand ``T16`` [G, N, 4, KH] with ``T_low`` [G, N, 4, 4], and optionally
@@ -185,7 +186,8 @@ class PersistentSchedule:
         pad = ``FC_GROUP_PAD``), and ``fc_scale`` and ``fc_beta`` [G, blocks * 64 + pad] and ``fc_u`` [blocks * 64, 4]
         in each view, ``fc_omega`` [8 D8, 4] the directions. A sweep that aborts ORs its status code into the device's status word
         (``persistent_status_sink``), which the callers' next host read reports (``decode_status``); ``spin_limit``
-        > 0 replaces the default bound of each dependency wait."""
+        > 0 replaces the default bound of each dependency wait; ``waves_per_cu`` > 0 caps the resident waves per CU
+        (0: the kernel's occupancy)."""
         if len(views) != len(self.shapes):
             raise ValueError(f"the plan has {len(self.shapes)} views, got {len(views)}")
         if nsh not in PERSISTENT_MOMENTS:
@@ -250,10 +252,12 @@ class PersistentSchedule:
         ws = self.workspace(dc)
         if spin_limit < 0:
             raise ValueError("spin_limit must be >= 0")
+        if waves_per_cu < 0:
+            raise ValueError("waves_per_cu must be >= 0")
         args = _hip.PersistentArgs(G=G, nsh=nsh, D8=D8, d0=d0, dc=dc, accumulate=int(accumulate),
                                    source_layout=int(source_layout), Y=Y.data_ptr(), Yw=Yw.data_ptr(), views=arr,
                                    status=persistent_status_sink(self.device).data_ptr(), spin_limit=int(spin_limit),
-                                   fc_omega=_hip.ptr(fc_omega))
+                                   fc_omega=_hip.ptr(fc_omega), waves_per_cu=int(waves_per_cu))
         lib = _hip.library()
         with torch.cuda.device(self.device):
             _hip.check(lib.citrus_persistent_sweep_run(self.handle, ctypes.byref(args), ws.data_ptr(), ws.numel(),

@@ -324,7 +324,7 @@ def _mixed(moments):
 
 def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out: Optional[Tensor] = None,
                 direction_chunk: Optional[int] = None, source: Optional[Tensor] = None,
-                contraction: str = "fp32", schedule: str = "auto") -> Tensor:
+                contraction: str = "fp32", schedule: str = "auto", waves_per_cu: int = 0) -> Tensor:
     """Sweep V views that share the material fields, each from its own scattering-source moments, in one launch
     sequence: the batched form of one ``sweep`` per view of an outer iteration (citrus.py:1920-1935).
 
@@ -346,6 +346,8 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
         schedule: "launches" (one launch per block diagonal), "persistent" (one launch of resident waves per
             direction chunk and at most 16 views, P3 and P5) or "auto" (persistent where it is built and passed its
             self-check on this device, else launches); all give the same moments bit for bit.
+        waves_per_cu: the persistent schedule's resident waves per CU (0: the kernel's occupancy); the moments do
+            not depend on it.
 
     Returns:
         [V, G, R, C, S, nsh, 4] fluence moments.
@@ -391,13 +393,15 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
             from .autotune import default_tuner
 
             schedule, direction_chunk = default_tuner().best_config(sweep_context, mu_t, V, contraction)
+            if schedule == "persistent" and waves_per_cu == 0:
+                waves_per_cu = default_tuner().best_waves(sweep_context, mu_t, V, direction_chunk, contraction)
         elif nsh in PERSISTENT_MOMENTS and persistent_usable(mu_t.device)[0]:
             schedule = "persistent"
         else:
             schedule = "launches"
     if schedule == "persistent":
         return _sweep_grids_persistent([(sweep_context, mu_t, moments, deferred if deferred is not None else source,
-                                         out)], direction_chunk, contraction)[0]
+                                         out)], direction_chunk, contraction, waves_per_cu)[0]
     if deferred is not None:
         source, layout = deferred.apply_(), 2
     D8 = sweep_context.directions_per_octant
@@ -419,7 +423,8 @@ def sweep_views(sweep_context: SweepContext, mu_t: Tensor, moments: Tensor, out:
     return out
 
 
-def sweep_grids(items, direction_chunk: Optional[int] = None, contraction: str = "fp32") -> list:
+def sweep_grids(items, direction_chunk: Optional[int] = None, contraction: str = "fp32",
+                waves_per_cu: int = 0) -> list:
     """Sweep the views of several grids (views cropped to their beams, coarse and fine multigrid stages) together in
     persistent launches of at most 16 views each: every view's block tasks in one launch per direction chunk.
 
@@ -429,6 +434,7 @@ def sweep_grids(items, direction_chunk: Optional[int] = None, contraction: str =
             contexts share the quadrature and the SH order; their grids may differ in shape and cell size.
         direction_chunk: directions per octant per launch (default all).
         contraction: "fp32" or "bf16" as in ``sweep_views``.
+        waves_per_cu: resident waves per CU as in ``sweep_views``.
 
     Returns:
         One [V_i, G, R_i, C_i, S_i, nsh, 4] moment tensor per item, equal bit for bit to ``sweep_views`` of the item.
@@ -444,7 +450,7 @@ def sweep_grids(items, direction_chunk: Optional[int] = None, contraction: str =
         out = torch.empty((V, int(mu_t.shape[0])) + tuple(ctx.shape) + (ctx.num_moments, 4), device=mu_t.device,
                           dtype=torch.float32)
         prepared.append((ctx, mu_t, moments, source, out))
-    return _sweep_grids_persistent(prepared, direction_chunk, contraction)
+    return _sweep_grids_persistent(prepared, direction_chunk, contraction, waves_per_cu)
 
 
 def _is_deferred(source) -> bool:
@@ -470,7 +476,7 @@ def _check_deferred(src, V: int, G: int, spatial: tuple, M: int) -> None:
                              f"{tuple(t.shape)} {t.dtype}")
 
 
-def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
+def _sweep_grids_persistent(items, direction_chunk, contraction: str, waves_per_cu: int = 0) -> list:
     """Persistent sweeps of (context, mu_t, moments, source, out) items, 16 views per launch. A
     ``DeferredFirstCollision`` source not yet applied runs as source layout 3 (the sweep applies the correction)."""
     from .kernels.sweep_kernels import PERSISTENT_MAX_VIEWS, PERSISTENT_MOMENTS, persistent_schedule
@@ -540,7 +546,8 @@ def _sweep_grids_persistent(items, direction_chunk, contraction: str) -> list:
         for d0 in range(0, D8, Dc):
             dc = min(d0 + Dc, D8) - d0
             sched.run(views=[spec for _, spec in batch], Y=ctx0.hip.Y, Yw=ctx0.hip.Yw, nsh=nsh, D8=D8, d0=d0,
-                      dc=dc, accumulate=d0 > 0, source_layout=layout, fc_omega=omega if layout == 3 else None)
+                      dc=dc, accumulate=d0 > 0, source_layout=layout, fc_omega=omega if layout == 3 else None,
+                      waves_per_cu=waves_per_cu)
     return [it[4] for it in items]
 
 

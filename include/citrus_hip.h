@@ -144,6 +144,10 @@ typedef struct CitrusPersistentArgs {
   int spin_limit;            /* sleeps of one dependency wait before the sweep aborts; 0: the default (about a
                               * second). A small value forces the abort path from the host, for tests */
   const float* fc_omega;     /* source_layout 3: [8*D8][4] the sweep directions (x, y, z, 0) in float32 */
+  int waves_per_cu;          /* resident single-wave workgroups per CU: 0 as many as fit (the kernel's occupancy),
+                              * else at most that many (a launch shape to tune: fewer waves contend less for L2 and
+                              * HBM; any count >= 1 per XCD completes, since every task's dependencies precede it
+                              * in its own queue) */
 } CitrusPersistentArgs;
 
 /* Schedule of nviews view grids, shapes[v] = (R, C, S), with G groups. Uploads the task queues (hipMalloc). */

@@ -401,6 +401,32 @@ def test_sweep_autotuner_picks_and_caches_a_chunk():
     assert rel_err(auto, sweep_views(ctx, mu, T)) < F32_REL
 
 
+def test_persistent_sweep_with_fewer_resident_waves():
+    """The persistent schedule with its resident waves per CU capped (down to one) gives the moments of the full
+    occupancy bit for bit; the tuner measures the caps and keeps one of them; a negative cap is refused."""
+    from citrus_amd.scatter_solver.autotune import WAVE_CANDIDATES, SweepTuner
+    from citrus_amd.scatter_solver.transport_sweep import sweep_views
+
+    dev = gpu_device()
+    shape, G, V = (9, 8, 6), 3, 2
+    ctx = make_sweep_context(Grid(shape, 0.5), level_symmetric_quadrature(6), 3, device=dev)
+    gen = torch.Generator().manual_seed(37)
+    mu = (torch.rand(G, *shape, generator=gen) + 0.3).to(dev)
+    T = torch.randn(V, G, *shape, ctx.num_moments, 4, generator=gen).to(dev)
+    full = sweep_views(ctx, mu, T, schedule="persistent")
+    for waves in (1, 3, 64):
+        assert torch.equal(sweep_views(ctx, mu, T, schedule="persistent", waves_per_cu=waves), full)
+    with pytest.raises(ValueError):
+        sweep_views(ctx, mu, T, schedule="persistent", waves_per_cu=-1)
+    tuner = SweepTuner(repeats=1)
+    chunk = ctx.directions_per_octant
+    waves = tuner.best_waves(ctx, mu, V, chunk)
+    key = ("waves", chunk) + tuner.key(ctx, G, V, "fp32")
+    assert waves in WAVE_CANDIDATES and set(tuner.timings[key]) == set(WAVE_CANDIDATES)
+    calls = []
+    assert tuner.best_waves(ctx, mu, V, chunk, run=lambda w: calls.append(w)) == waves and not calls  # cached
+
+
 def test_bf16_operand_past_two_to_the_32_work_items():
     """The operand conversion of 8 views x 16 groups x 64^3 cells x P5 spans 4.8e9 (cell, component, moment) slots,
     more than one dispatch's 2^32 work-items: the kernel loops with a stride, so the last view, which lies past the
