@@ -9,8 +9,8 @@ intersection setup in float64 and compensated float32 sums. There is no CPU path
 
 from __future__ import annotations
 
+import contextlib
 import math
-import weakref
 from typing import Optional, Protocol
 
 import numpy as np
@@ -128,17 +128,25 @@ class HipProjectors:
             raise ValueError("backend must be 'auto' or 'hip'")
         self.samples_per_pixel_axis = int(samples_per_pixel_axis)
         self.backend = backend
-        self._channel_last = None  # (weak reference to a field, its version, its channel-last copy)
+        self._held = None  # (field, its channel-last copy) while ``holding`` it
+
+    @contextlib.contextmanager
+    def holding(self, field: Tensor, grid: Grid):
+        """Within the block the traversals of ``field`` (the same tensor object) reuse one channel-last copy of it
+        instead of forming one per call: for the direction chunks of one image, which trace one density field
+        several times and do not change it. The copy is dropped on leaving the block."""
+        previous = self._held
+        self._held = (field, _channel_last(field, grid))
+        try:
+            yield self
+        finally:
+            self._held = previous
 
     def _channel_last_of(self, field: Tensor, grid: Grid) -> Tensor:
-        """``_channel_last(field, grid)``, reused while ``field`` is the same tensor at the same version (a view's
-        image chunks trace one density field several times)."""
-        held = self._channel_last
-        if held is not None and held[0]() is field and held[1] == field._version:
-            return held[2]
-        vol = _channel_last(field, grid)
-        self._channel_last = (weakref.ref(field), field._version, vol)
-        return vol
+        held = self._held
+        if held is not None and held[0] is field:
+            return held[1]
+        return _channel_last(field, grid)
 
     def line_integrals(self, volume: Tensor, starts: Tensor, ends: Tensor, grid: Grid) -> Tensor:
         return line_integrals(volume, starts, ends, grid)

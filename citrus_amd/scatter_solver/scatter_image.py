@@ -21,6 +21,7 @@ single channel is traced. Anti-scatter grids whose transmission depends on the a
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import math
 from dataclasses import dataclass, field, replace
@@ -694,47 +695,51 @@ def combined_scatter_image(library: MultigroupLibrary, rho_quadrature: Tensor, p
     Rd, Cd = context.image_shape
     acc = torch.zeros(context.image_shape, device=device, dtype=torch.float32) if signal_only else \
         context.new_accumulator()
-    for i, directions in enumerate(context.omega_chunks):
-        k = int(directions.shape[0])
-        start, stop = context.chunk_bounds[i]
-        rays = context.omega_chunks_f64()[i]  # the same directions as the ray kernels take them
-        if mix is not None:  # density line integrals; the emission kernel mixes them into group optical depths
-            tau = projectors.downstream_cumulative_mixed(density_coarse, None, grid_coarse, geometry, angle,
-                                                         rays, sequences[start:stop])
-        else:
-            tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, rays)
-        weight = None if group_weight is None else group_weight[start:stop].to(device, torch.float32).contiguous()
-        collapsed = (f == 1 and weight is not None and (mix is None or int(mix.shape[1]) <= 2)
-                     and (3 * G + 1) * Rc * Cc * Sc * 4 < 2 ** 32)
-        if collapsed:  # signal-only at one point per cell: expansion, attenuation and group sum in one pass
-            tau_c = tau.reshape(k, -1, Rc * Cc * Sc).to(device, torch.float32).contiguous()
-            multiple = multiple_emission(harmonics[i].contiguous(), by_cell.view(nsh, G, -1), tau_c, weight, mix)
-        else:
-            multiple = sh_expand(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
-        emission, partial = _attenuated_emission(
-            library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse, multiple, f,
-            None if fp is None else fp.heights, weight, mix, collapsed=collapsed)
-        del tau, multiple
-        image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, rays)
-        del emission
-        if fp is not None and group_weight is not None:  # blur, channel weights and the sum over k, banded
-            pw = None if context.planar_weight is None else context.planar_weight[start:stop].contiguous()
-            fp.blur_accumulate(acc, image, i, partial, pw)
+    # the chunks trace the one density field: one channel-last copy of it for all of them
+    hold = (projectors.holding(density_coarse, grid_coarse) if mix is not None and hasattr(projectors, "holding")
+            else contextlib.nullcontext())
+    with hold:
+        for i, directions in enumerate(context.omega_chunks):
+            k = int(directions.shape[0])
+            start, stop = context.chunk_bounds[i]
+            rays = context.omega_chunks_f64()[i]  # the same directions as the ray kernels take them
+            if mix is not None:  # density line integrals; the emission kernel mixes them into group optical depths
+                tau = projectors.downstream_cumulative_mixed(density_coarse, None, grid_coarse, geometry, angle,
+                                                             rays, sequences[start:stop])
+            else:
+                tau = projectors.downstream_cumulative(mu_t_coarse, grid_coarse, geometry, angle, rays)
+            weight = None if group_weight is None else group_weight[start:stop].to(device, torch.float32).contiguous()
+            collapsed = (f == 1 and weight is not None and (mix is None or int(mix.shape[1]) <= 2)
+                         and (3 * G + 1) * Rc * Cc * Sc * 4 < 2 ** 32)
+            if collapsed:  # signal-only at one point per cell: expansion, attenuation and group sum in one pass
+                tau_c = tau.reshape(k, -1, Rc * Cc * Sc).to(device, torch.float32).contiguous()
+                multiple = multiple_emission(harmonics[i].contiguous(), by_cell.view(nsh, G, -1), tau_c, weight, mix)
+            else:
+                multiple = sh_expand(harmonics[i], by_cell).view(k, G, Rc, Cc, Sc)
+            emission, partial = _attenuated_emission(
+                library, rho, phi_uncollided, grid_quadrature, geometry, angle, directions, tau, mu_t_coarse,
+                multiple, f, None if fp is None else fp.heights, weight, mix, collapsed=collapsed)
+            del tau, multiple
+            image = projectors.oblique_project(emission, grid_quadrature, geometry, angle, rays)
+            del emission
+            if fp is not None and group_weight is not None:  # blur, channel weights and the sum over k, banded
+                pw = None if context.planar_weight is None else context.planar_weight[start:stop].contiguous()
+                fp.blur_accumulate(acc, image, i, partial, pw)
+                del image
+                continue
+            if fp is not None:
+                image = fp.blur_with(image, *fp.filters(i, partial))
+            if group_weight is not None and context.planar_weight is None:
+                acc.add_(image.sum(dim=(0, 1)))
+            elif group_weight is not None:  # curved panel, angle-independent detector: w cos(theta_c) per channel
+                acc.add_(torch.einsum("kgrc,kc->rc", image, context.planar_weight[start:stop]))
+            elif signal_only and context.pixel_grid is not None:  # an angle-dependent grid: weights per pixel
+                acc.add_((image * context.pixel_signal_weight(i)).sum(dim=(0, 1)))
+            elif signal_only:  # curved panel: the signal weights vary along the channels
+                acc.add_(torch.einsum("kgrc,kgc->rc", image, context.signal_weight[start:stop]))
+            else:
+                context.accumulate(acc, image, i)
             del image
-            continue
-        if fp is not None:
-            image = fp.blur_with(image, *fp.filters(i, partial))
-        if group_weight is not None and context.planar_weight is None:
-            acc.add_(image.sum(dim=(0, 1)))
-        elif group_weight is not None:  # curved panel, angle-independent detector: w cos(theta_c) per channel
-            acc.add_(torch.einsum("kgrc,kc->rc", image, context.planar_weight[start:stop]))
-        elif signal_only and context.pixel_grid is not None:  # an angle-dependent grid: weights per pixel
-            acc.add_((image * context.pixel_signal_weight(i)).sum(dim=(0, 1)))
-        elif signal_only:  # curved panel: the signal weights vary along the channels
-            acc.add_(torch.einsum("kgrc,kgc->rc", image, context.signal_weight[start:stop]))
-        else:
-            context.accumulate(acc, image, i)
-        del image
     if signal_only:
         return {"signal": acc}
     return context.finalize(acc)

@@ -241,3 +241,25 @@ def test_upstream_closed_form_uniform_box():
     centers = grid.center_coordinates(dtype=torch.float64).reshape(-1, 3).numpy()
     exact = np.array([0.2 * orays.box_chord(src, c, lo, hi) for c in centers])
     assert np.abs(up - exact).max() < 1e-5 * exact.max()
+
+
+def test_held_channel_last_copy_is_scoped_to_its_block():
+    """``HipProjectors.holding`` reuses one channel-last copy of a field for the traversals inside its block (the same
+    depths as without it) and drops it on leaving: a field changed in place afterwards is traced as it is then."""
+    dev = gpu_device()
+    grid = Grid((6, 7, 5), 0.4)
+    geo = _geometry(False)
+    rng = np.random.default_rng(8)
+    rho = torch.from_numpy(rng.random((2, *grid.shape)).astype(np.float32)).to(dev)
+    _, _, _, nh = orays.panel_frame(geo.source_iso_dist, geo.source_detector_dist, 0.4)
+    w = rng.standard_normal((3, 3)) - 2.5 * nh
+    w = torch.from_numpy(w / np.linalg.norm(w, axis=1, keepdims=True))
+    proj = HipProjectors()
+    plain = proj.downstream_cumulative_mixed(rho, None, grid, geo, 0.4, w)
+    with proj.holding(rho, grid):
+        held = proj.downstream_cumulative_mixed(rho, None, grid, geo, 0.4, w)
+        other = proj.downstream_cumulative_mixed(rho * 3.0, None, grid, geo, 0.4, w)  # another tensor: not held
+    assert torch.equal(held, plain)
+    assert rel_err(other, 3.0 * plain.double()) < F32_REL
+    rho.mul_(2.0)
+    assert rel_err(proj.downstream_cumulative_mixed(rho, None, grid, geo, 0.4, w), 2.0 * plain.double()) < F32_REL
