@@ -941,6 +941,8 @@ extern "C" int citrus_fc_sums_blocked(const CitrusFcArgs* args, const float* wei
   if (!weights || !mu_s_l || !out || !samples) return fail(CITRUS_ERR_ARG, "NULL weights, moments, output or samples");
   if ((scale == nullptr) != (beta == nullptr) || (scale == nullptr) != (u == nullptr))
     return fail(CITRUS_ERR_ARG, "scale, beta and u are given together or not at all");
+  if ((reinterpret_cast<uintptr_t>(samples) & 15) != 0 || (reinterpret_cast<uintptr_t>(u) & 15) != 0)
+    return fail(CITRUS_ERR_ARG, "samples and u must be 16-byte aligned (they are stored four floats at a time)");
   if (lp1 < 2) return fail(CITRUS_ERR_ARG, "the exact moments need Legendre order >= 1");
   if (D8 < 1 || args->K != 8 * D8 || args->K > kBlock)
     return fail(CITRUS_ERR_ARG, "the block order needs K = 8 D8 sweep directions (octant-major), at most 256");

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
       const int n = row / G, go = row - n * G;
       const float* src = mstage + (n * GMAX + go) * RC;
       float* dst = means + (((int64_t)v * nsh + n) * G + go) * N + run0;
-      if (ncells == RC && (N & 3) == 0) {
+      if (ncells == RC && (N & 3) == 0 && (reinterpret_cast<uintptr_t>(means) & 15) == 0) {
 #pragma unroll
         for (int j = 0; j < RC; j += 4)
           *reinterpret_cast<float4*>(dst + j) = float4{src[j], src[j + 1], src[j + 2], src[j + 3]};

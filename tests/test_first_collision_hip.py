@@ -254,3 +254,32 @@ def test_deferred_correction_equals_the_corrected_samples_and_sweeps_alike(sn, L
     assert not deferred.applied and torch.equal(got, want)
     got = sweep_views(ctx, mu, T, source=deferred, schedule="launches", contraction=contraction)
     assert deferred.applied and torch.equal(deferred.apply_(), corrected) and torch.equal(got, want)
+
+
+def test_blocked_sums_refuse_misaligned_outputs():
+    """citrus_fc_sums_blocked stores the samples and the primary directions four floats at a time, so a samples or u
+    pointer off a 16-byte boundary is refused with an error before any launch."""
+    from citrus_amd.scatter_solver.uncollided import DeferredFirstCollision, _fc_args, _fc_sums_raw
+
+    dev = gpu_device()
+    library = seeded_library(4, 2, 120.0, 20.0, seed=15, num_eta=65)
+    grid = Grid((4, 4, 4), 0.5)
+    q = level_symmetric_quadrature(4)
+    om, w = q.omega.to(dev, torch.float32), q.weight
+    rho, phi = _problem(library, grid.shape, 3)
+    d = DeferredFirstCollision.empty(1, library.num_groups, grid.shape, int(om.shape[0]), dev)
+    a, keep, mu_s_l = _fc_args(library, rho.float().to(dev), phi.float().to(dev), grid, GEOMETRY, ANGLE, om)
+    wt = w.to(dev, torch.float32).contiguous()
+    out = torch.empty(5, library.num_groups, *grid.shape, device=dev)
+    lib = _hip.library()
+    stream = _hip.stream_of(dev)
+    D8 = int(om.shape[0]) // 8
+    good = [d.samples[0].data_ptr(), d.scale[0].data_ptr(), d.beta[0].data_ptr(), d.u[0].data_ptr()]
+    assert lib.citrus_fc_sums_blocked(ctypes.byref(a), wt.data_ptr(), mu_s_l.data_ptr(), int(mu_s_l.shape[-1]),
+                                      out.data_ptr(), good[0], D8, good[1], good[2], good[3], stream) == 0
+    for bad in ([good[0] + 4] + good[1:], good[:3] + [good[3] + 8]):
+        rc = lib.citrus_fc_sums_blocked(ctypes.byref(a), wt.data_ptr(), mu_s_l.data_ptr(), int(mu_s_l.shape[-1]),
+                                        out.data_ptr(), bad[0], D8, bad[1], bad[2], bad[3], stream)
+        assert rc != 0 and "16-byte" in lib.citrus_last_error().decode()
+    torch.cuda.synchronize()
+    del keep
