@@ -132,14 +132,17 @@ def test_source_iteration_cap_and_thick_cells():
     assert rel_err(result.phi_total, phi_ref) < 1e-5
 
 
-def test_scatter_moments_also_write_moment_major_cell_means():
+@pytest.mark.parametrize("G,shape", [(5, (3, 4, 6)), (8, (3, 4, 6)), (16, (5, 5, 5)), (16, (4, 4, 4))])
+def test_scatter_moments_also_write_moment_major_cell_means(G, shape):
     """``scatter_moments_views(cell_means=True)``: the second output is the first's cell means, moment-major, bit for
-    bit (the same accumulator written twice)."""
+    bit (the same accumulator written twice), and T is the float64 contraction over the input groups to float32
+    rounding. G = 8 and 16 with downscatter only take the view step's form (a run of 4 groups of 4 cells per
+    workgroup, the means staged in LDS): 72 and 125 cells leave a partial last run, 64 cells none."""
     from citrus_amd.scatter_solver.source_iteration import MaterialTransfer, scatter_moments_views
 
     dev = gpu_device()
     gen = torch.Generator().manual_seed(12)
-    V, G, shape, nsh, M, L = 2, 5, (3, 4, 6), 16, 2, 3
+    V, nsh, M, L = 2, 16, 2, 3
     rho = torch.rand(M, *shape, generator=gen).to(dev)
     sigma = (torch.rand(M, G, G, L + 1, generator=gen) * torch.triu(torch.ones(G, G))[None, :, :, None]).to(dev)
     phi = torch.randn(V, G, *shape, nsh, 4, generator=gen).to(dev)
@@ -148,3 +151,7 @@ def test_scatter_moments_also_write_moment_major_cell_means():
     T2, means = scatter_moments_views(transfer, phi, cell_means=True)
     assert torch.equal(T, T2)
     assert torch.equal(means, T[..., 0].permute(0, 5, 1, 2, 3, 4))  # [V, G, R, C, S, nsh] -> [V, nsh, G, R, C, S]
+    degree = torch.tensor([int(n ** 0.5) for n in range(nsh)])
+    mu = torch.einsum("mrcs,mgho->rcsgho", rho.double().cpu(), sigma.double().cpu())  # [R, C, S, gi, go, l]
+    ref = torch.einsum("rcsghn,vgrcsnk->vhrcsnk", mu[..., degree], phi.double().cpu())
+    assert rel_err(T, ref) < F32_REL
