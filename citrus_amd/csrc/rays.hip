@@ -139,7 +139,12 @@ __device__ void trace(const RayJob& J, const double* p0, const double* p1, const
     double a = a_lo;
     const int max_steps = B.n[0] + B.n[1] + B.n[2] + 3;
     for (int s = 0; s < max_steps; ++s) {
-      const double nxt = fmin(fmin(an[0], an[1]), fmin(an[2], a_hi));
+      // the axis crossed next, by the DDA's tie rule (x on a tie with y or z, y on a tie with z), from two
+      // comparisons; its plane parameter is the least of the three
+      const bool xy = an[0] <= an[1];
+      const double a01 = xy ? an[0] : an[1];
+      const bool first = a01 <= an[2];
+      const double nxt = fmin(first ? a01 : an[2], a_hi);
       const double seg = fmax(nxt - a, 0.0);
       const bool inb = idx[0] >= 0 && idx[0] < B.n[0] && idx[1] >= 0 && idx[1] < B.n[1] && idx[2] >= 0 &&
                        idx[2] < B.n[2];
@@ -165,9 +170,7 @@ __device__ void trace(const RayJob& J, const double* p0, const double* p1, const
         }
         ++count;
       }
-      const bool sx = (an[0] <= an[1]) && (an[0] <= an[2]);
-      const bool sy = (an[1] < an[0]) && (an[1] <= an[2]);
-      const bool sz = (an[2] < an[0]) && (an[2] < an[1]);
+      const bool sx = first && xy, sy = first && !xy, sz = !first;
       if (INCR) {
         if (sx) { idx[0] += step[0]; an[0] += spacing[0]; }
         if (sy) { idx[1] += step[1]; an[1] += spacing[1]; }
