@@ -499,17 +499,32 @@ def learned_collided_moments(solver, fields, angle: float, net: UNet3D, threshol
 # ---- several views of one volume at once (batched sweeps) ---------------------------------------------------------
 
 def _views_setup(solver, fields, angles):
+    """The sweep context, the views' first-collision source and zero scattering moments. With one quadrature point
+    per cell the source stays uncorrected in the sweep's block order (``DeferredFirstCollision``, as the CITRUS view
+    step keeps it: the persistent sweep corrects it as it reads it, bit for bit the corrected source), else the
+    group-major corrected LD source."""
+    from .scatter_solver.uncollided import DeferredFirstCollision, first_collision_deferred_samples
+
     ctx = solver.sweep_context(fields.grid_coarse)
     V = len(angles)
     G, nsh, M = solver.num_groups, ctx.num_moments, ctx.num_directions
     R, C, S = fields.grid_coarse.shape
     dev = solver.device
-    shape = (V, G, R, C, S, M) if fields.q == 1 else (V, G, R, C, S, M, 4)
-    source = torch.empty(shape, device=dev, dtype=torch.float32)
-    for v, a in enumerate(angles):
-        phi_uc = solver.uncollided_fluence(fields, a)
-        solver.first_collision_group_major(fields, a, phi_uc, ctx, out=source[v])
-        del phi_uc
+    if fields.q == 1:
+        source = DeferredFirstCollision.empty(V, G, (R, C, S), M, dev)
+        omega = ctx.quadrature.omega.to(dev, torch.float32)
+        for v, a in enumerate(angles):
+            phi_uc = solver.uncollided_fluence(fields, a)
+            first_collision_deferred_samples(solver.library, fields.rho_quadrature, phi_uc, fields.grid_quadrature,
+                                             solver.geometry, a, omega, ctx.quadrature.weight, source, v,
+                                             return_correction=False)
+            del phi_uc
+    else:
+        source = torch.empty((V, G, R, C, S, M, 4), device=dev, dtype=torch.float32)
+        for v, a in enumerate(angles):
+            phi_uc = solver.uncollided_fluence(fields, a)
+            solver.first_collision_group_major(fields, a, phi_uc, ctx, out=source[v])
+            del phi_uc
     T = torch.zeros(V, G, R, C, S, nsh, 4, device=dev, dtype=torch.float32)
     return ctx, source, T
 
