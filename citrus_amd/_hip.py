@@ -13,7 +13,7 @@ from pathlib import Path
 import torch
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libcitrus_hip.so"
-ABI_VERSION = 19
+ABI_VERSION = 20
 # points per block of the emission kernels' partial sums (CITRUS_EMISSION_BLOCK of include/citrus_hip.h)
 EMISSION_BLOCK = 32
 # unused floats after each group's region of the block-order first-collision source (CITRUS_FC_GROUP_PAD)
@@ -102,6 +102,8 @@ _SIGNATURES = {
     "citrus_scatter_moments_dense": (c_int, [c_void, c_int, c_void, c_void, c_int, c_int, c_i64, c_int, c_void]),
     "citrus_scatter_moments_factored": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_int, c_int,
                                                 c_i64, c_int, c_void, c_void]),
+    "citrus_scatter_moments_stored": (c_int, [c_void, c_int, c_void, c_int, c_int, c_void, c_void, c_void, c_int,
+                                              c_int, c_i64, c_int, c_void]),
     "citrus_ray_segments": (c_int, [c_void, c_int, c_int, c_int, c_int, c_void, c_void, c_void, c_int, c_i64, c_void,
                                     c_void]),
     "citrus_ray_trace_indices": (c_int, [c_int, c_int, c_int, c_void, c_void, c_void, c_i64, c_int, c_void, c_void,

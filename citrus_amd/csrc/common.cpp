@@ -17,4 +17,4 @@ int fail(int code, const std::string& msg) {
 
 extern "C" const char* citrus_last_error(void) { return citrus::last_error.c_str(); }
 
-extern "C" int citrus_abi_version(void) { return 19; }
+extern "C" int citrus_abi_version(void) { return 20; }

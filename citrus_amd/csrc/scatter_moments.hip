@@ -6,6 +6,7 @@
 // registers and accumulates all output groups from the matrix of its degree. phi and T are streamed once, coalesced
 // along the contiguous [n][k] of each cell.
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -43,15 +44,18 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
 // T (float32 [V][G][N][4 comp][4]), degrees l >= 2 to T16 (bfloat16 [V][G][N][4 comp][kh], zero padded).
 // EXACT: G == GMAX and downscatter only (lower_only), the view step's form: every group and triangle bound is a
 // compile-time constant, so the matrix formation and the contraction carry no per-entry tests.
+// STORED: phi is a stored accumulator instead (the state store's format: q [V][G][nsh][N] float16 and scale
+// [V][G][nsh], the cell means q scale with zero LD slopes, decoded as citrus_state_unpack decodes them), so T equals
+// that of the unpacked accumulator bit for bit without the float32 accumulator being written and read.
 // RUN: a workgroup takes RUN consecutive groups of CELLS cells in turn: the material table is staged in LDS once per
 // workgroup, and the next group's phi is in flight while the current group is contracted. With RUN > 1 the cell
 // means of the run are gathered in LDS ([nsh][GMAX][CELLS RUN] after the table) and leave as one run of CELLS RUN
 // cells per (moment, group): as 16-byte pieces per group of cells they reached memory as partly written sectors.
-template <int GMAX, int CELLS, bool FACTORED, bool MIXED, bool EXACT, int RUN>
+template <int GMAX, int CELLS, bool FACTORED, bool MIXED, bool EXACT, int RUN, bool STORED>
 __global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
     const float* __restrict__ transfer, const float* __restrict__ rho, int M, int Lt, const float* __restrict__ phi,
     float* __restrict__ T, int G, int nsh, int L, int N, int V, int lower_only, float* __restrict__ means,
-    uint16_t* __restrict__ T16, int kh) {
+    uint16_t* __restrict__ T16, int kh, const __half* __restrict__ qs, const float* __restrict__ qscale) {
   extern __shared__ float lds[];
   constexpr int DS = degree_stride<GMAX>();
   const int per_cell = (L + 1) * DS + 1;  // odd: the two cells' matrices also fall in different banks
@@ -80,6 +84,16 @@ __global__ __launch_bounds__(CELLS * 256) void scatter_moments_kernel(
   auto load_phi = [&](const int c0, float (&p)[GMAX]) __attribute__((always_inline)) {
     const int cell = c0 + cc;
     const bool on = cc < CELLS && cell < N;
+    if constexpr (STORED) {  // the cell mean (component 0) decoded, the slopes zero
+      const int sc = on ? cell : min(c0, N - 1), sn = on ? (r >> 2) : 0;
+#pragma unroll
+      for (int gi = 0; gi < GMAX; ++gi) {
+        const int64_t gs = ((int64_t)v * G + (gi < G ? gi : G - 1)) * nsh + sn;
+        const float x = __half2float(qs[gs * N + sc]) * qscale[gs];
+        p[gi] = gi < G && (r & 3) == 0 ? x : 0.0f;
+      }
+      return;
+    }
     const int64_t safe = on ? (int64_t)v * view_stride + (int64_t)cell * elems + r
                             : (int64_t)v * view_stride + (int64_t)min(c0, N - 1) * elems;
 #pragma unroll
@@ -251,7 +265,8 @@ int check(int G, int nsh, int Lt, int64_t N, int V, int& L) {
 
 template <bool FACTORED, bool MIXED = false>
 int launch(const float* transfer, const float* rho, int M, int Lt, const float* phi, float* T, int G, int nsh,
-           int64_t N, int V, int lower_only, float* means, void* stream, uint16_t* T16 = nullptr) {
+           int64_t N, int V, int lower_only, float* means, void* stream, uint16_t* T16 = nullptr,
+           const __half* qs = nullptr, const float* qscale = nullptr) {
   int L = 0;
   if (int rc = check(G, nsh, Lt, N, V, L)) return rc;
   const int gmax = G <= 8 ? 8 : (G <= 16 ? 16 : 32);
@@ -277,12 +292,27 @@ int launch(const float* transfer, const float* rho, int M, int Lt, const float* 
   auto go = [&](auto gm) {
     constexpr int GM = decltype(gm)::value;
     constexpr int CELLS = GM <= 16 ? 4 : 2;
+    if (qs != nullptr) {  // a stored accumulator (the factored float32 form only)
+      if constexpr (FACTORED && !MIXED) {
+        if (exact)
+          hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, true, false, true, kExactRun, true>), grid, block,
+                             lds, s, transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh,
+                             qs, qscale);
+        else
+          hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, true, false, false, 1, true>), grid, block, lds, s,
+                             transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh, qs,
+                             qscale);
+      }
+      return;
+    }
     if (exact)
-      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, true, kExactRun>), grid, block, lds, s,
-                         transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, true, kExactRun, false>), grid, block,
+                         lds, s, transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh,
+                         nullptr, nullptr);
     else
-      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, false, 1>), grid, block, lds, s,
-                         transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh);
+      hipLaunchKernelGGL((scatter_moments_kernel<GM, CELLS, FACTORED, MIXED, false, 1, false>), grid, block, lds, s,
+                         transfer, rho, M, Lt, phi, T, G, nsh, L, (int)N, V, lower_only, means, T16, kh, nullptr,
+                         nullptr);
   };
   if (G <= 8)
     go(std::integral_constant<int, 8>{});
@@ -354,6 +384,16 @@ extern "C" int citrus_scatter_moments_dense(const float* transfer, int Lt, const
   using namespace citrus;
   if (!transfer || !phi || !T) return fail(CITRUS_ERR_ARG, "NULL buffer");
   return launch<false>(transfer, nullptr, 0, Lt, phi, T, G, nsh, N, V, 0, nullptr, stream);
+}
+
+extern "C" int citrus_scatter_moments_stored(const float* rho, int num_materials, const float* sigma, int Lt,
+                                             int downscatter_only, const void* q, const float* scale, float* T, int G,
+                                             int nsh, int64_t N, int V, void* stream) {
+  using namespace citrus;
+  if (!rho || !sigma || !q || !scale || !T) return fail(CITRUS_ERR_ARG, "NULL buffer");
+  if (num_materials < 1 || num_materials > kMaxMaterials) return fail(CITRUS_ERR_ARG, "1..8 materials supported");
+  return launch<true>(sigma, rho, num_materials, Lt, nullptr, T, G, nsh, N, V, downscatter_only ? 1 : 0, nullptr,
+                      stream, nullptr, static_cast<const __half*>(q), scale);
 }
 
 extern "C" int citrus_scatter_moments_factored(const float* rho, int num_materials, const float* sigma, int Lt,

@@ -180,6 +180,32 @@ def scatter_moments_views(transfer: MaterialTransfer, phi_views: Tensor, out: Op
     return (out, means) if cell_means else out
 
 
+def scatter_moments_stored(transfer: MaterialTransfer, q: Tensor, scale: Tensor,
+                           out: Optional[Tensor] = None) -> Tensor:
+    """``scatter_moments_views`` of stored accumulators (the state store's format: q [V, G, nsh, R, C, S] float16
+    and scale [V, G, nsh], cell means ``q scale``, zero LD slopes), equal bit for bit to the contraction of their
+    unpacked form (``state_store.unpack_accumulators``) without it being formed: [V, G, R, C, S, nsh, 4]."""
+    if q.dtype != torch.float16 or q.dim() != 6 or not q.is_contiguous():
+        raise ValueError("q must be a contiguous float16 [V, G, nsh, R, C, S] tensor")
+    _hip.require_gpu_tensor(q, "q", torch.float16)
+    _hip.require_gpu_tensor(scale, "scale")
+    V, G, nsh, R, C, S = (int(v) for v in q.shape)
+    if tuple(scale.shape) != (V, G, nsh) or not scale.is_contiguous() or scale.dtype != torch.float32:
+        raise ValueError(f"scale must be a contiguous float32 [{V}, {G}, {nsh}] tensor")
+    rho, sigma = transfer.rho.contiguous(), transfer.sigma.contiguous()
+    shape = (V, G, R, C, S, nsh, 4)
+    if out is None:
+        out = torch.empty(shape, device=q.device, dtype=torch.float32)
+    elif tuple(out.shape) != shape or not out.is_contiguous() or out.dtype != torch.float32:
+        raise ValueError(f"out must be a contiguous float32 {list(shape)} tensor")
+    with torch.cuda.device(q.device):
+        _hip.check(_hip.library().citrus_scatter_moments_stored(
+            rho.data_ptr(), int(rho.shape[0]), sigma.data_ptr(), int(sigma.shape[3]) - 1,
+            int(transfer.downscatter_only), q.data_ptr(), scale.data_ptr(), out.data_ptr(), G, nsh, R * C * S, V,
+            _hip.stream_of(q.device)), "citrus_scatter_moments_stored")
+    return out
+
+
 @dataclass(frozen=True, eq=False)
 class MixedOperand:
     """The mixed-precision operand of the sweep's source expansion for V views: the scattering-source moments of

@@ -219,6 +219,22 @@ class StateStore:
         self._events.append((a, b))
         return out
 
+    def stored_views(self, slots: Sequence[int]) -> list:
+        """The stored form of several slots on the device, without decoding: (q [1, G, N, Rc, Cc, Sc] float16, scale
+        [1, G, N] float32) per slot, or None for a slot never put."""
+        slots = [self._slot(s) for s in slots]
+        G, N = self.full_shape[:2]
+        cells = self.full_shape[3:]
+        out = []
+        for s in slots:
+            if self._data[s] is None:
+                out.append(None)
+                continue
+            q = self._data[s].to(self.device, non_blocking=self.pinned).reshape(1, G, N, *cells)
+            scale = self._scale[s].to(self.device, non_blocking=self.pinned).reshape(1, G, N)
+            out.append((q, scale))
+        return out
+
     def close(self) -> None:
         """Release the stored accumulators."""
         self._data = [None] * self.num_slots
@@ -255,9 +271,13 @@ def citrus_scan_step(solver, store: StateStore, states: Sequence, fields, angles
     scatter, prim, new_states = [None] * V, [None] * V, list(states)
     for b0 in range(0, V, B):
         idx = list(range(b0, min(V, b0 + B)))
-        accs = store.get_views(idx)
-        batch = [replace(states[i], collided_moments=accs[j], scatter_source=None, scatter_source_fields=None,
-                         _batch=None) for j, i in enumerate(idx)]
+        stored = store.stored_views(idx)
+        if all(x is not None for x in stored) and contraction == "fp32":
+            batch = _states_from_store(store, [states[i] for i in idx], stored, fields)
+        else:
+            accs = store.get_views(idx)
+            batch = [replace(states[i], collided_moments=accs[j], scatter_source=None, scatter_source_fields=None,
+                             _batch=None) for j, i in enumerate(idx)]
         results, stepped = citrus_views_step(solver, batch, fields, [angles[i] for i in idx], primary=primary,
                                              contraction=contraction, schedule=schedule)
         store.put_views(idx, [s.collided_moments for s in stepped])
@@ -271,6 +291,29 @@ def citrus_scan_step(solver, store: StateStore, states: Sequence, fields, angles
     primary_t = torch.stack(prim)
     tiny = torch.finfo(torch.float32).tiny
     return scatter, primary_t, scatter / primary_t.clamp_min(tiny), new_states
+
+
+def _states_from_store(store: StateStore, states: Sequence, stored: list, fields) -> list:
+    """The views' states for one batched step straight from their stored accumulators: the step's scattering-source
+    moments formed from the stored cell means by one kernel per view (``scatter_moments_stored``: bit for bit those
+    of the unpacked accumulators, which are never formed), shared by the batch as its previous step's moments, and
+    as the accumulator only the decoded scalar-flux cell means [G, 1, 1, Rc, Cc, Sc] that the step's change norms
+    read."""
+    from .scatter_solver.source_iteration import scatter_moments_stored
+
+    G, N = store.full_shape[:2]
+    cells = store.full_shape[3:]
+    stream, a, b = store._timed()
+    T = torch.empty((len(states), G) + tuple(cells) + (N, 4), device=store.device, dtype=torch.float32)
+    out = []
+    for j, (st, (q, scale)) in enumerate(zip(states, stored)):
+        scatter_moments_stored(fields.transfer_coarse, q, scale, out=T[j:j + 1])
+        means = (q[0, :, 0].float() * scale[0, :, 0].reshape(G, 1, 1, 1))[:, None, None]  # as citrus_state_unpack
+        out.append(replace(st, collided_moments=means, scatter_source=T[j], scatter_source_fields=fields,
+                           _batch=(T, j)))
+    b.record(stream)
+    store._events.append((a, b))
+    return out
 
 
 def scatter_corrected_counts(counts: Tensor, spr: Optional[Tensor]) -> Tensor:

@@ -282,6 +282,14 @@ int citrus_scatter_moments_factored(const float* rho, int num_materials, const f
                                     int downscatter_only, const float* phi, float* T, int G, int nsh, int64_t N,
                                     int V, float* cell_means, void* stream);
 
+/* citrus_scatter_moments_factored of stored accumulators (the state store's format, citrus/citrus.py:623-737): q
+ * [V][G][nsh][N] float16 and scale [V][G][nsh], the cell means q scale with zero LD slopes decoded as
+ * citrus_state_unpack decodes them, so T [V][G][N][nsh][4] equals the factored contraction of the unpacked
+ * accumulators bit for bit, without the float32 accumulators being written and read. */
+int citrus_scatter_moments_stored(const float* rho, int num_materials, const float* sigma, int Lt,
+                                  int downscatter_only, const void* q, const float* scale, float* T, int G, int nsh,
+                                  int64_t N, int V, void* stream);
+
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Ray traversal (Siddon / Amanatides-Woo): exact line integrals of a channel-last field [R][C][S][nch] (nch <= 16).

@@ -98,9 +98,34 @@ def test_scan_step_carries_accumulators_through_the_store():
     # the second stage starts from the decoded accumulators
     decoded = [replace(s, collided_moments=store.get(v)) for v, s in enumerate(states)]
     scatter2, _, _, states2 = citrus_scan_step(solver, store, states, fields, angles, views_per_batch=2)
-    want, _ = citrus_views_step(solver, decoded[2:4], fields, angles[2:4])
+    want, want_states = citrus_views_step(solver, decoded[2:4], fields, angles[2:4])
     assert torch.equal(scatter2[2], want[0][0]) and torch.equal(scatter2[3], want[1][0])
+    # the stages form the step's moments from the stored means directly: the same change norms as the decoded form
+    assert states2[2].history == want_states[0].history and states2[3].history == want_states[1].history
     assert [s.num_steps for s in states2] == [2] * 5
     y = torch.rand_like(primary) + 1.0
     assert torch.equal(scatter_corrected_counts(y, ratio), y / (1.0 + ratio))
     assert scatter_corrected_counts(y, None) is y
+
+
+@pytest.mark.parametrize("G,shape", [(16, (5, 6, 7)), (5, (4, 4, 3))])
+def test_scattering_moments_from_stored_accumulators(G, shape):
+    """scatter_moments_stored forms T from the stored float16 cell means and scales (zero slopes) bit for bit as
+    scatter_moments_views forms it from their unpacked float32 form; G = 16 downscatter-only takes the view step's
+    run form, G = 5 the general one."""
+    from citrus_amd.scatter_solver.source_iteration import MaterialTransfer, scatter_moments_stored, \
+        scatter_moments_views
+    from citrus_amd.state_store import pack_accumulators, unpack_accumulators
+
+    dev = gpu_device()
+    gen = torch.Generator().manual_seed(23)
+    V, nsh, M, L = 3, 36, 2, 5
+    rho = torch.rand(M, *shape, generator=gen).to(dev)
+    sigma = (torch.rand(M, G, G, L + 1, generator=gen) * torch.triu(torch.ones(G, G))[None, :, :, None]).to(dev)
+    transfer = MaterialTransfer(rho, sigma)
+    phi = torch.randn(V, G, *shape, nsh, 4, generator=gen).to(dev)
+    q, scale = pack_accumulators(phi)
+    want = scatter_moments_views(transfer, unpack_accumulators(q, scale))
+    got = scatter_moments_stored(transfer, q.contiguous(), scale.contiguous())
+    assert torch.equal(got, want)
+    assert not bool(got[..., 1:].any())  # zero slopes in, zero slopes out
