@@ -631,6 +631,36 @@ __device__ __forceinline__ f4 contract_node_channel(const FcDev& d, const PointS
   return acc;
 }
 
+// contract_node_channel for G == GMAX (the emission kernel's straight-line form): the table through a buffer
+// resource, one 32-bit offset per (node, slot), the rows of a batch in the loads' immediate offsets, so no address is
+// formed per load; the same terms in the same order, so the value is the same bit for bit.
+constexpr int kTableRsrcWord3 = 0x00020000;  // raw buffer, 32-bit data format
+
+template <int GMAX>
+__device__ __forceinline__ f4 contract_node_channel_exact(__amdgpu_buffer_rsrc_t table, const FcDev& d,
+                                                          const PointSlots& ps, const float* ph, int j, int ch) {
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    if (s >= d.n_slots) break;
+    const float rho = ps.rho[s];
+    if (rho == 0.0f) continue;  // as in eval_source
+    const unsigned base = (unsigned)((j * d.M + ps.mat[s]) * GMAX * GMAX * 4 + ch * 16);
+#pragma unroll
+    for (int gb = 0; gb < GMAX / 4; ++gb) {
+      if (gb > ch) break;
+      f4 x[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        x[q] = __builtin_bit_cast(
+            f4, __builtin_amdgcn_raw_buffer_load_b128(table, (int)(base + (4 * gb + q) * GMAX * 4), 0, 0));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc = fma4(rho * ph[4 * gb + q], x[q], acc);
+    }
+  }
+  return acc;
+}
+
 // Workgroup shape of the emission kernel: a run of kEmPoints consecutive points (coalesced outputs) times the
 // directions of a tile of kEmTile, each point's directions spread over kEmLanes threads (kEmPerLane each).
 constexpr int kEmThreads = 512;
@@ -675,6 +705,9 @@ __global__ __launch_bounds__(kEmThreads) __attribute__((amdgpu_waves_per_eu(4)))
   __shared__ float s_sum[2][kEmPoints * kEmSumStride];    // per (point, direction): sum_g e and h sum_g e
   __shared__ float s_ph[kEmPoints][GMAX + 1];              // phi_UC of the workgroup's points
   __shared__ int s_window[2];
+  // the table as a buffer (the straight-line form's node contraction: 32-bit offsets)
+  const __amdgpu_buffer_rsrc_t table_r = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(d.table), 0, FAST ? d.n_eta * d.M * d.G * GMAX * 4 : 0, kTableRsrcWord3);
   float* const s_node = reinterpret_cast<float*>(s_nodes);
 
   const int tid = threadIdx.x;
@@ -829,7 +862,8 @@ __global__ __launch_bounds__(kEmThreads) __attribute__((amdgpu_waves_per_eu(4)))
         for (int q = dl; q < ns * NCH; q += kEmLanes) {
           const int jj = q / NCH, ch = q % NCH;
           reinterpret_cast<f4*>(s_node + pl * Shape::kStride + jj * GMAX)[ch] =
-              contract_node_channel<GMAX>(d, ps, s_ph[pl], a + jj, ch);
+              FAST ? contract_node_channel_exact<GMAX>(table_r, d, ps, s_ph[pl], a + jj, ch)
+                   : contract_node_channel<GMAX>(d, ps, s_ph[pl], a + jj, ch);
         }
       }
       __syncthreads();
