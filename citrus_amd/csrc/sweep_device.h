@@ -350,38 +350,48 @@ __device__ __forceinline__ void mfma_task(const MfmaTask& a, const DirTables& dt
   auto plane = [&](const int p, const Staged cur) __attribute__((always_inline)) -> Staged {
     const int pb = ptab.offset[p];
     const int P = ptab.offset[p + 1] - pb;
-    Staged next = cur;
-    if (p + 1 < NPL) next = load_plane(p + 1);
-    f4 pold[3][NT2];  // moments this plane adds to (zero on first visits: those loads are dropped)
-    int cells[3];
-#pragma unroll
-    for (int mt = 0; mt < 3; ++mt) {
-      if (mt * 4 >= P) break;
-      const int sd = 4 * mt + q;
-      cells[mt] = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
-#pragma unroll
-      for (int t = 0; t < NT2; ++t) {
-        const int n = t * 16 + r16;
-        const bool use = !first_visit && cells[mt] >= 0 && n < NSH;
-        pold[mt][t] = buf_load4_c<COH>(phi_r, use ? (unsigned)cells[mt] * CB + 16u * n : kDrop);
-      }
-    }
-    // SRC 3: this plane's correction factors (scale, beta, Omega_UC) of the cell of (4 mt + l/16), requested here and
-    // used after the plane's expansion (held for one plane only: prefetching them a plane ahead with the means doubles
-    // their registers)
+    // Loads retire in issue order, so a wait for one load also waits for every load issued before it. Below 8
+    // direction tiles (at 8 the kernel then spills) the SRC 3 correction factors, used right after the expansion, are
+    // requested before the next plane's prefetch, which the wait for them then leaves in flight.
+    constexpr bool kFactorsFirst = SRC == 3 && DC4 < 8;
+    // SRC 3: this plane's correction factors (scale, beta, Omega_UC) of the cell of (4 mt + l/16); held for one plane
+    // only: prefetching them a plane ahead with the means doubles their registers
     float fs[3], fb[3];
     f4 fu[3];
-    if (SRC == 3) {
+    int cells[3];  // the flat cell of (4 mt + l/16), or -1
+    auto load_factors = [&]() __attribute__((always_inline)) {
 #pragma unroll
       for (int mt = 0; mt < 3; ++mt) {
         if (mt * 4 >= P) break;
         const unsigned bcell = block_cell(pb, P, 4 * mt + q);
-        const bool on = cells[mt] >= 0;
+        const int sd = 4 * mt + q;
+        const bool on = kFactorsFirst ? sd < P && slot_cell[pb + (sd < P ? sd : P - 1)] >= 0 : cells[mt] >= 0;
         fs[mt] = buf_load(fs_r, on ? 4u * bcell : kDrop);
         fb[mt] = buf_load(fb_r, on ? 4u * bcell : kDrop);
         fu[mt] = buf_load4(fu_r, on ? 16u * bcell : kDrop);
       }
-    }
+    };
+    if (kFactorsFirst) load_factors();
+    // the moments this plane adds to (zero on first visits: those loads are dropped)
+    f4 pold[3][NT2];
+    auto load_pold = [&]() __attribute__((always_inline)) {
+#pragma unroll
+      for (int mt = 0; mt < 3; ++mt) {
+        if (mt * 4 >= P) break;
+        const int sd = 4 * mt + q;
+        cells[mt] = sd < P ? slot_cell[pb + (sd < P ? sd : P - 1)] : -1;
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) {
+          const int n = t * 16 + r16;
+          const bool use = !first_visit && cells[mt] >= 0 && n < NSH;
+          pold[mt][t] = buf_load4_c<COH>(phi_r, use ? (unsigned)cells[mt] * CB + 16u * n : kDrop);
+        }
+      }
+    };
+    Staged next = cur;
+    if (p + 1 < NPL) next = load_plane(p + 1);
+    load_pold();
+    if (SRC == 3 && !kFactorsFirst) load_factors();
     f4 srcv[3][NT];  // explicit source of (cell 4 mt + l/16, direction 16 nt + l%16)
     if (SRC == 2) {
 #pragma unroll
